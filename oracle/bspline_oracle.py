@@ -1,0 +1,309 @@
+"""numpy restatement of the reference's B-spline port-solution path (oracle).
+
+Test infrastructure only (see oracle/__init__.py).  Each function names the
+reference lines it follows; paths are relative to the reference checkout.
+
+Scope: free space, reduced thin-wire kernel (extended_kernel=False), point
+delta-gap feeds, junctions with KCL rows, junction ports and node gaps,
+degree 1 or 2 -- the configuration space of the port-solution path.
+"""
+
+from math import comb  # noqa: F401  (kept for readers cross-checking weights)
+
+import numpy as np
+import scipy.linalg
+from scipy.interpolate import BSpline
+
+from .static_moments import static_moments_uniform
+
+EPS0 = 8.8541878188e-12  # bspline.py:480
+MU0 = 1.25663706127e-6   # bspline.py:481
+
+_V_UNIT_INV = {  # bspline.py:112-115
+    1: np.array([[1.0, 0.0], [-1.0, 1.0]]),
+    2: np.array([[1.0, 0.0, 0.0], [-3.0, 4.0, -1.0], [2.0, -4.0, 2.0]]),
+}
+
+
+def gl01(n):
+    """Gauss-Legendre nodes/weights mapped to [0, 1] (_bspline_kernels.py:634-636)."""
+    xi, w = np.polynomial.legendre.leggauss(n)
+    return 0.5 * (xi + 1.0), 0.5 * w
+
+
+class OracleSolver:
+    """Mirror of BSplineSolver's constructor surface for the path (bspline.py:511-982)."""
+
+    def __init__(self, *, wires, n_per_edge_per_wire=None, degree=2, feeds=None,
+                 feed_wire_index=0, feed_arclength=None, junctions=None,
+                 junction_ports=None, node_gaps=None, n_qp_pair=4, wavelength=22,
+                 wire_radius=0.0005, nsegs=101):
+        self.degree = int(degree)
+        self.wires = [np.asarray(w, dtype=float) for w in wires]
+        n_w = len(self.wires)
+        r = np.asarray(wire_radius, dtype=float)
+        self.radius = np.full(n_w, float(r)) if r.ndim == 0 else r.copy()
+        if n_per_edge_per_wire is None:
+            n_per_edge_per_wire = [None] * n_w
+        self.npe = []
+        for pl, npe in zip(self.wires, n_per_edge_per_wire):
+            ne = pl.shape[0] - 1
+            if npe is None:
+                npe = nsegs
+            if np.isscalar(npe):
+                npe = [int(npe)] * ne
+            self.npe.append([int(x) for x in npe])
+        if feeds is None:
+            feeds = [(feed_wire_index, feed_arclength, 1.0)]
+        self.feeds = [(int(w), None if s is None else float(s), complex(v)) for w, s, v in feeds]
+        self.junctions = [list(j) for j in (junctions or [])]
+        self.junction_ports = [(int(p), 0j) if np.isscalar(p) else (int(p[0]), complex(p[1]))
+                               for p in (junction_ports or [])]
+        self.node_gaps = [(int(w), e, complex(v)) for w, e, v in (node_gaps or [])]
+        self.n_qp_pair = int(n_qp_pair)
+        # bspline.py:672-676
+        self.c = 1.0 / np.sqrt(EPS0 * MU0)
+        self.set_wavelength(wavelength)
+
+    def set_wavelength(self, wavelength):
+        self.wavelength = wavelength
+        self.freq = self.c / wavelength
+        self.omega = 2 * np.pi * self.freq
+        self.k = self.omega / self.c
+
+    def set_k(self, k):  # _port_solution.py:101-105
+        self.k = float(k)
+        self.omega = self.k * self.c
+        self.wavelength = self.c / (self.omega / (2 * np.pi))
+
+    # -- geometry: bspline.py:988-1093 ------------------------------------
+    def geometry(self):
+        per_wire, seg_off = [], [0]
+        seg_l, seg_r, tan, hs, edge_id, edge_idx = [], [], [], [], [], []
+        eid = 0
+        for pl, npe in zip(self.wires, self.npe):
+            sl_w, sr_w, h_w, ed_off, ed_arc = [], [], [], [0], []
+            for e in range(pl.shape[0] - 1):
+                p0, p1 = pl[e], pl[e + 1]
+                L = float(np.linalg.norm(p1 - p0))
+                n_e = npe[e]
+                t = np.linspace(0.0, 1.0, n_e + 1)
+                pts = (1 - t[:, None]) * p0[None, :] + t[:, None] * p1[None, :]
+                sl_w.append(pts[:-1])
+                sr_w.append(pts[1:])
+                tan.append(np.tile((p1 - p0) / L, (n_e, 1)))
+                h_w.append(np.full(n_e, L / n_e))
+                ed_arc.append(np.linspace(0.0, L, n_e + 1))
+                ed_off.append(ed_off[-1] + n_e)
+                edge_id.append(np.full(n_e, eid))
+                edge_idx.append(np.arange(n_e))
+                eid += 1
+            h_w = np.concatenate(h_w)
+            per_wire.append(dict(h=h_w, arc=np.concatenate([[0.0], np.cumsum(h_w)]),
+                                 ed_off=ed_off, ed_arc=ed_arc, n=len(h_w)))
+            seg_l += sl_w
+            seg_r += sr_w
+            hs.append(h_w)
+            seg_off.append(seg_off[-1] + len(h_w))
+        return dict(per_wire=per_wire, seg_off=seg_off, seg_l=np.vstack(seg_l),
+                    seg_r=np.vstack(seg_r), tangents=np.vstack(tan),
+                    h=np.concatenate(hs), n_segs=seg_off[-1])
+
+    def seg_radius(self, geom):  # bspline.py:2032-2038
+        return np.repeat(self.radius, np.diff(geom["seg_off"]))
+
+    # -- basis: bspline.py:1216-1414 ----------------------------------------
+    def endpoint_status(self):  # bspline.py:1108-1151 (free space)
+        n_w = len(self.wires)
+        st, en = ["free"] * n_w, ["free"] * n_w
+        for j, jw in enumerate(self.junctions):
+            for w, end in jw:
+                if end == "start":
+                    st[w] = j
+                else:
+                    en[w] = j
+        return st, en
+
+    def basis(self, geom):
+        d = self.degree
+        st, en = self.endpoint_status()
+        supp_all, polys_all, knots_all, wbg = [], [], [], []
+        jdirs = {j: [] for j in range(len(self.junctions))}
+        m = 0
+        for w, pw in enumerate(geom["per_wire"]):
+            arc = pw["arc"]
+            knots = np.concatenate([np.zeros(d), arc, np.full(d, arc[-1])])
+            knots_all.append(knots)
+            nb_w = len(knots) - d - 1
+            kept = []
+            if st[w] != "free":
+                kept.append((0, "dir", st[w], "start"))
+            for j in range(1, nb_w - 1):
+                kept.append((j, "int", None, None))
+            if en[w] != "free":
+                kept.append((nb_w - 1, "dir", en[w], "end"))
+            n_w = pw["n"]
+            unit = np.linspace(0.0, 1.0, d + 1)
+            u_glob = (arc[:-1, None] + pw["h"][:, None] * unit[None, :]).reshape(-1)
+            DM = BSpline.design_matrix(u_glob, knots, d).toarray().reshape(n_w, d + 1, nb_w)
+            l2g = {}
+            for kidx, (j, kind, jidx, endp) in enumerate(kept):
+                lo, hi = max(0, j - d), min(n_w, j + 1)
+                s_m = np.zeros(d + 1, dtype=np.int64)
+                p_m = np.zeros((d + 1, d + 1))
+                for a, s in enumerate(range(lo, hi)):
+                    s_m[a] = geom["seg_off"][w] + s
+                    coef = _V_UNIT_INV[d] @ DM[s, :, j]
+                    p_m[a] = coef * pw["h"][s] ** (-np.arange(d + 1))
+                supp_all.append(s_m)
+                polys_all.append(p_m)
+                l2g[kidx] = m
+                if kind == "dir":
+                    jdirs[jidx].append((m, 1.0 if endp == "start" else -1.0))
+                m += 1
+            wbg.append((kept, l2g))
+        supp = np.array(supp_all, dtype=np.int64)
+        polys = np.array(polys_all)
+        kcl = np.zeros((len(self.junctions), m))
+        for j in range(len(self.junctions)):
+            for mg, sgn in jdirs[j]:
+                kcl[j, mg] = sgn
+        return supp, polys, kcl, knots_all, wbg
+
+    # -- moments ------------------------------------------------------------
+    def full_moments(self, geom, a_row, k, il=None, jl=None):
+        """_bspline_kernels.py:716-747 (numpy body of the off-edge moments)."""
+        d = self.degree
+        t01, w01 = gl01(self.n_qp_pair)
+        sl, sr = geom["seg_l"], geom["seg_r"]
+        il = slice(None) if il is None else il
+        jl = slice(None) if jl is None else jl
+        sli, sri, slj, srj = sl[il], sr[il], sl[jl], sr[jl]
+        a = np.asarray(a_row)[il]
+        Li = np.linalg.norm(sri - sli, axis=1)
+        Lj = np.linalg.norm(srj - slj, axis=1)
+        pi = (1 - t01[None, :, None]) * sli[:, None, :] + t01[None, :, None] * sri[:, None, :]
+        pj = (1 - t01[None, :, None]) * slj[:, None, :] + t01[None, :, None] * srj[:, None, :]
+        diff = pi[:, :, None, None, :] - pj[None, None, :, :, :]
+        R = np.sqrt((diff * diff).sum(-1) + (a * a)[:, None, None, None])
+        G = np.exp(-1j * k * R) / (4 * np.pi * R)
+        ui, uj = t01[None, :] * Li[:, None], t01[None, :] * Lj[:, None]
+        wi, wj = w01[None, :] * Li[:, None], w01[None, :] * Lj[:, None]
+        wui = np.stack([wi * ui ** p for p in range(d + 1)])
+        wuj = np.stack([wj * uj ** p for p in range(d + 1)])
+        return np.einsum("piq,iqjr,Pjr->pPij", wui, G, wuj)
+
+    def reg_moments(self, ed_arc, a, k):
+        """_bspline_kernels.py:425-517 (smooth remainder on one edge)."""
+        d = self.degree
+        t01, w01 = gl01(self.n_qp_pair)
+        sl = ed_arc[:-1]
+        h = ed_arc[1:] - sl
+        N = len(sl)
+        s = (sl[:, None] + t01[None, :] * h[:, None]).ravel()
+        u = t01[None, :] * h[:, None]
+        w = w01[None, :] * h[:, None]
+        R = np.sqrt((s[:, None] - s[None, :]) ** 2 + a * a)
+        G = ((np.exp(-1j * k * R) - 1.0) / (4 * np.pi * R)).reshape(N, -1, N, len(t01))
+        wu = np.stack([w * u ** p for p in range(d + 1)])
+        return np.einsum("piq,iqjr,Pjr->pPij", wu, G, wu)
+
+    def J_blocks(self, geom, k):
+        """bspline.py:2222-2283: all pairs, then same-edge overwrite."""
+        d = self.degree
+        a_row = self.seg_radius(geom)
+        J = self.full_moments(geom, a_row, k)
+        for w, pw in enumerate(geom["per_wire"]):
+            base = geom["seg_off"][w]
+            for e in range(len(pw["ed_off"]) - 1):
+                sl = slice(base + pw["ed_off"][e], base + pw["ed_off"][e + 1])
+                arc = pw["ed_arc"][e]
+                n_e = len(arc) - 1
+                h = float(arc[1] - arc[0])
+                A_st = static_moments_uniform(h, float(self.radius[w]), n_e, d)
+                A_reg = self.reg_moments(arc, float(self.radius[w]), k)
+                J[:, :, sl, sl] = A_st + A_reg
+        return J
+
+    def assemble(self, J, supp, polys, geom):
+        """bspline.py:2322-2348 (numpy einsum assembly)."""
+        d = self.degree
+        td = geom["tangents"] @ geom["tangents"].T
+        nb = supp.shape[0]
+        ZA = np.zeros((nb, nb), dtype=complex)
+        ZP = np.zeros((nb, nb), dtype=complex)
+        pv = np.arange(1, d + 1, dtype=float)
+        for a in range(d + 1):
+            sm = supp[:, a]
+            for b in range(d + 1):
+                sn = supp[:, b]
+                Jb = J[:, :, sm[:, None], sn[None, :]]
+                ZA += td[sm[:, None], sn[None, :]] * np.einsum(
+                    "mp,pPmn,nP->mn", polys[:, a, :], Jb, polys[:, b, :])
+                dm = polys[:, a, 1:] * pv[None, :]
+                dn = polys[:, b, 1:] * pv[None, :]
+                ZP += np.einsum("mp,pPmn,nP->mn", dm, Jb[:d, :d], dn)
+        return 1j * self.omega * MU0 * ZA + ZP / (1j * self.omega * EPS0)
+
+    def Z(self, geom=None, basis=None):
+        geom = geom or self.geometry()
+        supp, polys = (basis or self.basis(geom))[:2]
+        return self.assemble(self.J_blocks(geom, self.k), supp, polys, geom)
+
+    # -- ports: bspline.py:2806-2870, 3784-3867, 1170-1210 ------------------
+    def port_columns(self, geom, basis):
+        supp, polys, kcl, knots, wbg = basis
+        d = self.degree
+        nb = supp.shape[0]
+        B = np.zeros((nb, len(self.feeds)), dtype=complex)
+        for f, (w, s_f, _v) in enumerate(self.feeds):
+            arc = geom["per_wire"][w]["arc"]
+            s_f = arc[-1] / 2.0 if s_f is None else s_f
+            DM = BSpline.design_matrix(np.array([s_f]), knots[w], d).toarray()[0]
+            kept, l2g = wbg[w]
+            for kidx, (j, *_r) in enumerate(kept):
+                B[l2g[kidx], f] = DM[j]
+        port_rows = [j for j, _v in self.junction_ports]
+        keep = [r for r in range(kcl.shape[0]) if r not in set(port_rows)]
+        kcl_con, port_A = kcl[keep], kcl[port_rows]
+        gap = np.zeros((nb, len(self.node_gaps)))
+        end_to_j = {tuple(mm): j for j, jw in enumerate(self.junctions) for mm in jw}
+        for p, (w, end, _v) in enumerate(self.node_gaps):
+            jidx = end_to_j[(w, end)]
+            kept, l2g = wbg[w]
+            for kidx, (_j, kind, ji, endp) in enumerate(kept):
+                if kind == "dir" and ji == jidx and endp == end:
+                    gap[l2g[kidx], p] = 1.0 if end == "start" else -1.0
+        return np.hstack([B, port_A.T.astype(complex), gap.astype(complex)]), kcl_con
+
+    @staticmethod
+    def solve_kcl(Z, V, A):  # bspline.py:2971-2989
+        if A.shape[0] == 0:
+            return scipy.linalg.solve(Z, V)
+        sol = scipy.linalg.solve(Z, np.hstack([V, A.T.astype(complex)]))
+        W, X = sol[:, :V.shape[1]], sol[:, V.shape[1]:]
+        return W - X @ scipy.linalg.solve(A @ X, A @ W)
+
+    def compute_port_solution(self):
+        """bspline.py:4053-4179 -> (y, coeffs)."""
+        geom = self.geometry()
+        basis = self.basis(geom)
+        Z = self.Z(geom, basis)
+        B, kcl_con = self.port_columns(geom, basis)
+        X = self.solve_kcl(Z, B, kcl_con)
+        return B.T @ X, X
+
+    def currents_at_knots(self, coeffs):  # bspline.py:4603-4685 (no enrichment)
+        geom = self.geometry()
+        _s, _p, _k, knots, wbg = self.basis(geom)
+        d = self.degree
+        out = []
+        for w, pw in enumerate(geom["per_wire"]):
+            kv = knots[w]
+            s_eval = np.clip(pw["arc"], kv[0], kv[-1])
+            c = np.zeros(len(kv) - d - 1, dtype=complex)
+            kept, l2g = wbg[w]
+            for kidx, (j, *_r) in enumerate(kept):
+                c[j] = coeffs[l2g[kidx]]
+            out.append(BSpline.design_matrix(s_eval, kv, d).toarray() @ c)
+        return out

@@ -1,0 +1,141 @@
+"""Pin the CPU oracle before anything is compared with it.
+
+* end to end against the reference solver's own golden vectors (1e-10);
+* kernel by kernel against the reference's compiled accelerator (oracle/_ref);
+* static moments against mpmath quadrature (closed form + far-pair rule);
+* basis polynomials against partition of unity.
+"""
+
+import numpy as np
+import pytest
+
+from conftest import golden, rel_err
+from momwire_amd import workloads as W
+from oracle.bspline_oracle import OracleSolver, gl01
+from oracle.static_moments import static_moment, static_moments_uniform
+
+SMALL = [
+    ("c1_dipole", W.straight_dipole()),
+    ("array2x2", W.dipole_array(2, 2, 11, seed=5)),
+    ("tee_feed", W.junction_tee("feed")),
+    ("tee_jport", W.junction_tee("junction_port")),
+    ("tee_nodegap", W.junction_tee("node_gap")),
+]
+
+
+@pytest.mark.parametrize("name,kw", SMALL, ids=[n for n, _ in SMALL])
+def test_oracle_matches_reference_golden(name, kw):
+    g = golden(name)
+    s = OracleSolver(**kw)
+    y, X = s.compute_port_solution()
+    assert rel_err(y, g["y"]) < 1e-10
+    assert rel_err(X, g["coeffs"]) < 1e-10
+    if "Z" in g:
+        assert rel_err(s.Z(), g["Z"]) < 1e-10
+    if "knot_currents" in g:
+        kc = np.concatenate(s.currents_at_knots(X[:, 0]))
+        assert rel_err(kc, g["knot_currents"]) < 1e-10
+    if "sweep_y" in g:
+        ys = []
+        for kk in g["sweep_k"]:
+            s.set_k(kk)
+            ys.append(s.compute_port_solution()[0])
+        assert rel_err(np.array(ys), g["sweep_y"]) < 1e-10
+
+
+@pytest.mark.slow
+def test_oracle_matches_reference_golden_8x8_array():
+    g = golden("c2_array8x8")
+    s = OracleSolver(**W.dipole_array(8, 8, 41, seed=1))
+    y, X = s.compute_port_solution()
+    assert rel_err(y, g["y"]) < 1e-10
+    assert rel_err(X[:, g["coeffs_cols"]], g["coeffs"]) < 1e-10
+
+
+def test_static_moments_vs_mpmath():
+    mp = pytest.importorskip("mpmath")
+    mp.mp.dps = 30
+    h, a = 0.0117, 1e-3
+    for p, q, d in [(0, 0, 0), (2, 2, 0), (1, 2, 1), (2, 1, -1), (2, 2, 2), (0, 2, 7), (2, 2, 39)]:
+        H, A, c = mp.mpf(h), mp.mpf(a), d * mp.mpf(h)
+
+        def inner(u):
+            f = lambda v: u**p * v**q / mp.sqrt((u - v - c) ** 2 + A * A)  # noqa: E731
+            pts = [0, u - c, H] if 0 < u - c < H else [0, H]
+            return mp.quad(f, pts)
+
+        exact = float(mp.quad(inner, [0, H]))
+        assert abs(static_moment(p, q, d, h, a) - exact) <= 1e-13 * abs(exact), (p, q, d)
+
+
+def test_static_moments_symmetry_and_small_radius():
+    h, a, N = 0.3, 2e-3, 9
+    S = static_moments_uniform(h, a, N, 2)
+    # J_pq[i, j] = J_qp[j, i]: swapping observer and source swaps p and q.
+    assert np.allclose(S, S.transpose(1, 0, 3, 2), rtol=1e-13, atol=0)
+    # p = q = 0 self term as a -> 0: 2[h asinh(h/a) - sqrt(h^2+a^2) + a] / (4 pi)
+    exact = 2 * (h * np.arcsinh(h / a) - np.sqrt(h * h + a * a) + a) / (4 * np.pi)
+    assert abs(S[0, 0, 3, 3] - exact) < 1e-14 * exact
+
+
+def test_static_moments_vs_reference_accelerator(ref_accel):
+    h, a, N = 0.0117, 1e-3, 41
+    ref = ref_accel.seg_seg_static_moments_bspline_uniform(h, a, N, 2)
+    mine = static_moments_uniform(h, a, N, 2)
+    # Near pairs agree to rounding; far pairs carry the reference's
+    # closed-form cancellation, which stays below 1e-5 of each moment.
+    near = np.abs(np.arange(N)[None, :] - np.arange(N)[:, None]) <= 2
+    r = np.abs(ref - mine) / np.abs(mine)
+    assert r[:, :, near].max() < 1e-12
+    assert r.max() < 1e-5
+
+
+def test_full_and_reg_moments_vs_reference_accelerator(ref_accel):
+    s = OracleSolver(**W.junction_tee("feed"))
+    geom = s.geometry()
+    k = s.k
+    t01, w01 = gl01(4)
+    a = float(s.radius[0])
+    ref = ref_accel.seg_seg_full_moments_bspline(
+        geom["seg_l"], geom["seg_r"], geom["seg_l"], geom["seg_r"], a * a, k, 2, t01, w01)
+    mine = s.full_moments(geom, s.seg_radius(geom), k)
+    assert rel_err(mine, ref) < 1e-13
+    arc = geom["per_wire"][1]["ed_arc"][0]
+    sl = arc[:-1]
+    hh = arc[1:] - sl
+    sq = (sl[:, None] + t01[None, :] * hh[:, None]).ravel()
+    R = np.sqrt((sq[:, None] - sq[None, :]) ** 2 + a * a)
+    u = t01[None, :] * hh[:, None]
+    w = w01[None, :] * hh[:, None]
+    wu = np.stack([w * u**p for p in range(3)])
+    ref_reg = ref_accel.seg_seg_reg_moments_bspline_swept(R, wu, np.array([k]))[0]
+    assert rel_err(s.reg_moments(arc, a, k), ref_reg) < 1e-13
+
+
+def test_assembly_vs_reference_accelerator(ref_accel):
+    s = OracleSolver(**W.junction_tee("feed"))
+    geom = s.geometry()
+    supp, polys = s.basis(geom)[:2]
+    J = s.J_blocks(geom, s.k)
+    td = geom["tangents"] @ geom["tangents"].T
+    ref = ref_accel.assemble_Z_bspline(J, supp, polys, td, s.omega, 8.8541878188e-12,
+                                       1.25663706127e-6, 2, 0)
+    assert rel_err(s.assemble(J, supp, polys, geom), ref) < 1e-13
+
+
+def test_basis_partition_of_unity():
+    # With junctions at both ends every B-spline is kept, so the kept bases'
+    # polynomials must sum to 1 on every segment at every local u.
+    kw = dict(wires=[[(0, 0, 0), (1, 0, 0)]], n_per_edge_per_wire=[[6]], degree=2,
+              junctions=[[(0, "start")], [(0, "end")]], wavelength=2.0)
+    s = OracleSolver(**kw)
+    geom = s.geometry()
+    supp, polys = s.basis(geom)[:2]
+    h = geom["h"][0]
+    for u in (0.0, 0.3 * h, h):
+        tot = np.zeros(geom["n_segs"])
+        for m in range(supp.shape[0]):
+            for a in range(3):
+                if np.any(polys[m, a]):
+                    tot[supp[m, a]] += polys[m, a] @ (u ** np.arange(3))
+        assert np.allclose(tot, 1.0, atol=1e-13)
