@@ -1,0 +1,158 @@
+/*
+ * momwire_hip.h -- C ABI of the MI355X (gfx950) B-spline port-solution path.
+ *
+ * Replaces, for BSplineSolver.compute_port_solution(), the reference's
+ * pybind11 accelerator module (reference src/momwire/_accelerators.cpp) and
+ * the LAPACK solve it calls through scipy (bspline.py:2971-2989).  Every entry
+ * point takes plain pointers and sizes: device pointers for arrays (any HIP
+ * allocation, e.g. torch tensors' data_ptr()), a hipStream_t passed as void*,
+ * and returns an int status (0 = ok; see mw_last_error() for the message).
+ * Work is enqueued asynchronously on the stream unless stated otherwise.
+ *
+ * Complex values are interleaved (re, im) pairs: double2 for the FP64 entry
+ * points (z*), float2 for the FP32 ones (c*).  Matrices are row-major with an
+ * explicit leading dimension counted in complex elements.  Z is complex
+ * symmetric, so row- and column-major storage describe the same matrix.
+ */
+#ifndef MOMWIRE_HIP_H
+#define MOMWIRE_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  MW_OK = 0,
+  MW_EINVAL = 1,
+  MW_ESINGULAR = 2,
+  MW_EHIP = 3,
+  MW_EABORT = 4,
+  MW_ERANGE = 5
+};
+
+/* Maximum segments one basis tile may touch (tiles hold <= 16 bases). */
+#define MW_TSMAX 18
+/* Basis tile edge (bases per tile side). */
+#define MW_TB 16
+
+/*
+ * Device-resident, k-independent description of a meshed structure.  Built
+ * once per geometry by the host (momwire_amd/_mesh.py) and reused for every
+ * frequency.  All pointers are device pointers.
+ */
+typedef struct mw_mesh {
+  int32_t n_seg;     /* segments over all wires                              */
+  int32_t n_edge;    /* straight edges (uniform segmentation on each)         */
+  int32_t n_basis;   /* B-spline unknowns                                     */
+  int32_t n_tile;    /* basis tiles partitioning [0, n_basis)                 */
+  int32_t n_se;      /* same-edge table entries: sum over edges of 2 n_e - 1  */
+  int32_t degree;    /* B-spline degree (2)                                   */
+  int32_t n_qp;      /* Gauss-Legendre points per segment per axis (4)        */
+  int32_t pad0;
+  const double* seg;       /* [n_seg][12] l.xyz r.xyz t.xyz length radius 0   */
+  const int32_t* seg_edge; /* [n_seg][2]  edge id, index along the edge       */
+  const double* edge;      /* [n_edge][2] segment length h, wire radius a     */
+  const int32_t* edge_i;   /* [n_edge][2] n_e, offset of the edge's table     */
+  const int32_t* se_edge;  /* [n_se] edge owning each same-edge table entry   */
+  void* se_tab;            /* [n_se][9] double2: static + smooth same-edge
+                              moments J_pq(j - i), written by
+                              mw_same_edge_table()                            */
+  const double* band_poly; /* [n_seg][3 slot][3 power] polynomial of the basis
+                              occupying each band slot of a segment           */
+  const int32_t* bw_loc;   /* [n_basis][3] wing segment's index in the tile's
+                              segment list, -1 for an absent wing             */
+  const int32_t* bw_slot;  /* [n_basis][3] band slot of the basis on that
+                              segment                                         */
+  const int32_t* tile_b0;  /* [n_tile+1] first basis of each tile            */
+  const int32_t* tile_seg; /* [n_tile][MW_TSMAX] sorted segment list          */
+  const int32_t* tile_nseg;/* [n_tile]                                        */
+  const double* gl;        /* [2 n_qp] t, w on [0,1]; then [32] 16-point
+                              Gauss-Legendre x, w on [-1,1] (far static rule) */
+} mw_mesh;
+
+/* Last error message of the calling thread ("" if none). */
+const char* mw_last_error(void);
+/* ABI version of this library. */
+int mw_abi_version(void);
+
+/*
+ * Same-edge moment table: J_pq(delta) = static (closed form, 1/sqrt kernel)
+ * + smooth remainder (exp(-jkR)-1)/(4 pi R) for every edge and offset delta.
+ * Replaces seg_seg_static_moments_bspline_uniform (_accelerators.cpp:2822-2951)
+ * and seg_seg_reg_moments_bspline_swept (_accelerators.cpp:165-327), which the
+ * reference calls per edge (_bspline_kernels.py:333-517).
+ */
+int mw_same_edge_table(const mw_mesh* mesh, double k, void* stream);
+
+/*
+ * Fused Galerkin fill Z[m][n] for all basis pairs, free space.  Replaces
+ * seg_seg_full_moments_bspline (_accelerators.cpp:349-542) followed by
+ * assemble_Z_bspline (_accelerators.cpp:1196-1309) or its windowed twin
+ * (_accelerators.cpp:1312-1476), as driven by bspline.py:2222-2550.
+ * tile_pairs: [n_pairs][2] (ti, tj) device array; symmetric != 0 means only
+ * ti <= tj is listed and the transposed tile is written too (valid when every
+ * wire shares one radius).  Z: n_basis x n_basis double2, row stride ldz.
+ * Needs mw_same_edge_table() for the same k first.
+ */
+int mw_fill_z(const mw_mesh* mesh, double k, double omega, double eps, double mu,
+              void* Z, int64_t ldz, const int32_t* tile_pairs, int32_t n_pairs,
+              int32_t symmetric, void* stream);
+
+/*
+ * LU factorisation with row pivoting (tournament pivot selection per panel),
+ * in place: A = P L U, L unit lower.  Replaces LAPACK zgetrf/cgetrf behind
+ * scipy.linalg.solve (bspline.py:2971-2989).  ipiv: [n] device int32,
+ * LAPACK-style sequential row interchanges (0-based).  work: device scratch of
+ * mw_getrf_workspace_bytes(n) bytes.  info: device int32, set to 1 + the
+ * first zero-pivot column (0 if none).  cancel: host int32 polled between
+ * panels (may be NULL); a set flag stops the factorisation with MW_EABORT.
+ */
+int64_t mw_getrf_workspace_bytes(int64_t n);
+int mw_zgetrf(int64_t n, void* A, int64_t lda, int32_t* ipiv, void* work,
+              int32_t* info, const volatile int32_t* cancel, void* stream);
+int mw_cgetrf(int64_t n, void* A, int64_t lda, int32_t* ipiv, void* work,
+              int32_t* info, const volatile int32_t* cancel, void* stream);
+
+/*
+ * Multi-RHS solve with the factors of mw_?getrf: X <- A^{-1} X for nrhs
+ * columns (X: n x nrhs, row stride ldx).  Replaces LAPACK zgetrs.
+ */
+int mw_zgetrs(int64_t n, int64_t nrhs, const void* A, int64_t lda,
+              const int32_t* ipiv, void* X, int64_t ldx, void* stream);
+int mw_cgetrs(int64_t n, int64_t nrhs, const void* A, int64_t lda,
+              const int32_t* ipiv, void* X, int64_t ldx, void* stream);
+
+/* C <- C - A B (complex, MFMA).  A: m x k, B: k x n, C: m x n. */
+int mw_zgemm_sub(int64_t m, int64_t n, int64_t k, const void* A, int64_t lda,
+                 const void* B, int64_t ldb, void* C, int64_t ldc, void* stream);
+int mw_cgemm_sub(int64_t m, int64_t n, int64_t k, const void* A, int64_t lda,
+                 const void* B, int64_t ldb, void* C, int64_t ldc, void* stream);
+
+/*
+ * out[i][j] = sum_t val[i][t] * X[idx[i][t]][j]  (sparse rows times X).
+ * The port readout Y = B^T X of bspline.py:4178-4179 (B's columns are sparse:
+ * a delta-gap drive touches d+1 bases) and the KCL products A X of
+ * bspline.py:2981-2989.  idx < 0 marks padding.
+ */
+int mw_zsparse_rows(int64_t nrows, int32_t maxnz, const int32_t* idx,
+                    const double* val, const void* X, int64_t ldx, int64_t ncols,
+                    void* out, int64_t ldo, void* stream);
+
+/* X[:, :] = 0, then X[idx[c][t]][c] = val[c][t]: dense right-hand sides. */
+int mw_zscatter_cols(int64_t n, int64_t ncols, int32_t maxnz, const int32_t* idx,
+                     const double* val, void* X, int64_t ldx, void* stream);
+
+/*
+ * Small dense solve T <- S^{-1} T in one workgroup (S: ns x ns, T: ns x nt,
+ * ns, nt <= 64): the Schur-complement multiplier solve of bspline.py:2988.
+ */
+int mw_zsmall_solve(int32_t ns, int32_t nt, void* S, int64_t lds, void* T,
+                    int64_t ldt, int32_t* info, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MOMWIRE_HIP_H */

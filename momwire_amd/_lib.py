@@ -1,0 +1,126 @@
+"""ctypes binding of libmomwire_hip.so (the C ABI in include/momwire_hip.h).
+
+The shared library is built in-tree by ``python -m momwire_amd.build``.  There
+is no fallback: if the library is missing or fails to load, every solve raises
+``AcceleratorUnavailable`` -- the port-solution path runs on the GPU or not at
+all.  ctypes releases the GIL for the duration of each foreign call, so a long
+factorisation does not block other Python threads (the property the
+reference's accelerators give by releasing the GIL, reference
+tests/test_gil_release.py:43-105).
+"""
+
+import ctypes
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libmomwire_hip.so")
+
+MW_OK, MW_EINVAL, MW_ESINGULAR, MW_EHIP, MW_EABORT, MW_ERANGE = range(6)
+TSMAX = 18  # MW_TSMAX
+TB = 16     # MW_TB
+
+
+class AcceleratorUnavailable(RuntimeError):
+    """The HIP library could not be loaded (not built, or no usable GPU)."""
+
+
+class SolveAborted(RuntimeError):
+    """A solve stopped because its cancel token was tripped (reference
+    src/momwire/_cancel.py:20-79 raises the same-named exception)."""
+
+
+class MwMesh(ctypes.Structure):
+    _fields_ = [
+        ("n_seg", ctypes.c_int32),
+        ("n_edge", ctypes.c_int32),
+        ("n_basis", ctypes.c_int32),
+        ("n_tile", ctypes.c_int32),
+        ("n_se", ctypes.c_int32),
+        ("degree", ctypes.c_int32),
+        ("n_qp", ctypes.c_int32),
+        ("pad0", ctypes.c_int32),
+        ("seg", ctypes.c_void_p),
+        ("seg_edge", ctypes.c_void_p),
+        ("edge", ctypes.c_void_p),
+        ("edge_i", ctypes.c_void_p),
+        ("se_edge", ctypes.c_void_p),
+        ("se_tab", ctypes.c_void_p),
+        ("band_poly", ctypes.c_void_p),
+        ("bw_loc", ctypes.c_void_p),
+        ("bw_slot", ctypes.c_void_p),
+        ("tile_b0", ctypes.c_void_p),
+        ("tile_seg", ctypes.c_void_p),
+        ("tile_nseg", ctypes.c_void_p),
+        ("gl", ctypes.c_void_p),
+    ]
+
+
+_P = ctypes.c_void_p
+_I64 = ctypes.c_int64
+_I32 = ctypes.c_int32
+_D = ctypes.c_double
+
+# name -> (restype, argtypes)
+SIGNATURES = {
+    "mw_last_error": (ctypes.c_char_p, []),
+    "mw_abi_version": (ctypes.c_int, []),
+    "mw_same_edge_table": (ctypes.c_int, [ctypes.POINTER(MwMesh), _D, _P]),
+    "mw_fill_z": (ctypes.c_int, [ctypes.POINTER(MwMesh), _D, _D, _D, _D, _P, _I64, _P, _I32,
+                                 _I32, _P]),
+    "mw_getrf_workspace_bytes": (_I64, [_I64]),
+    "mw_zgetrf": (ctypes.c_int, [_I64, _P, _I64, _P, _P, _P, _P, _P]),
+    "mw_cgetrf": (ctypes.c_int, [_I64, _P, _I64, _P, _P, _P, _P, _P]),
+    "mw_zgetrs": (ctypes.c_int, [_I64, _I64, _P, _I64, _P, _P, _I64, _P]),
+    "mw_cgetrs": (ctypes.c_int, [_I64, _I64, _P, _I64, _P, _P, _I64, _P]),
+    "mw_zgemm_sub": (ctypes.c_int, [_I64, _I64, _I64, _P, _I64, _P, _I64, _P, _I64, _P]),
+    "mw_cgemm_sub": (ctypes.c_int, [_I64, _I64, _I64, _P, _I64, _P, _I64, _P, _I64, _P]),
+    "mw_zsparse_rows": (ctypes.c_int, [_I64, _I32, _P, _P, _P, _I64, _I64, _P, _I64, _P]),
+    "mw_zscatter_cols": (ctypes.c_int, [_I64, _I64, _I32, _P, _P, _P, _I64, _P]),
+    "mw_zsmall_solve": (ctypes.c_int, [_I32, _I32, _P, _I64, _P, _I64, _P, _P]),
+}
+
+_lib = None
+_load_error = None
+
+
+def load():
+    """Return the loaded library, raising AcceleratorUnavailable if absent."""
+    global _lib, _load_error
+    if _lib is not None:
+        return _lib
+    if _load_error is not None:
+        raise AcceleratorUnavailable(_load_error)
+    if not os.path.exists(LIB_PATH):
+        _load_error = (f"{LIB_PATH} is not built; run `python -m momwire_amd.build` "
+                       "(the port-solution path has no CPU fallback)")
+        raise AcceleratorUnavailable(_load_error)
+    try:
+        lib = ctypes.CDLL(LIB_PATH)
+    except OSError as e:
+        _load_error = f"failed to load {LIB_PATH}: {e}"
+        raise AcceleratorUnavailable(_load_error) from e
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def check(rc, what):
+    """Map a C-ABI status code to the reference's exception types."""
+    if rc == MW_OK:
+        return
+    msg = load().mw_last_error().decode(errors="replace")
+    text = f"{what}: {msg}" if msg else f"{what}: status {rc}"
+    if rc in (MW_EINVAL,):
+        raise ValueError(text)
+    if rc == MW_ERANGE:
+        raise ValueError(text)
+    if rc == MW_EABORT:
+        raise SolveAborted(text)
+    if rc == MW_ESINGULAR:
+        import numpy as np
+
+        raise np.linalg.LinAlgError(text)
+    raise RuntimeError(text)

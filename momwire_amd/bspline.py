@@ -1,0 +1,481 @@
+"""BSplineSolver: the reference's degree-2 B-spline Galerkin port solve, on MI355X.
+
+Drop-in for ``momwire.BSplineSolver(...).compute_port_solution()`` (reference
+src/momwire/bspline.py:4053-4179): the same constructor keywords for the
+supported configuration space, the same port order ([feeds..., junction
+ports..., node gaps...]), the same ``PortSolution`` result.  The Z fill, the
+LU factorisation, the all-port solve, the KCL Schur step and the Y readout run
+as HIP kernels on the GPU (momwire_amd/csrc); Python only prepares the
+k-independent tables and moves results.  There is no CPU fallback.
+
+Supported: free space; degree 2; n_qp_pair 4; point delta-gap feeds; K-wire
+junctions (KCL rows), junction ports and node gaps; per-wire radii.
+Refused with NotImplementedError (as the reference refuses combinations it
+does not serve): ground planes, wire loading, extended kernel, singular
+enrichment, segment-gap and smoothed feeds.
+"""
+
+import numpy as np
+
+from . import _lib
+from ._mesh import EPS0, MU0, build_mesh, feed_column, sparse_pack, bspline_values
+from ._port_solution import PortSolution, _SweptPortSolutions
+from ._lib import SolveAborted  # noqa: F401  (re-exported, reference _cancel.py)
+
+
+class CancelToken:
+    """Cooperative cancel flag polled between LU panels and solve stages
+    (reference src/momwire/_cancel.py:20-79)."""
+
+    def __init__(self):
+        self._flag = np.zeros(1, dtype=np.int32)
+
+    def cancel(self):
+        self._flag[0] = 1
+
+    def reset(self):
+        self._flag[0] = 0
+
+    @property
+    def cancelled(self):
+        return bool(self._flag[0])
+
+    @property
+    def address(self):
+        return self._flag.ctypes.data
+
+
+def _torch():
+    import torch
+
+    if not torch.cuda.is_available():
+        raise _lib.AcceleratorUnavailable("no GPU visible to torch (HIP path needs an MI355X)")
+    return torch
+
+
+class _Device:
+    """Device copies of the mesh tables plus reusable work buffers."""
+
+    def __init__(self, mesh, cols, rows_con, rows_port):
+        torch = _torch()
+        self.torch = torch
+        dev = torch.device("cuda")
+        self.dev = dev
+
+        def up(a, dt=None):
+            t = torch.from_numpy(np.ascontiguousarray(a if dt is None else a.astype(dt)))
+            return t.to(dev)
+
+        self.keep = dict(
+            seg=up(mesh.seg, np.float64), seg_edge=up(mesh.seg_edge, np.int32),
+            edge=up(mesh.edge, np.float64), edge_i=up(mesh.edge_i, np.int32),
+            se_edge=up(mesh.se_edge, np.int32),
+            se_tab=torch.empty((max(len(mesh.se_edge), 1), 9), dtype=torch.complex128, device=dev),
+            band_poly=up(mesh.band_poly, np.float64), bw_loc=up(mesh.bw_loc, np.int32),
+            bw_slot=up(mesh.bw_slot, np.int32), tile_b0=up(mesh.tile_b0, np.int32),
+            tile_seg=up(mesh.tile_seg, np.int32), tile_nseg=up(mesh.tile_nseg, np.int32),
+            gl=up(mesh.gl, np.float64), pairs=up(mesh.tile_pairs, np.int32),
+        )
+        k = self.keep
+        self.mw = _lib.MwMesh(
+            n_seg=mesh.n_seg, n_edge=len(mesh.edge), n_basis=mesh.n_basis,
+            n_tile=len(mesh.tile_nseg), n_se=len(mesh.se_edge), degree=mesh.degree, n_qp=4,
+            pad0=0,
+            seg=k["seg"].data_ptr(), seg_edge=k["seg_edge"].data_ptr(), edge=k["edge"].data_ptr(),
+            edge_i=k["edge_i"].data_ptr(), se_edge=k["se_edge"].data_ptr(),
+            se_tab=k["se_tab"].data_ptr(), band_poly=k["band_poly"].data_ptr(),
+            bw_loc=k["bw_loc"].data_ptr(), bw_slot=k["bw_slot"].data_ptr(),
+            tile_b0=k["tile_b0"].data_ptr(), tile_seg=k["tile_seg"].data_ptr(),
+            tile_nseg=k["tile_nseg"].data_ptr(), gl=k["gl"].data_ptr())
+        nb = mesh.n_basis
+        self.nb = nb
+        self.n_ports = len(rows_port)
+        self.n_con = len(rows_con)
+        self.ncols = len(cols)
+        ci, cv, cmax = sparse_pack(cols)
+        self.col_idx, self.col_val, self.col_max = up(ci), up(cv), cmax
+        pi, pv, pmax = sparse_pack(rows_port)
+        self.port_idx, self.port_val, self.port_max = up(pi), up(pv), pmax
+        if self.n_con:
+            ai, av, amax = sparse_pack(rows_con)
+            self.con_idx, self.con_val, self.con_max = up(ai), up(av), amax
+        self.Z = None
+        self.X = None
+
+    def buffers(self):
+        torch = self.torch
+        if self.Z is None:
+            nb = self.nb
+            self.Z = torch.empty((nb, nb), dtype=torch.complex128, device=self.dev)
+            self.X = torch.empty((nb, max(self.ncols, 1)), dtype=torch.complex128, device=self.dev)
+            self.ipiv = torch.empty(nb, dtype=torch.int32, device=self.dev)
+            lib = _lib.load()
+            self.work = torch.empty(int(lib.mw_getrf_workspace_bytes(nb)), dtype=torch.uint8,
+                                    device=self.dev)
+            self.info = torch.zeros(2, dtype=torch.int32, device=self.dev)
+            self.Y = torch.empty((max(self.n_ports, 1), max(self.n_ports, 1)),
+                                 dtype=torch.complex128, device=self.dev)
+            if self.n_con:
+                self.S = torch.empty((self.n_con, self.n_con), dtype=torch.complex128,
+                                     device=self.dev)
+                self.T = torch.empty((self.n_con, max(self.n_ports, 1)), dtype=torch.complex128,
+                                     device=self.dev)
+        return self
+
+
+class BSplineSolver(_SweptPortSolutions):
+    """Degree-2 B-spline Galerkin MoM port solve on the GPU (see module doc)."""
+
+    eps = EPS0
+    mu = MU0
+
+    def __init__(self, *, wires, n_per_edge_per_wire=None, degree=2, feed_wire_index=0,
+                 feed_arclength=None, feeds=None, feed_model="point",
+                 feed_smoothing_factor=None, junctions=None, junction_ports=None,
+                 node_gaps=None, n_qp_pair=4, n_qp_source=16, extended_kernel=False,
+                 wavelength=22, halfdriver_factor=0.962, wire_radius=0.0005,
+                 wire_conductivity=None, insulation_radius=None, insulation_eps_r=None,
+                 nsegs=101, ground_z=None, ground_eps=None, ground_phi_mode="normal",
+                 ground_model="refl-coef", n_qp_sommerfeld=3, use_singular_enrichment=False,
+                 swept_mem_mb=256, cancel=None, **unsupported):
+        if unsupported:
+            raise NotImplementedError(f"options not served on the GPU path: {sorted(unsupported)}")
+        if degree < 1:
+            raise ValueError(f"degree must be >= 1, got {degree}")
+        if degree != 2:
+            raise NotImplementedError("the GPU path is built for degree 2")
+        if not wires:
+            raise ValueError("wires must be non-empty")
+        if int(n_qp_pair) != 4:
+            raise NotImplementedError("the GPU fill is built for n_qp_pair=4")
+        if feed_model not in ("point", "segment"):
+            raise ValueError(f"feed_model must be 'point' or 'segment', got {feed_model!r}")
+        if feed_model != "point" or feed_smoothing_factor is not None:
+            raise NotImplementedError("only the point delta-gap feed is served")
+        if extended_kernel or use_singular_enrichment:
+            raise NotImplementedError("extended kernel / singular enrichment are not served")
+        if wire_conductivity is not None or insulation_radius is not None or insulation_eps_r is not None:
+            raise NotImplementedError("wire loading is not served")
+        if ground_eps is not None and ground_z is None:
+            raise ValueError("ground_eps requires ground_z to be set")
+        if ground_z is not None:
+            raise NotImplementedError("ground planes are not served by this solver yet")
+        if int(swept_mem_mb) < 1:
+            raise ValueError(f"swept_mem_mb must be >= 1, got {swept_mem_mb}")
+
+        self.degree = int(degree)
+        self.n_qp_pair = 4
+        self.wavelength = wavelength
+        self.halfdriver_factor = halfdriver_factor
+        self.wire_radius = wire_radius
+        self.nsegs = nsegs
+        self.ground_z = None
+        self.swept_mem_mb = int(swept_mem_mb)
+        self._cancel = cancel
+        # bspline.py:672-676
+        self.c = 1 / np.sqrt(self.eps * self.mu)
+        self.freq = self.c / self.wavelength
+        self.omega = 2 * np.pi * self.freq
+        self.k = self.omega / self.c
+
+        self.wires_polylines = [np.asarray(w, dtype=float) for w in wires]
+        n_w = len(self.wires_polylines)
+        for i, pl in enumerate(self.wires_polylines):
+            if pl.ndim != 2 or pl.shape[0] < 2 or pl.shape[1] != 3:
+                raise ValueError(f"wire {i}: polyline must be (M, 3) with M >= 2")
+        r = np.asarray(wire_radius, dtype=float)
+        if r.ndim == 0:
+            r = np.full(n_w, float(r))
+        elif r.shape != (n_w,):
+            raise ValueError(f"wire_radius: expected a scalar or a length-{n_w} sequence")
+        if not np.all(np.isfinite(r)) or np.any(r <= 0):
+            raise ValueError(f"wire_radius entries must be positive and finite, got {r}")
+        self._radius_per_wire = r
+
+        if n_per_edge_per_wire is None:
+            n_per_edge_per_wire = [None] * n_w
+        if len(n_per_edge_per_wire) != n_w:
+            raise ValueError(f"n_per_edge_per_wire length {len(n_per_edge_per_wire)} != n_wires {n_w}")
+        self.n_per_edge_per_wire = []
+        for i, (pl, npe) in enumerate(zip(self.wires_polylines, n_per_edge_per_wire)):
+            ne = pl.shape[0] - 1
+            if npe is None:
+                npe = self.nsegs
+            if np.isscalar(npe):
+                npe = [int(npe)] * ne
+            npe = list(npe)
+            if len(npe) != ne:
+                raise ValueError(f"wire {i}: n_per_edge length {len(npe)} != n_edges {ne}")
+            self.n_per_edge_per_wire.append([int(x) for x in npe])
+
+        if feeds is None:
+            if not (0 <= feed_wire_index < n_w):
+                raise ValueError(f"feed_wire_index {feed_wire_index} out of range")
+            self.feeds = [(int(feed_wire_index), feed_arclength, 1.0 + 0.0j)]
+        else:
+            if len(feeds) == 0 and not junction_ports and not node_gaps:
+                raise ValueError("feeds must contain at least one entry")
+            norm = []
+            for i, f in enumerate(feeds):
+                if len(f) != 3:
+                    raise ValueError(f"feeds[{i}]: expected (wire_index, arclength, voltage), got {f!r}")
+                w_i, arc_i, v_i = f
+                if not (0 <= w_i < n_w):
+                    raise ValueError(f"feeds[{i}]: wire_index {w_i} out of range [0, {n_w})")
+                norm.append((int(w_i), None if arc_i is None else float(arc_i), complex(v_i)))
+            self.feeds = norm
+
+        self.junctions = []
+        for j, jw in enumerate(junctions or []):
+            if len(jw) < 1:
+                raise ValueError(f"junction {j}: need >= 1 wire-end")
+            grp = []
+            for w, end in jw:
+                if not (0 <= w < n_w):
+                    raise ValueError(f"junction {j}: wire_idx {w} out of range [0, {n_w})")
+                if end not in ("start", "end"):
+                    raise ValueError(f"junction {j}: end must be 'start' or 'end', got {end!r}")
+                grp.append((int(w), end))
+            self.junctions.append(grp)
+        self._check_junction_coincidence()
+
+        self.junction_ports = []
+        seen = set()
+        for p in junction_ports or []:
+            j_idx, volt = (int(p), 0j) if isinstance(p, (int, np.integer)) else (int(p[0]), complex(p[1]))
+            if not (0 <= j_idx < len(self.junctions)):
+                raise ValueError(f"junction_ports: junction index {j_idx} out of range")
+            if j_idx in seen:
+                raise ValueError(f"junction_ports: junction {j_idx} listed twice")
+            seen.add(j_idx)
+            self.junction_ports.append((j_idx, volt))
+
+        self.node_gaps = []
+        end_to_j = {m: j for j, jw in enumerate(self.junctions) for m in jw}
+        used_j, used_m = set(), set()
+        ported = {j for j, _v in self.junction_ports}
+        for i, g in enumerate(node_gaps or []):
+            if len(g) != 3:
+                raise ValueError(f"node_gaps[{i}]: expected (wire_index, 'start'|'end', voltage)")
+            w_i, end_i, v_i = g
+            if not (0 <= w_i < n_w):
+                raise ValueError(f"node_gaps[{i}]: wire_index {w_i} out of range")
+            if end_i not in ("start", "end"):
+                raise ValueError(f"node_gaps[{i}]: end must be 'start' or 'end'")
+            member = (int(w_i), end_i)
+            j_idx = end_to_j.get(member)
+            if j_idx is None:
+                raise ValueError(f"node_gaps[{i}]: wire {w_i} {end_i!r} is not a member of any junction")
+            if len(self.junctions[j_idx]) < 2:
+                raise ValueError(f"node_gaps[{i}]: junction {j_idx} has a single member")
+            if member in used_m or j_idx in used_j:
+                raise ValueError(f"node_gaps[{i}]: wire-end or junction already carries a node gap")
+            if j_idx in ported:
+                raise ValueError(f"node_gaps[{i}]: junction {j_idx} is also a junction port")
+            used_m.add(member)
+            used_j.add(j_idx)
+            self.node_gaps.append((int(w_i), end_i, complex(v_i)))
+
+        self._mesh = None
+        self._ports = None
+        self._dev = None
+
+    # ------------------------------------------------------------------
+    def _check_junction_coincidence(self):
+        """Refuse junction groups whose ends do not meet (reference _wire_spec.py:62-104)."""
+        for j, grp in enumerate(self.junctions):
+            pts, lens = [], []
+            for w, end in grp:
+                pl = self.wires_polylines[w]
+                npe = self.n_per_edge_per_wire[w]
+                if end == "start":
+                    pts.append(pl[0])
+                    lens.append(np.linalg.norm(pl[1] - pl[0]) / npe[0])
+                else:
+                    pts.append(pl[-1])
+                    lens.append(np.linalg.norm(pl[-1] - pl[-2]) / npe[-1])
+            tol = max(1e-3 * min(lens), 1e-5)
+            for p in pts[1:]:
+                dist = float(np.linalg.norm(p - pts[0]))
+                if dist > tol:
+                    raise ValueError(f"junction {j}: member ends are {dist:.3g} apart (tolerance {tol:.3g})")
+
+    def _build(self):
+        if self._mesh is None:
+            self._mesh = build_mesh(self.wires_polylines, self.n_per_edge_per_wire,
+                                    self._radius_per_wire, self.degree, self.junctions)
+        return self._mesh
+
+    def _port_count(self):
+        return len(self.feeds) + len(self.junction_ports) + len(self.node_gaps)
+
+    def _port_plan(self):
+        """Sparse port columns (drive == readout) and remaining KCL rows
+        (bspline.py:3784-3867, 1170-1210)."""
+        if self._ports is None:
+            mesh = self._build()
+            cols = [feed_column(mesh, w, s) for w, s, _v in self.feeds]
+            rows = {j: ([m for m, _s in dirs], [s for _m, s in dirs]) for j, dirs in mesh.kcl}
+            port_js = [j for j, _v in self.junction_ports]
+            cols += [rows[j] for j in port_js]
+            for w, end, _v in self.node_gaps:
+                j = next(jj for jj, jw in enumerate(self.junctions) if (w, end) in jw)
+                pw = mesh.per_wire[w]
+                m = None
+                for kidx, (_jl, kind, ji, endp) in enumerate(pw["kept"]):
+                    if kind == "dir" and ji == j and endp == end:
+                        m = pw["l2g"][kidx]
+                cols.append(([m], [1.0 if end == "start" else -1.0]))
+            con = [rows[j] for j in range(len(self.junctions)) if j not in set(port_js)]
+            if len(con) > 32:
+                raise NotImplementedError("more than 32 KCL constraint rows")
+            self._ports = (cols, con)
+        return self._ports
+
+    def _device(self):
+        if self._dev is None:
+            cols, con = self._port_plan()
+            self._dev = _Device(self._build(), cols + con, con, cols)
+        return self._dev.buffers()
+
+    # ------------------------------------------------------------------
+    def _solve_device(self, k=None, timers=None):
+        """One fill + one factorisation + all-port solve on the GPU.
+
+        Returns device tensors (Y, W): Y (n_ports x n_ports) and W, whose first
+        n_ports columns are the port coefficient columns.  Nothing is copied
+        to the host and nothing is synchronised.  ``timers``: optional dict
+        that collects (start, end) HIP events per stage on the launch stream."""
+        lib = _lib.load()
+        D = self._device()
+        torch = D.torch
+        st = torch.cuda.current_stream().cuda_stream
+
+        def mark(stage, first):
+            if timers is not None:
+                ev = torch.cuda.Event(enable_timing=True)
+                ev.record()
+                if first:
+                    timers.setdefault(stage, []).append([ev, None])
+                else:
+                    timers[stage][-1][1] = ev
+
+        kk = self.k if k is None else float(k)
+        omega = kk * self.c
+        mesh = self._mesh
+        nb, ncols = D.nb, D.ncols
+        cancel = self._cancel.address if self._cancel is not None else None
+        mark("fill", True)
+        _lib.check(lib.mw_same_edge_table(D.mw, kk, st), "same-edge table")
+        pairs = D.keep["pairs"]
+        _lib.check(lib.mw_fill_z(D.mw, kk, omega, self.eps, self.mu, D.Z.data_ptr(), nb,
+                                 pairs.data_ptr(), int(pairs.shape[0]), int(mesh.symmetric), st),
+                   "fill")
+        mark("fill", False)
+        _lib.check(lib.mw_zscatter_cols(nb, ncols, D.col_max, D.col_idx.data_ptr(),
+                                        D.col_val.data_ptr(), D.X.data_ptr(), D.X.shape[1], st),
+                   "right-hand sides")
+        mark("getrf", True)
+        _lib.check(lib.mw_zgetrf(nb, D.Z.data_ptr(), nb, D.ipiv.data_ptr(), D.work.data_ptr(),
+                                 D.info.data_ptr(), cancel, st), "factorisation")
+        mark("getrf", False)
+        ldx = D.X.shape[1]
+        mark("getrs", True)
+        _lib.check(lib.mw_zgetrs(nb, ncols, D.Z.data_ptr(), nb, D.ipiv.data_ptr(),
+                                 D.X.data_ptr(), ldx, st), "solve")
+        mark("getrs", False)
+        npt = D.n_ports
+        if D.n_con:
+            Xc = D.X.data_ptr() + 16 * npt
+            _lib.check(lib.mw_zsparse_rows(D.n_con, D.con_max, D.con_idx.data_ptr(),
+                                           D.con_val.data_ptr(), Xc, ldx, D.n_con,
+                                           D.S.data_ptr(), D.n_con, st), "KCL A.X")
+            _lib.check(lib.mw_zsparse_rows(D.n_con, D.con_max, D.con_idx.data_ptr(),
+                                           D.con_val.data_ptr(), D.X.data_ptr(), ldx, npt,
+                                           D.T.data_ptr(), D.T.shape[1], st), "KCL A.W")
+            _lib.check(lib.mw_zsmall_solve(D.n_con, npt, D.S.data_ptr(), D.n_con,
+                                           D.T.data_ptr(), D.T.shape[1],
+                                           D.info.data_ptr() + 4, st), "KCL Schur")
+            _lib.check(lib.mw_zgemm_sub(nb, npt, D.n_con, Xc, ldx, D.T.data_ptr(),
+                                        D.T.shape[1], D.X.data_ptr(), ldx, st), "KCL update")
+        _lib.check(lib.mw_zsparse_rows(npt, D.port_max, D.port_idx.data_ptr(),
+                                       D.port_val.data_ptr(), D.X.data_ptr(), ldx, npt,
+                                       D.Y.data_ptr(), D.Y.shape[1], st), "port readout")
+        return D.Y[:npt, :npt], D.X[:, :npt]
+
+    def _check_info(self):
+        info = self._dev.info.cpu().numpy()
+        if info[0] != 0:
+            raise np.linalg.LinAlgError(f"Z is singular: zero pivot at column {info[0] - 1}")
+        if info[1] != 0:
+            raise np.linalg.LinAlgError("KCL Schur complement is singular")
+
+    def compute_port_solution(self, same_edge_prep=None) -> PortSolution:
+        """Solve every port from ONE fill and ONE factorisation (bspline.py:4053-4179)."""
+        Y, W = self._solve_device()
+        y = Y.cpu().numpy()
+        coeffs = W.cpu().numpy()
+        self._check_info()
+        return PortSolution(y=y, coeffs=coeffs, port_currents=y,
+                            basis=("bspline-gpu", self._mesh, id(self)))
+
+    def compute_y_matrix(self):
+        return self.compute_port_solution().y
+
+    def compute_impedance(self):
+        """Driving-point impedance of the (single) feed: 1 / Y[0, 0]."""
+        return 1.0 / self.compute_y_matrix()[0, 0]
+
+    def _port_solutions_swept(self, k_array):
+        with self._k_restored():
+            for kk in np.asarray(k_array, dtype=float):
+                self._set_k(kk)
+                yield self.compute_port_solution()
+
+    # ------------------------------------------------------------------
+    # Readout (bspline.py:4603-4685; _element_currents.py:33-120)
+    # ------------------------------------------------------------------
+    def currents_at_knots(self, coeffs, s_array=None):
+        coeffs = np.asarray(coeffs)
+        mesh = self._build()
+        d = self.degree
+        out = []
+        for w, pw in enumerate(mesh.per_wire):
+            kv = pw["knots"]
+            s_eval = pw["arc"] if s_array is None else np.asarray(s_array[w], dtype=float)
+            s_eval = np.clip(s_eval, kv[0], kv[-1])
+            c = np.zeros(pw["nb_w"], dtype=np.complex128)
+            for kidx, (j, *_r) in enumerate(pw["kept"]):
+                c[j] = coeffs[pw["l2g"][kidx]]
+            vals = np.array([bspline_values(kv, d, pw["nb_w"], s) for s in s_eval])
+            out.append(vals @ c if len(s_eval) else np.zeros(0, dtype=np.complex128))
+        return out
+
+    def element_currents(self, coeffs, subdiv: int = 1):
+        subdiv = int(subdiv)
+        if subdiv < 1:
+            raise ValueError(f"subdiv must be >= 1, got {subdiv}")
+        coeffs = np.asarray(coeffs)
+        if coeffs.ndim != 1:
+            raise ValueError("element_currents takes ONE solution vector")
+        fine, arcs = [], []
+        for pl, npe in zip(self.wires_polylines, self.n_per_edge_per_wire):
+            parts = []
+            for i, n_e in enumerate(npe):
+                s = np.linspace(pl[i], pl[i + 1], n_e * subdiv + 1)
+                parts.append(s if i == 0 else s[1:])
+            kn = np.vstack(parts)
+            fine.append(kn)
+            step = np.linalg.norm(kn[1:] - kn[:-1], axis=1)
+            arcs.append(np.concatenate([[0.0], np.cumsum(step)]))
+        cur = self.currents_at_knots(coeffs, None if subdiv == 1 else arcs)
+        mids, moms, nodes, deltas = [], [], [], []
+        for kn, c in zip(fine, cur):
+            el = 0.5 * (c[1:] + c[:-1])
+            mids.append(0.5 * (kn[1:] + kn[:-1]))
+            moms.append(el[:, None] * (kn[1:] - kn[:-1]))
+            nodes.append(kn)
+            z = np.zeros(1, dtype=np.complex128)
+            deltas.append(np.concatenate([z, el]) - np.concatenate([el, z]))
+        return (np.concatenate(mids), np.concatenate(moms), np.concatenate(nodes),
+                np.concatenate(deltas))

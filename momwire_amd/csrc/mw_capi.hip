@@ -1,0 +1,187 @@
+// extern "C" entry points of libmomwire_hip.so (declared in include/momwire_hip.h).
+#include <stdio.h>
+#include <string.h>
+
+#include "mw_common.h"
+
+hipError_t mw_launch_same_edge(const mw_mesh& M, double k, hipStream_t st);
+hipError_t mw_launch_fill(const mw_mesh& M, double k, double omega, double eps, double mu,
+                          double2* Z, int64_t ldz, const int32_t* pairs, int32_t n_pairs,
+                          bool symmetric, hipStream_t st);
+int64_t mw_getrf_ws_bytes(int64_t n);
+template <typename T>
+int mw_getrf_t(int64_t n, void* A, int64_t lda, int32_t* ipiv, void* work, int32_t* info,
+               const volatile int32_t* cancel, hipStream_t st, hipError_t* herr);
+template <typename T>
+int mw_getrs_t(int64_t n, int64_t nrhs, const void* A, int64_t lda, const int32_t* ipiv, void* X,
+               int64_t ldx, hipStream_t st, hipError_t* herr);
+hipError_t mw_gemm_sub_z(int64_t m, int64_t n, int64_t k, const void* A, int64_t lda,
+                         const void* B, int64_t ldb, void* C, int64_t ldc, hipStream_t st);
+hipError_t mw_gemm_sub_c(int64_t m, int64_t n, int64_t k, const void* A, int64_t lda,
+                         const void* B, int64_t ldb, void* C, int64_t ldc, hipStream_t st);
+hipError_t mw_launch_sparse_rows(int64_t nrows, int maxnz, const int32_t* idx, const double* val,
+                                 const void* X, int64_t ldx, int64_t ncols, void* out,
+                                 int64_t ldo, hipStream_t st);
+hipError_t mw_launch_scatter_cols(int64_t n, int64_t ncols, int maxnz, const int32_t* idx,
+                                  const double* val, void* X, int64_t ldx, hipStream_t st);
+hipError_t mw_launch_small_solve(int ns, int nt, void* S, int64_t lds, void* T, int64_t ldt,
+                                 int32_t* info, hipStream_t st);
+
+namespace {
+thread_local char g_err[512] = "";
+
+int fail(int code, const char* fmt, const char* what = "", long long a = 0, long long b = 0) {
+  snprintf(g_err, sizeof g_err, fmt, what, a, b);
+  return code;
+}
+
+int hip_status(hipError_t e, const char* where) {
+  if (e == hipSuccess) return MW_OK;
+  snprintf(g_err, sizeof g_err, "%s: HIP error %d (%s)", where, (int)e, hipGetErrorString(e));
+  return MW_EHIP;
+}
+
+int check_mesh(const mw_mesh* M) {
+  if (!M) return fail(MW_EINVAL, "%smesh is NULL");
+  if (M->degree != 2) return fail(MW_EINVAL, "%sdegree %lld not built (degree 2 only)", "", M->degree);
+  if (M->n_qp != 4) return fail(MW_EINVAL, "%sn_qp_pair %lld not built (4 only)", "", M->n_qp);
+  if (M->n_seg <= 0 || M->n_basis <= 0 || M->n_tile <= 0)
+    return fail(MW_EINVAL, "%sempty mesh (n_seg=%lld, n_basis=%lld)", "", M->n_seg, M->n_basis);
+  if (!M->seg || !M->seg_edge || !M->edge || !M->edge_i || !M->se_tab || !M->band_poly ||
+      !M->bw_loc || !M->bw_slot || !M->tile_b0 || !M->tile_seg || !M->tile_nseg || !M->gl)
+    return fail(MW_EINVAL, "%smesh has a NULL table");
+  return MW_OK;
+}
+}  // namespace
+
+extern "C" {
+
+const char* mw_last_error(void) { return g_err; }
+int mw_abi_version(void) { return 1; }
+
+int mw_same_edge_table(const mw_mesh* mesh, double k, void* stream) {
+  g_err[0] = 0;
+  int rc = check_mesh(mesh);
+  if (rc) return rc;
+  if (!(k > 0.0)) return fail(MW_EINVAL, "%swavenumber must be positive");
+  return hip_status(mw_launch_same_edge(*mesh, k, (hipStream_t)stream), "mw_same_edge_table");
+}
+
+int mw_fill_z(const mw_mesh* mesh, double k, double omega, double eps, double mu, void* Z,
+              int64_t ldz, const int32_t* tile_pairs, int32_t n_pairs, int32_t symmetric,
+              void* stream) {
+  g_err[0] = 0;
+  int rc = check_mesh(mesh);
+  if (rc) return rc;
+  if (!Z) return fail(MW_EINVAL, "%sZ is NULL");
+  if (ldz < mesh->n_basis) return fail(MW_EINVAL, "%sldz %lld < n_basis %lld", "", ldz, mesh->n_basis);
+  if (symmetric && (!tile_pairs || n_pairs <= 0))
+    return fail(MW_EINVAL, "%ssymmetric fill needs the tile-pair list");
+  return hip_status(mw_launch_fill(*mesh, k, omega, eps, mu, (double2*)Z, ldz, tile_pairs,
+                                   n_pairs, symmetric != 0, (hipStream_t)stream),
+                    "mw_fill_z");
+}
+
+int64_t mw_getrf_workspace_bytes(int64_t n) { return n > 0 ? mw_getrf_ws_bytes(n) : 0; }
+
+static int getrf_common(bool dbl, int64_t n, void* A, int64_t lda, int32_t* ipiv, void* work,
+                        int32_t* info, const volatile int32_t* cancel, void* stream) {
+  g_err[0] = 0;
+  if (n <= 0) return fail(MW_EINVAL, "%sn must be positive");
+  if (!A || !ipiv || !work || !info) return fail(MW_EINVAL, "%sNULL argument");
+  if (lda < n) return fail(MW_EINVAL, "%slda %lld < n %lld", "", lda, n);
+  hipError_t herr = hipSuccess;
+  int rc = dbl ? mw_getrf_t<double>(n, A, lda, ipiv, work, info, cancel, (hipStream_t)stream, &herr)
+               : mw_getrf_t<float>(n, A, lda, ipiv, work, info, cancel, (hipStream_t)stream, &herr);
+  if (rc == MW_EHIP) return hip_status(herr, "getrf");
+  if (rc == MW_EABORT) return fail(MW_EABORT, "%sfactorisation cancelled");
+  return rc;
+}
+
+int mw_zgetrf(int64_t n, void* A, int64_t lda, int32_t* ipiv, void* work, int32_t* info,
+              const volatile int32_t* cancel, void* stream) {
+  return getrf_common(true, n, A, lda, ipiv, work, info, cancel, stream);
+}
+int mw_cgetrf(int64_t n, void* A, int64_t lda, int32_t* ipiv, void* work, int32_t* info,
+              const volatile int32_t* cancel, void* stream) {
+  return getrf_common(false, n, A, lda, ipiv, work, info, cancel, stream);
+}
+
+static int getrs_common(bool dbl, int64_t n, int64_t nrhs, const void* A, int64_t lda,
+                        const int32_t* ipiv, void* X, int64_t ldx, void* stream) {
+  g_err[0] = 0;
+  if (n <= 0 || nrhs < 0) return fail(MW_EINVAL, "%sbad sizes n=%lld nrhs=%lld", "", n, nrhs);
+  if (!A || !ipiv || !X) return fail(MW_EINVAL, "%sNULL argument");
+  if (lda < n || ldx < nrhs) return fail(MW_EINVAL, "%sbad leading dimension (lda=%lld ldx=%lld)", "", lda, ldx);
+  if (nrhs == 0) return MW_OK;
+  hipError_t herr = hipSuccess;
+  int rc = dbl ? mw_getrs_t<double>(n, nrhs, A, lda, ipiv, X, ldx, (hipStream_t)stream, &herr)
+               : mw_getrs_t<float>(n, nrhs, A, lda, ipiv, X, ldx, (hipStream_t)stream, &herr);
+  if (rc == MW_EHIP) return hip_status(herr, "getrs");
+  return rc;
+}
+
+int mw_zgetrs(int64_t n, int64_t nrhs, const void* A, int64_t lda, const int32_t* ipiv, void* X,
+              int64_t ldx, void* stream) {
+  return getrs_common(true, n, nrhs, A, lda, ipiv, X, ldx, stream);
+}
+int mw_cgetrs(int64_t n, int64_t nrhs, const void* A, int64_t lda, const int32_t* ipiv, void* X,
+              int64_t ldx, void* stream) {
+  return getrs_common(false, n, nrhs, A, lda, ipiv, X, ldx, stream);
+}
+
+int mw_zgemm_sub(int64_t m, int64_t n, int64_t k, const void* A, int64_t lda, const void* B,
+                 int64_t ldb, void* C, int64_t ldc, void* stream) {
+  g_err[0] = 0;
+  if (m < 0 || n < 0 || k < 0) return fail(MW_EINVAL, "%snegative size");
+  if ((m && n && k) && (!A || !B || !C)) return fail(MW_EINVAL, "%sNULL argument");
+  if (lda < k || ldb < n || ldc < n) return fail(MW_EINVAL, "%sbad leading dimension");
+  return hip_status(mw_gemm_sub_z(m, n, k, A, lda, B, ldb, C, ldc, (hipStream_t)stream),
+                    "mw_zgemm_sub");
+}
+int mw_cgemm_sub(int64_t m, int64_t n, int64_t k, const void* A, int64_t lda, const void* B,
+                 int64_t ldb, void* C, int64_t ldc, void* stream) {
+  g_err[0] = 0;
+  if (m < 0 || n < 0 || k < 0) return fail(MW_EINVAL, "%snegative size");
+  if ((m && n && k) && (!A || !B || !C)) return fail(MW_EINVAL, "%sNULL argument");
+  if (lda < k || ldb < n || ldc < n) return fail(MW_EINVAL, "%sbad leading dimension");
+  return hip_status(mw_gemm_sub_c(m, n, k, A, lda, B, ldb, C, ldc, (hipStream_t)stream),
+                    "mw_cgemm_sub");
+}
+
+int mw_zsparse_rows(int64_t nrows, int32_t maxnz, const int32_t* idx, const double* val,
+                    const void* X, int64_t ldx, int64_t ncols, void* out, int64_t ldo,
+                    void* stream) {
+  g_err[0] = 0;
+  if (nrows < 0 || ncols < 0 || maxnz < 0) return fail(MW_EINVAL, "%snegative size");
+  if (nrows == 0 || ncols == 0) return MW_OK;
+  if (!idx || !val || !X || !out) return fail(MW_EINVAL, "%sNULL argument");
+  if (ldx < ncols || ldo < ncols) return fail(MW_EINVAL, "%sbad leading dimension");
+  return hip_status(mw_launch_sparse_rows(nrows, maxnz, idx, val, X, ldx, ncols, out, ldo,
+                                          (hipStream_t)stream),
+                    "mw_zsparse_rows");
+}
+
+int mw_zscatter_cols(int64_t n, int64_t ncols, int32_t maxnz, const int32_t* idx,
+                     const double* val, void* X, int64_t ldx, void* stream) {
+  g_err[0] = 0;
+  if (n < 0 || ncols < 0 || maxnz < 0) return fail(MW_EINVAL, "%snegative size");
+  if (n == 0 || ncols == 0) return MW_OK;
+  if (!idx || !val || !X) return fail(MW_EINVAL, "%sNULL argument");
+  if (ldx < ncols) return fail(MW_EINVAL, "%sbad leading dimension");
+  return hip_status(mw_launch_scatter_cols(n, ncols, maxnz, idx, val, X, ldx, (hipStream_t)stream),
+                    "mw_zscatter_cols");
+}
+
+int mw_zsmall_solve(int32_t ns, int32_t nt, void* S, int64_t lds, void* T, int64_t ldt,
+                    int32_t* info, void* stream) {
+  g_err[0] = 0;
+  if (ns <= 0 || nt <= 0) return MW_OK;
+  if (ns > 32 || nt > 64) return fail(MW_ERANGE, "%ssmall solve limited to ns<=32, nt<=64 (got %lld, %lld)", "", ns, nt);
+  if (!S || !T) return fail(MW_EINVAL, "%sNULL argument");
+  if (lds < ns || ldt < nt) return fail(MW_EINVAL, "%sbad leading dimension");
+  return hip_status(mw_launch_small_solve(ns, nt, S, lds, T, ldt, info, (hipStream_t)stream),
+                    "mw_zsmall_solve");
+}
+
+}  // extern "C"

@@ -1,0 +1,373 @@
+// Galerkin Z fill for the degree-2 B-spline thin-wire EFIE, free space, on
+// gfx950.
+//
+//   Z[m][n] = j w mu  sum_{a,b} (t_i . t_j) sum_{p,q} C[m,a,p] C[n,b,q] J_pq(i,j)
+//           + 1/(j w eps) sum_{a,b} sum_{p,q>=1} p q C[m,a,p] C[n,b,q] J_{p-1,q-1}(i,j)
+//   (reference _accelerators.cpp:1196-1309 / bspline.py:2289-2348), with
+//   i = supp(m,a), j = supp(n,b) and the segment-pair moments
+//   J_pq(i,j) = int int u^p u'^q exp(-jkR)/(4 pi R),  R^2 = |r_i(u)-r_j(u')|^2 + a_i^2
+//   by 4x4 Gauss-Legendre (_accelerators.cpp:349-542) on every pair except
+//   pairs on one straight edge, which take static closed form + smooth
+//   remainder instead (bspline.py:2254-2281).
+//
+// Layout of the work: the basis index range is cut into tiles of <= 16 bases;
+// a workgroup owns one 16x16 tile of Z.  It stages the <= 18 segments each
+// tile side touches, evaluates every segment pair once (two lanes per pair,
+// eight quadrature points each, one shuffle to combine), contracts the nine
+// moments straight into the 3x3 "band block" B_ij -- the contribution of the
+// pair to the (<= 3) bases living on segment i times those on segment j --
+// and keeps B in LDS.  Each thread then sums the nine band blocks its Z entry
+// needs.  The (d+1)^2 N^2 moment tensor of the reference never exists.
+#include "mw_common.h"
+
+namespace {
+
+constexpr int NM = 3;       // moments per axis (degree 2)
+constexpr int NQ = 4;       // quadrature points per axis
+constexpr double INV4PI = 0.079577471545947667884441881686257181017229822870228;
+
+// ---------------------------------------------------------------------------
+// Same-edge table
+// ---------------------------------------------------------------------------
+
+// Coefficients (ascending powers of eta) of the unit-length weight
+//   w(eta) = int int delta(eta - (U - V)) U^p V^q dU dV over [0,1]^2
+// on the piece eta in [0,1] (plus) or [-1,0] (minus); degree <= p+q+1 <= 5.
+MW_DEV void weight_poly(int p, int q, bool plus, double* c /*[6]*/) {
+  for (int n = 0; n < 6; ++n) c[n] = 0.0;
+  const int top = plus ? p : q;
+  for (int k = 0; k <= top; ++k) {
+    // plus : C(p,k) eta^{p-k} (1-eta)^{k+q+1} / (k+q+1)
+    // minus: C(q,k) (-eta)^{q-k} (1+eta)^{p+k+1} / (p+k+1)
+    double binom = 1.0;
+    for (int t = 0; t < k; ++t) binom = binom * (top - t) / (t + 1);
+    const int e1 = top - k;                        // power of (+-eta)
+    const int e2 = plus ? (k + q + 1) : (p + k + 1);
+    const double s1 = (plus || (e1 % 2 == 0)) ? 1.0 : -1.0;
+    const double sg = plus ? -1.0 : 1.0;           // (1 + sg*eta)^{e2}
+    double b2 = 1.0;                               // C(e2, t)
+    for (int t = 0; t <= e2; ++t) {
+      const double term = binom * s1 * b2 * ((t % 2 == 1) ? sg : 1.0) / e2;
+      c[e1 + t] += term;
+      b2 = b2 * (e2 - t) / (t + 1);
+    }
+  }
+}
+
+// int_{x0}^{x1} P(x) / sqrt(x^2 + ah^2) dx for P of degree <= 5, exactly.
+MW_DEV double closed_piece(const double* b, double x0, double x1, double ah) {
+  double F0a = asinh(x0 / ah), F0b = asinh(x1 / ah);
+  double F1a = sqrt(x0 * x0 + ah * ah), F1b = sqrt(x1 * x1 + ah * ah);
+  double Fa[6], Fb[6];
+  Fa[0] = F0a; Fb[0] = F0b; Fa[1] = F1a; Fb[1] = F1b;
+  double xa = 1.0, xb = 1.0;  // x^{n-1}
+  for (int n = 2; n < 6; ++n) {
+    xa *= x0; xb *= x1;
+    Fa[n] = (xa * F1a - (n - 1) * ah * ah * Fa[n - 2]) / n;
+    Fb[n] = (xb * F1b - (n - 1) * ah * ah * Fb[n - 2]) / n;
+  }
+  double s = 0.0;
+  for (int n = 0; n < 6; ++n) s += b[n] * (Fb[n] - Fa[n]);
+  return s;
+}
+
+// Unit-length static moment I(p,q,delta; ah): source offset delta segments.
+MW_DEV double static_unit(int p, int q, int delta, double ah, const double* glx,
+                          const double* glw) {
+  double total = 0.0;
+  for (int piece = 0; piece < 2; ++piece) {
+    const bool plus = piece == 1;
+    double c[6];
+    weight_poly(p, q, plus, c);
+    const double e0 = plus ? 0.0 : -1.0, e1 = plus ? 1.0 : 0.0;
+    if (delta >= -1 && delta <= 1) {
+      // shift eta = x + delta: b_m = sum_{n>=m} c_n C(n,m) delta^{n-m}
+      double b[6];
+      for (int m = 0; m < 6; ++m) {
+        double s = 0.0, bin = 1.0;  // C(n,m) for n = m..5
+        double dp = 1.0;            // delta^{n-m}
+        for (int n = m; n < 6; ++n) {
+          s += c[n] * bin * dp;
+          bin = bin * (n + 1) / (n + 1 - m);
+          dp *= (double)delta;
+        }
+        b[m] = s;
+      }
+      total += closed_piece(b, e0 - delta, e1 - delta, ah);
+    } else {
+      // The kernel's poles are >= one segment away: 16-point Gauss-Legendre
+      // is exact to rounding and avoids the far-pair cancellation of the
+      // expanded closed forms.
+      const double mid = 0.5 * (e0 + e1), half = 0.5 * (e1 - e0);
+      double s = 0.0;
+      for (int g = 0; g < 16; ++g) {
+        const double eta = mid + half * glx[g];
+        double w = c[5];
+        for (int n = 4; n >= 0; --n) w = w * eta + c[n];
+        const double x = eta - delta;
+        s += glw[g] * w / sqrt(x * x + ah * ah);
+      }
+      total += half * s;
+    }
+  }
+  return total;
+}
+
+__global__ __launch_bounds__(256) void k_same_edge(mw_mesh M, double k) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= M.n_se) return;
+  const int ed = M.se_edge[e];
+  const int n_e = M.edge_i[2 * ed], off = M.edge_i[2 * ed + 1];
+  const int delta = e - off - (n_e - 1);
+  const double h = M.edge[2 * ed], a = M.edge[2 * ed + 1];
+  const double* t01 = M.gl;
+  const double* w01 = M.gl + NQ;
+  const double* glx = M.gl + 2 * NQ;
+  const double* glw = glx + 16;
+
+  double jr[NM][NM], ji[NM][NM];
+  // static part: h^{p+q+1} * I_unit / (4 pi)
+  const double ah = a / h;
+  for (int p = 0; p < NM; ++p)
+    for (int q = 0; q < NM; ++q) {
+      double hp = h;
+      for (int t = 0; t < p + q; ++t) hp *= h;
+      jr[p][q] = hp * static_unit(p, q, delta, ah, glx, glw) * INV4PI;
+      ji[p][q] = 0.0;
+    }
+  // smooth remainder (exp(-jkR)-1)/(4 pi R) on the arc distance
+  for (int q = 0; q < NQ; ++q) {
+    double wq[NM];
+    const double uq = t01[q] * h;
+    wq[0] = w01[q] * h; wq[1] = wq[0] * uq; wq[2] = wq[1] * uq;
+    for (int r = 0; r < NQ; ++r) {
+      double wr[NM];
+      const double ur = t01[r] * h;
+      wr[0] = w01[r] * h; wr[1] = wr[0] * ur; wr[2] = wr[1] * ur;
+      const double ds = (t01[q] - t01[r] - (double)delta) * h;
+      const double R = sqrt(ds * ds + a * a);
+      double sn, cs, sh, chh;
+      sincos(k * R, &sn, &cs);
+      sincos(0.5 * k * R, &sh, &chh);
+      const double inv = INV4PI / R;
+      const double gr = -2.0 * sh * sh * inv;  // cos(kR) - 1
+      const double gi = -sn * inv;
+      for (int p = 0; p < NM; ++p)
+        for (int P = 0; P < NM; ++P) {
+          const double w = wq[p] * wr[P];
+          jr[p][P] += w * gr;
+          ji[p][P] += w * gi;
+        }
+    }
+  }
+  double2* out = reinterpret_cast<double2*>(M.se_tab) + (size_t)e * 9;
+  for (int p = 0; p < NM; ++p)
+    for (int q = 0; q < NM; ++q) out[p * NM + q] = make_double2(jr[p][q], ji[p][q]);
+}
+
+// ---------------------------------------------------------------------------
+// Fused tile fill
+// ---------------------------------------------------------------------------
+
+struct SegStage {
+  double pos[NQ][3];   // quadrature points
+  double wu[NM][NQ];   // w_q L (t_q L)^p
+  double tan[3];
+  double a;            // observer-row radius
+  double poly[NM][NM]; // band polynomials [slot][power]
+  int edge, eidx;
+};
+
+MW_DEV void stage_segment(const mw_mesh& M, int s, SegStage& S) {
+  const double* g = M.seg + (size_t)s * 12;
+  const double L = g[9];
+  for (int q = 0; q < NQ; ++q) {
+    const double t = M.gl[q];
+    for (int c = 0; c < 3; ++c) S.pos[q][c] = (1.0 - t) * g[c] + t * g[3 + c];
+    const double w = M.gl[NQ + q] * L, u = t * L;
+    S.wu[0][q] = w; S.wu[1][q] = w * u; S.wu[2][q] = w * u * u;
+  }
+  S.tan[0] = g[6]; S.tan[1] = g[7]; S.tan[2] = g[8];
+  S.a = g[10];
+  const double* bp = M.band_poly + (size_t)s * 9;
+  for (int i = 0; i < 9; ++i) S.poly[i / 3][i % 3] = bp[i];
+  S.edge = M.seg_edge[2 * s];
+  S.eidx = M.seg_edge[2 * s + 1];
+}
+
+template <bool SYM>
+__global__ __launch_bounds__(256) void k_fill(mw_mesh M, double k, double wmu,
+                                              double inv_weps, double2* __restrict__ Z,
+                                              int64_t ldz, const int32_t* __restrict__ pairs) {
+  __shared__ SegStage rs[MW_TSMAX], cs[MW_TSMAX];
+  __shared__ double2 Bt[MW_TSMAX * MW_TSMAX * 9];
+  __shared__ int nr_s, nc_s;
+
+  int ti, tj;
+  if (SYM) {
+    ti = pairs[2 * blockIdx.x];
+    tj = pairs[2 * blockIdx.x + 1];
+  } else {
+    ti = blockIdx.y;
+    tj = blockIdx.x;
+  }
+  const int tid = threadIdx.x;
+  const int nr = M.tile_nseg[ti], nc = M.tile_nseg[tj];
+  if (tid < nr) stage_segment(M, M.tile_seg[ti * MW_TSMAX + tid], rs[tid]);
+  if (tid >= 64 && tid - 64 < nc) stage_segment(M, M.tile_seg[tj * MW_TSMAX + tid - 64], cs[tid - 64]);
+  if (tid == 0) { nr_s = nr; nc_s = nc; }
+  __syncthreads();
+
+  const double2* se = reinterpret_cast<const double2*>(M.se_tab);
+  const int npair = nr * nc;
+  // Two lanes per pair: half 0 takes q = 0,1; half 1 takes q = 2,3.
+  for (int item = tid; item < 2 * ((npair + 127) / 128) * 128; item += 256) {
+    const int pair = item >> 1, half = item & 1;
+    const bool live = pair < npair;
+    const int li = live ? pair / nc : 0, lj = live ? pair % nc : 0;
+    const SegStage& I = rs[li];
+    const SegStage& J = cs[lj];
+    double jr[NM][NM], jim[NM][NM];
+#pragma unroll
+    for (int p = 0; p < NM; ++p)
+#pragma unroll
+      for (int P = 0; P < NM; ++P) { jr[p][P] = 0.0; jim[p][P] = 0.0; }
+    const bool same_edge = I.edge == J.edge;
+    if (live && same_edge) {
+      if (half == 0) {
+        const int ed = I.edge;
+        const int n_e = M.edge_i[2 * ed], off = M.edge_i[2 * ed + 1];
+        const double2* tab = se + (size_t)(off + (J.eidx - I.eidx) + n_e - 1) * 9;
+#pragma unroll
+        for (int p = 0; p < NM; ++p)
+#pragma unroll
+          for (int P = 0; P < NM; ++P) {
+            const double2 v = tab[p * NM + P];
+            jr[p][P] = v.x; jim[p][P] = v.y;
+          }
+      }
+    } else if (live) {
+      const double a2 = I.a * I.a;
+#pragma unroll
+      for (int qq = 0; qq < 2; ++qq) {
+        const int q = 2 * half + qq;
+        const double x = I.pos[q][0], y = I.pos[q][1], z = I.pos[q][2];
+        const double wq0 = I.wu[0][q], wq1 = I.wu[1][q], wq2 = I.wu[2][q];
+#pragma unroll
+        for (int r = 0; r < NQ; ++r) {
+          const double dx = x - J.pos[r][0], dy = y - J.pos[r][1], dz = z - J.pos[r][2];
+          const double R = sqrt(dx * dx + dy * dy + dz * dz + a2);
+          double sn, cn;
+          sincos(k * R, &sn, &cn);
+          const double inv = INV4PI / R;
+          const double gr = cn * inv, gi = -sn * inv;  // exp(-jkR)/(4 pi R)
+          const double w0 = J.wu[0][r], w1 = J.wu[1][r], w2 = J.wu[2][r];
+          const double ar[3] = {w0 * gr, w1 * gr, w2 * gr};
+          const double ai[3] = {w0 * gi, w1 * gi, w2 * gi};
+#pragma unroll
+          for (int P = 0; P < NM; ++P) {
+            jr[0][P] += wq0 * ar[P]; jim[0][P] += wq0 * ai[P];
+            jr[1][P] += wq1 * ar[P]; jim[1][P] += wq1 * ai[P];
+            jr[2][P] += wq2 * ar[P]; jim[2][P] += wq2 * ai[P];
+          }
+        }
+      }
+    }
+    // combine the two halves (partner lane = item ^ 1 = lane ^ 1)
+#pragma unroll
+    for (int p = 0; p < NM; ++p)
+#pragma unroll
+      for (int P = 0; P < NM; ++P) {
+        jr[p][P] += __shfl_xor(jr[p][P], 1);
+        jim[p][P] += __shfl_xor(jim[p][P], 1);
+      }
+    if (!live) continue;
+    // Contract to the band block B[al][be].  A-term T = J C_j, then C_i^T T;
+    // Phi-term uses derivative polynomials C'[p'][s] = (p'+1) C[s][p'+1].
+    const double td = I.tan[0] * J.tan[0] + I.tan[1] * J.tan[1] + I.tan[2] * J.tan[2];
+    const int be0 = half == 0 ? 0 : 2, be1 = half == 0 ? 2 : 3;
+    for (int be = be0; be < be1; ++be) {
+      const double cj0 = J.poly[be][0], cj1 = J.poly[be][1], cj2 = J.poly[be][2];
+      double tr[NM], tim[NM];
+#pragma unroll
+      for (int p = 0; p < NM; ++p) {
+        tr[p] = jr[p][0] * cj0 + jr[p][1] * cj1 + jr[p][2] * cj2;
+        tim[p] = jim[p][0] * cj0 + jim[p][1] * cj1 + jim[p][2] * cj2;
+      }
+      const double dj0 = cj1, dj1 = 2.0 * cj2;  // derivative coefficients
+      double fr[2], fi[2];
+#pragma unroll
+      for (int p = 0; p < 2; ++p) {
+        fr[p] = jr[p][0] * dj0 + jr[p][1] * dj1;
+        fi[p] = jim[p][0] * dj0 + jim[p][1] * dj1;
+      }
+#pragma unroll
+      for (int al = 0; al < NM; ++al) {
+        const double ci0 = I.poly[al][0], ci1 = I.poly[al][1], ci2 = I.poly[al][2];
+        const double Ar = ci0 * tr[0] + ci1 * tr[1] + ci2 * tr[2];
+        const double Ai = ci0 * tim[0] + ci1 * tim[1] + ci2 * tim[2];
+        const double di0 = ci1, di1 = 2.0 * ci2;
+        const double Pr = di0 * fr[0] + di1 * fr[1];
+        const double Pi = di0 * fi[0] + di1 * fi[1];
+        // j w mu td A + Phi / (j w eps)
+        const double Br = -wmu * td * Ai + Pi * inv_weps;
+        const double Bi = wmu * td * Ar - Pr * inv_weps;
+        Bt[(li * nc + lj) * 9 + al * 3 + be] = make_double2(Br, Bi);
+      }
+    }
+  }
+  __syncthreads();
+
+  // Gather: Z[m][n] = sum over wing pairs of the band-block entries.
+  const int m0 = M.tile_b0[ti], m1 = M.tile_b0[ti + 1];
+  const int n0 = M.tile_b0[tj], n1 = M.tile_b0[tj + 1];
+  const int ml = tid / MW_TB, nl = tid % MW_TB;
+  const int m = m0 + ml, n = n0 + nl;
+  if (m < m1 && n < n1) {
+    double zr = 0.0, zi = 0.0;
+    int la[3], sa[3], lb[3], sb[3];
+#pragma unroll
+    for (int w = 0; w < 3; ++w) {
+      la[w] = M.bw_loc[3 * m + w]; sa[w] = M.bw_slot[3 * m + w];
+      lb[w] = M.bw_loc[3 * n + w]; sb[w] = M.bw_slot[3 * n + w];
+    }
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      if (la[a] < 0) continue;
+#pragma unroll
+      for (int b = 0; b < 3; ++b) {
+        if (lb[b] < 0) continue;
+        const double2 v = Bt[(la[a] * nc + lb[b]) * 9 + sa[a] * 3 + sb[b]];
+        zr += v.x; zi += v.y;
+      }
+    }
+    Z[(int64_t)m * ldz + n] = make_double2(zr, zi);
+    if (SYM && ti != tj) Z[(int64_t)n * ldz + m] = make_double2(zr, zi);
+  }
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// Host launchers (called from mw_capi.cpp)
+// ---------------------------------------------------------------------------
+hipError_t mw_launch_same_edge(const mw_mesh& M, double k, hipStream_t st) {
+  if (M.n_se == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_same_edge, dim3((M.n_se + 255) / 256), dim3(256), 0, st, M, k);
+  return hipGetLastError();
+}
+
+hipError_t mw_launch_fill(const mw_mesh& M, double k, double omega, double eps, double mu,
+                          double2* Z, int64_t ldz, const int32_t* pairs, int32_t n_pairs,
+                          bool symmetric, hipStream_t st) {
+  const double wmu = omega * mu, inv_weps = 1.0 / (omega * eps);
+  if (symmetric) {
+    if (n_pairs == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_fill<true>, dim3(n_pairs), dim3(256), 0, st, M, k, wmu, inv_weps, Z,
+                       ldz, pairs);
+  } else {
+    hipLaunchKernelGGL(k_fill<false>, dim3(M.n_tile, M.n_tile), dim3(256), 0, st, M, k, wmu,
+                       inv_weps, Z, ldz, pairs);
+  }
+  return hipGetLastError();
+}

@@ -1,0 +1,124 @@
+"""CPU checks of the host-side tables the HIP kernels read (no GPU needed).
+
+The fill kernel never sees supp/polys directly: it reads per-segment band
+polynomials, per-basis (tile-local segment, band slot) pairs and tile segment
+lists.  Here the kernel's data flow is replayed in numpy on the oracle's
+moments -- band blocks B_ij, then the per-entry gather -- and the resulting Z
+must equal the oracle's Galerkin assembly.  The basis polynomials of the host
+set-up (Cox-de Boor on polynomials) are checked against the oracle's (the
+reference's design-matrix + Vandermonde route).
+"""
+
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+from conftest import ROOT, rel_err
+from momwire_amd import _lib, workloads as W
+from momwire_amd._mesh import build_mesh, bspline_values, feed_column
+from momwire_amd.bspline import BSplineSolver
+from oracle.bspline_oracle import OracleSolver, EPS0, MU0
+
+CASES = [
+    ("c1", W.straight_dipole()),
+    ("array2x2", W.dipole_array(2, 2, 11, seed=5)),
+    ("tee", W.junction_tee("feed")),
+    ("tee_jport", W.junction_tee("junction_port")),
+]
+
+
+def _host(kw):
+    s = BSplineSolver(**kw)
+    return s, s._build()
+
+
+@pytest.mark.parametrize("name,kw", CASES, ids=[c[0] for c in CASES])
+def test_basis_polys_match_oracle(name, kw):
+    s, mesh = _host(kw)
+    o = OracleSolver(**kw)
+    supp, polys = o.basis(o.geometry())[:2]
+    assert mesh.n_basis == supp.shape[0]
+    for m in range(mesh.n_basis):
+        for a in range(3):
+            if mesh.supp[m, a] >= 0:
+                assert mesh.supp[m, a] == supp[m, a]
+                assert np.allclose(mesh.polys[m, a], polys[m, a], rtol=1e-12, atol=1e-12 * np.abs(polys[m]).max())
+            else:
+                assert not np.any(polys[m, a])
+
+
+@pytest.mark.parametrize("name,kw", CASES, ids=[c[0] for c in CASES])
+def test_band_gather_replay_equals_oracle_Z(name, kw):
+    s, mesh = _host(kw)
+    o = OracleSolver(**kw)
+    geom = o.geometry()
+    J = o.J_blocks(geom, o.k)
+    Zref = o.Z(geom)
+    wmu, inv_weps = o.omega * MU0, 1.0 / (o.omega * EPS0)
+    N = mesh.n_seg
+    t = mesh.seg[:, 6:9]
+    td = t @ t.T
+    C = mesh.band_poly                      # [seg][slot][p]
+    Cd = C[:, :, 1:] * np.array([1.0, 2.0])  # derivative coefficients
+    A = np.einsum("iap,pqij,jbq->ijab", C, J, C)
+    Phi = np.einsum("iap,pqij,jbq->ijab", Cd, J[:2, :2], Cd)
+    B = 1j * wmu * td[:, :, None, None] * A + Phi / (1j * o.omega * EPS0) * (o.omega * EPS0) * inv_weps
+    Z = np.zeros((mesh.n_basis, mesh.n_basis), dtype=complex)
+    for ti in range(len(mesh.tile_nseg)):
+        rs = mesh.tile_seg[ti, :mesh.tile_nseg[ti]]
+        for tj in range(len(mesh.tile_nseg)):
+            cs = mesh.tile_seg[tj, :mesh.tile_nseg[tj]]
+            for m in range(mesh.tile_b0[ti], mesh.tile_b0[ti + 1]):
+                for n in range(mesh.tile_b0[tj], mesh.tile_b0[tj + 1]):
+                    z = 0
+                    for a in range(3):
+                        if mesh.bw_loc[m, a] < 0:
+                            continue
+                        for b in range(3):
+                            if mesh.bw_loc[n, b] < 0:
+                                continue
+                            i, j = rs[mesh.bw_loc[m, a]], cs[mesh.bw_loc[n, b]]
+                            z += B[i, j, mesh.bw_slot[m, a], mesh.bw_slot[n, b]]
+                    Z[m, n] = z
+    assert rel_err(Z, Zref) < 1e-13
+    assert mesh.tile_nseg.max() <= _lib.TSMAX
+    assert np.all(np.diff(mesh.tile_b0) <= _lib.TB)
+
+
+def test_feed_column_is_basis_value():
+    kw = W.straight_dipole()
+    s, mesh = _host(kw)
+    o = OracleSolver(**kw)
+    geom = o.geometry()
+    B, _ = o.port_columns(geom, o.basis(geom))
+    idx, val = feed_column(mesh, 0, None)
+    dense = np.zeros(mesh.n_basis)
+    dense[idx] = val
+    assert np.allclose(dense, B[:, 0].real, atol=1e-15)
+
+
+def test_bspline_values_partition_of_unity():
+    knots = np.concatenate([[0, 0], np.linspace(0, 2.0, 8), [2.0, 2.0]])
+    for x in np.linspace(0, 2.0, 37):
+        v = bspline_values(knots, 2, len(knots) - 3, x)
+        assert abs(v.sum() - 1.0) < 1e-14
+        assert np.all(v >= -1e-15)
+
+
+def test_c_abi_exports_every_declared_symbol():
+    """The built library exports every function include/momwire_hip.h declares."""
+    if not os.path.exists(_lib.LIB_PATH):
+        pytest.skip("libmomwire_hip.so not built")
+    hdr = open(os.path.join(ROOT, "include", "momwire_hip.h")).read()
+    names = set(re.findall(r"^\s*(?:int|int64_t|const char\*)\s+(mw_\w+)\s*\(", hdr, re.M))
+    assert names, "no declarations parsed"
+    lib = ctypes.CDLL(_lib.LIB_PATH)
+    for nm in names:
+        assert hasattr(lib, nm), nm
+    assert names <= set(_lib.SIGNATURES)
+    lib2 = _lib.load()
+    assert lib2.mw_abi_version() == 1
+    assert lib2.mw_getrf_workspace_bytes(2624) > 0
