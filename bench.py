@@ -89,6 +89,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="array8x8", choices=sorted(WORKLOADS))
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--eager", action="store_true",
+                    help="launch kernels one by one instead of replaying the captured graph")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
@@ -109,19 +111,25 @@ def main():
     print(f"[bench] {args.workload}: {nb} unknowns, {ns} segments, {n_ports} ports; "
           f"warmup {args.warmup}", file=sys.stderr, flush=True)
 
+    step = s._solve_device if args.eager else s.solve_graph
     for _ in range(args.warmup):
-        s._solve_device()
+        step()
     torch.cuda.synchronize()
 
-    timers = {}
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(args.steps):
-        s._solve_device(timers=timers)
+        step()
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
     print(f"[bench] timed {args.steps} steps in {dt:.3f} s", file=sys.stderr, flush=True)
 
+    # stage split: HIP events on the launch stream around each stage of eager
+    # solves (a graph replay is one launch and cannot be split)
+    timers = {}
+    for _ in range(max(3, args.steps // 4)):
+        s._solve_device(timers=timers)
+    torch.cuda.synchronize()
     stage_ms = {k: float(np.mean([a.elapsed_time(b) for a, b in v])) for k, v in timers.items()}
     ms = 1e3 * dt / args.steps
     fill_s = stage_ms["fill"] * 1e-3
@@ -173,7 +181,8 @@ def main():
         "dtype": "f64",
         "data": "synthetic",
         "config": {"workload": wl["desc"], "unknowns": nb, "segments": ns, "ports": n_ports,
-                   "precision": "complex128 fill, LU and solve"},
+                   "precision": "complex128 fill, LU and solve",
+                   "launch": "eager" if args.eager else "hip graph replay"},
         "fill_gentries_per_s": nb * nb / fill_s / 1e9,
         "stage_ms": stage_ms,
         "fill_tflops": fill_tf,

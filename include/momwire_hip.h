@@ -117,12 +117,16 @@ int mw_cgetrf(int64_t n, void* A, int64_t lda, int32_t* ipiv, void* work,
 
 /*
  * Multi-RHS solve with the factors of mw_?getrf: X <- A^{-1} X for nrhs
- * columns (X: n x nrhs, row stride ldx).  Replaces LAPACK zgetrs.
+ * columns (X: n x nrhs, row stride ldx).  work: the same workspace the
+ * factorisation wrote (it keeps the inverted 32x32 diagonal blocks).
+ * Replaces LAPACK zgetrs.
  */
 int mw_zgetrs(int64_t n, int64_t nrhs, const void* A, int64_t lda,
-              const int32_t* ipiv, void* X, int64_t ldx, void* stream);
+              const int32_t* ipiv, const void* work, void* X, int64_t ldx,
+              void* stream);
 int mw_cgetrs(int64_t n, int64_t nrhs, const void* A, int64_t lda,
-              const int32_t* ipiv, void* X, int64_t ldx, void* stream);
+              const int32_t* ipiv, const void* work, void* X, int64_t ldx,
+              void* stream);
 
 /* C <- C - A B (complex, MFMA).  A: m x k, B: k x n, C: m x n. */
 int mw_zgemm_sub(int64_t m, int64_t n, int64_t k, const void* A, int64_t lda,

@@ -70,8 +70,8 @@ SIGNATURES = {
     "mw_getrf_workspace_bytes": (_I64, [_I64]),
     "mw_zgetrf": (ctypes.c_int, [_I64, _P, _I64, _P, _P, _P, _P, _P]),
     "mw_cgetrf": (ctypes.c_int, [_I64, _P, _I64, _P, _P, _P, _P, _P]),
-    "mw_zgetrs": (ctypes.c_int, [_I64, _I64, _P, _I64, _P, _P, _I64, _P]),
-    "mw_cgetrs": (ctypes.c_int, [_I64, _I64, _P, _I64, _P, _P, _I64, _P]),
+    "mw_zgetrs": (ctypes.c_int, [_I64, _I64, _P, _I64, _P, _P, _P, _I64, _P]),
+    "mw_cgetrs": (ctypes.c_int, [_I64, _I64, _P, _I64, _P, _P, _P, _I64, _P]),
     "mw_zgemm_sub": (ctypes.c_int, [_I64, _I64, _I64, _P, _I64, _P, _I64, _P, _I64, _P]),
     "mw_cgemm_sub": (ctypes.c_int, [_I64, _I64, _I64, _P, _I64, _P, _I64, _P, _I64, _P]),
     "mw_zsparse_rows": (ctypes.c_int, [_I64, _I32, _P, _P, _P, _I64, _I64, _P, _I64, _P]),

@@ -382,7 +382,7 @@ class BSplineSolver(_SweptPortSolutions):
         ldx = D.X.shape[1]
         mark("getrs", True)
         _lib.check(lib.mw_zgetrs(nb, ncols, D.Z.data_ptr(), nb, D.ipiv.data_ptr(),
-                                 D.X.data_ptr(), ldx, st), "solve")
+                                 D.work.data_ptr(), D.X.data_ptr(), ldx, st), "solve")
         mark("getrs", False)
         npt = D.n_ports
         if D.n_con:
@@ -402,6 +402,27 @@ class BSplineSolver(_SweptPortSolutions):
                                        D.port_val.data_ptr(), D.X.data_ptr(), ldx, npt,
                                        D.Y.data_ptr(), D.Y.shape[1], st), "port readout")
         return D.Y[:npt, :npt], D.X[:, :npt]
+
+    def solve_graph(self):
+        """The whole device solve captured once as a HIP graph for the current
+        k and replayed on each call (launch-bound panels cost one graph launch
+        instead of ~10 kernel launches per 32 columns).  Returns the device
+        (Y, W) views the replay writes; cancellation is not polled inside a
+        replay.  Re-captured when k changes."""
+        torch = _torch()
+        if getattr(self, "_graph_k", None) != self.k:
+            self._device()
+            side = torch.cuda.Stream()
+            side.wait_stream(torch.cuda.current_stream())
+            with torch.cuda.stream(side):
+                self._solve_device()
+            torch.cuda.current_stream().wait_stream(side)
+            g = torch.cuda.CUDAGraph()
+            with torch.cuda.graph(g):
+                out = self._solve_device()
+            self._graph, self._graph_out, self._graph_k = g, out, self.k
+        self._graph.replay()
+        return self._graph_out
 
     def _check_info(self):
         info = self._dev.info.cpu().numpy()

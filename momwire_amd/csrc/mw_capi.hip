@@ -13,8 +13,8 @@ template <typename T>
 int mw_getrf_t(int64_t n, void* A, int64_t lda, int32_t* ipiv, void* work, int32_t* info,
                const volatile int32_t* cancel, hipStream_t st, hipError_t* herr);
 template <typename T>
-int mw_getrs_t(int64_t n, int64_t nrhs, const void* A, int64_t lda, const int32_t* ipiv, void* X,
-               int64_t ldx, hipStream_t st, hipError_t* herr);
+int mw_getrs_t(int64_t n, int64_t nrhs, const void* A, int64_t lda, const int32_t* ipiv,
+               const void* work, void* X, int64_t ldx, hipStream_t st, hipError_t* herr);
 hipError_t mw_gemm_sub_z(int64_t m, int64_t n, int64_t k, const void* A, int64_t lda,
                          const void* B, int64_t ldb, void* C, int64_t ldc, hipStream_t st);
 hipError_t mw_gemm_sub_c(int64_t m, int64_t n, int64_t k, const void* A, int64_t lda,
@@ -108,26 +108,27 @@ int mw_cgetrf(int64_t n, void* A, int64_t lda, int32_t* ipiv, void* work, int32_
 }
 
 static int getrs_common(bool dbl, int64_t n, int64_t nrhs, const void* A, int64_t lda,
-                        const int32_t* ipiv, void* X, int64_t ldx, void* stream) {
+                        const int32_t* ipiv, const void* work, void* X, int64_t ldx,
+                        void* stream) {
   g_err[0] = 0;
   if (n <= 0 || nrhs < 0) return fail(MW_EINVAL, "%sbad sizes n=%lld nrhs=%lld", "", n, nrhs);
-  if (!A || !ipiv || !X) return fail(MW_EINVAL, "%sNULL argument");
+  if (!A || !ipiv || !X || !work) return fail(MW_EINVAL, "%sNULL argument");
   if (lda < n || ldx < nrhs) return fail(MW_EINVAL, "%sbad leading dimension (lda=%lld ldx=%lld)", "", lda, ldx);
   if (nrhs == 0) return MW_OK;
   hipError_t herr = hipSuccess;
-  int rc = dbl ? mw_getrs_t<double>(n, nrhs, A, lda, ipiv, X, ldx, (hipStream_t)stream, &herr)
-               : mw_getrs_t<float>(n, nrhs, A, lda, ipiv, X, ldx, (hipStream_t)stream, &herr);
+  int rc = dbl ? mw_getrs_t<double>(n, nrhs, A, lda, ipiv, work, X, ldx, (hipStream_t)stream, &herr)
+               : mw_getrs_t<float>(n, nrhs, A, lda, ipiv, work, X, ldx, (hipStream_t)stream, &herr);
   if (rc == MW_EHIP) return hip_status(herr, "getrs");
   return rc;
 }
 
-int mw_zgetrs(int64_t n, int64_t nrhs, const void* A, int64_t lda, const int32_t* ipiv, void* X,
-              int64_t ldx, void* stream) {
-  return getrs_common(true, n, nrhs, A, lda, ipiv, X, ldx, stream);
+int mw_zgetrs(int64_t n, int64_t nrhs, const void* A, int64_t lda, const int32_t* ipiv,
+              const void* work, void* X, int64_t ldx, void* stream) {
+  return getrs_common(true, n, nrhs, A, lda, ipiv, work, X, ldx, stream);
 }
-int mw_cgetrs(int64_t n, int64_t nrhs, const void* A, int64_t lda, const int32_t* ipiv, void* X,
-              int64_t ldx, void* stream) {
-  return getrs_common(false, n, nrhs, A, lda, ipiv, X, ldx, stream);
+int mw_cgetrs(int64_t n, int64_t nrhs, const void* A, int64_t lda, const int32_t* ipiv,
+              const void* work, void* X, int64_t ldx, void* stream) {
+  return getrs_common(false, n, nrhs, A, lda, ipiv, work, X, ldx, stream);
 }
 
 int mw_zgemm_sub(int64_t m, int64_t n, int64_t k, const void* A, int64_t lda, const void* B,

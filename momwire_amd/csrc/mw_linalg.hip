@@ -137,156 +137,207 @@ hipError_t launch_gemm_sub(int64_t m, int64_t n, int64_t k, const void* A, int64
 }
 
 // ---------------------------------------------------------------------------
-// Tournament pivot selection for one panel (columns k0 .. k0+bw).
+// Tournament pivot selection for one 32-column panel (columns k0 .. k0+bw).
 //
-// Leaves: each workgroup factors a 64-row chunk of the panel in LDS with
-// partial pivoting and nominates its bw pivot rows.  Merges: the last of two
-// sibling workgroups to finish (counted with a device-scope atomic) reloads
-// the two nominee sets' ORIGINAL rows (<= 64 rows), factors them the same
-// way and nominates again; the root's nominees are the panel's pivot rows,
-// and its in-LDS factorisation of them is L11\U11.  The matrix itself is
-// read-only here.
+// Leaves: each workgroup takes 128 rows of the panel, factors them with
+// partial pivoting and nominates its bw pivot rows.  Merges: a workgroup
+// reloads the ORIGINAL rows nominated by up to four siblings (<= 128 rows),
+// factors them the same way and nominates again; one kernel launch per tree
+// level, so kernel boundaries carry the nominees between levels.  The last
+// level's nominees are the panel's pivot rows, and its factorisation of them
+// is L11\U11.  The matrix is read-only here.
+//
+// The 128 x 32 block lives in registers: wave w holds columns 8w..8w+7, lane
+// l holds rows l and l+64.  Rows are never moved: a pivoted row is masked out
+// of later searches and updates, so one barrier per column suffices (the
+// owner wave publishes the pivot and the multipliers through LDS).
 // ---------------------------------------------------------------------------
-struct PivWs {
-  int32_t* cand;     // [levels][nodes][NB] nominated row ids
-  int32_t* ncand;    // [levels][nodes]
-  int32_t* counter;  // [levels][nodes] arrival counts (root resets them)
-  void* lu11;        // [NB][NB] complex
-};
+constexpr int RPL = 2;              // rows per lane
+constexpr int TROWS = 64 * RPL;     // rows per tournament block
+constexpr int FANIN = TROWS / NB;   // nominee sets merged per block
+
+MW_DEV double bcast_d(double x, int lane) {
+  const int lo = __builtin_amdgcn_readlane(__double2loint(x), lane);
+  const int hi = __builtin_amdgcn_readlane(__double2hiint(x), lane);
+  return __hiloint2double(hi, lo);
+}
+MW_DEV float bcast_f(float x, int lane) {
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x), lane));
+}
+MW_DEV double2 bcast(double2 v, int lane) { return make_double2(bcast_d(v.x, lane), bcast_d(v.y, lane)); }
+MW_DEV float2 bcast(float2 v, int lane) { return make_float2(bcast_f(v.x, lane), bcast_f(v.y, lane)); }
 
 template <typename CT>
-MW_DEV void lds_gepp(CT (*S)[NB + 1], int32_t* rid, int rows, int cols, int* piv_s,
-                     int32_t* info, int k0) {
-  const int tid = threadIdx.x, lane = tid & 63;
+struct TourLds {
+  CT mult[2][TROWS];     // multipliers of the current column (double-buffered)
+  int p[2];              // pivot row of the current column
+  int piv[NB];           // pivot row (block-local) of each column
+  int rank[TROWS];       // column at which a block-local row was pivoted
+  int32_t rid[TROWS];    // global row id of each block-local row
+  int rows;
+};
+
+// Factor the register block; on return L.piv[0..steps) holds the pivot rows.
+template <typename CT>
+MW_DEV void reg_gepp(CT (&S)[RPL][8], int rows, int cols, TourLds<CT>& L, int32_t* info,
+                     int64_t k0) {
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   const int steps = rows < cols ? rows : cols;
+  unsigned done = 0;  // bit t: row lane + 64 t already pivoted
   for (int c = 0; c < steps; ++c) {
-    if (tid < 64) {
-      double v = (lane >= c && lane < rows) ? cabs1(S[lane][c]) : -1.0;
-      int idx = lane;
+    const int buf = c & 1;
+    if (w == (c >> 3)) {
+      const int jj = c & 7;
+      // Magnitudes rank NaN as +inf and rows without a candidate as -1 with
+      // index -1, so the reduction always returns an unpivoted row of this
+      // block (steps <= rows guarantees one exists) whatever the data.
+      double best = -1.0;
+      int bidx = -1;
+      // column c of my rows (register select: no dynamic indexing)
+      // (masked sums, not selects: the compiler must not turn a uniform
+      // register choice into an indexed access, which would move S to scratch)
+      CT colv[RPL];
+#pragma unroll
+      for (int t = 0; t < RPL; ++t) {
+        colv[t].x = 0; colv[t].y = 0;
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+          colv[t].x += (j == jj) ? S[t][j].x : 0;
+          colv[t].y += (j == jj) ? S[t][j].y : 0;
+        }
+      }
+#pragma unroll
+      for (int t = 0; t < RPL; ++t) {
+        const int r = lane + 64 * t;
+        if (r < rows && !((done >> t) & 1)) {
+          double v = cabs1(colv[t]);
+          if (v != v) v = HUGE_VAL;
+          if (bidx < 0 || v > best) { best = v; bidx = r; }
+        }
+      }
 #pragma unroll
       for (int off = 32; off > 0; off >>= 1) {
-        const double ov = __shfl_xor(v, off);
-        const int oi = __shfl_xor(idx, off);
-        if (ov > v || (ov == v && oi < idx)) { v = ov; idx = oi; }
+        const double ov = __shfl_xor(best, off);
+        const int oi = __shfl_xor(bidx, off);
+        if (oi >= 0 && (bidx < 0 || ov > best || (ov == best && oi < bidx))) {
+          best = ov;
+          bidx = oi;
+        }
       }
-      if (lane == 0) *piv_s = idx;
-    }
-    __syncthreads();
-    const int p = *piv_s;
-    if (p != c) {
-      for (int j = tid; j < cols; j += blockDim.x) {
-        const CT t = S[c][j]; S[c][j] = S[p][j]; S[p][j] = t;
+      int p = __builtin_amdgcn_readfirstlane(bidx);
+      p = p < 0 ? 0 : (p >= rows ? rows - 1 : p);
+      const int pl = p & 63, pt = p >> 6;
+      CT src;
+      src.x = 0; src.y = 0;
+#pragma unroll
+      for (int t = 0; t < RPL; ++t) {
+        src.x += (t == pt) ? colv[t].x : 0;
+        src.y += (t == pt) ? colv[t].y : 0;
       }
-      if (tid == 0) { const int32_t t = rid[c]; rid[c] = rid[p]; rid[p] = t; }
-    }
-    __syncthreads();
-    const CT d = S[c][c];
-    const bool zero = (d.x == 0 && d.y == 0);
-    if (zero && info && tid == 0 && *info == 0) *info = k0 + c + 1;
-    if (!zero) {
-      const CT inv = crecip(d);
-      for (int r = c + 1 + tid; r < rows; r += blockDim.x) S[r][c] = cmul(S[r][c], inv);
-    }
-    __syncthreads();
-    if (!zero) {
-      const int nr = rows - c - 1, ncl = cols - c - 1;
-      for (int e = tid; e < nr * ncl; e += blockDim.x) {
-        const int r = c + 1 + e / ncl, j = c + 1 + e % ncl;
-        S[r][j] = cfms(S[r][j], S[r][c], S[c][j]);
+      const CT apc = bcast(src, pl);
+      const bool zero = (apc.x == 0 && apc.y == 0);
+      if (zero && info && lane == 0 && *info == 0) *info = (int32_t)(k0 + c + 1);
+      CT inv;
+      inv.x = 0; inv.y = 0;
+      if (!zero) inv = crecip(apc);
+#pragma unroll
+      for (int t = 0; t < RPL; ++t) {
+        const int r = lane + 64 * t;
+        CT mlt;
+        mlt.x = 0; mlt.y = 0;
+        if (r < rows && !((done >> t) & 1) && r != p) {
+          mlt = cmul(colv[t], inv);
+#pragma unroll
+          for (int j = 0; j < 8; ++j) {
+            S[t][j].x = (j == jj) ? mlt.x : S[t][j].x;
+            S[t][j].y = (j == jj) ? mlt.y : S[t][j].y;
+          }
+        }
+        L.mult[buf][r] = mlt;
+      }
+      if (lane == 0) {
+        L.p[buf] = p;
+        L.piv[c] = p;
+        L.rank[p] = c;
       }
     }
     __syncthreads();
+    const int p = __builtin_amdgcn_readfirstlane(L.p[buf]);
+    const int pl = p & 63, pt = p >> 6;
+    if (lane == pl) done |= 1u << pt;
+    CT u[8];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      CT src;
+      src.x = 0; src.y = 0;
+#pragma unroll
+      for (int t = 0; t < RPL; ++t) {
+        src.x += (t == pt) ? S[t][j].x : 0;
+        src.y += (t == pt) ? S[t][j].y : 0;
+      }
+      u[j] = bcast(src, pl);
+    }
+#pragma unroll
+    for (int t = 0; t < RPL; ++t) {
+      const CT mlt = L.mult[buf][lane + 64 * t];
+#pragma unroll
+      for (int j = 0; j < 8; ++j)
+        if (8 * w + j > c) S[t][j] = cfms(S[t][j], mlt, u[j]);
+    }
   }
 }
 
-template <typename T>
-__global__ __launch_bounds__(256) void k_tournament(int64_t n, typename cplx_t<T>::type* A,
-                                                    int64_t lda, int64_t k0, int bw, int nleaf,
-                                                    PivWs ws, int32_t* ipiv, int32_t* info) {
-  typedef typename cplx_t<T>::type CT;
-  __shared__ CT S[CH][NB + 1];
-  __shared__ int32_t rid[CH];
-  __shared__ int piv_s, last_s, rows_s;
-  __shared__ int32_t pos_s[2 * NB], val_s[2 * NB];
-
-  const int tid = threadIdx.x;
-  const int64_t m = n - k0;
-  int level = 0, node = blockIdx.x, nodes = nleaf;
-  int lvl_off = 0;  // node offset of this level in the flattened arrays
-
-  // leaf: rows k0 + node*CH ...
-  int rows = (int)((m - (int64_t)node * CH) < CH ? (m - (int64_t)node * CH) : CH);
-  for (int e = tid; e < rows * bw; e += blockDim.x) {
-    const int r = e / bw, c = e % bw;
-    S[r][c] = A[(k0 + (int64_t)node * CH + r) * lda + k0 + c];
-  }
-  if (tid < rows) rid[tid] = (int32_t)(k0 + (int64_t)node * CH + tid);
-  __syncthreads();
-
-  while (true) {
-    lds_gepp(S, rid, rows, bw, &piv_s, nullptr, k0);
-    const int nom = rows < bw ? rows : bw;
-    if (nodes == 1) break;  // root
-    // publish nominees
-    if (tid < nom) ws.cand[(size_t)(lvl_off + node) * NB + tid] = rid[tid];
-    if (tid == 0) ws.ncand[lvl_off + node] = nom;
-    __threadfence();
-    __syncthreads();
-    const int parent = node >> 1;
-    const int pnodes = (nodes + 1) >> 1;
-    const int nchild = (2 * parent + 1 < nodes) ? 2 : 1;
-    const int plvl_off = lvl_off + nodes;
-    if (tid == 0) {
-      const int old = __hip_atomic_fetch_add(&ws.counter[plvl_off + parent], 1, __ATOMIC_ACQ_REL,
-                                             __HIP_MEMORY_SCOPE_AGENT);
-      last_s = (old == nchild - 1);
+// Load rows L.rid[0..rows) (global ids) of panel columns k0..k0+bw.
+template <typename CT>
+MW_DEV void load_block(CT (&S)[RPL][8], const CT* A, int64_t lda, int64_t k0, int bw,
+                       const TourLds<CT>& L, int rows) {
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+#pragma unroll
+  for (int t = 0; t < RPL; ++t) {
+    const int r = lane + 64 * t;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      const int c = 8 * w + j;
+      CT v;
+      v.x = 0; v.y = 0;
+      if (r < rows && c < bw) v = A[(int64_t)L.rid[r] * lda + k0 + c];
+      S[t][j] = v;
     }
-    __syncthreads();
-    if (!last_s) return;
-    __threadfence();
-    // gather both children's nominees (original rows)
-    if (tid == 0) {
-      int cnt = 0;
-      for (int ch = 0; ch < nchild; ++ch) {
-        const int cn = 2 * parent + ch;
-        const int nc = __hip_atomic_load(&ws.ncand[lvl_off + cn], __ATOMIC_ACQUIRE,
-                                         __HIP_MEMORY_SCOPE_AGENT);
-        for (int t = 0; t < nc; ++t)
-          rid[cnt++] = __hip_atomic_load(&ws.cand[(size_t)(lvl_off + cn) * NB + t],
-                                         __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+  }
+}
+
+// Root: write L11\U11 of the pivot rows in pivot order and the LAPACK-style
+// interchange sequence.
+template <typename CT>
+MW_DEV void finish_root(const CT (&S)[RPL][8], int rows, int bw, TourLds<CT>& L, CT* lu11,
+                        int32_t* ipiv, int64_t k0, int32_t* info) {
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int steps = rows < bw ? rows : bw;
+#pragma unroll
+  for (int t = 0; t < RPL; ++t) {
+    const int r = lane + 64 * t;
+    if (r < rows) {
+      bool pivoted = false;
+      for (int i = 0; i < steps; ++i)
+        if (L.piv[i] == r) pivoted = true;
+      if (pivoted) {
+        const int i = L.rank[r];
+#pragma unroll
+        for (int j = 0; j < 8; ++j)
+          if (8 * w + j < bw) lu11[i * NB + 8 * w + j] = S[t][j];
       }
-      rows_s = cnt;
     }
-    __syncthreads();
-    rows = rows_s;
-    for (int e = tid; e < rows * bw; e += blockDim.x) {
-      const int r = e / bw, c = e % bw;
-      S[r][c] = A[(int64_t)rid[r] * lda + k0 + c];
-    }
-    __syncthreads();
-    level += 1;
-    node = parent;
-    lvl_off = plvl_off;
-    nodes = pnodes;
-  }
-
-  // Root: rid[0..bw) are the pivot rows in order; S holds L11\U11.
-  CT* lu = reinterpret_cast<CT*>(ws.lu11);
-  for (int e = tid; e < bw * bw; e += blockDim.x) lu[e] = S[e / bw][e % bw];
-  for (int c = 0; c < bw; ++c) {
-    const CT d = S[c][c];
-    if (d.x == 0 && d.y == 0 && tid == 0 && *info == 0) *info = (int32_t)(k0 + c + 1);
   }
   if (tid == 0) {
-    // Sequential interchanges that bring rid[i] to position k0+i.
-    int nt = 0;  // tracked positions whose content changed
-    for (int i = 0; i < bw; ++i) {
-      const int32_t g = rid[i];
-      int32_t pg = g;  // current position of row id g
+    __shared__ int32_t pos_s[2 * NB], val_s[2 * NB];
+    int nt = 0;
+    for (int i = 0; i < steps; ++i) {
+      const int32_t g = L.rid[L.piv[i]];
+      int32_t pg = g;
       for (int t = 0; t < nt; ++t)
         if (val_s[t] == g) { pg = pos_s[t]; break; }
       const int32_t p1 = (int32_t)(k0 + i);
-      int32_t id1 = p1;  // current content of position p1
+      int32_t id1 = p1;
       int t1 = -1, t2 = -1;
       for (int t = 0; t < nt; ++t) {
         if (pos_s[t] == p1) { id1 = val_s[t]; t1 = t; }
@@ -300,12 +351,53 @@ __global__ __launch_bounds__(256) void k_tournament(int64_t n, typename cplx_t<T
         val_s[t2] = id1;
       }
     }
+    if (steps < bw && info && *info == 0) *info = (int32_t)(k0 + steps + 1);
   }
-  // reset the arrival counters for the next panel
-  int total = 0;
-  for (int nd = nleaf; nd > 1; nd = (nd + 1) >> 1) total += nd;
-  total += 1;
-  for (int e = tid; e < total; e += blockDim.x) ws.counter[e] = 0;
+}
+
+// level < 0: leaf over the panel rows; otherwise merge nominee sets of the
+// previous level.  is_root: this launch has one block and finishes the panel.
+template <typename T>
+__global__ __launch_bounds__(256) void k_tournament(const typename cplx_t<T>::type* A, int64_t lda,
+                                                    int64_t n, int64_t k0, int bw, int leaf,
+                                                    const int32_t* in_cand, const int32_t* in_n,
+                                                    int n_in, int32_t* out_cand, int32_t* out_n,
+                                                    int is_root, void* lu11, int32_t* ipiv,
+                                                    int32_t* info) {
+  typedef typename cplx_t<T>::type CT;
+  __shared__ TourLds<CT> L;
+  const int tid = threadIdx.x;
+  const int b = blockIdx.x;
+  if (tid == 0) {
+    int rows = 0;
+    if (leaf) {
+      const int64_t r0 = k0 + (int64_t)b * TROWS;
+      const int64_t left = n - r0;
+      rows = (int)(left < TROWS ? left : TROWS);
+      for (int r = 0; r < rows; ++r) L.rid[r] = (int32_t)(r0 + r);
+    } else {
+      for (int s = 0; s < FANIN; ++s) {
+        const int src = b * FANIN + s;
+        if (src >= n_in) break;
+        const int cnt = in_n[src];
+        for (int t = 0; t < cnt; ++t) L.rid[rows++] = in_cand[src * NB + t];
+      }
+    }
+    L.rows = rows;
+  }
+  __syncthreads();
+  const int rows = L.rows;
+  CT S[RPL][8];
+  load_block(S, A, lda, k0, bw, L, rows);
+  reg_gepp(S, rows, bw, L, is_root ? info : (int32_t*)nullptr, k0);
+  __syncthreads();
+  const int steps = rows < bw ? rows : bw;
+  if (is_root) {
+    finish_root(S, rows, bw, L, reinterpret_cast<CT*>(lu11), ipiv, k0, info);
+  } else {
+    if (tid < steps) out_cand[b * NB + tid] = L.rid[L.piv[tid]];
+    if (tid == 0) out_n[b] = steps;
+  }
 }
 
 // Row interchanges ipiv[k0 .. k0+bw) on columns [c0, c1).
@@ -325,148 +417,186 @@ __global__ void k_laswp(typename cplx_t<T>::type* A, int64_t lda, int64_t k0, in
   }
 }
 
-// After the interchanges: A11 <- L11\U11, L21 <- A21 U11^{-1}, U12 <- L11^{-1} A12.
-// Blocks [0, nrb): rows of L21; [nrb, nrb+ncb): columns of U12; last: A11.
+// Diagonal blocks are inverted once per panel (L11^{-1}, U11^{-1}, padded to
+// NB with the identity) and applied as small LDS-tiled products: the usual
+// GPU substitute for a triangular solve with many right-hand sides.
 template <typename T>
-__global__ __launch_bounds__(256) void k_panel_trsm(int64_t n, typename cplx_t<T>::type* A,
-                                                    int64_t lda, int64_t k0, int bw,
-                                                    const void* lu11, int nrb, int ncb) {
+__global__ __launch_bounds__(64) void k_tri_inv(const typename cplx_t<T>::type* lu, int64_t ldl,
+                                                int bw, typename cplx_t<T>::type* out) {
   typedef typename cplx_t<T>::type CT;
-  __shared__ CT L[NB][NB + 1];
-  __shared__ CT dinv[NB];
+  __shared__ CT M[NB][NB + 1], Li[NB][NB + 1], Ui[NB][NB + 1];
   const int tid = threadIdx.x;
-  const CT* lu = reinterpret_cast<const CT*>(lu11);
-  for (int e = tid; e < bw * bw; e += blockDim.x) L[e / bw][e % bw] = lu[e];
+  for (int e = tid; e < NB * NB; e += blockDim.x) {
+    const int i = e / NB, j = e % NB;
+    CT v;
+    v.x = (i == j) ? 1 : 0; v.y = 0;
+    if (i < bw && j < bw) v = lu[(int64_t)i * ldl + j];
+    M[i][j] = v;
+  }
   __syncthreads();
-  if (tid < bw) dinv[tid] = crecip(L[tid][tid]);
+  if (tid < NB) {
+    // column j of L^{-1}: L x = e_j, x_i = -sum_{l=j}^{i-1} L[i][l] x_l
+    const int j = tid;
+    for (int i = 0; i < NB; ++i) {
+      CT v;
+      v.x = (i == j) ? 1 : 0; v.y = 0;
+      if (i > j) {
+        for (int l = j; l < i; ++l) v = cfms(v, M[i][l], Li[l][j]);
+      }
+      Li[i][j] = v;
+    }
+  } else if (tid < 2 * NB) {
+    // column j of U^{-1}: U x = e_j, back substitution
+    const int j = tid - NB;
+    for (int i = NB - 1; i >= 0; --i) {
+      CT v;
+      v.x = 0; v.y = 0;
+      if (i <= j) {
+        if (i == j) v.x = 1;
+        for (int l = i + 1; l <= j; ++l) v = cfms(v, M[i][l], Ui[l][j]);
+        // an exactly zero pivot (already reported through info) maps to a
+        // zero inverse entry rather than inf/NaN
+        if (M[i][i].x != 0 || M[i][i].y != 0) v = cdiv(v, M[i][i]);
+        else { v.x = 0; v.y = 0; }
+      }
+      Ui[i][j] = v;
+    }
+  }
   __syncthreads();
-  const int64_t rest = n - k0 - bw;
+  for (int e = tid; e < NB * NB; e += blockDim.x) {
+    out[e] = Li[e / NB][e % NB];
+    out[NB * NB + e] = Ui[e / NB][e % NB];
+  }
+}
+
+constexpr int AW = 64;  // rows (or columns) per workgroup in the block products
+
+template <typename CT>
+struct ApplyLds {
+  CT Tm[NB][NB + 1];
+  CT B[AW][NB + 1];     // rows mode: AW rows x NB; cols mode: used as [NB][AW] below
+};
+
+// out[i][c] = sum_l Tm[i][l] X[k0+l][c] for the NB x AW block at columns c0.
+template <typename CT>
+MW_DEV void apply_left(ApplyLds<CT>& L, CT* X, int64_t ldx, int64_t k0, int bw, int64_t c0,
+                       int64_t ncols) {
+  const int tid = threadIdx.x;
+  CT (*Bc)[AW + 1] = reinterpret_cast<CT (*)[AW + 1]>(&L.B[0][0]);
+  for (int e = tid; e < NB * AW; e += blockDim.x) {
+    const int i = e / AW, cc = e % AW;
+    CT v;
+    v.x = 0; v.y = 0;
+    if (i < bw && c0 + cc < ncols) v = X[(k0 + i) * ldx + c0 + cc];
+    Bc[i][cc] = v;
+  }
+  __syncthreads();
+  const int cc = tid % AW, i0 = (tid / AW) * 8;
+  CT acc[8];
+#pragma unroll
+  for (int q = 0; q < 8; ++q) { acc[q].x = 0; acc[q].y = 0; }
+#pragma unroll 4
+  for (int l = 0; l < NB; ++l) {
+    const CT b = Bc[l][cc];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      const CT t = L.Tm[i0 + q][l];
+      acc[q].x += t.x * b.x - t.y * b.y;
+      acc[q].y += t.x * b.y + t.y * b.x;
+    }
+  }
+  if (c0 + cc < ncols) {
+#pragma unroll
+    for (int q = 0; q < 8; ++q)
+      if (i0 + q < bw) X[(k0 + i0 + q) * ldx + c0 + cc] = acc[q];
+  }
+}
+
+// After the interchanges: A11 <- L11\U11, L21 <- A21 U11^{-1}, U12 <- L11^{-1} A12.
+// Blocks [0, nrb): 64-row blocks of L21; [nrb, nrb+ncb): 64-column blocks of
+// U12; last: A11.
+template <typename T>
+__global__ __launch_bounds__(256) void k_panel_apply(int64_t n, typename cplx_t<T>::type* A,
+                                                     int64_t lda, int64_t k0, int bw,
+                                                     const typename cplx_t<T>::type* inv,
+                                                     const typename cplx_t<T>::type* lu11,
+                                                     int nrb, int ncb) {
+  typedef typename cplx_t<T>::type CT;
+  __shared__ ApplyLds<CT> L;
+  const int tid = threadIdx.x;
   const int b = blockIdx.x;
   if (b < nrb) {
-    // one row of L21 per thread: x U11 = a
-    const int64_t r = k0 + bw + (int64_t)b * 256 + tid;
-    if (r >= n) return;
-    CT* row = A + r * lda + k0;
-    CT x[NB];
-#pragma unroll
-    for (int j = 0; j < NB; ++j)
-      if (j < bw) x[j] = row[j];
-#pragma unroll
-    for (int j = 0; j < NB; ++j) {
-      if (j >= bw) break;
-      CT s = x[j];
-#pragma unroll
-      for (int l = 0; l < NB; ++l) {
-        if (l >= j) break;
-        s = cfms(s, x[l], L[l][j]);
-      }
-      x[j] = cmul(s, dinv[j]);
+    const CT* Ui = inv + NB * NB;
+    for (int e = tid; e < NB * NB; e += blockDim.x) L.Tm[e / NB][e % NB] = Ui[e];
+    const int64_t r0 = k0 + bw + (int64_t)b * AW;
+    for (int e = tid; e < AW * NB; e += blockDim.x) {
+      const int rr = e / NB, l = e % NB;
+      CT v;
+      v.x = 0; v.y = 0;
+      if (r0 + rr < n && l < bw) v = A[(r0 + rr) * lda + k0 + l];
+      L.B[rr][l] = v;
     }
+    __syncthreads();
+    const int rr = tid / 4, j0 = (tid % 4) * 8;
+    CT acc[8];
 #pragma unroll
-    for (int j = 0; j < NB; ++j)
-      if (j < bw) row[j] = x[j];
+    for (int q = 0; q < 8; ++q) { acc[q].x = 0; acc[q].y = 0; }
+#pragma unroll 4
+    for (int l = 0; l < NB; ++l) {
+      const CT a = L.B[rr][l];
+#pragma unroll
+      for (int q = 0; q < 8; ++q) {
+        const CT t = L.Tm[l][j0 + q];
+        acc[q].x += a.x * t.x - a.y * t.y;
+        acc[q].y += a.x * t.y + a.y * t.x;
+      }
+    }
+    if (r0 + rr < n) {
+#pragma unroll
+      for (int q = 0; q < 8; ++q)
+        if (j0 + q < bw) A[(r0 + rr) * lda + k0 + j0 + q] = acc[q];
+    }
   } else if (b < nrb + ncb) {
-    // one column of U12 per thread: L11 x = a (unit lower)
-    const int64_t c = k0 + bw + (int64_t)(b - nrb) * 256 + tid;
-    if (c >= n) return;
-    CT x[NB];
-#pragma unroll
-    for (int i = 0; i < NB; ++i)
-      if (i < bw) x[i] = A[(k0 + i) * lda + c];
-#pragma unroll
-    for (int i = 0; i < NB; ++i) {
-      if (i >= bw) break;
-      CT s = x[i];
-#pragma unroll
-      for (int l = 0; l < NB; ++l) {
-        if (l >= i) break;
-        s = cfms(s, L[i][l], x[l]);
-      }
-      x[i] = s;
-    }
-#pragma unroll
-    for (int i = 0; i < NB; ++i)
-      if (i < bw) A[(k0 + i) * lda + c] = x[i];
+    for (int e = tid; e < NB * NB; e += blockDim.x) L.Tm[e / NB][e % NB] = inv[e];
+    __syncthreads();
+    apply_left(L, A, lda, k0, bw, k0 + bw + (int64_t)(b - nrb) * AW, n);
   } else {
     for (int e = tid; e < bw * bw; e += blockDim.x)
-      A[(k0 + e / bw) * lda + k0 + e % bw] = L[e / bw][e % bw];
+      A[(k0 + e / bw) * lda + k0 + e % bw] = lu11[(e / bw) * NB + e % bw];
   }
-  (void)rest;
 }
 
-// Forward block of the solve: per RHS column, apply the panel's interchanges
-// then X_p <- L11^{-1} X_p.
+// Solve blocks on AW right-hand-side columns per workgroup.
+// fwd: interchanges of the panel, then X_p <- L11^{-1} X_p; bwd: X_p <- U11^{-1} X_p.
 template <typename T>
-__global__ __launch_bounds__(256) void k_solve_fwd_block(int64_t nrhs, const typename cplx_t<T>::type* A,
-                                                         int64_t lda, int64_t k0, int bw,
-                                                         const int32_t* ipiv,
-                                                         typename cplx_t<T>::type* X, int64_t ldx) {
+__global__ __launch_bounds__(256) void k_solve_apply(int64_t nrhs, const typename cplx_t<T>::type* inv,
+                                                     int64_t k0, int bw, const int32_t* ipiv,
+                                                     typename cplx_t<T>::type* X, int64_t ldx,
+                                                     int fwd) {
   typedef typename cplx_t<T>::type CT;
-  __shared__ CT L[NB][NB + 1];
+  __shared__ ApplyLds<CT> L;
   const int tid = threadIdx.x;
-  for (int e = tid; e < bw * bw; e += blockDim.x) L[e / bw][e % bw] = A[(k0 + e / bw) * lda + k0 + e % bw];
+  const int64_t c0 = (int64_t)blockIdx.x * AW;
+  const CT* Tsrc = fwd ? inv : inv + NB * NB;
+  for (int e = tid; e < NB * NB; e += blockDim.x) L.Tm[e / NB][e % NB] = Tsrc[e];
+  if (fwd && tid < AW && c0 + tid < nrhs) {
+    const int64_t c = c0 + tid;
+    for (int i = 0; i < bw; ++i) {
+      const int64_t p1 = k0 + i, p2 = ipiv[k0 + i];
+      if (p2 != p1) {
+        const CT t = X[p1 * ldx + c]; X[p1 * ldx + c] = X[p2 * ldx + c]; X[p2 * ldx + c] = t;
+      }
+    }
+  }
   __syncthreads();
-  const int64_t c = (int64_t)blockIdx.x * blockDim.x + tid;
-  if (c >= nrhs) return;
-  for (int i = 0; i < bw; ++i) {
-    const int64_t p1 = k0 + i, p2 = ipiv[k0 + i];
-    if (p2 != p1) {
-      const CT t = X[p1 * ldx + c]; X[p1 * ldx + c] = X[p2 * ldx + c]; X[p2 * ldx + c] = t;
-    }
-  }
-  CT x[NB];
-#pragma unroll
-  for (int i = 0; i < NB; ++i)
-    if (i < bw) x[i] = X[(k0 + i) * ldx + c];
-#pragma unroll
-  for (int i = 0; i < NB; ++i) {
-    if (i >= bw) break;
-    CT s = x[i];
-#pragma unroll
-    for (int l = 0; l < NB; ++l) {
-      if (l >= i) break;
-      s = cfms(s, L[i][l], x[l]);
-    }
-    x[i] = s;
-  }
-#pragma unroll
-  for (int i = 0; i < NB; ++i)
-    if (i < bw) X[(k0 + i) * ldx + c] = x[i];
+  apply_left(L, X, ldx, k0, bw, c0, nrhs);
 }
 
-// Backward block: X_p <- U11^{-1} X_p.
-template <typename T>
-__global__ __launch_bounds__(256) void k_solve_bwd_block(int64_t nrhs, const typename cplx_t<T>::type* A,
-                                                         int64_t lda, int64_t k0, int bw,
-                                                         typename cplx_t<T>::type* X, int64_t ldx) {
-  typedef typename cplx_t<T>::type CT;
-  __shared__ CT U[NB][NB + 1];
-  __shared__ CT dinv[NB];
-  const int tid = threadIdx.x;
-  for (int e = tid; e < bw * bw; e += blockDim.x) U[e / bw][e % bw] = A[(k0 + e / bw) * lda + k0 + e % bw];
-  __syncthreads();
-  if (tid < bw) dinv[tid] = crecip(U[tid][tid]);
-  __syncthreads();
-  const int64_t c = (int64_t)blockIdx.x * blockDim.x + tid;
-  if (c >= nrhs) return;
-  CT x[NB];
-#pragma unroll
-  for (int i = 0; i < NB; ++i)
-    if (i < bw) x[i] = X[(k0 + i) * ldx + c];
-#pragma unroll
-  for (int ii = NB - 1; ii >= 0; --ii) {
-    if (ii >= bw) continue;
-    CT s = x[ii];
-#pragma unroll
-    for (int l = NB - 1; l >= 0; --l) {
-      if (l <= ii) break;
-      if (l < bw) s = cfms(s, U[ii][l], x[l]);
-    }
-    x[ii] = cmul(s, dinv[ii]);
-  }
-#pragma unroll
-  for (int i = 0; i < NB; ++i)
-    if (i < bw) X[(k0 + i) * ldx + c] = x[i];
+// Zero a small int32 buffer (used instead of hipMemsetAsync so the whole
+// factorisation can be captured in a graph).
+__global__ void k_zero_i32(int32_t* p, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0;
 }
 
 // ---------------------------------------------------------------------------
@@ -521,14 +651,16 @@ __global__ __launch_bounds__(256) void k_small_solve(int ns, int nt, double2* S,
   __syncthreads();
   for (int c = 0; c < ns; ++c) {
     if (tid < 64) {
-      double v = (lane >= c && lane < ns) ? cabs1(M[lane][c]) : -1.0;
-      int idx = lane;
+      const bool cand = lane >= c && lane < ns;
+      double v = cand ? cabs1(M[lane][c]) : -1.0;
+      if (v != v) v = HUGE_VAL;
+      int idx = cand ? lane : -1;
       for (int off = 32; off > 0; off >>= 1) {
         const double ov = __shfl_xor(v, off);
         const int oi = __shfl_xor(idx, off);
-        if (ov > v || (ov == v && oi < idx)) { v = ov; idx = oi; }
+        if (oi >= 0 && (idx < 0 || ov > v || (ov == v && oi < idx))) { v = ov; idx = oi; }
       }
-      if (lane == 0) piv_s = idx;
+      if (lane == 0) piv_s = idx < c ? c : (idx >= ns ? ns - 1 : idx);
     }
     __syncthreads();
     const int p = piv_s;
@@ -558,29 +690,19 @@ __global__ __launch_bounds__(256) void k_small_solve(int ns, int nt, double2* S,
 }  // namespace
 
 // ---------------------------------------------------------------------------
-// Host drivers (called from mw_capi.cpp)
+// Host drivers (called from mw_capi.hip)
 // ---------------------------------------------------------------------------
+// workspace: lu11 [NB*NB complex double] | diagonal-block inverses
+// [n_panels][2][NB*NB complex double] | cand ping/pong [2][nleaf*NB] | counts [2][nleaf]
 int64_t mw_getrf_ws_bytes(int64_t n) {
-  int64_t nleaf = (n + CH - 1) / CH;
-  int64_t nodes = 0;
-  for (int64_t nd = nleaf; nd > 1; nd = (nd + 1) / 2) nodes += nd;
-  nodes += 1;
-  int64_t bytes = nodes * NB * 4 + nodes * 4 + nodes * 4 + (int64_t)NB * NB * 16;
-  return (bytes + 255) / 256 * 256 + 1024;
+  const int64_t nleaf = (n + TROWS - 1) / TROWS;
+  const int64_t npan = (n + NB - 1) / NB;
+  const int64_t bytes = (int64_t)NB * NB * 16 * (1 + 2 * npan) + 2 * nleaf * NB * 4 + 2 * nleaf * 4;
+  return (bytes + 255) / 256 * 256;
 }
 
-static PivWs carve_ws(void* work, int64_t n) {
-  int64_t nleaf = (n + CH - 1) / CH;
-  int64_t nodes = 0;
-  for (int64_t nd = nleaf; nd > 1; nd = (nd + 1) / 2) nodes += nd;
-  nodes += 1;
-  char* p = reinterpret_cast<char*>(work);
-  PivWs ws;
-  ws.lu11 = p;                 p += (int64_t)NB * NB * 16;
-  ws.counter = (int32_t*)p;    p += nodes * 4;
-  ws.ncand = (int32_t*)p;      p += nodes * 4;
-  ws.cand = (int32_t*)p;
-  return ws;
+static void* panel_inv(void* work, int64_t k0) {
+  return reinterpret_cast<char*>(work) + (int64_t)NB * NB * 16 * (1 + 2 * (k0 / NB));
 }
 
 template <typename T>
@@ -588,24 +710,40 @@ int mw_getrf_t(int64_t n, void* Av, int64_t lda, int32_t* ipiv, void* work, int3
                const volatile int32_t* cancel, hipStream_t st, hipError_t* herr) {
   typedef typename cplx_t<T>::type CT;
   CT* A = reinterpret_cast<CT*>(Av);
-  PivWs ws = carve_ws(work, n);
-  *herr = hipMemsetAsync(work, 0, mw_getrf_ws_bytes(n), st);
-  if (*herr != hipSuccess) return MW_EHIP;
-  *herr = hipMemsetAsync(info, 0, sizeof(int32_t), st);
-  if (*herr != hipSuccess) return MW_EHIP;
+  const int64_t nleaf_max = (n + TROWS - 1) / TROWS;
+  char* wp = reinterpret_cast<char*>(work);
+  void* lu11 = wp;
+  const int64_t npan = (n + NB - 1) / NB;
+  char* cbase = wp + (int64_t)NB * NB * 16 * (1 + 2 * npan);
+  int32_t* cand[2] = {(int32_t*)cbase, (int32_t*)cbase + nleaf_max * NB};
+  int32_t* cnt[2] = {cand[1] + nleaf_max * NB, cand[1] + nleaf_max * NB + nleaf_max};
+  hipLaunchKernelGGL(k_zero_i32, dim3(1), dim3(64), 0, st, info, (int64_t)1);
   for (int64_t k0 = 0; k0 < n; k0 += NB) {
     if (cancel && *cancel) return MW_EABORT;
     const int bw = (int)((n - k0) < NB ? (n - k0) : NB);
-    const int nleaf = (int)((n - k0 + CH - 1) / CH);
-    hipLaunchKernelGGL(k_tournament<T>, dim3(nleaf), dim3(256), 0, st, n, A, lda, k0, bw, nleaf,
-                       ws, ipiv, info);
+    int nodes = (int)((n - k0 + TROWS - 1) / TROWS);
+    int cur = 0;
+    hipLaunchKernelGGL(k_tournament<T>, dim3(nodes), dim3(256), 0, st, A, lda, n, k0, bw, 1,
+                       (const int32_t*)nullptr, (const int32_t*)nullptr, 0, cand[0], cnt[0],
+                       nodes == 1 ? 1 : 0, lu11, ipiv, info);
+    while (nodes > 1) {
+      const int out = (nodes + FANIN - 1) / FANIN;
+      hipLaunchKernelGGL(k_tournament<T>, dim3(out), dim3(256), 0, st, A, lda, n, k0, bw, 0,
+                         cand[cur], cnt[cur], nodes, cand[cur ^ 1], cnt[cur ^ 1],
+                         out == 1 ? 1 : 0, lu11, ipiv, info);
+      cur ^= 1;
+      nodes = out;
+    }
     const int64_t ncol = n - k0;
     hipLaunchKernelGGL(k_laswp<T>, dim3((unsigned)((ncol + 255) / 256)), dim3(256), 0, st, A, lda,
                        k0, bw, ipiv, k0, n);
     const int64_t rest = n - k0 - bw;
-    const int nrb = (int)((rest + 255) / 256);
-    hipLaunchKernelGGL(k_panel_trsm<T>, dim3(2 * nrb + 1), dim3(256), 0, st, n, A, lda, k0, bw,
-                       ws.lu11, nrb, nrb);
+    const int nrb = (int)((rest + AW - 1) / AW);
+    CT* inv = reinterpret_cast<CT*>(panel_inv(work, k0));
+    hipLaunchKernelGGL(k_tri_inv<T>, dim3(1), dim3(64), 0, st, (const CT*)lu11, (int64_t)NB, bw,
+                       inv);
+    hipLaunchKernelGGL(k_panel_apply<T>, dim3(2 * nrb + 1), dim3(256), 0, st, n, A, lda, k0, bw,
+                       (const CT*)inv, (const CT*)lu11, nrb, nrb);
     if (rest > 0) {
       *herr = launch_gemm_sub<T>(rest, rest, bw, A + (k0 + bw) * lda + k0, lda,
                                  A + k0 * lda + k0 + bw, lda, A + (k0 + bw) * lda + k0 + bw, lda,
@@ -620,15 +758,16 @@ int mw_getrf_t(int64_t n, void* Av, int64_t lda, int32_t* ipiv, void* work, int3
 
 template <typename T>
 int mw_getrs_t(int64_t n, int64_t nrhs, const void* Av, int64_t lda, const int32_t* ipiv,
-               void* Xv, int64_t ldx, hipStream_t st, hipError_t* herr) {
+               const void* work, void* Xv, int64_t ldx, hipStream_t st, hipError_t* herr) {
   typedef typename cplx_t<T>::type CT;
   const CT* A = reinterpret_cast<const CT*>(Av);
   CT* X = reinterpret_cast<CT*>(Xv);
-  const unsigned gx = (unsigned)((nrhs + 255) / 256);
+  const unsigned gx = (unsigned)((nrhs + AW - 1) / AW);
   for (int64_t k0 = 0; k0 < n; k0 += NB) {
     const int bw = (int)((n - k0) < NB ? (n - k0) : NB);
-    hipLaunchKernelGGL(k_solve_fwd_block<T>, dim3(gx), dim3(256), 0, st, nrhs, A, lda, k0, bw,
-                       ipiv, X, ldx);
+    const CT* inv = reinterpret_cast<const CT*>(panel_inv(const_cast<void*>(work), k0));
+    hipLaunchKernelGGL(k_solve_apply<T>, dim3(gx), dim3(256), 0, st, nrhs, inv, k0, bw, ipiv, X,
+                       ldx, 1);
     const int64_t rest = n - k0 - bw;
     if (rest > 0) {
       *herr = launch_gemm_sub<T>(rest, nrhs, bw, A + (k0 + bw) * lda + k0, lda, X + k0 * ldx,
@@ -639,8 +778,9 @@ int mw_getrs_t(int64_t n, int64_t nrhs, const void* Av, int64_t lda, const int32
   const int64_t nlast = ((n + NB - 1) / NB - 1) * NB;
   for (int64_t k0 = nlast; k0 >= 0; k0 -= NB) {
     const int bw = (int)((n - k0) < NB ? (n - k0) : NB);
-    hipLaunchKernelGGL(k_solve_bwd_block<T>, dim3(gx), dim3(256), 0, st, nrhs, A, lda, k0, bw, X,
-                       ldx);
+    const CT* inv = reinterpret_cast<const CT*>(panel_inv(const_cast<void*>(work), k0));
+    hipLaunchKernelGGL(k_solve_apply<T>, dim3(gx), dim3(256), 0, st, nrhs, inv, k0, bw,
+                       (const int32_t*)nullptr, X, ldx, 0);
     if (k0 > 0) {
       *herr = launch_gemm_sub<T>(k0, nrhs, bw, A + k0, lda, X + k0 * ldx, ldx, X, ldx, st);
       if (*herr != hipSuccess) return MW_EHIP;
@@ -654,10 +794,10 @@ template int mw_getrf_t<double>(int64_t, void*, int64_t, int32_t*, void*, int32_
                                 const volatile int32_t*, hipStream_t, hipError_t*);
 template int mw_getrf_t<float>(int64_t, void*, int64_t, int32_t*, void*, int32_t*,
                                const volatile int32_t*, hipStream_t, hipError_t*);
-template int mw_getrs_t<double>(int64_t, int64_t, const void*, int64_t, const int32_t*, void*,
-                                int64_t, hipStream_t, hipError_t*);
-template int mw_getrs_t<float>(int64_t, int64_t, const void*, int64_t, const int32_t*, void*,
-                               int64_t, hipStream_t, hipError_t*);
+template int mw_getrs_t<double>(int64_t, int64_t, const void*, int64_t, const int32_t*,
+                                const void*, void*, int64_t, hipStream_t, hipError_t*);
+template int mw_getrs_t<float>(int64_t, int64_t, const void*, int64_t, const int32_t*,
+                               const void*, void*, int64_t, hipStream_t, hipError_t*);
 
 hipError_t mw_gemm_sub_z(int64_t m, int64_t n, int64_t k, const void* A, int64_t lda,
                          const void* B, int64_t ldb, void* C, int64_t ldc, hipStream_t st) {

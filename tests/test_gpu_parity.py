@@ -60,7 +60,7 @@ def _lu_solve(M, RHS, dtype="z"):
     info = torch.zeros(1, dtype=torch.int32, device="cuda")
     f, s = (lib.mw_zgetrf, lib.mw_zgetrs) if dtype == "z" else (lib.mw_cgetrf, lib.mw_cgetrs)
     _lib.check(f(n, dM.data_ptr(), n, ipiv.data_ptr(), work.data_ptr(), info.data_ptr(), None, _stream()), "getrf")
-    _lib.check(s(n, p, dM.data_ptr(), n, ipiv.data_ptr(), dX.data_ptr(), p, _stream()), "getrs")
+    _lib.check(s(n, p, dM.data_ptr(), n, ipiv.data_ptr(), work.data_ptr(), dX.data_ptr(), p, _stream()), "getrs")
     torch.cuda.synchronize()
     return dX.cpu().numpy().astype(np.complex128), int(info.cpu()[0])
 
@@ -101,6 +101,23 @@ def test_lu_singular_reports_info():
     M[:, :40] = np.random.default_rng(1).standard_normal((n, 40))
     _X, info = _lu_solve(M, np.ones((n, 1), dtype=complex))
     assert info > 0
+
+
+def test_lu_nan_input_completes():
+    """Regression: NaN magnitudes in a panel must still yield an in-range pivot
+    row (a NaN-only column once left the pivot search without a candidate)."""
+    _say("lu nan input")
+    n = 150
+    rng = np.random.default_rng(5)
+    M = rng.standard_normal((n, n)) + 0j
+    M[40:, 33:] = np.nan
+    X, _info = _lu_solve(M, np.ones((n, 2), dtype=complex))
+    assert X.shape == (n, 2)
+    # the library is still healthy afterwards
+    M2 = rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))
+    R2 = rng.standard_normal((n, 1)) + 0j
+    X2, info2 = _lu_solve(M2, R2)
+    assert info2 == 0 and rel_err(M2 @ X2, R2) < 1e-10
 
 
 def test_lu_solve_fp32():
@@ -169,3 +186,32 @@ def test_c2_array_8x8_matches_reference_golden():
     assert rel_err(sol.coeffs[:, g["coeffs_cols"]], g["coeffs"]) < 1e-10
     assert rel_err(sol.coeffs.sum(axis=1), g["coeffs_colsum"]) < 1e-10
     assert rel_err(sol.y, sol.y.T) < 1e-12  # reciprocity
+
+
+def test_graph_replay_matches_eager():
+    """The captured solve replays to the eager answer, replay after replay,
+    with eager work and host synchronisations in between."""
+    _say("graph replay")
+    s = BSplineSolver(**W.dipole_array(2, 3, 17, seed=7))
+    Y0, W0 = s._solve_device()
+    y0, w0 = Y0.cpu().numpy().copy(), W0.cpu().numpy().copy()
+    for rep in range(4):
+        Yg, Wg = s.solve_graph()
+        torch.cuda.synchronize()
+        assert rel_err(Yg.cpu().numpy(), y0) < 1e-14, rep
+        assert rel_err(Wg.cpu().numpy(), w0) < 1e-14, rep
+        s._solve_device()  # eager work between replays
+        torch.cuda.synchronize()
+
+
+def test_cancel_token_stops_factorisation():
+    from momwire_amd.bspline import CancelToken, SolveAborted
+    _say("cancel")
+    tok = CancelToken()
+    s = BSplineSolver(**W.dipole_array(2, 2, 11, seed=5), cancel=tok)
+    tok.cancel()
+    with pytest.raises(SolveAborted):
+        s.compute_port_solution()
+    tok.reset()
+    g = golden("array2x2")
+    assert rel_err(s.compute_port_solution().y, g["y"]) < 1e-10
