@@ -101,8 +101,9 @@ int mw_fill_z(const mw_mesh* mesh, double k, double omega, double eps, double mu
               int32_t symmetric, void* stream);
 
 /*
- * LU factorisation with row pivoting (tournament pivot selection per panel),
- * in place: A = P L U, L unit lower.  Replaces LAPACK zgetrf/cgetrf behind
+ * LU factorisation with row pivoting (threshold pivoting per 32-column panel,
+ * tournament partial pivoting where the threshold fails), in place:
+ * A = P L U, L unit lower.  Replaces LAPACK zgetrf/cgetrf behind
  * scipy.linalg.solve (bspline.py:2971-2989).  ipiv: [n] device int32,
  * LAPACK-style sequential row interchanges (0-based).  work: device scratch of
  * mw_getrf_workspace_bytes(n) bytes.  info: device int32, set to 1 + the
@@ -110,6 +111,15 @@ int mw_fill_z(const mw_mesh* mesh, double k, double omega, double eps, double mu
  * panels (may be NULL); a set flag stops the factorisation with MW_EABORT.
  */
 int64_t mw_getrf_workspace_bytes(int64_t n);
+
+/*
+ * Threshold pivoting: a panel keeps its diagonal pivots when every multiplier
+ * of the panel satisfies |re| + |im| <= bound (checked on the GPU); otherwise
+ * the panel is refactored with tournament partial pivoting.  bound <= 0
+ * pivots every panel.  Default 8.  Process-wide setting.
+ */
+int mw_set_pivot_threshold(double bound);
+double mw_get_pivot_threshold(void);
 int mw_zgetrf(int64_t n, void* A, int64_t lda, int32_t* ipiv, void* work,
               int32_t* info, const volatile int32_t* cancel, void* stream);
 int mw_cgetrf(int64_t n, void* A, int64_t lda, int32_t* ipiv, void* work,

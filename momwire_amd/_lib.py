@@ -68,6 +68,8 @@ SIGNATURES = {
     "mw_fill_z": (ctypes.c_int, [ctypes.POINTER(MwMesh), _D, _D, _D, _D, _P, _I64, _P, _I32,
                                  _I32, _P]),
     "mw_getrf_workspace_bytes": (_I64, [_I64]),
+    "mw_set_pivot_threshold": (ctypes.c_int, [_D]),
+    "mw_get_pivot_threshold": (_D, []),
     "mw_zgetrf": (ctypes.c_int, [_I64, _P, _I64, _P, _P, _P, _P, _P]),
     "mw_cgetrf": (ctypes.c_int, [_I64, _P, _I64, _P, _P, _P, _P, _P]),
     "mw_zgetrs": (ctypes.c_int, [_I64, _I64, _P, _I64, _P, _P, _P, _I64, _P]),

@@ -24,6 +24,8 @@ hipError_t mw_launch_sparse_rows(int64_t nrows, int maxnz, const int32_t* idx, c
                                  int64_t ldo, hipStream_t st);
 hipError_t mw_launch_scatter_cols(int64_t n, int64_t ncols, int maxnz, const int32_t* idx,
                                   const double* val, void* X, int64_t ldx, hipStream_t st);
+void mw_set_pivot_bound(double lbound);
+double mw_get_pivot_bound();
 hipError_t mw_launch_small_solve(int ns, int nt, void* S, int64_t lds, void* T, int64_t ldt,
                                  int32_t* info, hipStream_t st);
 
@@ -83,6 +85,14 @@ int mw_fill_z(const mw_mesh* mesh, double k, double omega, double eps, double mu
 }
 
 int64_t mw_getrf_workspace_bytes(int64_t n) { return n > 0 ? mw_getrf_ws_bytes(n) : 0; }
+
+int mw_set_pivot_threshold(double bound) {
+  g_err[0] = 0;
+  if (!(bound == bound)) return fail(MW_EINVAL, "%spivot threshold is NaN");
+  mw_set_pivot_bound(bound);
+  return MW_OK;
+}
+double mw_get_pivot_threshold(void) { return mw_get_pivot_bound(); }
 
 static int getrf_common(bool dbl, int64_t n, void* A, int64_t lda, int32_t* ipiv, void* work,
                         int32_t* info, const volatile int32_t* cancel, void* stream) {

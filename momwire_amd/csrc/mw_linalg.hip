@@ -12,7 +12,6 @@ typedef double v4d __attribute__((ext_vector_type(4)));
 typedef float v4f __attribute__((ext_vector_type(4)));
 
 constexpr int NB = 32;   // LU panel width
-constexpr int CH = 64;   // rows per tournament leaf (= 2 NB: a merge node's size)
 
 // ---------------------------------------------------------------------------
 // Complex GEMM  C <- C - A B  on MFMA.
@@ -306,13 +305,240 @@ MW_DEV void load_block(CT (&S)[RPL][8], const CT* A, int64_t lda, int64_t k0, in
   }
 }
 
-// Root: write L11\U11 of the pivot rows in pivot order and the LAPACK-style
-// interchange sequence.
+// Inverses of the unit-lower and upper factors held in M (L\U, padded to NB
+// with the identity): Li = L^{-1}, Ui = U^{-1}.  Row recurrences, one barrier
+// per row; threads [0, NB) build Li top-down, [NB, 2 NB) build Ui bottom-up.
 template <typename CT>
-MW_DEV void finish_root(const CT (&S)[RPL][8], int rows, int bw, TourLds<CT>& L, CT* lu11,
-                        int32_t* ipiv, int64_t k0, int32_t* info) {
-  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+MW_DEV void tri_inverses(const CT (*M)[NB + 1], CT (*Li)[NB + 1], CT (*Ui)[NB + 1]) {
+  const int tid = threadIdx.x;
+  for (int e = tid; e < NB * NB; e += blockDim.x) {
+    const int i = e / NB, j = e % NB;
+    CT v;
+    v.x = (i == j) ? 1 : 0; v.y = 0;
+    Li[i][j] = v;
+    v.x = 0;
+    Ui[i][j] = v;
+  }
+  __syncthreads();
+  for (int s = 0; s < NB; ++s) {
+    if (tid < NB) {
+      const int i = s, j = tid;  // Li[i][j] = -sum_{l=j}^{i-1} M[i][l] Li[l][j]
+      if (j < i) {
+        CT v;
+        v.x = 0; v.y = 0;
+        for (int l = j; l < i; ++l) v = cfms(v, M[i][l], Li[l][j]);
+        Li[i][j] = v;
+      }
+    } else if (tid < 2 * NB) {
+      const int i = NB - 1 - s, j = tid - NB;  // Ui[i][j], j >= i
+      if (j >= i) {
+        CT v;
+        v.x = (i == j) ? 1 : 0; v.y = 0;
+        for (int l = i + 1; l <= j; ++l) v = cfms(v, M[i][l], Ui[l][j]);
+        // an exactly zero pivot (reported through info) maps to a zero
+        // inverse entry rather than inf/NaN
+        if (M[i][i].x != 0 || M[i][i].y != 0) v = cdiv(v, M[i][i]);
+        else { v.x = 0; v.y = 0; }
+        Ui[i][j] = v;
+      }
+    }
+    __syncthreads();
+  }
+}
+
+template <typename CT>
+struct PanelLds {
+  CT M[NB][NB + 1];   // L11\U11 (rows workgroups reuse M and Li as a 64-row buffer)
+  CT Li[NB][NB + 1];
+  CT Ui[NB][NB + 1];
+  int bad;
+  int piv_s;
+};
+
+// Write L11\U11 (from M) and the two inverses for panel use and for the solve.
+template <typename CT>
+MW_DEV void store_factors(const PanelLds<CT>& P, int bw, CT* lu11, CT* inv) {
+  const int tid = threadIdx.x;
+  for (int e = tid; e < NB * NB; e += blockDim.x) {
+    const int i = e / NB, j = e % NB;
+    lu11[e] = P.M[i][j];
+    inv[e] = P.Li[i][j];
+    inv[NB * NB + e] = P.Ui[i][j];
+  }
+  (void)bw;
+}
+
+// Fast path of a panel: factor A11 without row exchanges (every workgroup
+// redundantly, in LDS), invert its factors, and check that every multiplier
+// of the panel -- those of A11 and L21 = A21 U11^{-1} -- stays within lbound
+// (threshold pivoting with tau = 1/lbound).  Nothing in A is written; a
+// violated bound (or a zero pivot) sets flag for the tournament fallback.
+// Block 0 stores L11\U11, the inverses and identity interchanges.
+template <typename T>
+__global__ __launch_bounds__(256) void k_panel_nopiv(int64_t n, const typename cplx_t<T>::type* A,
+                                                     int64_t lda, int64_t k0, int bw,
+                                                     typename cplx_t<T>::type* lu11,
+                                                     typename cplx_t<T>::type* inv,
+                                                     int32_t* ipiv, int32_t* flag, double lbound) {
+  typedef typename cplx_t<T>::type CT;
+  __shared__ PanelLds<CT> P;
+  const int tid = threadIdx.x;
+  for (int e = tid; e < NB * NB; e += blockDim.x) {
+    const int i = e / NB, j = e % NB;
+    CT v;
+    v.x = (i == j) ? 1 : 0; v.y = 0;
+    if (i < bw && j < bw) v = A[(k0 + i) * lda + k0 + j];
+    P.M[i][j] = v;
+  }
+  if (tid == 0) P.bad = (lbound <= 0.0) ? 1 : 0;
+  __syncthreads();
+  for (int c = 0; c < NB - 1; ++c) {
+    const CT d = P.M[c][c];
+    const bool zero = (d.x == 0 && d.y == 0);
+    if (zero) {
+      if (tid == 0) P.bad = 1;
+    } else if (tid > c && tid < NB) {
+      const CT l = cmul(P.M[tid][c], crecip(d));
+      if (!(cabs1(l) <= lbound)) P.bad = 1;
+      P.M[tid][c] = l;
+    }
+    __syncthreads();
+    if (!zero) {
+      const int w = NB - 1 - c;
+      for (int e = tid; e < w * w; e += blockDim.x) {
+        const int i = c + 1 + e / w, j = c + 1 + e % w;
+        P.M[i][j] = cfms(P.M[i][j], P.M[i][c], P.M[c][j]);
+      }
+    }
+    __syncthreads();
+  }
+  if (tid == 0 && P.M[NB - 1][NB - 1].x == 0 && P.M[NB - 1][NB - 1].y == 0) P.bad = 1;
+  tri_inverses(P.M, P.Li, P.Ui);
+  const int b = blockIdx.x;
+  if (b == 0) {
+    store_factors(P, bw, lu11, inv);
+    if (tid < bw) ipiv[k0 + tid] = (int32_t)(k0 + tid);
+    if (tid == 0 && P.bad) atomicOr(flag, 1);
+    return;
+  }
+  if (P.bad) return;
+  // rows of L21 handled by this block: check only
+  CT (*B)[NB + 1] = reinterpret_cast<CT (*)[NB + 1]>(&P.M[0][0]);  // 64 rows (M and Li)
+  const int64_t r0 = k0 + bw + (int64_t)(b - 1) * 64;
+  __syncthreads();
+  for (int e = tid; e < 64 * NB; e += blockDim.x) {
+    const int rr = e / NB, l = e % NB;
+    CT v;
+    v.x = 0; v.y = 0;
+    if (r0 + rr < n && l < bw) v = A[(r0 + rr) * lda + k0 + l];
+    B[rr][l] = v;
+  }
+  __syncthreads();
+  const int rr = tid / 4, j0 = (tid % 4) * 8;
+  CT acc[8];
+#pragma unroll
+  for (int q = 0; q < 8; ++q) { acc[q].x = 0; acc[q].y = 0; }
+#pragma unroll 4
+  for (int l = 0; l < NB; ++l) {
+    const CT a = B[rr][l];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      const CT t = P.Ui[l][j0 + q];
+      acc[q].x += a.x * t.x - a.y * t.y;
+      acc[q].y += a.x * t.y + a.y * t.x;
+    }
+  }
+  bool over = false;
+#pragma unroll
+  for (int q = 0; q < 8; ++q) over |= !(cabs1(acc[q]) <= lbound);
+  if (over && r0 + rr < n) atomicOr(flag, 1);
+}
+
+// Tournament fallback for a panel whose diagonal failed the threshold test:
+// one launch.  Leaves factor 128-row chunks with partial pivoting in
+// registers; the last of up to four siblings to finish (device-scope arrival
+// counter) merges their nominees' original rows, and so on up to the root,
+// which writes L11\U11, the inverses and the interchanges, and applies the
+// interchanges to columns [k0, n).  Exits at once when the flag is clear.
+struct TourWs {
+  int32_t* cand;     // [nodes][NB]
+  int32_t* ncand;    // [nodes]
+  int32_t* counter;  // [nodes], zero on entry, reset by the root
+};
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_panel_pivot(typename cplx_t<T>::type* A, int64_t lda,
+                                                     int64_t n, int64_t k0, int bw, int nleaf,
+                                                     TourWs ws, const int32_t* flag,
+                                                     typename cplx_t<T>::type* lu11,
+                                                     typename cplx_t<T>::type* inv,
+                                                     int32_t* ipiv, int32_t* info) {
+  typedef typename cplx_t<T>::type CT;
+  if (*flag == 0) return;
+  __shared__ TourLds<CT> L;
+  __shared__ PanelLds<CT> P;
+  __shared__ int last_s;
+  const int tid = threadIdx.x;
+  int node = blockIdx.x, nodes = nleaf, lvl_off = 0;
+  if (tid == 0) {
+    const int64_t r0 = k0 + (int64_t)node * TROWS;
+    const int64_t left = n - r0;
+    const int rows = (int)(left < TROWS ? left : TROWS);
+    for (int r = 0; r < rows; ++r) L.rid[r] = (int32_t)(r0 + r);
+    L.rows = rows;
+  }
+  __syncthreads();
+  CT S[RPL][8];
+  while (true) {
+    const int rows = L.rows;
+    load_block(S, A, lda, k0, bw, L, rows);
+    reg_gepp(S, rows, bw, L, nodes == 1 ? info : (int32_t*)nullptr, k0);
+    __syncthreads();
+    const int nom = rows < bw ? rows : bw;
+    if (nodes == 1) break;
+    if (tid < nom) ws.cand[(size_t)(lvl_off + node) * NB + tid] = L.rid[L.piv[tid]];
+    if (tid == 0) ws.ncand[lvl_off + node] = nom;
+    __threadfence();
+    __syncthreads();
+    const int parent = node / FANIN;
+    const int pnodes = (nodes + FANIN - 1) / FANIN;
+    const int nchild = (parent * FANIN + FANIN <= nodes) ? FANIN : nodes - parent * FANIN;
+    const int plvl_off = lvl_off + nodes;
+    if (tid == 0) {
+      const int old = __hip_atomic_fetch_add(&ws.counter[plvl_off + parent], 1, __ATOMIC_ACQ_REL,
+                                             __HIP_MEMORY_SCOPE_AGENT);
+      last_s = (old == nchild - 1);
+    }
+    __syncthreads();
+    if (!last_s) return;
+    __threadfence();
+    if (tid == 0) {
+      int cnt = 0;
+      for (int ch = 0; ch < nchild; ++ch) {
+        const int cn = parent * FANIN + ch;
+        const int nc = __hip_atomic_load(&ws.ncand[lvl_off + cn], __ATOMIC_ACQUIRE,
+                                         __HIP_MEMORY_SCOPE_AGENT);
+        for (int t = 0; t < nc; ++t)
+          L.rid[cnt++] = __hip_atomic_load(&ws.cand[(size_t)(lvl_off + cn) * NB + t],
+                                           __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+      }
+      L.rows = cnt;
+    }
+    __syncthreads();
+    node = parent;
+    lvl_off = plvl_off;
+    nodes = pnodes;
+  }
+  // Root.  P.M <- L11\U11 of the pivot rows in pivot order (identity-padded).
+  const int rows = L.rows;
   const int steps = rows < bw ? rows : bw;
+  const int lane = tid & 63, w = tid >> 6;
+  for (int e = tid; e < NB * NB; e += blockDim.x) {
+    CT v;
+    v.x = (e / NB == e % NB) ? 1 : 0; v.y = 0;
+    P.M[e / NB][e % NB] = v;
+  }
+  __syncthreads();
 #pragma unroll
   for (int t = 0; t < RPL; ++t) {
     const int r = lane + 64 * t;
@@ -324,7 +550,7 @@ MW_DEV void finish_root(const CT (&S)[RPL][8], int rows, int bw, TourLds<CT>& L,
         const int i = L.rank[r];
 #pragma unroll
         for (int j = 0; j < 8; ++j)
-          if (8 * w + j < bw) lu11[i * NB + 8 * w + j] = S[t][j];
+          if (8 * w + j < bw) P.M[i][8 * w + j] = S[t][j];
       }
     }
   }
@@ -351,121 +577,27 @@ MW_DEV void finish_root(const CT (&S)[RPL][8], int rows, int bw, TourLds<CT>& L,
         val_s[t2] = id1;
       }
     }
-    if (steps < bw && info && *info == 0) *info = (int32_t)(k0 + steps + 1);
+    if (steps < bw && *info == 0) *info = (int32_t)(k0 + steps + 1);
   }
-}
-
-// level < 0: leaf over the panel rows; otherwise merge nominee sets of the
-// previous level.  is_root: this launch has one block and finishes the panel.
-template <typename T>
-__global__ __launch_bounds__(256) void k_tournament(const typename cplx_t<T>::type* A, int64_t lda,
-                                                    int64_t n, int64_t k0, int bw, int leaf,
-                                                    const int32_t* in_cand, const int32_t* in_n,
-                                                    int n_in, int32_t* out_cand, int32_t* out_n,
-                                                    int is_root, void* lu11, int32_t* ipiv,
-                                                    int32_t* info) {
-  typedef typename cplx_t<T>::type CT;
-  __shared__ TourLds<CT> L;
-  const int tid = threadIdx.x;
-  const int b = blockIdx.x;
-  if (tid == 0) {
-    int rows = 0;
-    if (leaf) {
-      const int64_t r0 = k0 + (int64_t)b * TROWS;
-      const int64_t left = n - r0;
-      rows = (int)(left < TROWS ? left : TROWS);
-      for (int r = 0; r < rows; ++r) L.rid[r] = (int32_t)(r0 + r);
-    } else {
-      for (int s = 0; s < FANIN; ++s) {
-        const int src = b * FANIN + s;
-        if (src >= n_in) break;
-        const int cnt = in_n[src];
-        for (int t = 0; t < cnt; ++t) L.rid[rows++] = in_cand[src * NB + t];
+  __syncthreads();
+  tri_inverses(P.M, P.Li, P.Ui);
+  store_factors(P, bw, lu11, inv);
+  // reset the arrival counters for the next fallback
+  int total = 0;
+  for (int nd = nleaf; nd > 1; nd = (nd + FANIN - 1) / FANIN) total += nd;
+  total += 1;
+  for (int e = tid; e < total; e += blockDim.x) ws.counter[e] = 0;
+  // interchanges on columns [k0, n): nobody else reads A any more
+  __syncthreads();
+  for (int64_t c = k0 + tid; c < n; c += blockDim.x) {
+    for (int i = 0; i < bw; ++i) {
+      const int64_t p1 = k0 + i, p2 = ipiv[k0 + i];
+      if (p2 != p1) {
+        const CT t = A[p1 * lda + c];
+        A[p1 * lda + c] = A[p2 * lda + c];
+        A[p2 * lda + c] = t;
       }
     }
-    L.rows = rows;
-  }
-  __syncthreads();
-  const int rows = L.rows;
-  CT S[RPL][8];
-  load_block(S, A, lda, k0, bw, L, rows);
-  reg_gepp(S, rows, bw, L, is_root ? info : (int32_t*)nullptr, k0);
-  __syncthreads();
-  const int steps = rows < bw ? rows : bw;
-  if (is_root) {
-    finish_root(S, rows, bw, L, reinterpret_cast<CT*>(lu11), ipiv, k0, info);
-  } else {
-    if (tid < steps) out_cand[b * NB + tid] = L.rid[L.piv[tid]];
-    if (tid == 0) out_n[b] = steps;
-  }
-}
-
-// Row interchanges ipiv[k0 .. k0+bw) on columns [c0, c1).
-template <typename T>
-__global__ void k_laswp(typename cplx_t<T>::type* A, int64_t lda, int64_t k0, int bw,
-                        const int32_t* ipiv, int64_t c0, int64_t c1) {
-  typedef typename cplx_t<T>::type CT;
-  const int64_t c = c0 + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= c1) return;
-  for (int i = 0; i < bw; ++i) {
-    const int64_t p1 = k0 + i, p2 = ipiv[k0 + i];
-    if (p2 != p1) {
-      const CT t = A[p1 * lda + c];
-      A[p1 * lda + c] = A[p2 * lda + c];
-      A[p2 * lda + c] = t;
-    }
-  }
-}
-
-// Diagonal blocks are inverted once per panel (L11^{-1}, U11^{-1}, padded to
-// NB with the identity) and applied as small LDS-tiled products: the usual
-// GPU substitute for a triangular solve with many right-hand sides.
-template <typename T>
-__global__ __launch_bounds__(64) void k_tri_inv(const typename cplx_t<T>::type* lu, int64_t ldl,
-                                                int bw, typename cplx_t<T>::type* out) {
-  typedef typename cplx_t<T>::type CT;
-  __shared__ CT M[NB][NB + 1], Li[NB][NB + 1], Ui[NB][NB + 1];
-  const int tid = threadIdx.x;
-  for (int e = tid; e < NB * NB; e += blockDim.x) {
-    const int i = e / NB, j = e % NB;
-    CT v;
-    v.x = (i == j) ? 1 : 0; v.y = 0;
-    if (i < bw && j < bw) v = lu[(int64_t)i * ldl + j];
-    M[i][j] = v;
-  }
-  __syncthreads();
-  if (tid < NB) {
-    // column j of L^{-1}: L x = e_j, x_i = -sum_{l=j}^{i-1} L[i][l] x_l
-    const int j = tid;
-    for (int i = 0; i < NB; ++i) {
-      CT v;
-      v.x = (i == j) ? 1 : 0; v.y = 0;
-      if (i > j) {
-        for (int l = j; l < i; ++l) v = cfms(v, M[i][l], Li[l][j]);
-      }
-      Li[i][j] = v;
-    }
-  } else if (tid < 2 * NB) {
-    // column j of U^{-1}: U x = e_j, back substitution
-    const int j = tid - NB;
-    for (int i = NB - 1; i >= 0; --i) {
-      CT v;
-      v.x = 0; v.y = 0;
-      if (i <= j) {
-        if (i == j) v.x = 1;
-        for (int l = i + 1; l <= j; ++l) v = cfms(v, M[i][l], Ui[l][j]);
-        // an exactly zero pivot (already reported through info) maps to a
-        // zero inverse entry rather than inf/NaN
-        if (M[i][i].x != 0 || M[i][i].y != 0) v = cdiv(v, M[i][i]);
-        else { v.x = 0; v.y = 0; }
-      }
-      Ui[i][j] = v;
-    }
-  }
-  __syncthreads();
-  for (int e = tid; e < NB * NB; e += blockDim.x) {
-    out[e] = Li[e / NB][e % NB];
-    out[NB * NB + e] = Ui[e / NB][e % NB];
   }
 }
 
@@ -693,11 +825,18 @@ __global__ __launch_bounds__(256) void k_small_solve(int ns, int nt, double2* S,
 // Host drivers (called from mw_capi.hip)
 // ---------------------------------------------------------------------------
 // workspace: lu11 [NB*NB complex double] | diagonal-block inverses
-// [n_panels][2][NB*NB complex double] | cand ping/pong [2][nleaf*NB] | counts [2][nleaf]
+// [n_panels][2][NB*NB complex double] | flags [n_panels] | counters [nodes] |
+// ncand [nodes] | cand [nodes][NB]
+static int64_t tour_nodes(int64_t n) {
+  int64_t nodes = 0;
+  for (int64_t nd = (n + TROWS - 1) / TROWS; nd > 1; nd = (nd + FANIN - 1) / FANIN) nodes += nd;
+  return nodes + 1;
+}
+
 int64_t mw_getrf_ws_bytes(int64_t n) {
-  const int64_t nleaf = (n + TROWS - 1) / TROWS;
   const int64_t npan = (n + NB - 1) / NB;
-  const int64_t bytes = (int64_t)NB * NB * 16 * (1 + 2 * npan) + 2 * nleaf * NB * 4 + 2 * nleaf * 4;
+  const int64_t nodes = tour_nodes(n);
+  const int64_t bytes = (int64_t)NB * NB * 16 * (1 + 2 * npan) + 4 * npan + 4 * nodes * (2 + NB);
   return (bytes + 255) / 256 * 256;
 }
 
@@ -705,43 +844,39 @@ static void* panel_inv(void* work, int64_t k0) {
   return reinterpret_cast<char*>(work) + (int64_t)NB * NB * 16 * (1 + 2 * (k0 / NB));
 }
 
+static double g_lbound = 8.0;  // threshold-pivoting multiplier bound (1/tau); <= 0: always pivot
+
 template <typename T>
 int mw_getrf_t(int64_t n, void* Av, int64_t lda, int32_t* ipiv, void* work, int32_t* info,
                const volatile int32_t* cancel, hipStream_t st, hipError_t* herr) {
   typedef typename cplx_t<T>::type CT;
   CT* A = reinterpret_cast<CT*>(Av);
-  const int64_t nleaf_max = (n + TROWS - 1) / TROWS;
-  char* wp = reinterpret_cast<char*>(work);
-  void* lu11 = wp;
   const int64_t npan = (n + NB - 1) / NB;
-  char* cbase = wp + (int64_t)NB * NB * 16 * (1 + 2 * npan);
-  int32_t* cand[2] = {(int32_t*)cbase, (int32_t*)cbase + nleaf_max * NB};
-  int32_t* cnt[2] = {cand[1] + nleaf_max * NB, cand[1] + nleaf_max * NB + nleaf_max};
+  const int64_t nodes = tour_nodes(n);
+  char* wp = reinterpret_cast<char*>(work);
+  CT* lu11 = reinterpret_cast<CT*>(wp);
+  int32_t* flags = reinterpret_cast<int32_t*>(wp + (int64_t)NB * NB * 16 * (1 + 2 * npan));
+  TourWs tw;
+  tw.counter = flags + npan;
+  tw.ncand = tw.counter + nodes;
+  tw.cand = tw.ncand + nodes;
+  const int64_t nzero = npan + nodes;
+  hipLaunchKernelGGL(k_zero_i32, dim3((unsigned)((nzero + 255) / 256)), dim3(256), 0, st, flags,
+                     nzero);
   hipLaunchKernelGGL(k_zero_i32, dim3(1), dim3(64), 0, st, info, (int64_t)1);
+  const double lbound = g_lbound;
   for (int64_t k0 = 0; k0 < n; k0 += NB) {
     if (cancel && *cancel) return MW_EABORT;
     const int bw = (int)((n - k0) < NB ? (n - k0) : NB);
-    int nodes = (int)((n - k0 + TROWS - 1) / TROWS);
-    int cur = 0;
-    hipLaunchKernelGGL(k_tournament<T>, dim3(nodes), dim3(256), 0, st, A, lda, n, k0, bw, 1,
-                       (const int32_t*)nullptr, (const int32_t*)nullptr, 0, cand[0], cnt[0],
-                       nodes == 1 ? 1 : 0, lu11, ipiv, info);
-    while (nodes > 1) {
-      const int out = (nodes + FANIN - 1) / FANIN;
-      hipLaunchKernelGGL(k_tournament<T>, dim3(out), dim3(256), 0, st, A, lda, n, k0, bw, 0,
-                         cand[cur], cnt[cur], nodes, cand[cur ^ 1], cnt[cur ^ 1],
-                         out == 1 ? 1 : 0, lu11, ipiv, info);
-      cur ^= 1;
-      nodes = out;
-    }
-    const int64_t ncol = n - k0;
-    hipLaunchKernelGGL(k_laswp<T>, dim3((unsigned)((ncol + 255) / 256)), dim3(256), 0, st, A, lda,
-                       k0, bw, ipiv, k0, n);
     const int64_t rest = n - k0 - bw;
     const int nrb = (int)((rest + AW - 1) / AW);
     CT* inv = reinterpret_cast<CT*>(panel_inv(work, k0));
-    hipLaunchKernelGGL(k_tri_inv<T>, dim3(1), dim3(64), 0, st, (const CT*)lu11, (int64_t)NB, bw,
-                       inv);
+    int32_t* flag = flags + k0 / NB;
+    hipLaunchKernelGGL(k_panel_nopiv<T>, dim3(nrb + 1), dim3(256), 0, st, n, (const CT*)A, lda, k0,
+                       bw, lu11, inv, ipiv, flag, lbound);
+    const int nleaf = (int)((n - k0 + TROWS - 1) / TROWS);
+    hipLaunchKernelGGL(k_panel_pivot<T>, dim3(nleaf), dim3(256), 0, st, A, lda, n, k0, bw, nleaf,
+                       tw, (const int32_t*)flag, lu11, inv, ipiv, info);
     hipLaunchKernelGGL(k_panel_apply<T>, dim3(2 * nrb + 1), dim3(256), 0, st, n, A, lda, k0, bw,
                        (const CT*)inv, (const CT*)lu11, nrb, nrb);
     if (rest > 0) {
@@ -755,6 +890,9 @@ int mw_getrf_t(int64_t n, void* Av, int64_t lda, int32_t* ipiv, void* work, int3
   }
   return MW_OK;
 }
+
+void mw_set_pivot_bound(double lbound) { g_lbound = lbound; }
+double mw_get_pivot_bound() { return g_lbound; }
 
 template <typename T>
 int mw_getrs_t(int64_t n, int64_t nrhs, const void* Av, int64_t lda, const int32_t* ipiv,

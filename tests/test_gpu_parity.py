@@ -120,6 +120,28 @@ def test_lu_nan_input_completes():
     assert info2 == 0 and rel_err(M2 @ X2, R2) < 1e-10
 
 
+@pytest.mark.parametrize("name,kw", [("array2x2", W.dipole_array(2, 2, 11, seed=5)),
+                                     ("tee_jport", W.junction_tee("junction_port"))])
+def test_threshold_and_full_pivoting_agree(name, kw):
+    """Threshold pivoting (diagonal kept while multipliers stay <= 8) and
+    tournament partial pivoting on every panel give the same port solution."""
+    _say("pivot modes", name)
+    lib = _lib.load()
+    g = golden(name)
+    default = lib.mw_get_pivot_threshold()
+    try:
+        ys = []
+        for bound in (default, 0.0):
+            _lib.check(lib.mw_set_pivot_threshold(bound), "threshold")
+            sol = BSplineSolver(**kw).compute_port_solution()
+            assert rel_err(sol.y, g["y"]) < 1e-10
+            assert rel_err(sol.coeffs, g["coeffs"]) < 1e-10
+            ys.append(sol.y)
+        assert rel_err(ys[0], ys[1]) < 1e-12
+    finally:
+        lib.mw_set_pivot_threshold(default)
+
+
 def test_lu_solve_fp32():
     _say("lu fp32")
     n = 700

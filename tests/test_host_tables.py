@@ -113,7 +113,7 @@ def test_c_abi_exports_every_declared_symbol():
     if not os.path.exists(_lib.LIB_PATH):
         pytest.skip("libmomwire_hip.so not built")
     hdr = open(os.path.join(ROOT, "include", "momwire_hip.h")).read()
-    names = set(re.findall(r"^\s*(?:int|int64_t|const char\*)\s+(mw_\w+)\s*\(", hdr, re.M))
+    names = set(re.findall(r"^\s*(?:int|int64_t|double|const char\*)\s+(mw_\w+)\s*\(", hdr, re.M))
     assert names, "no declarations parsed"
     lib = ctypes.CDLL(_lib.LIB_PATH)
     for nm in names:
