@@ -138,6 +138,22 @@ int mw_cgetrs(int64_t n, int64_t nrhs, const void* A, int64_t lda,
               const int32_t* ipiv, const void* work, void* X, int64_t ldx,
               void* stream);
 
+/*
+ * Factor and solve in one pass (LAPACK ?gesv, which scipy.linalg.solve calls
+ * in bspline.py:2971-2989): A is n x (n + nrhs), row stride lda >= n + nrhs;
+ * columns [n, n + nrhs) hold the right-hand sides on entry and the solution
+ * on return.  The right-hand sides ride through the factorisation as extra
+ * columns, so only the backward substitution runs afterwards.  ipiv, work,
+ * info and cancel as for mw_?getrf; the factors stay in A[:, :n] and work
+ * for later mw_?getrs calls.
+ */
+int mw_zgesv(int64_t n, int64_t nrhs, void* A, int64_t lda, int32_t* ipiv,
+             void* work, int32_t* info, const volatile int32_t* cancel,
+             void* stream);
+int mw_cgesv(int64_t n, int64_t nrhs, void* A, int64_t lda, int32_t* ipiv,
+             void* work, int32_t* info, const volatile int32_t* cancel,
+             void* stream);
+
 /* C <- C - A B (complex, MFMA).  A: m x k, B: k x n, C: m x n. */
 int mw_zgemm_sub(int64_t m, int64_t n, int64_t k, const void* A, int64_t lda,
                  const void* B, int64_t ldb, void* C, int64_t ldc, void* stream);

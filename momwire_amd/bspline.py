@@ -106,8 +106,11 @@ class _Device:
         torch = self.torch
         if self.Z is None:
             nb = self.nb
-            self.Z = torch.empty((nb, nb), dtype=torch.complex128, device=self.dev)
-            self.X = torch.empty((nb, max(self.ncols, 1)), dtype=torch.complex128, device=self.dev)
+            # [Z | B]: the right-hand sides ride through the factorisation as
+            # extra columns (mw_zgesv), so X is a column block of the same buffer
+            self.ld = nb + self.ncols
+            self.Z = torch.empty((nb, self.ld), dtype=torch.complex128, device=self.dev)
+            self.X = self.Z[:, nb:]
             self.ipiv = torch.empty(nb, dtype=torch.int32, device=self.dev)
             lib = _lib.load()
             self.work = torch.empty(int(lib.mw_getrf_workspace_bytes(nb)), dtype=torch.uint8,
@@ -368,22 +371,20 @@ class BSplineSolver(_SweptPortSolutions):
         mark("fill", True)
         _lib.check(lib.mw_same_edge_table(D.mw, kk, st), "same-edge table")
         pairs = D.keep["pairs"]
-        _lib.check(lib.mw_fill_z(D.mw, kk, omega, self.eps, self.mu, D.Z.data_ptr(), nb,
+        ld = D.ld
+        _lib.check(lib.mw_fill_z(D.mw, kk, omega, self.eps, self.mu, D.Z.data_ptr(), ld,
                                  pairs.data_ptr(), int(pairs.shape[0]), int(mesh.symmetric), st),
                    "fill")
         mark("fill", False)
         _lib.check(lib.mw_zscatter_cols(nb, ncols, D.col_max, D.col_idx.data_ptr(),
-                                        D.col_val.data_ptr(), D.X.data_ptr(), D.X.shape[1], st),
+                                        D.col_val.data_ptr(), D.X.data_ptr(), ld, st),
                    "right-hand sides")
-        mark("getrf", True)
-        _lib.check(lib.mw_zgetrf(nb, D.Z.data_ptr(), nb, D.ipiv.data_ptr(), D.work.data_ptr(),
-                                 D.info.data_ptr(), cancel, st), "factorisation")
-        mark("getrf", False)
-        ldx = D.X.shape[1]
-        mark("getrs", True)
-        _lib.check(lib.mw_zgetrs(nb, ncols, D.Z.data_ptr(), nb, D.ipiv.data_ptr(),
-                                 D.work.data_ptr(), D.X.data_ptr(), ldx, st), "solve")
-        mark("getrs", False)
+        mark("solve", True)
+        _lib.check(lib.mw_zgesv(nb, ncols, D.Z.data_ptr(), ld, D.ipiv.data_ptr(),
+                                D.work.data_ptr(), D.info.data_ptr(), cancel, st),
+                   "factorisation and solve")
+        mark("solve", False)
+        ldx = ld
         npt = D.n_ports
         if D.n_con:
             Xc = D.X.data_ptr() + 16 * npt

@@ -10,8 +10,11 @@ hipError_t mw_launch_fill(const mw_mesh& M, double k, double omega, double eps, 
                           bool symmetric, hipStream_t st);
 int64_t mw_getrf_ws_bytes(int64_t n);
 template <typename T>
-int mw_getrf_t(int64_t n, void* A, int64_t lda, int32_t* ipiv, void* work, int32_t* info,
-               const volatile int32_t* cancel, hipStream_t st, hipError_t* herr);
+int mw_getrf_t(int64_t n, int64_t nc, void* A, int64_t lda, int32_t* ipiv, void* work,
+               int32_t* info, const volatile int32_t* cancel, hipStream_t st, hipError_t* herr);
+template <typename T>
+int mw_gesv_t(int64_t n, int64_t nrhs, void* A, int64_t lda, int32_t* ipiv, void* work,
+              int32_t* info, const volatile int32_t* cancel, hipStream_t st, hipError_t* herr);
 template <typename T>
 int mw_getrs_t(int64_t n, int64_t nrhs, const void* A, int64_t lda, const int32_t* ipiv,
                const void* work, void* X, int64_t ldx, hipStream_t st, hipError_t* herr);
@@ -101,8 +104,8 @@ static int getrf_common(bool dbl, int64_t n, void* A, int64_t lda, int32_t* ipiv
   if (!A || !ipiv || !work || !info) return fail(MW_EINVAL, "%sNULL argument");
   if (lda < n) return fail(MW_EINVAL, "%slda %lld < n %lld", "", lda, n);
   hipError_t herr = hipSuccess;
-  int rc = dbl ? mw_getrf_t<double>(n, A, lda, ipiv, work, info, cancel, (hipStream_t)stream, &herr)
-               : mw_getrf_t<float>(n, A, lda, ipiv, work, info, cancel, (hipStream_t)stream, &herr);
+  int rc = dbl ? mw_getrf_t<double>(n, n, A, lda, ipiv, work, info, cancel, (hipStream_t)stream, &herr)
+               : mw_getrf_t<float>(n, n, A, lda, ipiv, work, info, cancel, (hipStream_t)stream, &herr);
   if (rc == MW_EHIP) return hip_status(herr, "getrf");
   if (rc == MW_EABORT) return fail(MW_EABORT, "%sfactorisation cancelled");
   return rc;
@@ -115,6 +118,29 @@ int mw_zgetrf(int64_t n, void* A, int64_t lda, int32_t* ipiv, void* work, int32_
 int mw_cgetrf(int64_t n, void* A, int64_t lda, int32_t* ipiv, void* work, int32_t* info,
               const volatile int32_t* cancel, void* stream) {
   return getrf_common(false, n, A, lda, ipiv, work, info, cancel, stream);
+}
+
+static int gesv_common(bool dbl, int64_t n, int64_t nrhs, void* A, int64_t lda, int32_t* ipiv,
+                       void* work, int32_t* info, const volatile int32_t* cancel, void* stream) {
+  g_err[0] = 0;
+  if (n <= 0 || nrhs < 0) return fail(MW_EINVAL, "%sbad sizes n=%lld nrhs=%lld", "", n, nrhs);
+  if (!A || !ipiv || !work || !info) return fail(MW_EINVAL, "%sNULL argument");
+  if (lda < n + nrhs) return fail(MW_EINVAL, "%slda %lld < n + nrhs %lld", "", lda, n + nrhs);
+  hipError_t herr = hipSuccess;
+  int rc = dbl ? mw_gesv_t<double>(n, nrhs, A, lda, ipiv, work, info, cancel, (hipStream_t)stream, &herr)
+               : mw_gesv_t<float>(n, nrhs, A, lda, ipiv, work, info, cancel, (hipStream_t)stream, &herr);
+  if (rc == MW_EHIP) return hip_status(herr, "gesv");
+  if (rc == MW_EABORT) return fail(MW_EABORT, "%sfactorisation cancelled");
+  return rc;
+}
+
+int mw_zgesv(int64_t n, int64_t nrhs, void* A, int64_t lda, int32_t* ipiv, void* work,
+             int32_t* info, const volatile int32_t* cancel, void* stream) {
+  return gesv_common(true, n, nrhs, A, lda, ipiv, work, info, cancel, stream);
+}
+int mw_cgesv(int64_t n, int64_t nrhs, void* A, int64_t lda, int32_t* ipiv, void* work,
+             int32_t* info, const volatile int32_t* cancel, void* stream) {
+  return gesv_common(false, n, nrhs, A, lda, ipiv, work, info, cancel, stream);
 }
 
 static int getrs_common(bool dbl, int64_t n, int64_t nrhs, const void* A, int64_t lda,

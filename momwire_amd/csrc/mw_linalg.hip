@@ -306,44 +306,42 @@ MW_DEV void load_block(CT (&S)[RPL][8], const CT* A, int64_t lda, int64_t k0, in
 }
 
 // Inverses of the unit-lower and upper factors held in M (L\U, padded to NB
-// with the identity): Li = L^{-1}, Ui = U^{-1}.  Row recurrences, one barrier
-// per row; threads [0, NB) build Li top-down, [NB, 2 NB) build Ui bottom-up.
+// with the identity): Li = L^{-1}, Ui = U^{-1}, by Gauss-Jordan sweeps with
+// one barrier per column.  U^{-1} = (D Ut)^{-1} = Ut^{-1} D^{-1} with Ut unit
+// upper, so both sweeps are unit-triangular and the diagonal is applied last.
+// Each of the 256 threads owns four entries of Li and four of Ui.
 template <typename CT>
-MW_DEV void tri_inverses(const CT (*M)[NB + 1], CT (*Li)[NB + 1], CT (*Ui)[NB + 1]) {
+MW_DEV void tri_inverses(const CT (*M)[NB + 1], CT (*Li)[NB + 1], CT (*Ui)[NB + 1], CT* dinv) {
   const int tid = threadIdx.x;
   for (int e = tid; e < NB * NB; e += blockDim.x) {
     const int i = e / NB, j = e % NB;
     CT v;
     v.x = (i == j) ? 1 : 0; v.y = 0;
     Li[i][j] = v;
-    v.x = 0;
     Ui[i][j] = v;
+  }
+  if (tid < NB) {
+    CT d = M[tid][tid];
+    if (d.x != 0 || d.y != 0) d = crecip(d);  // a zero pivot (reported via info) stays zero
+    dinv[tid] = d;
   }
   __syncthreads();
   for (int s = 0; s < NB; ++s) {
-    if (tid < NB) {
-      const int i = s, j = tid;  // Li[i][j] = -sum_{l=j}^{i-1} M[i][l] Li[l][j]
-      if (j < i) {
-        CT v;
-        v.x = 0; v.y = 0;
-        for (int l = j; l < i; ++l) v = cfms(v, M[i][l], Li[l][j]);
-        Li[i][j] = v;
-      }
-    } else if (tid < 2 * NB) {
-      const int i = NB - 1 - s, j = tid - NB;  // Ui[i][j], j >= i
-      if (j >= i) {
-        CT v;
-        v.x = (i == j) ? 1 : 0; v.y = 0;
-        for (int l = i + 1; l <= j; ++l) v = cfms(v, M[i][l], Ui[l][j]);
-        // an exactly zero pivot (reported through info) maps to a zero
-        // inverse entry rather than inf/NaN
-        if (M[i][i].x != 0 || M[i][i].y != 0) v = cdiv(v, M[i][i]);
-        else { v.x = 0; v.y = 0; }
-        Ui[i][j] = v;
-      }
+    const int cl = s, cu = NB - 1 - s;
+#pragma unroll
+    for (int q = 0; q < NB * NB / 256; ++q) {
+      const int e = tid + 256 * q;
+      const int i = e / NB, j = e % NB;
+      if (i > cl && j <= cl) Li[i][j] = cfms(Li[i][j], M[i][cl], Li[cl][j]);
+      if (i < cu && j >= cu) Ui[i][j] = cfms(Ui[i][j], cmul(M[i][cu], dinv[i]), Ui[cu][j]);
     }
     __syncthreads();
   }
+  for (int e = tid; e < NB * NB; e += blockDim.x) {
+    const int i = e / NB, j = e % NB;
+    if (j >= i) Ui[i][j] = cmul(Ui[i][j], dinv[j]);
+  }
+  __syncthreads();
 }
 
 template <typename CT>
@@ -351,6 +349,7 @@ struct PanelLds {
   CT M[NB][NB + 1];   // L11\U11 (rows workgroups reuse M and Li as a 64-row buffer)
   CT Li[NB][NB + 1];
   CT Ui[NB][NB + 1];
+  CT dinv[NB];
   int bad;
   int piv_s;
 };
@@ -398,7 +397,7 @@ __global__ __launch_bounds__(256) void k_panel_nopiv(int64_t n, const typename c
     if (zero) {
       if (tid == 0) P.bad = 1;
     } else if (tid > c && tid < NB) {
-      const CT l = cmul(P.M[tid][c], crecip(d));
+      const CT l = cdiv(P.M[tid][c], d);
       if (!(cabs1(l) <= lbound)) P.bad = 1;
       P.M[tid][c] = l;
     }
@@ -413,7 +412,7 @@ __global__ __launch_bounds__(256) void k_panel_nopiv(int64_t n, const typename c
     __syncthreads();
   }
   if (tid == 0 && P.M[NB - 1][NB - 1].x == 0 && P.M[NB - 1][NB - 1].y == 0) P.bad = 1;
-  tri_inverses(P.M, P.Li, P.Ui);
+  tri_inverses(P.M, P.Li, P.Ui, P.dinv);
   const int b = blockIdx.x;
   if (b == 0) {
     store_factors(P, bw, lu11, inv);
@@ -468,7 +467,8 @@ struct TourWs {
 
 template <typename T>
 __global__ __launch_bounds__(256) void k_panel_pivot(typename cplx_t<T>::type* A, int64_t lda,
-                                                     int64_t n, int64_t k0, int bw, int nleaf,
+                                                     int64_t n, int64_t nc, int64_t k0, int bw,
+                                                     int nleaf,
                                                      TourWs ws, const int32_t* flag,
                                                      typename cplx_t<T>::type* lu11,
                                                      typename cplx_t<T>::type* inv,
@@ -580,16 +580,16 @@ __global__ __launch_bounds__(256) void k_panel_pivot(typename cplx_t<T>::type* A
     if (steps < bw && *info == 0) *info = (int32_t)(k0 + steps + 1);
   }
   __syncthreads();
-  tri_inverses(P.M, P.Li, P.Ui);
+  tri_inverses(P.M, P.Li, P.Ui, P.dinv);
   store_factors(P, bw, lu11, inv);
   // reset the arrival counters for the next fallback
   int total = 0;
   for (int nd = nleaf; nd > 1; nd = (nd + FANIN - 1) / FANIN) total += nd;
   total += 1;
   for (int e = tid; e < total; e += blockDim.x) ws.counter[e] = 0;
-  // interchanges on columns [k0, n): nobody else reads A any more
+  // interchanges on columns [k0, nc): nobody else reads A any more
   __syncthreads();
-  for (int64_t c = k0 + tid; c < n; c += blockDim.x) {
+  for (int64_t c = k0 + tid; c < nc; c += blockDim.x) {
     for (int i = 0; i < bw; ++i) {
       const int64_t p1 = k0 + i, p2 = ipiv[k0 + i];
       if (p2 != p1) {
@@ -648,7 +648,7 @@ MW_DEV void apply_left(ApplyLds<CT>& L, CT* X, int64_t ldx, int64_t k0, int bw, 
 // Blocks [0, nrb): 64-row blocks of L21; [nrb, nrb+ncb): 64-column blocks of
 // U12; last: A11.
 template <typename T>
-__global__ __launch_bounds__(256) void k_panel_apply(int64_t n, typename cplx_t<T>::type* A,
+__global__ __launch_bounds__(256) void k_panel_apply(int64_t n, int64_t nc, typename cplx_t<T>::type* A,
                                                      int64_t lda, int64_t k0, int bw,
                                                      const typename cplx_t<T>::type* inv,
                                                      const typename cplx_t<T>::type* lu11,
@@ -691,7 +691,7 @@ __global__ __launch_bounds__(256) void k_panel_apply(int64_t n, typename cplx_t<
   } else if (b < nrb + ncb) {
     for (int e = tid; e < NB * NB; e += blockDim.x) L.Tm[e / NB][e % NB] = inv[e];
     __syncthreads();
-    apply_left(L, A, lda, k0, bw, k0 + bw + (int64_t)(b - nrb) * AW, n);
+    apply_left(L, A, lda, k0, bw, k0 + bw + (int64_t)(b - nrb) * AW, nc);
   } else {
     for (int e = tid; e < bw * bw; e += blockDim.x)
       A[(k0 + e / bw) * lda + k0 + e % bw] = lu11[(e / bw) * NB + e % bw];
@@ -846,9 +846,12 @@ static void* panel_inv(void* work, int64_t k0) {
 
 static double g_lbound = 8.0;  // threshold-pivoting multiplier bound (1/tau); <= 0: always pivot
 
+// Factor the n x n block of A; columns [n, nc) are carried along (row
+// interchanges, L^{-1} and trailing updates), so on return they hold
+// L^{-1} P B for right-hand sides B stored there: the forward half of a solve.
 template <typename T>
-int mw_getrf_t(int64_t n, void* Av, int64_t lda, int32_t* ipiv, void* work, int32_t* info,
-               const volatile int32_t* cancel, hipStream_t st, hipError_t* herr) {
+int mw_getrf_t(int64_t n, int64_t nc, void* Av, int64_t lda, int32_t* ipiv, void* work,
+               int32_t* info, const volatile int32_t* cancel, hipStream_t st, hipError_t* herr) {
   typedef typename cplx_t<T>::type CT;
   CT* A = reinterpret_cast<CT*>(Av);
   const int64_t npan = (n + NB - 1) / NB;
@@ -868,19 +871,21 @@ int mw_getrf_t(int64_t n, void* Av, int64_t lda, int32_t* ipiv, void* work, int3
   for (int64_t k0 = 0; k0 < n; k0 += NB) {
     if (cancel && *cancel) return MW_EABORT;
     const int bw = (int)((n - k0) < NB ? (n - k0) : NB);
-    const int64_t rest = n - k0 - bw;
+    const int64_t rest = n - k0 - bw;        // rows below the panel
+    const int64_t restc = nc - k0 - bw;      // columns right of the panel
     const int nrb = (int)((rest + AW - 1) / AW);
+    const int ncb = (int)((restc + AW - 1) / AW);
     CT* inv = reinterpret_cast<CT*>(panel_inv(work, k0));
     int32_t* flag = flags + k0 / NB;
     hipLaunchKernelGGL(k_panel_nopiv<T>, dim3(nrb + 1), dim3(256), 0, st, n, (const CT*)A, lda, k0,
                        bw, lu11, inv, ipiv, flag, lbound);
     const int nleaf = (int)((n - k0 + TROWS - 1) / TROWS);
-    hipLaunchKernelGGL(k_panel_pivot<T>, dim3(nleaf), dim3(256), 0, st, A, lda, n, k0, bw, nleaf,
-                       tw, (const int32_t*)flag, lu11, inv, ipiv, info);
-    hipLaunchKernelGGL(k_panel_apply<T>, dim3(2 * nrb + 1), dim3(256), 0, st, n, A, lda, k0, bw,
-                       (const CT*)inv, (const CT*)lu11, nrb, nrb);
-    if (rest > 0) {
-      *herr = launch_gemm_sub<T>(rest, rest, bw, A + (k0 + bw) * lda + k0, lda,
+    hipLaunchKernelGGL(k_panel_pivot<T>, dim3(nleaf), dim3(256), 0, st, A, lda, n, nc, k0, bw,
+                       nleaf, tw, (const int32_t*)flag, lu11, inv, ipiv, info);
+    hipLaunchKernelGGL(k_panel_apply<T>, dim3(nrb + ncb + 1), dim3(256), 0, st, n, nc, A, lda, k0,
+                       bw, (const CT*)inv, (const CT*)lu11, nrb, ncb);
+    if (rest > 0 && restc > 0) {
+      *herr = launch_gemm_sub<T>(rest, restc, bw, A + (k0 + bw) * lda + k0, lda,
                                  A + k0 * lda + k0 + bw, lda, A + (k0 + bw) * lda + k0 + bw, lda,
                                  st);
       if (*herr != hipSuccess) return MW_EHIP;
@@ -895,11 +900,10 @@ void mw_set_pivot_bound(double lbound) { g_lbound = lbound; }
 double mw_get_pivot_bound() { return g_lbound; }
 
 template <typename T>
-int mw_getrs_t(int64_t n, int64_t nrhs, const void* Av, int64_t lda, const int32_t* ipiv,
-               const void* work, void* Xv, int64_t ldx, hipStream_t st, hipError_t* herr) {
+static int solve_fwd(int64_t n, int64_t nrhs, const typename cplx_t<T>::type* A, int64_t lda,
+                     const int32_t* ipiv, const void* work, typename cplx_t<T>::type* X,
+                     int64_t ldx, hipStream_t st, hipError_t* herr) {
   typedef typename cplx_t<T>::type CT;
-  const CT* A = reinterpret_cast<const CT*>(Av);
-  CT* X = reinterpret_cast<CT*>(Xv);
   const unsigned gx = (unsigned)((nrhs + AW - 1) / AW);
   for (int64_t k0 = 0; k0 < n; k0 += NB) {
     const int bw = (int)((n - k0) < NB ? (n - k0) : NB);
@@ -913,6 +917,15 @@ int mw_getrs_t(int64_t n, int64_t nrhs, const void* Av, int64_t lda, const int32
       if (*herr != hipSuccess) return MW_EHIP;
     }
   }
+  return MW_OK;
+}
+
+template <typename T>
+static int solve_bwd(int64_t n, int64_t nrhs, const typename cplx_t<T>::type* A, int64_t lda,
+                     const void* work, typename cplx_t<T>::type* X, int64_t ldx, hipStream_t st,
+                     hipError_t* herr) {
+  typedef typename cplx_t<T>::type CT;
+  const unsigned gx = (unsigned)((nrhs + AW - 1) / AW);
   const int64_t nlast = ((n + NB - 1) / NB - 1) * NB;
   for (int64_t k0 = nlast; k0 >= 0; k0 -= NB) {
     const int bw = (int)((n - k0) < NB ? (n - k0) : NB);
@@ -924,14 +937,48 @@ int mw_getrs_t(int64_t n, int64_t nrhs, const void* Av, int64_t lda, const int32
       if (*herr != hipSuccess) return MW_EHIP;
     }
   }
+  return MW_OK;
+}
+
+template <typename T>
+int mw_getrs_t(int64_t n, int64_t nrhs, const void* Av, int64_t lda, const int32_t* ipiv,
+               const void* work, void* Xv, int64_t ldx, hipStream_t st, hipError_t* herr) {
+  typedef typename cplx_t<T>::type CT;
+  const CT* A = reinterpret_cast<const CT*>(Av);
+  CT* X = reinterpret_cast<CT*>(Xv);
+  int rc = solve_fwd<T>(n, nrhs, A, lda, ipiv, work, X, ldx, st, herr);
+  if (rc) return rc;
+  rc = solve_bwd<T>(n, nrhs, A, lda, work, X, ldx, st, herr);
+  if (rc) return rc;
   *herr = hipGetLastError();
   return *herr == hipSuccess ? MW_OK : MW_EHIP;
 }
 
-template int mw_getrf_t<double>(int64_t, void*, int64_t, int32_t*, void*, int32_t*,
+// A is n x (n + nrhs): columns [n, n+nrhs) hold B on entry and X = A^{-1} B on
+// return; the forward half rides inside the factorisation.
+template <typename T>
+int mw_gesv_t(int64_t n, int64_t nrhs, void* Av, int64_t lda, int32_t* ipiv, void* work,
+              int32_t* info, const volatile int32_t* cancel, hipStream_t st, hipError_t* herr) {
+  typedef typename cplx_t<T>::type CT;
+  int rc = mw_getrf_t<T>(n, n + nrhs, Av, lda, ipiv, work, info, cancel, st, herr);
+  if (rc) return rc;
+  if (nrhs > 0) {
+    CT* A = reinterpret_cast<CT*>(Av);
+    rc = solve_bwd<T>(n, nrhs, A, lda, work, A + n, lda, st, herr);
+    if (rc) return rc;
+  }
+  *herr = hipGetLastError();
+  return *herr == hipSuccess ? MW_OK : MW_EHIP;
+}
+
+template int mw_getrf_t<double>(int64_t, int64_t, void*, int64_t, int32_t*, void*, int32_t*,
                                 const volatile int32_t*, hipStream_t, hipError_t*);
-template int mw_getrf_t<float>(int64_t, void*, int64_t, int32_t*, void*, int32_t*,
+template int mw_getrf_t<float>(int64_t, int64_t, void*, int64_t, int32_t*, void*, int32_t*,
                                const volatile int32_t*, hipStream_t, hipError_t*);
+template int mw_gesv_t<double>(int64_t, int64_t, void*, int64_t, int32_t*, void*, int32_t*,
+                               const volatile int32_t*, hipStream_t, hipError_t*);
+template int mw_gesv_t<float>(int64_t, int64_t, void*, int64_t, int32_t*, void*, int32_t*,
+                              const volatile int32_t*, hipStream_t, hipError_t*);
 template int mw_getrs_t<double>(int64_t, int64_t, const void*, int64_t, const int32_t*,
                                 const void*, void*, int64_t, hipStream_t, hipError_t*);
 template int mw_getrs_t<float>(int64_t, int64_t, const void*, int64_t, const int32_t*,

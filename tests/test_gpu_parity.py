@@ -82,6 +82,32 @@ def test_lu_solve_random(n, p):
     assert res <= 20 * res_ref + 1e-13
 
 
+@pytest.mark.parametrize("dtype", ["z", "c"])
+@pytest.mark.parametrize("n,p", [(1, 1), (40, 3), (300, 64), (1111, 9)])
+def test_gesv_augmented(dtype, n, p):
+    """mw_?gesv: the right-hand sides ride through the factorisation as extra
+    columns of [A | B]; the answer must match LAPACK's."""
+    _say("gesv", dtype, n, p)
+    rng = np.random.default_rng(n + p)
+    M = rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n)) + (3 * np.eye(n) if dtype == "c" else 0)
+    RHS = rng.standard_normal((n, p)) + 1j * rng.standard_normal((n, p))
+    lib = _lib.load()
+    ct = torch.complex128 if dtype == "z" else torch.complex64
+    aug = torch.from_numpy(np.hstack([M, RHS])).to(ct).cuda()
+    ipiv = torch.empty(n, dtype=torch.int32, device="cuda")
+    work = torch.empty(int(lib.mw_getrf_workspace_bytes(n)), dtype=torch.uint8, device="cuda")
+    info = torch.zeros(1, dtype=torch.int32, device="cuda")
+    fn = lib.mw_zgesv if dtype == "z" else lib.mw_cgesv
+    _lib.check(fn(n, p, aug.data_ptr(), n + p, ipiv.data_ptr(), work.data_ptr(), info.data_ptr(), None, _stream()), "gesv")
+    torch.cuda.synchronize()
+    X = aug[:, n:].cpu().numpy().astype(np.complex128)
+    ref = np.linalg.solve(M, RHS)
+    eps = 1.1e-16 if dtype == "z" else 6e-8
+    tol = eps * np.linalg.cond(M) * 200
+    assert int(info.cpu()[0]) == 0
+    assert rel_err(X, ref) < tol
+
+
 def test_lu_needs_pivoting():
     _say("lu pivoting")
     n = 300
@@ -191,10 +217,10 @@ def test_fill_matches_oracle_Z():
         for sym in (1, 0):
             D.Z.zero_()
             pairs = D.keep["pairs"]
-            _lib.check(lib.mw_fill_z(D.mw, s.k, s.omega, s.eps, s.mu, D.Z.data_ptr(), D.nb,
+            _lib.check(lib.mw_fill_z(D.mw, s.k, s.omega, s.eps, s.mu, D.Z.data_ptr(), D.ld,
                                      pairs.data_ptr(), int(pairs.shape[0]), sym, _stream()), "fill")
             torch.cuda.synchronize()
-            Z = D.Z.cpu().numpy()
+            Z = D.Z[:, :D.nb].cpu().numpy()
             o = OracleSolver(**kw)
             assert rel_err(Z, o.Z()) < 1e-11, (name, sym)
 
