@@ -34,7 +34,77 @@ WORKLOADS = {
                           "0.6-lambda pitch, +-0.02-lambda seeded jitter (seed 1), radius "
                           "1e-3 lambda, free space, 300 MHz, 64 centre feeds",
                      args=(8, 8, 41, 1), golden="c2_array8x8"),
+    "array32x32": dict(desc="32x32 array of z-directed 0.48-lambda dipoles, 15 segments each "
+                            "(15,360 unknowns), 0.6-lambda pitch, +-0.02-lambda seeded jitter "
+                            "(seed 2), radius 1e-3 lambda, free space, 300 MHz, 1024 centre "
+                            "feeds; mixed precision (FP32 LU on MFMA, FP64 residual refinement) "
+                            "against the FP64 GPU path",
+                       args=(32, 32, 15, 2), golden=None),
 }
+
+
+def mixed_main(args, wl):
+    """Mixed precision vs the FP64 GPU path on one structure: solves/s of
+    each, and the mixed form's Y error against FP64 after every correction."""
+    import torch
+
+    from momwire_amd import workloads
+    from momwire_amd.bspline import BSplineSolver
+
+    kw = workloads.dipole_array(*wl["args"][:3], seed=wl["args"][3])
+    s64 = BSplineSolver(**kw)
+    smx = BSplineSolver(**kw, precision="mixed", refine_steps=args.refine_steps)
+    mesh = s64._build()
+    nb, ns = mesh.n_basis, mesh.n_seg
+    print(f"[bench] {args.workload}: {nb} unknowns, {s64._port_count()} ports", file=sys.stderr,
+          flush=True)
+    res = {}
+    for name, solver, fn in (("fp64", s64, s64._solve_device), ("mixed", smx, smx._solve_device_mixed)):
+        for _ in range(args.warmup):
+            fn()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            fn()
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        timers = {}
+        fn(timers=timers)
+        torch.cuda.synchronize()
+        stage = {k: float(np.mean([a.elapsed_time(b) for a, b in v])) for k, v in timers.items()}
+        res[name] = dict(rate=args.steps / dt, ms=1e3 * dt / args.steps, stage_ms=stage)
+        print(f"[bench] {name}: {1e3 * dt / args.steps:.1f} ms per solve {stage}", file=sys.stderr,
+              flush=True)
+    Y64 = s64._solve_device()[0].cpu().numpy()
+    hist = []
+    smx._solve_device_mixed(history=hist)
+    ynorm = np.abs(Y64).max()
+    per_step = [float(np.abs(h.cpu().numpy() - Y64).max() / ynorm) for h in hist
+                if not isinstance(h, tuple)]
+    adjoint = [float(np.abs(h[1].cpu().numpy() - Y64).max() / ynorm) for h in hist
+               if isinstance(h, tuple)]
+    s64._check_info()
+    fill_s = res["fp64"]["stage_ms"]["fill"] * 1e-3
+    out = {
+        "metric": METRIC,
+        "value": res["mixed"]["rate"],
+        "unit": "port-solves/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": res["mixed"]["ms"],
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "c64 factor / f64 residual", "data": "synthetic",
+        "config": {"workload": wl["desc"], "unknowns": nb, "segments": ns,
+                   "ports": s64._port_count(), "refine_steps": args.refine_steps},
+        "fp64_port_solves_per_s": res["fp64"]["rate"],
+        "mixed_over_fp64": res["mixed"]["rate"] / res["fp64"]["rate"],
+        "y_rel_err_vs_fp64_per_step": per_step,
+        "y_rel_err_adjoint_corrected_per_step": adjoint,
+        "stage_ms": {"fp64": res["fp64"]["stage_ms"], "mixed": res["mixed"]["stage_ms"]},
+        "fill_gentries_per_s": nb * nb / fill_s / 1e9,
+        "fp64_solve_tflops": (lu_flops(nb) + 8.0 * nb * nb * s64._port_count())
+                             / (res["fp64"]["stage_ms"]["solve"] * 1e-3) / 1e12,
+    }
+    print(json.dumps(out))
 
 
 def fill_flops(n_seg, n_b):
@@ -91,18 +161,24 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--eager", action="store_true",
                     help="launch kernels one by one instead of replaying the captured graph")
+    ap.add_argument("--refine-steps", type=int, default=None,
+                    help="FP64-residual corrections of the mixed form (array32x32); default: "
+                         "one, plus the adjoint-corrected readout")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
               "(the multi-GPU layout is described, not built)", file=sys.stderr)
         sys.exit(2)
 
+    wl = WORKLOADS[args.workload]
+    if wl["golden"] is None:
+        return mixed_main(args, wl)
+
     import torch
 
     from momwire_amd import workloads
     from momwire_amd.bspline import BSplineSolver
 
-    wl = WORKLOADS[args.workload]
     kw = workloads.dipole_array(*wl["args"][:3], seed=wl["args"][3])
     s = BSplineSolver(**kw)
     mesh = s._build()

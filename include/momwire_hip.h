@@ -161,6 +161,14 @@ int mw_cgemm_sub(int64_t m, int64_t n, int64_t k, const void* A, int64_t lda,
                  const void* B, int64_t ldb, void* C, int64_t ldc, void* stream);
 
 /*
+ * C <- C + A^T B (complex, MFMA; A stored k x m, plain transpose).  The
+ * adjoint-corrected port readout of the mixed form, Y + X^T R with
+ * R = B - Z X: Z is complex symmetric, so its error is quadratic in X's.
+ */
+int mw_zgemm_tn_add(int64_t m, int64_t n, int64_t k, const void* A, int64_t lda,
+                    const void* B, int64_t ldb, void* C, int64_t ldc, void* stream);
+
+/*
  * out[i][j] = sum_t val[i][t] * X[idx[i][t]][j]  (sparse rows times X).
  * The port readout Y = B^T X of bspline.py:4178-4179 (B's columns are sparse:
  * a delta-gap drive touches d+1 bases) and the KCL products A X of
@@ -180,6 +188,15 @@ int mw_zscatter_cols(int64_t n, int64_t ncols, int32_t maxnz, const int32_t* idx
  */
 int mw_zsmall_solve(int32_t ns, int32_t nt, void* S, int64_t lds, void* T,
                     int64_t ldt, int32_t* info, void* stream);
+
+/*
+ * Precision changes for the mixed form (FP32 factorisation, FP64 residual):
+ * dst <- (float) src, and dst (+)= (double) src, over a rows x cols block.
+ */
+int mw_z2c(int64_t rows, int64_t cols, const void* src, int64_t lds, void* dst,
+           int64_t ldd, void* stream);
+int mw_c2z(int64_t rows, int64_t cols, const void* src, int64_t lds, void* dst,
+           int64_t ldd, int32_t accumulate, void* stream);
 
 #ifdef __cplusplus
 }

@@ -125,6 +125,25 @@ class _Device:
                                      device=self.dev)
         return self
 
+    def buffers_mixed(self):
+        """Extra buffers of the mixed form: [Z32 | B32] factor block, the FP64
+        iterate and residual, and the FP32 correction."""
+        torch = self.torch
+        self.buffers()
+        if getattr(self, "Z32", None) is None:
+            nb, nc = self.nb, max(self.ncols, 1)
+            self.ld32 = nb + self.ncols
+            self.Z32 = torch.empty((nb, self.ld32), dtype=torch.complex64, device=self.dev)
+            self.X32 = self.Z32[:, nb:]
+            self.X64 = torch.empty((nb, nc), dtype=torch.complex128, device=self.dev)
+            self.R64 = torch.empty((nb, nc), dtype=torch.complex128, device=self.dev)
+            self.D32 = torch.empty((nb, nc), dtype=torch.complex64, device=self.dev)
+            self.ipiv32 = torch.empty(nb, dtype=torch.int32, device=self.dev)
+            lib = _lib.load()
+            self.work32 = torch.empty(int(lib.mw_getrf_workspace_bytes(nb)), dtype=torch.uint8,
+                                      device=self.dev)
+        return self
+
 
 class BSplineSolver(_SweptPortSolutions):
     """Degree-2 B-spline Galerkin MoM port solve on the GPU (see module doc)."""
@@ -140,7 +159,8 @@ class BSplineSolver(_SweptPortSolutions):
                  wire_conductivity=None, insulation_radius=None, insulation_eps_r=None,
                  nsegs=101, ground_z=None, ground_eps=None, ground_phi_mode="normal",
                  ground_model="refl-coef", n_qp_sommerfeld=3, use_singular_enrichment=False,
-                 swept_mem_mb=256, cancel=None, **unsupported):
+                 swept_mem_mb=256, cancel=None, precision="fp64", refine_steps=None,
+                 **unsupported):
         if unsupported:
             raise NotImplementedError(f"options not served on the GPU path: {sorted(unsupported)}")
         if degree < 1:
@@ -166,6 +186,15 @@ class BSplineSolver(_SweptPortSolutions):
         if int(swept_mem_mb) < 1:
             raise ValueError(f"swept_mem_mb must be >= 1, got {swept_mem_mb}")
 
+        if precision not in ("fp64", "mixed"):
+            raise ValueError(f"precision must be 'fp64' or 'mixed', got {precision!r}")
+        if refine_steps is not None and int(refine_steps) < 0:
+            raise ValueError("refine_steps must be >= 0")
+        # "mixed": FP64 fill, FP32 LU and solves on MFMA, iterative refinement
+        # with the residual B - Z X in FP64 (refine_steps corrections; default
+        # 1 plus the adjoint-corrected readout, or 3 when KCL rows are present).
+        self.precision = precision
+        self.refine_steps = None if refine_steps is None else int(refine_steps)
         self.degree = int(degree)
         self.n_qp_pair = 4
         self.wavelength = wavelength
@@ -384,25 +413,142 @@ class BSplineSolver(_SweptPortSolutions):
                                 D.work.data_ptr(), D.info.data_ptr(), cancel, st),
                    "factorisation and solve")
         mark("solve", False)
-        ldx = ld
+        self._post(D, D.X.data_ptr(), ld, D.Y.data_ptr(), st)
+        return D.Y[:D.n_ports, :D.n_ports], D.X[:, :D.n_ports]
+
+    def _post(self, D, xptr, ldx, yptr, st):
+        """KCL Schur step (bspline.py:2981-2989) and the readout Y = B^T X
+        (bspline.py:4178-4179), both on the device, in place on X."""
+        lib = _lib.load()
+        nb = D.nb
         npt = D.n_ports
         if D.n_con:
-            Xc = D.X.data_ptr() + 16 * npt
+            Xc = xptr + 16 * npt
             _lib.check(lib.mw_zsparse_rows(D.n_con, D.con_max, D.con_idx.data_ptr(),
                                            D.con_val.data_ptr(), Xc, ldx, D.n_con,
                                            D.S.data_ptr(), D.n_con, st), "KCL A.X")
             _lib.check(lib.mw_zsparse_rows(D.n_con, D.con_max, D.con_idx.data_ptr(),
-                                           D.con_val.data_ptr(), D.X.data_ptr(), ldx, npt,
+                                           D.con_val.data_ptr(), xptr, ldx, npt,
                                            D.T.data_ptr(), D.T.shape[1], st), "KCL A.W")
             _lib.check(lib.mw_zsmall_solve(D.n_con, npt, D.S.data_ptr(), D.n_con,
                                            D.T.data_ptr(), D.T.shape[1],
                                            D.info.data_ptr() + 4, st), "KCL Schur")
             _lib.check(lib.mw_zgemm_sub(nb, npt, D.n_con, Xc, ldx, D.T.data_ptr(),
-                                        D.T.shape[1], D.X.data_ptr(), ldx, st), "KCL update")
+                                        D.T.shape[1], xptr, ldx, st), "KCL update")
         _lib.check(lib.mw_zsparse_rows(npt, D.port_max, D.port_idx.data_ptr(),
-                                       D.port_val.data_ptr(), D.X.data_ptr(), ldx, npt,
-                                       D.Y.data_ptr(), D.Y.shape[1], st), "port readout")
-        return D.Y[:npt, :npt], D.X[:, :npt]
+                                       D.port_val.data_ptr(), xptr, ldx, npt,
+                                       yptr, max(npt, 1), st), "port readout")
+
+    def _solve_device_mixed(self, k=None, timers=None, history=None):
+        """Mixed form on the GPU: FP64 fill; [Z | B] factored and solved in
+        FP32 on MFMA; then refine_steps corrections X += Z32^{-1} (B - Z X)
+        with the residual in FP64.  ``history``: optional list that receives
+        a device Y after the FP32 solve and after each correction."""
+        lib = _lib.load()
+        D = self._device().buffers_mixed()
+        torch = D.torch
+        st = torch.cuda.current_stream().cuda_stream
+
+        def mark(stage, first):
+            if timers is not None:
+                ev = torch.cuda.Event(enable_timing=True)
+                ev.record()
+                if first:
+                    timers.setdefault(stage, []).append([ev, None])
+                else:
+                    timers[stage][-1][1] = ev
+
+        kk = self.k if k is None else float(k)
+        omega = kk * self.c
+        mesh = self._mesh
+        nb, ncols = D.nb, D.ncols
+        nc = max(ncols, 1)
+        cancel = self._cancel.address if self._cancel is not None else None
+        pairs = D.keep["pairs"]
+        mark("fill", True)
+        _lib.check(lib.mw_same_edge_table(D.mw, kk, st), "same-edge table")
+        _lib.check(lib.mw_fill_z(D.mw, kk, omega, self.eps, self.mu, D.Z.data_ptr(), nb,
+                                 pairs.data_ptr(), int(pairs.shape[0]), int(mesh.symmetric), st),
+                   "fill")
+        mark("fill", False)
+        mark("solve32", True)
+        _lib.check(lib.mw_zscatter_cols(nb, ncols, D.col_max, D.col_idx.data_ptr(),
+                                        D.col_val.data_ptr(), D.R64.data_ptr(), nc, st), "rhs")
+        _lib.check(lib.mw_z2c(nb, nb, D.Z.data_ptr(), nb, D.Z32.data_ptr(), D.ld32, st), "Z to FP32")
+        _lib.check(lib.mw_z2c(nb, ncols, D.R64.data_ptr(), nc, D.X32.data_ptr(), D.ld32, st),
+                   "B to FP32")
+        _lib.check(lib.mw_cgesv(nb, ncols, D.Z32.data_ptr(), D.ld32, D.ipiv32.data_ptr(),
+                                D.work32.data_ptr(), D.info.data_ptr(), cancel, st),
+                   "FP32 factorisation and solve")
+        _lib.check(lib.mw_c2z(nb, ncols, D.X32.data_ptr(), D.ld32, D.X64.data_ptr(), nc, 0, st),
+                   "X to FP64")
+        mark("solve32", False)
+
+        def record():
+            if history is None:
+                return
+            Xt = D.X64.clone()
+            Yt = torch.empty((max(D.n_ports, 1), max(D.n_ports, 1)), dtype=torch.complex128,
+                             device=D.dev)
+            self._post(D, Xt.data_ptr(), nc, Yt.data_ptr(), st)
+            history.append(Yt[:D.n_ports, :D.n_ports])
+
+        npt = D.n_ports
+        adjoint = D.n_con == 0
+        steps = self.refine_steps if self.refine_steps is not None else (1 if adjoint else 3)
+
+        def residual():
+            # R = B - Z X in FP64 (the port and constraint columns together)
+            _lib.check(lib.mw_zscatter_cols(nb, ncols, D.col_max, D.col_idx.data_ptr(),
+                                            D.col_val.data_ptr(), D.R64.data_ptr(), nc, st), "rhs")
+            _lib.check(lib.mw_zgemm_sub(nb, ncols, nb, D.Z.data_ptr(), nb, D.X64.data_ptr(), nc,
+                                        D.R64.data_ptr(), nc, st), "FP64 residual")
+
+        def adjoint_readout(yptr):
+            # Y + X^T R with R = B - Z X: the adjoint-corrected readout, whose
+            # error is quadratic in the iterate's (Z is complex symmetric)
+            self._post(D, D.X64.data_ptr(), nc, yptr, st)
+            _lib.check(lib.mw_zgemm_tn_add(npt, npt, nb, D.X64.data_ptr(), nc, D.R64.data_ptr(),
+                                           nc, yptr, max(npt, 1), st), "adjoint readout")
+
+        def record_adjoint():
+            if history is None or not adjoint:
+                return
+            Yt = torch.empty((max(npt, 1), max(npt, 1)), dtype=torch.complex128, device=D.dev)
+            adjoint_readout(Yt.data_ptr())
+            history.append(("adjoint", Yt[:npt, :npt]))
+
+        record()
+        for _step in range(steps):
+            if self._cancel is not None and self._cancel.cancelled:
+                raise _lib.SolveAborted("refinement cancelled")
+            mark("refine", True)
+            residual()
+            record_adjoint()
+            _lib.check(lib.mw_z2c(nb, ncols, D.R64.data_ptr(), nc, D.D32.data_ptr(), nc, st),
+                       "residual to FP32")
+            _lib.check(lib.mw_cgetrs(nb, ncols, D.Z32.data_ptr(), D.ld32, D.ipiv32.data_ptr(),
+                                     D.work32.data_ptr(), D.D32.data_ptr(), nc, st),
+                       "FP32 correction")
+            _lib.check(lib.mw_c2z(nb, ncols, D.D32.data_ptr(), nc, D.X64.data_ptr(), nc, 1, st),
+                       "X += correction")
+            mark("refine", False)
+            record()
+        if adjoint:
+            mark("readout", True)
+            residual()
+            adjoint_readout(D.Y.data_ptr())
+            mark("readout", False)
+            if history is not None:
+                history.append(("adjoint", D.Y[:npt, :npt].clone()))
+        else:
+            self._post(D, D.X64.data_ptr(), nc, D.Y.data_ptr(), st)
+        return D.Y[:npt, :npt], D.X64[:, :npt]
+
+    def _solve(self, k=None, timers=None):
+        if self.precision == "mixed":
+            return self._solve_device_mixed(k, timers=timers)
+        return self._solve_device(k, timers=timers)
 
     def solve_graph(self):
         """The whole device solve captured once as a HIP graph for the current
@@ -416,11 +562,11 @@ class BSplineSolver(_SweptPortSolutions):
             side = torch.cuda.Stream()
             side.wait_stream(torch.cuda.current_stream())
             with torch.cuda.stream(side):
-                self._solve_device()
+                self._solve()
             torch.cuda.current_stream().wait_stream(side)
             g = torch.cuda.CUDAGraph()
             with torch.cuda.graph(g):
-                out = self._solve_device()
+                out = self._solve()
             self._graph, self._graph_out, self._graph_k = g, out, self.k
         self._graph.replay()
         return self._graph_out
@@ -434,7 +580,7 @@ class BSplineSolver(_SweptPortSolutions):
 
     def compute_port_solution(self, same_edge_prep=None) -> PortSolution:
         """Solve every port from ONE fill and ONE factorisation (bspline.py:4053-4179)."""
-        Y, W = self._solve_device()
+        Y, W = self._solve()
         y = Y.cpu().numpy()
         coeffs = W.cpu().numpy()
         self._check_info()

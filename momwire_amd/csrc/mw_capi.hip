@@ -22,11 +22,17 @@ hipError_t mw_gemm_sub_z(int64_t m, int64_t n, int64_t k, const void* A, int64_t
                          const void* B, int64_t ldb, void* C, int64_t ldc, hipStream_t st);
 hipError_t mw_gemm_sub_c(int64_t m, int64_t n, int64_t k, const void* A, int64_t lda,
                          const void* B, int64_t ldb, void* C, int64_t ldc, hipStream_t st);
+hipError_t mw_gemm_tn_add_z(int64_t m, int64_t n, int64_t k, const void* A, int64_t lda,
+                            const void* B, int64_t ldb, void* C, int64_t ldc, hipStream_t st);
 hipError_t mw_launch_sparse_rows(int64_t nrows, int maxnz, const int32_t* idx, const double* val,
                                  const void* X, int64_t ldx, int64_t ncols, void* out,
                                  int64_t ldo, hipStream_t st);
 hipError_t mw_launch_scatter_cols(int64_t n, int64_t ncols, int maxnz, const int32_t* idx,
                                   const double* val, void* X, int64_t ldx, hipStream_t st);
+hipError_t mw_launch_z2c(int64_t rows, int64_t cols, const void* src, int64_t lds, void* dst,
+                         int64_t ldd, hipStream_t st);
+hipError_t mw_launch_c2z(int64_t rows, int64_t cols, const void* src, int64_t lds, void* dst,
+                         int64_t ldd, int accumulate, hipStream_t st);
 void mw_set_pivot_bound(double lbound);
 double mw_get_pivot_bound();
 hipError_t mw_launch_small_solve(int ns, int nt, void* S, int64_t lds, void* T, int64_t ldt,
@@ -186,6 +192,16 @@ int mw_cgemm_sub(int64_t m, int64_t n, int64_t k, const void* A, int64_t lda, co
                     "mw_cgemm_sub");
 }
 
+int mw_zgemm_tn_add(int64_t m, int64_t n, int64_t k, const void* A, int64_t lda, const void* B,
+                    int64_t ldb, void* C, int64_t ldc, void* stream) {
+  g_err[0] = 0;
+  if (m < 0 || n < 0 || k < 0) return fail(MW_EINVAL, "%snegative size");
+  if ((m && n && k) && (!A || !B || !C)) return fail(MW_EINVAL, "%sNULL argument");
+  if (lda < m || ldb < n || ldc < n) return fail(MW_EINVAL, "%sbad leading dimension");
+  return hip_status(mw_gemm_tn_add_z(m, n, k, A, lda, B, ldb, C, ldc, (hipStream_t)stream),
+                    "mw_zgemm_tn_add");
+}
+
 int mw_zsparse_rows(int64_t nrows, int32_t maxnz, const int32_t* idx, const double* val,
                     const void* X, int64_t ldx, int64_t ncols, void* out, int64_t ldo,
                     void* stream) {
@@ -219,6 +235,25 @@ int mw_zsmall_solve(int32_t ns, int32_t nt, void* S, int64_t lds, void* T, int64
   if (lds < ns || ldt < nt) return fail(MW_EINVAL, "%sbad leading dimension");
   return hip_status(mw_launch_small_solve(ns, nt, S, lds, T, ldt, info, (hipStream_t)stream),
                     "mw_zsmall_solve");
+}
+
+int mw_z2c(int64_t rows, int64_t cols, const void* src, int64_t lds, void* dst, int64_t ldd,
+           void* stream) {
+  g_err[0] = 0;
+  if (rows < 0 || cols < 0) return fail(MW_EINVAL, "%snegative size");
+  if (rows && cols && (!src || !dst)) return fail(MW_EINVAL, "%sNULL argument");
+  if (lds < cols || ldd < cols) return fail(MW_EINVAL, "%sbad leading dimension");
+  return hip_status(mw_launch_z2c(rows, cols, src, lds, dst, ldd, (hipStream_t)stream), "mw_z2c");
+}
+
+int mw_c2z(int64_t rows, int64_t cols, const void* src, int64_t lds, void* dst, int64_t ldd,
+           int32_t accumulate, void* stream) {
+  g_err[0] = 0;
+  if (rows < 0 || cols < 0) return fail(MW_EINVAL, "%snegative size");
+  if (rows && cols && (!src || !dst)) return fail(MW_EINVAL, "%sNULL argument");
+  if (lds < cols || ldd < cols) return fail(MW_EINVAL, "%sbad leading dimension");
+  return hip_status(mw_launch_c2z(rows, cols, src, lds, dst, ldd, accumulate, (hipStream_t)stream),
+                    "mw_c2z");
 }
 
 }  // extern "C"

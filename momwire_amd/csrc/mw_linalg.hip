@@ -43,11 +43,24 @@ template <> struct Mfma<float> {
   static MW_DEV int row(int lane, int r) { return 4 * (lane >> 4) + r; }
 };
 
-constexpr int GBM = 64;      // C rows per workgroup
-constexpr int GBN = 32;      // C complex columns per workgroup (64 real)
+// C tile per workgroup: 128 rows x 64 complex columns (128 real columns);
+// four waves in 2 x 2, each a 4 x 4 grid of 16 x 16 MFMA tiles (64 x 64 real).
+// The accumulators start from C itself and MFMA accumulates (-A) B, so C is
+// read once and written once.  K advances 8 complex (16 real) per LDS stage,
+// double-buffered: the next stage's global loads are issued before the MFMAs
+// of the current one.
+constexpr int GBM = 128;     // C rows per workgroup
+constexpr int GBN = 64;      // C complex columns per workgroup (128 real)
 constexpr int GKC = 8;       // complex k per LDS stage (16 real)
 
 template <typename T>
+struct GemmLds {
+  T As[2][GBM][2 * GKC + 1];                     // A as reals, padded rows
+  typename cplx_t<T>::type Bs[2][GKC][GBN + 1];  // B as complex
+};
+
+// TA = false: C <- C - A B.  TA = true: C <- C + A^T B with A stored k x m.
+template <typename T, bool TA>
 __global__ __launch_bounds__(256) void k_gemm_sub(int64_t m, int64_t n, int64_t kdim,
                                                   const typename cplx_t<T>::type* __restrict__ A,
                                                   int64_t lda,
@@ -56,33 +69,152 @@ __global__ __launch_bounds__(256) void k_gemm_sub(int64_t m, int64_t n, int64_t 
                                                   int64_t ldc) {
   typedef typename cplx_t<T>::type CT;
   typedef typename Mfma<T>::acc_t acc_t;
-  __shared__ T As[GBM][2 * GKC + 1];
-  __shared__ CT Bs[GKC][GBN + 1];
+  __shared__ GemmLds<T> L;
 
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int wr = wave >> 1, wc = wave & 1;           // wave's 32x32 (real) quadrant
+  const int wr = wave >> 1, wc = wave & 1;  // wave's 64x64 (real) quadrant
   const int64_t row0 = (int64_t)blockIdx.y * GBM;
-  const int64_t col0 = (int64_t)blockIdx.x * GBN;    // complex columns
+  const int64_t col0 = (int64_t)blockIdx.x * GBN;  // complex columns
+  T* Cr = reinterpret_cast<T*>(C);
 
+  // accumulators <- C
+  acc_t acc[4][4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int64_t gr = row0 + wr * 64 + i * 16 + Mfma<T>::row(lane, r);
+        const int64_t gcr = 2 * col0 + wc * 64 + j * 16 + (lane & 15);
+        acc[i][j][r] = (gr < m && gcr < 2 * n) ? Cr[gr * 2 * ldc + gcr] : (T)0;
+      }
+
+  // staging: A 128 x 8 complex (4 per thread), B 8 x 64 complex (2 per thread)
+  CT ra[4], rb[2];
+  auto gload = [&](int64_t k0) {
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int e = tid + 256 * q;
+      const int r = TA ? (e & 127) : (e >> 3), c = TA ? (e >> 7) : (e & 7);
+      CT v;
+      v.x = 0; v.y = 0;
+      if (row0 + r < m && k0 + c < kdim)
+        v = TA ? A[(k0 + c) * lda + row0 + r] : A[(row0 + r) * lda + k0 + c];
+      ra[q] = v;
+    }
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+      const int e = tid + 256 * q;
+      const int r = e >> 6, c = e & 63;
+      CT v;
+      v.x = 0; v.y = 0;
+      if (k0 + r < kdim && col0 + c < n) v = B[(k0 + r) * ldb + col0 + c];
+      rb[q] = v;
+    }
+  };
+  auto sstore = [&](int buf) {
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int e = tid + 256 * q;
+      const int r = TA ? (e & 127) : (e >> 3), c = TA ? (e >> 7) : (e & 7);
+      L.As[buf][r][2 * c] = TA ? ra[q].x : -ra[q].x;  // C - A B  =  C + (-A) B
+      L.As[buf][r][2 * c + 1] = TA ? ra[q].y : -ra[q].y;
+    }
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+      const int e = tid + 256 * q;
+      L.Bs[buf][e >> 6][e & 63] = rb[q];
+    }
+  };
+
+  gload(0);
+  sstore(0);
+  __syncthreads();
+  int buf = 0;
+  for (int64_t k0 = 0; k0 < kdim; k0 += GKC) {
+    const bool more = k0 + GKC < kdim;
+    if (more) gload(k0 + GKC);
+#pragma unroll
+    for (int ks = 0; ks < 2 * GKC; ks += 4) {
+      const int kr = ks + (lane >> 4);        // real k of this lane
+      const int kc = kr >> 1, kpar = kr & 1;  // complex row of B, re/im half
+      T a[4], b[4];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) a[i] = L.As[buf][wr * 64 + i * 16 + (lane & 15)][kr];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const int cr = wc * 64 + j * 16 + (lane & 15);  // real column
+        const CT bv = L.Bs[buf][kc][cr >> 1];
+        // B_r[2l+s][2j+t] = s==0 ? (t==0 ? br : bi) : (t==0 ? -bi : br)
+        b[j] = (kpar == 0) ? ((cr & 1) ? bv.y : bv.x) : ((cr & 1) ? bv.x : -bv.y);
+      }
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc[i][j] = Mfma<T>::mma(a[i], b[j], acc[i][j]);
+    }
+    if (more) {
+      sstore(buf ^ 1);
+      __syncthreads();
+      buf ^= 1;
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int64_t gr = row0 + wr * 64 + i * 16 + Mfma<T>::row(lane, r);
+        const int64_t gcr = 2 * col0 + wc * 64 + j * 16 + (lane & 15);
+        if (gr < m && gcr < 2 * n) Cr[gr * 2 * ldc + gcr] = acc[i][j][r];
+      }
+}
+
+// Small-tile variant (64 rows x 32 complex columns, each wave 2 x 2 MFMA
+// tiles) for products too small to occupy the chip with 128 x 64 tiles.
+constexpr int SBM = 64, SBN = 32;
+
+template <typename T, bool TA>
+__global__ __launch_bounds__(256) void k_gemm_sub_s(int64_t m, int64_t n, int64_t kdim,
+                                                    const typename cplx_t<T>::type* __restrict__ A,
+                                                    int64_t lda,
+                                                    const typename cplx_t<T>::type* __restrict__ B,
+                                                    int64_t ldb, typename cplx_t<T>::type* C,
+                                                    int64_t ldc) {
+  typedef typename cplx_t<T>::type CT;
+  typedef typename Mfma<T>::acc_t acc_t;
+  __shared__ T As[SBM][2 * GKC + 1];
+  __shared__ CT Bs[GKC][SBN + 1];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wr = wave >> 1, wc = wave & 1;
+  const int64_t row0 = (int64_t)blockIdx.y * SBM;
+  const int64_t col0 = (int64_t)blockIdx.x * SBN;
+  T* Cr = reinterpret_cast<T*>(C);
   acc_t acc[2][2];
 #pragma unroll
   for (int i = 0; i < 2; ++i)
 #pragma unroll
-    for (int j = 0; j < 2; ++j) acc[i][j] = acc_t{0, 0, 0, 0};
-
+    for (int j = 0; j < 2; ++j)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int64_t gr = row0 + wr * 32 + i * 16 + Mfma<T>::row(lane, r);
+        const int64_t gcr = 2 * col0 + wc * 32 + j * 16 + (lane & 15);
+        acc[i][j][r] = (gr < m && gcr < 2 * n) ? Cr[gr * 2 * ldc + gcr] : (T)0;
+      }
   for (int64_t k0 = 0; k0 < kdim; k0 += GKC) {
-    // stage A: 64 rows x 8 complex
-    for (int e = tid; e < GBM * GKC; e += 256) {
-      const int r = e / GKC, c = e % GKC;
+    for (int e = tid; e < SBM * GKC; e += 256) {
+      const int r = TA ? (e % SBM) : (e / GKC), c = TA ? (e / SBM) : (e % GKC);
       CT v;
       v.x = 0; v.y = 0;
-      if (row0 + r < m && k0 + c < kdim) v = A[(row0 + r) * lda + k0 + c];
-      As[r][2 * c] = v.x;
-      As[r][2 * c + 1] = v.y;
+      if (row0 + r < m && k0 + c < kdim)
+        v = TA ? A[(k0 + c) * lda + row0 + r] : A[(row0 + r) * lda + k0 + c];
+      As[r][2 * c] = TA ? v.x : -v.x;
+      As[r][2 * c + 1] = TA ? v.y : -v.y;
     }
-    // stage B: 8 complex rows x 32 complex columns
     {
-      const int r = tid / GBN, c = tid % GBN;
+      const int r = tid / SBN, c = tid % SBN;
       CT v;
       v.x = 0; v.y = 0;
       if (k0 + r < kdim && col0 + c < n) v = B[(k0 + r) * ldb + col0 + c];
@@ -91,16 +223,15 @@ __global__ __launch_bounds__(256) void k_gemm_sub(int64_t m, int64_t n, int64_t 
     __syncthreads();
 #pragma unroll
     for (int ks = 0; ks < 2 * GKC; ks += 4) {
-      const int kr = ks + (lane >> 4);       // real k of this lane
-      const int kc = kr >> 1, kpar = kr & 1; // complex row of B, re/im half
+      const int kr = ks + (lane >> 4);
+      const int kc = kr >> 1, kpar = kr & 1;
       T a[2], b[2];
 #pragma unroll
       for (int i = 0; i < 2; ++i) a[i] = As[wr * 32 + i * 16 + (lane & 15)][kr];
 #pragma unroll
       for (int j = 0; j < 2; ++j) {
-        const int cr = wc * 32 + j * 16 + (lane & 15);  // real column
+        const int cr = wc * 32 + j * 16 + (lane & 15);
         const CT bv = Bs[kc][cr >> 1];
-        // B_r[2l+s][2j+t] = s==0 ? (t==0 ? br : bi) : (t==0 ? -bi : br)
         b[j] = (kpar == 0) ? ((cr & 1) ? bv.y : bv.x) : ((cr & 1) ? bv.x : -bv.y);
       }
 #pragma unroll
@@ -110,8 +241,6 @@ __global__ __launch_bounds__(256) void k_gemm_sub(int64_t m, int64_t n, int64_t 
     }
     __syncthreads();
   }
-  // C_r -= acc
-  T* Cr = reinterpret_cast<T*>(C);
 #pragma unroll
   for (int i = 0; i < 2; ++i)
 #pragma unroll
@@ -119,19 +248,26 @@ __global__ __launch_bounds__(256) void k_gemm_sub(int64_t m, int64_t n, int64_t 
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         const int64_t gr = row0 + wr * 32 + i * 16 + Mfma<T>::row(lane, r);
-        const int64_t gcr = 2 * col0 + wc * 32 + j * 16 + (lane & 15);  // real column
-        if (gr < m && gcr < 2 * n) Cr[gr * 2 * ldc + gcr] -= acc[i][j][r];
+        const int64_t gcr = 2 * col0 + wc * 32 + j * 16 + (lane & 15);
+        if (gr < m && gcr < 2 * n) Cr[gr * 2 * ldc + gcr] = acc[i][j][r];
       }
 }
 
-template <typename T>
+template <typename T, bool TA = false>
 hipError_t launch_gemm_sub(int64_t m, int64_t n, int64_t k, const void* A, int64_t lda,
                            const void* B, int64_t ldb, void* C, int64_t ldc, hipStream_t st) {
   typedef typename cplx_t<T>::type CT;
   if (m <= 0 || n <= 0 || k <= 0) return hipSuccess;
-  dim3 grid((unsigned)((n + GBN - 1) / GBN), (unsigned)((m + GBM - 1) / GBM));
-  hipLaunchKernelGGL(k_gemm_sub<T>, grid, dim3(256), 0, st, m, n, k, (const CT*)A, lda,
-                     (const CT*)B, ldb, (CT*)C, ldc);
+  const int64_t big_tiles = ((n + GBN - 1) / GBN) * ((m + GBM - 1) / GBM);
+  if (big_tiles >= 1024) {
+    dim3 grid((unsigned)((n + GBN - 1) / GBN), (unsigned)((m + GBM - 1) / GBM));
+    hipLaunchKernelGGL((k_gemm_sub<T, TA>), grid, dim3(256), 0, st, m, n, k, (const CT*)A, lda,
+                       (const CT*)B, ldb, (CT*)C, ldc);
+  } else {
+    dim3 grid((unsigned)((n + SBN - 1) / SBN), (unsigned)((m + SBM - 1) / SBM));
+    hipLaunchKernelGGL((k_gemm_sub_s<T, TA>), grid, dim3(256), 0, st, m, n, k, (const CT*)A, lda,
+                       (const CT*)B, ldb, (CT*)C, ldc);
+  }
   return hipGetLastError();
 }
 
@@ -467,8 +603,8 @@ struct TourWs {
 
 template <typename T>
 __global__ __launch_bounds__(256) void k_panel_pivot(typename cplx_t<T>::type* A, int64_t lda,
-                                                     int64_t n, int64_t nc, int64_t k0, int bw,
-                                                     int nleaf,
+                                                     int64_t n, int64_t c_lo, int64_t nc,
+                                                     int64_t k0, int bw, int nleaf,
                                                      TourWs ws, const int32_t* flag,
                                                      typename cplx_t<T>::type* lu11,
                                                      typename cplx_t<T>::type* inv,
@@ -587,9 +723,11 @@ __global__ __launch_bounds__(256) void k_panel_pivot(typename cplx_t<T>::type* A
   for (int nd = nleaf; nd > 1; nd = (nd + FANIN - 1) / FANIN) total += nd;
   total += 1;
   for (int e = tid; e < total; e += blockDim.x) ws.counter[e] = 0;
-  // interchanges on columns [k0, nc): nobody else reads A any more
+  // interchanges on columns [c_lo, nc): from the start of the current outer
+  // block, so the block's multiplier columns end up in the block's final row
+  // order; nobody else reads A any more
   __syncthreads();
-  for (int64_t c = k0 + tid; c < nc; c += blockDim.x) {
+  for (int64_t c = c_lo + tid; c < nc; c += blockDim.x) {
     for (int i = 0; i < bw; ++i) {
       const int64_t p1 = k0 + i, p2 = ipiv[k0 + i];
       if (p2 != p1) {
@@ -698,30 +836,36 @@ __global__ __launch_bounds__(256) void k_panel_apply(int64_t n, int64_t nc, type
   }
 }
 
-// Solve blocks on AW right-hand-side columns per workgroup.
-// fwd: interchanges of the panel, then X_p <- L11^{-1} X_p; bwd: X_p <- U11^{-1} X_p.
+// Solve blocks on AW right-hand-side columns per workgroup:
+// lower != 0: X_p <- L11^{-1} X_p; lower == 0: X_p <- U11^{-1} X_p.
 template <typename T>
 __global__ __launch_bounds__(256) void k_solve_apply(int64_t nrhs, const typename cplx_t<T>::type* inv,
-                                                     int64_t k0, int bw, const int32_t* ipiv,
+                                                     int64_t k0, int bw,
                                                      typename cplx_t<T>::type* X, int64_t ldx,
-                                                     int fwd) {
+                                                     int lower) {
   typedef typename cplx_t<T>::type CT;
   __shared__ ApplyLds<CT> L;
   const int tid = threadIdx.x;
   const int64_t c0 = (int64_t)blockIdx.x * AW;
-  const CT* Tsrc = fwd ? inv : inv + NB * NB;
+  const CT* Tsrc = lower ? inv : inv + NB * NB;
   for (int e = tid; e < NB * NB; e += blockDim.x) L.Tm[e / NB][e % NB] = Tsrc[e];
-  if (fwd && tid < AW && c0 + tid < nrhs) {
-    const int64_t c = c0 + tid;
-    for (int i = 0; i < bw; ++i) {
-      const int64_t p1 = k0 + i, p2 = ipiv[k0 + i];
-      if (p2 != p1) {
-        const CT t = X[p1 * ldx + c]; X[p1 * ldx + c] = X[p2 * ldx + c]; X[p2 * ldx + c] = t;
-      }
-    }
-  }
   __syncthreads();
   apply_left(L, X, ldx, k0, bw, c0, nrhs);
+}
+
+// Row interchanges ipiv[r0 .. r1) applied in order to columns [0, ncols) of X.
+template <typename T>
+__global__ void k_laswp_rows(typename cplx_t<T>::type* X, int64_t ldx, int64_t ncols,
+                             const int32_t* ipiv, int64_t r0, int64_t r1) {
+  typedef typename cplx_t<T>::type CT;
+  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= ncols) return;
+  for (int64_t i = r0; i < r1; ++i) {
+    const int64_t p = ipiv[i];
+    if (p != i) {
+      const CT t = X[i * ldx + c]; X[i * ldx + c] = X[p * ldx + c]; X[p * ldx + c] = t;
+    }
+  }
 }
 
 // Zero a small int32 buffer (used instead of hipMemsetAsync so the whole
@@ -766,6 +910,31 @@ __global__ void k_scatter_cols(int64_t ncols, int maxnz, const int32_t* idx, con
   const int32_t r = idx[e];
   if (r < 0) return;
   X[(int64_t)r * ldx + c] = make_double2(val[e], 0.0);
+}
+
+// Precision changes between the FP64 system and the FP32 factorisation.
+__global__ void k_z2c(int64_t rows, int64_t cols, const double2* src, int64_t lds, float2* dst,
+                      int64_t ldd) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= rows * cols) return;
+  const int64_t r = e / cols, c = e % cols;
+  const double2 v = src[r * lds + c];
+  dst[r * ldd + c] = make_float2((float)v.x, (float)v.y);
+}
+
+__global__ void k_c2z(int64_t rows, int64_t cols, const float2* src, int64_t lds, double2* dst,
+                      int64_t ldd, int accumulate) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= rows * cols) return;
+  const int64_t r = e / cols, c = e % cols;
+  const float2 v = src[r * lds + c];
+  double2 o = make_double2((double)v.x, (double)v.y);
+  if (accumulate) {
+    const double2 d = dst[r * ldd + c];
+    o.x += d.x;
+    o.y += d.y;
+  }
+  dst[r * ldd + c] = o;
 }
 
 // T <- S^{-1} T, one workgroup, Gauss-Jordan with partial pivoting in LDS
@@ -846,9 +1015,18 @@ static void* panel_inv(void* work, int64_t k0) {
 
 static double g_lbound = 8.0;  // threshold-pivoting multiplier bound (1/tau); <= 0: always pivot
 
+// Outer block width: trailing updates use K = nbo.  Small systems are bound
+// by the panel chain rather than by the updates, so they keep one level.
+static int64_t outer_block(int64_t n) { return n > 4096 ? 128 : NB; }
+
 // Factor the n x n block of A; columns [n, nc) are carried along (row
 // interchanges, L^{-1} and trailing updates), so on return they hold
 // L^{-1} P B for right-hand sides B stored there: the forward half of a solve.
+//
+// Two-level blocking: 32-column panels (threshold-pivot check, tournament
+// fallback, apply) factor a 128-column block, updating only that block; the
+// block's U rows for the remaining columns follow by blocked substitution and
+// the rest of the matrix takes one K = 128 update per block.
 template <typename T>
 int mw_getrf_t(int64_t n, int64_t nc, void* Av, int64_t lda, int32_t* ipiv, void* work,
                int32_t* info, const volatile int32_t* cancel, hipStream_t st, hipError_t* herr) {
@@ -868,27 +1046,53 @@ int mw_getrf_t(int64_t n, int64_t nc, void* Av, int64_t lda, int32_t* ipiv, void
                      nzero);
   hipLaunchKernelGGL(k_zero_i32, dim3(1), dim3(64), 0, st, info, (int64_t)1);
   const double lbound = g_lbound;
-  for (int64_t k0 = 0; k0 < n; k0 += NB) {
-    if (cancel && *cancel) return MW_EABORT;
-    const int bw = (int)((n - k0) < NB ? (n - k0) : NB);
-    const int64_t rest = n - k0 - bw;        // rows below the panel
-    const int64_t restc = nc - k0 - bw;      // columns right of the panel
-    const int nrb = (int)((rest + AW - 1) / AW);
-    const int ncb = (int)((restc + AW - 1) / AW);
-    CT* inv = reinterpret_cast<CT*>(panel_inv(work, k0));
-    int32_t* flag = flags + k0 / NB;
-    hipLaunchKernelGGL(k_panel_nopiv<T>, dim3(nrb + 1), dim3(256), 0, st, n, (const CT*)A, lda, k0,
-                       bw, lu11, inv, ipiv, flag, lbound);
-    const int nleaf = (int)((n - k0 + TROWS - 1) / TROWS);
-    hipLaunchKernelGGL(k_panel_pivot<T>, dim3(nleaf), dim3(256), 0, st, A, lda, n, nc, k0, bw,
-                       nleaf, tw, (const int32_t*)flag, lu11, inv, ipiv, info);
-    hipLaunchKernelGGL(k_panel_apply<T>, dim3(nrb + ncb + 1), dim3(256), 0, st, n, nc, A, lda, k0,
-                       bw, (const CT*)inv, (const CT*)lu11, nrb, ncb);
-    if (rest > 0 && restc > 0) {
-      *herr = launch_gemm_sub<T>(rest, restc, bw, A + (k0 + bw) * lda + k0, lda,
-                                 A + k0 * lda + k0 + bw, lda, A + (k0 + bw) * lda + k0 + bw, lda,
-                                 st);
-      if (*herr != hipSuccess) return MW_EHIP;
+  const int64_t nbo = outer_block(n);
+  for (int64_t K0 = 0; K0 < n; K0 += nbo) {
+    const int64_t kend = (n - K0) < nbo ? n : K0 + nbo;
+    for (int64_t k0 = K0; k0 < kend; k0 += NB) {
+      if (cancel && *cancel) return MW_EABORT;
+      const int bw = (int)((kend - k0) < NB ? (kend - k0) : NB);
+      const int64_t rest = n - k0 - bw;         // rows below the panel
+      const int64_t restc = kend - k0 - bw;     // block columns right of the panel
+      const int nrb = (int)((rest + AW - 1) / AW);
+      const int ncb = (int)((restc + AW - 1) / AW);
+      CT* inv = reinterpret_cast<CT*>(panel_inv(work, k0));
+      int32_t* flag = flags + k0 / NB;
+      hipLaunchKernelGGL(k_panel_nopiv<T>, dim3(nrb + 1), dim3(256), 0, st, n, (const CT*)A, lda,
+                         k0, bw, lu11, inv, ipiv, flag, lbound);
+      const int nleaf = (int)((n - k0 + TROWS - 1) / TROWS);
+      hipLaunchKernelGGL(k_panel_pivot<T>, dim3(nleaf), dim3(256), 0, st, A, lda, n, K0, nc, k0,
+                         bw, nleaf, tw, (const int32_t*)flag, lu11, inv, ipiv, info);
+      hipLaunchKernelGGL(k_panel_apply<T>, dim3(nrb + ncb + 1), dim3(256), 0, st, n, kend, A,
+                         lda, k0, bw, (const CT*)inv, (const CT*)lu11, nrb, ncb);
+      if (rest > 0 && restc > 0) {
+        *herr = launch_gemm_sub<T>(rest, restc, bw, A + (k0 + bw) * lda + k0, lda,
+                                   A + k0 * lda + k0 + bw, lda, A + (k0 + bw) * lda + k0 + bw,
+                                   lda, st);
+        if (*herr != hipSuccess) return MW_EHIP;
+      }
+    }
+    const int64_t ncr = nc - kend;  // columns right of the block
+    if (ncr > 0) {
+      CT* X = A + kend;  // U rows of the block, columns [kend, nc)
+      const unsigned gx = (unsigned)((ncr + AW - 1) / AW);
+      for (int64_t k0 = K0; k0 < kend; k0 += NB) {
+        const int bw = (int)((kend - k0) < NB ? (kend - k0) : NB);
+        const CT* inv = reinterpret_cast<const CT*>(panel_inv(work, k0));
+        hipLaunchKernelGGL(k_solve_apply<T>, dim3(gx), dim3(256), 0, st, ncr, inv, k0, bw, X,
+                           lda, 1);
+        const int64_t below = kend - k0 - bw;
+        if (below > 0) {
+          *herr = launch_gemm_sub<T>(below, ncr, bw, A + (k0 + bw) * lda + k0, lda, X + k0 * lda,
+                                     lda, X + (k0 + bw) * lda, lda, st);
+          if (*herr != hipSuccess) return MW_EHIP;
+        }
+      }
+      if (n > kend) {
+        *herr = launch_gemm_sub<T>(n - kend, ncr, kend - K0, A + kend * lda + K0, lda,
+                                   A + K0 * lda + kend, lda, A + kend * lda + kend, lda, st);
+        if (*herr != hipSuccess) return MW_EHIP;
+      }
     }
     *herr = hipGetLastError();
     if (*herr != hipSuccess) return MW_EHIP;
@@ -905,16 +1109,23 @@ static int solve_fwd(int64_t n, int64_t nrhs, const typename cplx_t<T>::type* A,
                      int64_t ldx, hipStream_t st, hipError_t* herr) {
   typedef typename cplx_t<T>::type CT;
   const unsigned gx = (unsigned)((nrhs + AW - 1) / AW);
-  for (int64_t k0 = 0; k0 < n; k0 += NB) {
-    const int bw = (int)((n - k0) < NB ? (n - k0) : NB);
-    const CT* inv = reinterpret_cast<const CT*>(panel_inv(const_cast<void*>(work), k0));
-    hipLaunchKernelGGL(k_solve_apply<T>, dim3(gx), dim3(256), 0, st, nrhs, inv, k0, bw, ipiv, X,
-                       ldx, 1);
-    const int64_t rest = n - k0 - bw;
-    if (rest > 0) {
-      *herr = launch_gemm_sub<T>(rest, nrhs, bw, A + (k0 + bw) * lda + k0, lda, X + k0 * ldx,
-                                 ldx, X + (k0 + bw) * ldx, ldx, st);
-      if (*herr != hipSuccess) return MW_EHIP;
+  const int64_t nbo = outer_block(n);
+  for (int64_t K0 = 0; K0 < n; K0 += nbo) {
+    const int64_t kend = (n - K0) < nbo ? n : K0 + nbo;
+    // a block's multiplier columns are stored in the block's final row order
+    hipLaunchKernelGGL(k_laswp_rows<T>, dim3((unsigned)((nrhs + 255) / 256)), dim3(256), 0, st, X,
+                       ldx, nrhs, ipiv, K0, kend);
+    for (int64_t k0 = K0; k0 < kend; k0 += NB) {
+      const int bw = (int)((kend - k0) < NB ? (kend - k0) : NB);
+      const CT* inv = reinterpret_cast<const CT*>(panel_inv(const_cast<void*>(work), k0));
+      hipLaunchKernelGGL(k_solve_apply<T>, dim3(gx), dim3(256), 0, st, nrhs, inv, k0, bw, X, ldx,
+                         1);
+      const int64_t rest = n - k0 - bw;
+      if (rest > 0) {
+        *herr = launch_gemm_sub<T>(rest, nrhs, bw, A + (k0 + bw) * lda + k0, lda, X + k0 * ldx,
+                                   ldx, X + (k0 + bw) * ldx, ldx, st);
+        if (*herr != hipSuccess) return MW_EHIP;
+      }
     }
   }
   return MW_OK;
@@ -930,8 +1141,8 @@ static int solve_bwd(int64_t n, int64_t nrhs, const typename cplx_t<T>::type* A,
   for (int64_t k0 = nlast; k0 >= 0; k0 -= NB) {
     const int bw = (int)((n - k0) < NB ? (n - k0) : NB);
     const CT* inv = reinterpret_cast<const CT*>(panel_inv(const_cast<void*>(work), k0));
-    hipLaunchKernelGGL(k_solve_apply<T>, dim3(gx), dim3(256), 0, st, nrhs, inv, k0, bw,
-                       (const int32_t*)nullptr, X, ldx, 0);
+    hipLaunchKernelGGL(k_solve_apply<T>, dim3(gx), dim3(256), 0, st, nrhs, inv, k0, bw, X, ldx,
+                       0);
     if (k0 > 0) {
       *herr = launch_gemm_sub<T>(k0, nrhs, bw, A + k0, lda, X + k0 * ldx, ldx, X, ldx, st);
       if (*herr != hipSuccess) return MW_EHIP;
@@ -992,6 +1203,10 @@ hipError_t mw_gemm_sub_c(int64_t m, int64_t n, int64_t k, const void* A, int64_t
                          const void* B, int64_t ldb, void* C, int64_t ldc, hipStream_t st) {
   return launch_gemm_sub<float>(m, n, k, A, lda, B, ldb, C, ldc, st);
 }
+hipError_t mw_gemm_tn_add_z(int64_t m, int64_t n, int64_t k, const void* A, int64_t lda,
+                            const void* B, int64_t ldb, void* C, int64_t ldc, hipStream_t st) {
+  return launch_gemm_sub<double, true>(m, n, k, A, lda, B, ldb, C, ldc, st);
+}
 
 hipError_t mw_launch_sparse_rows(int64_t nrows, int maxnz, const int32_t* idx, const double* val,
                                  const void* X, int64_t ldx, int64_t ncols, void* out,
@@ -1019,5 +1234,23 @@ hipError_t mw_launch_small_solve(int ns, int nt, void* S, int64_t lds, void* T, 
                                  int32_t* info, hipStream_t st) {
   hipLaunchKernelGGL(k_small_solve, dim3(1), dim3(256), 0, st, ns, nt, (double2*)S, lds,
                      (double2*)T, ldt, info);
+  return hipGetLastError();
+}
+
+hipError_t mw_launch_z2c(int64_t rows, int64_t cols, const void* src, int64_t lds, void* dst,
+                         int64_t ldd, hipStream_t st) {
+  if (rows <= 0 || cols <= 0) return hipSuccess;
+  const int64_t tot = rows * cols;
+  hipLaunchKernelGGL(k_z2c, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st, rows, cols,
+                     (const double2*)src, lds, (float2*)dst, ldd);
+  return hipGetLastError();
+}
+
+hipError_t mw_launch_c2z(int64_t rows, int64_t cols, const void* src, int64_t lds, void* dst,
+                         int64_t ldd, int accumulate, hipStream_t st) {
+  if (rows <= 0 || cols <= 0) return hipSuccess;
+  const int64_t tot = rows * cols;
+  hipLaunchKernelGGL(k_c2z, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st, rows, cols,
+                     (const float2*)src, lds, (double2*)dst, ldd, accumulate);
   return hipGetLastError();
 }

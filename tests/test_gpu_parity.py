@@ -263,3 +263,64 @@ def test_cancel_token_stops_factorisation():
     tok.reset()
     g = golden("array2x2")
     assert rel_err(s.compute_port_solution().y, g["y"]) < 1e-10
+
+
+@pytest.mark.parametrize("dtype", ["z", "c"])
+def test_two_level_lu_large(dtype):
+    """n > 4096 takes the two-level blocking (128-column blocks, K = 128
+    trailing updates); random entries exercise the tournament fallback and a
+    dominant diagonal the threshold path."""
+    for dom in (0.0, 1.0):
+        n, p = 4500, 24
+        _say("two-level lu", dtype, n, "diag-dominant" if dom else "random")
+        rng = np.random.default_rng(11 + int(dom))
+        M = rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))
+        M += dom * 8 * np.sqrt(n) * np.eye(n)
+        RHS = rng.standard_normal((n, p)) + 1j * rng.standard_normal((n, p))
+        X, info = _lu_solve(M, RHS, dtype)
+        assert info == 0
+        ref = np.linalg.solve(M, RHS)
+        eps = 1.1e-16 if dtype == "z" else 6e-8
+        assert rel_err(X, ref) < eps * np.linalg.cond(M) * 200
+
+
+@pytest.mark.parametrize("name,kw", [("array2x2", W.dipole_array(2, 2, 11, seed=5)),
+                                     ("tee_jport", W.junction_tee("junction_port")),
+                                     ("c2_array8x8", W.dipole_array(8, 8, 41, seed=1))])
+def test_mixed_precision_refines_to_fp64(name, kw):
+    """FP32 factorisation + FP64-residual refinement reaches the FP64 answer:
+    Y within 1e-8 of the reference golden after the default two corrections,
+    and the error shrinks with every correction."""
+    _say("mixed", name)
+    g = golden(name)
+    s = BSplineSolver(**kw, precision="mixed", refine_steps=3)
+    hist = []
+    s._solve_device_mixed(history=hist)
+    plain = [h for h in hist if not isinstance(h, tuple)]
+    adj = [h[1] for h in hist if isinstance(h, tuple)]
+    errs = [rel_err(h.cpu().numpy(), g["y"]) for h in plain]
+    aerrs = [rel_err(h.cpu().numpy(), g["y"]) for h in adj]
+    _say("mixed Y errors per step", name, ["%.2e" % e for e in errs],
+         "adjoint-corrected", ["%.2e" % e for e in aerrs])
+    assert errs[0] < 1e-2
+    assert errs[-1] < 1e-8
+    assert errs[2] <= errs[1] <= errs[0] or errs[2] < 1e-12
+    # default: one correction plus the adjoint-corrected readout (three
+    # corrections when KCL rows are present)
+    sol = BSplineSolver(**kw, precision="mixed").compute_port_solution()
+    assert rel_err(sol.y, g["y"]) < 1e-8
+    assert rel_err(sol.y, sol.y.T) < 1e-9
+
+
+def test_gemm_tn_add():
+    _say("gemm tn add")
+    rng = np.random.default_rng(4)
+    for m, n, k in [(64, 64, 2624), (200, 130, 517), (1024, 1024, 700)]:
+        A = rng.standard_normal((k, m)) + 1j * rng.standard_normal((k, m))
+        B = rng.standard_normal((k, n)) + 1j * rng.standard_normal((k, n))
+        C = rng.standard_normal((m, n)) + 1j * rng.standard_normal((m, n))
+        dA, dB, dC = _dev(A), _dev(B), _dev(C)
+        _lib.check(_lib.load().mw_zgemm_tn_add(m, n, k, dA.data_ptr(), m, dB.data_ptr(), n,
+                                               dC.data_ptr(), n, _stream()), "tn")
+        torch.cuda.synchronize()
+        assert rel_err(dC.cpu().numpy(), C + A.T @ B) < 1e-13
