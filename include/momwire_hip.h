@@ -72,6 +72,46 @@ typedef struct mw_mesh {
                               Gauss-Legendre x, w on [-1,1] (far static rule) */
 } mw_mesh;
 
+/*
+ * One problem of a batched sweep: a structure (mesh index) at one wavenumber.
+ * A is the problem's [Z | B] buffer (n x (n + ncols) complex double, row
+ * stride lda); the port columns and KCL rows come from the mesh's port
+ * tables.  All pointers are device pointers.
+ */
+typedef struct mw_sweep_prob {
+  int32_t mesh;           /* index into the mesh / port tables               */
+  int32_t symmetric;      /* every wire of the mesh shares one radius        */
+  double k, omega;        /* wavenumber, angular frequency                   */
+  void* A;                /* [Z | B], written by the fill and the solve      */
+  int64_t lda;
+  int64_t n;              /* unknowns                                        */
+  int64_t ncols;          /* port columns + KCL constraint columns           */
+  void* se_tab;           /* same-edge table of this (mesh, k): [n_se][9]    */
+  const int32_t* tile_pairs;  /* the mesh's (ti, tj) list                     */
+  int32_t n_pairs;
+  int32_t n_se;
+  int32_t* ipiv;          /* [n]                                             */
+  void* work;             /* mw_getrf_workspace_bytes(n) bytes               */
+  int32_t* info;          /* [2]: factorisation, KCL Schur                   */
+  void* Y;                /* [n_ports][n_ports] output                       */
+  void* S;                /* [ncon][ncon] scratch                            */
+  void* T;                /* [ncon][n_ports] scratch                         */
+} mw_sweep_prob;
+
+/* Sparse port columns and KCL rows of one mesh (device pointers). */
+typedef struct mw_ports {
+  int32_t n_ports, n_con;
+  int32_t col_max, con_max, port_max, pad;
+  const int32_t* col_idx;  /* [n_ports + n_con][col_max]: right-hand sides  */
+  const double* col_val;
+  const int32_t* con_idx;  /* [n_con][con_max]: KCL constraint rows         */
+  const double* con_val;
+  const int32_t* port_idx; /* [n_ports][port_max]: readout rows (= columns) */
+  const double* port_val;
+} mw_ports;
+
+typedef struct mw_sweep_plan mw_sweep_plan;
+
 /* Last error message of the calling thread ("" if none). */
 const char* mw_last_error(void);
 /* ABI version of this library. */
@@ -188,6 +228,26 @@ int mw_zscatter_cols(int64_t n, int64_t ncols, int32_t maxnz, const int32_t* idx
  */
 int mw_zsmall_solve(int32_t ns, int32_t nt, void* S, int64_t lds, void* T,
                     int64_t ldt, int32_t* info, void* stream);
+
+/*
+ * Batched sweep over many (structure, wavenumber) problems of uneven size:
+ * plan once (copies the descriptors to the device, sorts problems by size
+ * and fixes the launch schedule), then run: the same-edge tables and fills
+ * of every problem, the right-hand sides, the factorisations with the
+ * forward solve riding along, the back substitutions, the KCL Schur steps
+ * and the Y readouts -- a fixed sequence of batched launches (about 4 per
+ * 32-column panel of the largest problem), so the whole sweep can be
+ * captured as one HIP graph.  No counterpart in the reference, which sweeps
+ * one structure at a time (bspline.py:4181-4439); per problem it computes
+ * what compute_port_solution computes.  gemm_variant: 0 = 64x32 tiles,
+ * 1 = 128x64 tiles for the batched updates (chosen by the autotuner).
+ */
+int mw_sweep_plan_create(int32_t n_mesh, const mw_mesh* meshes, const mw_ports* ports,
+                         int32_t nprob, const mw_sweep_prob* probs, int32_t gemm_variant,
+                         mw_sweep_plan** plan);
+int mw_sweep_run(mw_sweep_plan* plan, double eps, double mu,
+                 const volatile int32_t* cancel, void* stream);
+int mw_sweep_plan_destroy(mw_sweep_plan* plan);
 
 /*
  * Precision changes for the mixed form (FP32 factorisation, FP64 residual):

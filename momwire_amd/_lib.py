@@ -55,6 +55,29 @@ class MwMesh(ctypes.Structure):
     ]
 
 
+class MwPorts(ctypes.Structure):
+    _fields_ = [
+        ("n_ports", ctypes.c_int32), ("n_con", ctypes.c_int32),
+        ("col_max", ctypes.c_int32), ("con_max", ctypes.c_int32),
+        ("port_max", ctypes.c_int32), ("pad", ctypes.c_int32),
+        ("col_idx", ctypes.c_void_p), ("col_val", ctypes.c_void_p),
+        ("con_idx", ctypes.c_void_p), ("con_val", ctypes.c_void_p),
+        ("port_idx", ctypes.c_void_p), ("port_val", ctypes.c_void_p),
+    ]
+
+
+class MwSweepProb(ctypes.Structure):
+    _fields_ = [
+        ("mesh", ctypes.c_int32), ("symmetric", ctypes.c_int32),
+        ("k", ctypes.c_double), ("omega", ctypes.c_double),
+        ("A", ctypes.c_void_p), ("lda", ctypes.c_int64), ("n", ctypes.c_int64),
+        ("ncols", ctypes.c_int64), ("se_tab", ctypes.c_void_p),
+        ("tile_pairs", ctypes.c_void_p), ("n_pairs", ctypes.c_int32), ("n_se", ctypes.c_int32),
+        ("ipiv", ctypes.c_void_p), ("work", ctypes.c_void_p), ("info", ctypes.c_void_p),
+        ("Y", ctypes.c_void_p), ("S", ctypes.c_void_p), ("T", ctypes.c_void_p),
+    ]
+
+
 _P = ctypes.c_void_p
 _I64 = ctypes.c_int64
 _I32 = ctypes.c_int32
@@ -82,6 +105,10 @@ SIGNATURES = {
     "mw_zsparse_rows": (ctypes.c_int, [_I64, _I32, _P, _P, _P, _I64, _I64, _P, _I64, _P]),
     "mw_zscatter_cols": (ctypes.c_int, [_I64, _I64, _I32, _P, _P, _P, _I64, _P]),
     "mw_zsmall_solve": (ctypes.c_int, [_I32, _I32, _P, _I64, _P, _I64, _P, _P]),
+    "mw_sweep_plan_create": (ctypes.c_int, [_I32, _P, _P, _I32, _P, _I32,
+                                            ctypes.POINTER(ctypes.c_void_p)]),
+    "mw_sweep_run": (ctypes.c_int, [_P, _D, _D, _P, _P]),
+    "mw_sweep_plan_destroy": (ctypes.c_int, [_P]),
     "mw_z2c": (ctypes.c_int, [_I64, _I64, _P, _I64, _P, _I64, _P]),
     "mw_c2z": (ctypes.c_int, [_I64, _I64, _P, _I64, _P, _I64, _I32, _P]),
 }

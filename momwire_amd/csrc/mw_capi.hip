@@ -2,6 +2,9 @@
 #include <stdio.h>
 #include <string.h>
 
+#include <algorithm>
+#include <vector>
+
 #include "mw_common.h"
 
 hipError_t mw_launch_same_edge(const mw_mesh& M, double k, hipStream_t st);
@@ -35,6 +38,13 @@ hipError_t mw_launch_c2z(int64_t rows, int64_t cols, const void* src, int64_t ld
                          int64_t ldd, int accumulate, hipStream_t st);
 void mw_set_pivot_bound(double lbound);
 double mw_get_pivot_bound();
+hipError_t mw_launch_batch_fill(const mw_sweep_prob* P, const mw_mesh* meshes, int nprob,
+                                int max_nse, const int32_t* item_prob, const int32_t* item_pair,
+                                int64_t n_items, double eps, double mu, hipStream_t st);
+int mw_sweep_lu_run(const mw_sweep_prob* dP, const mw_ports* dports, const mw_sweep_prob* hp,
+                    int nprob, int max_ports, int max_con, int64_t max_scatter,
+                    int gemm_variant, const volatile int32_t* cancel, hipStream_t st,
+                    hipError_t* herr);
 hipError_t mw_launch_small_solve(int ns, int nt, void* S, int64_t lds, void* T, int64_t ldt,
                                  int32_t* info, hipStream_t st);
 
@@ -254,6 +264,109 @@ int mw_c2z(int64_t rows, int64_t cols, const void* src, int64_t lds, void* dst, 
   if (lds < cols || ldd < cols) return fail(MW_EINVAL, "%sbad leading dimension");
   return hip_status(mw_launch_c2z(rows, cols, src, lds, dst, ldd, accumulate, (hipStream_t)stream),
                     "mw_c2z");
+}
+
+struct mw_sweep_plan {
+  int nprob = 0, n_mesh = 0, gemm_variant = 0, max_ports = 0, max_con = 0, max_nse = 0;
+  int64_t max_scatter = 0, n_items = 0;
+  mw_sweep_prob* dP = nullptr;
+  mw_ports* dports = nullptr;
+  mw_mesh* dmeshes = nullptr;
+  int32_t* d_item_prob = nullptr;
+  int32_t* d_item_pair = nullptr;
+  std::vector<mw_sweep_prob> hp;
+};
+
+static void plan_free(mw_sweep_plan* p) {
+  if (!p) return;
+  hipFree(p->dP);
+  hipFree(p->dports);
+  hipFree(p->dmeshes);
+  hipFree(p->d_item_prob);
+  hipFree(p->d_item_pair);
+  delete p;
+}
+
+int mw_sweep_plan_create(int32_t n_mesh, const mw_mesh* meshes, const mw_ports* ports,
+                         int32_t nprob, const mw_sweep_prob* probs, int32_t gemm_variant,
+                         mw_sweep_plan** out) {
+  g_err[0] = 0;
+  if (!out) return fail(MW_EINVAL, "%splan output is NULL");
+  *out = nullptr;
+  if (n_mesh <= 0 || nprob <= 0 || !meshes || !ports || !probs)
+    return fail(MW_EINVAL, "%sempty sweep");
+  for (int i = 0; i < n_mesh; ++i) {
+    int rc = check_mesh(&meshes[i]);
+    if (rc) return rc;
+    if (ports[i].n_ports < 1 || ports[i].n_ports > 32 || ports[i].n_con < 0 ||
+        ports[i].n_con > 32 || ports[i].n_ports + ports[i].n_con > 64)
+      return fail(MW_ERANGE, "%smesh %lld: ports/constraints beyond the batched limits", "", i);
+  }
+  mw_sweep_plan* p = new mw_sweep_plan();
+  p->nprob = nprob;
+  p->n_mesh = n_mesh;
+  p->gemm_variant = gemm_variant;
+  p->hp.assign(probs, probs + nprob);
+  for (const auto& q : p->hp) {
+    if (q.mesh < 0 || q.mesh >= n_mesh || q.n != meshes[q.mesh].n_basis ||
+        q.ncols != ports[q.mesh].n_ports + ports[q.mesh].n_con || q.lda < q.n + q.ncols ||
+        q.n > 4096 || !q.A || !q.ipiv || !q.work || !q.info || !q.Y || !q.se_tab || !q.tile_pairs) {
+      plan_free(p);
+      return fail(MW_EINVAL, "%sinconsistent sweep problem (mesh %lld, n %lld)", "", q.mesh, q.n);
+    }
+    const mw_ports& pt = ports[q.mesh];
+    p->max_ports = std::max(p->max_ports, pt.n_ports);
+    p->max_con = std::max(p->max_con, pt.n_con);
+    p->max_scatter = std::max<int64_t>(p->max_scatter, (int64_t)q.ncols * pt.col_max);
+    p->max_nse = std::max(p->max_nse, meshes[q.mesh].n_se);
+    p->n_items += q.n_pairs;
+  }
+  std::stable_sort(p->hp.begin(), p->hp.end(),
+                   [](const mw_sweep_prob& a, const mw_sweep_prob& b) { return a.n > b.n; });
+  std::vector<int32_t> ip, iq;
+  ip.reserve(p->n_items);
+  iq.reserve(p->n_items);
+  for (int i = 0; i < nprob; ++i)
+    for (int j = 0; j < p->hp[i].n_pairs; ++j) {
+      ip.push_back(i);
+      iq.push_back(j);
+    }
+  hipError_t e = hipSuccess;
+  if ((e = hipMalloc(&p->dP, sizeof(mw_sweep_prob) * nprob)) != hipSuccess ||
+      (e = hipMalloc(&p->dports, sizeof(mw_ports) * n_mesh)) != hipSuccess ||
+      (e = hipMalloc(&p->dmeshes, sizeof(mw_mesh) * n_mesh)) != hipSuccess ||
+      (e = hipMalloc(&p->d_item_prob, sizeof(int32_t) * std::max<int64_t>(p->n_items, 1))) != hipSuccess ||
+      (e = hipMalloc(&p->d_item_pair, sizeof(int32_t) * std::max<int64_t>(p->n_items, 1))) != hipSuccess ||
+      (e = hipMemcpy(p->dP, p->hp.data(), sizeof(mw_sweep_prob) * nprob, hipMemcpyHostToDevice)) != hipSuccess ||
+      (e = hipMemcpy(p->dports, ports, sizeof(mw_ports) * n_mesh, hipMemcpyHostToDevice)) != hipSuccess ||
+      (e = hipMemcpy(p->dmeshes, meshes, sizeof(mw_mesh) * n_mesh, hipMemcpyHostToDevice)) != hipSuccess ||
+      (e = hipMemcpy(p->d_item_prob, ip.data(), sizeof(int32_t) * ip.size(), hipMemcpyHostToDevice)) != hipSuccess ||
+      (e = hipMemcpy(p->d_item_pair, iq.data(), sizeof(int32_t) * iq.size(), hipMemcpyHostToDevice)) != hipSuccess) {
+    plan_free(p);
+    return hip_status(e, "mw_sweep_plan_create");
+  }
+  *out = p;
+  return MW_OK;
+}
+
+int mw_sweep_run(mw_sweep_plan* p, double eps, double mu, const volatile int32_t* cancel,
+                 void* stream) {
+  g_err[0] = 0;
+  if (!p) return fail(MW_EINVAL, "%splan is NULL");
+  hipStream_t st = (hipStream_t)stream;
+  hipError_t herr = mw_launch_batch_fill(p->dP, p->dmeshes, p->nprob, p->max_nse, p->d_item_prob,
+                                         p->d_item_pair, p->n_items, eps, mu, st);
+  if (herr != hipSuccess) return hip_status(herr, "sweep fill");
+  int rc = mw_sweep_lu_run(p->dP, p->dports, p->hp.data(), p->nprob, p->max_ports, p->max_con,
+                           p->max_scatter, p->gemm_variant, cancel, st, &herr);
+  if (rc == MW_EHIP) return hip_status(herr, "sweep solve");
+  if (rc == MW_EABORT) return fail(MW_EABORT, "%ssweep cancelled");
+  return rc;
+}
+
+int mw_sweep_plan_destroy(mw_sweep_plan* p) {
+  plan_free(p);
+  return MW_OK;
 }
 
 }  // extern "C"

@@ -113,8 +113,7 @@ MW_DEV double static_unit(int p, int q, int delta, double ah, const double* glx,
   return total;
 }
 
-__global__ __launch_bounds__(256) void k_same_edge(mw_mesh M, double k) {
-  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+MW_DEV void same_edge_body(int e, const mw_mesh& M, double k) {
   if (e >= M.n_se) return;
   const int ed = M.se_edge[e];
   const int n_e = M.edge_i[2 * ed], off = M.edge_i[2 * ed + 1];
@@ -165,6 +164,10 @@ __global__ __launch_bounds__(256) void k_same_edge(mw_mesh M, double k) {
     for (int q = 0; q < NM; ++q) out[p * NM + q] = make_double2(jr[p][q], ji[p][q]);
 }
 
+__global__ __launch_bounds__(256) void k_same_edge(mw_mesh M, double k) {
+  same_edge_body(blockIdx.x * blockDim.x + threadIdx.x, M, k);
+}
+
 // ---------------------------------------------------------------------------
 // Fused tile fill
 // ---------------------------------------------------------------------------
@@ -196,21 +199,12 @@ MW_DEV void stage_segment(const mw_mesh& M, int s, SegStage& S) {
 }
 
 template <bool SYM>
-__global__ __launch_bounds__(256) void k_fill(mw_mesh M, double k, double wmu,
-                                              double inv_weps, double2* __restrict__ Z,
-                                              int64_t ldz, const int32_t* __restrict__ pairs) {
+MW_DEV void fill_body(int ti, int tj, const mw_mesh& M, double k, double wmu, double inv_weps,
+                      double2* __restrict__ Z, int64_t ldz) {
   __shared__ SegStage rs[MW_TSMAX], cs[MW_TSMAX];
   __shared__ double2 Bt[MW_TSMAX * MW_TSMAX * 9];
   __shared__ int nr_s, nc_s;
 
-  int ti, tj;
-  if (SYM) {
-    ti = pairs[2 * blockIdx.x];
-    tj = pairs[2 * blockIdx.x + 1];
-  } else {
-    ti = blockIdx.y;
-    tj = blockIdx.x;
-  }
   const int tid = threadIdx.x;
   const int nr = M.tile_nseg[ti], nc = M.tile_nseg[tj];
   if (tid < nr) stage_segment(M, M.tile_seg[ti * MW_TSMAX + tid], rs[tid]);
@@ -346,6 +340,45 @@ __global__ __launch_bounds__(256) void k_fill(mw_mesh M, double k, double wmu,
   }
 }
 
+template <bool SYM>
+__global__ __launch_bounds__(256) void k_fill(mw_mesh M, double k, double wmu,
+                                              double inv_weps, double2* __restrict__ Z,
+                                              int64_t ldz, const int32_t* __restrict__ pairs) {
+  if (SYM)
+    fill_body<true>(pairs[2 * blockIdx.x], pairs[2 * blockIdx.x + 1], M, k, wmu, inv_weps, Z, ldz);
+  else
+    fill_body<false>(blockIdx.y, blockIdx.x, M, k, wmu, inv_weps, Z, ldz);
+}
+
+// ---------------------------------------------------------------------------
+// Batched fill over many (structure, wavenumber) problems
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void kb_same_edge(const mw_sweep_prob* P, const mw_mesh* meshes) {
+  const mw_sweep_prob& p = P[blockIdx.y];
+  mw_mesh M = meshes[p.mesh];
+  M.se_tab = p.se_tab;
+  same_edge_body(blockIdx.x * blockDim.x + threadIdx.x, M, p.k);
+}
+
+// one workgroup per (problem, tile pair) work item
+__global__ __launch_bounds__(256) void kb_fill(const mw_sweep_prob* P, const mw_mesh* meshes,
+                                               const int32_t* __restrict__ item_prob,
+                                               const int32_t* __restrict__ item_pair,
+                                               double eps, double mu) {
+  const mw_sweep_prob& p = P[item_prob[blockIdx.x]];
+  mw_mesh M = meshes[p.mesh];
+  M.se_tab = p.se_tab;
+  const int32_t* pairs = p.tile_pairs;
+  const int q = item_pair[blockIdx.x];
+  const double wmu = p.omega * mu, inv_weps = 1.0 / (p.omega * eps);
+  if (p.symmetric)
+    fill_body<true>(pairs[2 * q], pairs[2 * q + 1], M, p.k, wmu, inv_weps,
+                    reinterpret_cast<double2*>(p.A), p.lda);
+  else
+    fill_body<false>(pairs[2 * q], pairs[2 * q + 1], M, p.k, wmu, inv_weps,
+                     reinterpret_cast<double2*>(p.A), p.lda);
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------------------
@@ -369,5 +402,18 @@ hipError_t mw_launch_fill(const mw_mesh& M, double k, double omega, double eps, 
     hipLaunchKernelGGL(k_fill<false>, dim3(M.n_tile, M.n_tile), dim3(256), 0, st, M, k, wmu,
                        inv_weps, Z, ldz, pairs);
   }
+  return hipGetLastError();
+}
+
+hipError_t mw_launch_batch_fill(const mw_sweep_prob* P, const mw_mesh* meshes, int nprob,
+                                int max_nse, const int32_t* item_prob, const int32_t* item_pair,
+                                int64_t n_items, double eps, double mu, hipStream_t st) {
+  if (nprob <= 0) return hipSuccess;
+  if (max_nse > 0)
+    hipLaunchKernelGGL(kb_same_edge, dim3((max_nse + 255) / 256, nprob), dim3(256), 0, st, P,
+                       meshes);
+  if (n_items > 0)
+    hipLaunchKernelGGL(kb_fill, dim3((unsigned)n_items), dim3(256), 0, st, P, meshes, item_prob,
+                       item_pair, eps, mu);
   return hipGetLastError();
 }

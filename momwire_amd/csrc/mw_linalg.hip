@@ -61,20 +61,18 @@ struct GemmLds {
 
 // TA = false: C <- C - A B.  TA = true: C <- C + A^T B with A stored k x m.
 template <typename T, bool TA>
-__global__ __launch_bounds__(256) void k_gemm_sub(int64_t m, int64_t n, int64_t kdim,
-                                                  const typename cplx_t<T>::type* __restrict__ A,
-                                                  int64_t lda,
-                                                  const typename cplx_t<T>::type* __restrict__ B,
-                                                  int64_t ldb, typename cplx_t<T>::type* C,
-                                                  int64_t ldc) {
+MW_DEV void gemm_big_body(int bx, int by, int64_t m, int64_t n, int64_t kdim,
+                          const typename cplx_t<T>::type* __restrict__ A, int64_t lda,
+                          const typename cplx_t<T>::type* __restrict__ B, int64_t ldb,
+                          typename cplx_t<T>::type* C, int64_t ldc) {
   typedef typename cplx_t<T>::type CT;
   typedef typename Mfma<T>::acc_t acc_t;
   __shared__ GemmLds<T> L;
 
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wr = wave >> 1, wc = wave & 1;  // wave's 64x64 (real) quadrant
-  const int64_t row0 = (int64_t)blockIdx.y * GBM;
-  const int64_t col0 = (int64_t)blockIdx.x * GBN;  // complex columns
+  const int64_t row0 = (int64_t)by * GBM;
+  const int64_t col0 = (int64_t)bx * GBN;  // complex columns
   T* Cr = reinterpret_cast<T*>(C);
 
   // accumulators <- C
@@ -172,25 +170,33 @@ __global__ __launch_bounds__(256) void k_gemm_sub(int64_t m, int64_t n, int64_t 
       }
 }
 
+template <typename T, bool TA>
+__global__ __launch_bounds__(256) void k_gemm_sub(int64_t m, int64_t n, int64_t kdim,
+                                                  const typename cplx_t<T>::type* __restrict__ A,
+                                                  int64_t lda,
+                                                  const typename cplx_t<T>::type* __restrict__ B,
+                                                  int64_t ldb, typename cplx_t<T>::type* C,
+                                                  int64_t ldc) {
+  gemm_big_body<T, TA>(blockIdx.x, blockIdx.y, m, n, kdim, A, lda, B, ldb, C, ldc);
+}
+
 // Small-tile variant (64 rows x 32 complex columns, each wave 2 x 2 MFMA
 // tiles) for products too small to occupy the chip with 128 x 64 tiles.
 constexpr int SBM = 64, SBN = 32;
 
 template <typename T, bool TA>
-__global__ __launch_bounds__(256) void k_gemm_sub_s(int64_t m, int64_t n, int64_t kdim,
-                                                    const typename cplx_t<T>::type* __restrict__ A,
-                                                    int64_t lda,
-                                                    const typename cplx_t<T>::type* __restrict__ B,
-                                                    int64_t ldb, typename cplx_t<T>::type* C,
-                                                    int64_t ldc) {
+MW_DEV void gemm_small_body(int bx, int by, int64_t m, int64_t n, int64_t kdim,
+                            const typename cplx_t<T>::type* __restrict__ A, int64_t lda,
+                            const typename cplx_t<T>::type* __restrict__ B, int64_t ldb,
+                            typename cplx_t<T>::type* C, int64_t ldc) {
   typedef typename cplx_t<T>::type CT;
   typedef typename Mfma<T>::acc_t acc_t;
   __shared__ T As[SBM][2 * GKC + 1];
   __shared__ CT Bs[GKC][SBN + 1];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wr = wave >> 1, wc = wave & 1;
-  const int64_t row0 = (int64_t)blockIdx.y * SBM;
-  const int64_t col0 = (int64_t)blockIdx.x * SBN;
+  const int64_t row0 = (int64_t)by * SBM;
+  const int64_t col0 = (int64_t)bx * SBN;
   T* Cr = reinterpret_cast<T*>(C);
   acc_t acc[2][2];
 #pragma unroll
@@ -251,6 +257,16 @@ __global__ __launch_bounds__(256) void k_gemm_sub_s(int64_t m, int64_t n, int64_
         const int64_t gcr = 2 * col0 + wc * 32 + j * 16 + (lane & 15);
         if (gr < m && gcr < 2 * n) Cr[gr * 2 * ldc + gcr] = acc[i][j][r];
       }
+}
+
+template <typename T, bool TA>
+__global__ __launch_bounds__(256) void k_gemm_sub_s(int64_t m, int64_t n, int64_t kdim,
+                                                    const typename cplx_t<T>::type* __restrict__ A,
+                                                    int64_t lda,
+                                                    const typename cplx_t<T>::type* __restrict__ B,
+                                                    int64_t ldb, typename cplx_t<T>::type* C,
+                                                    int64_t ldc) {
+  gemm_small_body<T, TA>(blockIdx.x, blockIdx.y, m, n, kdim, A, lda, B, ldb, C, ldc);
 }
 
 template <typename T, bool TA = false>
@@ -510,11 +526,10 @@ MW_DEV void store_factors(const PanelLds<CT>& P, int bw, CT* lu11, CT* inv) {
 // violated bound (or a zero pivot) sets flag for the tournament fallback.
 // Block 0 stores L11\U11, the inverses and identity interchanges.
 template <typename T>
-__global__ __launch_bounds__(256) void k_panel_nopiv(int64_t n, const typename cplx_t<T>::type* A,
-                                                     int64_t lda, int64_t k0, int bw,
-                                                     typename cplx_t<T>::type* lu11,
-                                                     typename cplx_t<T>::type* inv,
-                                                     int32_t* ipiv, int32_t* flag, double lbound) {
+MW_DEV void panel_nopiv_body(int b, int64_t n, const typename cplx_t<T>::type* A, int64_t lda,
+                             int64_t k0, int bw, typename cplx_t<T>::type* lu11,
+                             typename cplx_t<T>::type* inv, int32_t* ipiv, int32_t* flag,
+                             double lbound) {
   typedef typename cplx_t<T>::type CT;
   __shared__ PanelLds<CT> P;
   const int tid = threadIdx.x;
@@ -549,7 +564,6 @@ __global__ __launch_bounds__(256) void k_panel_nopiv(int64_t n, const typename c
   }
   if (tid == 0 && P.M[NB - 1][NB - 1].x == 0 && P.M[NB - 1][NB - 1].y == 0) P.bad = 1;
   tri_inverses(P.M, P.Li, P.Ui, P.dinv);
-  const int b = blockIdx.x;
   if (b == 0) {
     store_factors(P, bw, lu11, inv);
     if (tid < bw) ipiv[k0 + tid] = (int32_t)(k0 + tid);
@@ -589,6 +603,15 @@ __global__ __launch_bounds__(256) void k_panel_nopiv(int64_t n, const typename c
   if (over && r0 + rr < n) atomicOr(flag, 1);
 }
 
+template <typename T>
+__global__ __launch_bounds__(256) void k_panel_nopiv(int64_t n, const typename cplx_t<T>::type* A,
+                                                     int64_t lda, int64_t k0, int bw,
+                                                     typename cplx_t<T>::type* lu11,
+                                                     typename cplx_t<T>::type* inv,
+                                                     int32_t* ipiv, int32_t* flag, double lbound) {
+  panel_nopiv_body<T>(blockIdx.x, n, A, lda, k0, bw, lu11, inv, ipiv, flag, lbound);
+}
+
 // Tournament fallback for a panel whose diagonal failed the threshold test:
 // one launch.  Leaves factor 128-row chunks with partial pivoting in
 // registers; the last of up to four siblings to finish (device-scope arrival
@@ -602,20 +625,17 @@ struct TourWs {
 };
 
 template <typename T>
-__global__ __launch_bounds__(256) void k_panel_pivot(typename cplx_t<T>::type* A, int64_t lda,
-                                                     int64_t n, int64_t c_lo, int64_t nc,
-                                                     int64_t k0, int bw, int nleaf,
-                                                     TourWs ws, const int32_t* flag,
-                                                     typename cplx_t<T>::type* lu11,
-                                                     typename cplx_t<T>::type* inv,
-                                                     int32_t* ipiv, int32_t* info) {
+MW_DEV void panel_pivot_body(int node0, typename cplx_t<T>::type* A, int64_t lda, int64_t n,
+                             int64_t c_lo, int64_t nc, int64_t k0, int bw, int nleaf, TourWs ws,
+                             const int32_t* flag, typename cplx_t<T>::type* lu11,
+                             typename cplx_t<T>::type* inv, int32_t* ipiv, int32_t* info) {
   typedef typename cplx_t<T>::type CT;
   if (*flag == 0) return;
   __shared__ TourLds<CT> L;
   __shared__ PanelLds<CT> P;
   __shared__ int last_s;
   const int tid = threadIdx.x;
-  int node = blockIdx.x, nodes = nleaf, lvl_off = 0;
+  int node = node0, nodes = nleaf, lvl_off = 0;
   if (tid == 0) {
     const int64_t r0 = k0 + (int64_t)node * TROWS;
     const int64_t left = n - r0;
@@ -739,6 +759,18 @@ __global__ __launch_bounds__(256) void k_panel_pivot(typename cplx_t<T>::type* A
   }
 }
 
+template <typename T>
+__global__ __launch_bounds__(256) void k_panel_pivot(typename cplx_t<T>::type* A, int64_t lda,
+                                                     int64_t n, int64_t c_lo, int64_t nc,
+                                                     int64_t k0, int bw, int nleaf,
+                                                     TourWs ws, const int32_t* flag,
+                                                     typename cplx_t<T>::type* lu11,
+                                                     typename cplx_t<T>::type* inv,
+                                                     int32_t* ipiv, int32_t* info) {
+  panel_pivot_body<T>(blockIdx.x, A, lda, n, c_lo, nc, k0, bw, nleaf, ws, flag, lu11, inv, ipiv,
+                      info);
+}
+
 constexpr int AW = 64;  // rows (or columns) per workgroup in the block products
 
 template <typename CT>
@@ -786,15 +818,13 @@ MW_DEV void apply_left(ApplyLds<CT>& L, CT* X, int64_t ldx, int64_t k0, int bw, 
 // Blocks [0, nrb): 64-row blocks of L21; [nrb, nrb+ncb): 64-column blocks of
 // U12; last: A11.
 template <typename T>
-__global__ __launch_bounds__(256) void k_panel_apply(int64_t n, int64_t nc, typename cplx_t<T>::type* A,
-                                                     int64_t lda, int64_t k0, int bw,
-                                                     const typename cplx_t<T>::type* inv,
-                                                     const typename cplx_t<T>::type* lu11,
-                                                     int nrb, int ncb) {
+MW_DEV void panel_apply_body(int b, int64_t n, int64_t nc, typename cplx_t<T>::type* A,
+                             int64_t lda, int64_t k0, int bw,
+                             const typename cplx_t<T>::type* inv,
+                             const typename cplx_t<T>::type* lu11, int nrb, int ncb) {
   typedef typename cplx_t<T>::type CT;
   __shared__ ApplyLds<CT> L;
   const int tid = threadIdx.x;
-  const int b = blockIdx.x;
   if (b < nrb) {
     const CT* Ui = inv + NB * NB;
     for (int e = tid; e < NB * NB; e += blockDim.x) L.Tm[e / NB][e % NB] = Ui[e];
@@ -836,21 +866,37 @@ __global__ __launch_bounds__(256) void k_panel_apply(int64_t n, int64_t nc, type
   }
 }
 
+template <typename T>
+__global__ __launch_bounds__(256) void k_panel_apply(int64_t n, int64_t nc, typename cplx_t<T>::type* A,
+                                                     int64_t lda, int64_t k0, int bw,
+                                                     const typename cplx_t<T>::type* inv,
+                                                     const typename cplx_t<T>::type* lu11,
+                                                     int nrb, int ncb) {
+  panel_apply_body<T>(blockIdx.x, n, nc, A, lda, k0, bw, inv, lu11, nrb, ncb);
+}
+
 // Solve blocks on AW right-hand-side columns per workgroup:
 // lower != 0: X_p <- L11^{-1} X_p; lower == 0: X_p <- U11^{-1} X_p.
+template <typename T>
+MW_DEV void solve_apply_body(int bx, int64_t nrhs, const typename cplx_t<T>::type* inv,
+                             int64_t k0, int bw, typename cplx_t<T>::type* X, int64_t ldx,
+                             int lower) {
+  typedef typename cplx_t<T>::type CT;
+  __shared__ ApplyLds<CT> L;
+  const int tid = threadIdx.x;
+  const int64_t c0 = (int64_t)bx * AW;
+  const CT* Tsrc = lower ? inv : inv + NB * NB;
+  for (int e = tid; e < NB * NB; e += blockDim.x) L.Tm[e / NB][e % NB] = Tsrc[e];
+  __syncthreads();
+  apply_left(L, X, ldx, k0, bw, c0, nrhs);
+}
+
 template <typename T>
 __global__ __launch_bounds__(256) void k_solve_apply(int64_t nrhs, const typename cplx_t<T>::type* inv,
                                                      int64_t k0, int bw,
                                                      typename cplx_t<T>::type* X, int64_t ldx,
                                                      int lower) {
-  typedef typename cplx_t<T>::type CT;
-  __shared__ ApplyLds<CT> L;
-  const int tid = threadIdx.x;
-  const int64_t c0 = (int64_t)blockIdx.x * AW;
-  const CT* Tsrc = lower ? inv : inv + NB * NB;
-  for (int e = tid; e < NB * NB; e += blockDim.x) L.Tm[e / NB][e % NB] = Tsrc[e];
-  __syncthreads();
-  apply_left(L, X, ldx, k0, bw, c0, nrhs);
+  solve_apply_body<T>(blockIdx.x, nrhs, inv, k0, bw, X, ldx, lower);
 }
 
 // Row interchanges ipiv[r0 .. r1) applied in order to columns [0, ncols) of X.
@@ -878,11 +924,9 @@ __global__ void k_zero_i32(int32_t* p, int64_t n) {
 // ---------------------------------------------------------------------------
 // Port algebra helpers (FP64)
 // ---------------------------------------------------------------------------
-__global__ void k_sparse_rows(int64_t nrows, int maxnz, const int32_t* idx, const double* val,
-                              const double2* X, int64_t ldx, int64_t ncols, double2* out,
-                              int64_t ldo) {
-  const int64_t i = blockIdx.y;
-  const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+MW_DEV void sparse_rows_body(int64_t i, int64_t j, int64_t nrows, int maxnz, const int32_t* idx,
+                             const double* val, const double2* X, int64_t ldx, int64_t ncols,
+                             double2* out, int64_t ldo) {
   if (i >= nrows || j >= ncols) return;
   double sr = 0.0, si = 0.0;
   for (int t = 0; t < maxnz; ++t) {
@@ -894,6 +938,13 @@ __global__ void k_sparse_rows(int64_t nrows, int maxnz, const int32_t* idx, cons
     si += v * x.y;
   }
   out[i * ldo + j] = make_double2(sr, si);
+}
+
+__global__ void k_sparse_rows(int64_t nrows, int maxnz, const int32_t* idx, const double* val,
+                              const double2* X, int64_t ldx, int64_t ncols, double2* out,
+                              int64_t ldo) {
+  sparse_rows_body(blockIdx.y, (int64_t)blockIdx.x * blockDim.x + threadIdx.x, nrows, maxnz, idx,
+                   val, X, ldx, ncols, out, ldo);
 }
 
 __global__ void k_zero_rows(int64_t n, int64_t ncols, double2* X, int64_t ldx) {
@@ -939,8 +990,8 @@ __global__ void k_c2z(int64_t rows, int64_t cols, const float2* src, int64_t lds
 
 // T <- S^{-1} T, one workgroup, Gauss-Jordan with partial pivoting in LDS
 // (ns <= 32, nt <= 64).
-__global__ __launch_bounds__(256) void k_small_solve(int ns, int nt, double2* S, int64_t lds,
-                                                     double2* Tm, int64_t ldt, int32_t* info) {
+MW_DEV void small_solve_body(int ns, int nt, double2* S, int64_t lds, double2* Tm, int64_t ldt,
+                             int32_t* info) {
   __shared__ double2 M[32][97];
   __shared__ int piv_s;
   const int tid = threadIdx.x, lane = tid & 63;
@@ -986,6 +1037,194 @@ __global__ __launch_bounds__(256) void k_small_solve(int ns, int nt, double2* S,
     __syncthreads();
   }
   for (int e = tid; e < ns * nt; e += blockDim.x) Tm[(e / nt) * ldt + e % nt] = M[e / nt][ns + e % nt];
+}
+
+__global__ __launch_bounds__(256) void k_small_solve(int ns, int nt, double2* S, int64_t lds,
+                                                     double2* Tm, int64_t ldt, int32_t* info) {
+  small_solve_body(ns, nt, S, lds, Tm, ldt, info);
+}
+
+// ---------------------------------------------------------------------------
+// Batched sweep: one launch per step serves every (structure, wavenumber)
+// problem; grid z (or y) indexes the problem, problems are sorted by size so
+// a step launches only the prefix still active.  Single-level blocking
+// (problems stay below 4096 unknowns), FP64.
+// ---------------------------------------------------------------------------
+MW_DEV int64_t d_tour_nodes(int64_t n) {
+  int64_t nodes = 0;
+  for (int64_t nd = (n + TROWS - 1) / TROWS; nd > 1; nd = (nd + FANIN - 1) / FANIN) nodes += nd;
+  return nodes + 1;
+}
+MW_DEV double2* ws_inv(void* work, int64_t k0) {
+  return reinterpret_cast<double2*>(reinterpret_cast<char*>(work) +
+                                    (int64_t)NB * NB * 16 * (1 + 2 * (k0 / NB)));
+}
+MW_DEV int32_t* ws_flags(void* work, int64_t n) {
+  const int64_t npan = (n + NB - 1) / NB;
+  return reinterpret_cast<int32_t*>(reinterpret_cast<char*>(work) +
+                                    (int64_t)NB * NB * 16 * (1 + 2 * npan));
+}
+MW_DEV TourWs ws_tour(void* work, int64_t n) {
+  TourWs tw;
+  tw.counter = ws_flags(work, n) + (n + NB - 1) / NB;
+  const int64_t nodes = d_tour_nodes(n);
+  tw.ncand = tw.counter + nodes;
+  tw.cand = tw.ncand + nodes;
+  return tw;
+}
+
+__global__ void kb_zero_ws(const mw_sweep_prob* P) {
+  const mw_sweep_prob& p = P[blockIdx.y];
+  const int64_t cnt = (p.n + NB - 1) / NB + d_tour_nodes(p.n);
+  int32_t* f = ws_flags(p.work, p.n);
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < cnt;
+       e += (int64_t)gridDim.x * blockDim.x)
+    f[e] = 0;
+  if (blockIdx.x == 0 && threadIdx.x < 2) p.info[threadIdx.x] = 0;
+}
+
+__global__ void kb_rhs(const mw_sweep_prob* P, const mw_ports* ports, int scatter) {
+  const mw_sweep_prob& p = P[blockIdx.y];
+  const mw_ports& q = ports[p.mesh];
+  double2* X = reinterpret_cast<double2*>(p.A) + p.n;
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (!scatter) {
+    if (e < p.n * p.ncols) X[(e / p.ncols) * p.lda + e % p.ncols] = make_double2(0.0, 0.0);
+  } else if (e < p.ncols * q.col_max) {
+    const int32_t r = q.col_idx[e];
+    if (r >= 0) X[(int64_t)r * p.lda + e / q.col_max] = make_double2(q.col_val[e], 0.0);
+  }
+}
+
+__global__ __launch_bounds__(256) void kb_panel_nopiv(const mw_sweep_prob* P, int64_t k0,
+                                                      double lbound) {
+  const mw_sweep_prob& p = P[blockIdx.y];
+  if (k0 >= p.n) return;
+  const int bw = (int)((p.n - k0) < NB ? (p.n - k0) : NB);
+  const int nrb = (int)((p.n - k0 - bw + AW - 1) / AW);
+  if ((int)blockIdx.x > nrb) return;
+  panel_nopiv_body<double>(blockIdx.x, p.n, (const double2*)p.A, p.lda, k0, bw, (double2*)p.work,
+                           ws_inv(p.work, k0), p.ipiv, ws_flags(p.work, p.n) + k0 / NB, lbound);
+}
+
+__global__ __launch_bounds__(256) void kb_panel_pivot(const mw_sweep_prob* P, int64_t k0) {
+  const mw_sweep_prob& p = P[blockIdx.y];
+  if (k0 >= p.n) return;
+  const int bw = (int)((p.n - k0) < NB ? (p.n - k0) : NB);
+  const int nleaf = (int)((p.n - k0 + TROWS - 1) / TROWS);
+  if ((int)blockIdx.x >= nleaf) return;
+  panel_pivot_body<double>(blockIdx.x, (double2*)p.A, p.lda, p.n, k0, p.n + p.ncols, k0, bw, nleaf,
+                           ws_tour(p.work, p.n), ws_flags(p.work, p.n) + k0 / NB, (double2*)p.work,
+                           ws_inv(p.work, k0), p.ipiv, p.info);
+}
+
+__global__ __launch_bounds__(256) void kb_panel_apply(const mw_sweep_prob* P, int64_t k0) {
+  const mw_sweep_prob& p = P[blockIdx.y];
+  if (k0 >= p.n) return;
+  const int bw = (int)((p.n - k0) < NB ? (p.n - k0) : NB);
+  const int64_t nc = p.n + p.ncols;
+  const int nrb = (int)((p.n - k0 - bw + AW - 1) / AW);
+  const int ncb = (int)((nc - k0 - bw + AW - 1) / AW);
+  if ((int)blockIdx.x > nrb + ncb) return;
+  panel_apply_body<double>(blockIdx.x, p.n, nc, (double2*)p.A, p.lda, k0, bw, ws_inv(p.work, k0),
+                           (const double2*)p.work, nrb, ncb);
+}
+
+// mode 0: trailing update after panel k0; mode 1: back-substitution update above panel k0
+template <bool BIG>
+__global__ __launch_bounds__(256) void kb_gemm(const mw_sweep_prob* P, int64_t k0, int mode) {
+  const mw_sweep_prob& p = P[blockIdx.z];
+  if (k0 >= p.n) return;
+  const int bw = (int)((p.n - k0) < NB ? (p.n - k0) : NB);
+  double2* A = reinterpret_cast<double2*>(p.A);
+  const int64_t lda = p.lda;
+  int64_t m, ncol;
+  const double2 *a, *b;
+  double2* c;
+  if (mode == 0) {
+    m = p.n - k0 - bw;
+    ncol = p.n + p.ncols - k0 - bw;
+    a = A + (k0 + bw) * lda + k0;
+    b = A + k0 * lda + k0 + bw;
+    c = A + (k0 + bw) * lda + k0 + bw;
+  } else {
+    m = k0;
+    ncol = p.ncols;
+    a = A + k0;
+    b = A + k0 * lda + p.n;
+    c = A + p.n;
+  }
+  const int tm = BIG ? GBM : SBM, tn = BIG ? GBN : SBN;
+  if (m <= 0 || ncol <= 0 || (int64_t)blockIdx.y * tm >= m || (int64_t)blockIdx.x * tn >= ncol)
+    return;
+  if (BIG)
+    gemm_big_body<double, false>(blockIdx.x, blockIdx.y, m, ncol, bw, a, lda, b, lda, c, lda);
+  else
+    gemm_small_body<double, false>(blockIdx.x, blockIdx.y, m, ncol, bw, a, lda, b, lda, c, lda);
+}
+
+__global__ __launch_bounds__(256) void kb_bwd_apply(const mw_sweep_prob* P, int64_t k0) {
+  const mw_sweep_prob& p = P[blockIdx.y];
+  if (k0 >= p.n || (int64_t)blockIdx.x * AW >= p.ncols) return;
+  const int bw = (int)((p.n - k0) < NB ? (p.n - k0) : NB);
+  solve_apply_body<double>(blockIdx.x, p.ncols, ws_inv(p.work, k0), k0, bw,
+                           reinterpret_cast<double2*>(p.A) + p.n, p.lda, 0);
+}
+
+// KCL Schur products: S = A Xc (ncon x ncon), T = A W (ncon x n_ports).
+__global__ void kb_schur_ax(const mw_sweep_prob* P, const mw_ports* ports) {
+  const mw_sweep_prob& p = P[blockIdx.y];
+  const mw_ports& q = ports[p.mesh];
+  const int i = blockIdx.x;
+  const int j = threadIdx.x;
+  if (i >= q.n_con || j >= q.n_con + q.n_ports) return;
+  const double2* X = reinterpret_cast<const double2*>(p.A) + p.n;
+  const int col = j < q.n_con ? q.n_ports + j : j - q.n_con;
+  double sr = 0.0, si = 0.0;
+  for (int t = 0; t < q.con_max; ++t) {
+    const int32_t r = q.con_idx[i * q.con_max + t];
+    if (r < 0) continue;
+    const double v = q.con_val[i * q.con_max + t];
+    const double2 x = X[(int64_t)r * p.lda + col];
+    sr += v * x.x;
+    si += v * x.y;
+  }
+  double2* out = j < q.n_con ? reinterpret_cast<double2*>(p.S) + i * q.n_con + j
+                             : reinterpret_cast<double2*>(p.T) + i * q.n_ports + (j - q.n_con);
+  *out = make_double2(sr, si);
+}
+
+__global__ __launch_bounds__(256) void kb_small_solve(const mw_sweep_prob* P, const mw_ports* ports) {
+  const mw_sweep_prob& p = P[blockIdx.x];
+  const mw_ports& q = ports[p.mesh];
+  if (q.n_con == 0) return;
+  small_solve_body(q.n_con, q.n_ports, (double2*)p.S, q.n_con, (double2*)p.T, q.n_ports,
+                   p.info + 1);
+}
+
+// W -= Xc Lambda, one row per thread.
+__global__ void kb_schur_update(const mw_sweep_prob* P, const mw_ports* ports) {
+  const mw_sweep_prob& p = P[blockIdx.y];
+  const mw_ports& q = ports[p.mesh];
+  const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q.n_con == 0 || r >= p.n) return;
+  double2* X = reinterpret_cast<double2*>(p.A) + p.n + r * p.lda;
+  const double2* Lm = reinterpret_cast<const double2*>(p.T);
+  for (int j = 0; j < q.n_ports; ++j) {
+    double2 acc = X[j];
+    for (int c = 0; c < q.n_con; ++c) acc = cfms(acc, X[q.n_ports + c], Lm[c * q.n_ports + j]);
+    X[j] = acc;
+  }
+}
+
+__global__ void kb_readout(const mw_sweep_prob* P, const mw_ports* ports) {
+  const mw_sweep_prob& p = P[blockIdx.y];
+  const mw_ports& q = ports[p.mesh];
+  const int i = blockIdx.x, j = threadIdx.x;
+  if (i >= q.n_ports || j >= q.n_ports) return;
+  sparse_rows_body(i, j, q.n_ports, q.port_max, q.port_idx, q.port_val,
+                   reinterpret_cast<const double2*>(p.A) + p.n, p.lda, q.n_ports,
+                   reinterpret_cast<double2*>(p.Y), q.n_ports);
 }
 
 }  // namespace
@@ -1253,4 +1492,84 @@ hipError_t mw_launch_c2z(int64_t rows, int64_t cols, const void* src, int64_t ld
   hipLaunchKernelGGL(k_c2z, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st, rows, cols,
                      (const float2*)src, lds, (double2*)dst, ldd, accumulate);
   return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Batched sweep driver.  `nact(k0)` = number of problems (sorted by n,
+// largest first) with n > k0; `hp` is the host copy of the sorted table.
+// ---------------------------------------------------------------------------
+int mw_sweep_lu_run(const mw_sweep_prob* dP, const mw_ports* dports, const mw_sweep_prob* hp,
+                    int nprob, int max_ports, int max_con, int64_t max_scatter,
+                    int gemm_variant, const volatile int32_t* cancel, hipStream_t st,
+                    hipError_t* herr) {
+  if (nprob <= 0) return MW_OK;
+  const int64_t nmax = hp[0].n;
+  int64_t maxcols = 0, maxrhs = 0, maxzero = 0;
+  for (int i = 0; i < nprob; ++i) {
+    maxcols = hp[i].ncols > maxcols ? hp[i].ncols : maxcols;
+    maxrhs = hp[i].n * hp[i].ncols > maxrhs ? hp[i].n * hp[i].ncols : maxrhs;
+  }
+  maxzero = (nmax + NB - 1) / NB + tour_nodes(nmax);
+  hipLaunchKernelGGL(kb_zero_ws, dim3((unsigned)((maxzero + 255) / 256), nprob), dim3(256), 0, st, dP);
+  hipLaunchKernelGGL(kb_rhs, dim3((unsigned)((maxrhs + 255) / 256), nprob), dim3(256), 0, st, dP,
+                     dports, 0);
+  hipLaunchKernelGGL(kb_rhs, dim3((unsigned)((max_scatter + 255) / 256), nprob), dim3(256), 0, st,
+                     dP, dports, 1);
+  auto nact = [&](int64_t k0) {
+    int lo = 0, hi = nprob;  // first index with n <= k0
+    while (lo < hi) {
+      const int mid = (lo + hi) / 2;
+      if (hp[mid].n > k0) lo = mid + 1; else hi = mid;
+    }
+    return lo;
+  };
+  const double lbound = g_lbound;
+  for (int64_t k0 = 0; k0 < nmax; k0 += NB) {
+    if (cancel && *cancel) return MW_EABORT;
+    const int na = nact(k0);
+    const int bw = (int)((nmax - k0) < NB ? (nmax - k0) : NB);
+    const int64_t rest = nmax - k0 - bw;
+    const int nrb = (int)((rest + AW - 1) / AW);
+    const int ncb = (int)((rest + maxcols + AW - 1) / AW);
+    const int nleaf = (int)((nmax - k0 + TROWS - 1) / TROWS);
+    hipLaunchKernelGGL(kb_panel_nopiv, dim3(nrb + 1, na), dim3(256), 0, st, dP, k0, lbound);
+    hipLaunchKernelGGL(kb_panel_pivot, dim3(nleaf, na), dim3(256), 0, st, dP, k0);
+    hipLaunchKernelGGL(kb_panel_apply, dim3(nrb + ncb + 1, na), dim3(256), 0, st, dP, k0);
+    if (rest > 0) {
+      const int64_t ncol = rest + maxcols;
+      if (gemm_variant)
+        hipLaunchKernelGGL(kb_gemm<true>, dim3((unsigned)((ncol + GBN - 1) / GBN),
+                                               (unsigned)((rest + GBM - 1) / GBM), na),
+                           dim3(256), 0, st, dP, k0, 0);
+      else
+        hipLaunchKernelGGL(kb_gemm<false>, dim3((unsigned)((ncol + SBN - 1) / SBN),
+                                                (unsigned)((rest + SBM - 1) / SBM), na),
+                           dim3(256), 0, st, dP, k0, 0);
+    }
+  }
+  const int64_t nlast = ((nmax + NB - 1) / NB - 1) * NB;
+  for (int64_t k0 = nlast; k0 >= 0; k0 -= NB) {
+    const int na = nact(k0);
+    hipLaunchKernelGGL(kb_bwd_apply, dim3((unsigned)((maxcols + AW - 1) / AW), na), dim3(256), 0, st,
+                       dP, k0);
+    if (k0 > 0) {
+      if (gemm_variant)
+        hipLaunchKernelGGL(kb_gemm<true>, dim3((unsigned)((maxcols + GBN - 1) / GBN),
+                                               (unsigned)((k0 + GBM - 1) / GBM), na),
+                           dim3(256), 0, st, dP, k0, 1);
+      else
+        hipLaunchKernelGGL(kb_gemm<false>, dim3((unsigned)((maxcols + SBN - 1) / SBN),
+                                                (unsigned)((k0 + SBM - 1) / SBM), na),
+                           dim3(256), 0, st, dP, k0, 1);
+    }
+  }
+  if (max_con > 0) {
+    hipLaunchKernelGGL(kb_schur_ax, dim3(max_con, nprob), dim3(64), 0, st, dP, dports);
+    hipLaunchKernelGGL(kb_small_solve, dim3(nprob), dim3(256), 0, st, dP, dports);
+    hipLaunchKernelGGL(kb_schur_update, dim3((unsigned)((nmax + 255) / 256), nprob), dim3(256), 0,
+                       st, dP, dports);
+  }
+  hipLaunchKernelGGL(kb_readout, dim3(max_ports, nprob), dim3(64), 0, st, dP, dports);
+  *herr = hipGetLastError();
+  return *herr == hipSuccess ? MW_OK : MW_EHIP;
 }

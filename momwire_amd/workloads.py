@@ -77,3 +77,124 @@ def junction_tee(ports="feed"):
         kw["feeds"] = [(0, None, 1.0)]
         kw["node_gaps"] = [(1, "start", 1.0)]
     return kw
+
+
+def _seg_seg_distance(p0, p1, q0, q1):
+    """Minimum distance between segments p0-p1 and q0-q1 (sampled exactly
+    enough for a clearance test: endpoints against segments plus the
+    closest-approach of the two carrier lines when it falls inside both)."""
+    def pt_seg(p, a, b):
+        ab = b - a
+        t = np.clip(np.dot(p - a, ab) / max(np.dot(ab, ab), 1e-300), 0.0, 1.0)
+        return np.linalg.norm(p - (a + t * ab))
+    d = min(pt_seg(p0, q0, q1), pt_seg(p1, q0, q1), pt_seg(q0, p0, p1), pt_seg(q1, p0, p1))
+    u, v, w = p1 - p0, q1 - q0, p0 - q0
+    a, b, c, dd, e = u @ u, u @ v, v @ v, u @ w, v @ w
+    den = a * c - b * b
+    if den > 1e-12 * a * c:
+        s = (b * e - c * dd) / den
+        t = (a * e - b * dd) / den
+        if 0 < s < 1 and 0 < t < 1:
+            d = min(d, np.linalg.norm(w + s * u - t * v))
+    return d
+
+
+def random_structure(seed, f0_hz=300e6, nb_range=(40, 2000), max_tries=200):
+    """One seeded random wire structure for the batched sweep.
+
+    1-12 polylines of 1-3 straight edges each, grown as a tree: every wire
+    after the first starts at an existing node (a junction shared with the
+    wires already there).  Each wire's length is U(0.5, 6) lambda0, split over
+    its edges; radius log-U(1e-4, 1e-2) lambda0; one segment length
+    U(lambda0/20, lambda0/10) per structure; 1-4 delta-gap feeds at random
+    interior arc positions.  Edges keep a clearance of one segment length
+    from every edge they do not touch.  Structures whose unknown count falls
+    outside nb_range are redrawn.
+    """
+    lam = C_LIGHT / f0_hz
+    rng = np.random.default_rng(seed)
+    for _ in range(max_tries):
+        n_w = int(rng.integers(1, 13))
+        seg_len = rng.uniform(lam / 20, lam / 10)
+        radius = float(np.exp(rng.uniform(np.log(1e-4), np.log(1e-2))) * lam)
+        nodes = [np.zeros(3)]
+        wires, npe, ends = [], [], []  # ends: (start node id, end node id)
+        edges = []  # (p0, p1, wire)
+        ok = True
+        for w in range(n_w):
+            start = int(rng.integers(0, len(nodes))) if w else 0
+            n_e = int(rng.integers(1, 4))
+            length = rng.uniform(0.5, 6.0) * lam
+            parts = rng.dirichlet(np.ones(n_e)) * length
+            pts = [nodes[start]]
+            direc = rng.normal(size=3)
+            placed = True
+            for e in range(n_e):
+                for _try in range(30):
+                    d = direc + 0.8 * rng.normal(size=3)
+                    d /= np.linalg.norm(d)
+                    p1 = pts[-1] + max(parts[e], 2 * seg_len) * d
+                    clear = True
+                    for (a, b, ww) in edges:
+                        touching = (np.linalg.norm(a - pts[-1]) < 1e-9 or
+                                    np.linalg.norm(b - pts[-1]) < 1e-9)
+                        if touching:
+                            # share only the junction point: keep an angle
+                            other = b if np.linalg.norm(a - pts[-1]) < 1e-9 else a
+                            cosang = np.dot(other - pts[-1], d) / np.linalg.norm(other - pts[-1])
+                            if cosang > 0.9:
+                                clear = False
+                                break
+                            continue
+                        if _seg_seg_distance(pts[-1], p1, a, b) < seg_len:
+                            clear = False
+                            break
+                    if clear:
+                        break
+                else:
+                    placed = False
+                    break
+                edges.append((pts[-1], p1, w))
+                pts.append(p1)
+                direc = d
+            if not placed:
+                ok = False
+                break
+            nodes.append(pts[-1])
+            wires.append([tuple(p) for p in pts])
+            npe.append([max(2, int(round(np.linalg.norm(pts[i + 1] - pts[i]) / seg_len)))
+                        for i in range(len(pts) - 1)])
+            ends.append((start, len(nodes) - 1))
+        if not ok:
+            continue
+        # junction groups: every node touched by >= 2 wire ends
+        groups = {}
+        for w, (s_node, e_node) in enumerate(ends):
+            groups.setdefault(s_node, []).append((w, "start"))
+            groups.setdefault(e_node, []).append((w, "end"))
+        junctions = [g for g in groups.values() if len(g) >= 2]
+        n_junc_ends = sum(len(g) for g in junctions)
+        n_b = sum(sum(e) for e in npe) + 2 * n_w * 0 + n_junc_ends - 0
+        # unknowns: per wire N + d - 2 interior bases plus kept junction-end bases
+        n_b = sum(sum(e) for e in npe) + n_junc_ends
+        if not (nb_range[0] <= n_b <= nb_range[1]):
+            continue
+        n_feeds = int(rng.integers(1, 5))
+        feed_w = rng.integers(0, n_w, size=n_feeds)
+        feeds = []
+        for w in feed_w:
+            L = sum(np.linalg.norm(np.subtract(wires[w][i + 1], wires[w][i]))
+                    for i in range(len(wires[w]) - 1))
+            feeds.append((int(w), float(rng.uniform(0.2, 0.8) * L), 1.0))
+        return dict(wires=wires, n_per_edge_per_wire=npe, degree=2, wire_radius=radius,
+                    wavelength=lam, junctions=junctions, feeds=feeds)
+    raise RuntimeError(f"no admissible structure for seed {seed}")
+
+
+def sweep_structures(n=128, seed=3):
+    """The batched-sweep workload: n seeded random structures."""
+    return [random_structure(seed * 100003 + i) for i in range(n)]
+
+
+def sweep_wavenumbers(n_freq=16, f0_hz=300e6, lo=0.8, hi=1.2):
+    return 2 * np.pi * np.linspace(lo, hi, n_freq) * f0_hz / C_LIGHT

@@ -77,7 +77,20 @@ def _tee(ports):
     return out, dt
 
 
+def case_sweep_struct(index):
+    """One random structure of the batched sweep, swept over 3 k."""
+    kw = workloads.random_structure(3 * 100003 + index)
+    s = BSplineSolver(**kw)
+    ks = workloads.sweep_wavenumbers()[[0, 7, 15]]
+    t0 = time.time()
+    ys = s.compute_y_matrix_swept(ks)
+    sol = s.compute_port_solution()
+    return dict(sweep_k=ks, sweep_y=ys, y=sol.y, coeffs=sol.coeffs), time.time() - t0
+
+
 CASES = {
+    "sweep_struct4": lambda: case_sweep_struct(4),
+    "sweep_struct8": lambda: case_sweep_struct(8),
     "c1_dipole": case_c1,
     "c2_array8x8": case_c2,
     "array2x2": case_small_array,
