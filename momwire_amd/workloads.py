@@ -13,6 +13,12 @@ fixed so the same geometry is rebuilt on every machine.
   exercise the KCL constraint rows, junction ports and node gaps.
 * ``random_structure`` -- seeded random polyline structures (1-12 wires sharing
   junction nodes) for the batched sweep.
+* ``grounded_array``   -- an nx x ny array of horizontal x-directed 0.48-lambda
+  dipoles at 0.5-lambda pitch over lossy Sommerfeld ground; height, eps_r,
+  sigma and frequency are drawn from a seeded generator (4x4 x 21 segments is
+  336 unknowns and 16 ports).
+* ``grounded_tee``     -- the junction tee above Sommerfeld ground (vertical,
+  horizontal and slanted segments, so every remainder surface is exercised).
 """
 
 import numpy as np
@@ -76,6 +82,58 @@ def junction_tee(ports="feed"):
     elif ports == "node_gap":
         kw["feeds"] = [(0, None, 1.0)]
         kw["node_gaps"] = [(1, "start", 1.0)]
+    return kw
+
+
+def ground_parameters(seed):
+    """(height / lambda, eps_r, sigma S/m, frequency Hz) of one grounded-array
+    draw: height U(0.05, 1) lambda, eps_r U(5, 30), sigma log-U(1e-3, 5e-2),
+    frequency U(3, 30) MHz."""
+    rng = np.random.default_rng(seed)
+    height = float(rng.uniform(0.05, 1.0))
+    eps_r = float(rng.uniform(5.0, 30.0))
+    sigma = float(np.exp(rng.uniform(np.log(1e-3), np.log(5e-2))))
+    freq = float(rng.uniform(3e6, 30e6))
+    return height, eps_r, sigma, freq
+
+
+def grounded_array(nx=4, ny=4, n_seg=21, seed=4, length=0.48, radius=1e-3, pitch=0.5,
+                   params=None):
+    """Horizontal dipole array over Sommerfeld ground (ground plane z = 0).
+
+    ``params`` overrides the seeded (height/lambda, eps_r, sigma, freq) draw.
+    """
+    height, eps_r, sigma, freq = params if params is not None else ground_parameters(seed)
+    lam = C_LIGHT / freq
+    half = 0.5 * length * lam
+    z = height * lam
+    wires = []
+    for ix in range(nx):
+        for iy in range(ny):
+            x = ix * pitch * lam
+            y = iy * pitch * lam
+            wires.append([(x - half, y, z), (x + half, y, z)])
+    return dict(
+        wires=wires,
+        n_per_edge_per_wire=[[n_seg]] * len(wires),
+        degree=2,
+        wire_radius=radius * lam,
+        wavelength=lam,
+        feeds=[(w, None, 1.0) for w in range(len(wires))],
+        ground_z=0.0,
+        ground_eps=(eps_r, sigma),
+        ground_model="sommerfeld",
+    )
+
+
+def grounded_tee(eps_r=12.0, sigma=5e-3, freq_hz=10e6):
+    """The junction tee scaled to ``freq_hz`` and lifted above Sommerfeld ground."""
+    kw = junction_tee("feed")
+    lam = C_LIGHT / freq_hz
+    kw["wires"] = [[(p[0] * lam, p[1] * lam, p[2] * lam) for p in w] for w in kw["wires"]]
+    kw["wire_radius"] = 1e-3 * lam
+    kw["wavelength"] = lam
+    kw.update(ground_z=0.0, ground_eps=(eps_r, sigma), ground_model="sommerfeld")
     return kw
 
 

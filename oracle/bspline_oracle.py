@@ -37,8 +37,11 @@ class OracleSolver:
     def __init__(self, *, wires, n_per_edge_per_wire=None, degree=2, feeds=None,
                  feed_wire_index=0, feed_arclength=None, junctions=None,
                  junction_ports=None, node_gaps=None, n_qp_pair=4, wavelength=22,
-                 wire_radius=0.0005, nsegs=101):
+                 wire_radius=0.0005, nsegs=101, ground_z=None, ground_eps=None,
+                 ground_model="refl-coef", n_qp_sommerfeld=3):
         self.degree = int(degree)
+        self.ground_z, self.ground_eps = ground_z, ground_eps
+        self.ground_model, self.n_qp_sommerfeld = ground_model, int(n_qp_sommerfeld)
         self.wires = [np.asarray(w, dtype=float) for w in wires]
         n_w = len(self.wires)
         r = np.asarray(wire_radius, dtype=float)
@@ -171,14 +174,16 @@ class OracleSolver:
         return supp, polys, kcl, knots_all, wbg
 
     # -- moments ------------------------------------------------------------
-    def full_moments(self, geom, a_row, k, il=None, jl=None):
-        """_bspline_kernels.py:716-747 (numpy body of the off-edge moments)."""
+    def full_moments(self, geom, a_row, k, il=None, jl=None, src=None):
+        """_bspline_kernels.py:716-747 (numpy body of the off-edge moments);
+        ``src`` = (seg_l, seg_r) of other source segments (the mirrored ones)."""
         d = self.degree
         t01, w01 = gl01(self.n_qp_pair)
         sl, sr = geom["seg_l"], geom["seg_r"]
         il = slice(None) if il is None else il
         jl = slice(None) if jl is None else jl
-        sli, sri, slj, srj = sl[il], sr[il], sl[jl], sr[jl]
+        sli, sri = sl[il], sr[il]
+        slj, srj = (sl[jl], sr[jl]) if src is None else (src[0][jl], src[1][jl])
         a = np.asarray(a_row)[il]
         Li = np.linalg.norm(sri - sli, axis=1)
         Lj = np.linalg.norm(srj - slj, axis=1)
@@ -225,10 +230,11 @@ class OracleSolver:
                 J[:, :, sl, sl] = A_st + A_reg
         return J
 
-    def assemble(self, J, supp, polys, geom):
+    def assemble(self, J, supp, polys, geom, td=None):
         """bspline.py:2322-2348 (numpy einsum assembly)."""
         d = self.degree
-        td = geom["tangents"] @ geom["tangents"].T
+        if td is None:
+            td = geom["tangents"] @ geom["tangents"].T
         nb = supp.shape[0]
         ZA = np.zeros((nb, nb), dtype=complex)
         ZP = np.zeros((nb, nb), dtype=complex)
@@ -245,10 +251,40 @@ class OracleSolver:
                 ZP += np.einsum("mp,pPmn,nP->mn", dm, Jb[:d, :d], dn)
         return 1j * self.omega * MU0 * ZA + ZP / (1j * self.omega * EPS0)
 
-    def Z(self, geom=None, basis=None):
+    def Z(self, geom=None, basis=None, remainder=True):
         geom = geom or self.geometry()
         supp, polys = (basis or self.basis(geom))[:2]
-        return self.assemble(self.J_blocks(geom, self.k), supp, polys, geom)
+        Z = self.assemble(self.J_blocks(geom, self.k), supp, polys, geom)
+        if self.ground_z is not None:
+            Z = Z - self.ground_terms(geom, supp, polys, remainder)
+        return Z
+
+    # -- Sommerfeld ground: bspline.py:3730-3770, 2552-2661, 1791-1874 --------
+    def ground_terms(self, geom, supp, polys, remainder=True):
+        """C2 * image + Q, the block the grounded fill subtracts."""
+        from . import sommerfeld_oracle as so
+        if self.ground_eps is None or self.ground_model != "sommerfeld":
+            raise NotImplementedError("the oracle covers the Sommerfeld ground only")
+        eps_t = so.eps_tilde(self.ground_eps, self.omega, EPS0)
+        c2 = so.image_coefficient(eps_t)
+        gz = self.ground_z
+        mir = np.array([1.0, 1.0, -1.0])
+        img_l, img_r = geom["seg_l"] * mir, geom["seg_r"] * mir
+        img_l[:, 2] += 2 * gz
+        img_r[:, 2] += 2 * gz
+        J_img = self.full_moments(geom, self.seg_radius(geom), self.k, src=(img_l, img_r))
+        td_img = geom["tangents"] @ (geom["tangents"] * mir).T
+        out = c2 * self.assemble(J_img, supp, polys, geom, td=td_img)
+        if remainder:
+            grid = self.ground_grid(geom, eps_t)
+            out = out + so.remainder_block(geom, supp, polys, gz, self.k, grid,
+                                           self.n_qp_sommerfeld)
+        return out
+
+    def ground_grid(self, geom, eps_t):
+        from . import sommerfeld_oracle as so
+        r1 = so.max_image_distance(geom["seg_l"], geom["seg_r"], self.ground_z)
+        return so.get_grid(eps_t, self.k, r1, self.omega, MU0)
 
     # -- ports: bspline.py:2806-2870, 3784-3867, 1170-1210 ------------------
     def port_columns(self, geom, basis):

@@ -1,0 +1,457 @@
+"""CPU oracle for the Sommerfeld ground path -- TEST INFRASTRUCTURE ONLY.
+
+A numpy/scipy restatement of the reference's Sommerfeld ground (NEC-2 theory
+manual eqs 136-172 as the reference implements them):
+
+* the six lambda-integrals on the deformed contours
+  (reference src/momwire/_sommerfeld.py:193-465),
+* the four interpolation surfaces and their R1 -> 0 limits (:511-584),
+* the interpolation grid: layout, fill, 4x4 Lagrange evaluation, the cached
+  normalised master and its rescaling (:642-914, 1016-1116),
+* the projected remainder field (:940-1010) and the Galerkin remainder block Q
+  (reference src/momwire/bspline.py:1791-1874, _accelerators.cpp:6081-6253),
+* the C2-scaled exact image (src/momwire/_ground_spec.py:90-125,
+  src/momwire/_potential_ground.py:779-788, src/momwire/bspline.py:2552-2661).
+
+Only tests, the smoke check and the CPU-baseline leg of the benchmark may
+import it.  How it is pinned:
+
+* six integrals, surfaces, limits, grid tables and grid evaluation: the
+  reference's own values (tests/golden/somm_engine.npz, somm_grid_far.npz and
+  the grid_* arrays of the grounded cases), written by the reference's
+  pure-Python engine;
+* the contours additionally by the Sommerfeld identity (the same contours
+  must reproduce exp(-j k R)/R) and the pointwise Laplacian identity
+  integrand[0] + integrand[3] + lambda^2 integrand[4] = 0;
+* the R1 -> 0 limits by the limit of the direct evaluation at small R1;
+* the image coefficient by its eps -> 1 and eps -> infinity limits;
+* Q and the grounded Z and Y by the reference's golden values
+  (ground_dipole.npz, ground_tee.npz, c5_ground4x4.npz).
+"""
+
+import math
+
+import numpy as np
+from scipy.special import hankel2, jv
+
+C_LIGHT = 299792458.0
+MU0_EXACT = 4e-7 * np.pi          # _sommerfeld.py:133 (normalised masters)
+K2_REF = 2.0 * np.pi             # _sommerfeld.py:130
+R1_CAP_WL = 15.0                 # _sommerfeld.py:67
+R1_NEAR_WL = 4.0                 # :99
+DR_FAR_WL, DTH_FAR_DEG = 0.2, 2.5  # :102-103
+N0_CAP = 513                     # :110
+TH_SPLIT_DEG = 30.0              # :113
+EPS_IM_STEP = 0.01               # :133 (geometric ladder step of Im eps)
+ADAPT_DEPTH = 14
+GX, GW = np.polynomial.legendre.leggauss(24)
+SURF = ("IrhoV", "IzV", "IrhoH", "IphiH")
+
+
+# ---------------------------------------------------------------------------
+# the six lambda-integrals
+# ---------------------------------------------------------------------------
+
+def gamma(lam, k):
+    """(lam^2 - k^2)^(1/2) with NEC's vertical cuts (_sommerfeld.py:193-195)."""
+    return np.sqrt(-1j * (lam - k)) * np.sqrt(1j * (lam + k))
+
+
+def integrands(lam, rho, h, k1, k2, bessel):
+    """(6, n) integrands of eqs 148-153 (_sommerfeld.py:220-256)."""
+    lam = np.asarray(lam, dtype=complex)
+    g1, g2 = gamma(lam, k1), gamma(lam, k2)
+    k1s, k2s = k1 * k1, k2 * k2
+    d1 = 2.0 / (g1 + g2) - 2.0 * k2s / (g2 * (k1s + k2s))
+    d2 = 2.0 / (k1s * g2 + k2s * g1) - 2.0 / (g2 * (k1s + k2s))
+    e = np.exp(-g2 * h)
+    x = lam * rho
+    if bessel:
+        small = np.abs(x) < 1e-6
+        xs = np.where(small, 1.0, x)
+        b0 = np.where(small, 1.0 - 0.25 * x * x, jv(0, xs))
+        b1x = np.where(small, 0.5 - x * x / 16.0, jv(1, xs) / xs)
+    else:
+        b0 = 0.5 * hankel2(0, x)
+        b1x = 0.5 * hankel2(1, x) / x
+    c = d2 * e
+    return np.stack([c * (b1x - b0) * lam ** 3, c * g2 * g2 * b0 * lam, c * g2 * b1x * x * lam ** 2,
+                     -c * b1x * lam ** 3, c * b0 * lam, d1 * e * b0 * lam])
+
+
+def _gauss(f, z0, z1):
+    mid, half = 0.5 * (z0 + z1), 0.5 * (z1 - z0)
+    return f(mid + half * GX) @ (GW * half)
+
+
+def _adaptive(f, z0, z1, rtol, depth=ADAPT_DEPTH, whole=None):
+    """Bisection with a relative stop test (_sommerfeld.py:268-288)."""
+    if whole is None:
+        whole = _gauss(f, z0, z1)
+    mid = 0.5 * (z0 + z1)
+    left, right = _gauss(f, z0, mid), _gauss(f, mid, z1)
+    both = left + right
+    if depth <= 0 or np.max(np.abs(both - whole)) <= rtol * max(np.max(np.abs(both)), 1e-300):
+        return both
+    return (_adaptive(f, z0, mid, rtol, depth - 1, left)
+            + _adaptive(f, mid, z1, rtol, depth - 1, right))
+
+
+def _tail(f, z0, direction, panel, rtol, ref, panel0=None, max_panels=800):
+    """Panel sum toward infinity, geometric ramp, two quiet panels stop it
+    (_sommerfeld.py:291-325)."""
+    total, quiet, z = 0.0, 0, z0
+    step = panel if panel0 is None else min(panel0, panel)
+    for _ in range(max_panels):
+        nxt = z + step * direction
+        part = _gauss(f, z, nxt)
+        total = total + part
+        z = nxt
+        step = min(2.0 * step, panel)
+        cmax = np.max(np.abs(part))
+        if cmax == 0.0 or cmax < rtol * max(np.max(np.abs(total)), ref):
+            quiet += 1
+            if quiet >= 2:
+                break
+        else:
+            quiet = 0
+    return total
+
+
+def six_integrals(eps_t, k2, rho, h, rtol=1e-9, form=None):
+    """(6,) integrals at one (rho, h) (_sommerfeld.py:328-465): Bessel form
+    (fig 13) for rho < 2h, Hankel form (fig 14) otherwise."""
+    eps_t = complex(eps_t)
+    if eps_t == 1.0:
+        return np.zeros(6, dtype=complex)
+    k2 = float(k2)
+    k1 = k2 * np.sqrt(eps_t)
+    if k1.imag > 0:
+        k1 = np.conj(k1)
+    rho, h = float(rho), float(h)
+    scale = max(rho, h)
+    panel = 0.2 * np.pi / scale
+    kmax = max(abs(k1), k2)
+    kcap = 1.2 * k2 + 50.0 / scale
+    qtol = min(rtol, 1e-11)
+    bessel = rho < 2.0 * h if form is None else form == "J"
+
+    def f(lam):
+        return integrands(lam, rho, h, k1, k2, bessel)
+
+    if bessel:
+        p = min(1.0 / rho if rho > 0 else np.inf, 1.0 / h)
+        brk = p * (1.0 + 1.0j)
+        end = 1.3 * min(kmax, kcap) + 3.0 * p + 1.0j * p
+        total = _adaptive(f, 0j, brk, qtol)
+        start = brk
+        if end.real > brk.real:
+            total = total + _adaptive(f, brk, end, qtol)
+            start = end
+        return total + _tail(f, start, 1.0 + 0j, panel, rtol, np.max(np.abs(total)))
+    r1 = math.hypot(rho, h)
+    right, left = (h - 1j * rho) / r1, (-h - 1j * rho) / r1
+    a, b, c = -0.4j * k2, (0.6 + 0.2j) * k2, (1.02 + 0.2j) * k2
+    dead = k1.real * h - k1.imag * rho >= 50.0
+    cap_d = kcap if (dead or abs(k1) > 200.0 * k2) else max(kcap, 1.01 * k1.real)
+    if 1.01 * k1.real <= cap_d:
+        d = 1.01 * k1.real + 0.99j * max(k1.imag, -cap_d)
+    else:
+        d = cap_d + 0j
+    if d.real < 1.1 * k2:
+        d = 1.1 * k2 + 1j * d.imag
+    total = _adaptive(f, a, b, qtol) + _adaptive(f, b, c, qtol) + _adaptive(f, c, d, qtol)
+    ref = np.max(np.abs(total))
+    total = total + _tail(f, d, right, panel, rtol, ref, panel0=0.5 * kmax)
+    return total - _tail(f, a, left, panel, rtol, ref, panel0=0.5 * kmax)
+
+
+def identity_check(k2, rho, h, bessel, rtol=1e-9):
+    """Sommerfeld identity on the module's contours (_sommerfeld.py:587-638):
+    int (lam/gamma2) exp(-gamma2 h) J0(lam rho) dlam = exp(-j k2 R)/R."""
+    def f(lam):
+        lam = np.asarray(lam, dtype=complex)
+        g2 = gamma(lam, k2)
+        x = lam * rho
+        b0 = jv(0, x) if bessel else 0.5 * hankel2(0, x)
+        return np.stack([(lam / g2) * np.exp(-g2 * h) * b0] * 6)
+    scale = max(rho, h)
+    panel = 0.2 * np.pi / scale
+    r = math.hypot(rho, h)
+    if bessel:
+        p = min(1.0 / rho if rho > 0 else np.inf, 1.0 / h, 10.0 * k2)
+        brk = p * (1.0 + 1.0j)
+        end = 1.3 * k2 + 3.0 * p + 1.0j * p
+        tot = _adaptive(f, 0j, brk, rtol)
+        start = brk
+        if end.real > brk.real:
+            tot = tot + _adaptive(f, brk, end, rtol)
+            start = end
+        tot = tot + _tail(f, start, 1.0 + 0j, panel, rtol, np.max(np.abs(tot)))
+    else:
+        a, b, c, d = -0.4j * k2, (0.6 + 0.2j) * k2, (1.02 + 0.2j) * k2, 1.3 * k2 + 0j
+        tot = _adaptive(f, a, b, rtol) + _adaptive(f, b, c, rtol) + _adaptive(f, c, d, rtol)
+        ref = np.max(np.abs(tot))
+        tot = tot + _tail(f, d, (h - 1j * rho) / r, panel, rtol, ref, panel0=0.5 * k2)
+        tot = tot - _tail(f, a, (-h - 1j * rho) / r, panel, rtol, ref, panel0=0.5 * k2)
+    return complex(tot[0]), complex(np.exp(-1j * k2 * r) / r)
+
+
+# ---------------------------------------------------------------------------
+# the four surfaces
+# ---------------------------------------------------------------------------
+
+def c1_norm(k2, omega, mu):
+    """Unit-moment normalisation of eq 123 (_sommerfeld.py:505-508)."""
+    return -1j * omega * mu / (4.0 * np.pi * k2 * k2)
+
+
+def limits_r1_zero(eps_t, k2, theta, omega, mu):
+    """Eqs 169-172 (_sommerfeld.py:511-532); (4, n)."""
+    theta = np.asarray(theta, dtype=float)
+    k1s, k2s = k2 * k2 * complex(eps_t), k2 * k2
+    c1 = c1_norm(k2, omega, mu)
+    c2 = (k1s - k2s) / (k1s + k2s)
+    c3 = k2s * (k1s - k2s) / (k1s + k2s) ** 2
+    s, co = np.sin(theta), np.cos(theta)
+    near = np.abs(co) < 1e-8
+    cs = np.where(near, 1.0, co)
+    q1 = np.where(near, 0.0, (1.0 - s) / cs)
+    q2 = np.where(near, 0.5, (1.0 - s) / (cs * cs))
+    return np.stack([c1 * c3 * k1s * q1, np.full(q1.shape, c1 * c3 * k1s, dtype=complex),
+                     c1 * k2s * (c2 - c3 + c3 * q2), -c1 * k2s * (c2 - c3 * q2)])
+
+
+def surfaces(eps_t, k2, R1, theta, rtol=1e-9, omega=None, mu=MU0_EXACT):
+    """(4, n) surfaces I_rho^V, I_z^V, I_rho^H, I_phi^H (_sommerfeld.py:535-584)."""
+    omega = k2 * C_LIGHT if omega is None else omega
+    R1 = np.atleast_1d(np.asarray(R1, dtype=float))
+    theta = np.broadcast_to(np.asarray(theta, dtype=float), R1.shape)
+    k1s, k2s = k2 * k2 * complex(eps_t), k2 * k2
+    c1 = c1_norm(k2, omega, mu)
+    out = np.zeros((4, R1.size), dtype=complex)
+    zero = R1 == 0.0
+    if zero.any():
+        out[:, zero] = limits_r1_zero(eps_t, k2, theta[zero], omega, mu)
+    for i in np.nonzero(~zero)[0]:
+        r1 = R1[i]
+        rho = max(r1 * np.cos(theta[i]), 0.0)
+        h = max(r1 * np.sin(theta[i]), 0.0)
+        v_rr, v_zz, v_rz, v_r1, v, u = six_integrals(eps_t, k2, rho, h, rtol)
+        ph = c1 * r1 * np.exp(1j * k2 * r1)
+        out[:, i] = [ph * k1s * v_rz, ph * k1s * (v_zz + k2s * v), ph * k2s * (v_rr + u),
+                     -ph * k2s * (v_r1 + u)]
+    return out
+
+
+# ---------------------------------------------------------------------------
+# the interpolation grid
+# ---------------------------------------------------------------------------
+
+def grid_layout(eps_t, k2, r1_max):
+    """Region list (r0, r1, dr, th0_deg, th1_deg, dth_deg) plus the scalars
+    (r1_max, r_break, r_near, r0_fill) of SommerfeldGrid.__init__
+    (_sommerfeld.py:688-812)."""
+    lam = 2.0 * np.pi / k2
+    r1_max = min(max(float(r1_max), 0.35 * lam), R1_CAP_WL * lam)
+    k1 = k2 * np.sqrt(complex(eps_t))
+    if k1.imag > 0:
+        k1 = np.conj(k1)
+    beat = 2.0 * np.pi / max(abs(k1 - k2), 1e-30) if abs(k1) <= 12.0 * k2 else np.inf
+    r_near = min(r1_max, R1_NEAR_WL * lam)
+    dth2_t = min(5.0, np.degrees(0.07 * lam / r_near))
+    n_th2 = int(np.ceil(TH_SPLIT_DEG / dth2_t)) + 1
+    dth2 = TH_SPLIT_DEG / (n_th2 - 1)
+    r_break = 0.2 * lam
+    layer = 1.0 / max(abs(k1), 1e-300)
+    dr0 = min(0.01 * lam, max(layer / 4.0, r_break / (N0_CAP - 1)))
+    r0_fill = 0.5 * dr0 if layer < 0.25 * dr0 else 0.0
+    sp = TH_SPLIT_DEG
+    regions = [(0.0, r_break, 0.01 * lam, 0.0, sp, 10.0),
+               (0.0, r_break, dr0, sp, 90.0, 10.0),
+               (r_break, r_near, min(0.05 * lam, beat / 6.0), 0.0, sp, dth2),
+               (r_break, r_near, min(0.1 * lam, beat / 6.0), sp, 90.0, 10.0)]
+    if r1_max > r_near * (1.0 + 1e-9):
+        regions += [(r_near, r1_max, DR_FAR_WL * lam, 0.0, sp, DTH_FAR_DEG),
+                    (r_near, r1_max, DR_FAR_WL * lam, sp, 90.0, 10.0)]
+    return regions, (r1_max, r_break, r_near, r0_fill)
+
+
+class Grid:
+    """Tabulated surfaces with 4x4 Lagrange evaluation (_sommerfeld.py:642-876)."""
+
+    def __init__(self, regions, scalars, vals, k2, omega, mu, eps_t):
+        self.r1_max, self.r_break, self.r_near, self.r0_fill = scalars
+        self.k2, self.omega, self.mu, self.eps_t = k2, omega, mu, eps_t
+        self.regions = []
+        for (r0, _r1, dr, th0, th1, dth), v in zip(regions, vals):
+            self.regions.append(dict(r0=r0, dr=dr, th0=np.radians(th0), dth=np.radians(dth),
+                                     n_r=v.shape[1], n_th=v.shape[2], vals=v))
+
+    @staticmethod
+    def region_nodes(region, idx, r0_fill):
+        r0, r1, dr, th0, th1, dth = region
+        n_r = max(int(np.ceil((r1 - r0) / dr)) + 1, 4)
+        n_th = int(round((th1 - th0) / dth)) + 1
+        rr, tt = np.meshgrid(r0 + dr * np.arange(n_r), np.radians(th0 + dth * np.arange(n_th)),
+                             indexing="ij")
+        if idx == 1 and r0_fill > 0.0:
+            rr = rr.copy()
+            rr[0, :] = r0_fill
+        return rr, tt
+
+    @classmethod
+    def build(cls, eps_t, k2, r1_max, rtol=1e-6, omega=None, mu=MU0_EXACT):
+        omega = k2 * C_LIGHT if omega is None else omega
+        regions, scalars = grid_layout(eps_t, k2, r1_max)
+        vals = []
+        for i, reg in enumerate(regions):
+            rr, tt = cls.region_nodes(reg, i, scalars[3])
+            vals.append(surfaces(eps_t, k2, rr.ravel(), tt.ravel(), rtol, omega, mu)
+                        .reshape((4,) + rr.shape))
+        return cls(regions, scalars, vals, k2, omega, mu, eps_t)
+
+    @staticmethod
+    def lagrange4(u):
+        return np.stack([-(u - 1) * (u - 2) * (u - 3) / 6.0, u * (u - 2) * (u - 3) / 2.0,
+                         -u * (u - 1) * (u - 3) / 2.0, u * (u - 1) * (u - 2) / 6.0], axis=-1)
+
+    def eval(self, R1, theta):
+        """(4, n) interpolated surfaces (_sommerfeld.py:831-876)."""
+        r = np.minimum(np.atleast_1d(np.asarray(R1, dtype=float)), self.r1_max)
+        th = np.clip(np.atleast_1d(np.asarray(theta, dtype=float)), 0.0, 0.5 * np.pi)
+        graze = th <= np.radians(TH_SPLIT_DEG)
+        reg = np.where(r <= self.r_break, np.where(graze, 0, 1),
+                       np.where(r <= self.r_near, np.where(graze, 2, 3), np.where(graze, 4, 5)))
+        out = np.empty((4, r.size), dtype=complex)
+        for idx, g in enumerate(self.regions):
+            s = np.nonzero(reg == idx)[0]
+            if s.size == 0:
+                continue
+            fr = (r[s] - g["r0"]) / g["dr"]
+            ft = (th[s] - g["th0"]) / g["dth"]
+            i0 = np.clip(np.floor(fr).astype(int) - 1, 0, g["n_r"] - 4)
+            j0 = np.clip(np.floor(ft).astype(int) - 1, 0, g["n_th"] - 4)
+            wr, wt = self.lagrange4(fr - i0), self.lagrange4(ft - j0)
+            ii = i0[:, None] + np.arange(4)
+            jj = j0[:, None] + np.arange(4)
+            blk = g["vals"][:, ii[:, :, None], jj[:, None, :]]
+            out[:, s] = np.einsum("snij,ni,nj->sn", blk, wr, wt)
+        return out
+
+    def scaled(self, k2, omega, mu):
+        """Physical-units view of a normalised master (_sommerfeld.py:878-914)."""
+        sc = self.k2 / k2
+        fac = (omega * mu) / (self.omega * self.mu)
+        g = object.__new__(Grid)
+        g.r1_max, g.r_break, g.r_near, g.r0_fill = (self.r1_max * sc, self.r_break * sc,
+                                                    self.r_near * sc, self.r0_fill * sc)
+        g.k2, g.omega, g.mu, g.eps_t = k2, omega, mu, self.eps_t
+        g.regions = [dict(r, r0=r["r0"] * sc, dr=r["dr"] * sc, vals=r["vals"] * fac)
+                     for r in self.regions]
+        return g
+
+
+def r1_bucket_wl(r1_wl):
+    """Round up to the next 1.25^n wavelengths, floor 0.1 (_sommerfeld.py:1048-1055)."""
+    x = max(float(r1_wl), 0.1)
+    b = 1.25 ** math.ceil(math.log(x, 1.25) - 1e-12)
+    return float(b * 1.25 if b < r1_wl else b)
+
+
+def eps_bucket(eps_t):
+    """Im(eps) onto the 1 % geometric ladder (_sommerfeld.py:1064-1078)."""
+    step = 1.0 + EPS_IM_STEP
+    re, im = eps_t.real, eps_t.imag
+    if not (re > 0.0) or im >= 0.0:
+        return eps_t
+    return complex(re, -(step ** round(math.log(-im, step))))
+
+
+_MASTERS = {}
+
+
+def get_grid(eps_t, k2, r1_max, omega, mu):
+    """Normalised master at lambda = 1, rescaled (_sommerfeld.py:1081-1116)."""
+    lam = 2.0 * np.pi / k2
+    r1b = r1_bucket_wl(min(r1_max / lam, R1_CAP_WL))
+    eb = eps_bucket(complex(eps_t))
+    key = (eb, r1b)
+    if key not in _MASTERS:
+        _MASTERS[key] = Grid.build(eb, K2_REF, r1b, omega=K2_REF * C_LIGHT, mu=MU0_EXACT)
+    return _MASTERS[key].scaled(k2, omega, mu)
+
+
+def max_image_distance(seg_l, seg_r, gz):
+    """Largest observer-to-image endpoint distance x 1.001 (_sommerfeld.py:154-190)."""
+    ex = np.concatenate([seg_l, seg_r])
+    d = np.sqrt((ex[:, None, 0] - ex[None, :, 0]) ** 2 + (ex[:, None, 1] - ex[None, :, 1]) ** 2
+                + ((ex[:, None, 2] - gz) + (ex[None, :, 2] - gz)) ** 2)
+    return float(d.max()) * 1.001
+
+
+# ---------------------------------------------------------------------------
+# remainder field and the Galerkin block
+# ---------------------------------------------------------------------------
+
+def remainder_proj(obs, t_obs, src, t_src, gz, k, grid):
+    """t_m . F(r_m, r_n) . t_n for a unit current moment (_sommerfeld.py:970-1010)."""
+    th_s = np.hypot(t_src[:, 0], t_src[:, 1])
+    ok = th_s > 1e-12
+    ux = np.where(ok, t_src[:, 0] / np.where(ok, th_s, 1.0), 1.0)
+    uy = np.where(ok, t_src[:, 1] / np.where(ok, th_s, 1.0), 0.0)
+    tz = t_src[:, 2]
+    dx = obs[:, 0][:, None] - src[:, 0][None, :]
+    dy = obs[:, 1][:, None] - src[:, 1][None, :]
+    rho = np.hypot(dx, dy)
+    hh = (obs[:, 2] - gz)[:, None] + (src[:, 2] - gz)[None, :]
+    r1 = np.sqrt(rho * rho + hh * hh)
+    IrV, IzV, IrH, IpH = grid.eval(r1.ravel(), np.arctan2(hh, rho).ravel()).reshape((4,) + r1.shape)
+    g = np.exp(-1j * k * r1) / r1
+    safe = rho > 1e-12 * grid.r1_max
+    inv = np.where(safe, 1.0 / np.where(safe, rho, 1.0), 0.0)
+    dhx = np.where(safe, dx * inv, ux[None, :])
+    dhy = np.where(safe, dy * inv, uy[None, :])
+    cphi = ux[None, :] * dhx + uy[None, :] * dhy
+    sphi = ux[None, :] * dhy - uy[None, :] * dhx
+    e_rho = g * (tz[None, :] * IrV + th_s[None, :] * cphi * IrH)
+    e_phi = g * th_s[None, :] * sphi * IpH
+    e_z = g * (tz[None, :] * IzV - th_s[None, :] * cphi * IrV)
+    return (t_obs[:, 0][:, None] * (dhx * e_rho - dhy * e_phi)
+            + t_obs[:, 1][:, None] * (dhy * e_rho + dhx * e_phi) + t_obs[:, 2][:, None] * e_z)
+
+
+def remainder_block(geom, supp, polys, gz, k, grid, n_qp=3):
+    """Q[m, n] (reference bspline.py:1791-1874 / _accelerators.cpp:6081-6253):
+    segment-pair field moments Jf[p, P, i, j] by n_qp-point Gauss-Legendre on
+    both segments, contracted with the basis polynomials."""
+    sl, sr, tan, h = geom["seg_l"], geom["seg_r"], geom["tangents"], geom["h"]
+    n = sl.shape[0]
+    d1 = polys.shape[1]
+    xg, wg = np.polynomial.legendre.leggauss(n_qp)
+    tq = 0.5 * (xg + 1.0)
+    nodes = sl[:, None, :] + tq[None, :, None] * (sr - sl)[:, None, :]
+    W = (0.5 * h[:, None] * wg[None, :])[None] * (h[:, None] * tq[None, :])[None] ** \
+        np.arange(d1)[:, None, None]
+    proj = remainder_proj(nodes.reshape(-1, 3), np.repeat(tan, n_qp, 0), nodes.reshape(-1, 3),
+                          np.repeat(tan, n_qp, 0), gz, k, grid).reshape(n, n_qp, n, n_qp)
+    Jf = np.einsum("piq,iqjr,Pjr->pPij", W, proj, W)
+    nb = supp.shape[0]
+    Q = np.zeros((nb, nb), dtype=complex)
+    for a in range(d1):
+        for b in range(d1):
+            blk = Jf[:, :, supp[:, a][:, None], supp[:, b][None, :]]
+            Q += np.einsum("mp,pPmn,nP->mn", polys[:, a, :], blk, polys[:, b, :])
+    return Q
+
+
+def eps_tilde(ground_eps, omega, eps0):
+    """(eps_r, sigma) -> eps_r - j sigma/(omega eps0) (_ground_refl.py:46-60)."""
+    if isinstance(ground_eps, (tuple, list)):
+        return complex(float(ground_eps[0]), -float(ground_eps[1]) / (omega * eps0))
+    return complex(ground_eps)
+
+
+def image_coefficient(eps_t):
+    """C2 = (eps - 1)/(eps + 1) (_ground_spec.py:114-121)."""
+    return (eps_t - 1.0) / (eps_t + 1.0)

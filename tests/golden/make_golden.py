@@ -88,7 +88,118 @@ def case_sweep_struct(index):
     return dict(sweep_k=ks, sweep_y=ys, y=sol.y, coeffs=sol.coeffs), time.time() - t0
 
 
+K2N = 2 * np.pi           # normalised wavenumber (lambda = 1)
+OMEGA_N = K2N * 299792458.0
+MU0 = 4e-7 * np.pi
+GROUNDS = [complex(5.0, -0.6), complex(15.0, -20.0), complex(30.0, -300.0)]
+RHO = np.array([0.0, 0.004, 0.03, 0.1, 0.3, 0.7, 1.5, 3.0])
+HH = np.array([0.002, 0.03, 0.12, 0.45, 1.3, 2.0])
+
+
+def case_somm_engine():
+    """Six lambda-integrals, the four surfaces and the R1 -> 0 limits at a
+    lattice of (rho, h) nodes for three grounds (reference
+    src/momwire/_sommerfeld.py:342-584)."""
+    from momwire import _sommerfeld as S
+    t0 = time.time()
+    rr, hh = np.meshgrid(RHO, HH, indexing="ij")
+    rho, h = rr.ravel(), hh.ravel()
+    out = dict(grounds=np.array(GROUNDS), rho=rho, h=h)
+    for g, eps in enumerate(GROUNDS):
+        out[f"six_{g}"] = S._six_integrals_batch(eps, K2N, rho, h, rtol=1e-6)
+        sel = np.arange(0, rho.size, 3)
+        out[f"six9_idx"] = sel
+        out[f"six9_{g}"] = S._six_integrals_batch(eps, K2N, rho[sel], h[sel], rtol=1e-9)
+        # forced forms in the overlap band rho ~ h .. 2h
+        ro = np.array([0.05, 0.2, 0.6, 1.0])
+        ho = np.array([0.04, 0.15, 0.4, 0.6])
+        out["form_rho"], out["form_h"] = ro, ho
+        out[f"formJ_{g}"] = S._six_integrals_batch(eps, K2N, ro, ho, rtol=1e-9, form="J")
+        out[f"formH_{g}"] = S._six_integrals_batch(eps, K2N, ro, ho, rtol=1e-9, form="H")
+        r1 = np.hypot(rho, h)
+        th = np.arctan2(h, rho)
+        surf = S.iv_surfaces_direct(eps, K2N, r1, th, rtol=1e-6, omega=OMEGA_N, mu=MU0)
+        out[f"surf_{g}"] = np.stack([surf[k] for k in S._SURF_KEYS])
+        th0 = np.radians([0.0, 15.0, 45.0, 89.9, 90.0])
+        lim = S.iv_surfaces_direct(eps, K2N, np.zeros_like(th0), th0, omega=OMEGA_N, mu=MU0)
+        out["lim_theta"] = th0
+        out[f"lim_{g}"] = np.stack([lim[k] for k in S._SURF_KEYS])
+    return out, time.time() - t0
+
+
+def _grid_dump(grid, prefix, rng, n_eval=300):
+    from momwire import _sommerfeld as S
+    out = {}
+    regs = grid._regions
+    out[prefix + "scalars"] = np.array([grid.r1_max, grid.r_break, grid.r_near, grid._r0_fill,
+                                        grid.k2, grid.omega, grid.mu])
+    out[prefix + "eps"] = np.array(grid.eps_t)
+    out[prefix + "layout"] = np.array([[r["r0"], r["dr"], r["n_r"], r["th0"], r["dth"], r["n_th"]]
+                                       for r in regs])
+    for i, r in enumerate(regs):
+        out[prefix + f"vals{i}"] = r["vals"]
+    R1 = rng.uniform(0.0, grid.r1_max, n_eval)
+    R1[:20] = rng.uniform(0.0, 0.05 * grid.r1_max, 20)
+    th = rng.uniform(0.0, 0.5 * np.pi, n_eval)
+    ev = grid.eval(R1, th)
+    out[prefix + "eval_r1"] = R1
+    out[prefix + "eval_th"] = th
+    out[prefix + "eval"] = np.stack([ev[k] for k in S._SURF_KEYS])
+    return out
+
+
+def _ground_solve(kw, extra=True):
+    from momwire import _sommerfeld as S
+    s, sol, dt = _solve(kw)
+    out = dict(y=sol.y, coeffs=sol.coeffs, k=s.k, omega=s.omega)
+    geom = s._build_geometry()
+    supp, polys, kcl, _wk, _wb = s._build_basis_polynomials(geom)
+    from momwire import _ground_refl
+    eps_t = _ground_refl.eps_tilde(s.ground_eps, s.omega, s.eps)
+    out["eps_t"] = np.array(eps_t)
+    r1_max = S.max_image_distance(geom["seg_l"], geom["seg_r"], s.ground_z)
+    out["r1_max"] = np.array(r1_max)
+    grid = s._somm_grid(eps_t, r1_max)
+    out.update(_grid_dump(grid, "grid_", np.random.default_rng(11)))
+    if extra:
+        out["Z"] = np.asarray(s._compute_Z_operator(geom, supp, polys))
+        out["Q"] = np.asarray(s._Z_sommerfeld_remainder(geom, supp, polys, eps_t))
+    return out, dt
+
+
+def case_somm_grid_far():
+    """A master grid whose extent reaches the far zone (six regions)."""
+    from momwire import _sommerfeld as S
+    t0 = time.time()
+    grid = S.get_grid(GROUNDS[1], K2N, 6.0, omega=OMEGA_N, mu=MU0)
+    out = _grid_dump(grid, "grid_", np.random.default_rng(12), n_eval=400)
+    for i in range(len(grid._regions)):
+        del out[f"grid_vals{i}"]
+    return out, time.time() - t0
+
+
+def case_ground_dipole():
+    kw = workloads.grounded_array(1, 1, 21, params=(0.3, 12.0, 0.01, 10e6))
+    return _ground_solve(kw)
+
+
+def case_ground_tee():
+    return _ground_solve(workloads.grounded_tee())
+
+
+def case_c5():
+    out, dt = _ground_solve(workloads.grounded_array(4, 4, 21, seed=4), extra=False)
+    for k in [k for k in out if k.startswith("grid_vals")]:
+        del out[k]
+    return out, dt
+
+
 CASES = {
+    "somm_engine": case_somm_engine,
+    "somm_grid_far": case_somm_grid_far,
+    "ground_dipole": case_ground_dipole,
+    "ground_tee": case_ground_tee,
+    "c5_ground4x4": case_c5,
     "sweep_struct4": lambda: case_sweep_struct(4),
     "sweep_struct8": lambda: case_sweep_struct(8),
     "c1_dipole": case_c1,
