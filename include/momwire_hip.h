@@ -258,6 +258,75 @@ int mw_z2c(int64_t rows, int64_t cols, const void* src, int64_t lds, void* dst,
 int mw_c2z(int64_t rows, int64_t cols, const void* src, int64_t lds, void* dst,
            int64_t ldd, int32_t accumulate, void* stream);
 
+/* ------------------------------------------------------------------------
+ * Sommerfeld ground (ground_model="sommerfeld")
+ * ------------------------------------------------------------------------ */
+
+/*
+ * The tabulated remainder surfaces I_rho^V, I_z^V, I_rho^H, I_phi^H of one
+ * ground: 4 or 6 (R1, theta) regions, each a uniform n_r x n_th lattice
+ * starting at (r0, th0) with spacings (dr, dth).  vals holds, region after
+ * region, [n_r][n_th][4] interleaved complex values; off[g] is region g's
+ * first node.  Lengths are physical (the lattice of a normalised master
+ * scaled by lambda); every interpolated value is multiplied by value_scale
+ * (the omega*mu ratio that rescales a normalised master).  Regions: 0/1 are
+ * R1 <= r_break, 2/3 up to r_near, 4/5 beyond; even ones theta <= th_split.
+ * Mirrors SommerfeldGrid (reference src/momwire/_sommerfeld.py:642-914) and
+ * its flattened form grid_cpp_args (:917-937).
+ */
+typedef struct mw_somm_grid {
+  int32_t n_reg, pad;
+  double r1_max, r_break, th_split, r_near, value_scale;
+  double r0[6], dr[6], th0[6], dth[6];
+  int32_t n_r[6], n_th[6];
+  int64_t off[6];
+  const double* vals;
+} mw_somm_grid;
+
+/*
+ * The six lambda-integrals (V_rr, V_zz, V_rz, V_r1, V, U) of NEC eqs 148-153 at
+ * n (rho, h) nodes (device arrays), out: [n][6] complex.  form 0 = automatic
+ * contour (Bessel for rho < 2h), 1 = Bessel, 2 = Hankel (needs rho > 0).
+ * Replaces somm_six_integrals_batch (reference _accelerators.cpp:5771-5823,
+ * python twin _sommerfeld.py:468-502).  Nodes must satisfy rho, h >= 0 and
+ * rho + h > 0 (not checked on the device: the arrays are device memory).
+ */
+int mw_somm_six_integrals(double eps_re, double eps_im, double k2, const double* rho,
+                          const double* h, int64_t n, double rtol, int32_t form, void* out,
+                          const volatile int32_t* cancel, void* stream);
+
+/*
+ * The four surfaces at n (R1, theta) nodes, out: [n][4] complex (R1 = 0 takes
+ * the limits of eqs 169-172).  Replaces iv_surfaces_direct
+ * (reference _sommerfeld.py:535-584), the grid's fill function.
+ */
+int mw_somm_surfaces(double eps_re, double eps_im, double k2, double omega, double mu,
+                     const double* R1, const double* theta, int64_t n, double rtol,
+                     void* out, const volatile int32_t* cancel, void* stream);
+
+/*
+ * 4x4 Lagrange interpolation of a grid at n points, out: [n][4] complex.
+ * Replaces SommerfeldGrid.eval (reference _sommerfeld.py:831-876).
+ */
+int mw_somm_eval(const mw_somm_grid* grid, const double* R1, const double* theta, int64_t n,
+                 void* out, void* stream);
+
+/*
+ * Grounded fill: Z -= C2 * Z_image + Q, accumulated into a Z already holding
+ * the free-space matrix (row-major, ld ldz).  Z_image is the exact image
+ * (mirrored sources, tangent dot t_m . (t_x, t_y, -t_z), charge weight 1);
+ * Q the Galerkin remainder block of the tabulated field (3-point Gauss per
+ * segment).  grid = NULL drops Q (image only).  Same tiling and symmetric
+ * tile-pair list as mw_fill_z.  Replaces the weighted image assembly and the
+ * fused remainder kernel: assemble_Z_bspline_weighted_windowed and
+ * sommerfeld_remainder_bspline_Q (reference _accelerators.cpp:1498-1608,
+ * 6081-6253; callers bspline.py:2552-2661, 1791-1874, 3730-3770).
+ */
+int mw_fill_ground(const mw_mesh* mesh, double k, double omega, double eps, double mu,
+                   double ground_z, double c2_re, double c2_im, const mw_somm_grid* grid,
+                   void* Z, int64_t ldz, const int32_t* tile_pairs, int32_t n_pairs,
+                   int32_t symmetric, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -109,6 +109,11 @@ SIGNATURES = {
                                             ctypes.POINTER(ctypes.c_void_p)]),
     "mw_sweep_run": (ctypes.c_int, [_P, _D, _D, _P, _P]),
     "mw_sweep_plan_destroy": (ctypes.c_int, [_P]),
+    "mw_somm_six_integrals": (ctypes.c_int, [_D, _D, _D, _P, _P, _I64, _D, _I32, _P, _P, _P]),
+    "mw_somm_surfaces": (ctypes.c_int, [_D, _D, _D, _D, _D, _P, _P, _I64, _D, _P, _P, _P]),
+    "mw_somm_eval": (ctypes.c_int, [_P, _P, _P, _I64, _P, _P]),
+    "mw_fill_ground": (ctypes.c_int, [ctypes.POINTER(MwMesh), _D, _D, _D, _D, _D, _D, _D, _P, _P,
+                                      _I64, _P, _I32, _I32, _P]),
     "mw_z2c": (ctypes.c_int, [_I64, _I64, _P, _I64, _P, _I64, _P]),
     "mw_c2z": (ctypes.c_int, [_I64, _I64, _P, _I64, _P, _I64, _I32, _P]),
 }

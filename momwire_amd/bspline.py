@@ -8,16 +8,21 @@ LU factorisation, the all-port solve, the KCL Schur step and the Y readout run
 as HIP kernels on the GPU (momwire_amd/csrc); Python only prepares the
 k-independent tables and moves results.  There is no CPU fallback.
 
-Supported: free space; degree 2; n_qp_pair 4; point delta-gap feeds; K-wire
-junctions (KCL rows), junction ports and node gaps; per-wire radii.
+Supported: free space, PEC ground and the Sommerfeld ground
+(``ground_model="sommerfeld"``: C2-scaled exact image plus the interpolated
+remainder); degree 2; n_qp_pair 4; point delta-gap feeds; K-wire junctions
+(KCL rows), junction ports and node gaps; per-wire radii.
 Refused with NotImplementedError (as the reference refuses combinations it
-does not serve): ground planes, wire loading, extended kernel, singular
-enrichment, segment-gap and smoothed feeds.
+does not serve): the reflection-coefficient ground, wires touching the ground
+plane, wire loading, extended kernel, singular enrichment, segment-gap and
+smoothed feeds.
 """
+
+import ctypes
 
 import numpy as np
 
-from . import _lib
+from . import _ground, _lib
 from ._mesh import EPS0, MU0, build_mesh, feed_column, sparse_pack, bspline_values
 from ._port_solution import PortSolution, _SweptPortSolutions
 from ._lib import SolveAborted  # noqa: F401  (re-exported, reference _cancel.py)
@@ -181,8 +186,15 @@ class BSplineSolver(_SweptPortSolutions):
             raise NotImplementedError("wire loading is not served")
         if ground_eps is not None and ground_z is None:
             raise ValueError("ground_eps requires ground_z to be set")
-        if ground_z is not None:
-            raise NotImplementedError("ground planes are not served by this solver yet")
+        if ground_model not in ("refl-coef", "sommerfeld"):
+            raise ValueError(f"ground_model must be 'refl-coef' or 'sommerfeld', got {ground_model!r}")
+        if ground_model == "sommerfeld" and ground_eps is None:
+            raise ValueError("ground_model='sommerfeld' requires ground_eps")
+        if ground_z is not None and ground_eps is not None and ground_model == "refl-coef":
+            raise NotImplementedError("the reflection-coefficient ground is not served; "
+                                      "use ground_model='sommerfeld'")
+        if ground_model == "sommerfeld" and int(n_qp_sommerfeld) != 3:
+            raise NotImplementedError("the remainder block is built for n_qp_sommerfeld=3")
         if int(swept_mem_mb) < 1:
             raise ValueError(f"swept_mem_mb must be >= 1, got {swept_mem_mb}")
 
@@ -201,7 +213,12 @@ class BSplineSolver(_SweptPortSolutions):
         self.halfdriver_factor = halfdriver_factor
         self.wire_radius = wire_radius
         self.nsegs = nsegs
-        self.ground_z = None
+        self.ground_z = None if ground_z is None else float(ground_z)
+        self.ground_eps = ground_eps
+        self.ground_model = ground_model
+        self.ground_phi_mode = ground_phi_mode
+        self.n_qp_sommerfeld = int(n_qp_sommerfeld)
+        self.somm_surrogate = None
         self.swept_mem_mb = int(swept_mem_mb)
         self._cancel = cancel
         # bspline.py:672-676
@@ -223,6 +240,13 @@ class BSplineSolver(_SweptPortSolutions):
         if not np.all(np.isfinite(r)) or np.any(r <= 0):
             raise ValueError(f"wire_radius entries must be positive and finite, got {r}")
         self._radius_per_wire = r
+        if self.ground_z is not None:
+            zmin = min(float(pl[:, 2].min()) for pl in self.wires_polylines) - self.ground_z
+            if zmin < -1e-12:
+                raise ValueError("every wire must lie at or above ground_z "
+                                 f"(min height above the plane: {zmin:.3g})")
+            if zmin <= 1e-12:
+                raise NotImplementedError("wires touching the ground plane are not served")
 
         if n_per_edge_per_wire is None:
             n_per_edge_per_wire = [None] * n_w
@@ -399,11 +423,8 @@ class BSplineSolver(_SweptPortSolutions):
         cancel = self._cancel.address if self._cancel is not None else None
         mark("fill", True)
         _lib.check(lib.mw_same_edge_table(D.mw, kk, st), "same-edge table")
-        pairs = D.keep["pairs"]
         ld = D.ld
-        _lib.check(lib.mw_fill_z(D.mw, kk, omega, self.eps, self.mu, D.Z.data_ptr(), ld,
-                                 pairs.data_ptr(), int(pairs.shape[0]), int(mesh.symmetric), st),
-                   "fill")
+        self._fill(D, kk, omega, D.Z.data_ptr(), ld, st)
         mark("fill", False)
         _lib.check(lib.mw_zscatter_cols(nb, ncols, D.col_max, D.col_idx.data_ptr(),
                                         D.col_val.data_ptr(), D.X.data_ptr(), ld, st),
@@ -415,6 +436,39 @@ class BSplineSolver(_SweptPortSolutions):
         mark("solve", False)
         self._post(D, D.X.data_ptr(), ld, D.Y.data_ptr(), st)
         return D.Y[:D.n_ports, :D.n_ports], D.X[:, :D.n_ports]
+
+    def _fill(self, D, k, omega, zptr, ld, st):
+        """Free-space fill, then the ground terms Z -= C2 * image + Q."""
+        lib = _lib.load()
+        mesh = self._mesh
+        pairs = D.keep["pairs"]
+        _lib.check(lib.mw_fill_z(D.mw, k, omega, self.eps, self.mu, zptr, ld, pairs.data_ptr(),
+                                 int(pairs.shape[0]), int(mesh.symmetric), st), "fill")
+        if self.ground_z is None:
+            return
+        grid = self._ground_grid(k, omega)
+        eps_t = None if self.ground_eps is None else _ground.eps_tilde(self.ground_eps, omega,
+                                                                      self.eps)
+        c2 = _ground.image_coefficient(eps_t)
+        gref = ctypes.byref(grid) if grid is not None else None
+        _lib.check(lib.mw_fill_ground(D.mw, k, omega, self.eps, self.mu, self.ground_z, c2.real,
+                                      c2.imag, gref, zptr, ld, pairs.data_ptr(),
+                                      int(pairs.shape[0]), int(mesh.symmetric), st),
+                   "ground fill")
+
+    def _ground_grid(self, k, omega):
+        """The Sommerfeld grid view at this frequency (None for PEC)."""
+        if self.ground_model != "sommerfeld" or self.ground_eps is None:
+            return None
+        mesh = self._build()
+        seg = mesh.seg
+        eps_t = _ground.eps_tilde(self.ground_eps, omega, self.eps)
+        r1 = _ground.max_image_distance(seg[:, 0:3], seg[:, 3:6], self.ground_z)
+        filler = None if self.somm_surrogate is None else self.somm_surrogate.grid_filler()
+        key = None if self.somm_surrogate is None else id(self.somm_surrogate)
+        master = _ground.get_master(eps_t, k, r1, cancel=self._cancel, filler=filler, key_extra=key)
+        self._grid_master = master
+        return master.struct(k, omega, self.mu)
 
     def _post(self, D, xptr, ldx, yptr, st):
         """KCL Schur step (bspline.py:2981-2989) and the readout Y = B^T X
@@ -467,9 +521,7 @@ class BSplineSolver(_SweptPortSolutions):
         pairs = D.keep["pairs"]
         mark("fill", True)
         _lib.check(lib.mw_same_edge_table(D.mw, kk, st), "same-edge table")
-        _lib.check(lib.mw_fill_z(D.mw, kk, omega, self.eps, self.mu, D.Z.data_ptr(), nb,
-                                 pairs.data_ptr(), int(pairs.shape[0]), int(mesh.symmetric), st),
-                   "fill")
+        self._fill(D, kk, omega, D.Z.data_ptr(), nb, st)
         mark("fill", False)
         mark("solve32", True)
         _lib.check(lib.mw_zscatter_cols(nb, ncols, D.col_max, D.col_idx.data_ptr(),

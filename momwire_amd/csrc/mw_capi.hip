@@ -48,6 +48,19 @@ int mw_sweep_lu_run(const mw_sweep_prob* dP, const mw_ports* dports, const mw_sw
 hipError_t mw_launch_small_solve(int ns, int nt, void* S, int64_t lds, void* T, int64_t ldt,
                                  int32_t* info, hipStream_t st);
 
+hipError_t mw_launch_somm_six(double er, double ei, double k2, const double* rho, const double* h,
+                              int64_t n, double rtol, int form, double2* out,
+                              const volatile int32_t* cancel, hipStream_t st);
+hipError_t mw_launch_somm_surfaces(double er, double ei, double k2, double omega, double mu,
+                                   const double* R1, const double* th, int64_t n, double rtol,
+                                   double2* out, const volatile int32_t* cancel, hipStream_t st);
+hipError_t mw_launch_somm_eval(const mw_somm_grid& G, const double* R1, const double* th, int64_t n,
+                               double2* out, hipStream_t st);
+hipError_t mw_launch_fill_ground(const mw_mesh& M, double k, double omega, double eps, double mu,
+                                 double gz, double c2r, double c2i, const mw_somm_grid* G,
+                                 double2* Z, int64_t ldz, const int32_t* pairs, int32_t n_pairs,
+                                 bool symmetric, hipStream_t st);
+
 namespace {
 thread_local char g_err[512] = "";
 
@@ -370,3 +383,86 @@ int mw_sweep_plan_destroy(mw_sweep_plan* p) {
 }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Sommerfeld ground
+// ---------------------------------------------------------------------------
+namespace {
+int check_grid(const mw_somm_grid* G) {
+  if (!G) return fail(MW_EINVAL, "%sgrid is NULL");
+  if (G->n_reg != 4 && G->n_reg != 6)
+    return fail(MW_EINVAL, "%sgrid must have 4 or 6 regions, got %lld", "", G->n_reg);
+  if (!G->vals) return fail(MW_EINVAL, "%sgrid values are NULL");
+  if (!(G->r1_max > 0.0) || !(G->r_break > 0.0) || !(G->r_near > 0.0))
+    return fail(MW_EINVAL, "%sgrid radii must be positive");
+  for (int g = 0; g < G->n_reg; ++g) {
+    if (G->n_r[g] < 4 || G->n_th[g] < 4)
+      return fail(MW_EINVAL, "%sregion %lld has fewer than 4 x 4 nodes", "", g);
+    if (!(G->dr[g] > 0.0) || !(G->dth[g] > 0.0))
+      return fail(MW_EINVAL, "%sregion %lld spacing must be positive", "", g);
+    if (G->off[g] < 0) return fail(MW_EINVAL, "%sregion %lld offset is negative", "", g);
+  }
+  return MW_OK;
+}
+}  // namespace
+
+int mw_somm_six_integrals(double eps_re, double eps_im, double k2, const double* rho,
+                          const double* h, int64_t n, double rtol, int32_t form, void* out,
+                          const volatile int32_t* cancel, void* stream) {
+  g_err[0] = 0;
+  if (n < 0) return fail(MW_EINVAL, "%snode count is negative");
+  if (n > 0 && (!rho || !h || !out)) return fail(MW_EINVAL, "%snode or output array is NULL");
+  if (!(k2 > 0.0)) return fail(MW_EINVAL, "%sk2 must be positive");
+  if (!(rtol > 0.0)) return fail(MW_EINVAL, "%srtol must be positive");
+  if (form < 0 || form > 2) return fail(MW_EINVAL, "%sform must be 0 (auto), 1 (J) or 2 (H)");
+  if (!(eps_re == eps_re) || !(eps_im == eps_im)) return fail(MW_EINVAL, "%seps is NaN");
+  int rc = hip_status(mw_launch_somm_six(eps_re, eps_im, k2, rho, h, n, rtol, form,
+                                         (double2*)out, cancel, (hipStream_t)stream),
+                      "mw_somm_six_integrals");
+  return rc;
+}
+
+int mw_somm_surfaces(double eps_re, double eps_im, double k2, double omega, double mu,
+                     const double* R1, const double* theta, int64_t n, double rtol,
+                     void* out, const volatile int32_t* cancel, void* stream) {
+  g_err[0] = 0;
+  if (n < 0) return fail(MW_EINVAL, "%snode count is negative");
+  if (n > 0 && (!R1 || !theta || !out)) return fail(MW_EINVAL, "%snode or output array is NULL");
+  if (!(k2 > 0.0) || !(omega > 0.0) || !(mu > 0.0))
+    return fail(MW_EINVAL, "%sk2, omega and mu must be positive");
+  if (!(rtol > 0.0)) return fail(MW_EINVAL, "%srtol must be positive");
+  if (!(eps_re == eps_re) || !(eps_im == eps_im)) return fail(MW_EINVAL, "%seps is NaN");
+  return hip_status(mw_launch_somm_surfaces(eps_re, eps_im, k2, omega, mu, R1, theta, n, rtol,
+                                            (double2*)out, cancel, (hipStream_t)stream),
+                    "mw_somm_surfaces");
+}
+
+int mw_somm_eval(const mw_somm_grid* grid, const double* R1, const double* theta, int64_t n,
+                 void* out, void* stream) {
+  g_err[0] = 0;
+  int rc = check_grid(grid);
+  if (rc) return rc;
+  if (n < 0) return fail(MW_EINVAL, "%spoint count is negative");
+  if (n > 0 && (!R1 || !theta || !out)) return fail(MW_EINVAL, "%spoint or output array is NULL");
+  return hip_status(mw_launch_somm_eval(*grid, R1, theta, n, (double2*)out, (hipStream_t)stream),
+                    "mw_somm_eval");
+}
+
+int mw_fill_ground(const mw_mesh* mesh, double k, double omega, double eps, double mu,
+                   double ground_z, double c2_re, double c2_im, const mw_somm_grid* grid,
+                   void* Z, int64_t ldz, const int32_t* tile_pairs, int32_t n_pairs,
+                   int32_t symmetric, void* stream) {
+  g_err[0] = 0;
+  int rc = check_mesh(mesh);
+  if (rc) return rc;
+  if (grid && (rc = check_grid(grid))) return rc;
+  if (!Z) return fail(MW_EINVAL, "%sZ is NULL");
+  if (ldz < mesh->n_basis) return fail(MW_EINVAL, "%sldz %lld < n_basis %lld", "", ldz, mesh->n_basis);
+  if (!(k > 0.0) || !(omega > 0.0)) return fail(MW_EINVAL, "%sk and omega must be positive");
+  if (symmetric && (!tile_pairs || n_pairs <= 0))
+    return fail(MW_EINVAL, "%ssymmetric fill needs the tile-pair list");
+  return hip_status(mw_launch_fill_ground(*mesh, k, omega, eps, mu, ground_z, c2_re, c2_im, grid,
+                                          (double2*)Z, ldz, tile_pairs, n_pairs, symmetric != 0,
+                                          (hipStream_t)stream),
+                    "mw_fill_ground");
+}
