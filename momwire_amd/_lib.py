@@ -78,6 +78,12 @@ class MwSweepProb(ctypes.Structure):
     ]
 
 
+class MwMlp(ctypes.Structure):
+    _fields_ = [("params", ctypes.c_void_p), ("grads", ctypes.c_void_p),
+                ("adam_m", ctypes.c_void_p), ("adam_v", ctypes.c_void_p),
+                ("w16", ctypes.c_void_p), ("wt16", ctypes.c_void_p)]
+
+
 _P = ctypes.c_void_p
 _I64 = ctypes.c_int64
 _I32 = ctypes.c_int32
@@ -114,6 +120,17 @@ SIGNATURES = {
     "mw_somm_eval": (ctypes.c_int, [_P, _P, _P, _I64, _P, _P]),
     "mw_fill_ground": (ctypes.c_int, [ctypes.POINTER(MwMesh), _D, _D, _D, _D, _D, _D, _D, _P, _P,
                                       _I64, _P, _I32, _I32, _P]),
+    "mw_somm_surfaces_varied": (ctypes.c_int, [_P, _D, _D, _D, _P, _P, _I64, _D, _P, _P, _P]),
+    "mw_mlp_param_count": (_I64, []),
+    "mw_mlp_w16_count": (_I64, []),
+    "mw_mlp_workspace_bytes": (_I64, [_I64]),
+    "mw_mlp_sync_weights": (ctypes.c_int, [ctypes.POINTER(MwMlp), _P]),
+    "mw_mlp_forward": (ctypes.c_int, [ctypes.POINTER(MwMlp), _P, _I64, _P, _P, _P]),
+    "mw_mlp_backward": (ctypes.c_int, [ctypes.POINTER(MwMlp), _P, _P, _I64, _P, _P, _P]),
+    "mw_mlp_adam_step": (ctypes.c_int, [ctypes.POINTER(MwMlp), ctypes.c_float, ctypes.c_float,
+                                        ctypes.c_float, ctypes.c_float, _I32, _P]),
+    "mw_mlp_grid_values": (ctypes.c_int, [ctypes.POINTER(MwMlp), _D, _D, _P, _P, _P, _I64, _P, _D,
+                                          _D, _P, _P]),
     "mw_z2c": (ctypes.c_int, [_I64, _I64, _P, _I64, _P, _I64, _P]),
     "mw_c2z": (ctypes.c_int, [_I64, _I64, _P, _I64, _P, _I64, _I32, _P]),
 }

@@ -165,7 +165,7 @@ class BSplineSolver(_SweptPortSolutions):
                  nsegs=101, ground_z=None, ground_eps=None, ground_phi_mode="normal",
                  ground_model="refl-coef", n_qp_sommerfeld=3, use_singular_enrichment=False,
                  swept_mem_mb=256, cancel=None, precision="fp64", refine_steps=None,
-                 **unsupported):
+                 somm_surrogate=None, **unsupported):
         if unsupported:
             raise NotImplementedError(f"options not served on the GPU path: {sorted(unsupported)}")
         if degree < 1:
@@ -218,7 +218,9 @@ class BSplineSolver(_SweptPortSolutions):
         self.ground_model = ground_model
         self.ground_phi_mode = ground_phi_mode
         self.n_qp_sommerfeld = int(n_qp_sommerfeld)
-        self.somm_surrogate = None
+        # a trained momwire_amd.surrogate.SommerfeldSurrogate fills the grid
+        # nodes of its domain in place of the exact engine
+        self.somm_surrogate = somm_surrogate
         self.swept_mem_mb = int(swept_mem_mb)
         self._cancel = cancel
         # bspline.py:672-676

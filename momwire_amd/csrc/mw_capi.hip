@@ -61,6 +61,24 @@ hipError_t mw_launch_fill_ground(const mw_mesh& M, double k, double omega, doubl
                                  double2* Z, int64_t ldz, const int32_t* pairs, int32_t n_pairs,
                                  bool symmetric, hipStream_t st);
 
+hipError_t mw_launch_somm_surfaces_varied(const double* eps, double k2, double omega, double mu,
+                                          const double* R1, const double* th, int64_t n, double rtol,
+                                          double2* out, const volatile int32_t* cancel,
+                                          hipStream_t st);
+int64_t mw_mlp_n_params();
+int64_t mw_mlp_n_w16();
+int64_t mw_mlp_ws_bytes(int64_t n);
+hipError_t mw_mlp_launch_sync(const mw_mlp& net, hipStream_t st);
+hipError_t mw_mlp_launch_forward(const mw_mlp& net, const float* x, int64_t n, float* out, void* ws,
+                                 hipStream_t st);
+hipError_t mw_mlp_launch_backward(const mw_mlp& net, const float* x, const float* t, int64_t n,
+                                  float* loss_out, void* ws, hipStream_t st);
+hipError_t mw_mlp_launch_adam(const mw_mlp& net, float lr, float b1, float b2, float eps, int step,
+                              hipStream_t st);
+hipError_t mw_mlp_launch_grid(const mw_mlp& net, double eps_re, double eps_im, const double* R1,
+                              const double* th, const int32_t* idx, int64_t n, double2* vals,
+                              double sre, double sim, void* ws, hipStream_t st);
+
 namespace {
 thread_local char g_err[512] = "";
 
@@ -465,4 +483,91 @@ int mw_fill_ground(const mw_mesh* mesh, double k, double omega, double eps, doub
                                           (double2*)Z, ldz, tile_pairs, n_pairs, symmetric != 0,
                                           (hipStream_t)stream),
                     "mw_fill_ground");
+}
+
+int mw_somm_surfaces_varied(const double* eps, double k2, double omega, double mu,
+                            const double* R1, const double* theta, int64_t n, double rtol,
+                            void* out, const volatile int32_t* cancel, void* stream) {
+  g_err[0] = 0;
+  if (n < 0) return fail(MW_EINVAL, "%snode count is negative");
+  if (n > 0 && (!eps || !R1 || !theta || !out))
+    return fail(MW_EINVAL, "%snode, permittivity or output array is NULL");
+  if (!(k2 > 0.0) || !(omega > 0.0) || !(mu > 0.0))
+    return fail(MW_EINVAL, "%sk2, omega and mu must be positive");
+  if (!(rtol > 0.0)) return fail(MW_EINVAL, "%srtol must be positive");
+  return hip_status(mw_launch_somm_surfaces_varied(eps, k2, omega, mu, R1, theta, n, rtol,
+                                                   (double2*)out, cancel, (hipStream_t)stream),
+                    "mw_somm_surfaces_varied");
+}
+
+// ---------------------------------------------------------------------------
+// learned surrogate
+// ---------------------------------------------------------------------------
+namespace {
+int check_mlp(const mw_mlp* net, bool train) {
+  if (!net || !net->params || !net->w16 || !net->wt16)
+    return fail(MW_EINVAL, "%snetwork buffers are NULL");
+  if (train && (!net->grads || !net->adam_m || !net->adam_v))
+    return fail(MW_EINVAL, "%straining buffers are NULL");
+  return MW_OK;
+}
+}  // namespace
+
+int64_t mw_mlp_param_count(void) { return mw_mlp_n_params(); }
+int64_t mw_mlp_w16_count(void) { return mw_mlp_n_w16(); }
+int64_t mw_mlp_workspace_bytes(int64_t n) { return n > 0 ? mw_mlp_ws_bytes((n + 63) / 64 * 64) : 0; }
+
+int mw_mlp_sync_weights(const mw_mlp* net, void* stream) {
+  g_err[0] = 0;
+  int rc = check_mlp(net, false);
+  if (rc) return rc;
+  return hip_status(mw_mlp_launch_sync(*net, (hipStream_t)stream), "mw_mlp_sync_weights");
+}
+
+int mw_mlp_forward(const mw_mlp* net, const float* x, int64_t n, float* out, void* ws,
+                   void* stream) {
+  g_err[0] = 0;
+  int rc = check_mlp(net, false);
+  if (rc) return rc;
+  if (n <= 0) return fail(MW_EINVAL, "%sbatch must be positive");
+  if (!x || !out || !ws) return fail(MW_EINVAL, "%sinput, output or workspace is NULL");
+  return hip_status(mw_mlp_launch_forward(*net, x, n, out, ws, (hipStream_t)stream),
+                    "mw_mlp_forward");
+}
+
+int mw_mlp_backward(const mw_mlp* net, const float* x, const float* target, int64_t n,
+                    float* loss, void* ws, void* stream) {
+  g_err[0] = 0;
+  int rc = check_mlp(net, true);
+  if (rc) return rc;
+  if (n <= 0) return fail(MW_EINVAL, "%sbatch must be positive");
+  if (!x || !target || !ws) return fail(MW_EINVAL, "%sinput, target or workspace is NULL");
+  return hip_status(mw_mlp_launch_backward(*net, x, target, n, loss, ws, (hipStream_t)stream),
+                    "mw_mlp_backward");
+}
+
+int mw_mlp_adam_step(const mw_mlp* net, float lr, float beta1, float beta2, float eps,
+                     int32_t step, void* stream) {
+  g_err[0] = 0;
+  int rc = check_mlp(net, true);
+  if (rc) return rc;
+  if (step < 1) return fail(MW_EINVAL, "%sAdam step counts from 1");
+  if (!(lr >= 0.f) || !(beta1 >= 0.f && beta1 < 1.f) || !(beta2 >= 0.f && beta2 < 1.f) || !(eps > 0.f))
+    return fail(MW_EINVAL, "%sAdam hyper-parameters out of range");
+  return hip_status(mw_mlp_launch_adam(*net, lr, beta1, beta2, eps, step, (hipStream_t)stream),
+                    "mw_mlp_adam_step");
+}
+
+int mw_mlp_grid_values(const mw_mlp* net, double eps_re, double eps_im, const double* R1,
+                       const double* theta, const int32_t* idx, int64_t n, void* vals,
+                       double value_re, double value_im, void* ws, void* stream) {
+  g_err[0] = 0;
+  int rc = check_mlp(net, false);
+  if (rc) return rc;
+  if (n < 0) return fail(MW_EINVAL, "%snode count is negative");
+  if (n > 0 && (!R1 || !theta || !idx || !vals || !ws))
+    return fail(MW_EINVAL, "%snode, index, value or workspace array is NULL");
+  return hip_status(mw_mlp_launch_grid(*net, eps_re, eps_im, R1, theta, idx, n, (double2*)vals,
+                                       value_re, value_im, ws, (hipStream_t)stream),
+                    "mw_mlp_grid_values");
 }

@@ -466,8 +466,9 @@ __global__ __launch_bounds__(256) void k_somm_six(double er, double ei, double k
 
 // The four surfaces I_rho^V, I_z^V, I_rho^H, I_phi^H at (R1, theta)
 // (reference _sommerfeld.py:535-584), R1 = 0 by the limits of eqs 169-172.
-__global__ __launch_bounds__(256) void k_somm_surfaces(double er, double ei, double k2,
-                                                       double omega, double mu,
+__global__ __launch_bounds__(256) void k_somm_surfaces(double er, double ei,
+                                                       const double* __restrict__ eps_node,
+                                                       double k2, double omega, double mu,
                                                        const double* __restrict__ R1,
                                                        const double* __restrict__ theta,
                                                        int64_t n, double rtol,
@@ -477,7 +478,7 @@ __global__ __launch_bounds__(256) void k_somm_surfaces(double er, double ei, dou
   if (node >= n) return;
   if (cancel && *cancel) return;
   const int lane = threadIdx.x & 63;
-  const cd eps = cz(er, ei);
+  const cd eps = eps_node ? cz(eps_node[2 * node], eps_node[2 * node + 1]) : cz(er, ei);
   const double k2s = k2 * k2;
   const cd k1s = zsc(eps, k2s);
   const cd c1 = cz(0.0, -omega * mu / (4.0 * PI * k2s));
@@ -870,8 +871,18 @@ hipError_t mw_launch_somm_surfaces(double er, double ei, double k2, double omega
                                    const double* R1, const double* th, int64_t n, double rtol,
                                    double2* out, const volatile int32_t* cancel, hipStream_t st) {
   if (n <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_somm_surfaces, dim3((unsigned)((n + 3) / 4)), dim3(256), 0, st, er, ei, k2,
-                     omega, mu, R1, th, n, rtol, out, cancel);
+  hipLaunchKernelGGL(k_somm_surfaces, dim3((unsigned)((n + 3) / 4)), dim3(256), 0, st, er, ei,
+                     (const double*)nullptr, k2, omega, mu, R1, th, n, rtol, out, cancel);
+  return hipGetLastError();
+}
+
+hipError_t mw_launch_somm_surfaces_varied(const double* eps, double k2, double omega, double mu,
+                                          const double* R1, const double* th, int64_t n, double rtol,
+                                          double2* out, const volatile int32_t* cancel,
+                                          hipStream_t st) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_somm_surfaces, dim3((unsigned)((n + 3) / 4)), dim3(256), 0, st, 0.0, 0.0, eps,
+                     k2, omega, mu, R1, th, n, rtol, out, cancel);
   return hipGetLastError();
 }
 

@@ -1,0 +1,154 @@
+"""The learned Sommerfeld surrogate on the GPU: forward and gradients against
+the CPU oracle (oracle/mlp_oracle.py), the Adam step, a short training run,
+the engine fallback, and what the trained network gains.
+
+Tolerances: the GPU accumulates in FP32 (MFMA) where the oracle sums in FP64,
+so a BF16 rounding of an activation can land on the other side of a rounding
+boundary; outputs and per-layer gradients are compared at 2e-3 of their
+largest entry (one BF16 ulp is 3.9e-3 relative).
+"""
+
+import numpy as np
+import pytest
+
+from conftest import rel_err
+from momwire_amd import _ground, surrogate as sg, workloads as W
+from momwire_amd.bspline import BSplineSolver
+from oracle import mlp_oracle as mo
+
+pytestmark = pytest.mark.gpu
+torch = pytest.importorskip("torch")
+
+
+def _say(*a):
+    print("[gpu-test]", *a, flush=True)
+
+
+def _batch(n, seed):
+    x = sg.sample_inputs(n, seed).astype(np.float32)
+    t = np.random.default_rng(seed).standard_normal((n, 8)).astype(np.float32) * 0.2
+    return x, t
+
+
+def test_forward_matches_oracle():
+    _say("mlp forward")
+    p = mo.init_params(3)
+    net = sg.SommerfeldSurrogate(p)
+    x, _t = _batch(200, 1)
+    got = net.predict(x)
+    ref = mo.forward(p, x)["out"][:, :8].view(np.complex128)
+    assert rel_err(got, ref) < 2e-3
+
+
+def test_gradients_match_oracle():
+    _say("mlp gradients")
+    p = mo.init_params(4)
+    net = sg.SommerfeldSurrogate(p)
+    x, t = _batch(256, 2)
+    dx, dt = torch.from_numpy(x).cuda(), torch.from_numpy(t).cuda()
+    loss = net.loss_and_grads(dx, dt, 256)
+    g = net.grads.cpu().numpy().astype(np.float64)
+    lref, gref = mo.loss_and_grads(p, x, t)
+    assert abs(loss - lref) < 1e-4 * lref
+    o = 0
+    for l, (i, f) in enumerate(zip(mo.FIN, mo.FOUT)):
+        gw, gwr = g[o:o + f * i], gref[o:o + f * i]
+        gb, gbr = g[o + f * i:o + f * i + f], gref[o + f * i:o + f * i + f]
+        _say(f"  layer {l}: dW {rel_err(gw, gwr):.2e} db {rel_err(gb, gbr):.2e}")
+        assert rel_err(gw, gwr) < 2e-3
+        assert rel_err(gb, gbr) < 2e-3
+        o += f * i + f
+    # the oracle's own backward against finite differences of its loss
+    idx = [3, 5 * 64 + 2, 16640 + 10, 16640 + 65536 + 5 * 256 + 7]
+    _l, g64 = mo.loss_and_grads(p.astype(np.float64), x, t, rounding=False)
+    fd = mo.fd_check(p, x, t, idx)
+    assert np.allclose(g64[idx], fd, rtol=1e-5, atol=1e-12)
+
+
+def test_adam_step_matches_numpy():
+    _say("adam")
+    p = mo.init_params(5)
+    net = sg.SommerfeldSurrogate(p)
+    x, t = _batch(128, 3)
+    net.loss_and_grads(torch.from_numpy(x).cuda(), torch.from_numpy(t).cuda(), 128)
+    g = net.grads.cpu().numpy().astype(np.float64)
+    lib = net.lib
+    import ctypes
+    from momwire_amd import _lib
+    _lib.check(lib.mw_mlp_adam_step(ctypes.byref(net.net), 1e-3, 0.9, 0.999, 1e-8, 1, net._st()),
+               "adam")
+    m = 0.1 * g
+    v = 0.001 * g * g
+    ref = p - 1e-3 * (m / 0.1) / (np.sqrt(v / 0.001) + 1e-8)
+    got = net.params.cpu().numpy()
+    assert np.abs(got - ref).max() < 1e-6
+    # the BF16 copies follow the master
+    w16 = net.w16.cpu().numpy().view(np.uint16)
+    assert np.array_equal((w16[:10].astype(np.uint32) << 16).view(np.float32),
+                          mo.bf16(got[:10]))
+
+
+def test_short_training_reduces_loss():
+    _say("short training on engine samples")
+    x = sg.sample_inputs(16384, 11)
+    t = sg.engine_targets(x)
+    net = sg.SommerfeldSurrogate(seed=11)
+    dx = torch.from_numpy(x.astype(np.float32)).cuda()
+    l0 = net.loss_and_grads(dx[:4096], t[:4096], 4096)
+    hist = net.train(dx, t, 400, batch=1024, lr=2e-3, log_every=100)
+    _say("  loss", l0, "->", hist[-1][1])
+    assert hist[-1][1] < 0.2 * l0
+
+
+def test_engine_fallback_outside_trained_domain():
+    """A master reaching past the trained extent: the engine fills those nodes,
+    bit for bit what the pure-engine master holds."""
+    _say("fallback")
+    net = sg.SommerfeldSurrogate(seed=1)
+    eps = complex(12.0, -20.0)
+    _ground.clear_cache()
+    exact = _ground.GridMaster(eps, 6.0)
+    learned = _ground.GridMaster(eps, 6.0, filler=net.grid_filler())
+    assert learned.engine_nodes > 0 and learned.learned_nodes > 0
+    R = exact.R1.cpu().numpy()
+    T = exact.theta.cpu().numpy()
+    rho, h = R * np.cos(T), R * np.sin(T)
+    outside = ~sg.SommerfeldSurrogate.in_domain(np.maximum(rho, 0), np.maximum(h, 0), eps)
+    assert outside.sum() == learned.engine_nodes
+    a = exact.vals.cpu().numpy()[outside]
+    b = learned.vals.cpu().numpy()[outside]
+    assert np.array_equal(a, b)
+
+
+@pytest.mark.skipif(not __import__("os").path.exists(sg.WEIGHTS), reason="no trained weights")
+@pytest.mark.parametrize("seed", [4, 5, 6])
+def test_trained_surrogate_gain(seed):
+    """Grounded 4x4 arrays: the learned grid against the exact engine (the
+    judge), and against the method without it -- the C2 image alone."""
+    import time
+
+    _say("trained surrogate, array seed", seed)
+    net = sg.SommerfeldSurrogate.load()
+    kw = W.grounded_array(4, 4, 21, seed=seed)
+    _ground.clear_cache()
+    s = BSplineSolver(**kw)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    y_exact = s.compute_port_solution().y
+    t_exact = time.perf_counter() - t0
+    _ground.clear_cache()
+    sl = BSplineSolver(**kw, somm_surrogate=net)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    y_learned = sl.compute_port_solution().y
+    t_learned = time.perf_counter() - t0
+    m = sl._grid_master
+    # image only: the remainder dropped
+    kw0 = dict(kw)
+    s0 = BSplineSolver(**kw0)
+    s0._ground_grid = lambda k, omega: None
+    y_image = s0.compute_port_solution().y
+    e_l, e_0 = rel_err(y_learned, y_exact), rel_err(y_image, y_exact)
+    _say(f"  Y err learned {e_l:.2e}, image only {e_0:.2e}; cold solve exact {t_exact*1e3:.1f} ms, "
+         f"learned {t_learned*1e3:.1f} ms; engine nodes {m.engine_nodes}/{m.n_nodes}")
+    assert e_l < 0.1 * e_0
