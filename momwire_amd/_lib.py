@@ -150,6 +150,11 @@ def load():
         _load_error = (f"{LIB_PATH} is not built; run `python -m momwire_amd.build` "
                        "(the port-solution path has no CPU fallback)")
         raise AcceleratorUnavailable(_load_error)
+    # PyTorch-ROCm ships its own HIP runtime (soname libamdhip64.so.7, needed
+    # by torch under another name).  Importing torch first makes this
+    # library bind to that same runtime instead of loading a second one from
+    # /opt/rocm, which would leave two runtimes competing for the device.
+    import torch  # noqa: F401
     try:
         lib = ctypes.CDLL(LIB_PATH)
     except OSError as e:

@@ -19,9 +19,12 @@ template <typename C> MW_DEV C csub(C a, C b) { C z; z.x = a.x - b.x; z.y = a.y 
 template <typename C> MW_DEV C cmul(C a, C b) {
   C z; z.x = a.x * b.x - a.y * b.y; z.y = a.x * b.y + a.y * b.x; return z;
 }
-// a - b*c
+// a - b*c, as two fused multiply-adds per component
 template <typename C> MW_DEV C cfms(C a, C b, C c) {
-  C z; z.x = a.x - (b.x * c.x - b.y * c.y); z.y = a.y - (b.x * c.y + b.y * c.x); return z;
+  C z;
+  z.x = fma(-b.x, c.x, fma(b.y, c.y, a.x));
+  z.y = fma(-b.x, c.y, fma(-b.y, c.x, a.y));
+  return z;
 }
 template <typename C> MW_DEV C cdiv(C a, C b) {
   // Smith's algorithm: no overflow for well-scaled inputs.

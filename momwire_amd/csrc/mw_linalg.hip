@@ -457,43 +457,21 @@ MW_DEV void load_block(CT (&S)[RPL][8], const CT* A, int64_t lda, int64_t k0, in
   }
 }
 
-// Inverses of the unit-lower and upper factors held in M (L\U, padded to NB
-// with the identity): Li = L^{-1}, Ui = U^{-1}, by Gauss-Jordan sweeps with
-// one barrier per column.  U^{-1} = (D Ut)^{-1} = Ut^{-1} D^{-1} with Ut unit
-// upper, so both sweeps are unit-triangular and the diagonal is applied last.
-// Each of the 256 threads owns four entries of Li and four of Ui.
-template <typename CT>
-MW_DEV void tri_inverses(const CT (*M)[NB + 1], CT (*Li)[NB + 1], CT (*Ui)[NB + 1], CT* dinv) {
-  const int tid = threadIdx.x;
-  for (int e = tid; e < NB * NB; e += blockDim.x) {
-    const int i = e / NB, j = e % NB;
-    CT v;
-    v.x = (i == j) ? 1 : 0; v.y = 0;
-    Li[i][j] = v;
-    Ui[i][j] = v;
-  }
-  if (tid < NB) {
-    CT d = M[tid][tid];
-    if (d.x != 0 || d.y != 0) d = crecip(d);  // a zero pivot (reported via info) stays zero
-    dinv[tid] = d;
-  }
-  __syncthreads();
-  for (int s = 0; s < NB; ++s) {
-    const int cl = s, cu = NB - 1 - s;
-#pragma unroll
-    for (int q = 0; q < NB * NB / 256; ++q) {
-      const int e = tid + 256 * q;
-      const int i = e / NB, j = e % NB;
-      if (i > cl && j <= cl) Li[i][j] = cfms(Li[i][j], M[i][cl], Li[cl][j]);
-      if (i < cu && j >= cu) Ui[i][j] = cfms(Ui[i][j], cmul(M[i][cu], dinv[i]), Ui[cu][j]);
-    }
-    __syncthreads();
-  }
-  for (int e = tid; e < NB * NB; e += blockDim.x) {
-    const int i = e / NB, j = e % NB;
-    if (j >= i) Ui[i][j] = cmul(Ui[i][j], dinv[j]);
-  }
-  __syncthreads();
+// Broadcast of lane k's value (k wave-uniform): v_readlane into scalar
+// registers, so every lane gets it without an LDS round trip.
+MW_DEV double lane_bcast(double x, int k) {
+  const int lo = __builtin_amdgcn_readlane(__double2loint(x), k);
+  const int hi = __builtin_amdgcn_readlane(__double2hiint(x), k);
+  return __hiloint2double(hi, lo);
+}
+MW_DEV float lane_bcast(float x, int k) {
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x), k));
+}
+template <typename CT> MW_DEV CT lane_bcast(CT v, int k) {
+  CT o;
+  o.x = lane_bcast(v.x, k);
+  o.y = lane_bcast(v.y, k);
+  return o;
 }
 
 template <typename CT>
@@ -505,6 +483,227 @@ struct PanelLds {
   int bad;
   int piv_s;
 };
+
+// 1/d for a complex d != 0: conj(d) / |d|^2 with the real reciprocal from
+// v_rcp and two Newton steps (a short dependency chain; the elimination step
+// waits on it).
+MW_DEV double rcp_nr(double a) {
+  double r = __builtin_amdgcn_rcp(a);
+  double e = fma(-a, r, 1.0);
+  r = fma(r, e, r);
+  e = fma(-a, r, 1.0);
+  return fma(r, e, r);
+}
+MW_DEV float rcp_nr(float a) { return 1.0f / a; }
+template <typename CT> MW_DEV CT crecip_fast(CT d) {
+  const auto m = rcp_nr(d.x * d.x + d.y * d.y);
+  CT o;
+  o.x = d.x * m;
+  o.y = -d.y * m;
+  return o;
+}
+
+// LU without row exchanges of the NB x NB block in P.M (identity-padded),
+// by wavefront 0 alone: lane r holds row r in registers and the pivot row is
+// broadcast with v_readlane, so the 31 elimination steps need no barrier.
+// Rows at or above the pivot get a zero multiplier instead of a predicated
+// update (x - 0*u is x exactly for finite u).  A zero pivot zeroes the step's
+// multipliers; it, or a multiplier beyond lbound, sets P.bad (the panel is
+// then refactored with pivoting from A).  Called by every thread of
+// the workgroup; ends with a barrier.
+template <typename CT>
+MW_DEV void wave_lu(PanelLds<CT>& P, double lbound) {
+  const int tid = threadIdx.x;
+  if (tid < 64) {
+    const int r = tid & 31;
+    CT R[NB];
+#pragma unroll
+    for (int j = 0; j < NB; ++j) R[j] = P.M[r][j];
+    bool bad = false;
+#pragma unroll
+    for (int k = 0; k < NB - 1; ++k) {
+      const CT d = lane_bcast(R[k], k);
+      // a zero pivot leaves the step without multipliers (and flags the panel)
+      const bool zero = d.x == 0 && d.y == 0;
+      bad |= zero;
+      CT inv = crecip_fast(d);
+      inv.x = zero ? 0 : inv.x;
+      inv.y = zero ? 0 : inv.y;
+      const bool below = r > k;
+      CT l = cmul(R[k], inv);
+      if (below && !(cabs1(l) <= lbound)) bad = true;
+      l.x = below ? l.x : 0;
+      l.y = below ? l.y : 0;
+      R[k].x = below ? l.x : R[k].x;
+      R[k].y = below ? l.y : R[k].y;
+#pragma unroll
+      for (int j = k + 1; j < NB; ++j) R[j] = cfms(R[j], l, lane_bcast(R[j], k));
+    }
+    const CT last = lane_bcast(R[NB - 1], NB - 1);
+    if (last.x == 0 && last.y == 0) bad = true;
+    if (tid < 32) {
+#pragma unroll
+      for (int j = 0; j < NB; ++j) P.M[r][j] = R[j];
+    }
+    if (__any(bad) && tid == 0) P.bad = 1;
+  }
+  __syncthreads();
+}
+
+// Inverses of the factors held in P.M (L\U, padded to NB with the
+// identity): P.Li = L^{-1}, P.Ui = U^{-1}.  One column per lane of wave 0 by
+// forward substitution: lanes 0-31 solve L x = e_c; lanes 32-63 solve the
+// index-reversed unit-upper system of U~ = D^{-1} U, so both halves run the
+// same instructions; U^{-1} = U~^{-1} D^{-1}.  Each row's
+// sum runs as four independent partial sums.  A zero pivot (reported
+// elsewhere) leaves a zero diagonal inverse.  Called by every thread; ends
+// with a barrier.
+template <typename CT>
+MW_DEV void wave_inverses(PanelLds<CT>& P) {
+  const int tid = threadIdx.x;
+  if (tid < NB) {
+    CT d = P.M[tid][tid];
+    if (d.x != 0 || d.y != 0) d = crecip(d);
+    P.dinv[tid] = d;
+  }
+  __syncthreads();
+  // the two unit-lower systems side by side in Li, Ui (contiguous):
+  // Li[i][j] = L[i][j], Ui[i][j] = U~[31-i][31-j] for j < i
+  for (int e = tid; e < NB * NB; e += blockDim.x) {
+    const int i = e / NB, j = e % NB;
+    CT v, w;
+    v.x = 0; v.y = 0; w = v;
+    if (j < i) {
+      v = P.M[i][j];
+      w = cmul(P.M[NB - 1 - i][NB - 1 - j], P.dinv[NB - 1 - i]);
+    }
+    P.Li[i][j] = v;
+    P.Ui[i][j] = w;
+  }
+  __syncthreads();
+  if (tid < 64) {
+    const int upper = tid >> 5;
+    const int c = tid & 31;
+    // one address per lane: no select between two arrays
+    const CT* A = &P.Li[0][0] + upper * (NB * (NB + 1));
+    CT x[NB];
+#pragma unroll
+    for (int r = 0; r < NB; ++r) {
+      CT s[4];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) { s[q].x = 0; s[q].y = 0; }
+      s[0].x = (r == c) ? 1 : 0;
+#pragma unroll
+      for (int j = 0; j < r; ++j) s[j & 3] = cfms(s[j & 3], A[r * (NB + 1) + j], x[j]);
+      x[r].x = (s[0].x + s[1].x) + (s[2].x + s[3].x);
+      x[r].y = (s[0].y + s[1].y) + (s[2].y + s[3].y);
+    }
+    if (!upper) {
+#pragma unroll
+      for (int r = 0; r < NB; ++r) P.Li[r][c] = x[r];
+    } else {
+      const int co = NB - 1 - c;
+      const CT dc = P.dinv[co];
+#pragma unroll
+      for (int r = 0; r < NB; ++r) P.Ui[NB - 1 - r][co] = cmul(x[r], dc);
+    }
+  }
+  __syncthreads();
+}
+
+constexpr int AW = 64;  // rows (or columns) per workgroup in the block products
+
+// Block products with a 32 x 32 complex factor T on MFMA, one workgroup:
+//   ROWS (RA = 64, CB = 32): Out (64 x 32) = In (64 x 32) T
+//   COLS (RA = 32, CB = 64): Out (32 x 64) = T In (32 x 64)
+// as the real product A_r (RA x 64) B_exp (64 x 2 CB), A_r the left factor's
+// (re, im) pairs and B_exp the right factor with each b expanded to
+// [[br, bi], [-bi, br]] when fragments are formed.  The 16 output tiles of
+// 16 x 16 are four per wave.  In is read whole into LDS before anything is
+// written, so Out may alias In.  Rows/columns of In beyond (nrow, ncol) read
+// as zero; Out is written only inside them (or not at all when check: then
+// the result is returned as "some |entry| > lbound").
+template <typename T> struct BlkLds {
+  T As[64][65];                       // left factor as reals
+  typename cplx_t<T>::type Bs[32][65];  // right factor as complex
+};
+
+template <typename T, int RA, int CB>
+MW_DEV bool block_product(BlkLds<T>& L, const typename cplx_t<T>::type* Tg, int ldt,
+                          typename cplx_t<T>::type* X, int64_t ldx, int nrow, int ncol,
+                          bool check, double lbound) {
+  typedef typename cplx_t<T>::type CT;
+  typedef typename Mfma<T>::acc_t acc_t;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  // stage: left factor (reals) and right factor (complex)
+  if (RA == 64) {
+    for (int e = tid; e < 64 * 32; e += 256) {
+      const int r = e >> 5, c = e & 31;
+      CT v;
+      v.x = 0; v.y = 0;
+      if (r < nrow && c < ncol) v = X[(int64_t)r * ldx + c];
+      L.As[r][2 * c] = v.x;
+      L.As[r][2 * c + 1] = v.y;
+    }
+    for (int e = tid; e < 32 * 32; e += 256) L.Bs[e >> 5][e & 31] = Tg[(e >> 5) * ldt + (e & 31)];
+  } else {
+    for (int e = tid; e < 32 * 32; e += 256) {
+      const CT v = Tg[(e >> 5) * ldt + (e & 31)];
+      L.As[e >> 5][2 * (e & 31)] = v.x;
+      L.As[e >> 5][2 * (e & 31) + 1] = v.y;
+    }
+    for (int e = tid; e < 32 * 64; e += 256) {
+      const int r = e >> 6, c = e & 63;
+      CT v;
+      v.x = 0; v.y = 0;
+      if (r < nrow && c < ncol) v = X[(int64_t)r * ldx + c];
+      L.Bs[r][c] = v;
+    }
+  }
+  __syncthreads();
+  constexpr int TC = 2 * CB / 16;  // tile columns
+  acc_t acc[4];
+#pragma unroll
+  for (int j = 0; j < 4; ++j)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) acc[j][r] = 0;
+#pragma unroll
+  for (int ks = 0; ks < 64; ks += 4) {
+    const int kr = ks + (lane >> 4);
+    const int kc = kr >> 1, kpar = kr & 1;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int t = 4 * wave + j;
+      const int tr = t / TC, tc = t % TC;
+      const T a = L.As[16 * tr + (lane & 15)][kr];
+      const int cr = 16 * tc + (lane & 15);
+      const CT bv = L.Bs[kc][cr >> 1];
+      const T bb = (kpar == 0) ? ((cr & 1) ? bv.y : bv.x) : ((cr & 1) ? bv.x : -bv.y);
+      acc[j] = Mfma<T>::mma(a, bb, acc[j]);
+    }
+  }
+  bool over = false;
+  T* Xr = reinterpret_cast<T*>(X);
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    const int t = 4 * wave + j;
+    const int tr = t / TC, tc = t % TC;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int row = 16 * tr + Mfma<T>::row(lane, r);
+      const int cr = 16 * tc + (lane & 15);
+      const bool in = row < nrow && (cr >> 1) < ncol;
+      if (check) {
+        // |re| + |im| of the complex entry: its two parts sit in lanes l, l ^ 1
+        const double mag = fabs((double)acc[j][r]) + fabs((double)__shfl_xor(acc[j][r], 1));
+        if (in && !(mag <= lbound)) over = true;
+      } else if (in) {
+        Xr[(int64_t)row * 2 * ldx + cr] = acc[j][r];
+      }
+    }
+  }
+  return over;
+}
 
 // Write L11\U11 (from M) and the two inverses for panel use and for the solve.
 template <typename CT>
@@ -542,28 +741,8 @@ MW_DEV void panel_nopiv_body(int b, int64_t n, const typename cplx_t<T>::type* A
   }
   if (tid == 0) P.bad = (lbound <= 0.0) ? 1 : 0;
   __syncthreads();
-  for (int c = 0; c < NB - 1; ++c) {
-    const CT d = P.M[c][c];
-    const bool zero = (d.x == 0 && d.y == 0);
-    if (zero) {
-      if (tid == 0) P.bad = 1;
-    } else if (tid > c && tid < NB) {
-      const CT l = cdiv(P.M[tid][c], d);
-      if (!(cabs1(l) <= lbound)) P.bad = 1;
-      P.M[tid][c] = l;
-    }
-    __syncthreads();
-    if (!zero) {
-      const int w = NB - 1 - c;
-      for (int e = tid; e < w * w; e += blockDim.x) {
-        const int i = c + 1 + e / w, j = c + 1 + e % w;
-        P.M[i][j] = cfms(P.M[i][j], P.M[i][c], P.M[c][j]);
-      }
-    }
-    __syncthreads();
-  }
-  if (tid == 0 && P.M[NB - 1][NB - 1].x == 0 && P.M[NB - 1][NB - 1].y == 0) P.bad = 1;
-  tri_inverses(P.M, P.Li, P.Ui, P.dinv);
+  wave_lu(P, lbound);
+  wave_inverses(P);
   if (b == 0) {
     store_factors(P, bw, lu11, inv);
     if (tid < bw) ipiv[k0 + tid] = (int32_t)(k0 + tid);
@@ -571,36 +750,14 @@ MW_DEV void panel_nopiv_body(int b, int64_t n, const typename cplx_t<T>::type* A
     return;
   }
   if (P.bad) return;
-  // rows of L21 handled by this block: check only
-  CT (*B)[NB + 1] = reinterpret_cast<CT (*)[NB + 1]>(&P.M[0][0]);  // 64 rows (M and Li)
+  // rows of L21 handled by this block: L21 = A21 U11^{-1} on MFMA, check only
+  __shared__ BlkLds<T> Lb;
   const int64_t r0 = k0 + bw + (int64_t)(b - 1) * 64;
-  __syncthreads();
-  for (int e = tid; e < 64 * NB; e += blockDim.x) {
-    const int rr = e / NB, l = e % NB;
-    CT v;
-    v.x = 0; v.y = 0;
-    if (r0 + rr < n && l < bw) v = A[(r0 + rr) * lda + k0 + l];
-    B[rr][l] = v;
-  }
-  __syncthreads();
-  const int rr = tid / 4, j0 = (tid % 4) * 8;
-  CT acc[8];
-#pragma unroll
-  for (int q = 0; q < 8; ++q) { acc[q].x = 0; acc[q].y = 0; }
-#pragma unroll 4
-  for (int l = 0; l < NB; ++l) {
-    const CT a = B[rr][l];
-#pragma unroll
-    for (int q = 0; q < 8; ++q) {
-      const CT t = P.Ui[l][j0 + q];
-      acc[q].x += a.x * t.x - a.y * t.y;
-      acc[q].y += a.x * t.y + a.y * t.x;
-    }
-  }
-  bool over = false;
-#pragma unroll
-  for (int q = 0; q < 8; ++q) over |= !(cabs1(acc[q]) <= lbound);
-  if (over && r0 + rr < n) atomicOr(flag, 1);
+  const int rows = (int)((n - r0) < 64 ? (n - r0) : 64);
+  const bool over = block_product<T, 64, 32>(Lb, &P.Ui[0][0], NB + 1,
+                                             const_cast<CT*>(A) + r0 * lda + k0, lda, rows, bw,
+                                             true, lbound);
+  if (over) atomicOr(flag, 1);
 }
 
 template <typename T>
@@ -736,7 +893,7 @@ MW_DEV void panel_pivot_body(int node0, typename cplx_t<T>::type* A, int64_t lda
     if (steps < bw && *info == 0) *info = (int32_t)(k0 + steps + 1);
   }
   __syncthreads();
-  tri_inverses(P.M, P.Li, P.Ui, P.dinv);
+  wave_inverses(P);
   store_factors(P, bw, lu11, inv);
   // reset the arrival counters for the next fallback
   int total = 0;
@@ -771,49 +928,6 @@ __global__ __launch_bounds__(256) void k_panel_pivot(typename cplx_t<T>::type* A
                       info);
 }
 
-constexpr int AW = 64;  // rows (or columns) per workgroup in the block products
-
-template <typename CT>
-struct ApplyLds {
-  CT Tm[NB][NB + 1];
-  CT B[AW][NB + 1];     // rows mode: AW rows x NB; cols mode: used as [NB][AW] below
-};
-
-// out[i][c] = sum_l Tm[i][l] X[k0+l][c] for the NB x AW block at columns c0.
-template <typename CT>
-MW_DEV void apply_left(ApplyLds<CT>& L, CT* X, int64_t ldx, int64_t k0, int bw, int64_t c0,
-                       int64_t ncols) {
-  const int tid = threadIdx.x;
-  CT (*Bc)[AW + 1] = reinterpret_cast<CT (*)[AW + 1]>(&L.B[0][0]);
-  for (int e = tid; e < NB * AW; e += blockDim.x) {
-    const int i = e / AW, cc = e % AW;
-    CT v;
-    v.x = 0; v.y = 0;
-    if (i < bw && c0 + cc < ncols) v = X[(k0 + i) * ldx + c0 + cc];
-    Bc[i][cc] = v;
-  }
-  __syncthreads();
-  const int cc = tid % AW, i0 = (tid / AW) * 8;
-  CT acc[8];
-#pragma unroll
-  for (int q = 0; q < 8; ++q) { acc[q].x = 0; acc[q].y = 0; }
-#pragma unroll 4
-  for (int l = 0; l < NB; ++l) {
-    const CT b = Bc[l][cc];
-#pragma unroll
-    for (int q = 0; q < 8; ++q) {
-      const CT t = L.Tm[i0 + q][l];
-      acc[q].x += t.x * b.x - t.y * b.y;
-      acc[q].y += t.x * b.y + t.y * b.x;
-    }
-  }
-  if (c0 + cc < ncols) {
-#pragma unroll
-    for (int q = 0; q < 8; ++q)
-      if (i0 + q < bw) X[(k0 + i0 + q) * ldx + c0 + cc] = acc[q];
-  }
-}
-
 // After the interchanges: A11 <- L11\U11, L21 <- A21 U11^{-1}, U12 <- L11^{-1} A12.
 // Blocks [0, nrb): 64-row blocks of L21; [nrb, nrb+ncb): 64-column blocks of
 // U12; last: A11.
@@ -823,43 +937,16 @@ MW_DEV void panel_apply_body(int b, int64_t n, int64_t nc, typename cplx_t<T>::t
                              const typename cplx_t<T>::type* inv,
                              const typename cplx_t<T>::type* lu11, int nrb, int ncb) {
   typedef typename cplx_t<T>::type CT;
-  __shared__ ApplyLds<CT> L;
+  __shared__ BlkLds<T> L;
   const int tid = threadIdx.x;
   if (b < nrb) {
-    const CT* Ui = inv + NB * NB;
-    for (int e = tid; e < NB * NB; e += blockDim.x) L.Tm[e / NB][e % NB] = Ui[e];
     const int64_t r0 = k0 + bw + (int64_t)b * AW;
-    for (int e = tid; e < AW * NB; e += blockDim.x) {
-      const int rr = e / NB, l = e % NB;
-      CT v;
-      v.x = 0; v.y = 0;
-      if (r0 + rr < n && l < bw) v = A[(r0 + rr) * lda + k0 + l];
-      L.B[rr][l] = v;
-    }
-    __syncthreads();
-    const int rr = tid / 4, j0 = (tid % 4) * 8;
-    CT acc[8];
-#pragma unroll
-    for (int q = 0; q < 8; ++q) { acc[q].x = 0; acc[q].y = 0; }
-#pragma unroll 4
-    for (int l = 0; l < NB; ++l) {
-      const CT a = L.B[rr][l];
-#pragma unroll
-      for (int q = 0; q < 8; ++q) {
-        const CT t = L.Tm[l][j0 + q];
-        acc[q].x += a.x * t.x - a.y * t.y;
-        acc[q].y += a.x * t.y + a.y * t.x;
-      }
-    }
-    if (r0 + rr < n) {
-#pragma unroll
-      for (int q = 0; q < 8; ++q)
-        if (j0 + q < bw) A[(r0 + rr) * lda + k0 + j0 + q] = acc[q];
-    }
+    const int rows = (int)((n - r0) < AW ? (n - r0) : AW);
+    block_product<T, 64, 32>(L, inv + NB * NB, NB, A + r0 * lda + k0, lda, rows, bw, false, 0.0);
   } else if (b < nrb + ncb) {
-    for (int e = tid; e < NB * NB; e += blockDim.x) L.Tm[e / NB][e % NB] = inv[e];
-    __syncthreads();
-    apply_left(L, A, lda, k0, bw, k0 + bw + (int64_t)(b - nrb) * AW, nc);
+    const int64_t c0 = k0 + bw + (int64_t)(b - nrb) * AW;
+    const int cols = (int)((nc - c0) < AW ? (nc - c0) : AW);
+    block_product<T, 32, 64>(L, inv, NB, A + k0 * lda + c0, lda, bw, cols, false, 0.0);
   } else {
     for (int e = tid; e < bw * bw; e += blockDim.x)
       A[(k0 + e / bw) * lda + k0 + e % bw] = lu11[(e / bw) * NB + e % bw];
@@ -881,14 +968,11 @@ template <typename T>
 MW_DEV void solve_apply_body(int bx, int64_t nrhs, const typename cplx_t<T>::type* inv,
                              int64_t k0, int bw, typename cplx_t<T>::type* X, int64_t ldx,
                              int lower) {
-  typedef typename cplx_t<T>::type CT;
-  __shared__ ApplyLds<CT> L;
-  const int tid = threadIdx.x;
+  __shared__ BlkLds<T> L;
   const int64_t c0 = (int64_t)bx * AW;
-  const CT* Tsrc = lower ? inv : inv + NB * NB;
-  for (int e = tid; e < NB * NB; e += blockDim.x) L.Tm[e / NB][e % NB] = Tsrc[e];
-  __syncthreads();
-  apply_left(L, X, ldx, k0, bw, c0, nrhs);
+  const int cols = (int)((nrhs - c0) < AW ? (nrhs - c0) : AW);
+  block_product<T, 32, 64>(L, lower ? inv : inv + NB * NB, NB, X + k0 * ldx + c0, ldx, bw, cols,
+                           false, 0.0);
 }
 
 template <typename T>
