@@ -351,6 +351,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--eager", action="store_true",
                     help="launch kernels one by one instead of replaying the captured graph")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="only run a plain device-to-device copy of MIB MiB three times (the "
+                         "calibration kernel for HBM byte counters) and exit")
     ap.add_argument("--refine-steps", type=int, default=None,
                     help="FP64-residual corrections of the mixed form (array32x32); default: "
                          "one, plus the adjoint-corrected readout")
@@ -359,6 +362,18 @@ def main():
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
               "(the multi-GPU layout is described, not built)", file=sys.stderr)
         sys.exit(2)
+
+    if args.copy_probe:
+        import torch
+
+        n = args.copy_probe << 20
+        a = torch.ones(n, dtype=torch.uint8, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_bytes": n, "launches": 3}))
+        return
 
     wl = WORKLOADS[args.workload]
     if args.workload == "sweep128":
@@ -440,6 +455,17 @@ def main():
                 "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s", "frac": fill_tf / PEAK_FP64_TFLOPS,
                 "traffic": None,
                 "algorithmic": f"16*53*N_seg^2 + 726*n_b^2 = {fill_flops(ns, nb):.3e} flop"}
+
+    # HBM bytes per solve of the dominant kernels, measured by rocprofv3 PMC
+    # runs (FETCH_SIZE, WRITE_SIZE) calibrated on a plain copy
+    tpath = os.path.join(ROOT, "profiles", "hbm_traffic.json")
+    if os.path.exists(tpath) and dominant == "solve":
+        tr = json.load(open(tpath)).get(args.workload)
+        if tr:
+            roof["traffic"] = tr["traffic_bytes_per_solve"]
+            roof["traffic_source"] = (f"{tr['source']}: FETCH_SIZE x2.0 + WRITE_SIZE x1.0 "
+                                      "(copy-kernel calibration), per solve")
+            roof["algorithmic_bytes"] = 2 * 16 * nb * (nb + ncols)
 
     out = {
         "metric": METRIC,
