@@ -983,6 +983,121 @@ __global__ __launch_bounds__(256) void k_solve_apply(int64_t nrhs, const typenam
   solve_apply_body<T>(blockIdx.x, nrhs, inv, k0, bw, X, ldx, lower);
 }
 
+// One step of the back substitution, fused: every workgroup forms
+// X_p' = U_pp^{-1} X_p for its 32 right-hand-side columns in LDS (MFMA, from
+// the panel's stored inverse) and then updates its 64-row tile of the rows
+// above, X[rows][cols] -= U[rows][p] X_p'.  X_p itself is left as it is:
+// nothing later changes it, and one launch at the end (k_solve_apply_all)
+// replaces every panel by U_pp^{-1} X_p.
+template <typename T>
+__global__ __launch_bounds__(256) void k_bwd_fused(int64_t k0, int bw, int64_t nrhs,
+                                                   const typename cplx_t<T>::type* __restrict__ A,
+                                                   int64_t lda,
+                                                   const typename cplx_t<T>::type* __restrict__ uinv,
+                                                   typename cplx_t<T>::type* X, int64_t ldx) {
+  typedef typename cplx_t<T>::type CT;
+  typedef typename Mfma<T>::acc_t acc_t;
+  __shared__ T As[64][65];     // U_pp^{-1} (phase 1), then the A tile (phase 2), as reals
+  __shared__ CT Bs[32][33];    // X_p (phase 1), then X_p' (phase 2)
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int64_t c0 = (int64_t)blockIdx.x * 32;
+  const int64_t r0 = (int64_t)blockIdx.y * 64;
+  const int cols = (int)((nrhs - c0) < 32 ? (nrhs - c0) : 32);
+  const int rows = (int)((k0 - r0) < 64 ? (k0 - r0) : 64);
+  for (int e = tid; e < 32 * 32; e += 256) {
+    const int r = e >> 5, c = e & 31;
+    const CT u = uinv[e];
+    As[r][2 * c] = u.x;
+    As[r][2 * c + 1] = u.y;
+    CT v;
+    v.x = 0; v.y = 0;
+    if (r < bw && c < cols) v = X[(k0 + r) * ldx + c0 + c];
+    Bs[r][c] = v;
+  }
+  __syncthreads();
+  // phase 1: X_p' (32 x 64 reals, 8 tiles: two per wave)
+  acc_t acc[4];
+#pragma unroll
+  for (int j = 0; j < 2; ++j)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) acc[j][r] = 0;
+#pragma unroll
+  for (int ks = 0; ks < 64; ks += 4) {
+    const int kr = ks + (lane >> 4), kc = kr >> 1, kpar = kr & 1;
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+      const int t = 2 * wave + j, tr = t >> 2, tc = t & 3;
+      const T a = As[16 * tr + (lane & 15)][kr];
+      const int cr = 16 * tc + (lane & 15);
+      const CT bv = Bs[kc][cr >> 1];
+      const T bb = (kpar == 0) ? ((cr & 1) ? bv.y : bv.x) : ((cr & 1) ? bv.x : -bv.y);
+      acc[j] = Mfma<T>::mma(a, bb, acc[j]);
+    }
+  }
+  __syncthreads();  // every wave is done reading U^{-1} and X_p
+#pragma unroll
+  for (int j = 0; j < 2; ++j) {
+    const int t = 2 * wave + j, tr = t >> 2, tc = t & 3;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int row = 16 * tr + Mfma<T>::row(lane, r);
+      const int cr = 16 * tc + (lane & 15);
+      T* dst = (cr & 1) ? &Bs[row][cr >> 1].y : &Bs[row][cr >> 1].x;
+      *dst = (row < bw) ? acc[j][r] : (T)0;
+    }
+  }
+  for (int e = tid; e < 64 * 32; e += 256) {
+    const int r = e >> 5, c = e & 31;
+    CT v;
+    v.x = 0; v.y = 0;
+    if (r < rows && c < bw) v = A[(r0 + r) * lda + k0 + c];
+    As[r][2 * c] = v.x;
+    As[r][2 * c + 1] = v.y;
+  }
+  __syncthreads();
+  // phase 2: tile (64 x 64 reals, 16 tiles: four per wave) -= A_tile X_p'
+#pragma unroll
+  for (int j = 0; j < 4; ++j)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) acc[j][r] = 0;
+#pragma unroll
+  for (int ks = 0; ks < 64; ks += 4) {
+    const int kr = ks + (lane >> 4), kc = kr >> 1, kpar = kr & 1;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const T a = As[16 * wave + (lane & 15)][kr];
+      const int cr = 16 * j + (lane & 15);
+      const CT bv = Bs[kc][cr >> 1];
+      const T bb = (kpar == 0) ? ((cr & 1) ? bv.y : bv.x) : ((cr & 1) ? bv.x : -bv.y);
+      acc[j] = Mfma<T>::mma(a, bb, acc[j]);
+    }
+  }
+  T* Xr = reinterpret_cast<T*>(X);
+#pragma unroll
+  for (int j = 0; j < 4; ++j)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int row = 16 * wave + Mfma<T>::row(lane, r);
+      const int cr = 16 * j + (lane & 15);
+      if (row < rows && (cr >> 1) < cols) {
+        T* p = Xr + (r0 + row) * 2 * ldx + 2 * c0 + cr;
+        *p -= acc[j][r];
+      }
+    }
+}
+
+// X_p <- U_pp^{-1} X_p for every panel p = blockIdx.y, AW columns per block.
+template <typename T>
+__global__ __launch_bounds__(256) void k_solve_apply_all(int64_t n, int64_t nrhs, const void* work,
+                                                         typename cplx_t<T>::type* X, int64_t ldx) {
+  const int64_t k0 = (int64_t)blockIdx.y * NB;
+  const int bw = (int)((n - k0) < NB ? (n - k0) : NB);
+  // the panel's inverses (same layout as panel_inv on the host)
+  const char* inv = reinterpret_cast<const char*>(work) + (int64_t)NB * NB * 16 * (1 + 2 * (k0 / NB));
+  solve_apply_body<T>(blockIdx.x, nrhs, reinterpret_cast<const typename cplx_t<T>::type*>(inv), k0,
+                      bw, X, ldx, 0);
+}
+
 // Row interchanges ipiv[r0 .. r1) applied in order to columns [0, ncols) of X.
 template <typename T>
 __global__ void k_laswp_rows(typename cplx_t<T>::type* X, int64_t ldx, int64_t ncols,
@@ -1370,6 +1485,7 @@ int mw_getrf_t(int64_t n, int64_t nc, void* Av, int64_t lda, int32_t* ipiv, void
   hipLaunchKernelGGL(k_zero_i32, dim3(1), dim3(64), 0, st, info, (int64_t)1);
   const double lbound = g_lbound;
   const int64_t nbo = outer_block(n);
+  const bool one_level = nbo == NB;
   for (int64_t K0 = 0; K0 < n; K0 += nbo) {
     const int64_t kend = (n - K0) < nbo ? n : K0 + nbo;
     for (int64_t k0 = K0; k0 < kend; k0 += NB) {
@@ -1377,8 +1493,10 @@ int mw_getrf_t(int64_t n, int64_t nc, void* Av, int64_t lda, int32_t* ipiv, void
       const int bw = (int)((kend - k0) < NB ? (kend - k0) : NB);
       const int64_t rest = n - k0 - bw;         // rows below the panel
       const int64_t restc = kend - k0 - bw;     // block columns right of the panel
+      // one level: the apply also forms U12 for every column to the right
+      const int64_t capply = one_level ? nc : kend;
       const int nrb = (int)((rest + AW - 1) / AW);
-      const int ncb = (int)((restc + AW - 1) / AW);
+      const int ncb = (int)((capply - k0 - bw + AW - 1) / AW);
       CT* inv = reinterpret_cast<CT*>(panel_inv(work, k0));
       int32_t* flag = flags + k0 / NB;
       hipLaunchKernelGGL(k_panel_nopiv<T>, dim3(nrb + 1), dim3(256), 0, st, n, (const CT*)A, lda,
@@ -1386,7 +1504,7 @@ int mw_getrf_t(int64_t n, int64_t nc, void* Av, int64_t lda, int32_t* ipiv, void
       const int nleaf = (int)((n - k0 + TROWS - 1) / TROWS);
       hipLaunchKernelGGL(k_panel_pivot<T>, dim3(nleaf), dim3(256), 0, st, A, lda, n, K0, nc, k0,
                          bw, nleaf, tw, (const int32_t*)flag, lu11, inv, ipiv, info);
-      hipLaunchKernelGGL(k_panel_apply<T>, dim3(nrb + ncb + 1), dim3(256), 0, st, n, kend, A,
+      hipLaunchKernelGGL(k_panel_apply<T>, dim3(nrb + ncb + 1), dim3(256), 0, st, n, capply, A,
                          lda, k0, bw, (const CT*)inv, (const CT*)lu11, nrb, ncb);
       if (rest > 0 && restc > 0) {
         *herr = launch_gemm_sub<T>(rest, restc, bw, A + (k0 + bw) * lda + k0, lda,
@@ -1399,7 +1517,7 @@ int mw_getrf_t(int64_t n, int64_t nc, void* Av, int64_t lda, int32_t* ipiv, void
     if (ncr > 0) {
       CT* X = A + kend;  // U rows of the block, columns [kend, nc)
       const unsigned gx = (unsigned)((ncr + AW - 1) / AW);
-      for (int64_t k0 = K0; k0 < kend; k0 += NB) {
+      for (int64_t k0 = K0; k0 < kend && !one_level; k0 += NB) {
         const int bw = (int)((kend - k0) < NB ? (kend - k0) : NB);
         const CT* inv = reinterpret_cast<const CT*>(panel_inv(work, k0));
         hipLaunchKernelGGL(k_solve_apply<T>, dim3(gx), dim3(256), 0, st, ncr, inv, k0, bw, X,
@@ -1433,6 +1551,7 @@ static int solve_fwd(int64_t n, int64_t nrhs, const typename cplx_t<T>::type* A,
   typedef typename cplx_t<T>::type CT;
   const unsigned gx = (unsigned)((nrhs + AW - 1) / AW);
   const int64_t nbo = outer_block(n);
+  const bool one_level = nbo == NB;
   for (int64_t K0 = 0; K0 < n; K0 += nbo) {
     const int64_t kend = (n - K0) < nbo ? n : K0 + nbo;
     // a block's multiplier columns are stored in the block's final row order
@@ -1459,19 +1578,19 @@ static int solve_bwd(int64_t n, int64_t nrhs, const typename cplx_t<T>::type* A,
                      const void* work, typename cplx_t<T>::type* X, int64_t ldx, hipStream_t st,
                      hipError_t* herr) {
   typedef typename cplx_t<T>::type CT;
-  const unsigned gx = (unsigned)((nrhs + AW - 1) / AW);
-  const int64_t nlast = ((n + NB - 1) / NB - 1) * NB;
-  for (int64_t k0 = nlast; k0 >= 0; k0 -= NB) {
+  if (nrhs <= 0 || n <= 0) return MW_OK;
+  const int64_t npan = (n + NB - 1) / NB;
+  const int64_t nlast = (npan - 1) * NB;
+  for (int64_t k0 = nlast; k0 > 0; k0 -= NB) {
     const int bw = (int)((n - k0) < NB ? (n - k0) : NB);
-    const CT* inv = reinterpret_cast<const CT*>(panel_inv(const_cast<void*>(work), k0));
-    hipLaunchKernelGGL(k_solve_apply<T>, dim3(gx), dim3(256), 0, st, nrhs, inv, k0, bw, X, ldx,
-                       0);
-    if (k0 > 0) {
-      *herr = launch_gemm_sub<T>(k0, nrhs, bw, A + k0, lda, X + k0 * ldx, ldx, X, ldx, st);
-      if (*herr != hipSuccess) return MW_EHIP;
-    }
+    const CT* ui = reinterpret_cast<const CT*>(panel_inv(const_cast<void*>(work), k0)) + NB * NB;
+    hipLaunchKernelGGL(k_bwd_fused<T>, dim3((unsigned)((nrhs + 31) / 32), (unsigned)((k0 + 63) / 64)),
+                       dim3(256), 0, st, k0, bw, nrhs, A, lda, ui, X, ldx);
   }
-  return MW_OK;
+  hipLaunchKernelGGL(k_solve_apply_all<T>, dim3((unsigned)((nrhs + AW - 1) / AW), (unsigned)npan),
+                     dim3(256), 0, st, n, nrhs, work, X, ldx);
+  *herr = hipGetLastError();
+  return *herr == hipSuccess ? MW_OK : MW_EHIP;
 }
 
 template <typename T>
