@@ -327,6 +327,57 @@ int mw_fill_ground(const mw_mesh* mesh, double k, double omega, double eps, doub
                    void* Z, int64_t ldz, const int32_t* tile_pairs, int32_t n_pairs,
                    int32_t symmetric, void* stream);
 
+/*
+ * The same surfaces with a complex permittivity per node (eps: [n][2]):
+ * the sample generator of the learned surrogate.  No counterpart in the
+ * reference (its engine takes one ground per call, _sommerfeld.py:468-502).
+ */
+int mw_somm_surfaces_varied(const double* eps, double k2, double omega, double mu,
+                            const double* R1, const double* theta, int64_t n, double rtol,
+                            void* out, const volatile int32_t* cancel, void* stream);
+
+/* ------------------------------------------------------------------------
+ * Learned Sommerfeld surrogate (no counterpart in the reference)
+ *
+ * A SiLU MLP, 4 hidden layers of 256, BF16 weights, FP32 accumulation on
+ * MFMA: inputs (rho/lambda, (z+z')/lambda, eps_r, sigma/(omega eps0)) of one
+ * node, outputs the four remainder surfaces (8 reals) divided by
+ * -j omega mu/(4 pi).  All buffers are device memory owned by the caller:
+ * params/grads/adam_m/adam_v hold mw_mlp_param_count() floats, w16/wt16 hold
+ * mw_mlp_w16_count() bf16 values (the BF16 weights and their transposes).
+ * ------------------------------------------------------------------------ */
+typedef struct mw_mlp {
+  float* params;
+  float* grads;
+  float* adam_m;
+  float* adam_v;
+  uint16_t* w16;
+  uint16_t* wt16;
+} mw_mlp;
+
+int64_t mw_mlp_param_count(void);
+int64_t mw_mlp_w16_count(void);
+/* workspace bytes for a batch of n nodes (inference or training) */
+int64_t mw_mlp_workspace_bytes(int64_t n);
+/* refresh the BF16 copies from the FP32 master parameters */
+int mw_mlp_sync_weights(const mw_mlp* net, void* stream);
+/* out [n][8] float = network(x [n][4] float) */
+int mw_mlp_forward(const mw_mlp* net, const float* x, int64_t n, float* out, void* ws,
+                   void* stream);
+/* forward, mean-square loss against target [n][8] (written to *loss, device
+ * float), and grads = dLoss/dparams (overwritten) */
+int mw_mlp_backward(const mw_mlp* net, const float* x, const float* target, int64_t n,
+                    float* loss, void* ws, void* stream);
+/* one Adam step (step counts from 1) on params from grads, then the BF16 copies */
+int mw_mlp_adam_step(const mw_mlp* net, float lr, float beta1, float beta2, float eps,
+                     int32_t step, void* stream);
+/* surfaces of nodes idx[0..n) of a normalised master (R1, theta device arrays,
+ * lambda = 1) for one ground, written into vals[idx[i]][4] (complex);
+ * value_re/im = -j omega mu/(4 pi) of the master */
+int mw_mlp_grid_values(const mw_mlp* net, double eps_re, double eps_im, const double* R1,
+                       const double* theta, const int32_t* idx, int64_t n, void* vals,
+                       double value_re, double value_im, void* ws, void* stream);
+
 #ifdef __cplusplus
 }
 #endif
