@@ -31,7 +31,7 @@ namespace {
 constexpr int NL = 5;
 constexpr int FIN[NL] = {64, 256, 256, 256, 256};
 constexpr int FOUT[NL] = {256, 256, 256, 256, 64};
-constexpr int NF_USED = 8;   // features in use
+constexpr int NF_USED = 12;  // features in use
 constexpr int NO_USED = 8;   // outputs in use
 
 __host__ __device__ constexpr int64_t param_off(int l) {
@@ -172,7 +172,10 @@ __global__ __launch_bounds__(256) void k_bias_grad(const uint16_t* __restrict__ 
 }
 
 // Features of the raw inputs x = (rho, h, eps_r, s) [n][4] (lambda = 1):
-// scaled inputs, R1, theta, exp(-2 R1), sin(theta); 56 zero pads.
+// scaled inputs, R1, theta, exp(-2 R1), sin(theta), and the ground's own
+// scales: the beat (Re k1 - k2) R1 damped by Im k1 (cos and sin parts) and
+// the interface layer exp(-|k1| R1), with k1 = 2 pi sqrt(eps_r - j s);
+// 52 zero pads.
 MW_DEV void features(const float* x, float* f) {
   const float rho = x[0], h = x[1], er = x[2], s = x[3];
   const float r1 = sqrtf(rho * rho + h * h);
@@ -185,6 +188,16 @@ MW_DEV void features(const float* x, float* f) {
   f[5] = th * 1.27323954f - 1.0f;
   f[6] = expf(-2.0f * r1);
   f[7] = sinf(th);
+  // k1 / (2 pi) = sqrt(er - j s), branch with Im <= 0
+  const float mag = sqrtf(sqrtf(er * er + s * s));
+  const float half = 0.5f * atan2f(-s, er);
+  const float k1r = 6.28318531f * mag * cosf(half), k1i = 6.28318531f * mag * sinf(half);
+  const float damp = expf(k1i * r1);
+  const float beat = (k1r - 6.28318531f) * r1;
+  f[8] = cosf(beat) * damp;
+  f[9] = sinf(beat) * damp;
+  f[10] = expf(-6.28318531f * mag * r1);
+  f[11] = logf(mag) * 0.5f - 1.0f;
 }
 
 __global__ __launch_bounds__(256) void k_features(const float* __restrict__ x, int64_t n, int64_t n_pad,

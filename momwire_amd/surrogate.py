@@ -40,7 +40,8 @@ OUT_UNIT = complex(0.0, -_ground.K2_REF * _ground.C_LIGHT * _ground.MU0_EXACT / 
 
 FIN = [64, 256, 256, 256, 256]
 FOUT = [256, 256, 256, 256, 64]
-N_USED = 8
+N_USED = 8      # outputs in use
+NF_USED = 12    # input features in use
 
 
 def _torch():
@@ -56,10 +57,10 @@ def init_params(seed=0):
     rng = np.random.default_rng(seed)
     flat = []
     for l, (i, f) in enumerate(zip(FIN, FOUT)):
-        lim = math.sqrt(6.0 / (N_USED if l == 0 else i))
+        lim = math.sqrt(6.0 / (NF_USED if l == 0 else i))
         W = rng.uniform(-lim, lim, (f, i))
         if l == 0:
-            W[:, N_USED:] = 0.0
+            W[:, NF_USED:] = 0.0
         if l == len(FIN) - 1:
             W *= 0.1
             W[N_USED:, :] = 0.0

@@ -14,6 +14,7 @@ import numpy as np
 FIN = [64, 256, 256, 256, 256]
 FOUT = [256, 256, 256, 256, 64]
 N_USED = 8
+NF_USED = 12
 
 
 def n_params():
@@ -40,21 +41,33 @@ def unpack(params):
 
 
 def features(x):
-    """(n, 4) raw inputs (rho, h, eps_r, sigma/(omega eps0)) -> (n, 64)."""
-    x = np.asarray(x, dtype=np.float32).astype(np.float64)
+    """(n, 4) raw inputs (rho, h, eps_r, sigma/(omega eps0)) -> (n, 64), in
+    float32 arithmetic like the GPU's."""
+    f32 = np.float32
+    x = np.asarray(x, dtype=f32)
     rho, h, er, s = x.T
     r1 = np.sqrt(rho * rho + h * h)
     th = np.arctan2(h, rho)
-    f = np.zeros((x.shape[0], FIN[0]))
-    f[:, 0] = 0.5 * rho - 1.0
-    f[:, 1] = 0.5 * h - 1.0
-    f[:, 2] = (er - 17.5) * np.float32(0.08)
-    f[:, 3] = np.log(np.maximum(s, 1e-30) * np.float32(0.0745355992)) * np.float32(0.321810032)
-    f[:, 4] = r1 * np.float32(0.353553391) - 1.0
-    f[:, 5] = th * np.float32(1.27323954) - 1.0
-    f[:, 6] = np.exp(-2.0 * r1)
+    f = np.zeros((x.shape[0], FIN[0]), dtype=f32)
+    f[:, 0] = f32(0.5) * rho - f32(1.0)
+    f[:, 1] = f32(0.5) * h - f32(1.0)
+    f[:, 2] = (er - f32(17.5)) * f32(0.08)
+    f[:, 3] = np.log(np.maximum(s, f32(1e-30)) * f32(0.0745355992)) * f32(0.321810032)
+    f[:, 4] = r1 * f32(0.353553391) - f32(1.0)
+    f[:, 5] = th * f32(1.27323954) - f32(1.0)
+    f[:, 6] = np.exp(f32(-2.0) * r1)
     f[:, 7] = np.sin(th)
-    return f.astype(np.float32)
+    mag = np.sqrt(np.sqrt(er * er + s * s))
+    half = f32(0.5) * np.arctan2(-s, er)
+    tp = f32(6.28318531)
+    k1r, k1i = tp * mag * np.cos(half), tp * mag * np.sin(half)
+    damp = np.exp(k1i * r1)
+    beat = (k1r - tp) * r1
+    f[:, 8] = np.cos(beat) * damp
+    f[:, 9] = np.sin(beat) * damp
+    f[:, 10] = np.exp(-tp * mag * r1)
+    f[:, 11] = np.log(mag) * f32(0.5) - f32(1.0)
+    return f
 
 
 def _silu(z):
@@ -136,10 +149,10 @@ def init_params(seed=0):
     rng = np.random.default_rng(seed)
     flat = []
     for l, (i, f) in enumerate(zip(FIN, FOUT)):
-        lim = np.sqrt(6.0 / (N_USED if l == 0 else i))
+        lim = np.sqrt(6.0 / (NF_USED if l == 0 else i))
         W = rng.uniform(-lim, lim, (f, i))
         if l == 0:
-            W[:, N_USED:] = 0.0
+            W[:, NF_USED:] = 0.0
         if l == len(FIN) - 1:
             W *= 0.1
             W[N_USED:, :] = 0.0

@@ -3,9 +3,11 @@ the CPU oracle (oracle/mlp_oracle.py), the Adam step, a short training run,
 the engine fallback, and what the trained network gains.
 
 Tolerances: the GPU accumulates in FP32 (MFMA) where the oracle sums in FP64,
-so a BF16 rounding of an activation can land on the other side of a rounding
-boundary; outputs and per-layer gradients are compared at 2e-3 of their
-largest entry (one BF16 ulp is 3.9e-3 relative).
+and its float32 transcendental functions differ from numpy's in the last
+bits, so a BF16 rounding of a feature or an activation can land on the other
+side of a rounding boundary.  Outputs are compared at 5e-3 of their largest
+entry (about one BF16 ulp, 3.9e-3 relative), per-layer gradients (sums over
+the batch) at 2e-3.
 """
 
 import numpy as np
@@ -37,7 +39,7 @@ def test_forward_matches_oracle():
     x, _t = _batch(200, 1)
     got = net.predict(x)
     ref = mo.forward(p, x)["out"][:, :8].view(np.complex128)
-    assert rel_err(got, ref) < 2e-3
+    assert rel_err(got, ref) < 5e-3
 
 
 def test_gradients_match_oracle():
