@@ -629,7 +629,8 @@ template <typename T> struct BlkLds {
 };
 
 template <typename T, int RA, int CB>
-MW_DEV bool block_product(BlkLds<T>& L, const typename cplx_t<T>::type* Tg, int ldt,
+MW_DEV bool block_product(T (*As)[65], typename cplx_t<T>::type* Bs, int ldbs,
+                          const typename cplx_t<T>::type* Tg, int ldt,
                           typename cplx_t<T>::type* X, int64_t ldx, int nrow, int ncol,
                           bool check, double lbound) {
   typedef typename cplx_t<T>::type CT;
@@ -642,22 +643,23 @@ MW_DEV bool block_product(BlkLds<T>& L, const typename cplx_t<T>::type* Tg, int 
       CT v;
       v.x = 0; v.y = 0;
       if (r < nrow && c < ncol) v = X[(int64_t)r * ldx + c];
-      L.As[r][2 * c] = v.x;
-      L.As[r][2 * c + 1] = v.y;
+      As[r][2 * c] = v.x;
+      As[r][2 * c + 1] = v.y;
     }
-    for (int e = tid; e < 32 * 32; e += 256) L.Bs[e >> 5][e & 31] = Tg[(e >> 5) * ldt + (e & 31)];
+    if (Tg)  // else T is already in Bs
+      for (int e = tid; e < 32 * 32; e += 256) Bs[(e >> 5) * ldbs + (e & 31)] = Tg[(e >> 5) * ldt + (e & 31)];
   } else {
     for (int e = tid; e < 32 * 32; e += 256) {
       const CT v = Tg[(e >> 5) * ldt + (e & 31)];
-      L.As[e >> 5][2 * (e & 31)] = v.x;
-      L.As[e >> 5][2 * (e & 31) + 1] = v.y;
+      As[e >> 5][2 * (e & 31)] = v.x;
+      As[e >> 5][2 * (e & 31) + 1] = v.y;
     }
     for (int e = tid; e < 32 * 64; e += 256) {
       const int r = e >> 6, c = e & 63;
       CT v;
       v.x = 0; v.y = 0;
       if (r < nrow && c < ncol) v = X[(int64_t)r * ldx + c];
-      L.Bs[r][c] = v;
+      Bs[r * ldbs + c] = v;
     }
   }
   __syncthreads();
@@ -675,9 +677,9 @@ MW_DEV bool block_product(BlkLds<T>& L, const typename cplx_t<T>::type* Tg, int 
     for (int j = 0; j < 4; ++j) {
       const int t = 4 * wave + j;
       const int tr = t / TC, tc = t % TC;
-      const T a = L.As[16 * tr + (lane & 15)][kr];
+      const T a = As[16 * tr + (lane & 15)][kr];
       const int cr = 16 * tc + (lane & 15);
-      const CT bv = L.Bs[kc][cr >> 1];
+      const CT bv = Bs[kc * ldbs + (cr >> 1)];
       const T bb = (kpar == 0) ? ((cr & 1) ? bv.y : bv.x) : ((cr & 1) ? bv.x : -bv.y);
       acc[j] = Mfma<T>::mma(a, bb, acc[j]);
     }
@@ -750,11 +752,14 @@ MW_DEV void panel_nopiv_body(int b, int64_t n, const typename cplx_t<T>::type* A
     return;
   }
   if (P.bad) return;
-  // rows of L21 handled by this block: L21 = A21 U11^{-1} on MFMA, check only
-  __shared__ BlkLds<T> Lb;
+  // rows of L21 handled by this block: L21 = A21 U11^{-1} on MFMA, check
+  // only.  U11^{-1} stays where it is (P.Ui); the 64 rows of A21 are staged
+  // over P.M and P.Li, which are free now.
   const int64_t r0 = k0 + bw + (int64_t)(b - 1) * 64;
   const int rows = (int)((n - r0) < 64 ? (n - r0) : 64);
-  const bool over = block_product<T, 64, 32>(Lb, &P.Ui[0][0], NB + 1,
+  __syncthreads();
+  const bool over = block_product<T, 64, 32>(reinterpret_cast<T (*)[65]>(&P.M[0][0]), &P.Ui[0][0],
+                                             NB + 1, nullptr, 0,
                                              const_cast<CT*>(A) + r0 * lda + k0, lda, rows, bw,
                                              true, lbound);
   if (over) atomicOr(flag, 1);
@@ -942,11 +947,13 @@ MW_DEV void panel_apply_body(int b, int64_t n, int64_t nc, typename cplx_t<T>::t
   if (b < nrb) {
     const int64_t r0 = k0 + bw + (int64_t)b * AW;
     const int rows = (int)((n - r0) < AW ? (n - r0) : AW);
-    block_product<T, 64, 32>(L, inv + NB * NB, NB, A + r0 * lda + k0, lda, rows, bw, false, 0.0);
+    block_product<T, 64, 32>(L.As, &L.Bs[0][0], 65, inv + NB * NB, NB, A + r0 * lda + k0, lda, rows,
+                             bw, false, 0.0);
   } else if (b < nrb + ncb) {
     const int64_t c0 = k0 + bw + (int64_t)(b - nrb) * AW;
     const int cols = (int)((nc - c0) < AW ? (nc - c0) : AW);
-    block_product<T, 32, 64>(L, inv, NB, A + k0 * lda + c0, lda, bw, cols, false, 0.0);
+    block_product<T, 32, 64>(L.As, &L.Bs[0][0], 65, inv, NB, A + k0 * lda + c0, lda, bw, cols, false,
+                             0.0);
   } else {
     for (int e = tid; e < bw * bw; e += blockDim.x)
       A[(k0 + e / bw) * lda + k0 + e % bw] = lu11[(e / bw) * NB + e % bw];
@@ -971,8 +978,8 @@ MW_DEV void solve_apply_body(int bx, int64_t nrhs, const typename cplx_t<T>::typ
   __shared__ BlkLds<T> L;
   const int64_t c0 = (int64_t)bx * AW;
   const int cols = (int)((nrhs - c0) < AW ? (nrhs - c0) : AW);
-  block_product<T, 32, 64>(L, lower ? inv : inv + NB * NB, NB, X + k0 * ldx + c0, ldx, bw, cols,
-                           false, 0.0);
+  block_product<T, 32, 64>(L.As, &L.Bs[0][0], 65, lower ? inv : inv + NB * NB, NB,
+                           X + k0 * ldx + c0, ldx, bw, cols, false, 0.0);
 }
 
 template <typename T>
@@ -1306,15 +1313,23 @@ __global__ __launch_bounds__(256) void kb_panel_nopiv(const mw_sweep_prob* P, in
                            ws_inv(p.work, k0), p.ipiv, ws_flags(p.work, p.n) + k0 / NB, lbound);
 }
 
+// One workgroup per problem: problems whose panel passed the threshold test
+// (nearly all) exit at once; a flagged one runs its tournament leaves in
+// sequence -- the arrival counters make the last leaf of each group the
+// merger, exactly as when the leaves run in parallel.
 __global__ __launch_bounds__(256) void kb_panel_pivot(const mw_sweep_prob* P, int64_t k0) {
-  const mw_sweep_prob& p = P[blockIdx.y];
+  const mw_sweep_prob& p = P[blockIdx.x];
   if (k0 >= p.n) return;
+  const int32_t* flag = ws_flags(p.work, p.n) + k0 / NB;
+  if (*flag == 0) return;
   const int bw = (int)((p.n - k0) < NB ? (p.n - k0) : NB);
   const int nleaf = (int)((p.n - k0 + TROWS - 1) / TROWS);
-  if ((int)blockIdx.x >= nleaf) return;
-  panel_pivot_body<double>(blockIdx.x, (double2*)p.A, p.lda, p.n, k0, p.n + p.ncols, k0, bw, nleaf,
-                           ws_tour(p.work, p.n), ws_flags(p.work, p.n) + k0 / NB, (double2*)p.work,
-                           ws_inv(p.work, k0), p.ipiv, p.info);
+  for (int leaf = 0; leaf < nleaf; ++leaf) {
+    panel_pivot_body<double>(leaf, (double2*)p.A, p.lda, p.n, k0, p.n + p.ncols, k0, bw, nleaf,
+                             ws_tour(p.work, p.n), flag, (double2*)p.work, ws_inv(p.work, k0),
+                             p.ipiv, p.info);
+    __syncthreads();
+  }
 }
 
 __global__ __launch_bounds__(256) void kb_panel_apply(const mw_sweep_prob* P, int64_t k0) {
@@ -1734,9 +1749,8 @@ int mw_sweep_lu_run(const mw_sweep_prob* dP, const mw_ports* dports, const mw_sw
     const int64_t rest = nmax - k0 - bw;
     const int nrb = (int)((rest + AW - 1) / AW);
     const int ncb = (int)((rest + maxcols + AW - 1) / AW);
-    const int nleaf = (int)((nmax - k0 + TROWS - 1) / TROWS);
     hipLaunchKernelGGL(kb_panel_nopiv, dim3(nrb + 1, na), dim3(256), 0, st, dP, k0, lbound);
-    hipLaunchKernelGGL(kb_panel_pivot, dim3(nleaf, na), dim3(256), 0, st, dP, k0);
+    hipLaunchKernelGGL(kb_panel_pivot, dim3(na), dim3(256), 0, st, dP, k0);
     hipLaunchKernelGGL(kb_panel_apply, dim3(nrb + ncb + 1, na), dim3(256), 0, st, dP, k0);
     if (rest > 0) {
       const int64_t ncol = rest + maxcols;
