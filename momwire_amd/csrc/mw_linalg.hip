@@ -4,6 +4,8 @@
 // (double: FP64 path, float: the FP32 factor of the mixed path).
 //
 // Storage: row-major, leading dimension in complex elements.
+#include <algorithm>
+
 #include "mw_common.h"
 
 namespace {
@@ -61,13 +63,12 @@ struct GemmLds {
 
 // TA = false: C <- C - A B.  TA = true: C <- C + A^T B with A stored k x m.
 template <typename T, bool TA>
-MW_DEV void gemm_big_body(int bx, int by, int64_t m, int64_t n, int64_t kdim,
+MW_DEV void gemm_big_tile(GemmLds<T>& L, int bx, int by, int64_t m, int64_t n, int64_t kdim,
                           const typename cplx_t<T>::type* __restrict__ A, int64_t lda,
                           const typename cplx_t<T>::type* __restrict__ B, int64_t ldb,
                           typename cplx_t<T>::type* C, int64_t ldc) {
   typedef typename cplx_t<T>::type CT;
   typedef typename Mfma<T>::acc_t acc_t;
-  __shared__ GemmLds<T> L;
 
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wr = wave >> 1, wc = wave & 1;  // wave's 64x64 (real) quadrant
@@ -171,6 +172,15 @@ MW_DEV void gemm_big_body(int bx, int by, int64_t m, int64_t n, int64_t kdim,
 }
 
 template <typename T, bool TA>
+MW_DEV void gemm_big_body(int bx, int by, int64_t m, int64_t n, int64_t kdim,
+                          const typename cplx_t<T>::type* __restrict__ A, int64_t lda,
+                          const typename cplx_t<T>::type* __restrict__ B, int64_t ldb,
+                          typename cplx_t<T>::type* C, int64_t ldc) {
+  __shared__ GemmLds<T> L;
+  gemm_big_tile<T, TA>(L, bx, by, m, n, kdim, A, lda, B, ldb, C, ldc);
+}
+
+template <typename T, bool TA>
 __global__ __launch_bounds__(256) void k_gemm_sub(int64_t m, int64_t n, int64_t kdim,
                                                   const typename cplx_t<T>::type* __restrict__ A,
                                                   int64_t lda,
@@ -184,15 +194,21 @@ __global__ __launch_bounds__(256) void k_gemm_sub(int64_t m, int64_t n, int64_t 
 // tiles) for products too small to occupy the chip with 128 x 64 tiles.
 constexpr int SBM = 64, SBN = 32;
 
+template <typename T>
+struct GemmSmallLds {
+  T As[SBM][2 * GKC + 1];
+  typename cplx_t<T>::type Bs[GKC][SBN + 1];
+};
+
 template <typename T, bool TA>
-MW_DEV void gemm_small_body(int bx, int by, int64_t m, int64_t n, int64_t kdim,
-                            const typename cplx_t<T>::type* __restrict__ A, int64_t lda,
-                            const typename cplx_t<T>::type* __restrict__ B, int64_t ldb,
-                            typename cplx_t<T>::type* C, int64_t ldc) {
+MW_DEV void gemm_small_tile(GemmSmallLds<T>& L, int bx, int by, int64_t m, int64_t n,
+                            int64_t kdim, const typename cplx_t<T>::type* __restrict__ A,
+                            int64_t lda, const typename cplx_t<T>::type* __restrict__ B,
+                            int64_t ldb, typename cplx_t<T>::type* C, int64_t ldc) {
   typedef typename cplx_t<T>::type CT;
   typedef typename Mfma<T>::acc_t acc_t;
-  __shared__ T As[SBM][2 * GKC + 1];
-  __shared__ CT Bs[GKC][SBN + 1];
+  T (*As)[2 * GKC + 1] = L.As;
+  CT (*Bs)[SBN + 1] = L.Bs;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wr = wave >> 1, wc = wave & 1;
   const int64_t row0 = (int64_t)by * SBM;
@@ -257,6 +273,15 @@ MW_DEV void gemm_small_body(int bx, int by, int64_t m, int64_t n, int64_t kdim,
         const int64_t gcr = 2 * col0 + wc * 32 + j * 16 + (lane & 15);
         if (gr < m && gcr < 2 * n) Cr[gr * 2 * ldc + gcr] = acc[i][j][r];
       }
+}
+
+template <typename T, bool TA>
+MW_DEV void gemm_small_body(int bx, int by, int64_t m, int64_t n, int64_t kdim,
+                            const typename cplx_t<T>::type* __restrict__ A, int64_t lda,
+                            const typename cplx_t<T>::type* __restrict__ B, int64_t ldb,
+                            typename cplx_t<T>::type* C, int64_t ldc) {
+  __shared__ GemmSmallLds<T> L;
+  gemm_small_tile<T, TA>(L, bx, by, m, n, kdim, A, lda, B, ldb, C, ldc);
 }
 
 template <typename T, bool TA>
@@ -628,17 +653,59 @@ template <typename T> struct BlkLds {
   typename cplx_t<T>::type Bs[32][65];  // right factor as complex
 };
 
-template <typename T, int RA, int CB>
+// The MFMA core of the block products: acc = A_r B_exp with A_r (RA x 64
+// reals, LDS rows of 65) and B (32 x CB complex, LDS rows of ldbs), expanded
+// as above.  The (RA/16) x (2CB/16) output tiles of 16 x 16 are numbered
+// row-major, TPW per wave: wave w holds tiles w TPW .. w TPW + TPW - 1, and
+// acc[j][r] sits at row 16 tr + Mfma<T>::row(lane, r), real column
+// 16 tc + lane % 16 of tile t = (tr, tc).
+template <int RA, int CB>
+struct TileGrid {
+  static constexpr int TC = 2 * CB / 16;          // tile columns
+  static constexpr int TPW = (RA / 16) * TC / 4;  // tiles per wave
+};
+
+template <typename T, int RA, int CB, bool ZERO = true>
+MW_DEV void lds_mfma(const T (*As)[65], const typename cplx_t<T>::type* Bs, int ldbs,
+                     typename Mfma<T>::acc_t (&acc)[TileGrid<RA, CB>::TPW]) {
+  typedef typename cplx_t<T>::type CT;
+  constexpr int TC = TileGrid<RA, CB>::TC, TPW = TileGrid<RA, CB>::TPW;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  if (ZERO)
+#pragma unroll
+    for (int j = 0; j < TPW; ++j)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) acc[j][r] = 0;
+#pragma unroll
+  for (int ks = 0; ks < 64; ks += 4) {
+    const int kr = ks + (lane >> 4);
+    const int kc = kr >> 1, kpar = kr & 1;
+#pragma unroll
+    for (int j = 0; j < TPW; ++j) {
+      const int t = TPW * wave + j;
+      const int tr = t / TC, tc = t % TC;
+      const T a = As[16 * tr + (lane & 15)][kr];
+      const int cr = 16 * tc + (lane & 15);
+      const CT bv = Bs[kc * ldbs + (cr >> 1)];
+      const T bb = (kpar == 0) ? ((cr & 1) ? bv.y : bv.x) : ((cr & 1) ? bv.x : -bv.y);
+      acc[j] = Mfma<T>::mma(a, bb, acc[j]);
+    }
+  }
+}
+
+template <typename T, int RA, int CB, bool ROWS = (RA == 64)>
 MW_DEV bool block_product(T (*As)[65], typename cplx_t<T>::type* Bs, int ldbs,
                           const typename cplx_t<T>::type* Tg, int ldt,
                           typename cplx_t<T>::type* X, int64_t ldx, int nrow, int ncol,
                           bool check, double lbound) {
   typedef typename cplx_t<T>::type CT;
   typedef typename Mfma<T>::acc_t acc_t;
+  constexpr int TPW = TileGrid<RA, CB>::TPW;
+  static_assert(ROWS ? CB == 32 : RA == 32, "T is 32 x 32");
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   // stage: left factor (reals) and right factor (complex)
-  if (RA == 64) {
-    for (int e = tid; e < 64 * 32; e += 256) {
+  if (ROWS) {
+    for (int e = tid; e < RA * 32; e += 256) {
       const int r = e >> 5, c = e & 31;
       CT v;
       v.x = 0; v.y = 0;
@@ -654,8 +721,8 @@ MW_DEV bool block_product(T (*As)[65], typename cplx_t<T>::type* Bs, int ldbs,
       As[e >> 5][2 * (e & 31)] = v.x;
       As[e >> 5][2 * (e & 31) + 1] = v.y;
     }
-    for (int e = tid; e < 32 * 64; e += 256) {
-      const int r = e >> 6, c = e & 63;
+    for (int e = tid; e < 32 * CB; e += 256) {
+      const int r = e / CB, c = e % CB;
       CT v;
       v.x = 0; v.y = 0;
       if (r < nrow && c < ncol) v = X[(int64_t)r * ldx + c];
@@ -663,32 +730,14 @@ MW_DEV bool block_product(T (*As)[65], typename cplx_t<T>::type* Bs, int ldbs,
     }
   }
   __syncthreads();
-  constexpr int TC = 2 * CB / 16;  // tile columns
-  acc_t acc[4];
-#pragma unroll
-  for (int j = 0; j < 4; ++j)
-#pragma unroll
-    for (int r = 0; r < 4; ++r) acc[j][r] = 0;
-#pragma unroll
-  for (int ks = 0; ks < 64; ks += 4) {
-    const int kr = ks + (lane >> 4);
-    const int kc = kr >> 1, kpar = kr & 1;
-#pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      const int t = 4 * wave + j;
-      const int tr = t / TC, tc = t % TC;
-      const T a = As[16 * tr + (lane & 15)][kr];
-      const int cr = 16 * tc + (lane & 15);
-      const CT bv = Bs[kc * ldbs + (cr >> 1)];
-      const T bb = (kpar == 0) ? ((cr & 1) ? bv.y : bv.x) : ((cr & 1) ? bv.x : -bv.y);
-      acc[j] = Mfma<T>::mma(a, bb, acc[j]);
-    }
-  }
+  constexpr int TC = TileGrid<RA, CB>::TC;
+  acc_t acc[TPW];
+  lds_mfma<T, RA, CB>(As, Bs, ldbs, acc);
   bool over = false;
   T* Xr = reinterpret_cast<T*>(X);
 #pragma unroll
-  for (int j = 0; j < 4; ++j) {
-    const int t = 4 * wave + j;
+  for (int j = 0; j < TPW; ++j) {
+    const int t = TPW * wave + j;
     const int tr = t / TC, tc = t % TC;
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
@@ -727,13 +776,15 @@ MW_DEV void store_factors(const PanelLds<CT>& P, int bw, CT* lu11, CT* inv) {
 // violated bound (or a zero pivot) sets flag for the tournament fallback.
 // Block 0 stores L11\U11, the inverses and identity interchanges.
 template <typename T>
-MW_DEV void panel_nopiv_body(int b, int64_t n, const typename cplx_t<T>::type* A, int64_t lda,
-                             int64_t k0, int bw, typename cplx_t<T>::type* lu11,
-                             typename cplx_t<T>::type* inv, int32_t* ipiv, int32_t* flag,
-                             double lbound) {
+MW_DEV void panel_nopiv_body(PanelLds<typename cplx_t<T>::type>& P, int b, int64_t n,
+                             const typename cplx_t<T>::type* A, int64_t lda, int64_t k0, int bw,
+                             typename cplx_t<T>::type* lu11, typename cplx_t<T>::type* inv,
+                             int32_t* ipiv, int32_t* flag, double lbound) {
   typedef typename cplx_t<T>::type CT;
-  __shared__ PanelLds<CT> P;
   const int tid = threadIdx.x;
+  // the elimination is a latency chain: let it issue ahead of update waves
+  // sharing its SIMD (when it overlaps a trailing update)
+  __builtin_amdgcn_s_setprio(3);
   for (int e = tid; e < NB * NB; e += blockDim.x) {
     const int i = e / NB, j = e % NB;
     CT v;
@@ -771,7 +822,8 @@ __global__ __launch_bounds__(256) void k_panel_nopiv(int64_t n, const typename c
                                                      typename cplx_t<T>::type* lu11,
                                                      typename cplx_t<T>::type* inv,
                                                      int32_t* ipiv, int32_t* flag, double lbound) {
-  panel_nopiv_body<T>(blockIdx.x, n, A, lda, k0, bw, lu11, inv, ipiv, flag, lbound);
+  __shared__ PanelLds<typename cplx_t<T>::type> P;
+  panel_nopiv_body<T>(P, blockIdx.x, n, A, lda, k0, bw, lu11, inv, ipiv, flag, lbound);
 }
 
 // Tournament fallback for a panel whose diagonal failed the threshold test:
@@ -967,6 +1019,264 @@ __global__ __launch_bounds__(256) void k_panel_apply(int64_t n, int64_t nc, type
                                                      const typename cplx_t<T>::type* lu11,
                                                      int nrb, int ncb) {
   panel_apply_body<T>(blockIdx.x, n, nc, A, lda, k0, bw, inv, lu11, nrb, ncb);
+}
+
+// ---------------------------------------------------------------------------
+// Look-ahead for the one-level factorisation.  The step after panel k
+// (columns k0 .. kn = k0 + bw) splits its trailing update in two:
+//  - k_panel_apply_la: besides L21, U12 and A11, each 64-row L21 block
+//    updates its own rows of the next panel's columns [kn, kn + bwn):
+//    A -= L21 U12n, with U12n = L11^{-1} A12n formed in the block from the
+//    untouched A12n.  The U12 blocks start right of the next panel.
+//  - k_lookahead: the next panel's factor and threshold check (which read
+//    only the next panel's columns, final by now) run in the same launch as
+//    the rest of the trailing update (columns right of the next panel), with
+//    one more block that stores U12n in A.
+// The next panel's 32 x 32 elimination, a latency chain of one wave, thus
+// overlaps the bulk of the update instead of following it.
+// ---------------------------------------------------------------------------
+constexpr int LAW = 32;  // rows of an L21 block, columns of a U12 block
+
+template <typename T>
+struct LaLds {
+  T As[LAW][65];                               // A21, then L21, as reals
+  typename cplx_t<T>::type Bs[2][NB][NB + 1];  // U11^{-1} then U12n | A12n
+  T Ls[NB][65];                                // L11^{-1} as reals
+};
+
+template <typename T>
+MW_DEV void l21_lookahead_body(LaLds<T>& L, int b, int64_t n, typename cplx_t<T>::type* A,
+                               int64_t lda, int64_t k0, int bw, int bwn,
+                               const typename cplx_t<T>::type* inv) {
+  typedef typename cplx_t<T>::type CT;
+  typedef typename Mfma<T>::acc_t acc_t;
+  // 32 x 32 products: tiles t = 2 wave + j of a 2 x 4 grid
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int64_t r0 = k0 + bw + (int64_t)b * LAW;
+  const int rows = (int)((n - r0) < LAW ? (n - r0) : LAW);
+  const int64_t kn = k0 + bw;
+  T* Ar = reinterpret_cast<T*>(A);
+  // this block's rows of the next panel, loaded with the operands (no load
+  // waits on a store later on; every global store comes at the end)
+  acc_t nxt[2];
+#pragma unroll
+  for (int j = 0; j < 2; ++j) {
+    const int t = 2 * wave + j, tr = t >> 2, tc = t & 3;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int row = 16 * tr + Mfma<T>::row(lane, r);
+      const int cr = 16 * tc + (lane & 15);
+      nxt[j][r] = (row < rows && (cr >> 1) < bwn) ? Ar[(r0 + row) * 2 * lda + 2 * kn + cr] : (T)0;
+    }
+  }
+  for (int e = tid; e < NB * NB; e += 256) {
+    const int r = e >> 5, c = e & 31;
+    CT v;
+    v.x = 0; v.y = 0;
+    if (r < rows && c < bw) v = A[(r0 + r) * lda + k0 + c];
+    L.As[r][2 * c] = v.x;
+    L.As[r][2 * c + 1] = v.y;
+    L.Bs[0][r][c] = inv[NB * NB + e];
+    const CT li = inv[e];
+    L.Ls[r][2 * c] = li.x;
+    L.Ls[r][2 * c + 1] = li.y;
+    CT w;
+    w.x = 0; w.y = 0;
+    if (r < bw && c < bwn) w = A[(k0 + r) * lda + kn + c];
+    L.Bs[1][r][c] = w;
+  }
+  __syncthreads();
+  acc_t l21[2], u12[2];
+  lds_mfma<T, LAW, 32>(L.As, &L.Bs[0][0][0], NB + 1, l21);
+  lds_mfma<T, 32, 32>(L.Ls, &L.Bs[1][0][0], NB + 1, u12);
+  __syncthreads();
+  // L21 as reals into As (its real columns are As's layout), U12n into Bs[0]
+  T* Bt = reinterpret_cast<T*>(&L.Bs[0][0][0]);
+#pragma unroll
+  for (int j = 0; j < 2; ++j) {
+    const int t = 2 * wave + j, tr = t >> 2, tc = t & 3;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int row = 16 * tr + Mfma<T>::row(lane, r);
+      const int cr = 16 * tc + (lane & 15);
+      L.As[row][cr] = l21[j][r];
+      Bt[row * 2 * (NB + 1) + cr] = u12[j][r];
+    }
+  }
+  __syncthreads();
+  acc_t upd[2];
+  lds_mfma<T, LAW, 32>(L.As, &L.Bs[0][0][0], NB + 1, upd);
+#pragma unroll
+  for (int j = 0; j < 2; ++j) {
+    const int t = 2 * wave + j, tr = t >> 2, tc = t & 3;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int row = 16 * tr + Mfma<T>::row(lane, r);
+      const int cr = 16 * tc + (lane & 15);
+      if (row < rows && (cr >> 1) < bw) Ar[(r0 + row) * 2 * lda + 2 * k0 + cr] = l21[j][r];
+      if (row < rows && (cr >> 1) < bwn)
+        Ar[(r0 + row) * 2 * lda + 2 * kn + cr] = nxt[j][r] - upd[j][r];
+    }
+  }
+}
+
+// Blocks [0, nrb): L21 with the look-ahead update, LAW rows each;
+// [nrb, nrb + ncb): U12 on LAW-column blocks from kn + bwn; last: A11.  Small
+// blocks: each one's MFMA work is on the panel chain's critical path.
+template <typename T>
+__global__ __launch_bounds__(256) void k_panel_apply_la(int64_t n, int64_t nc,
+                                                        typename cplx_t<T>::type* A, int64_t lda,
+                                                        int64_t k0, int bw, int bwn,
+                                                        const typename cplx_t<T>::type* inv,
+                                                        const typename cplx_t<T>::type* lu11,
+                                                        int nrb, int ncb) {
+  __shared__ LaLds<T> L;
+  const int b = blockIdx.x;
+  if (b < nrb) {
+    l21_lookahead_body<T>(L, b, n, A, lda, k0, bw, bwn, inv);
+  } else if (b < nrb + ncb) {
+    const int64_t c0 = k0 + bw + bwn + (int64_t)(b - nrb) * LAW;
+    const int cols = (int)((nc - c0) < LAW ? (nc - c0) : LAW);
+    block_product<T, 32, LAW, false>(L.As, &L.Bs[0][0][0], NB + 1, inv, NB, A + k0 * lda + c0, lda,
+                                     bw, cols, false, 0.0);
+  } else {
+    for (int e = threadIdx.x; e < bw * bw; e += blockDim.x)
+      A[(k0 + e / bw) * lda + k0 + e % bw] = lu11[(e / bw) * NB + e % bw];
+  }
+}
+
+// C -= A B on 64 x 32 (complex) tiles with kdim <= 32: the trailing update
+// inside the look-ahead launch, whose occupancy the panel factor's registers
+// hold to two workgroups per CU.  So each update workgroup walks tiles
+// b, b + G, b + 2G, ... and loads the next tile's C, A and B into registers
+// while the MFMAs of the current one run: a tile's global latency hides
+// behind the previous tile's work instead of behind other workgroups.
+template <typename T>
+struct K32Lds {
+  T As[64][65];                          // -A as reals
+  typename cplx_t<T>::type Bs[NB][NB + 1];
+};
+
+template <typename T>
+MW_DEV void gemm_k32_tiles(K32Lds<T>& L, int b, int G, int ntiles, int gxn, int64_t m, int64_t n,
+                           int kdim, const typename cplx_t<T>::type* __restrict__ A, int64_t lda,
+                           const typename cplx_t<T>::type* __restrict__ B, int64_t ldb,
+                           typename cplx_t<T>::type* C, int64_t ldc) {
+  typedef typename cplx_t<T>::type CT;
+  typedef typename Mfma<T>::acc_t acc_t;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  T* Cr = reinterpret_cast<T*>(C);
+  acc_t cn[4];
+  CT ra[8], rb[4];
+  // tile t: rows 64 (t / gxn) .., complex columns 32 (t % gxn) ..; acc[j][r]
+  // at row 16 wave + Mfma row(lane, r), real column 16 j + lane % 16
+  auto load = [&](int t) {
+    const int64_t row0 = (int64_t)(t / gxn) * 64, col0 = (int64_t)(t % gxn) * NB;
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int64_t gr = row0 + 16 * wave + Mfma<T>::row(lane, r);
+        const int cr = 16 * j + (lane & 15);
+        cn[j][r] = (gr < m && col0 + (cr >> 1) < n) ? Cr[gr * 2 * ldc + 2 * col0 + cr] : (T)0;
+      }
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      const int e = tid + 256 * q, r = e >> 5, c = e & 31;
+      CT v;
+      v.x = 0; v.y = 0;
+      if (row0 + r < m && c < kdim) v = A[(row0 + r) * lda + c];
+      ra[q] = v;
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int e = tid + 256 * q, r = e >> 5, c = e & 31;
+      CT v;
+      v.x = 0; v.y = 0;
+      if (r < kdim && col0 + c < n) v = B[r * ldb + col0 + c];
+      rb[q] = v;
+    }
+  };
+  int t = b;
+  if (t >= ntiles) return;
+  load(t);
+  while (true) {
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      const int e = tid + 256 * q, r = e >> 5, c = e & 31;
+      L.As[r][2 * c] = -ra[q].x;
+      L.As[r][2 * c + 1] = -ra[q].y;
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int e = tid + 256 * q;
+      L.Bs[e >> 5][e & 31] = rb[q];
+    }
+    acc_t acc[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) acc[j] = cn[j];
+    const int cur = t;
+    t += G;
+    __syncthreads();
+    if (t < ntiles) load(t);
+    lds_mfma<T, 64, 32, false>(L.As, &L.Bs[0][0], NB + 1, acc);
+    const int64_t row0 = (int64_t)(cur / gxn) * 64, col0 = (int64_t)(cur % gxn) * NB;
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int64_t gr = row0 + 16 * wave + Mfma<T>::row(lane, r);
+        const int cr = 16 * j + (lane & 15);
+        if (gr < m && col0 + (cr >> 1) < n) Cr[gr * 2 * ldc + 2 * col0 + cr] = acc[j][r];
+      }
+    if (t >= ntiles) break;
+    __syncthreads();  // LDS free for the next tile's operands
+  }
+}
+
+template <typename T>
+struct NextLds {
+  typedef typename cplx_t<T>::type CT;
+  static constexpr size_t mx(size_t a, size_t b) { return a > b ? a : b; }
+  static constexpr size_t bytes =
+      mx(mx(sizeof(PanelLds<CT>), sizeof(K32Lds<T>)), sizeof(T) * NB * 65 + sizeof(CT) * NB * 65);
+};
+
+// Blocks [0, nnp): the next panel's k_panel_nopiv; nnp: U12n into A; then G
+// blocks for the trailing update of rows [kn, n) x columns [kn + bwn, nc)
+// (ntiles 64 x 32 tiles, gxn per row of tiles).  One LDS buffer serves
+// whichever part a block runs.  The factor's blocks come first so that they
+// are dispatched first.
+template <typename T>
+__global__ __launch_bounds__(256) void k_lookahead(int64_t n, int64_t nc,
+                                                   typename cplx_t<T>::type* A, int64_t lda,
+                                                   int64_t k0, int bw, int bwn,
+                                                   const typename cplx_t<T>::type* inv,
+                                                   typename cplx_t<T>::type* lu11,
+                                                   typename cplx_t<T>::type* inv_next,
+                                                   int32_t* ipiv, int32_t* flag_next,
+                                                   double lbound, int nnp, int gxn,
+                                                   int ntiles, int G) {
+  typedef typename cplx_t<T>::type CT;
+  __shared__ __align__(16) unsigned char smem[NextLds<T>::bytes];
+  const int64_t kn = k0 + bw;
+  int b = blockIdx.x;
+  if (b < nnp) {
+    panel_nopiv_body<T>(*reinterpret_cast<PanelLds<CT>*>(smem), b, n, A, lda, kn, bwn, lu11,
+                        inv_next, ipiv, flag_next, lbound);
+    return;
+  }
+  b -= nnp;
+  if (b == 0) {
+    block_product<T, 32, 64>(reinterpret_cast<T (*)[65]>(smem),
+                             reinterpret_cast<CT*>(smem + sizeof(T) * NB * 65), 65, inv, NB,
+                             A + k0 * lda + kn, lda, bw, bwn, false, 0.0);
+    return;
+  }
+  b -= 1;
+  const int64_t c0 = kn + bwn;
+  gemm_k32_tiles<T>(*reinterpret_cast<K32Lds<T>*>(smem), b, G, ntiles, gxn, n - kn, nc - c0, bw,
+                    A + kn * lda + k0, lda, A + k0 * lda + c0, lda, A + kn * lda + c0, lda);
 }
 
 // Solve blocks on AW right-hand-side columns per workgroup:
@@ -1309,7 +1619,8 @@ __global__ __launch_bounds__(256) void kb_panel_nopiv(const mw_sweep_prob* P, in
   const int bw = (int)((p.n - k0) < NB ? (p.n - k0) : NB);
   const int nrb = (int)((p.n - k0 - bw + AW - 1) / AW);
   if ((int)blockIdx.x > nrb) return;
-  panel_nopiv_body<double>(blockIdx.x, p.n, (const double2*)p.A, p.lda, k0, bw, (double2*)p.work,
+  __shared__ PanelLds<double2> L;
+  panel_nopiv_body<double>(L, blockIdx.x, p.n, (const double2*)p.A, p.lda, k0, bw, (double2*)p.work,
                            ws_inv(p.work, k0), p.ipiv, ws_flags(p.work, p.n) + k0 / NB, lbound);
 }
 
@@ -1470,16 +1781,99 @@ static double g_lbound = 8.0;  // threshold-pivoting multiplier bound (1/tau); <
 
 // Outer block width: trailing updates use K = nbo.  Small systems are bound
 // by the panel chain rather than by the updates, so they keep one level.
-static int64_t outer_block(int64_t n) { return n > 4096 ? 128 : NB; }
+// 512 measured best at 15,360 unknowns (FP64 solve 389 ms at 128, 353 at 256,
+// 338 at 512, 340 at 768, 344 at 1024).
+static int64_t outer_block(int64_t n) { return n > 4096 ? 512 : NB; }
+
+// Workgroups of k_lookahead<T> resident on the whole device at once (per
+// device, queried on first use).
+template <typename T>
+static hipError_t lookahead_resident(int* out) {
+  constexpr int kMaxDev = 64;
+  static int cached[kMaxDev] = {0};
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return e;
+  if (dev < 0 || dev >= kMaxDev) return hipErrorInvalidDevice;
+  if (cached[dev] == 0) {
+    int cus = 0, per_cu = 0;
+    if ((e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev)) != hipSuccess)
+      return e;
+    if ((e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_lookahead<T>, 256, 0)) !=
+        hipSuccess)
+      return e;
+    cached[dev] = std::max(1, cus * per_cu);
+  }
+  *out = cached[dev];
+  return hipSuccess;
+}
+
+// One-level factorisation with look-ahead (see k_lookahead): per panel the
+// tournament fallback, the apply with the next panel's update, then the next
+// panel's factor together with the rest of the trailing update, in one
+// launch.  (The same overlap from two streams joined by events ran slower:
+// the update's workgroups, dispatched first, held the CUs and the factor's
+// blocks waited behind them.)
+template <typename T>
+static int getrf_lookahead(int64_t n, int64_t nc, typename cplx_t<T>::type* A, int64_t lda,
+                           int32_t* ipiv, void* work, int32_t* flags, const TourWs& tw,
+                           int32_t* info, const volatile int32_t* cancel, hipStream_t st,
+                           hipError_t* herr) {
+  typedef typename cplx_t<T>::type CT;
+  const double lbound = g_lbound;
+  CT* lu11 = reinterpret_cast<CT*>(work);
+  int resident = 0;
+  if ((*herr = lookahead_resident<T>(&resident)) != hipSuccess) return MW_EHIP;
+  {
+    const int bw = (int)(n < NB ? n : NB);
+    const int nrb = (int)((n - bw + AW - 1) / AW);
+    hipLaunchKernelGGL(k_panel_nopiv<T>, dim3(nrb + 1), dim3(256), 0, st, n, (const CT*)A, lda,
+                       (int64_t)0, bw, lu11, reinterpret_cast<CT*>(panel_inv(work, 0)), ipiv,
+                       flags, lbound);
+  }
+  for (int64_t k0 = 0; k0 < n; k0 += NB) {
+    if (cancel && *cancel) return MW_EABORT;
+    const int bw = (int)((n - k0) < NB ? (n - k0) : NB);
+    const int64_t kn = k0 + bw;
+    const int64_t rest = n - kn;
+    const int bwn = (int)(rest < NB ? rest : NB);
+    const int nrb = (int)((rest + LAW - 1) / LAW);
+    const int ncb = (int)((nc - kn - bwn + LAW - 1) / LAW);
+    CT* inv = reinterpret_cast<CT*>(panel_inv(work, k0));
+    int32_t* flag = flags + k0 / NB;
+    const int nleaf = (int)((n - k0 + TROWS - 1) / TROWS);
+    // interchanges from column k0 on: earlier multiplier columns keep their
+    // order, as the per-panel forward solve expects
+    hipLaunchKernelGGL(k_panel_pivot<T>, dim3(nleaf), dim3(256), 0, st, A, lda, n, k0, nc, k0, bw,
+                       nleaf, tw, (const int32_t*)flag, lu11, inv, ipiv, info);
+    hipLaunchKernelGGL(k_panel_apply_la<T>, dim3(nrb + ncb + 1), dim3(256), 0, st, n, nc, A, lda,
+                       k0, bw, bwn, (const CT*)inv, (const CT*)lu11, nrb, ncb);
+    if (bwn > 0) {
+      const int nnp = (int)((rest - bwn + AW - 1) / AW) + 1;
+      const int64_t ncols = nc - kn - bwn;
+      const int gxn = (int)((ncols + NB - 1) / NB);  // 0: no update left (no RHS)
+      const int ntiles = gxn * (int)((rest + 63) / 64);
+      // update blocks: what stays resident beside the factor's blocks
+      const int G = std::max(1, std::min(ntiles, resident - nnp - 1));
+      hipLaunchKernelGGL(k_lookahead<T>, dim3((unsigned)(nnp + 1 + (ntiles > 0 ? G : 0))),
+                         dim3(256), 0, st, n, nc, A, lda, k0, bw, bwn, (const CT*)inv, lu11,
+                         reinterpret_cast<CT*>(panel_inv(work, kn)), ipiv, flag + 1, lbound, nnp,
+                         gxn > 0 ? gxn : 1, ntiles, G);
+    }
+    *herr = hipGetLastError();
+    if (*herr != hipSuccess) return MW_EHIP;
+  }
+  return MW_OK;
+}
 
 // Factor the n x n block of A; columns [n, nc) are carried along (row
 // interchanges, L^{-1} and trailing updates), so on return they hold
 // L^{-1} P B for right-hand sides B stored there: the forward half of a solve.
 //
 // Two-level blocking: 32-column panels (threshold-pivot check, tournament
-// fallback, apply) factor a 128-column block, updating only that block; the
+// fallback, apply) factor an nbo-column block, updating only that block; the
 // block's U rows for the remaining columns follow by blocked substitution and
-// the rest of the matrix takes one K = 128 update per block.
+// the rest of the matrix takes one K = nbo update per block.
 template <typename T>
 int mw_getrf_t(int64_t n, int64_t nc, void* Av, int64_t lda, int32_t* ipiv, void* work,
                int32_t* info, const volatile int32_t* cancel, hipStream_t st, hipError_t* herr) {
@@ -1501,6 +1895,8 @@ int mw_getrf_t(int64_t n, int64_t nc, void* Av, int64_t lda, int32_t* ipiv, void
   const double lbound = g_lbound;
   const int64_t nbo = outer_block(n);
   const bool one_level = nbo == NB;
+  if (one_level) return getrf_lookahead<T>(n, nc, A, lda, ipiv, work, flags, tw, info, cancel, st,
+                                           herr);
   for (int64_t K0 = 0; K0 < n; K0 += nbo) {
     const int64_t kend = (n - K0) < nbo ? n : K0 + nbo;
     for (int64_t k0 = K0; k0 < kend; k0 += NB) {
