@@ -665,8 +665,11 @@ struct TileGrid {
   static constexpr int TPW = (RA / 16) * TC / 4;  // tiles per wave
 };
 
-template <typename T, int RA, int CB, bool ZERO = true>
-MW_DEV void lds_mfma(const T (*As)[65], const typename cplx_t<T>::type* Bs, int ldbs,
+// KR: real k extent (2 x the complex k); LDA: row stride of As in reals.
+// The operands of k-step s + 1 are read from LDS before the MFMAs of step s
+// issue, so an LDS read's latency hides behind the previous step's MFMAs.
+template <typename T, int RA, int CB, bool ZERO = true, int KR = 64, int LDA = 65>
+MW_DEV void lds_mfma(const T* As, const typename cplx_t<T>::type* Bs, int ldbs,
                      typename Mfma<T>::acc_t (&acc)[TileGrid<RA, CB>::TPW]) {
   typedef typename cplx_t<T>::type CT;
   constexpr int TC = TileGrid<RA, CB>::TC, TPW = TileGrid<RA, CB>::TPW;
@@ -676,19 +679,30 @@ MW_DEV void lds_mfma(const T (*As)[65], const typename cplx_t<T>::type* Bs, int 
     for (int j = 0; j < TPW; ++j)
 #pragma unroll
       for (int r = 0; r < 4; ++r) acc[j][r] = 0;
-#pragma unroll
-  for (int ks = 0; ks < 64; ks += 4) {
+  T a[2][TPW];
+  CT bv[2][TPW];
+  auto fetch = [&](int buf, int ks) {
     const int kr = ks + (lane >> 4);
-    const int kc = kr >> 1, kpar = kr & 1;
 #pragma unroll
     for (int j = 0; j < TPW; ++j) {
       const int t = TPW * wave + j;
       const int tr = t / TC, tc = t % TC;
-      const T a = As[16 * tr + (lane & 15)][kr];
-      const int cr = 16 * tc + (lane & 15);
-      const CT bv = Bs[kc * ldbs + (cr >> 1)];
-      const T bb = (kpar == 0) ? ((cr & 1) ? bv.y : bv.x) : ((cr & 1) ? bv.x : -bv.y);
-      acc[j] = Mfma<T>::mma(a, bb, acc[j]);
+      a[buf][j] = As[(16 * tr + (lane & 15)) * LDA + kr];
+      bv[buf][j] = Bs[(kr >> 1) * ldbs + ((16 * tc + (lane & 15)) >> 1)];
+    }
+  };
+  fetch(0, 0);
+#pragma unroll
+  for (int ks = 0; ks < KR; ks += 4) {
+    const int cur = (ks >> 2) & 1;
+    if (ks + 4 < KR) fetch(cur ^ 1, ks + 4);
+    const int kpar = (ks + (lane >> 4)) & 1;
+#pragma unroll
+    for (int j = 0; j < TPW; ++j) {
+      const int cr = 16 * ((TPW * wave + j) % TC) + (lane & 15);
+      const CT b = bv[cur][j];
+      const T bb = (kpar == 0) ? ((cr & 1) ? b.y : b.x) : ((cr & 1) ? b.x : -b.y);
+      acc[j] = Mfma<T>::mma(a[cur][j], bb, acc[j]);
     }
   }
 }
@@ -732,7 +746,7 @@ MW_DEV bool block_product(T (*As)[65], typename cplx_t<T>::type* Bs, int ldbs,
   __syncthreads();
   constexpr int TC = TileGrid<RA, CB>::TC;
   acc_t acc[TPW];
-  lds_mfma<T, RA, CB>(As, Bs, ldbs, acc);
+  lds_mfma<T, RA, CB>(&As[0][0], Bs, ldbs, acc);
   bool over = false;
   T* Xr = reinterpret_cast<T*>(X);
 #pragma unroll
@@ -1087,8 +1101,8 @@ MW_DEV void l21_lookahead_body(LaLds<T>& L, int b, int64_t n, typename cplx_t<T>
   }
   __syncthreads();
   acc_t l21[2], u12[2];
-  lds_mfma<T, LAW, 32>(L.As, &L.Bs[0][0][0], NB + 1, l21);
-  lds_mfma<T, 32, 32>(L.Ls, &L.Bs[1][0][0], NB + 1, u12);
+  lds_mfma<T, LAW, 32>(&L.As[0][0], &L.Bs[0][0][0], NB + 1, l21);
+  lds_mfma<T, 32, 32>(&L.Ls[0][0], &L.Bs[1][0][0], NB + 1, u12);
   __syncthreads();
   // L21 as reals into As (its real columns are As's layout), U12n into Bs[0]
   T* Bt = reinterpret_cast<T*>(&L.Bs[0][0][0]);
@@ -1105,7 +1119,7 @@ MW_DEV void l21_lookahead_body(LaLds<T>& L, int b, int64_t n, typename cplx_t<T>
   }
   __syncthreads();
   acc_t upd[2];
-  lds_mfma<T, LAW, 32>(L.As, &L.Bs[0][0][0], NB + 1, upd);
+  lds_mfma<T, LAW, 32>(&L.As[0][0], &L.Bs[0][0][0], NB + 1, upd);
 #pragma unroll
   for (int j = 0; j < 2; ++j) {
     const int t = 2 * wave + j, tr = t >> 2, tc = t & 3;
@@ -1219,7 +1233,7 @@ MW_DEV void gemm_k32_tiles(K32Lds<T>& L, int b, int G, int ntiles, int gxn, int6
     t += G;
     __syncthreads();
     if (t < ntiles) load(t);
-    lds_mfma<T, 64, 32, false>(L.As, &L.Bs[0][0], NB + 1, acc);
+    lds_mfma<T, 64, 32, false>(&L.As[0][0], &L.Bs[0][0], NB + 1, acc);
     const int64_t row0 = (int64_t)(cur / gxn) * 64, col0 = (int64_t)(cur % gxn) * NB;
 #pragma unroll
     for (int j = 0; j < 4; ++j)
@@ -1300,12 +1314,12 @@ __global__ __launch_bounds__(256) void k_solve_apply(int64_t nrhs, const typenam
   solve_apply_body<T>(blockIdx.x, nrhs, inv, k0, bw, X, ldx, lower);
 }
 
-// One step of the back substitution, fused: every workgroup forms
-// X_p' = U_pp^{-1} X_p for its 32 right-hand-side columns in LDS (MFMA, from
-// the panel's stored inverse) and then updates its 64-row tile of the rows
-// above, X[rows][cols] -= U[rows][p] X_p'.  X_p itself is left as it is:
-// nothing later changes it, and one launch at the end (k_solve_apply_all)
-// replaces every panel by U_pp^{-1} X_p.
+// One step of the back substitution for a single panel p (an odd last panel
+// of the two-panel scheme below): every workgroup forms X_p' = U_pp^{-1} X_p
+// for its 32 right-hand-side columns in LDS (MFMA, from the panel's stored
+// inverse) and then updates its 64-row tile of the rows above,
+// X[rows][cols] -= U[rows][p] X_p'.  X_p itself is left as it is: nothing
+// later changes it, and a launch at the end replaces it by U_pp^{-1} X_p.
 template <typename T>
 __global__ __launch_bounds__(256) void k_bwd_fused(int64_t k0, int bw, int64_t nrhs,
                                                    const typename cplx_t<T>::type* __restrict__ A,
@@ -1321,6 +1335,26 @@ __global__ __launch_bounds__(256) void k_bwd_fused(int64_t k0, int bw, int64_t n
   const int64_t r0 = (int64_t)blockIdx.y * 64;
   const int cols = (int)((nrhs - c0) < 32 ? (nrhs - c0) : 32);
   const int rows = (int)((k0 - r0) < 64 ? (k0 - r0) : 64);
+  // the phase-2 tiles of U and X, loaded with the phase-1 operands
+  T* Xr = reinterpret_cast<T*>(X);
+  acc_t cx[4];
+#pragma unroll
+  for (int j = 0; j < 4; ++j)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int row = 16 * wave + Mfma<T>::row(lane, r);
+      const int cr = 16 * j + (lane & 15);
+      cx[j][r] = (row < rows && (cr >> 1) < cols) ? Xr[(r0 + row) * 2 * ldx + 2 * c0 + cr] : (T)0;
+    }
+  CT ra[8];
+#pragma unroll
+  for (int q = 0; q < 8; ++q) {
+    const int e = tid + 256 * q, r = e >> 5, c = e & 31;
+    CT v;
+    v.x = 0; v.y = 0;
+    if (r < rows && c < bw) v = A[(r0 + r) * lda + k0 + c];
+    ra[q] = v;
+  }
   for (int e = tid; e < 32 * 32; e += 256) {
     const int r = e >> 5, c = e & 31;
     const CT u = uinv[e];
@@ -1363,13 +1397,11 @@ __global__ __launch_bounds__(256) void k_bwd_fused(int64_t k0, int bw, int64_t n
       *dst = (row < bw) ? acc[j][r] : (T)0;
     }
   }
-  for (int e = tid; e < 64 * 32; e += 256) {
-    const int r = e >> 5, c = e & 31;
-    CT v;
-    v.x = 0; v.y = 0;
-    if (r < rows && c < bw) v = A[(r0 + r) * lda + k0 + c];
-    As[r][2 * c] = v.x;
-    As[r][2 * c + 1] = v.y;
+#pragma unroll
+  for (int q = 0; q < 8; ++q) {
+    const int e = tid + 256 * q, r = e >> 5, c = e & 31;
+    As[r][2 * c] = ra[q].x;
+    As[r][2 * c + 1] = ra[q].y;
   }
   __syncthreads();
   // phase 2: tile (64 x 64 reals, 16 tiles: four per wave) -= A_tile X_p'
@@ -1389,30 +1421,198 @@ __global__ __launch_bounds__(256) void k_bwd_fused(int64_t k0, int bw, int64_t n
       acc[j] = Mfma<T>::mma(a, bb, acc[j]);
     }
   }
-  T* Xr = reinterpret_cast<T*>(X);
 #pragma unroll
   for (int j = 0; j < 4; ++j)
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
       const int row = 16 * wave + Mfma<T>::row(lane, r);
       const int cr = 16 * j + (lane & 15);
-      if (row < rows && (cr >> 1) < cols) {
-        T* p = Xr + (r0 + row) * 2 * ldx + 2 * c0 + cr;
-        *p -= acc[j][r];
-      }
+      if (row < rows && (cr >> 1) < cols)
+        Xr[(r0 + row) * 2 * ldx + 2 * c0 + cr] = cx[j][r] - acc[j][r];
     }
 }
 
-// X_p <- U_pp^{-1} X_p for every panel p = blockIdx.y, AW columns per block.
+// The panel inverses in the workspace (the host's panel_inv): L^{-1}, then
+// U^{-1}, NB x NB each, 16-byte slots.
 template <typename T>
-__global__ __launch_bounds__(256) void k_solve_apply_all(int64_t n, int64_t nrhs, const void* work,
-                                                         typename cplx_t<T>::type* X, int64_t ldx) {
-  const int64_t k0 = (int64_t)blockIdx.y * NB;
-  const int bw = (int)((n - k0) < NB ? (n - k0) : NB);
-  // the panel's inverses (same layout as panel_inv on the host)
-  const char* inv = reinterpret_cast<const char*>(work) + (int64_t)NB * NB * 16 * (1 + 2 * (k0 / NB));
-  solve_apply_body<T>(blockIdx.x, nrhs, reinterpret_cast<const typename cplx_t<T>::type*>(inv), k0,
-                      bw, X, ldx, 0);
+MW_DEV const typename cplx_t<T>::type* panel_uinv(const void* work, int64_t k0) {
+  return reinterpret_cast<const typename cplx_t<T>::type*>(
+             reinterpret_cast<const char*>(work) + (int64_t)NB * NB * 16 * (1 + 2 * (k0 / NB))) +
+         NB * NB;
+}
+
+// ---------------------------------------------------------------------------
+// Back substitution, two panels per step.  Panels pair up from the top,
+// (0, 1), (2, 3), ...; an odd last panel steps alone first (k_bwd_fused).
+// For a pair (q, p = q + 1) at rows [kq, kq + NB + bwp):
+//   X_p' = U_pp^{-1} X_p,   X_q' = U_qq^{-1} (X_q - U_qp X_p'),
+// formed in LDS by every workgroup of the step for its 32 columns; each then
+// updates a 64-row tile of the rows above with K = 64:
+//   X[rows] -= [U_rq U_rp] [X_q'; X_p'].
+// The pair's own rows are left as they are (nothing later changes them);
+// k_bwd2_final forms every pair's X' at the end.  Half the launches of a
+// one-panel step, each a chain of global loads, three 32 x 32 products and
+// the update.
+// ---------------------------------------------------------------------------
+template <typename T>
+struct PairLds {
+  T U0[NB][65];                                // U_pp^{-1}, then U_qq^{-1}, as reals
+  T U1[NB][65];                                // U_qp as reals
+  typename cplx_t<T>::type Xs[2 * NB][NB + 1];  // [X_q; X_p] -> [X_q'; X_p']
+};
+
+template <typename T>
+MW_DEV void pair_stage_x(PairLds<T>& L, int64_t kq, int bwp, const typename cplx_t<T>::type* X,
+                         int64_t ldx, int64_t c0, int cols) {
+  typedef typename cplx_t<T>::type CT;
+  for (int e = threadIdx.x; e < 2 * NB * NB; e += 256) {
+    const int r = e >> 5, c = e & 31;
+    CT v;
+    v.x = 0; v.y = 0;
+    if (c < cols && (r < NB || r - NB < bwp)) v = X[(kq + r) * ldx + c0 + c];
+    L.Xs[r][c] = v;
+  }
+}
+
+// [X_q; X_p] in L.Xs -> [X_q'; X_p'].  Starts with a barrier-closed staging
+// round, so the caller's own staging needs no barrier of its own.
+template <typename T>
+MW_DEV void pair_solve(PairLds<T>& L, const void* work, const typename cplx_t<T>::type* A,
+                       int64_t lda, int64_t kq, int bwp) {
+  typedef typename cplx_t<T>::type CT;
+  typedef typename Mfma<T>::acc_t acc_t;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const CT* uq = panel_uinv<T>(work, kq);
+  const CT* up = panel_uinv<T>(work, kq + NB);
+  // every global operand is loaded here, so no later step waits on memory
+  CT rq[4];
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const int e = tid + 256 * q, r = e >> 5, c = e & 31;
+    const CT u = up[e];
+    L.U0[r][2 * c] = u.x;
+    L.U0[r][2 * c + 1] = u.y;
+    CT w;
+    w.x = 0; w.y = 0;
+    if (c < bwp) w = A[(kq + r) * lda + kq + NB + c];
+    L.U1[r][2 * c] = w.x;
+    L.U1[r][2 * c + 1] = w.y;
+    rq[q] = uq[e];
+  }
+  __syncthreads();
+  T* Xt = reinterpret_cast<T*>(&L.Xs[0][0]);
+  constexpr int LX = 2 * (NB + 1);  // row stride of Xs in reals
+  acc_t acc[2];
+  lds_mfma<T, 32, 32>(&L.U0[0][0], &L.Xs[NB][0], NB + 1, acc);  // X_p'
+  __syncthreads();
+#pragma unroll
+  for (int j = 0; j < 2; ++j) {
+    const int t = 2 * wave + j, tr = t >> 2, tc = t & 3;
+#pragma unroll
+    for (int r = 0; r < 4; ++r)
+      Xt[(NB + 16 * tr + Mfma<T>::row(lane, r)) * LX + 16 * tc + (lane & 15)] = acc[j][r];
+  }
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const int e = tid + 256 * q;
+    L.U0[e >> 5][2 * (e & 31)] = rq[q].x;
+    L.U0[e >> 5][2 * (e & 31) + 1] = rq[q].y;
+  }
+  __syncthreads();
+  lds_mfma<T, 32, 32>(&L.U1[0][0], &L.Xs[NB][0], NB + 1, acc);  // U_qp X_p'
+#pragma unroll
+  for (int j = 0; j < 2; ++j) {
+    const int t = 2 * wave + j, tr = t >> 2, tc = t & 3;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      T& x = Xt[(16 * tr + Mfma<T>::row(lane, r)) * LX + 16 * tc + (lane & 15)];
+      x = x - acc[j][r];
+    }
+  }
+  __syncthreads();
+  lds_mfma<T, 32, 32>(&L.U0[0][0], &L.Xs[0][0], NB + 1, acc);  // X_q'
+  __syncthreads();
+#pragma unroll
+  for (int j = 0; j < 2; ++j) {
+    const int t = 2 * wave + j, tr = t >> 2, tc = t & 3;
+#pragma unroll
+    for (int r = 0; r < 4; ++r)
+      Xt[(16 * tr + Mfma<T>::row(lane, r)) * LX + 16 * tc + (lane & 15)] = acc[j][r];
+  }
+  __syncthreads();
+}
+
+template <typename T>
+struct Bwd2Lds {
+  PairLds<T> P;
+  T As[64][4 * NB + 1];  // -[U_rq U_rp] of the tile's rows, as reals
+};
+
+// One step: pair (q, q + 1) at kq; blockIdx.x: 32 columns, blockIdx.y: a
+// 64-row tile of the rows above kq.
+template <typename T>
+__global__ __launch_bounds__(256) void k_bwd2(int64_t kq, int bwp, int64_t nrhs,
+                                              const typename cplx_t<T>::type* __restrict__ A,
+                                              int64_t lda, const void* work,
+                                              typename cplx_t<T>::type* X, int64_t ldx) {
+  typedef typename cplx_t<T>::type CT;
+  typedef typename Mfma<T>::acc_t acc_t;
+  __shared__ Bwd2Lds<T> L;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int64_t c0 = (int64_t)blockIdx.x * NB;
+  const int64_t r0 = (int64_t)blockIdx.y * 64;
+  const int cols = (int)((nrhs - c0) < NB ? (nrhs - c0) : NB);
+  const int rows = (int)((kq - r0) < 64 ? (kq - r0) : 64);
+  const int kw = NB + bwp;  // complex k of the update
+  // the tile to update first, in accumulator layout (tile t = wave, j)
+  T* Xr = reinterpret_cast<T*>(X);
+  acc_t c[4];
+#pragma unroll
+  for (int j = 0; j < 4; ++j)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int row = 16 * wave + Mfma<T>::row(lane, r);
+      const int cr = 16 * j + (lane & 15);
+      c[j][r] = (row < rows && (cr >> 1) < cols) ? Xr[(r0 + row) * 2 * ldx + 2 * c0 + cr] : (T)0;
+    }
+  for (int e = tid; e < 64 * 2 * NB; e += 256) {
+    const int r = e >> 6, k = e & 63;
+    CT v;
+    v.x = 0; v.y = 0;
+    if (r < rows && k < kw) v = A[(r0 + r) * lda + kq + k];
+    L.As[r][2 * k] = -v.x;
+    L.As[r][2 * k + 1] = -v.y;
+  }
+  pair_stage_x(L.P, kq, bwp, X, ldx, c0, cols);
+  pair_solve(L.P, work, A, lda, kq, bwp);
+  lds_mfma<T, 64, 32, false, 4 * NB, 4 * NB + 1>(&L.As[0][0], &L.P.Xs[0][0], NB + 1, c);
+#pragma unroll
+  for (int j = 0; j < 4; ++j)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int row = 16 * wave + Mfma<T>::row(lane, r);
+      const int cr = 16 * j + (lane & 15);
+      if (row < rows && (cr >> 1) < cols) Xr[(r0 + row) * 2 * ldx + 2 * c0 + cr] = c[j][r];
+    }
+}
+
+// Every pair's X' (blockIdx.y: pair, blockIdx.x: 32 columns).
+template <typename T>
+__global__ __launch_bounds__(256) void k_bwd2_final(int64_t n, int64_t nrhs,
+                                                    const typename cplx_t<T>::type* __restrict__ A,
+                                                    int64_t lda, const void* work,
+                                                    typename cplx_t<T>::type* X, int64_t ldx) {
+  __shared__ PairLds<T> L;
+  const int64_t kq = (int64_t)blockIdx.y * 2 * NB;
+  const int bwp = (int)((n - kq - NB) < NB ? (n - kq - NB) : NB);
+  const int64_t c0 = (int64_t)blockIdx.x * NB;
+  const int cols = (int)((nrhs - c0) < NB ? (nrhs - c0) : NB);
+  pair_stage_x(L, kq, bwp, X, ldx, c0, cols);
+  pair_solve(L, work, A, lda, kq, bwp);
+  for (int e = threadIdx.x; e < 2 * NB * NB; e += 256) {
+    const int r = e >> 5, c = e & 31;
+    if (c < cols && (r < NB || r - NB < bwp)) X[(kq + r) * ldx + c0 + c] = L.Xs[r][c];
+  }
 }
 
 // Row interchanges ipiv[r0 .. r1) applied in order to columns [0, ncols) of X.
@@ -1991,15 +2191,28 @@ static int solve_bwd(int64_t n, int64_t nrhs, const typename cplx_t<T>::type* A,
   typedef typename cplx_t<T>::type CT;
   if (nrhs <= 0 || n <= 0) return MW_OK;
   const int64_t npan = (n + NB - 1) / NB;
-  const int64_t nlast = (npan - 1) * NB;
-  for (int64_t k0 = nlast; k0 > 0; k0 -= NB) {
-    const int bw = (int)((n - k0) < NB ? (n - k0) : NB);
-    const CT* ui = reinterpret_cast<const CT*>(panel_inv(const_cast<void*>(work), k0)) + NB * NB;
-    hipLaunchKernelGGL(k_bwd_fused<T>, dim3((unsigned)((nrhs + 31) / 32), (unsigned)((k0 + 63) / 64)),
-                       dim3(256), 0, st, k0, bw, nrhs, A, lda, ui, X, ldx);
+  const int64_t npair = npan / 2;
+  const unsigned gx = (unsigned)((nrhs + NB - 1) / NB);
+  const int64_t klone = (npan - 1) * NB;  // an odd last panel steps alone
+  if (npan % 2 == 1 && klone > 0) {
+    const CT* ui = reinterpret_cast<const CT*>(panel_inv(const_cast<void*>(work), klone)) + NB * NB;
+    hipLaunchKernelGGL(k_bwd_fused<T>, dim3(gx, (unsigned)((klone + 63) / 64)), dim3(256), 0, st,
+                       klone, (int)(n - klone), nrhs, A, lda, ui, X, ldx);
   }
-  hipLaunchKernelGGL(k_solve_apply_all<T>, dim3((unsigned)((nrhs + AW - 1) / AW), (unsigned)npan),
-                     dim3(256), 0, st, n, nrhs, work, X, ldx);
+  for (int64_t i = npair - 1; i >= 1; --i) {
+    const int64_t kq = 2 * i * NB;
+    const int bwp = (int)((n - kq - NB) < NB ? (n - kq - NB) : NB);
+    hipLaunchKernelGGL(k_bwd2<T>, dim3(gx, (unsigned)((kq + 63) / 64)), dim3(256), 0, st, kq, bwp,
+                       nrhs, A, lda, work, X, ldx);
+  }
+  if (npair > 0)
+    hipLaunchKernelGGL(k_bwd2_final<T>, dim3(gx, (unsigned)npair), dim3(256), 0, st, n, nrhs, A,
+                       lda, work, X, ldx);
+  if (npan % 2 == 1) {
+    const CT* inv = reinterpret_cast<const CT*>(panel_inv(const_cast<void*>(work), klone));
+    hipLaunchKernelGGL(k_solve_apply<T>, dim3((unsigned)((nrhs + AW - 1) / AW)), dim3(256), 0, st,
+                       nrhs, inv, klone, (int)(n - klone), X, ldx, 0);
+  }
   *herr = hipGetLastError();
   return *herr == hipSuccess ? MW_OK : MW_EHIP;
 }
