@@ -507,6 +507,7 @@ struct PanelLds {
   CT dinv[NB];
   int bad;
   int piv_s;
+  int progress;       // columns of M published by wave_lu_inv's elimination
 };
 
 // 1/d for a complex d != 0: conj(d) / |d|^2 with the real reciprocal from
@@ -528,19 +529,41 @@ template <typename CT> MW_DEV CT crecip_fast(CT d) {
   return o;
 }
 
-// LU without row exchanges of the NB x NB block in P.M (identity-padded),
-// by wavefront 0 alone: lane r holds row r in registers and the pivot row is
-// broadcast with v_readlane, so the 31 elimination steps need no barrier.
-// Rows at or above the pivot get a zero multiplier instead of a predicated
-// update (x - 0*u is x exactly for finite u).  A zero pivot zeroes the step's
-// multipliers; it, or a multiplier beyond lbound, sets P.bad (the panel is
-// then refactored with pivoting from A).  Called by every thread of
-// the workgroup; ends with a barrier.
+// LU without row exchanges of the NB x NB block in P.M (identity-padded) with
+// both inverses formed alongside it.  Wave 0 eliminates: lane r holds row r
+// in registers and the pivot row is broadcast with v_readlane, so the 31
+// steps need no barrier.  Rows at or above the pivot get a zero multiplier
+// instead of a predicated update (x - 0*u is x exactly for finite u).  A zero
+// pivot zeroes the step's multipliers; it, or a multiplier beyond lbound,
+// sets P.bad (the panel is then refactored with pivoting from A).  After
+// elimination step k every row's column k is final: wave 0 publishes it to
+// P.M and advances P.progress.  Wave 1 forms L^{-1} a row at a time (lane c
+// solves L x = e_c; row r reads columns < r of L), wave 2 forms U^{-1} a
+// column at a time (lane r holds row r of X with X U = I; column c reads
+// column c of U), each waiting only for the columns it reads, so the three
+// chains overlap instead of following each other.  The waits poll a counter
+// that wave 0 of the same workgroup advances and that wave never waits on
+// the others, so every wait ends.  A zero pivot leaves a zero diagonal
+// inverse (and P.bad set).  Called by every thread; ends with a barrier.
 template <typename CT>
-MW_DEV void wave_lu(PanelLds<CT>& P, double lbound) {
-  const int tid = threadIdx.x;
-  if (tid < 64) {
-    const int r = tid & 31;
+MW_DEV void publish_progress(PanelLds<CT>& P, int v) {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+  if ((threadIdx.x & 63) == 0)
+    __hip_atomic_store(&P.progress, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
+template <typename CT>
+MW_DEV void wait_progress(PanelLds<CT>& P, int v) {
+  while (__hip_atomic_load(&P.progress, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) < v)
+    __builtin_amdgcn_s_sleep(1);
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+}
+
+template <typename CT>
+MW_DEV void wave_lu_inv(PanelLds<CT>& P, double lbound) {
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  if (wave == 0) {
+    const int r = lane & 31;
     CT R[NB];
 #pragma unroll
     for (int j = 0; j < NB; ++j) R[j] = P.M[r][j];
@@ -548,7 +571,6 @@ MW_DEV void wave_lu(PanelLds<CT>& P, double lbound) {
 #pragma unroll
     for (int k = 0; k < NB - 1; ++k) {
       const CT d = lane_bcast(R[k], k);
-      // a zero pivot leaves the step without multipliers (and flags the panel)
       const bool zero = d.x == 0 && d.y == 0;
       bad |= zero;
       CT inv = crecip_fast(d);
@@ -561,16 +583,70 @@ MW_DEV void wave_lu(PanelLds<CT>& P, double lbound) {
       l.y = below ? l.y : 0;
       R[k].x = below ? l.x : R[k].x;
       R[k].y = below ? l.y : R[k].y;
+      if (lane < 32) P.M[r][k] = R[k];
+      publish_progress(P, k + 1);
 #pragma unroll
       for (int j = k + 1; j < NB; ++j) R[j] = cfms(R[j], l, lane_bcast(R[j], k));
     }
     const CT last = lane_bcast(R[NB - 1], NB - 1);
     if (last.x == 0 && last.y == 0) bad = true;
-    if (tid < 32) {
+    if (lane < 32) P.M[r][NB - 1] = R[NB - 1];
+    publish_progress(P, NB);
+    if (__any(bad) && lane == 0) P.bad = 1;
+  } else if (wave == 1) {
+    const int c = lane & 31;
+    CT x[NB];
+    x[0].x = (c == 0) ? 1 : 0;
+    x[0].y = 0;
 #pragma unroll
-      for (int j = 0; j < NB; ++j) P.M[r][j] = R[j];
+    for (int r = 1; r < NB; ++r) {
+      __builtin_amdgcn_sched_barrier(0);  // keep each row's LDS reads after its wait
+      wait_progress(P, r);
+      CT s4[4];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) { s4[q].x = 0; s4[q].y = 0; }
+      s4[0].x = (r == c) ? 1 : 0;
+#pragma unroll
+      for (int j = 0; j < r; ++j) {
+        // at most eight operands in flight (registers)
+        if ((j & 7) == 7) __builtin_amdgcn_sched_barrier(0);
+        s4[j & 3] = cfms(s4[j & 3], P.M[r][j], x[j]);
+      }
+      x[r].x = (s4[0].x + s4[1].x) + (s4[2].x + s4[3].x);
+      x[r].y = (s4[0].y + s4[1].y) + (s4[2].y + s4[3].y);
+      // stored at once: the store pins the row's arithmetic before the
+      // next wait (left to the end, it would be sunk there, holding every
+      // loaded operand live)
+      if (lane < 32) P.Li[r][c] = x[r];
     }
-    if (__any(bad) && tid == 0) P.bad = 1;
+    if (lane < 32) P.Li[0][c] = x[0];
+  } else if (wave == 2) {
+    const int r = lane & 31;
+    CT x[NB];
+#pragma unroll
+    for (int c = 0; c < NB; ++c) {
+      __builtin_amdgcn_sched_barrier(0);  // keep each column's LDS reads after its wait
+      wait_progress(P, c + 1);
+      const CT u = P.M[c][c];
+      const bool zero = u.x == 0 && u.y == 0;
+      CT d = crecip_fast(u);
+      d.x = zero ? 0 : d.x;
+      d.y = zero ? 0 : d.y;
+      CT s4[4];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) { s4[q].x = 0; s4[q].y = 0; }
+      s4[0].x = (r == c) ? 1 : 0;
+#pragma unroll
+      for (int j = 0; j < c; ++j) {
+        if ((j & 7) == 7) __builtin_amdgcn_sched_barrier(0);
+        s4[j & 3] = cfms(s4[j & 3], x[j], P.M[j][c]);
+      }
+      CT t;
+      t.x = (s4[0].x + s4[1].x) + (s4[2].x + s4[3].x);
+      t.y = (s4[0].y + s4[1].y) + (s4[2].y + s4[3].y);
+      x[c] = cmul(t, d);
+      if (lane < 32) P.Ui[r][c] = x[c];
+    }
   }
   __syncthreads();
 }
@@ -806,10 +882,12 @@ MW_DEV void panel_nopiv_body(PanelLds<typename cplx_t<T>::type>& P, int b, int64
     if (i < bw && j < bw) v = A[(k0 + i) * lda + k0 + j];
     P.M[i][j] = v;
   }
-  if (tid == 0) P.bad = (lbound <= 0.0) ? 1 : 0;
+  if (tid == 0) {
+    P.bad = (lbound <= 0.0) ? 1 : 0;
+    P.progress = 0;
+  }
   __syncthreads();
-  wave_lu(P, lbound);
-  wave_inverses(P);
+  wave_lu_inv(P, lbound);
   if (b == 0) {
     store_factors(P, bw, lu11, inv);
     if (tid < bw) ipiv[k0 + tid] = (int32_t)(k0 + tid);
@@ -831,7 +909,7 @@ MW_DEV void panel_nopiv_body(PanelLds<typename cplx_t<T>::type>& P, int b, int64
 }
 
 template <typename T>
-__global__ __launch_bounds__(256) void k_panel_nopiv(int64_t n, const typename cplx_t<T>::type* A,
+__global__ __launch_bounds__(256, 2) void k_panel_nopiv(int64_t n, const typename cplx_t<T>::type* A,
                                                      int64_t lda, int64_t k0, int bw,
                                                      typename cplx_t<T>::type* lu11,
                                                      typename cplx_t<T>::type* inv,
@@ -1262,7 +1340,7 @@ struct NextLds {
 // whichever part a block runs.  The factor's blocks come first so that they
 // are dispatched first.
 template <typename T>
-__global__ __launch_bounds__(256) void k_lookahead(int64_t n, int64_t nc,
+__global__ __launch_bounds__(256, 2) void k_lookahead(int64_t n, int64_t nc,
                                                    typename cplx_t<T>::type* A, int64_t lda,
                                                    int64_t k0, int bw, int bwn,
                                                    const typename cplx_t<T>::type* inv,
@@ -1812,7 +1890,7 @@ __global__ void kb_rhs(const mw_sweep_prob* P, const mw_ports* ports, int scatte
   }
 }
 
-__global__ __launch_bounds__(256) void kb_panel_nopiv(const mw_sweep_prob* P, int64_t k0,
+__global__ __launch_bounds__(256, 2) void kb_panel_nopiv(const mw_sweep_prob* P, int64_t k0,
                                                       double lbound) {
   const mw_sweep_prob& p = P[blockIdx.y];
   if (k0 >= p.n) return;
