@@ -537,14 +537,13 @@ template <typename CT> MW_DEV CT crecip_fast(CT d) {
 // pivot zeroes the step's multipliers; it, or a multiplier beyond lbound,
 // sets P.bad (the panel is then refactored with pivoting from A).  After
 // elimination step k every row's column k is final: wave 0 publishes it to
-// P.M and advances P.progress.  Wave 1 forms L^{-1} a row at a time (lane c
-// solves L x = e_c; row r reads columns < r of L), wave 2 forms U^{-1} a
-// column at a time (lane r holds row r of X with X U = I; column c reads
-// column c of U), each waiting only for the columns it reads, so the three
-// chains overlap instead of following each other.  The waits poll a counter
-// that wave 0 of the same workgroup advances and that wave never waits on
-// the others, so every wait ends.  A zero pivot leaves a zero diagonal
-// inverse (and P.bad set).  Called by every thread; ends with a barrier.
+// P.M and advances P.progress.  Other waves consume the columns as they
+// appear -- L^{-1} a row at a time, U^{-1} or L21 = A21 U^{-1} a column at a
+// time -- each waiting only for the columns it reads, so the chains overlap
+// instead of following each other.  The waits poll a counter that wave 0 of
+// the same workgroup advances and that wave never waits on the others, so
+// every wait ends.  A zero pivot leaves a zero diagonal inverse (and P.bad
+// set).
 template <typename CT>
 MW_DEV void publish_progress(PanelLds<CT>& P, int v) {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
@@ -560,95 +559,127 @@ MW_DEV void wait_progress(PanelLds<CT>& P, int v) {
 }
 
 template <typename CT>
-MW_DEV void wave_lu_inv(PanelLds<CT>& P, double lbound) {
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  if (wave == 0) {
-    const int r = lane & 31;
-    CT R[NB];
+MW_DEV void wave_lu_publish(PanelLds<CT>& P, double lbound) {
+  const int lane = threadIdx.x & 63;
+  const int r = lane & 31;
+  CT R[NB];
 #pragma unroll
-    for (int j = 0; j < NB; ++j) R[j] = P.M[r][j];
-    bool bad = false;
+  for (int j = 0; j < NB; ++j) R[j] = P.M[r][j];
+  bool bad = false;
+  // 1 / pivot (0 for a zero pivot, which marks the panel)
+  auto pivot_inverse = [&](int k) {
+    const CT d = lane_bcast(R[k], k);
+    const bool zero = d.x == 0 && d.y == 0;
+    bad |= zero;
+    CT inv = crecip_fast(d);
+    inv.x = zero ? 0 : inv.x;
+    inv.y = zero ? 0 : inv.y;
+    return inv;
+  };
+  CT inv = pivot_inverse(0);
 #pragma unroll
-    for (int k = 0; k < NB - 1; ++k) {
-      const CT d = lane_bcast(R[k], k);
-      const bool zero = d.x == 0 && d.y == 0;
-      bad |= zero;
-      CT inv = crecip_fast(d);
-      inv.x = zero ? 0 : inv.x;
-      inv.y = zero ? 0 : inv.y;
-      const bool below = r > k;
-      CT l = cmul(R[k], inv);
-      if (below && !(cabs1(l) <= lbound)) bad = true;
-      l.x = below ? l.x : 0;
-      l.y = below ? l.y : 0;
-      R[k].x = below ? l.x : R[k].x;
-      R[k].y = below ? l.y : R[k].y;
-      if (lane < 32) P.M[r][k] = R[k];
-      publish_progress(P, k + 1);
+  for (int k = 0; k < NB - 1; ++k) {
+    const bool below = r > k;
+    CT l = cmul(R[k], inv);
+    if (below && !(cabs1(l) <= lbound)) bad = true;
+    l.x = below ? l.x : 0;
+    l.y = below ? l.y : 0;
+    R[k].x = below ? l.x : R[k].x;
+    R[k].y = below ? l.y : R[k].y;
+    if (lane < 32) P.M[r][k] = R[k];
+    // column k + 1 first: the next pivot's reciprocal (a chain of dependent
+    // operations) then issues while the rest of this step's updates do
+    R[k + 1] = cfms(R[k + 1], l, lane_bcast(R[k + 1], k));
+    if (k + 1 < NB - 1) inv = pivot_inverse(k + 1);
 #pragma unroll
-      for (int j = k + 1; j < NB; ++j) R[j] = cfms(R[j], l, lane_bcast(R[j], k));
+    for (int j = k + 2; j < NB; ++j) R[j] = cfms(R[j], l, lane_bcast(R[j], k));
+    // after the step's updates: the column's LDS write has landed meanwhile
+    publish_progress(P, k + 1);
+  }
+  const CT last = lane_bcast(R[NB - 1], NB - 1);
+  if (last.x == 0 && last.y == 0) bad = true;
+  if (lane < 32) P.M[r][NB - 1] = R[NB - 1];
+  publish_progress(P, NB);
+  if (__any(bad) && lane == 0) P.bad = 1;
+}
+
+// L^{-1}, a row at a time as the columns of L are published: lane c solves
+// L x = e_c (row r reads columns < r).
+template <typename CT>
+MW_DEV void wave_linv(PanelLds<CT>& P) {
+  const int c = threadIdx.x & 31;
+  CT x[NB];
+  x[0].x = (c == 0) ? 1 : 0;
+  x[0].y = 0;
+  if ((threadIdx.x & 63) < 32) P.Li[0][c] = x[0];
+#pragma unroll
+  for (int r = 1; r < NB; ++r) {
+    __builtin_amdgcn_sched_barrier(0);  // keep each row's LDS reads after its wait
+    wait_progress(P, r);
+    CT s4[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) { s4[q].x = 0; s4[q].y = 0; }
+    s4[0].x = (r == c) ? 1 : 0;
+#pragma unroll
+    for (int j = 0; j < r; ++j) {
+      // at most four operands in flight (registers)
+      if ((j & 3) == 3) __builtin_amdgcn_sched_barrier(0);
+      s4[j & 3] = cfms(s4[j & 3], P.M[r][j], x[j]);
     }
-    const CT last = lane_bcast(R[NB - 1], NB - 1);
-    if (last.x == 0 && last.y == 0) bad = true;
-    if (lane < 32) P.M[r][NB - 1] = R[NB - 1];
-    publish_progress(P, NB);
-    if (__any(bad) && lane == 0) P.bad = 1;
-  } else if (wave == 1) {
-    const int c = lane & 31;
-    CT x[NB];
-    x[0].x = (c == 0) ? 1 : 0;
-    x[0].y = 0;
+    x[r].x = (s4[0].x + s4[1].x) + (s4[2].x + s4[3].x);
+    x[r].y = (s4[0].y + s4[1].y) + (s4[2].y + s4[3].y);
+    // stored at once: the store pins the row's arithmetic before the next
+    // wait (left to the end, it would be sunk there, holding every loaded
+    // operand live)
+    if ((threadIdx.x & 63) < 32) P.Li[r][c] = x[r];
+  }
+}
+
+// Rows of X with X U = I (rhs == nullptr; lane r: row r of U^{-1}, into
+// P.Ui), or of L21 with L21 U = A21 (lane r: row r of A21 at rhs, LDS rows
+// of NB + 1, overwritten by L21), a column at a time as the columns of U are
+// published (column c reads column c of U).  Returns, per lane, whether an
+// entry of a checked row (check) exceeds lbound or is not finite.
+template <typename CT>
+MW_DEV bool wave_right_solve(PanelLds<CT>& P, const CT* rhs, bool check, double lbound) {
+  const int r = threadIdx.x & (rhs ? 63 : 31);
+  CT x[NB];
+  bool over = false;
 #pragma unroll
-    for (int r = 1; r < NB; ++r) {
-      __builtin_amdgcn_sched_barrier(0);  // keep each row's LDS reads after its wait
-      wait_progress(P, r);
-      CT s4[4];
+  for (int c = 0; c < NB; ++c) {
+    __builtin_amdgcn_sched_barrier(0);  // keep each column's LDS reads after its wait
+    wait_progress(P, c + 1);
+    const CT u = P.M[c][c];
+    const bool zero = u.x == 0 && u.y == 0;
+    CT d = crecip_fast(u);
+    d.x = zero ? 0 : d.x;
+    d.y = zero ? 0 : d.y;
+    CT s4[4];
 #pragma unroll
-      for (int q = 0; q < 4; ++q) { s4[q].x = 0; s4[q].y = 0; }
+    for (int q = 0; q < 4; ++q) { s4[q].x = 0; s4[q].y = 0; }
+    if (rhs)
+      s4[0] = rhs[r * (NB + 1) + c];
+    else
       s4[0].x = (r == c) ? 1 : 0;
 #pragma unroll
-      for (int j = 0; j < r; ++j) {
-        // at most eight operands in flight (registers)
-        if ((j & 7) == 7) __builtin_amdgcn_sched_barrier(0);
-        s4[j & 3] = cfms(s4[j & 3], P.M[r][j], x[j]);
-      }
-      x[r].x = (s4[0].x + s4[1].x) + (s4[2].x + s4[3].x);
-      x[r].y = (s4[0].y + s4[1].y) + (s4[2].y + s4[3].y);
-      // stored at once: the store pins the row's arithmetic before the
-      // next wait (left to the end, it would be sunk there, holding every
-      // loaded operand live)
-      if (lane < 32) P.Li[r][c] = x[r];
+    for (int j = 0; j < c; ++j) {
+      if ((j & 3) == 3) __builtin_amdgcn_sched_barrier(0);
+      s4[j & 3] = cfms(s4[j & 3], x[j], P.M[j][c]);
     }
-    if (lane < 32) P.Li[0][c] = x[0];
-  } else if (wave == 2) {
-    const int r = lane & 31;
-    CT x[NB];
-#pragma unroll
-    for (int c = 0; c < NB; ++c) {
-      __builtin_amdgcn_sched_barrier(0);  // keep each column's LDS reads after its wait
-      wait_progress(P, c + 1);
-      const CT u = P.M[c][c];
-      const bool zero = u.x == 0 && u.y == 0;
-      CT d = crecip_fast(u);
-      d.x = zero ? 0 : d.x;
-      d.y = zero ? 0 : d.y;
-      CT s4[4];
-#pragma unroll
-      for (int q = 0; q < 4; ++q) { s4[q].x = 0; s4[q].y = 0; }
-      s4[0].x = (r == c) ? 1 : 0;
-#pragma unroll
-      for (int j = 0; j < c; ++j) {
-        if ((j & 7) == 7) __builtin_amdgcn_sched_barrier(0);
-        s4[j & 3] = cfms(s4[j & 3], x[j], P.M[j][c]);
-      }
-      CT t;
-      t.x = (s4[0].x + s4[1].x) + (s4[2].x + s4[3].x);
-      t.y = (s4[0].y + s4[1].y) + (s4[2].y + s4[3].y);
-      x[c] = cmul(t, d);
-      if (lane < 32) P.Ui[r][c] = x[c];
+    CT t;
+    t.x = (s4[0].x + s4[1].x) + (s4[2].x + s4[3].x);
+    t.y = (s4[0].y + s4[1].y) + (s4[2].y + s4[3].y);
+    x[c] = cmul(t, d);
+    // each result is stored at once: the store pins the column's arithmetic
+    // before the next wait (see wave_linv)
+    if (check) {
+      over |= !(cabs1(x[c]) <= lbound);
+      const_cast<CT*>(rhs)[r * (NB + 1) + c] = x[c];
+    } else if ((threadIdx.x & 63) < 32) {
+      P.Ui[r][c] = x[c];
     }
   }
-  __syncthreads();
+  return over;
 }
 
 // Inverses of the factors held in P.M (L\U, padded to NB with the
@@ -886,26 +917,39 @@ MW_DEV void panel_nopiv_body(PanelLds<typename cplx_t<T>::type>& P, int b, int64
     P.bad = (lbound <= 0.0) ? 1 : 0;
     P.progress = 0;
   }
+  // blocks b > 0 check 64 rows of L21 = A21 U11^{-1}, one row per lane of
+  // wave 1, solved against U11 as its columns appear (no inverse needed);
+  // A21 is staged where the inverses would go (Li, Ui: 64 rows of NB + 1)
+  CT* a21 = &P.Li[0][0];
+  const int64_t r0 = k0 + bw + (int64_t)(b - 1) * 64;
+  if (b > 0)
+    for (int e = tid; e < 64 * NB; e += blockDim.x) {
+      const int r = e >> 5, c = e & 31;
+      CT v;
+      v.x = 0; v.y = 0;
+      if (r0 + r < n && c < bw) v = A[(r0 + r) * lda + k0 + c];
+      a21[r * (NB + 1) + c] = v;
+    }
+  // wave-uniform (readfirstlane): the roles below are scalar branches
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   __syncthreads();
-  wave_lu_inv(P, lbound);
+  bool over = false;
+  if (wave == 0)
+    wave_lu_publish(P, lbound);
+  else if (b == 0 && wave == 1)
+    wave_linv(P);
+  else if (b == 0 && wave == 2)
+    wave_right_solve(P, (const CT*)nullptr, false, 0.0);
+  else if (b > 0 && wave == 1)
+    over = wave_right_solve(P, a21, true, lbound) && r0 + (tid & 63) < n;
+  __syncthreads();
   if (b == 0) {
     store_factors(P, bw, lu11, inv);
     if (tid < bw) ipiv[k0 + tid] = (int32_t)(k0 + tid);
     if (tid == 0 && P.bad) atomicOr(flag, 1);
     return;
   }
-  if (P.bad) return;
-  // rows of L21 handled by this block: L21 = A21 U11^{-1} on MFMA, check
-  // only.  U11^{-1} stays where it is (P.Ui); the 64 rows of A21 are staged
-  // over P.M and P.Li, which are free now.
-  const int64_t r0 = k0 + bw + (int64_t)(b - 1) * 64;
-  const int rows = (int)((n - r0) < 64 ? (n - r0) : 64);
-  __syncthreads();
-  const bool over = block_product<T, 64, 32>(reinterpret_cast<T (*)[65]>(&P.M[0][0]), &P.Ui[0][0],
-                                             NB + 1, nullptr, 0,
-                                             const_cast<CT*>(A) + r0 * lda + k0, lda, rows, bw,
-                                             true, lbound);
-  if (over) atomicOr(flag, 1);
+  if (__any(over) && (tid & 63) == 0) atomicOr(flag, 1);
 }
 
 template <typename T>
