@@ -444,8 +444,9 @@ def main():
     dominant = "solve" if solve_s >= fill_s else "fill"
     if dominant == "solve":
         roof = {"kernel": "factor + all-port solve (mw_zgesv: per 32-column panel, threshold-"
-                          "pivot check, apply, MFMA trailing update over [Z | B]; then back "
-                          "substitution)",
+                          "pivot fallback, apply with the next panel's update, then the next "
+                          "panel's factor in one launch with the MFMA trailing update over "
+                          "[Z | B]; then back substitution two panels per step)",
                 "bound": "mfma", "achieved": lu_tf, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
                 "frac": lu_tf / PEAK_FP64_TFLOPS, "traffic": None,
                 "algorithmic": f"(8/3) n^3 + 8 n^2 p = {sv_flops:.3e} flop per solve "

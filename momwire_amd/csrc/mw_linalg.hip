@@ -1718,6 +1718,23 @@ __global__ __launch_bounds__(256) void k_bwd2(int64_t kq, int bwp, int64_t nrhs,
     }
 }
 
+// One pair's X' (blockIdx.x: 32 columns), written over the pair's rows.
+template <typename T>
+__global__ __launch_bounds__(256) void k_bwd2_pair(int64_t kq, int bwp, int64_t nrhs,
+                                                   const typename cplx_t<T>::type* __restrict__ A,
+                                                   int64_t lda, const void* work,
+                                                   typename cplx_t<T>::type* X, int64_t ldx) {
+  __shared__ PairLds<T> L;
+  const int64_t c0 = (int64_t)blockIdx.x * NB;
+  const int cols = (int)((nrhs - c0) < NB ? (nrhs - c0) : NB);
+  pair_stage_x(L, kq, bwp, X, ldx, c0, cols);
+  pair_solve(L, work, A, lda, kq, bwp);
+  for (int e = threadIdx.x; e < 2 * NB * NB; e += 256) {
+    const int r = e >> 5, c = e & 31;
+    if (c < cols && (r < NB || r - NB < bwp)) X[(kq + r) * ldx + c0 + c] = L.Xs[r][c];
+  }
+}
+
 // Every pair's X' (blockIdx.y: pair, blockIdx.x: 32 columns).
 template <typename T>
 __global__ __launch_bounds__(256) void k_bwd2_final(int64_t n, int64_t nrhs,
@@ -2316,6 +2333,29 @@ static int solve_bwd(int64_t n, int64_t nrhs, const typename cplx_t<T>::type* A,
   const int64_t npair = npan / 2;
   const unsigned gx = (unsigned)((nrhs + NB - 1) / NB);
   const int64_t klone = (npan - 1) * NB;  // an odd last panel steps alone
+  if (nrhs > 256) {
+    // Many right-hand sides: each step's grid fills the chip, so the pair's
+    // X' is formed once (not in every workgroup) and the rows above take a
+    // plain K = 64 update.
+    if (npan % 2 == 1) {
+      const CT* inv = reinterpret_cast<const CT*>(panel_inv(const_cast<void*>(work), klone));
+      hipLaunchKernelGGL(k_solve_apply<T>, dim3((unsigned)((nrhs + AW - 1) / AW)), dim3(256), 0,
+                         st, nrhs, inv, klone, (int)(n - klone), X, ldx, 0);
+      *herr = launch_gemm_sub<T>(klone, nrhs, n - klone, A + klone, lda, X + klone * ldx, ldx, X,
+                                 ldx, st);
+      if (*herr != hipSuccess) return MW_EHIP;
+    }
+    for (int64_t i = npair - 1; i >= 0; --i) {
+      const int64_t kq = 2 * i * NB;
+      const int bwp = (int)((n - kq - NB) < NB ? (n - kq - NB) : NB);
+      hipLaunchKernelGGL(k_bwd2_pair<T>, dim3(gx), dim3(256), 0, st, kq, bwp, nrhs, A, lda, work,
+                         X, ldx);
+      *herr = launch_gemm_sub<T>(kq, nrhs, NB + bwp, A + kq, lda, X + kq * ldx, ldx, X, ldx, st);
+      if (*herr != hipSuccess) return MW_EHIP;
+    }
+    *herr = hipGetLastError();
+    return *herr == hipSuccess ? MW_OK : MW_EHIP;
+  }
   if (npan % 2 == 1 && klone > 0) {
     const CT* ui = reinterpret_cast<const CT*>(panel_inv(const_cast<void*>(work), klone)) + NB * NB;
     hipLaunchKernelGGL(k_bwd_fused<T>, dim3(gx, (unsigned)((klone + 63) / 64)), dim3(256), 0, st,

@@ -83,6 +83,23 @@ def test_lu_solve_random(n, p):
 
 
 @pytest.mark.parametrize("dtype", ["z", "c"])
+@pytest.mark.parametrize("n,p", [(1100, 300), (1024, 257), (33, 260)])
+def test_lu_solve_many_rhs(dtype, n, p):
+    """More than 256 right-hand sides: the back substitution forms each panel
+    pair's solution once and updates the rows above with a plain GEMM (odd and
+    even panel counts, a ragged last panel)."""
+    _say("lu many rhs", dtype, n, p)
+    rng = np.random.default_rng(n + p)
+    M = rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))
+    RHS = rng.standard_normal((n, p)) + 1j * rng.standard_normal((n, p))
+    X, info = _lu_solve(M, RHS, dtype)
+    assert info == 0
+    ref = np.linalg.solve(M, RHS)
+    eps = 1.1e-16 if dtype == "z" else 6e-8
+    assert rel_err(X, ref) < eps * np.linalg.cond(M) * 200
+
+
+@pytest.mark.parametrize("dtype", ["z", "c"])
 @pytest.mark.parametrize("n,p", [(1, 1), (40, 3), (300, 64), (1111, 9)])
 def test_gesv_augmented(dtype, n, p):
     """mw_?gesv: the right-hand sides ride through the factorisation as extra
@@ -267,7 +284,7 @@ def test_cancel_token_stops_factorisation():
 
 @pytest.mark.parametrize("dtype", ["z", "c"])
 def test_two_level_lu_large(dtype):
-    """n > 4096 takes the two-level blocking (128-column blocks, K = 128
+    """n > 4096 takes the two-level blocking (512-column blocks, K = 512
     trailing updates); random entries exercise the tournament fallback and a
     dominant diagonal the threshold path."""
     for dom in (0.0, 1.0):
