@@ -328,6 +328,26 @@ int mw_fill_ground(const mw_mesh* mesh, double k, double omega, double eps, doub
                    int32_t symmetric, void* stream);
 
 /*
+ * Reflection-coefficient finite ground (NEC gn 0 style): Z -= Z_image with
+ * per-segment-pair Fresnel weights -- the A term by
+ * w_A = rho_v (t_m . M t_n - P) - rho_h P and the charge term by
+ * w_Phi = c0 + c1 rho_v, rho_v and rho_h taken at the specular angle of
+ * the ray from the source segment's image midpoint to the observer's
+ * midpoint for eps~ = eps_t (P: the tangents' products with the horizontal
+ * normal of the plane of incidence).  (c0, c1) select the charge weighting
+ * (reference _ground_refl.phi_mode_coeffs).  The weights are formed in the
+ * kernel; no Q term.  Replaces the weighted image assembly with Fresnel
+ * tables: assemble_Z_bspline_weighted(_windowed) (reference
+ * _accelerators.cpp:1790-1917; weights _ground_refl.py:65-289; caller
+ * bspline.py:1654-1740).
+ */
+int mw_fill_ground_refl(const mw_mesh* mesh, double k, double omega, double eps, double mu,
+                        double ground_z, double eps_t_re, double eps_t_im, double phi_c0_re,
+                        double phi_c0_im, double phi_c1_re, double phi_c1_im, void* Z,
+                        int64_t ldz, const int32_t* tile_pairs, int32_t n_pairs,
+                        int32_t symmetric, void* stream);
+
+/*
  * The same surfaces with a complex permittivity per node (eps: [n][2]):
  * the sample generator of the learned surrogate.  No counterpart in the
  * reference (its engine takes one ground per call, _sommerfeld.py:468-502).

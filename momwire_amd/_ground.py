@@ -18,6 +18,7 @@ masters stay resident in HBM in a small FIFO cache.
 """
 
 import ctypes
+import cmath
 import math
 
 import numpy as np
@@ -64,6 +65,27 @@ def eps_tilde(ground_eps, omega, eps0):
 def image_coefficient(eps_t):
     """C2 = (eps - 1)/(eps + 1); 1 for PEC (eps_t None)."""
     return 1.0 + 0j if eps_t is None else (eps_t - 1.0) / (eps_t + 1.0)
+
+
+PHI_MODES = ("rho_v", "image", "normal", "blend")
+
+
+def phi_coefficients(mode, eps_t):
+    """The reflection-coefficient ground's charge weight as w_Phi = c0 +
+    c1 rho_v (reference _ground_refl.py:276-289): "rho_v" the TM coefficient
+    at the pair's specular angle, "image" the quasi-static (eps-1)/(eps+1),
+    "normal" rho_v at normal incidence (the default), "blend" the mean of
+    "rho_v" and "image"."""
+    if mode == "rho_v":
+        return 0j, 1 + 0j
+    if mode == "image":
+        return (eps_t - 1.0) / (eps_t + 1.0), 0j
+    if mode == "normal":
+        r = cmath.sqrt(eps_t)
+        return (r - 1.0) / (r + 1.0), 0j
+    if mode == "blend":
+        return 0.5 * (eps_t - 1.0) / (eps_t + 1.0), 0.5 + 0j
+    raise ValueError(f"ground_phi_mode must be one of {PHI_MODES}, got {mode!r}")
 
 
 def eps_bucket(eps_t):

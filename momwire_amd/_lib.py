@@ -120,6 +120,8 @@ SIGNATURES = {
     "mw_somm_eval": (ctypes.c_int, [_P, _P, _P, _I64, _P, _P]),
     "mw_fill_ground": (ctypes.c_int, [ctypes.POINTER(MwMesh), _D, _D, _D, _D, _D, _D, _D, _P, _P,
                                       _I64, _P, _I32, _I32, _P]),
+    "mw_fill_ground_refl": (ctypes.c_int, [ctypes.POINTER(MwMesh), _D, _D, _D, _D, _D, _D, _D, _D,
+                                           _D, _D, _D, _P, _I64, _P, _I32, _I32, _P]),
     "mw_somm_surfaces_varied": (ctypes.c_int, [_P, _D, _D, _D, _P, _P, _I64, _D, _P, _P, _P]),
     "mw_mlp_param_count": (_I64, []),
     "mw_mlp_w16_count": (_I64, []),

@@ -8,14 +8,16 @@ LU factorisation, the all-port solve, the KCL Schur step and the Y readout run
 as HIP kernels on the GPU (momwire_amd/csrc); Python only prepares the
 k-independent tables and moves results.  There is no CPU fallback.
 
-Supported: free space, PEC ground and the Sommerfeld ground
-(``ground_model="sommerfeld"``: C2-scaled exact image plus the interpolated
-remainder); degree 2; n_qp_pair 4; point delta-gap feeds; K-wire junctions
-(KCL rows), junction ports and node gaps; per-wire radii.
+Supported: free space, PEC ground, the reflection-coefficient finite ground
+(``ground_model="refl-coef"``, the default with ``ground_eps``: the image
+weighted per segment pair by Fresnel coefficients, any ``ground_phi_mode``)
+and the Sommerfeld ground (``ground_model="sommerfeld"``: C2-scaled exact
+image plus the interpolated remainder); degree 2; n_qp_pair 4; point
+delta-gap feeds; K-wire junctions (KCL rows), junction ports and node gaps;
+per-wire radii.
 Refused with NotImplementedError (as the reference refuses combinations it
-does not serve): the reflection-coefficient ground, wires touching the ground
-plane, wire loading, extended kernel, singular enrichment, segment-gap and
-smoothed feeds.
+does not serve): wires touching the ground plane, wire loading, extended
+kernel, singular enrichment, segment-gap and smoothed feeds.
 """
 
 import ctypes
@@ -190,9 +192,9 @@ class BSplineSolver(_SweptPortSolutions):
             raise ValueError(f"ground_model must be 'refl-coef' or 'sommerfeld', got {ground_model!r}")
         if ground_model == "sommerfeld" and ground_eps is None:
             raise ValueError("ground_model='sommerfeld' requires ground_eps")
-        if ground_z is not None and ground_eps is not None and ground_model == "refl-coef":
-            raise NotImplementedError("the reflection-coefficient ground is not served; "
-                                      "use ground_model='sommerfeld'")
+        if ground_phi_mode not in _ground.PHI_MODES:
+            raise ValueError(f"ground_phi_mode must be one of {_ground.PHI_MODES}, "
+                             f"got {ground_phi_mode!r}")
         if ground_model == "sommerfeld" and int(n_qp_sommerfeld) != 3:
             raise NotImplementedError("the remainder block is built for n_qp_sommerfeld=3")
         if int(swept_mem_mb) < 1:
@@ -440,13 +442,24 @@ class BSplineSolver(_SweptPortSolutions):
         return D.Y[:D.n_ports, :D.n_ports], D.X[:, :D.n_ports]
 
     def _fill(self, D, k, omega, zptr, ld, st):
-        """Free-space fill, then the ground terms Z -= C2 * image + Q."""
+        """Free-space fill, then the ground terms: Z -= C2 * image + Q
+        (PEC: C2 = 1, no Q), or Z -= the Fresnel-weighted image for the
+        reflection-coefficient ground."""
         lib = _lib.load()
         mesh = self._mesh
         pairs = D.keep["pairs"]
         _lib.check(lib.mw_fill_z(D.mw, k, omega, self.eps, self.mu, zptr, ld, pairs.data_ptr(),
                                  int(pairs.shape[0]), int(mesh.symmetric), st), "fill")
         if self.ground_z is None:
+            return
+        if self.ground_eps is not None and self.ground_model == "refl-coef":
+            eps_t = _ground.eps_tilde(self.ground_eps, omega, self.eps)
+            c0, c1 = _ground.phi_coefficients(self.ground_phi_mode, eps_t)
+            _lib.check(lib.mw_fill_ground_refl(D.mw, k, omega, self.eps, self.mu, self.ground_z,
+                                               eps_t.real, eps_t.imag, c0.real, c0.imag, c1.real,
+                                               c1.imag, zptr, ld, pairs.data_ptr(),
+                                               int(pairs.shape[0]), int(mesh.symmetric), st),
+                       "reflection-coefficient ground fill")
             return
         grid = self._ground_grid(k, omega)
         eps_t = None if self.ground_eps is None else _ground.eps_tilde(self.ground_eps, omega,

@@ -3,6 +3,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <cmath>
 #include <vector>
 
 #include "mw_common.h"
@@ -58,8 +59,9 @@ hipError_t mw_launch_somm_eval(const mw_somm_grid& G, const double* R1, const do
                                double2* out, hipStream_t st);
 hipError_t mw_launch_fill_ground(const mw_mesh& M, double k, double omega, double eps, double mu,
                                  double gz, double c2r, double c2i, const mw_somm_grid* G,
-                                 double2* Z, int64_t ldz, const int32_t* pairs, int32_t n_pairs,
-                                 bool symmetric, hipStream_t st);
+                                 const double* refl, double2* Z, int64_t ldz,
+                                 const int32_t* pairs, int32_t n_pairs, bool symmetric,
+                                 hipStream_t st);
 
 hipError_t mw_launch_somm_surfaces_varied(const double* eps, double k2, double omega, double mu,
                                           const double* R1, const double* th, int64_t n, double rtol,
@@ -480,9 +482,31 @@ int mw_fill_ground(const mw_mesh* mesh, double k, double omega, double eps, doub
   if (symmetric && (!tile_pairs || n_pairs <= 0))
     return fail(MW_EINVAL, "%ssymmetric fill needs the tile-pair list");
   return hip_status(mw_launch_fill_ground(*mesh, k, omega, eps, mu, ground_z, c2_re, c2_im, grid,
-                                          (double2*)Z, ldz, tile_pairs, n_pairs, symmetric != 0,
-                                          (hipStream_t)stream),
+                                          nullptr, (double2*)Z, ldz, tile_pairs, n_pairs,
+                                          symmetric != 0, (hipStream_t)stream),
                     "mw_fill_ground");
+}
+
+int mw_fill_ground_refl(const mw_mesh* mesh, double k, double omega, double eps, double mu,
+                        double ground_z, double eps_t_re, double eps_t_im, double phi_c0_re,
+                        double phi_c0_im, double phi_c1_re, double phi_c1_im, void* Z,
+                        int64_t ldz, const int32_t* tile_pairs, int32_t n_pairs,
+                        int32_t symmetric, void* stream) {
+  g_err[0] = 0;
+  int rc = check_mesh(mesh);
+  if (rc) return rc;
+  if (!Z) return fail(MW_EINVAL, "%sZ is NULL");
+  if (ldz < mesh->n_basis) return fail(MW_EINVAL, "%sldz %lld < n_basis %lld", "", ldz, mesh->n_basis);
+  if (!(k > 0.0) || !(omega > 0.0)) return fail(MW_EINVAL, "%sk and omega must be positive");
+  if (!std::isfinite(eps_t_re) || !std::isfinite(eps_t_im))
+    return fail(MW_EINVAL, "%sground permittivity is not finite");
+  if (symmetric && (!tile_pairs || n_pairs <= 0))
+    return fail(MW_EINVAL, "%ssymmetric fill needs the tile-pair list");
+  const double refl[6] = {eps_t_re, eps_t_im, phi_c0_re, phi_c0_im, phi_c1_re, phi_c1_im};
+  return hip_status(mw_launch_fill_ground(*mesh, k, omega, eps, mu, ground_z, 1.0, 0.0, nullptr,
+                                          refl, (double2*)Z, ldz, tile_pairs, n_pairs,
+                                          symmetric != 0, (hipStream_t)stream),
+                    "mw_fill_ground_refl");
 }
 
 int mw_somm_surfaces_varied(const double* eps, double k2, double omega, double mu,

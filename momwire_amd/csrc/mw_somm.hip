@@ -630,6 +630,7 @@ constexpr int NM = 3, NQ = 4, NQS = 3;
 constexpr double INV4PI = 0.079577471545947667884441881686257181017229822870228;
 
 struct GStage {
+  double cen[3];         // segment midpoint (never mirrored)
   double pos[NQ][3];     // 4-point nodes (image moments; mirrored for sources)
   double wu[NM][NQ];
   double sp[NQS][3];     // 3-point remainder nodes
@@ -645,11 +646,15 @@ struct GroundArgs {
   double c2r, c2i;
   double t3[NQS], w3[NQS];  // remainder rule on [0, 1]
   int remainder;
+  // reflection-coefficient ground: eps~ and the charge weight c0 + c1 rho_v
+  int refl;
+  double er, ei, p0r, p0i, p1r, p1i;
 };
 
 MW_DEV void stage_ground(const mw_mesh& M, int s, bool source, const GroundArgs& A, GStage& S) {
   const double* g = M.seg + (size_t)s * 12;
   const double L = g[9];
+  for (int c = 0; c < 3; ++c) S.cen[c] = 0.5 * (g[c] + g[3 + c]);
   for (int q = 0; q < NQ; ++q) {
     const double t = M.gl[q];
     for (int c = 0; c < 3; ++c) S.pos[q][c] = (1.0 - t) * g[c] + t * g[3 + c];
@@ -668,6 +673,35 @@ MW_DEV void stage_ground(const mw_mesh& M, int s, bool source, const GroundArgs&
   const double* bp = M.band_poly + (size_t)s * 9;
   for (int i = 0; i < 9; ++i) S.poly[i / 3][i % 3] = bp[i];
   S.dir = src_dir(g[6], g[7], g[8]);
+}
+
+// Fresnel weights of one segment pair for the reflection-coefficient ground
+// (reference _ground_refl.py:65-273): the specular ray runs from the
+// source's image midpoint to the observer's midpoint; rho_v, rho_h at its
+// incidence angle (principal-branch sqrt); p the horizontal unit vector
+// normal to the plane of incidence (x for a vertical ray), P = (t_i.p)(t_j.p);
+// w_A = rho_v (td - P) - rho_h P, w_Phi = c0 + c1 rho_v.  Symmetric in the
+// pair, so the symmetric tile fill stays valid.
+MW_DEV void refl_weights(const GStage& I, const GStage& J, const GroundArgs& A, double td, cd& wa,
+                         cd& wp) {
+  constexpr double TINY = 1e-30;
+  const double dx = I.cen[0] - J.cen[0], dy = I.cen[1] - J.cen[1];
+  const double dz = I.cen[2] + J.cen[2] - 2.0 * A.gz;
+  const double hyp = hypot(dx, dy);
+  const double rmag = fmax(sqrt(dx * dx + dy * dy + dz * dz), TINY);
+  const double ct = dz / rmag;
+  const bool safe = hyp > TINY;
+  const double ih = 1.0 / (safe ? hyp : 1.0);
+  const double px = safe ? -dy * ih : 1.0, py = safe ? dx * ih : 0.0;
+  // x and y of the (mirrored) source tangent are the unmirrored ones
+  const double P = (I.tan[0] * px + I.tan[1] * py) * (J.tan[0] * px + J.tan[1] * py);
+  const cd e = cz(A.er, A.ei);
+  const cd root = zsqrt(cz(A.er - (1.0 - ct * ct), A.ei));
+  const cd ec = zsc(e, ct);
+  const cd rv = zdiv(zsub(ec, root), zadd(ec, root));
+  const cd rh = zdiv(zsub(cz(ct, 0.0), root), zadd(cz(ct, 0.0), root));
+  wa = zsub(zsc(rv, td - P), zsc(rh, P));
+  wp = zadd(cz(A.p0r, A.p0i), zmul(cz(A.p1r, A.p1i), rv));
 }
 
 template <bool SYM>
@@ -734,6 +768,8 @@ MW_DEV void ground_body(int ti, int tj, const mw_mesh& M, const GroundArgs& A,
       if (live) {
         // t_i . M t_j: the source stage holds the mirrored tangent
         const double td = I.tan[0] * J.tan[0] + I.tan[1] * J.tan[1] + I.tan[2] * J.tan[2];
+        cd wa = zsc(cz(A.c2r, A.c2i), td), wp = cz(A.c2r, A.c2i);
+        if (A.refl) refl_weights(I, J, A, td, wa, wp);
         for (int be = be0; be < be1; ++be) {
           const double cj0 = J.poly[be][0], cj1 = J.poly[be][1], cj2 = J.poly[be][2];
           double tr[NM], tm[NM];
@@ -756,11 +792,11 @@ MW_DEV void ground_body(int ti, int tj, const mw_mesh& M, const GroundArgs& A,
             const double Am = ci0 * tm[0] + ci1 * tm[1] + ci2 * tm[2];
             const double Pr = ci1 * pr[0] + 2.0 * ci2 * pr[1];
             const double Pm = ci1 * pm[0] + 2.0 * ci2 * pm[1];
-            // image entry j w mu td A + Phi / (j w eps), scaled by -C2
-            const double zr = -A.wmu * td * Am + Pm * A.inv_weps;
-            const double zm = A.wmu * td * Ar - Pr * A.inv_weps;
-            bt[2 * (al * 3 + be)] = -(A.c2r * zr - A.c2i * zm);
-            bt[2 * (al * 3 + be) + 1] = -(A.c2r * zm + A.c2i * zr);
+            // image entry -(j w mu w_A A + w_Phi Phi / (j w eps))
+            const double Wr = wa.x * Ar - wa.y * Am, Wi = wa.x * Am + wa.y * Ar;
+            const double Vr = wp.x * Pr - wp.y * Pm, Vi = wp.x * Pm + wp.y * Pr;
+            bt[2 * (al * 3 + be)] = A.wmu * Wi - Vi * A.inv_weps;
+            bt[2 * (al * 3 + be) + 1] = -A.wmu * Wr + Vr * A.inv_weps;
           }
         }
       }
@@ -896,9 +932,17 @@ hipError_t mw_launch_somm_eval(const mw_somm_grid& G, const double* R1, const do
 
 hipError_t mw_launch_fill_ground(const mw_mesh& M, double k, double omega, double eps, double mu,
                                  double gz, double c2r, double c2i, const mw_somm_grid* G,
-                                 double2* Z, int64_t ldz, const int32_t* pairs, int32_t n_pairs,
-                                 bool symmetric, hipStream_t st) {
+                                 const double* refl, double2* Z, int64_t ldz,
+                                 const int32_t* pairs, int32_t n_pairs, bool symmetric,
+                                 hipStream_t st) {
   GroundArgs A;
+  A.refl = refl != nullptr;
+  A.er = A.ei = A.p0r = A.p0i = A.p1r = A.p1i = 0.0;
+  if (refl) {
+    A.er = refl[0]; A.ei = refl[1];
+    A.p0r = refl[2]; A.p0i = refl[3];
+    A.p1r = refl[4]; A.p1i = refl[5];
+  }
   A.k = k;
   A.wmu = omega * mu;
   A.inv_weps = 1.0 / (omega * eps);

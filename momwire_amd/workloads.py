@@ -98,8 +98,10 @@ def ground_parameters(seed):
 
 
 def grounded_array(nx=4, ny=4, n_seg=21, seed=4, length=0.48, radius=1e-3, pitch=0.5,
-                   params=None):
-    """Horizontal dipole array over Sommerfeld ground (ground plane z = 0).
+                   params=None, ground_model="sommerfeld", phi_mode=None):
+    """Horizontal dipole array over finite ground (ground plane z = 0),
+    Sommerfeld by default; ``ground_model="refl-coef"`` (with an optional
+    ``phi_mode``) for the reflection-coefficient ground.
 
     ``params`` overrides the seeded (height/lambda, eps_r, sigma, freq) draw.
     """
@@ -122,18 +124,22 @@ def grounded_array(nx=4, ny=4, n_seg=21, seed=4, length=0.48, radius=1e-3, pitch
         feeds=[(w, None, 1.0) for w in range(len(wires))],
         ground_z=0.0,
         ground_eps=(eps_r, sigma),
-        ground_model="sommerfeld",
+        ground_model=ground_model,
+        **({} if phi_mode is None else {"ground_phi_mode": phi_mode}),
     )
 
 
-def grounded_tee(eps_r=12.0, sigma=5e-3, freq_hz=10e6):
-    """The junction tee scaled to ``freq_hz`` and lifted above Sommerfeld ground."""
+def grounded_tee(eps_r=12.0, sigma=5e-3, freq_hz=10e6, ground_model="sommerfeld", phi_mode=None):
+    """The junction tee scaled to ``freq_hz`` and lifted above finite ground
+    (Sommerfeld by default)."""
     kw = junction_tee("feed")
     lam = C_LIGHT / freq_hz
     kw["wires"] = [[(p[0] * lam, p[1] * lam, p[2] * lam) for p in w] for w in kw["wires"]]
     kw["wire_radius"] = 1e-3 * lam
     kw["wavelength"] = lam
-    kw.update(ground_z=0.0, ground_eps=(eps_r, sigma), ground_model="sommerfeld")
+    kw.update(ground_z=0.0, ground_eps=(eps_r, sigma), ground_model=ground_model)
+    if phi_mode is not None:
+        kw["ground_phi_mode"] = phi_mode
     return kw
 
 

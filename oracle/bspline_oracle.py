@@ -38,10 +38,11 @@ class OracleSolver:
                  feed_wire_index=0, feed_arclength=None, junctions=None,
                  junction_ports=None, node_gaps=None, n_qp_pair=4, wavelength=22,
                  wire_radius=0.0005, nsegs=101, ground_z=None, ground_eps=None,
-                 ground_model="refl-coef", n_qp_sommerfeld=3):
+                 ground_model="refl-coef", ground_phi_mode="normal", n_qp_sommerfeld=3):
         self.degree = int(degree)
         self.ground_z, self.ground_eps = ground_z, ground_eps
         self.ground_model, self.n_qp_sommerfeld = ground_model, int(n_qp_sommerfeld)
+        self.ground_phi_mode = ground_phi_mode
         self.wires = [np.asarray(w, dtype=float) for w in wires]
         n_w = len(self.wires)
         r = np.asarray(wire_radius, dtype=float)
@@ -230,8 +231,10 @@ class OracleSolver:
                 J[:, :, sl, sl] = A_st + A_reg
         return J
 
-    def assemble(self, J, supp, polys, geom, td=None):
-        """bspline.py:2322-2348 (numpy einsum assembly)."""
+    def assemble(self, J, supp, polys, geom, td=None, wphi=None):
+        """bspline.py:2322-2348 (numpy einsum assembly); with per-segment-pair
+        weights td on the A term and wphi on the charge term, the weighted
+        image assembly of bspline.py:1700-1740."""
         d = self.degree
         if td is None:
             td = geom["tangents"] @ geom["tangents"].T
@@ -248,7 +251,8 @@ class OracleSolver:
                     "mp,pPmn,nP->mn", polys[:, a, :], Jb, polys[:, b, :])
                 dm = polys[:, a, 1:] * pv[None, :]
                 dn = polys[:, b, 1:] * pv[None, :]
-                ZP += np.einsum("mp,pPmn,nP->mn", dm, Jb[:d, :d], dn)
+                zp = np.einsum("mp,pPmn,nP->mn", dm, Jb[:d, :d], dn)
+                ZP += zp if wphi is None else wphi[sm[:, None], sn[None, :]] * zp
         return 1j * self.omega * MU0 * ZA + ZP / (1j * self.omega * EPS0)
 
     def Z(self, geom=None, basis=None, remainder=True):
@@ -259,14 +263,13 @@ class OracleSolver:
             Z = Z - self.ground_terms(geom, supp, polys, remainder)
         return Z
 
-    # -- Sommerfeld ground: bspline.py:3730-3770, 2552-2661, 1791-1874 --------
+    # -- ground: bspline.py:3730-3770, 2552-2661, 1654-1740, 1791-1874 -------
     def ground_terms(self, geom, supp, polys, remainder=True):
-        """C2 * image + Q, the block the grounded fill subtracts."""
+        """The block the grounded fill subtracts: the PEC image (no
+        ground_eps), C2 * image + Q (Sommerfeld), or the Fresnel-weighted
+        image (reflection coefficient, _ground_refl.py)."""
+        from . import refl_oracle as ro
         from . import sommerfeld_oracle as so
-        if self.ground_eps is None or self.ground_model != "sommerfeld":
-            raise NotImplementedError("the oracle covers the Sommerfeld ground only")
-        eps_t = so.eps_tilde(self.ground_eps, self.omega, EPS0)
-        c2 = so.image_coefficient(eps_t)
         gz = self.ground_z
         mir = np.array([1.0, 1.0, -1.0])
         img_l, img_r = geom["seg_l"] * mir, geom["seg_r"] * mir
@@ -274,6 +277,15 @@ class OracleSolver:
         img_r[:, 2] += 2 * gz
         J_img = self.full_moments(geom, self.seg_radius(geom), self.k, src=(img_l, img_r))
         td_img = geom["tangents"] @ (geom["tangents"] * mir).T
+        if self.ground_eps is None:
+            return self.assemble(J_img, supp, polys, geom, td=td_img)
+        if self.ground_model == "refl-coef":
+            eps_t = ro.eps_tilde(self.ground_eps, self.omega, EPS0)
+            centers = 0.5 * (geom["seg_l"] + geom["seg_r"])
+            w_A, w_Phi = ro.weights(centers, geom["tangents"], gz, eps_t, self.ground_phi_mode)
+            return self.assemble(J_img, supp, polys, geom, td=w_A, wphi=w_Phi)
+        eps_t = so.eps_tilde(self.ground_eps, self.omega, EPS0)
+        c2 = so.image_coefficient(eps_t)
         out = c2 * self.assemble(J_img, supp, polys, geom, td=td_img)
         if remainder:
             grid = self.ground_grid(geom, eps_t)

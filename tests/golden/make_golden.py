@@ -194,7 +194,53 @@ def case_c5():
     return out, dt
 
 
+def _refl_solve(kw, extra=True):
+    """Reflection-coefficient ground (reference _ground_refl.py): Y, coeffs,
+    and for small cases Z and the per-segment-pair weight tables."""
+    from momwire import _ground_refl
+    s, sol, dt = _solve(kw)
+    out = dict(y=sol.y, coeffs=sol.coeffs, k=s.k, omega=s.omega)
+    eps_t = _ground_refl.eps_tilde(s.ground_eps, s.omega, s.eps)
+    out["eps_t"] = np.array(eps_t)
+    if extra:
+        geom = s._build_geometry()
+        supp, polys, _kcl, _wk, _wb = s._build_basis_polynomials(geom)
+        out["Z"] = np.asarray(s._compute_Z_operator(geom, supp, polys))
+        seg_c = 0.5 * (geom["seg_l"] + geom["seg_r"])
+        cos_th, td_img, P = _ground_refl.specular_pair_tables(seg_c, geom["tangents"], s.ground_z)
+        rho_v, rho_h = _ground_refl.fresnel_rho(eps_t, cos_th)
+        out["w_A"] = _ground_refl.a_term_weights(rho_v, rho_h, td_img, P)
+        out["w_Phi"] = np.broadcast_to(
+            _ground_refl.phi_term_weights(s.ground_phi_mode, eps_t, rho_v), cos_th.shape).copy()
+        out["cos_th"] = cos_th
+    return out, dt
+
+
+def case_refl_dipole():
+    kw = workloads.grounded_array(1, 1, 21, params=(0.3, 12.0, 0.01, 10e6),
+                                  ground_model="refl-coef")
+    return _refl_solve(kw)
+
+
+def case_refl_tee():
+    return _refl_solve(workloads.grounded_tee(ground_model="refl-coef", phi_mode="rho_v"))
+
+
+def case_refl_tee_blend():
+    return _refl_solve(workloads.grounded_tee(eps_r=25.0, sigma=2e-2, freq_hz=7e6,
+                                              ground_model="refl-coef", phi_mode="blend"))
+
+
+def case_refl_array4x4():
+    return _refl_solve(workloads.grounded_array(4, 4, 21, seed=4, ground_model="refl-coef"),
+                       extra=False)
+
+
 CASES = {
+    "refl_dipole": case_refl_dipole,
+    "refl_tee": case_refl_tee,
+    "refl_tee_blend": case_refl_tee_blend,
+    "refl_array4x4": case_refl_array4x4,
     "somm_engine": case_somm_engine,
     "somm_grid_far": case_somm_grid_far,
     "ground_dipole": case_ground_dipole,

@@ -212,6 +212,52 @@ def test_grounded_port_solution_matches_reference(case):
     assert rel_err(sol.coeffs, d["coeffs"]) < 1e-10
 
 
+REFL = {
+    "refl_dipole": lambda: W.grounded_array(1, 1, 21, params=(0.3, 12.0, 0.01, 10e6),
+                                            ground_model="refl-coef"),
+    "refl_tee": lambda: W.grounded_tee(ground_model="refl-coef", phi_mode="rho_v"),
+    "refl_tee_blend": lambda: W.grounded_tee(eps_r=25.0, sigma=2e-2, freq_hz=7e6,
+                                             ground_model="refl-coef", phi_mode="blend"),
+    "refl_array4x4": lambda: W.grounded_array(4, 4, 21, seed=4, ground_model="refl-coef"),
+}
+
+
+@pytest.mark.parametrize("case", ["refl_dipole", "refl_tee", "refl_tee_blend"])
+def test_refl_Z_matches_reference(case):
+    """Reflection-coefficient ground: the Fresnel weights formed in the fill
+    kernel, the whole grounded Z against the reference's."""
+    _say("refl-coef Z", case)
+    d = golden(case)
+    Z = _device_Z(BSplineSolver(**REFL[case]()))
+    assert rel_err(Z, d["Z"]) < 1e-10
+
+
+@pytest.mark.parametrize("case", sorted(REFL))
+def test_refl_port_solution_matches_reference(case):
+    _say("refl-coef port solution", case)
+    d = golden(case)
+    for precision in ("fp64", "mixed"):
+        sol = BSplineSolver(**REFL[case](), precision=precision).compute_port_solution()
+        _say(f"  {precision}: Y err", rel_err(sol.y, d["y"]))
+        assert rel_err(sol.y, d["y"]) < (1e-10 if precision == "fp64" else 1e-8)
+        if precision == "fp64":
+            assert rel_err(sol.coeffs, d["coeffs"]) < 1e-10
+
+
+def test_refl_pec_limit_matches_pec_image():
+    """As eps grows the Fresnel-weighted image tends to the PEC image (the
+    same geometry with no ground_eps)."""
+    _say("refl-coef PEC limit")
+    kw = REFL["refl_tee"]()
+    kw["ground_eps"] = complex(1e16, -1e15)
+    Zr = _device_Z(BSplineSolver(**kw))
+    kw.pop("ground_eps")
+    kw.pop("ground_model")
+    kw.pop("ground_phi_mode")
+    Zp = _device_Z(BSplineSolver(**kw))
+    assert rel_err(Zr, Zp) < 1e-7
+
+
 def test_grounded_graph_replay_matches_eager():
     _say("grounded graph replay")
     s = BSplineSolver(**W.grounded_array(2, 2, 21, seed=6))

@@ -24,8 +24,10 @@ def test_ground_options_validation():
         BSplineSolver(**_kw(ground_model="bogus"))
     with pytest.raises(ValueError):
         BSplineSolver(**_kw(ground_eps=None))  # sommerfeld needs eps
-    with pytest.raises(NotImplementedError):
-        BSplineSolver(**_kw(ground_model="refl-coef"))
+    with pytest.raises(ValueError):
+        BSplineSolver(**_kw(ground_model="refl-coef", ground_phi_mode="bogus"))
+    s = BSplineSolver(**_kw(ground_model="refl-coef"))
+    assert s.ground_phi_mode == "normal"
     with pytest.raises(NotImplementedError):
         BSplineSolver(**_kw(n_qp_sommerfeld=5))
     lam = _kw()["wavelength"]
@@ -97,3 +99,67 @@ def test_oracle_engine_matches_reference_compiled_engine(ref_accel):
                          for r, h in zip(g["rho"], g["h"])])
         err = np.abs(mine - ref).max(1) / np.abs(ref).max(1)
         assert err.max() < 1e-8
+
+
+# -- reflection-coefficient ground (reference _ground_refl.py) --------------
+REFL = {
+    "refl_dipole": lambda: W.grounded_array(1, 1, 21, params=(0.3, 12.0, 0.01, 10e6),
+                                            ground_model="refl-coef"),
+    "refl_tee": lambda: W.grounded_tee(ground_model="refl-coef", phi_mode="rho_v"),
+    "refl_tee_blend": lambda: W.grounded_tee(eps_r=25.0, sigma=2e-2, freq_hz=7e6,
+                                             ground_model="refl-coef", phi_mode="blend"),
+}
+
+
+@pytest.mark.parametrize("case", sorted(REFL))
+def test_refl_oracle_matches_reference(case):
+    """The oracle's Fresnel weight tables, Z and Y against the reference's own
+    (tests/golden/refl_*.npz, written by the reference's solver)."""
+    from oracle import refl_oracle as ro
+    from oracle.bspline_oracle import EPS0, OracleSolver
+    d = golden(case)
+    kw = REFL[case]()
+    o = OracleSolver(**kw)
+    geom = o.geometry()
+    eps_t = ro.eps_tilde(kw["ground_eps"], o.omega, EPS0)
+    assert eps_t == complex(d["eps_t"])
+    cen = 0.5 * (geom["seg_l"] + geom["seg_r"])
+    w_A, w_Phi = ro.weights(cen, geom["tangents"], kw["ground_z"], eps_t, o.ground_phi_mode)
+    assert np.abs(w_A - d["w_A"]).max() < 1e-15
+    assert np.abs(np.broadcast_to(w_Phi, w_A.shape) - d["w_Phi"]).max() < 1e-15
+    Z = o.Z(geom)
+    assert np.abs(Z - d["Z"]).max() < 1e-10 * np.abs(d["Z"]).max()
+    y, coeffs = o.compute_port_solution()
+    assert np.abs(y - d["y"]).max() < 1e-10 * np.abs(d["y"]).max()
+    assert np.abs(coeffs - d["coeffs"]).max() < 1e-10 * np.abs(d["coeffs"]).max()
+
+
+def test_refl_weights_closed_forms():
+    """What the mathematics fixes: at normal incidence rho_v = -rho_h =
+    (sqrt(eps) - 1)/(sqrt(eps) + 1); at grazing incidence rho_v = rho_h = -1;
+    as eps grows every weight tends to the PEC image's (w_A = t . M t,
+    w_Phi = 1 for every phi mode); each phi mode's (c0, c1) against its
+    definition; and the host's coefficients against the oracle's."""
+    from oracle import refl_oracle as ro
+    eps = 13.0 - 7.5j
+    rv, rh = ro.fresnel_rho(eps, np.array([1.0, 0.0]))
+    r = np.sqrt(eps)
+    assert abs(rv[0] - (r - 1) / (r + 1)) < 1e-15 and abs(rh[0] + rv[0]) < 1e-15
+    assert abs(rv[1] + 1) < 1e-15 and abs(rh[1] + 1) < 1e-15
+    rng = np.random.default_rng(3)
+    cen = rng.uniform(0.2, 2.0, (12, 3))
+    tan = rng.standard_normal((12, 3))
+    tan /= np.linalg.norm(tan, axis=1, keepdims=True)
+    td = tan @ (tan * [1.0, 1.0, -1.0]).T
+    for mode in ro.PHI_MODES:
+        w_A, w_Phi = ro.weights(cen, tan, 0.0, 1e14 - 1e13j, mode)
+        assert np.abs(w_A - td).max() < 1e-6
+        assert np.abs(w_Phi - 1).max() < 1e-6
+        c0, c1 = _ground.phi_coefficients(mode, eps)
+        assert (c0, c1) == ro.phi_coeffs(mode, eps)
+    rv, _rh = ro.fresnel_rho(eps, 0.4)
+    c0, c1 = ro.phi_coeffs("blend", eps)
+    assert abs(c0 + c1 * rv - 0.5 * (rv + (eps - 1) / (eps + 1))) < 1e-15
+    # a pair and its transpose see the same weights (the symmetric fill)
+    w_A, w_Phi = ro.weights(cen, tan, 0.0, eps, "rho_v")
+    assert np.abs(w_A - w_A.T).max() < 1e-15 and np.abs(w_Phi - w_Phi.T).max() < 1e-15
