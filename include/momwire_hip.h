@@ -50,7 +50,9 @@ typedef struct mw_mesh {
   int32_t n_se;      /* same-edge table entries: sum over edges of 2 n_e - 1  */
   int32_t degree;    /* B-spline degree (2)                                   */
   int32_t n_qp;      /* Gauss-Legendre points per segment per axis (4)        */
-  int32_t pad0;
+  int32_t ek;        /* 1: extended thin-wire kernel on coaxial equal-radius
+                        pairs (labels below), 0: the reduced kernel
+                        (reference _bspline_kernels.py:28-80, 198-322)     */
   const double* seg;       /* [n_seg][12] l.xyz r.xyz t.xyz length radius 0   */
   const int32_t* seg_edge; /* [n_seg][2]  edge id, index along the edge       */
   const double* edge;      /* [n_edge][2] segment length h, wire radius a     */
@@ -70,6 +72,12 @@ typedef struct mw_mesh {
   const int32_t* tile_nseg;/* [n_tile]                                        */
   const double* gl;        /* [2 n_qp] t, w on [0,1]; then [32] 16-point
                               Gauss-Legendre x, w on [-1,1] (far static rule) */
+  const int32_t* ek_label; /* [n_seg] coaxial labels (ek): a pair with equal
+                              labels takes the extended kernel; NULL if !ek  */
+  const int32_t* ek_img_obs; /* [n_seg] labels of the observers and of the  */
+  const int32_t* ek_img_src; /* mirrored sources in the ground image block,
+                              from one scan over both (NULL if !ek or no
+                              ground)                                        */
 } mw_mesh;
 
 /*

@@ -38,7 +38,7 @@ class MwMesh(ctypes.Structure):
         ("n_se", ctypes.c_int32),
         ("degree", ctypes.c_int32),
         ("n_qp", ctypes.c_int32),
-        ("pad0", ctypes.c_int32),
+        ("ek", ctypes.c_int32),
         ("seg", ctypes.c_void_p),
         ("seg_edge", ctypes.c_void_p),
         ("edge", ctypes.c_void_p),
@@ -52,6 +52,9 @@ class MwMesh(ctypes.Structure):
         ("tile_seg", ctypes.c_void_p),
         ("tile_nseg", ctypes.c_void_p),
         ("gl", ctypes.c_void_p),
+        ("ek_label", ctypes.c_void_p),
+        ("ek_img_obs", ctypes.c_void_p),
+        ("ek_img_src", ctypes.c_void_p),
     ]
 
 

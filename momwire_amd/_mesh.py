@@ -240,6 +240,52 @@ def build_mesh(wires, npe, radius, degree, junctions, n_qp=4):
         per_wire=per_wire, kcl=kcl)
 
 
+def coaxial_labels(seg_l, seg_r, tangents, seg_a, tol=1e-6):
+    """Coaxial-and-equal-radius labels of segments, for the extended
+    thin-wire kernel: a pair is extended iff its labels agree.  A segment
+    joins the first earlier representative with |t . t_rep| >= 1 - tol,
+    |a / a_rep - 1| <= tol and its centre within tol * a_rep of the
+    representative's line (perpendicular to its own axis), else it starts a
+    label (the rule and thresholds of reference
+    _bspline_kernels.py:249-322).  O(N G) for G labels."""
+    seg_l = np.asarray(seg_l, dtype=np.float64)
+    tangents = np.asarray(tangents, dtype=np.float64)
+    seg_a = np.asarray(seg_a, dtype=np.float64)
+    centers = 0.5 * (seg_l + np.asarray(seg_r, dtype=np.float64))
+    labels = np.empty(len(seg_a), dtype=np.int32)
+    reps = []
+    for i in range(len(seg_a)):
+        label = len(reps)
+        for g, r in enumerate(reps):
+            if abs(tangents[r] @ tangents[i]) < 1.0 - tol or abs(seg_a[i] / seg_a[r] - 1.0) > tol:
+                continue
+            d = centers[i] - centers[r]
+            if np.linalg.norm(d - (d @ tangents[i]) * tangents[i]) <= tol * seg_a[r]:
+                label = g
+                break
+        if label == len(reps):
+            reps.append(i)
+        labels[i] = label
+    return labels
+
+
+def ek_tables(mesh, ground_z=None):
+    """(labels, image observer labels, image source labels) of a mesh for the
+    extended kernel; the image pair comes from one scan over the real and the
+    mirrored segments so both sides stay comparable (None without ground)."""
+    seg = mesh.seg
+    sl, sr, t, a = seg[:, 0:3], seg[:, 3:6], seg[:, 6:9], seg[:, 10]
+    lab = coaxial_labels(sl, sr, t, a)
+    if ground_z is None:
+        return lab, None, None
+    mir = np.array([1.0, 1.0, -1.0])
+    off = np.array([0.0, 0.0, 2.0 * ground_z])
+    joint = coaxial_labels(np.vstack([sl, sl * mir + off]), np.vstack([sr, sr * mir + off]),
+                           np.vstack([t, t * mir]), np.concatenate([a, a]))
+    n = len(a)
+    return lab, joint[:n], joint[n:]
+
+
 # ---------------------------------------------------------------------------
 # Sparse port columns and constraint rows
 # ---------------------------------------------------------------------------

@@ -15,9 +15,12 @@ and the Sommerfeld ground (``ground_model="sommerfeld"``: C2-scaled exact
 image plus the interpolated remainder); degree 2; n_qp_pair 4; point
 delta-gap feeds; K-wire junctions (KCL rows), junction ports and node gaps;
 per-wire radii.
+The extended thin-wire kernel (``extended_kernel=True``) on coaxial
+equal-radius pairs, in free space and over every ground (the Sommerfeld
+remainder stays reduced, as in the reference).
 Refused with NotImplementedError (as the reference refuses combinations it
-does not serve): wires touching the ground plane, wire loading, extended
-kernel, singular enrichment, segment-gap and smoothed feeds.
+does not serve): wires touching the ground plane, wire loading, singular
+enrichment, segment-gap and smoothed feeds.
 """
 
 import ctypes
@@ -25,6 +28,7 @@ import ctypes
 import numpy as np
 
 from . import _ground, _lib
+from . import _mesh
 from ._mesh import EPS0, MU0, build_mesh, feed_column, sparse_pack, bspline_values
 from ._port_solution import PortSolution, _SweptPortSolutions
 from ._lib import SolveAborted  # noqa: F401  (re-exported, reference _cancel.py)
@@ -63,7 +67,7 @@ def _torch():
 class _Device:
     """Device copies of the mesh tables plus reusable work buffers."""
 
-    def __init__(self, mesh, cols, rows_con, rows_port):
+    def __init__(self, mesh, cols, rows_con, rows_port, ek=None):
         torch = _torch()
         self.torch = torch
         dev = torch.device("cuda")
@@ -83,17 +87,25 @@ class _Device:
             tile_seg=up(mesh.tile_seg, np.int32), tile_nseg=up(mesh.tile_nseg, np.int32),
             gl=up(mesh.gl, np.float64), pairs=up(mesh.tile_pairs, np.int32),
         )
+        # extended kernel: coaxial labels (free space; image observer, source)
+        ek_ptr = [None, None, None]
+        if ek is not None:
+            for i, lab in enumerate(ek):
+                if lab is not None:
+                    self.keep[f"ek{i}"] = up(lab, np.int32)
+                    ek_ptr[i] = self.keep[f"ek{i}"].data_ptr()
         k = self.keep
         self.mw = _lib.MwMesh(
             n_seg=mesh.n_seg, n_edge=len(mesh.edge), n_basis=mesh.n_basis,
             n_tile=len(mesh.tile_nseg), n_se=len(mesh.se_edge), degree=mesh.degree, n_qp=4,
-            pad0=0,
+            ek=int(ek is not None),
             seg=k["seg"].data_ptr(), seg_edge=k["seg_edge"].data_ptr(), edge=k["edge"].data_ptr(),
             edge_i=k["edge_i"].data_ptr(), se_edge=k["se_edge"].data_ptr(),
             se_tab=k["se_tab"].data_ptr(), band_poly=k["band_poly"].data_ptr(),
             bw_loc=k["bw_loc"].data_ptr(), bw_slot=k["bw_slot"].data_ptr(),
             tile_b0=k["tile_b0"].data_ptr(), tile_seg=k["tile_seg"].data_ptr(),
-            tile_nseg=k["tile_nseg"].data_ptr(), gl=k["gl"].data_ptr())
+            tile_nseg=k["tile_nseg"].data_ptr(), gl=k["gl"].data_ptr(),
+            ek_label=ek_ptr[0], ek_img_obs=ek_ptr[1], ek_img_src=ek_ptr[2])
         nb = mesh.n_basis
         self.nb = nb
         self.n_ports = len(rows_port)
@@ -182,8 +194,8 @@ class BSplineSolver(_SweptPortSolutions):
             raise ValueError(f"feed_model must be 'point' or 'segment', got {feed_model!r}")
         if feed_model != "point" or feed_smoothing_factor is not None:
             raise NotImplementedError("only the point delta-gap feed is served")
-        if extended_kernel or use_singular_enrichment:
-            raise NotImplementedError("extended kernel / singular enrichment are not served")
+        if use_singular_enrichment:
+            raise NotImplementedError("singular enrichment is not served")
         if wire_conductivity is not None or insulation_radius is not None or insulation_eps_r is not None:
             raise NotImplementedError("wire loading is not served")
         if ground_eps is not None and ground_z is None:
@@ -218,6 +230,7 @@ class BSplineSolver(_SweptPortSolutions):
         self.ground_z = None if ground_z is None else float(ground_z)
         self.ground_eps = ground_eps
         self.ground_model = ground_model
+        self.extended_kernel = bool(extended_kernel)
         self.ground_phi_mode = ground_phi_mode
         self.n_qp_sommerfeld = int(n_qp_sommerfeld)
         # a trained momwire_amd.surrogate.SommerfeldSurrogate fills the grid
@@ -395,7 +408,9 @@ class BSplineSolver(_SweptPortSolutions):
     def _device(self):
         if self._dev is None:
             cols, con = self._port_plan()
-            self._dev = _Device(self._build(), cols + con, con, cols)
+            mesh = self._build()
+            ek = _mesh.ek_tables(mesh, self.ground_z) if self.extended_kernel else None
+            self._dev = _Device(mesh, cols + con, con, cols, ek=ek)
         return self._dev.buffers()
 
     # ------------------------------------------------------------------

@@ -104,6 +104,15 @@ int check_mesh(const mw_mesh* M) {
   if (!M->seg || !M->seg_edge || !M->edge || !M->edge_i || !M->se_tab || !M->band_poly ||
       !M->bw_loc || !M->bw_slot || !M->tile_b0 || !M->tile_seg || !M->tile_nseg || !M->gl)
     return fail(MW_EINVAL, "%smesh has a NULL table");
+  if (M->ek && !M->ek_label) return fail(MW_EINVAL, "%sextended kernel without coaxial labels");
+  return MW_OK;
+}
+
+int check_mesh_ground(const mw_mesh* M) {
+  int rc = check_mesh(M);
+  if (rc) return rc;
+  if (M->ek && (!M->ek_img_obs || !M->ek_img_src))
+    return fail(MW_EINVAL, "%sextended kernel without the image block's coaxial labels");
   return MW_OK;
 }
 }  // namespace
@@ -473,7 +482,7 @@ int mw_fill_ground(const mw_mesh* mesh, double k, double omega, double eps, doub
                    void* Z, int64_t ldz, const int32_t* tile_pairs, int32_t n_pairs,
                    int32_t symmetric, void* stream) {
   g_err[0] = 0;
-  int rc = check_mesh(mesh);
+  int rc = check_mesh_ground(mesh);
   if (rc) return rc;
   if (grid && (rc = check_grid(grid))) return rc;
   if (!Z) return fail(MW_EINVAL, "%sZ is NULL");
@@ -493,7 +502,7 @@ int mw_fill_ground_refl(const mw_mesh* mesh, double k, double omega, double eps,
                         int64_t ldz, const int32_t* tile_pairs, int32_t n_pairs,
                         int32_t symmetric, void* stream) {
   g_err[0] = 0;
-  int rc = check_mesh(mesh);
+  int rc = check_mesh_ground(mesh);
   if (rc) return rc;
   if (!Z) return fail(MW_EINVAL, "%sZ is NULL");
   if (ldz < mesh->n_basis) return fail(MW_EINVAL, "%sldz %lld < n_basis %lld", "", ldz, mesh->n_basis);

@@ -43,4 +43,16 @@ template <typename C> MW_DEV C crecip(C b) { C one; one.x = 1; one.y = 0; return
 // |re| + |im|: the pivot magnitude LAPACK's izamax uses.
 template <typename C> MW_DEV double cabs1(C a) { return fabs((double)a.x) + fabs((double)a.y); }
 
+// Extended thin-wire kernel factor of a coaxial equal-radius pair (NEC's
+// Eq 89 with the observer on its own wire's surface: rho = b = a):
+//   1 + T1 C2 - T2 C1,  T1 = a^4/(4R^4),  T2 = a^2/(2R^2),  C1 = 1 + jkR,
+//   C2 = 3 C1 - (kR)^2,
+// multiplying exp(-jkR)/(4 pi R) at the reduced kernel's own R
+// (reference _bspline_kernels.py:198-222).  Exactly 1 at a = 0.
+MW_DEV double2 ek_factor(double R, double a, double k) {
+  const double r2 = R * R, kr = k * R, a2 = a * a;
+  const double t1 = 0.25 * a2 * a2 / (r2 * r2), t2 = 0.5 * a2 / r2;
+  return make_double2(1.0 + t1 * (3.0 - kr * kr) - t2, kr * (3.0 * t1 - t2));
+}
+
 // Status codes (MW_OK ...) come from the public header.

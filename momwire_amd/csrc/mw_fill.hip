@@ -54,26 +54,51 @@ MW_DEV void weight_poly(int p, int q, bool plus, double* c /*[6]*/) {
   }
 }
 
-// int_{x0}^{x1} P(x) / sqrt(x^2 + ah^2) dx for P of degree <= 5, exactly.
-MW_DEV double closed_piece(const double* b, double x0, double x1, double ah) {
-  double F0a = asinh(x0 / ah), F0b = asinh(x1 / ah);
-  double F1a = sqrt(x0 * x0 + ah * ah), F1b = sqrt(x1 * x1 + ah * ah);
+// int_{x0}^{x1} P(x) K(x) dx for P of degree <= 5, exactly, with
+// K = 1 / R, R = sqrt(x^2 + ah^2), or under the extended kernel (ek)
+// K = 1/R - ah^2/(2R^3) + 3ah^4/(4R^5) = 1/R + ah^2 (ah^2 - 2x^2)/(4R^5).
+// The correction's x^n moments have the antiderivative
+// (ah^2/4) (x^{n+1}/R^3 - n G_n), G_n = int x^n/R^3 (G_0 = x/(ah^2 R),
+// G_1 = -1/R, G_n = F_{n-2} - ah^2 G_{n-2}): one integrand, so its two
+// O(1) halves, which cancel to O(ah^2), never meet as separate terms.
+MW_DEV double closed_piece(const double* b, double x0, double x1, double ah, bool ek) {
+  const double a2 = ah * ah;
   double Fa[6], Fb[6];
-  Fa[0] = F0a; Fb[0] = F0b; Fa[1] = F1a; Fb[1] = F1b;
+  Fa[0] = asinh(x0 / ah); Fb[0] = asinh(x1 / ah);
+  const double Ra = sqrt(x0 * x0 + a2), Rb = sqrt(x1 * x1 + a2);
+  Fa[1] = Ra; Fb[1] = Rb;
   double xa = 1.0, xb = 1.0;  // x^{n-1}
   for (int n = 2; n < 6; ++n) {
     xa *= x0; xb *= x1;
-    Fa[n] = (xa * F1a - (n - 1) * ah * ah * Fa[n - 2]) / n;
-    Fb[n] = (xb * F1b - (n - 1) * ah * ah * Fb[n - 2]) / n;
+    Fa[n] = (xa * Ra - (n - 1) * a2 * Fa[n - 2]) / n;
+    Fb[n] = (xb * Rb - (n - 1) * a2 * Fb[n - 2]) / n;
   }
   double s = 0.0;
   for (int n = 0; n < 6; ++n) s += b[n] * (Fb[n] - Fa[n]);
+  if (ek) {
+    double Ga[6], Gb[6];
+    Ga[0] = x0 / (a2 * Ra); Gb[0] = x1 / (a2 * Rb);
+    Ga[1] = -1.0 / Ra; Gb[1] = -1.0 / Rb;
+    for (int n = 2; n < 6; ++n) {
+      Ga[n] = Fa[n - 2] - a2 * Ga[n - 2];
+      Gb[n] = Fb[n - 2] - a2 * Gb[n - 2];
+    }
+    const double ia3 = 1.0 / (Ra * Ra * Ra), ib3 = 1.0 / (Rb * Rb * Rb);
+    double pa = x0, pb = x1;  // x^{n+1}
+    double e = 0.0;
+    for (int n = 0; n < 6; ++n) {
+      e += b[n] * ((pb * ib3 - n * Gb[n]) - (pa * ia3 - n * Ga[n]));
+      pa *= x0; pb *= x1;
+    }
+    s += 0.25 * a2 * e;
+  }
   return s;
 }
 
-// Unit-length static moment I(p,q,delta; ah): source offset delta segments.
+// Unit-length static moment I(p,q,delta; ah): source offset delta segments
+// (ek: with the extended kernel's static correction).
 MW_DEV double static_unit(int p, int q, int delta, double ah, const double* glx,
-                          const double* glw) {
+                          const double* glw, bool ek) {
   double total = 0.0;
   for (int piece = 0; piece < 2; ++piece) {
     const bool plus = piece == 1;
@@ -93,7 +118,7 @@ MW_DEV double static_unit(int p, int q, int delta, double ah, const double* glx,
         }
         b[m] = s;
       }
-      total += closed_piece(b, e0 - delta, e1 - delta, ah);
+      total += closed_piece(b, e0 - delta, e1 - delta, ah, ek);
     } else {
       // The kernel's poles are >= one segment away: 16-point Gauss-Legendre
       // is exact to rounding and avoids the far-pair cancellation of the
@@ -105,7 +130,10 @@ MW_DEV double static_unit(int p, int q, int delta, double ah, const double* glx,
         double w = c[5];
         for (int n = 4; n >= 0; --n) w = w * eta + c[n];
         const double x = eta - delta;
-        s += glw[g] * w / sqrt(x * x + ah * ah);
+        const double r2 = x * x + ah * ah, ir = 1.0 / sqrt(r2);
+        double kern = ir;
+        if (ek) kern += ah * ah * (ah * ah - 2.0 * x * x) * 0.25 * ir * ir * ir * ir * ir;
+        s += glw[g] * w * kern;
       }
       total += half * s;
     }
@@ -131,7 +159,7 @@ MW_DEV void same_edge_body(int e, const mw_mesh& M, double k) {
     for (int q = 0; q < NM; ++q) {
       double hp = h;
       for (int t = 0; t < p + q; ++t) hp *= h;
-      jr[p][q] = hp * static_unit(p, q, delta, ah, glx, glw) * INV4PI;
+      jr[p][q] = hp * static_unit(p, q, delta, ah, glx, glw, M.ek != 0) * INV4PI;
       ji[p][q] = 0.0;
     }
   // smooth remainder (exp(-jkR)-1)/(4 pi R) on the arc distance
@@ -149,8 +177,20 @@ MW_DEV void same_edge_body(int e, const mw_mesh& M, double k) {
       sincos(k * R, &sn, &cs);
       sincos(0.5 * k * R, &sh, &chh);
       const double inv = INV4PI / R;
-      const double gr = -2.0 * sh * sh * inv;  // cos(kR) - 1
-      const double gi = -sn * inv;
+      double gr = -2.0 * sh * sh * inv;  // cos(kR) - 1
+      double gi = -sn * inv;
+      if (M.ek) {
+        // extended kernel: [(exp(-jkR) - 1) fac + extra] / (4 pi R),
+        // extra = fac - fac_static = T1 (3jkR - (kR)^2) - T2 jkR, term by term
+        // (reference _bspline_kernels.py:225-246, 464-477)
+        const double2 f = ek_factor(R, a, k);
+        const double r2 = R * R, kr = k * R;
+        const double t1 = 0.25 * a * a * a * a / (r2 * r2), t2 = 0.5 * a * a / r2;
+        const double er = -t1 * kr * kr, ei = kr * (3.0 * t1 - t2);
+        const double g0 = gr, g1 = gi;
+        gr = g0 * f.x - g1 * f.y + er * inv;
+        gi = g0 * f.y + g1 * f.x + ei * inv;
+      }
       for (int p = 0; p < NM; ++p)
         for (int P = 0; P < NM; ++P) {
           const double w = wq[p] * wr[P];
@@ -179,6 +219,7 @@ struct SegStage {
   double a;            // observer-row radius
   double poly[NM][NM]; // band polynomials [slot][power]
   int edge, eidx;
+  int ekl;             // coaxial label (extended kernel), -1 when off
 };
 
 MW_DEV void stage_segment(const mw_mesh& M, int s, SegStage& S) {
@@ -196,6 +237,7 @@ MW_DEV void stage_segment(const mw_mesh& M, int s, SegStage& S) {
   for (int i = 0; i < 9; ++i) S.poly[i / 3][i % 3] = bp[i];
   S.edge = M.seg_edge[2 * s];
   S.eidx = M.seg_edge[2 * s + 1];
+  S.ekl = M.ek ? M.ek_label[s] : -1;
 }
 
 template <bool SYM>
@@ -242,6 +284,7 @@ MW_DEV void fill_body(int ti, int tj, const mw_mesh& M, double k, double wmu, do
       }
     } else if (live) {
       const double a2 = I.a * I.a;
+      const bool ext = I.ekl >= 0 && I.ekl == J.ekl;  // extended kernel on this pair
 #pragma unroll
       for (int qq = 0; qq < 2; ++qq) {
         const int q = 2 * half + qq;
@@ -254,7 +297,13 @@ MW_DEV void fill_body(int ti, int tj, const mw_mesh& M, double k, double wmu, do
           double sn, cn;
           sincos(k * R, &sn, &cn);
           const double inv = INV4PI / R;
-          const double gr = cn * inv, gi = -sn * inv;  // exp(-jkR)/(4 pi R)
+          double gr = cn * inv, gi = -sn * inv;  // exp(-jkR)/(4 pi R)
+          if (ext) {
+            const double2 f = ek_factor(R, I.a, k);
+            const double g0 = gr;
+            gr = g0 * f.x - gi * f.y;
+            gi = g0 * f.y + gi * f.x;
+          }
           const double w0 = J.wu[0][r], w1 = J.wu[1][r], w2 = J.wu[2][r];
           const double ar[3] = {w0 * gr, w1 * gr, w2 * gr};
           const double ai[3] = {w0 * gi, w1 * gi, w2 * gi};

@@ -639,6 +639,7 @@ struct GStage {
   double a;
   double poly[NM][NM];
   SrcDir dir;            // horizontal/vertical split of the (unmirrored) tangent
+  int ekl;               // coaxial label in the image block (extended kernel), -1 off
 };
 
 struct GroundArgs {
@@ -673,6 +674,7 @@ MW_DEV void stage_ground(const mw_mesh& M, int s, bool source, const GroundArgs&
   const double* bp = M.band_poly + (size_t)s * 9;
   for (int i = 0; i < 9; ++i) S.poly[i / 3][i % 3] = bp[i];
   S.dir = src_dir(g[6], g[7], g[8]);
+  S.ekl = M.ek ? (source ? M.ek_img_src[s] : M.ek_img_obs[s]) : -1;
 }
 
 // Fresnel weights of one segment pair for the reflection-coefficient ground
@@ -735,6 +737,9 @@ MW_DEV void ground_body(int ti, int tj, const mw_mesh& M, const GroundArgs& A,
         for (int P = 0; P < NM; ++P) jr[p][P] = jm[p][P] = 0.0;
       if (live) {
         const double a2 = I.a * I.a;
+        // extended kernel on image pairs coaxial with the observer (a vertical
+        // wire and its own image); the remainder below stays reduced
+        const bool ext = I.ekl >= 0 && I.ekl == J.ekl;
 #pragma unroll
         for (int qq = 0; qq < 2; ++qq) {
           const int q = 2 * half + qq;
@@ -746,7 +751,13 @@ MW_DEV void ground_body(int ti, int tj, const mw_mesh& M, const GroundArgs& A,
             double sn, cn;
             sincos(A.k * R, &sn, &cn);
             const double inv = INV4PI / R;
-            const double gr = cn * inv, gi = -sn * inv;
+            double gr = cn * inv, gi = -sn * inv;
+            if (ext) {
+              const double2 f = ek_factor(R, I.a, A.k);
+              const double g0 = gr;
+              gr = g0 * f.x - gi * f.y;
+              gi = g0 * f.y + gi * f.x;
+            }
 #pragma unroll
             for (int p = 0; p < NM; ++p)
 #pragma unroll

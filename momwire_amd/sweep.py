@@ -20,7 +20,7 @@ import time
 
 import numpy as np
 
-from . import _lib
+from . import _lib, _mesh
 from ._lib import MwPorts, MwSweepProb
 from .bspline import BSplineSolver, _Device, _torch
 
@@ -47,7 +47,8 @@ class BatchedSweep:
         for i, s in enumerate(self.solvers):
             s._build()
             cols, con = s._port_plan()
-            D = _Device(s._mesh, cols + con, con, cols)
+            ek = _mesh.ek_tables(s._mesh) if s.extended_kernel else None
+            D = _Device(s._mesh, cols + con, con, cols, ek=ek)
             self.devs.append(D)
             meshes[i] = D.mw
             ports[i] = MwPorts(

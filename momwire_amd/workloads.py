@@ -262,3 +262,47 @@ def sweep_structures(n=128, seed=3):
 
 def sweep_wavenumbers(n_freq=16, f0_hz=300e6, lo=0.8, hi=1.2):
     return 2 * np.pi * np.linspace(lo, hi, n_freq) * f0_hz / C_LIGHT
+
+
+def ek_structure(name):
+    """Structures for the extended thin-wire kernel (``extended_kernel=True``,
+    reference bspline.py:393-450): fat wires (radius a few 1e-3 lambda) so
+    the kernel's O(a^2) factor shows.
+
+    dipole         one straight wire (every pair coaxial)
+    collinear      two collinear wires meeting at a junction (coaxial
+                   pairs across wires)
+    tee            the junction tee (a bend and a K = 3 junction: mostly
+                   non-coaxial pairs)
+    pec_vertical   a vertical wire and a horizontal one over PEC ground (the
+                   vertical wire's image is coaxial with it)
+    refl_dipole    a horizontal dipole over the reflection-coefficient ground
+    somm_tee       the tee over Sommerfeld ground (remainder kept reduced)
+    """
+    if name == "dipole":
+        return dict(wires=[[(0.0, 0.0, -0.24), (0.0, 0.0, 0.24)]], n_per_edge_per_wire=[[21]],
+                    degree=2, wire_radius=4e-3, wavelength=1.0, feeds=[(0, None, 1.0)],
+                    extended_kernel=True)
+    if name == "collinear":
+        return dict(wires=[[(0.0, 0.0, -0.25), (0.0, 0.0, 0.0)], [(0.0, 0.0, 0.0), (0.0, 0.0, 0.25)]],
+                    n_per_edge_per_wire=[[11], [11]], degree=2, wire_radius=3e-3, wavelength=1.0,
+                    junctions=[[(0, "end"), (1, "start")]], feeds=[(0, 0.2, 1.0)],
+                    extended_kernel=True)
+    if name == "tee":
+        kw = junction_tee("feed")
+        kw.update(wire_radius=4e-3, extended_kernel=True)
+        return kw
+    if name == "pec_vertical":
+        return dict(wires=[[(0.0, 0.0, 0.02), (0.0, 0.0, 0.27)], [(0.1, 0.0, 0.3), (0.4, 0.0, 0.3)]],
+                    n_per_edge_per_wire=[[15], [13]], degree=2, wire_radius=4e-3, wavelength=1.0,
+                    feeds=[(0, None, 1.0), (1, None, 1.0)], ground_z=0.0, extended_kernel=True)
+    if name == "refl_dipole":
+        kw = grounded_array(1, 1, 21, params=(0.3, 12.0, 0.01, 10e6), radius=4e-3,
+                            ground_model="refl-coef")
+        kw["extended_kernel"] = True
+        return kw
+    if name == "somm_tee":
+        kw = grounded_tee()
+        kw.update(wire_radius=4e-3 * kw["wavelength"], extended_kernel=True)
+        return kw
+    raise ValueError(f"unknown extended-kernel structure {name!r}")

@@ -3,9 +3,10 @@
 Test infrastructure only (see oracle/__init__.py).  Each function names the
 reference lines it follows; paths are relative to the reference checkout.
 
-Scope: free space, reduced thin-wire kernel (extended_kernel=False), point
-delta-gap feeds, junctions with KCL rows, junction ports and node gaps,
-degree 1 or 2 -- the configuration space of the port-solution path.
+Scope: free space and the PEC, reflection-coefficient and Sommerfeld grounds,
+the reduced or the extended thin-wire kernel, point delta-gap feeds,
+junctions with KCL rows, junction ports and node gaps, degree 1 or 2 -- the
+configuration space of the port-solution path.
 """
 
 from math import comb  # noqa: F401  (kept for readers cross-checking weights)
@@ -38,8 +39,10 @@ class OracleSolver:
                  feed_wire_index=0, feed_arclength=None, junctions=None,
                  junction_ports=None, node_gaps=None, n_qp_pair=4, wavelength=22,
                  wire_radius=0.0005, nsegs=101, ground_z=None, ground_eps=None,
-                 ground_model="refl-coef", ground_phi_mode="normal", n_qp_sommerfeld=3):
+                 ground_model="refl-coef", ground_phi_mode="normal", n_qp_sommerfeld=3,
+                 extended_kernel=False):
         self.degree = int(degree)
+        self.extended_kernel = bool(extended_kernel)
         self.ground_z, self.ground_eps = ground_z, ground_eps
         self.ground_model, self.n_qp_sommerfeld = ground_model, int(n_qp_sommerfeld)
         self.ground_phi_mode = ground_phi_mode
@@ -175,9 +178,11 @@ class OracleSolver:
         return supp, polys, kcl, knots_all, wbg
 
     # -- moments ------------------------------------------------------------
-    def full_moments(self, geom, a_row, k, il=None, jl=None, src=None):
+    def full_moments(self, geom, a_row, k, il=None, jl=None, src=None, ek=None):
         """_bspline_kernels.py:716-747 (numpy body of the off-edge moments);
-        ``src`` = (seg_l, seg_r) of other source segments (the mirrored ones)."""
+        ``src`` = (seg_l, seg_r) of other source segments (the mirrored ones);
+        ``ek`` = (labels_i, labels_j): pairs with equal labels take the
+        extended kernel's factor (_bspline_kernels.py:599-747)."""
         d = self.degree
         t01, w01 = gl01(self.n_qp_pair)
         sl, sr = geom["seg_l"], geom["seg_r"]
@@ -193,6 +198,11 @@ class OracleSolver:
         diff = pi[:, :, None, None, :] - pj[None, None, :, :, :]
         R = np.sqrt((diff * diff).sum(-1) + (a * a)[:, None, None, None])
         G = np.exp(-1j * k * R) / (4 * np.pi * R)
+        if ek is not None:
+            from . import ek_oracle
+            gi, gj = np.asarray(ek[0])[il], np.asarray(ek[1])[jl]
+            mask = (gi[:, None] == gj[None, :])[:, None, :, None]
+            G = np.where(mask, G * ek_oracle.factor(R, a[:, None, None, None], k), G)
         ui, uj = t01[None, :] * Li[:, None], t01[None, :] * Lj[:, None]
         wi, wj = w01[None, :] * Li[:, None], w01[None, :] * Lj[:, None]
         wui = np.stack([wi * ui ** p for p in range(d + 1)])
@@ -200,7 +210,8 @@ class OracleSolver:
         return np.einsum("piq,iqjr,Pjr->pPij", wui, G, wuj)
 
     def reg_moments(self, ed_arc, a, k):
-        """_bspline_kernels.py:425-517 (smooth remainder on one edge)."""
+        """_bspline_kernels.py:425-517 (smooth remainder on one edge; the
+        extended kernel's remainder under extended_kernel)."""
         d = self.degree
         t01, w01 = gl01(self.n_qp_pair)
         sl = ed_arc[:-1]
@@ -210,7 +221,11 @@ class OracleSolver:
         u = t01[None, :] * h[:, None]
         w = w01[None, :] * h[:, None]
         R = np.sqrt((s[:, None] - s[None, :]) ** 2 + a * a)
-        G = ((np.exp(-1j * k * R) - 1.0) / (4 * np.pi * R)).reshape(N, -1, N, len(t01))
+        if self.extended_kernel:
+            from . import ek_oracle
+            G = ek_oracle.reg_kernel(R, a, k).reshape(N, -1, N, len(t01))
+        else:
+            G = ((np.exp(-1j * k * R) - 1.0) / (4 * np.pi * R)).reshape(N, -1, N, len(t01))
         wu = np.stack([w * u ** p for p in range(d + 1)])
         return np.einsum("piq,iqjr,Pjr->pPij", wu, G, wu)
 
@@ -218,7 +233,7 @@ class OracleSolver:
         """bspline.py:2222-2283: all pairs, then same-edge overwrite."""
         d = self.degree
         a_row = self.seg_radius(geom)
-        J = self.full_moments(geom, a_row, k)
+        J = self.full_moments(geom, a_row, k, ek=self.ek_labels(geom))
         for w, pw in enumerate(geom["per_wire"]):
             base = geom["seg_off"][w]
             for e in range(len(pw["ed_off"]) - 1):
@@ -226,7 +241,8 @@ class OracleSolver:
                 arc = pw["ed_arc"][e]
                 n_e = len(arc) - 1
                 h = float(arc[1] - arc[0])
-                A_st = static_moments_uniform(h, float(self.radius[w]), n_e, d)
+                A_st = static_moments_uniform(h, float(self.radius[w]), n_e, d,
+                                              ek=self.extended_kernel)
                 A_reg = self.reg_moments(arc, float(self.radius[w]), k)
                 J[:, :, sl, sl] = A_st + A_reg
         return J
@@ -275,7 +291,8 @@ class OracleSolver:
         img_l, img_r = geom["seg_l"] * mir, geom["seg_r"] * mir
         img_l[:, 2] += 2 * gz
         img_r[:, 2] += 2 * gz
-        J_img = self.full_moments(geom, self.seg_radius(geom), self.k, src=(img_l, img_r))
+        J_img = self.full_moments(geom, self.seg_radius(geom), self.k, src=(img_l, img_r),
+                                  ek=self.ek_labels(geom, mirror=True))
         td_img = geom["tangents"] @ (geom["tangents"] * mir).T
         if self.ground_eps is None:
             return self.assemble(J_img, supp, polys, geom, td=td_img)
@@ -292,6 +309,28 @@ class OracleSolver:
             out = out + so.remainder_block(geom, supp, polys, gz, self.k, grid,
                                            self.n_qp_sommerfeld)
         return out
+
+    def ek_labels(self, geom, mirror=False):
+        """bspline.py:2040-2104: coaxial labels (observer, source), or None
+        for the reduced kernel.  For the image block one scan over the real
+        and the mirrored segments keeps both sides' labels comparable."""
+        if not self.extended_kernel:
+            return None
+        from . import ek_oracle
+        a = self.seg_radius(geom)
+        if not mirror:
+            g = ek_oracle.axis_groups(geom["seg_l"], geom["seg_r"], geom["tangents"], a)
+            return g, g
+        mir = np.array([1.0, 1.0, -1.0])
+        img_l, img_r = geom["seg_l"] * mir, geom["seg_r"] * mir
+        img_l[:, 2] += 2 * self.ground_z
+        img_r[:, 2] += 2 * self.ground_z
+        n = len(a)
+        g = ek_oracle.axis_groups(np.vstack([geom["seg_l"], img_l]),
+                                  np.vstack([geom["seg_r"], img_r]),
+                                  np.vstack([geom["tangents"], geom["tangents"] * mir]),
+                                  np.concatenate([a, a]))
+        return g[:n], g[n:]
 
     def ground_grid(self, geom, eps_t):
         from . import sommerfeld_oracle as so

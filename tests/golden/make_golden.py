@@ -236,7 +236,39 @@ def case_refl_array4x4():
                        extra=False)
 
 
+def case_ek(name):
+    """Extended thin-wire kernel (reference bspline.py:393-450,
+    _bspline_kernels.py:28-80, 198-322): Y, coeffs, Z and the coaxial labels."""
+    from momwire import _bspline_kernels as K
+    kw = workloads.ek_structure(name)
+    s, sol, dt = _solve(kw)
+    geom = s._build_geometry()
+    supp, polys, _kcl, _wk, _wb = s._build_basis_polynomials(geom)
+    out = dict(y=sol.y, coeffs=sol.coeffs, k=s.k, omega=s.omega,
+               Z=np.asarray(s._compute_Z_operator(geom, supp, polys)))
+    gi, _gj = s._ek_axis_labels(geom, False)
+    out["labels"] = np.asarray(gi)
+    if s.ground_z is not None:
+        gi, gj = s._ek_axis_labels(geom, True)
+        out["labels_img_obs"] = np.asarray(gi)
+        out["labels_img_src"] = np.asarray(gj)
+    # EK same-edge static moments of the first edge (h, a) for the closed forms
+    ed_arc = geom["per_wire"][0]["edge_arc_edges"][0]
+    a0 = float(s._radius_per_wire[0])
+    out["static_ek"] = np.asarray(K._seg_seg_static_moments(ed_arc, a0, max_d=2,
+                                                            ek=K._EK(a=None, group_i=None,
+                                                                     group_j=None)))
+    out["static"] = np.asarray(K._seg_seg_static_moments(ed_arc, a0, max_d=2))
+    return out, dt
+
+
 CASES = {
+    "ek_dipole": lambda: case_ek("dipole"),
+    "ek_collinear": lambda: case_ek("collinear"),
+    "ek_tee": lambda: case_ek("tee"),
+    "ek_pec_vertical": lambda: case_ek("pec_vertical"),
+    "ek_refl_dipole": lambda: case_ek("refl_dipole"),
+    "ek_somm_tee": lambda: case_ek("somm_tee"),
     "refl_dipole": case_refl_dipole,
     "refl_tee": case_refl_tee,
     "refl_tee_blend": case_refl_tee_blend,
