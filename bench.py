@@ -47,6 +47,11 @@ WORKLOADS = {
                            "ground: grid cache cleared), Sommerfeld grid from the learned "
                            "surrogate with exact-engine fallback, FP64 fill and LU",
                       args=None, golden="c5_ground4x4"),
+    "readout8x8": dict(desc="readout of all 64 port columns of the 8x8 array (the array8x8 "
+                            "structure, 2,624 elements): element moments and the far-field "
+                            "moments M_theta, M_phi on a 1-degree grid (theta 0-180, phi "
+                            "0-359: 65,160 directions), free space, FP64",
+                       args=(8, 8, 41, 1), golden=None),
     "sweep128": dict(desc="128 seeded random polyline structures (1-12 wires with shared "
                           "junction nodes, 43-827 unknowns) x 16 frequencies 0.8-1.2 x 300 MHz, "
                           "1-4 feeds, FP64; all fills and factorisations in one HIP graph",
@@ -279,6 +284,118 @@ def ground_main(args, wl):
     print(json.dumps(out))
 
 
+def far_flops(n_dir, n_el, p, ground=False):
+    """Algorithmic flops of the far-field moments: one complex multiply-add
+    (8 flops) per direction, element and product column (3 components x p
+    port columns), twice over a ground (direct and image sums)."""
+    return 8.0 * n_dir * n_el * 3 * p * (2 if ground else 1)
+
+
+def readout_main(args, wl):
+    """Every port column of the array8x8 solve read out as a far-field
+    pattern: element moments (mw_element_moments) then the far-field moments
+    (mw_far_moments) with the solution resident in HBM."""
+    import torch
+
+    from momwire_amd import readout, workloads
+    from momwire_amd.bspline import BSplineSolver
+    from oracle import readout_oracle as ro
+
+    kw = workloads.dipole_array(*wl["args"][:3], seed=wl["args"][3])
+    s = BSplineSolver(**kw)
+    _Y, X = s._solve_device()
+    torch.cuda.synchronize()
+    rm = s.readout_mesh(1)
+    mid = rm.device()["mid"]
+    theta = np.radians(np.arange(0.0, 181.0))
+    phi = np.radians(np.arange(0.0, 360.0))
+    n_dir, p = len(theta) * len(phi), int(X.shape[1])
+    print(f"[bench] {args.workload}: {rm.n_el} elements, {p} columns, {n_dir} directions; "
+          f"warmup {args.warmup}", file=sys.stderr, flush=True)
+
+    def step():
+        mom, _delta = readout.element_moments(rm, X)
+        return mom, readout.far_moments(mid, mom, s.k, theta, phi)
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    # the far-field kernel alone, HIP events on the launch stream
+    mom, _d = readout.element_moments(rm, X)
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+    ev[0].record()
+    for _ in range(args.steps):
+        mt, mp = readout.far_moments(mid, mom, s.k, theta, phi)
+    ev[1].record()
+    torch.cuda.synchronize()
+    far_ms = ev[0].elapsed_time(ev[1]) / args.steps
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        _m, (mt, mp) = step()
+    torch.cuda.synchronize()
+    ms = 1e3 * (time.perf_counter() - t0) / args.steps
+    # parity of a sample of directions and columns against the oracle
+    cols = [0, 27, 63]
+    rows = slice(0, 181, 30)
+    mom_h = mom.cpu().numpy()
+    mt_h, mp_h = mt.cpu().numpy(), mp.cpu().numpy()
+    errs = []
+    for j in cols:
+        rt, rp = ro.far_moments(rm.mid, mom_h[..., j], s.k, theta[rows], phi)
+        sc = max(np.abs(rt).max(), np.abs(rp).max())
+        errs.append(max(np.abs(mt_h[rows, :, j] - rt).max(), np.abs(mp_h[rows, :, j] - rp).max())
+                    / sc)
+    flops = far_flops(n_dir, rm.n_el, p)
+    tf = flops / (far_ms * 1e-3) / 1e12
+    out = {
+        "metric": "far-field pattern readout: port columns/s",
+        "value": p / (ms * 1e-3),
+        "unit": "port patterns/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": wl["desc"], "elements": rm.n_el, "columns": p,
+                   "directions": n_dir},
+        "far_kernel_ms": far_ms,
+        "pattern_rel_err_vs_oracle": float(max(errs)),
+        "roofline": {"kernel": "k_far_moments", "bound": "mfma", "achieved": tf,
+                     "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s", "frac": tf / PEAK_FP64_TFLOPS,
+                     "traffic": None,
+                     "algorithmic": f"8 flop x {n_dir} directions x {rm.n_el} elements x 3 x "
+                                    f"{p} columns = {flops:.3e} flop per launch"},
+    }
+    if not args.no_cpu:
+        # the reference's numpy far field (restated in oracle/readout_oracle.py), one
+        # column over a band of theta rows, scaled to the whole grid and all columns
+        print("[bench] CPU baseline leg", file=sys.stderr, flush=True)
+        band = slice(0, 10)
+        t0 = time.perf_counter()
+        reps = 0
+        while True:
+            ro.far_moments(rm.mid, mom_h[..., reps % p], s.k, theta[band], phi)
+            reps += 1
+            if time.perf_counter() - t0 > 15.0:
+                break
+        per_band = (time.perf_counter() - t0) / reps
+        per_pattern = per_band * len(theta) / 10
+        cores = len(os.sched_getaffinity(0))
+        if os.environ.get("OMP_NUM_THREADS"):
+            cores = min(cores, int(os.environ["OMP_NUM_THREADS"]))
+        out["cpu_baseline"] = {"value": 1.0 / per_pattern, "unit": "port patterns/s",
+                               "cores": cores, "kind": "port",
+                               "sample": f"{reps} single-column far fields over 10 theta rows "
+                                         f"x 360 phi (numpy: the reference's _far_moments "
+                                         f"restated; exp on one thread, the phase-moment "
+                                         f"product on threaded BLAS), scaled to 181 rows"}
+    print(json.dumps(out))
+
+
 def fill_flops(n_seg, n_b):
     """Algorithmic flops of the free-space fill: 16 quadrature points x 53 flops
     per segment pair plus 726 flops per Z entry (the reference's moment and
@@ -380,6 +497,8 @@ def main():
         return sweep_main(args, wl)
     if args.workload == "ground4x4":
         return ground_main(args, wl)
+    if args.workload == "readout8x8":
+        return readout_main(args, wl)
     if wl["golden"] is None:
         return mixed_main(args, wl)
 

@@ -365,6 +365,62 @@ int mw_somm_surfaces_varied(const double* eps, double k2, double omega, double m
                             void* out, const volatile int32_t* cancel, void* stream);
 
 /* ------------------------------------------------------------------------
+ * Readout of solved port columns: every column at once.
+ *
+ * Complex buffers are interleaved double2, row-major.  A structure's
+ * readout mesh has n_el straight elements (knot to knot along each wire)
+ * and n_node knots (n_el + n_wires); its tables are built once on the host.
+ * ------------------------------------------------------------------------ */
+
+/*
+ * Element current moments m_e = 0.5 (I_a + I_b) dl_e (mom [n_el][3][p],
+ * component stride p, row stride ldm >= 3p) and continuity steps
+ * delta_n = el_in - el_out (delta [n_node][p], ldd >= p; el = 0 past a wire
+ * end) of the solution columns X [n_b][p] (ldx >= p).  The current at sample
+ * s is sum_t tap_v[s][t] X[tap_i[s][t]] over three taps (tap_i = -1: none,
+ * every other index < n_b); el_tab [n_el][2] holds each element's end
+ * samples, dl [n_el][3] its vector length, node_tab [n_node][2] the element
+ * arriving at and leaving each knot (-1: none).  Replaces the per-column
+ * currents_at_knots + element_currents walk (reference bspline.py:4523-4685,
+ * _element_currents.py:33-120).
+ */
+int mw_element_moments(int32_t n_el, int32_t n_node, int32_t p, const int32_t* tap_i,
+                       const double* tap_v, const int32_t* el_tab, const double* dl,
+                       const int32_t* node_tab, const void* X, int64_t ldx, void* mom,
+                       int64_t ldm, void* delta, int64_t ldd, void* stream);
+
+/*
+ * Far-field current moments on a theta x phi grid for p columns:
+ * M_theta, M_phi [n_theta][n_phi][p] (direction stride ldo >= p) from
+ * element midpoints mid [n_el][3] and moments mom as above, with
+ * trig_theta [n_theta][2] = (sin, cos) of each theta and trig_phi likewise.
+ * ground: 0 free space, 1 PEC image, 2 finite ground through the Fresnel
+ * image multipliers at eps_c = eps_re + j eps_im, the image mirrored about
+ * z = ground_z.  Replaces _far_moments (reference portal/_portal.py:1520-1598,
+ * image terms :1384-1423), one column per call there.
+ */
+int mw_far_moments(int32_t n_el, const double* mid, const void* mom, int64_t ldm, int32_t p,
+                   int32_t n_theta, int32_t n_phi, const double* trig_theta,
+                   const double* trig_phi, double k, int32_t ground, double ground_z,
+                   double eps_re, double eps_im, void* m_theta, void* m_phi, int64_t ldo,
+                   void* stream);
+
+/*
+ * Near fields in mixed-potential form at pts [n_pts][3] for p columns:
+ * out [n_pts][3][p] (point stride ldo >= 3p) is E (magnetic = 0; element
+ * moments plus node charges delta * node_sign, nodes [n_node][3]) or H
+ * (magnetic = 1; moments only), thin-wire distance sqrt(|R|^2 + radius^2),
+ * eta the wave impedance.  accumulate = 1 adds to out (a PEC image is a
+ * second call on mirrored elements with node_sign = -1).  Replaces
+ * _element_fields (reference portal/_portal.py:1601-1651).
+ */
+int mw_near_fields(int32_t n_pts, const double* pts, int32_t n_el, const double* mid,
+                   const void* mom, int64_t ldm, int32_t n_node, const double* nodes,
+                   const void* delta, int64_t ldd, double node_sign, int32_t p, double k,
+                   double radius, double eta, int32_t magnetic, int32_t accumulate, void* out,
+                   int64_t ldo, void* stream);
+
+/* ------------------------------------------------------------------------
  * Learned Sommerfeld surrogate (no counterpart in the reference)
  *
  * A SiLU MLP, 4 hidden layers of 256, BF16 weights, FP32 accumulation on

@@ -125,6 +125,12 @@ SIGNATURES = {
                                       _I64, _P, _I32, _I32, _P]),
     "mw_fill_ground_refl": (ctypes.c_int, [ctypes.POINTER(MwMesh), _D, _D, _D, _D, _D, _D, _D, _D,
                                            _D, _D, _D, _P, _I64, _P, _I32, _I32, _P]),
+    "mw_element_moments": (ctypes.c_int, [_I32, _I32, _I32, _P, _P, _P, _P, _P, _P, _I64, _P,
+                                          _I64, _P, _I64, _P]),
+    "mw_far_moments": (ctypes.c_int, [_I32, _P, _P, _I64, _I32, _I32, _I32, _P, _P, _D, _I32, _D,
+                                      _D, _D, _P, _P, _I64, _P]),
+    "mw_near_fields": (ctypes.c_int, [_I32, _P, _I32, _P, _P, _I64, _I32, _P, _P, _I64, _D, _I32,
+                                      _D, _D, _D, _I32, _I32, _P, _I64, _P]),
     "mw_somm_surfaces_varied": (ctypes.c_int, [_P, _D, _D, _D, _P, _P, _I64, _D, _P, _P, _P]),
     "mw_mlp_param_count": (_I64, []),
     "mw_mlp_w16_count": (_I64, []),

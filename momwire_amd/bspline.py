@@ -701,6 +701,70 @@ class BSplineSolver(_SweptPortSolutions):
             out.append(vals @ c if len(s_eval) else np.zeros(0, dtype=np.complex128))
         return out
 
+    # -- readout of every port column on the GPU ------------------------
+    def readout_mesh(self, subdiv: int = 1):
+        """The readout tables at ``subdiv`` elements per segment (cached)."""
+        from . import readout
+        cache = self.__dict__.setdefault("_readout_meshes", {})
+        if subdiv not in cache:
+            cache[subdiv] = readout.ReadoutMesh(self, subdiv)
+        return cache[subdiv]
+
+    def _device_coeffs(self, coeffs):
+        """Solution columns on the device: the port columns of a fresh solve
+        (None) or the given (n_b, p) array."""
+        import torch
+        if coeffs is None:
+            _Y, W = self._solve()
+            return W
+        if isinstance(coeffs, torch.Tensor):
+            return coeffs.to(device="cuda", dtype=torch.complex128)
+        c = np.asarray(coeffs, dtype=np.complex128)
+        if c.ndim == 1:
+            c = c[:, None]
+        return torch.from_numpy(np.ascontiguousarray(c)).cuda()
+
+    def port_element_moments(self, coeffs=None, subdiv: int = 1):
+        """``(mid, moment (n_el, 3, p), nodes, delta (n_node, p))`` for every
+        column at once on the GPU (``element_currents`` per column,
+        reference _element_currents.py:33-120); moment and delta stay on the
+        device."""
+        from . import readout
+        rm = self.readout_mesh(subdiv)
+        mom, delta = readout.element_moments(rm, self._device_coeffs(coeffs))
+        return rm.mid, mom, rm.nodes, delta
+
+    def port_far_moments(self, theta, phi, coeffs=None, as_numpy=True):
+        """Far-field moments ``(M_theta, M_phi)``, each (n_theta, n_phi, p),
+        of every port column (or of ``coeffs``) over this solver's ground:
+        free space, the PEC image, or the Fresnel image of a finite ground
+        (reference portal/_portal.py:1520-1598)."""
+        from . import readout
+        rm = self.readout_mesh(1)
+        mom, _delta = readout.element_moments(rm, self._device_coeffs(coeffs))
+        gz = 0.0 if self.ground_z is None else self.ground_z
+        mt, mp = readout.far_moments(rm.device()["mid"], mom, self.k, theta, phi,
+                                     readout.solver_ground(self), gz, self.freq)
+        return (mt.cpu().numpy(), mp.cpu().numpy()) if as_numpy else (mt, mp)
+
+    def port_near_fields(self, points, magnetic=False, coeffs=None,
+                         subdiv: int = 8, as_numpy=True):
+        """E (or H) at ``points`` (n, 3) for every port column, (n, 3, p),
+        from the mesh resampled at ``subdiv`` elements per segment; over PEC
+        the image is added (reference portal/_portal.py:3160-3204, which
+        refuses a finite ground: so does this)."""
+        from . import readout
+        if self.ground_z is not None and self.ground_eps is not None:
+            raise NotImplementedError("near fields over a finite ground (the reference's "
+                                      "readout refuses it too)")
+        rm = self.readout_mesh(subdiv)
+        mom, delta = readout.element_moments(rm, self._device_coeffs(coeffs))
+        dev = rm.device()
+        out = readout.element_fields(points, (dev["mid"], mom, dev["nodes"], delta), self.k,
+                                     float(np.min(self._radius_per_wire)), magnetic,
+                                     image_z=self.ground_z)
+        return out.cpu().numpy() if as_numpy else out
+
     def element_currents(self, coeffs, subdiv: int = 1):
         subdiv = int(subdiv)
         if subdiv < 1:

@@ -67,6 +67,21 @@ hipError_t mw_launch_somm_surfaces_varied(const double* eps, double k2, double o
                                           const double* R1, const double* th, int64_t n, double rtol,
                                           double2* out, const volatile int32_t* cancel,
                                           hipStream_t st);
+hipError_t mw_launch_element_moments(int n_el, int n_node, int p, const int32_t* tap_i,
+                                     const double* tap_v, const int32_t* el_tab, const double* dl,
+                                     const int32_t* node_tab, const double2* X, int64_t ldx,
+                                     double2* mom, int64_t ldm, double2* delta, int64_t ldd,
+                                     hipStream_t st);
+hipError_t mw_launch_far_moments(int n_el, const double* mid, const double2* mom, int64_t ldm,
+                                 int p, int n_theta, int n_phi, const double* trig_t,
+                                 const double* trig_p, double k, int ground, double ground_z,
+                                 double eps_re, double eps_im, double2* m_theta, double2* m_phi,
+                                 int64_t ldo, hipStream_t st);
+hipError_t mw_launch_near_fields(int n_pts, const double* pts, int n_el, const double* mid,
+                                 const double2* mom, int64_t ldm, int n_node, const double* nodes,
+                                 const double2* delta, int64_t ldd, double node_sign, int p,
+                                 double k, double radius, double eta, int magnetic,
+                                 int accumulate, double2* out, int64_t ldo, hipStream_t st);
 int64_t mw_mlp_n_params();
 int64_t mw_mlp_n_w16();
 int64_t mw_mlp_ws_bytes(int64_t n);
@@ -276,6 +291,68 @@ int mw_zscatter_cols(int64_t n, int64_t ncols, int32_t maxnz, const int32_t* idx
   if (ldx < ncols) return fail(MW_EINVAL, "%sbad leading dimension");
   return hip_status(mw_launch_scatter_cols(n, ncols, maxnz, idx, val, X, ldx, (hipStream_t)stream),
                     "mw_zscatter_cols");
+}
+
+int mw_element_moments(int32_t n_el, int32_t n_node, int32_t p, const int32_t* tap_i,
+                       const double* tap_v, const int32_t* el_tab, const double* dl,
+                       const int32_t* node_tab, const void* X, int64_t ldx, void* mom,
+                       int64_t ldm, void* delta, int64_t ldd, void* stream) {
+  g_err[0] = 0;
+  if (n_el < 0 || n_node < 0 || p < 0) return fail(MW_EINVAL, "%snegative size");
+  if (p == 0 || n_el + (int64_t)n_node == 0) return MW_OK;
+  if (!tap_i || !tap_v || !el_tab || !X) return fail(MW_EINVAL, "%sNULL argument");
+  if (n_el && (!dl || !mom)) return fail(MW_EINVAL, "%sNULL element table or moments");
+  if (n_node && (!node_tab || !delta)) return fail(MW_EINVAL, "%sNULL node table or steps");
+  if (ldx < p || (n_el && ldm < 3 * (int64_t)p) || (n_node && ldd < p))
+    return fail(MW_EINVAL, "%sbad leading dimension");
+  return hip_status(mw_launch_element_moments(n_el, n_node, p, tap_i, tap_v, el_tab, dl, node_tab,
+                                              (const double2*)X, ldx, (double2*)mom, ldm,
+                                              (double2*)delta, ldd, (hipStream_t)stream),
+                    "mw_element_moments");
+}
+
+int mw_far_moments(int32_t n_el, const double* mid, const void* mom, int64_t ldm, int32_t p,
+                   int32_t n_theta, int32_t n_phi, const double* trig_theta,
+                   const double* trig_phi, double k, int32_t ground, double ground_z,
+                   double eps_re, double eps_im, void* m_theta, void* m_phi, int64_t ldo,
+                   void* stream) {
+  g_err[0] = 0;
+  if (n_el < 0 || p < 0 || n_theta < 0 || n_phi < 0) return fail(MW_EINVAL, "%snegative size");
+  if (ground < 0 || ground > 2)
+    return fail(MW_EINVAL, "%sground must be 0 (free space), 1 (PEC) or 2 (finite), got %lld", "",
+                ground);
+  if (!(k > 0.0)) return fail(MW_EINVAL, "%swavenumber must be positive");
+  if (p == 0 || (int64_t)n_theta * n_phi == 0) return MW_OK;
+  if (!trig_theta || !trig_phi || !m_theta || !m_phi) return fail(MW_EINVAL, "%sNULL argument");
+  if (n_el && (!mid || !mom)) return fail(MW_EINVAL, "%sNULL element table or moments");
+  if (ldo < p || (n_el && ldm < 3 * (int64_t)p)) return fail(MW_EINVAL, "%sbad leading dimension");
+  return hip_status(mw_launch_far_moments(n_el, mid, (const double2*)mom, ldm, p, n_theta, n_phi,
+                                          trig_theta, trig_phi, k, ground, ground_z, eps_re,
+                                          eps_im, (double2*)m_theta, (double2*)m_phi, ldo,
+                                          (hipStream_t)stream),
+                    "mw_far_moments");
+}
+
+int mw_near_fields(int32_t n_pts, const double* pts, int32_t n_el, const double* mid,
+                   const void* mom, int64_t ldm, int32_t n_node, const double* nodes,
+                   const void* delta, int64_t ldd, double node_sign, int32_t p, double k,
+                   double radius, double eta, int32_t magnetic, int32_t accumulate, void* out,
+                   int64_t ldo, void* stream) {
+  g_err[0] = 0;
+  if (n_pts < 0 || n_el < 0 || n_node < 0 || p < 0) return fail(MW_EINVAL, "%snegative size");
+  if (!(k > 0.0)) return fail(MW_EINVAL, "%swavenumber must be positive");
+  if (!(radius >= 0.0)) return fail(MW_EINVAL, "%sradius must be >= 0");
+  if (n_pts == 0 || p == 0) return MW_OK;
+  if (!pts || !out) return fail(MW_EINVAL, "%sNULL argument");
+  if (n_el && (!mid || !mom)) return fail(MW_EINVAL, "%sNULL element table or moments");
+  if (!magnetic && n_node && (!nodes || !delta)) return fail(MW_EINVAL, "%sNULL nodes or steps");
+  if (ldo < 3 * (int64_t)p || (n_el && ldm < 3 * (int64_t)p) || (!magnetic && n_node && ldd < p))
+    return fail(MW_EINVAL, "%sbad leading dimension");
+  return hip_status(mw_launch_near_fields(n_pts, pts, n_el, mid, (const double2*)mom, ldm,
+                                          magnetic ? 0 : n_node, nodes, (const double2*)delta, ldd,
+                                          node_sign, p, k, radius, eta, magnetic, accumulate,
+                                          (double2*)out, ldo, (hipStream_t)stream),
+                    "mw_near_fields");
 }
 
 int mw_zsmall_solve(int32_t ns, int32_t nt, void* S, int64_t lds, void* T, int64_t ldt,

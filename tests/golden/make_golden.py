@@ -262,7 +262,74 @@ def case_ek(name):
     return out, dt
 
 
+def case_readout(kind):
+    """Readout of every port column: the reference's element_currents at
+    subdiv 1 and 8 (_element_currents.py:33-120), _far_moments over free
+    space, PEC and a finite ground (portal/_portal.py:1384-1423, 1520-1598)
+    and _element_fields E and H, alone and with the PEC image added the way
+    the portal adds it (portal/_portal.py:1601-1651, 3160-3204)."""
+    from types import SimpleNamespace
+
+    from momwire.portal import _portal as P
+    if kind == "tee":
+        kw = workloads.junction_tee("junction_port")
+        gz = 0.0
+    else:
+        kw = workloads.dipole_array(2, 2, 11, seed=5)
+        gz = -0.3
+    s, sol, dt = _solve(kw)
+    X = np.asarray(sol.coeffs)
+    p = X.shape[1]
+    out = dict(coeffs=X, k=s.k, freq_hz=s.freq, ground_z=gz)
+    theta = np.radians(np.arange(0.0, 181.0, 15.0))
+    phi = np.radians(np.arange(0.0, 360.0, 40.0))
+    out.update(theta=theta, phi=phi)
+    for sub in (1, 8):
+        els = [s.element_currents(X[:, j], subdiv=sub) for j in range(p)]
+        out[f"mid{sub}"] = els[0][0]
+        out[f"nodes{sub}"] = els[0][2]
+        out[f"moment{sub}"] = np.stack([e[1] for e in els], axis=-1)
+        out[f"delta{sub}"] = np.stack([e[3] for e in els], axis=-1)
+    grounds = dict(free=SimpleNamespace(kind="free"), pec=SimpleNamespace(kind="pec"),
+                   finite=SimpleNamespace(kind="refl", eps_r=13.0, sigma=0.005))
+    for name, g in grounds.items():
+        mt, mp = zip(*[P._far_moments(out["mid1"], out["moment1"][..., j], s.k, theta, phi, g, gz,
+                                      s.freq) for j in range(p)])
+        out[f"far_{name}_theta"] = np.stack(mt, axis=-1)
+        out[f"far_{name}_phi"] = np.stack(mp, axis=-1)
+    rng = np.random.default_rng(17)
+    lo = np.min(np.vstack(kw["wires"] if kind == "tee" else
+                          [np.asarray(w) for w in kw["wires"]]), axis=0)
+    hi = np.max(np.vstack([np.asarray(w) for w in kw["wires"]]), axis=0)
+    pts = rng.uniform(lo - 0.3 * s.wavelength, hi + 0.3 * s.wavelength, size=(24, 3))
+    pts[:, 2] = np.maximum(pts[:, 2], gz + 0.02 * s.wavelength)
+    # one point 0.01 wavelength off the first wire's midpoint
+    w0 = np.asarray(kw["wires"][0], dtype=float)
+    pts[0] = 0.5 * (w0[0] + w0[1]) + np.array([0.01, 0.0, 0.0]) * s.wavelength
+    radius = float(np.min(np.atleast_1d(kw["wire_radius"])))
+    out.update(points=pts, radius=radius)
+    mid, nodes = out["mid8"], out["nodes8"]
+    img_mid = mid.copy()
+    img_mid[:, 2] = 2.0 * gz - mid[:, 2]
+    img_nodes = nodes.copy()
+    img_nodes[:, 2] = 2.0 * gz - nodes[:, 2]
+    for magnetic, tag in ((False, "E"), (True, "H")):
+        free, pec = [], []
+        for j in range(p):
+            mom, dlt = out["moment8"][..., j], out["delta8"][..., j]
+            f = P._element_fields(pts, (mid, mom, nodes, dlt), s.k, radius, magnetic)
+            img = P._element_fields(pts, (img_mid, mom * np.array([-1.0, -1.0, 1.0]), img_nodes,
+                                          -dlt), s.k, radius, magnetic)
+            free.append(f)
+            pec.append(f + img)
+        out[f"near_{tag}"] = np.stack(free, axis=-1)
+        out[f"near_{tag}_pec"] = np.stack(pec, axis=-1)
+    return out, dt
+
+
 CASES = {
+    "readout_tee": lambda: case_readout("tee"),
+    "readout_array2x2": lambda: case_readout("array"),
     "ek_dipole": lambda: case_ek("dipole"),
     "ek_collinear": lambda: case_ek("collinear"),
     "ek_tee": lambda: case_ek("tee"),
