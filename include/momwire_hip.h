@@ -364,6 +364,17 @@ int mw_somm_surfaces_varied(const double* eps, double k2, double omega, double m
                             const double* R1, const double* theta, int64_t n, double rtol,
                             void* out, const volatile int32_t* cancel, void* stream);
 
+/*
+ * Z[rows[e]][cols[e]] += vals[e] (complex), e < nnz, for distinct pairs with
+ * every index < n: the distributed wire loading term sum_w Z'_w(omega) S_w
+ * (series impedance per unit length times the basis Gram on wire w), merged
+ * per frequency by the caller.  Replaces BSplineSolver._apply_loading
+ * (reference bspline.py:2732-2753; Gram :2667-2730; Z'_w
+ * _wire_loading.py:88-178).
+ */
+int mw_zadd_sparse(int64_t n, int64_t nnz, const int32_t* rows, const int32_t* cols,
+                   const void* vals, void* Z, int64_t ldz, void* stream);
+
 /* ------------------------------------------------------------------------
  * Readout of solved port columns: every column at once.
  *

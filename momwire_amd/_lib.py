@@ -125,6 +125,7 @@ SIGNATURES = {
                                       _I64, _P, _I32, _I32, _P]),
     "mw_fill_ground_refl": (ctypes.c_int, [ctypes.POINTER(MwMesh), _D, _D, _D, _D, _D, _D, _D, _D,
                                            _D, _D, _D, _P, _I64, _P, _I32, _I32, _P]),
+    "mw_zadd_sparse": (ctypes.c_int, [_I64, _I64, _P, _P, _P, _P, _I64, _P]),
     "mw_element_moments": (ctypes.c_int, [_I32, _I32, _I32, _P, _P, _P, _P, _P, _P, _I64, _P,
                                           _I64, _P, _I64, _P]),
     "mw_far_moments": (ctypes.c_int, [_I32, _P, _P, _I64, _I32, _I32, _I32, _P, _P, _D, _I32, _D,

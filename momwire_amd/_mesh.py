@@ -300,15 +300,50 @@ def sparse_pack(cols):
     return idx, val, maxnz
 
 
-def feed_column(mesh, wire, s_f):
-    """v_m = Phi_m(s_f): the point delta-gap drive (reference bspline.py:2863-2870)."""
+def feed_column(mesh, wire, s_f, feed_model="point", smoothing=None, n_qp_source=16):
+    """The drive (and readout) column of a gap feed at arclength s_f of a wire
+    (reference bspline.py:2806-2942):
+
+    * point delta gap: v_m = Phi_m(s_f);
+    * segment gap (feed_model="segment"): v_m = (1/D) int Phi_m over the mesh
+      cell [s_lo, s_hi] holding s_f (a feed on a knot takes the cell to its
+      right, one at the wire end the last cell), D = s_hi - s_lo;
+    * smoothed (smoothing = alpha): v_m = int Phi_m g_w(s - s_f) ds with the
+      cos^2 bump g_w(x) = (2/w) cos^2(pi x / w) on |x| < w/2, w = alpha h of
+      the cell holding s_f, clipped to the wire.
+
+    The spread forms integrate with n_qp_source-point Gauss-Legendre over the
+    cell or the bump's support."""
     pw = mesh.per_wire[wire]
     d = mesh.degree
-    s_f = pw["arc"][-1] / 2.0 if s_f is None else float(s_f)
-    vals = bspline_values(pw["knots"], d, pw["nb_w"], s_f)
+    arc = np.asarray(pw["arc"], dtype=float)
+    s_f = arc[-1] / 2.0 if s_f is None else float(s_f)
+    if smoothing is None and feed_model == "point":
+        v_full = bspline_values(pw["knots"], d, pw["nb_w"], s_f)
+    else:
+        cell = int(np.searchsorted(arc, s_f, side="right")) - 1
+        cell = max(0, min(cell, len(arc) - 2))
+        xi, wq = np.polynomial.legendre.leggauss(int(n_qp_source))
+        if feed_model == "segment":
+            s_lo, s_hi = float(arc[cell]), float(arc[cell + 1])
+            h = s_hi - s_lo
+            t = 0.5 * (s_hi + s_lo) + 0.5 * h * xi
+            g = (0.5 * h * wq) / h
+        else:
+            width = float(smoothing) * float(arc[cell + 1] - arc[cell])
+            half = width / 2.0
+            s_lo, s_hi = max(0.0, s_f - half), min(float(arc[-1]), s_f + half)
+            if s_lo >= s_hi:
+                raise ValueError("feed_smoothing_factor too large for wire -- bump doesn't fit")
+            t = 0.5 * (s_hi + s_lo) + 0.5 * (s_hi - s_lo) * xi
+            dx = t - s_f
+            bump = np.where(np.abs(dx) < half, (2.0 / width) * np.cos(np.pi * dx / width) ** 2, 0.0)
+            g = bump * (0.5 * (s_hi - s_lo) * wq)
+        vals = np.array([bspline_values(pw["knots"], d, pw["nb_w"], x) for x in t])
+        v_full = vals.T @ g
     idx, val = [], []
     for kidx, (j, *_r) in enumerate(pw["kept"]):
-        if vals[j] != 0.0:
+        if v_full[j] != 0.0:
             idx.append(pw["l2g"][kidx])
-            val.append(vals[j])
+            val.append(float(v_full[j]))
     return idx, val

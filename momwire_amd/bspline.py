@@ -28,7 +28,7 @@ import ctypes
 import numpy as np
 
 from . import _ground, _lib
-from . import _mesh
+from . import _loading, _mesh
 from ._mesh import EPS0, MU0, build_mesh, feed_column, sparse_pack, bspline_values
 from ._port_solution import PortSolution, _SweptPortSolutions
 from ._lib import SolveAborted  # noqa: F401  (re-exported, reference _cancel.py)
@@ -138,10 +138,19 @@ class _Device:
             self.Y = torch.empty((max(self.n_ports, 1), max(self.n_ports, 1)),
                                  dtype=torch.complex128, device=self.dev)
             if self.n_con:
-                self.S = torch.empty((self.n_con, self.n_con), dtype=torch.complex128,
-                                     device=self.dev)
-                self.T = torch.empty((self.n_con, max(self.n_ports, 1)), dtype=torch.complex128,
-                                     device=self.dev)
+                # [S | T] = [A X_con | A W]: the Schur system and its right-hand
+                # sides side by side, so a large one rides through mw_zgesv
+                npt = max(self.n_ports, 1)
+                self.ld_st = self.n_con + npt
+                self.ST = torch.empty((self.n_con, self.ld_st), dtype=torch.complex128,
+                                      device=self.dev)
+                self.S = self.ST[:, :self.n_con]
+                self.T = self.ST[:, self.n_con:]
+                self.kcl_small = self.n_con <= 32 and npt <= 64
+                if not self.kcl_small:
+                    self.ipiv_con = torch.empty(self.n_con, dtype=torch.int32, device=self.dev)
+                    self.work_con = torch.empty(int(lib.mw_getrf_workspace_bytes(self.n_con)),
+                                                dtype=torch.uint8, device=self.dev)
         return self
 
     def buffers_mixed(self):
@@ -192,12 +201,11 @@ class BSplineSolver(_SweptPortSolutions):
             raise NotImplementedError("the GPU fill is built for n_qp_pair=4")
         if feed_model not in ("point", "segment"):
             raise ValueError(f"feed_model must be 'point' or 'segment', got {feed_model!r}")
-        if feed_model != "point" or feed_smoothing_factor is not None:
-            raise NotImplementedError("only the point delta-gap feed is served")
+        if feed_model == "segment" and feed_smoothing_factor is not None:
+            raise ValueError("feed_model='segment' and feed_smoothing_factor are mutually "
+                             "exclusive -- both replace the point drive with a spread one")
         if use_singular_enrichment:
             raise NotImplementedError("singular enrichment is not served")
-        if wire_conductivity is not None or insulation_radius is not None or insulation_eps_r is not None:
-            raise NotImplementedError("wire loading is not served")
         if ground_eps is not None and ground_z is None:
             raise ValueError("ground_eps requires ground_z to be set")
         if ground_model not in ("refl-coef", "sommerfeld"):
@@ -223,6 +231,9 @@ class BSplineSolver(_SweptPortSolutions):
         self.refine_steps = None if refine_steps is None else int(refine_steps)
         self.degree = int(degree)
         self.n_qp_pair = 4
+        self.feed_model = feed_model
+        self.feed_smoothing_factor = feed_smoothing_factor
+        self.n_qp_source = int(n_qp_source)
         self.wavelength = wavelength
         self.halfdriver_factor = halfdriver_factor
         self.wire_radius = wire_radius
@@ -257,6 +268,13 @@ class BSplineSolver(_SweptPortSolutions):
         if not np.all(np.isfinite(r)) or np.any(r <= 0):
             raise ValueError(f"wire_radius entries must be positive and finite, got {r}")
         self._radius_per_wire = r
+        # distributed series loading (conductor loss, insulation): validated here
+        self._loading = _loading.WireLoading(r, wire_conductivity, insulation_radius,
+                                             insulation_eps_r)
+        self.wire_conductivity = self._loading.conductivity
+        self.insulation_radius = self._loading.insulation_radius
+        self.insulation_eps_r = self._loading.insulation_eps_r
+        self._loading_active = self._loading.active
         if self.ground_z is not None:
             zmin = min(float(pl[:, 2].min()) for pl in self.wires_polylines) - self.ground_z
             if zmin < -1e-12:
@@ -387,7 +405,8 @@ class BSplineSolver(_SweptPortSolutions):
         (bspline.py:3784-3867, 1170-1210)."""
         if self._ports is None:
             mesh = self._build()
-            cols = [feed_column(mesh, w, s) for w, s, _v in self.feeds]
+            cols = [feed_column(mesh, w, s, self.feed_model, self.feed_smoothing_factor,
+                                self.n_qp_source) for w, s, _v in self.feeds]
             rows = {j: ([m for m, _s in dirs], [s for _m, s in dirs]) for j, dirs in mesh.kcl}
             port_js = [j for j, _v in self.junction_ports]
             cols += [rows[j] for j in port_js]
@@ -400,8 +419,6 @@ class BSplineSolver(_SweptPortSolutions):
                         m = pw["l2g"][kidx]
                 cols.append(([m], [1.0 if end == "start" else -1.0]))
             con = [rows[j] for j in range(len(self.junctions)) if j not in set(port_js)]
-            if len(con) > 32:
-                raise NotImplementedError("more than 32 KCL constraint rows")
             self._ports = (cols, con)
         return self._ports
 
@@ -465,6 +482,11 @@ class BSplineSolver(_SweptPortSolutions):
         pairs = D.keep["pairs"]
         _lib.check(lib.mw_fill_z(D.mw, k, omega, self.eps, self.mu, zptr, ld, pairs.data_ptr(),
                                  int(pairs.shape[0]), int(mesh.symmetric), st), "fill")
+        if self._loading_active:
+            rows, cols, vals = self._loading_device(D, omega)
+            _lib.check(lib.mw_zadd_sparse(D.nb, int(rows.shape[0]), rows.data_ptr(),
+                                          cols.data_ptr(), vals.data_ptr(), zptr, ld, st),
+                       "wire loading")
         if self.ground_z is None:
             return
         if self.ground_eps is not None and self.ground_model == "refl-coef":
@@ -485,6 +507,31 @@ class BSplineSolver(_SweptPortSolutions):
                                       c2.imag, gref, zptr, ld, pairs.data_ptr(),
                                       int(pairs.shape[0]), int(mesh.symmetric), st),
                    "ground fill")
+
+    def _loading_device(self, D, omega):
+        """Device (rows, cols, merged Z'_w S_w values) of the loading term at
+        omega, uploaded once per frequency (a graph replay reuses them)."""
+        cache = D.__dict__.setdefault("loading", {})
+        if "term" not in cache:
+            term = _loading.LoadingTerm(self._build(), self._loading)
+            torch = D.torch
+            cache["term"] = term
+            cache["rows"] = torch.from_numpy(term.rows).to(D.dev)
+            cache["cols"] = torch.from_numpy(term.cols).to(D.dev)
+        key = float(omega)
+        if cache.get("omega") != key:
+            cache["vals"] = D.torch.from_numpy(cache["term"].entries(key)).to(D.dev)
+            cache["omega"] = key
+        return cache["rows"], cache["cols"], cache["vals"]
+
+    def wire_loss_power(self, coeffs, omega=None):
+        """Ohmic power in the wire metal, 1/2 sum_w Re Z'_w c^H S_w c, with
+        per-wire parts (reference bspline.py:2775-2800)."""
+        n_w = len(self.wires_polylines)
+        if not self._loading_active:
+            return 0.0, np.zeros(n_w)
+        term = _loading.LoadingTerm(self._build(), self._loading)
+        return term.loss_power(coeffs, self.omega if omega is None else omega)
 
     def _ground_grid(self, k, omega):
         """The Sommerfeld grid view at this frequency (None for PEC)."""
@@ -510,15 +557,21 @@ class BSplineSolver(_SweptPortSolutions):
             Xc = xptr + 16 * npt
             _lib.check(lib.mw_zsparse_rows(D.n_con, D.con_max, D.con_idx.data_ptr(),
                                            D.con_val.data_ptr(), Xc, ldx, D.n_con,
-                                           D.S.data_ptr(), D.n_con, st), "KCL A.X")
+                                           D.S.data_ptr(), D.ld_st, st), "KCL A.X")
             _lib.check(lib.mw_zsparse_rows(D.n_con, D.con_max, D.con_idx.data_ptr(),
                                            D.con_val.data_ptr(), xptr, ldx, npt,
-                                           D.T.data_ptr(), D.T.shape[1], st), "KCL A.W")
-            _lib.check(lib.mw_zsmall_solve(D.n_con, npt, D.S.data_ptr(), D.n_con,
-                                           D.T.data_ptr(), D.T.shape[1],
-                                           D.info.data_ptr() + 4, st), "KCL Schur")
+                                           D.T.data_ptr(), D.ld_st, st), "KCL A.W")
+            if D.kcl_small:
+                _lib.check(lib.mw_zsmall_solve(D.n_con, npt, D.S.data_ptr(), D.ld_st,
+                                               D.T.data_ptr(), D.ld_st,
+                                               D.info.data_ptr() + 4, st), "KCL Schur")
+            else:
+                # more than 32 constraint rows or 64 ports: the blocked LU on [S | T]
+                _lib.check(lib.mw_zgesv(D.n_con, npt, D.ST.data_ptr(), D.ld_st,
+                                        D.ipiv_con.data_ptr(), D.work_con.data_ptr(),
+                                        D.info.data_ptr() + 4, None, st), "KCL Schur")
             _lib.check(lib.mw_zgemm_sub(nb, npt, D.n_con, Xc, ldx, D.T.data_ptr(),
-                                        D.T.shape[1], xptr, ldx, st), "KCL update")
+                                        D.ld_st, xptr, ldx, st), "KCL update")
         _lib.check(lib.mw_zsparse_rows(npt, D.port_max, D.port_idx.data_ptr(),
                                        D.port_val.data_ptr(), xptr, ldx, npt,
                                        yptr, max(npt, 1), st), "port readout")

@@ -46,6 +46,8 @@ int mw_sweep_lu_run(const mw_sweep_prob* dP, const mw_ports* dports, const mw_sw
                     int nprob, int max_ports, int max_con, int64_t max_scatter,
                     int gemm_variant, const volatile int32_t* cancel, hipStream_t st,
                     hipError_t* herr);
+hipError_t mw_launch_add_sparse(int64_t nnz, const int32_t* rows, const int32_t* cols,
+                                const void* vals, void* Z, int64_t ldz, hipStream_t st);
 hipError_t mw_launch_small_solve(int ns, int nt, void* S, int64_t lds, void* T, int64_t ldt,
                                  int32_t* info, hipStream_t st);
 
@@ -291,6 +293,17 @@ int mw_zscatter_cols(int64_t n, int64_t ncols, int32_t maxnz, const int32_t* idx
   if (ldx < ncols) return fail(MW_EINVAL, "%sbad leading dimension");
   return hip_status(mw_launch_scatter_cols(n, ncols, maxnz, idx, val, X, ldx, (hipStream_t)stream),
                     "mw_zscatter_cols");
+}
+
+int mw_zadd_sparse(int64_t n, int64_t nnz, const int32_t* rows, const int32_t* cols,
+                   const void* vals, void* Z, int64_t ldz, void* stream) {
+  g_err[0] = 0;
+  if (n < 0 || nnz < 0) return fail(MW_EINVAL, "%snegative size");
+  if (nnz == 0) return MW_OK;
+  if (!rows || !cols || !vals || !Z) return fail(MW_EINVAL, "%sNULL argument");
+  if (ldz < n) return fail(MW_EINVAL, "%sbad leading dimension");
+  return hip_status(mw_launch_add_sparse(nnz, rows, cols, vals, Z, ldz, (hipStream_t)stream),
+                    "mw_zadd_sparse");
 }
 
 int mw_element_moments(int32_t n_el, int32_t n_node, int32_t p, const int32_t* tap_i,

@@ -1744,6 +1744,17 @@ __global__ void k_scatter_cols(int64_t ncols, int maxnz, const int32_t* idx, con
   X[(int64_t)r * ldx + c] = make_double2(val[e], 0.0);
 }
 
+// Z[rows[e], cols[e]] += vals[e] for distinct (row, col) pairs: the wire
+// loading term, merged per frequency on the host.
+__global__ void k_add_sparse(int64_t nnz, const int32_t* rows, const int32_t* cols,
+                             const double2* vals, double2* Z, int64_t ldz) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= nnz) return;
+  double2* z = Z + (int64_t)rows[e] * ldz + cols[e];
+  const double2 v = vals[e];
+  *z = make_double2(z->x + v.x, z->y + v.y);
+}
+
 // Precision changes between the FP64 system and the FP32 factorisation.
 __global__ void k_z2c(int64_t rows, int64_t cols, const double2* src, int64_t lds, float2* dst,
                       int64_t ldd) {
@@ -2381,6 +2392,14 @@ hipError_t mw_launch_scatter_cols(int64_t n, int64_t ncols, int maxnz, const int
   const int64_t ne = ncols * maxnz;
   hipLaunchKernelGGL(k_scatter_cols, dim3((unsigned)((ne + 255) / 256)), dim3(256), 0, st, ncols,
                      maxnz, idx, val, (double2*)X, ldx);
+  return hipGetLastError();
+}
+
+hipError_t mw_launch_add_sparse(int64_t nnz, const int32_t* rows, const int32_t* cols,
+                                const void* vals, void* Z, int64_t ldz, hipStream_t st) {
+  if (nnz <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_add_sparse, dim3((unsigned)((nnz + 255) / 256)), dim3(256), 0, st, nnz,
+                     rows, cols, (const double2*)vals, (double2*)Z, ldz);
   return hipGetLastError();
 }
 

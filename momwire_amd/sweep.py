@@ -38,6 +38,10 @@ class BatchedSweep:
         self.k_array = np.asarray(k_array, dtype=float)
         self.cancel = cancel
         self.solvers = [BSplineSolver(**kw) for kw in structures]
+        for i, s in enumerate(self.solvers):
+            if s._loading_active:
+                raise NotImplementedError(f"structure {i}: wire loading in the batched sweep "
+                                          "(solve it with BSplineSolver one frequency at a time)")
         dev = torch.device("cuda")
         n_mesh = len(self.solvers)
         meshes = (_lib.MwMesh * n_mesh)()
@@ -49,6 +53,11 @@ class BatchedSweep:
             cols, con = s._port_plan()
             ek = _mesh.ek_tables(s._mesh) if s.extended_kernel else None
             D = _Device(s._mesh, cols + con, con, cols, ek=ek)
+            if D.n_ports > 32 or D.n_con > 32 or D.n_ports + D.n_con > 64:
+                raise NotImplementedError(
+                    f"structure {i}: the batched sweep serves up to 32 ports and 32 KCL rows "
+                    f"(64 together); got {D.n_ports} ports, {D.n_con} rows -- solve it with "
+                    "BSplineSolver.port_solutions_swept one frequency at a time")
             self.devs.append(D)
             meshes[i] = D.mw
             ports[i] = MwPorts(

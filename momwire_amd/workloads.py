@@ -85,6 +85,23 @@ def junction_tee(ports="feed"):
     return kw
 
 
+def wire_chain(n_wires=40, n_seg=3, feed_every=None, wavelength=1.0):
+    """A zig-zag of n_wires straight wires joined end to start (n_wires - 1
+    two-wire junctions, each a KCL constraint row): more constraint rows
+    than the 32 the small Schur solve takes.  feed_every=m puts a centre
+    feed on every m-th wire (else one feed on wire 0)."""
+    pts = [(0.0, 0.0, 0.0)]
+    for i in range(n_wires):
+        x, _y, z = pts[-1]
+        pts.append((x + 0.03, 0.012 * (1 if i % 2 == 0 else -1), z + 0.004 * (i % 3)))
+    wires = [[pts[i], pts[i + 1]] for i in range(n_wires)]
+    junctions = [[(i, "end"), (i + 1, "start")] for i in range(n_wires - 1)]
+    feeds = ([(i, None, 1.0) for i in range(0, n_wires, feed_every)] if feed_every
+             else [(0, None, 1.0)])
+    return dict(wires=wires, n_per_edge_per_wire=[[n_seg]] * n_wires, degree=2,
+                wire_radius=5e-4, wavelength=wavelength, junctions=junctions, feeds=feeds)
+
+
 def ground_parameters(seed):
     """(height / lambda, eps_r, sigma S/m, frequency Hz) of one grounded-array
     draw: height U(0.05, 1) lambda, eps_r U(5, 30), sigma log-U(1e-3, 5e-2),

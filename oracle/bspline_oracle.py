@@ -40,9 +40,14 @@ class OracleSolver:
                  junction_ports=None, node_gaps=None, n_qp_pair=4, wavelength=22,
                  wire_radius=0.0005, nsegs=101, ground_z=None, ground_eps=None,
                  ground_model="refl-coef", ground_phi_mode="normal", n_qp_sommerfeld=3,
-                 extended_kernel=False):
+                 extended_kernel=False, feed_model="point", feed_smoothing_factor=None,
+                 n_qp_source=16, wire_conductivity=None, insulation_radius=None,
+                 insulation_eps_r=None):
         self.degree = int(degree)
         self.extended_kernel = bool(extended_kernel)
+        self.feed_model, self.feed_smoothing_factor = feed_model, feed_smoothing_factor
+        self.n_qp_source = int(n_qp_source)
+        self.loading_args = (wire_conductivity, insulation_radius, insulation_eps_r)
         self.ground_z, self.ground_eps = ground_z, ground_eps
         self.ground_model, self.n_qp_sommerfeld = ground_model, int(n_qp_sommerfeld)
         self.ground_phi_mode = ground_phi_mode
@@ -277,7 +282,54 @@ class OracleSolver:
         Z = self.assemble(self.J_blocks(geom, self.k), supp, polys, geom)
         if self.ground_z is not None:
             Z = Z - self.ground_terms(geom, supp, polys, remainder)
+        if any(a is not None for a in self.loading_args):
+            Z = Z + self.loading_term(geom, supp, polys)
         return Z
+
+    # -- wire loading: bspline.py:2667-2753, _wire_loading.py:88-178 ---------
+    def z_wire(self, omega):
+        """Series impedance per unit length of each wire (NaN: off)."""
+        from scipy.special import ive
+        mu0 = 1.25663706127e-6
+        n_w = len(self.wires)
+        per = lambda v: None if v is None else (np.full(n_w, float(v)) if np.ndim(v) == 0  # noqa: E731
+                                                else np.asarray(v, dtype=float))
+        sig, b, er = (per(v) for v in self.loading_args)
+        out = np.zeros(n_w, dtype=complex)
+        for w in range(n_w):
+            a = self.radius[w]
+            if sig is not None and np.isfinite(sig[w]):
+                kc = np.sqrt(1j * omega * mu0 * sig[w])
+                out[w] += kc / (2 * np.pi * a * sig[w]) * ive(0, kc * a) / ive(1, kc * a)
+            if b is not None and np.isfinite(b[w]):
+                out[w] += 1j * omega * mu0 / (2 * np.pi) * (1 - 1 / er[w]) * np.log(b[w] / a)
+        return out
+
+    def loading_gram(self, geom, supp, polys):
+        """(rows, cols, vals, wire) of int Phi_m Phi_n over shared segments."""
+        on_seg = {}
+        nz = np.any(polys != 0.0, axis=2)
+        for m in range(supp.shape[0]):
+            for a in range(supp.shape[1]):
+                if nz[m, a]:
+                    on_seg.setdefault(int(supp[m, a]), []).append((m, a))
+        pq = np.arange(polys.shape[2])
+        ps = pq[:, None] + pq[None, :] + 1
+        out = []
+        for sg, ent in on_seg.items():
+            M = polys[[m for m, _ in ent], [a for _, a in ent]] @ (geom["h"][sg] ** ps / ps) \
+                @ polys[[m for m, _ in ent], [a for _, a in ent]].T
+            w = int(np.searchsorted(geom["seg_off"], sg, side="right") - 1)
+            out += [(mi, mj, M[i, j], w) for i, (mi, _a) in enumerate(ent)
+                    for j, (mj, _b) in enumerate(ent)]
+        r, c, v, w = (np.array(x) for x in zip(*out))
+        return r.astype(int), c.astype(int), v.astype(float), w.astype(int)
+
+    def loading_term(self, geom, supp, polys):
+        r, c, v, w = self.loading_gram(geom, supp, polys)
+        L = np.zeros((supp.shape[0], supp.shape[0]), dtype=complex)
+        np.add.at(L, (r, c), self.z_wire(self.omega)[w] * v)
+        return L
 
     # -- ground: bspline.py:3730-3770, 2552-2661, 1654-1740, 1791-1874 -------
     def ground_terms(self, geom, supp, polys, remainder=True):
@@ -338,6 +390,30 @@ class OracleSolver:
         return so.get_grid(eps_t, self.k, r1, self.omega, MU0)
 
     # -- ports: bspline.py:2806-2870, 3784-3867, 1170-1210 ------------------
+    def source_weights(self, arc, knots, s_f):
+        """bspline.py:2806-2942: the feed's weights on every basis of its wire
+        -- point delta gap, segment gap (mean over the cell holding s_f) or
+        the cos^2 bump of width alpha h, by n_qp_source-point Gauss-Legendre."""
+        d = self.degree
+        if self.feed_smoothing_factor is None and self.feed_model == "point":
+            return BSpline.design_matrix(np.array([s_f]), knots, d).toarray()[0]
+        cell = int(np.searchsorted(arc, s_f, side="right")) - 1
+        xi, wq = np.polynomial.legendre.leggauss(self.n_qp_source)
+        if self.feed_model == "segment":
+            cell = max(0, min(cell, len(arc) - 2))
+            lo, hi = float(arc[cell]), float(arc[cell + 1])
+            t = 0.5 * (hi + lo) + 0.5 * (hi - lo) * xi
+            return BSpline.design_matrix(t, knots, d).toarray().T @ ((0.5 * (hi - lo) * wq) / (hi - lo))
+        cell = max(0, min(cell, len(arc) - 2))
+        w = float(self.feed_smoothing_factor) * float(arc[cell + 1] - arc[cell])
+        lo, hi = max(0.0, s_f - w / 2), min(float(arc[-1]), s_f + w / 2)
+        if lo >= hi:
+            raise ValueError("feed_smoothing_factor too large for wire")
+        t = 0.5 * (hi + lo) + 0.5 * (hi - lo) * xi
+        g = np.where(np.abs(t - s_f) < w / 2, (2.0 / w) * np.cos(np.pi * (t - s_f) / w) ** 2, 0.0)
+        return np.einsum("qj,q,q->j", BSpline.design_matrix(t, knots, d).toarray(), g,
+                         0.5 * (hi - lo) * wq)
+
     def port_columns(self, geom, basis):
         supp, polys, kcl, knots, wbg = basis
         d = self.degree
@@ -346,7 +422,7 @@ class OracleSolver:
         for f, (w, s_f, _v) in enumerate(self.feeds):
             arc = geom["per_wire"][w]["arc"]
             s_f = arc[-1] / 2.0 if s_f is None else s_f
-            DM = BSpline.design_matrix(np.array([s_f]), knots[w], d).toarray()[0]
+            DM = self.source_weights(arc, knots[w], s_f)
             kept, l2g = wbg[w]
             for kidx, (j, *_r) in enumerate(kept):
                 B[l2g[kidx], f] = DM[j]

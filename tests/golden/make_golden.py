@@ -327,7 +327,62 @@ def case_readout(kind):
     return out, dt
 
 
+def case_chain(n_wires, feed_every):
+    """More KCL constraint rows than the small Schur solve takes (and, with a
+    feed on every wire, more ports than 64): reference bspline.py:2971-2989."""
+    s, sol, dt = _solve(workloads.wire_chain(n_wires, 3, feed_every))
+    return dict(y=sol.y, coeffs=sol.coeffs), dt
+
+
+def case_feeds(kind):
+    """Spread gap feeds (reference bspline.py:2806-2942): the segment gap and
+    the cos^2-smoothed source, on the junction tee (three and two feeds,
+    one on the bent arm) and on a dipole."""
+    if kind == "segment":
+        kw = workloads.junction_tee("junction_port")
+        kw["feeds"] = [(0, None, 1.0), (2, 0.1, 1.0), (1, 0.13, 1.0)]
+        kw["feed_model"] = "segment"
+    elif kind == "smooth_tee":
+        kw = workloads.junction_tee("feed")
+        kw["feeds"] = [(0, None, 1.0), (1, 0.21, 1.0)]
+        kw["feed_smoothing_factor"] = 2.5
+    else:
+        kw = workloads.straight_dipole()
+        kw["feed_smoothing_factor"] = 3.0
+    s, sol, dt = _solve(kw)
+    return dict(y=sol.y, coeffs=sol.coeffs), dt
+
+
+def case_loading(kind):
+    """Distributed wire loading (reference _wire_loading.py,
+    bspline.py:2667-2800): conductor loss on a 14 MHz dipole, insulation on
+    the tee, a per-wire mix (NaN switches a wire off) on a 2x2 array; Y,
+    coeffs and the wire loss power of the first column."""
+    if kind == "dipole":
+        kw = workloads.straight_dipole()
+        kw.update(wire_conductivity=5.8e7)
+    elif kind == "tee":
+        kw = workloads.junction_tee("junction_port")
+        kw.update(insulation_radius=2.5e-3, insulation_eps_r=3.2, wire_conductivity=3.5e7)
+    else:
+        kw = workloads.dipole_array(2, 2, 11, seed=5)
+        kw.update(wire_conductivity=[5.8e7, float("nan"), 1e6, 3e7],
+                  insulation_radius=[float("nan"), 2e-3, 3e-3, float("nan")],
+                  insulation_eps_r=[float("nan"), 2.3, 4.0, float("nan")])
+    s, sol, dt = _solve(kw)
+    p_tot, p_wire = s.wire_loss_power(sol.coeffs[:, 0])
+    return dict(y=sol.y, coeffs=sol.coeffs, loss=p_tot, loss_per_wire=p_wire), dt
+
+
 CASES = {
+    "loading_dipole": lambda: case_loading("dipole"),
+    "loading_tee": lambda: case_loading("tee"),
+    "loading_array2x2": lambda: case_loading("array"),
+    "feed_segment": lambda: case_feeds("segment"),
+    "feed_smooth_tee": lambda: case_feeds("smooth_tee"),
+    "feed_smooth_dipole": lambda: case_feeds("smooth_dipole"),
+    "chain40": lambda: case_chain(40, None),
+    "chain70_feeds": lambda: case_chain(70, 1),
     "readout_tee": lambda: case_readout("tee"),
     "readout_array2x2": lambda: case_readout("array"),
     "ek_dipole": lambda: case_ek("dipole"),
