@@ -105,8 +105,13 @@ class HostMesh:
     kcl: list = field(default_factory=list)       # [(junction, [(m, sign)])]
 
 
-def build_mesh(wires, npe, radius, degree, junctions, n_qp=4):
-    """Discretise the wires and build every k-independent table."""
+def build_mesh(wires, npe, radius, degree, junctions, n_qp=4, ground_ends=()):
+    """Discretise the wires and build every k-independent table.
+
+    ground_ends: (wire, "start"|"end") ends lying in an active ground plane
+    with no junction there; each keeps its value-1 boundary basis (kind
+    "gnd", no KCL partner: the image carries the current on through the
+    plane; reference bspline.py:1108-1151, 1296-1316)."""
     d = int(degree)
     n_w = len(wires)
     st = ["free"] * n_w
@@ -117,6 +122,11 @@ def build_mesh(wires, npe, radius, degree, junctions, n_qp=4):
                 st[w] = j
             else:
                 en[w] = j
+    for w, end in ground_ends:
+        if end == "start" and st[w] == "free":
+            st[w] = "ground"
+        elif end == "end" and en[w] == "free":
+            en[w] = "ground"
 
     seg_rows, seg_edge, edge_rows, edge_i, se_edge = [], [], [], [], []
     per_wire = []
@@ -165,10 +175,14 @@ def build_mesh(wires, npe, radius, degree, junctions, n_qp=4):
         nb_w = n_sw + d
         span_p = [span_polynomials(knots, d, s + d, knots[s + d]) for s in range(n_sw)]
         kept = []
-        if st[w] != "free":
+        if st[w] == "ground":
+            kept.append((0, "gnd", None, "start"))
+        elif st[w] != "free":
             kept.append((0, "dir", st[w], "start"))
         kept += [(j, "int", None, None) for j in range(1, nb_w - 1)]
-        if en[w] != "free":
+        if en[w] == "ground":
+            kept.append((nb_w - 1, "gnd", None, "end"))
+        elif en[w] != "free":
             kept.append((nb_w - 1, "dir", en[w], "end"))
         l2g = {}
         for kidx, (j, kind, jidx, endp) in enumerate(kept):

@@ -12,15 +12,17 @@ Supported: free space, PEC ground, the reflection-coefficient finite ground
 (``ground_model="refl-coef"``, the default with ``ground_eps``: the image
 weighted per segment pair by Fresnel coefficients, any ``ground_phi_mode``)
 and the Sommerfeld ground (``ground_model="sommerfeld"``: C2-scaled exact
-image plus the interpolated remainder); degree 2; n_qp_pair 4; point
-delta-gap feeds; K-wire junctions (KCL rows), junction ports and node gaps;
-per-wire radii.
+image plus the interpolated remainder); ground contact (wire ends and
+junctions in the plane, PEC or Sommerfeld; refused under the reflection-
+coefficient ground as the reference refuses it); degree 2; n_qp_pair 4;
+point delta-gap, segment-gap and cos^2-smoothed feeds; K-wire junctions (KCL
+rows, any number), junction ports and node gaps; per-wire radii;
+distributed wire loading (conductor loss, insulation).
 The extended thin-wire kernel (``extended_kernel=True``) on coaxial
 equal-radius pairs, in free space and over every ground (the Sommerfeld
 remainder stays reduced, as in the reference).
-Refused with NotImplementedError (as the reference refuses combinations it
-does not serve): wires touching the ground plane, wire loading, singular
-enrichment, segment-gap and smoothed feeds.
+Refused with NotImplementedError: singular enrichment, degree != 2,
+n_qp_pair != 4.
 """
 
 import ctypes
@@ -275,13 +277,34 @@ class BSplineSolver(_SweptPortSolutions):
         self.insulation_radius = self._loading.insulation_radius
         self.insulation_eps_r = self._loading.insulation_eps_r
         self._loading_active = self._loading.active
+        # ground contact (reference bspline.py:683-698, 1108-1169): a wire END
+        # within 1e-6 of its own length of the plane touches it
+        self._ground_ends = ()
         if self.ground_z is not None:
-            zmin = min(float(pl[:, 2].min()) for pl in self.wires_polylines) - self.ground_z
-            if zmin < -1e-12:
-                raise ValueError("every wire must lie at or above ground_z "
-                                 f"(min height above the plane: {zmin:.3g})")
-            if zmin <= 1e-12:
-                raise NotImplementedError("wires touching the ground plane are not served")
+            gz = self.ground_z
+            ends = []
+            for w, pl in enumerate(self.wires_polylines):
+                tol = 1e-6 * max(float(np.sum(np.linalg.norm(np.diff(pl, axis=0), axis=1))), 1e-30)
+                if float(pl[:, 2].min()) < gz - tol:
+                    raise ValueError(f"wire {w} dips below the ground plane (min z = "
+                                     f"{pl[:, 2].min():.6g} < ground_z = {gz:g})")
+                at = np.abs(pl[:, 2] - gz) <= tol
+                if np.any(at[:-1] & at[1:]):
+                    raise ValueError(f"wire {w} has an edge lying in the ground plane (both "
+                                     "endpoints at ground_z) -- degenerate over a conducting "
+                                     "ground")
+                if at[0]:
+                    ends.append((w, "start"))
+                if at[-1]:
+                    ends.append((w, "end"))
+            if ends and ground_eps is not None and ground_model == "refl-coef":
+                where = ", ".join(f"wire {w} {e}" for w, e in ends)
+                raise NotImplementedError(
+                    f"{where} lies in the ground plane: ground contact under "
+                    "ground_model='refl-coef' is refused (the reflection-coefficient "
+                    "charge weight has no validity at zero clearance); use "
+                    "ground_model='sommerfeld' or a PEC ground")
+            self._ground_ends = tuple(ends)
 
         if n_per_edge_per_wire is None:
             n_per_edge_per_wire = [None] * n_w
@@ -393,9 +416,27 @@ class BSplineSolver(_SweptPortSolutions):
 
     def _build(self):
         if self._mesh is None:
+            jends = {(w, e) for jw in self.junctions for w, e in jw}
             self._mesh = build_mesh(self.wires_polylines, self.n_per_edge_per_wire,
-                                    self._radius_per_wire, self.degree, self.junctions)
+                                    self._radius_per_wire, self.degree, self.junctions,
+                                    ground_ends=[g for g in self._ground_ends if g not in jends])
         return self._mesh
+
+    def _grounded_junctions(self):
+        """Junctions whose shared point lies in the ground plane: their KCL
+        row is dropped (current may leave through the image; reference
+        bspline.py:1152-1169, 1399-1406)."""
+        if self.ground_z is None:
+            return frozenset()
+        out = set()
+        for j, jw in enumerate(self.junctions):
+            w, end = jw[0]
+            pl = self.wires_polylines[w]
+            pt = pl[0] if end == "start" else pl[-1]
+            tol = 1e-6 * max(float(np.sum(np.linalg.norm(np.diff(pl, axis=0), axis=1))), 1e-30)
+            if abs(pt[2] - self.ground_z) <= tol:
+                out.add(j)
+        return frozenset(out)
 
     def _port_count(self):
         return len(self.feeds) + len(self.junction_ports) + len(self.node_gaps)
@@ -409,6 +450,12 @@ class BSplineSolver(_SweptPortSolutions):
                                 self.n_qp_source) for w, s, _v in self.feeds]
             rows = {j: ([m for m, _s in dirs], [s for _m, s in dirs]) for j, dirs in mesh.kcl}
             port_js = [j for j, _v in self.junction_ports]
+            grounded = self._grounded_junctions()
+            for j in port_js:
+                if j in grounded:
+                    raise ValueError(f"junction {j} is both grounded and a junction port -- a "
+                                     "grounded node's voltage is pinned by the ground image, "
+                                     "so it cannot also be a driven port")
             cols += [rows[j] for j in port_js]
             for w, end, _v in self.node_gaps:
                 j = next(jj for jj, jw in enumerate(self.junctions) if (w, end) in jw)
@@ -418,7 +465,8 @@ class BSplineSolver(_SweptPortSolutions):
                     if kind == "dir" and ji == j and endp == end:
                         m = pw["l2g"][kidx]
                 cols.append(([m], [1.0 if end == "start" else -1.0]))
-            con = [rows[j] for j in range(len(self.junctions)) if j not in set(port_js)]
+            con = [rows[j] for j in range(len(self.junctions))
+                   if j not in set(port_js) and j not in grounded]
             self._ports = (cols, con)
         return self._ports
 

@@ -102,6 +102,31 @@ def wire_chain(n_wires=40, n_seg=3, feed_every=None, wavelength=1.0):
                 wire_radius=5e-4, wavelength=wavelength, junctions=junctions, feeds=feeds)
 
 
+def grounded_monopole(kind="pec", base_feed=True):
+    """A quarter-wave vertical standing on the ground plane (ground contact:
+    the base keeps its value-1 basis, the image carries the current on):
+    base-fed (s = 0) or fed at the first segment's centre, over PEC or the
+    Sommerfeld ground (eps_r 13, sigma 0.005 S/m) at 14 MHz; 'radials' adds
+    three sloping radials meeting the vertical at a junction in the plane
+    (a grounded junction: its KCL row is dropped)."""
+    lam = 299792458.0 / 14e6
+    h = 0.25 * lam
+    n = 17
+    kw = dict(wires=[[(0.0, 0.0, 0.0), (0.0, 0.0, h)]], n_per_edge_per_wire=[[n]], degree=2,
+              wire_radius=0.01, wavelength=lam, ground_z=0.0,
+              feeds=[(0, 0.0 if base_feed else 0.5 * h / n, 1.0)])
+    if kind == "sommerfeld":
+        kw.update(ground_eps=(13.0, 0.005), ground_model="sommerfeld")
+    if kind == "radials":
+        kw["wires"] = [[(0.0, 0.0, 0.0), (0.0, 0.0, h)]] + [
+            [(0.0, 0.0, 0.0), (0.2 * lam * np.cos(t), 0.2 * lam * np.sin(t), 0.05 * lam)]
+            for t in (0.3, 2.4, 4.5)]
+        kw["n_per_edge_per_wire"] = [[n], [9], [9], [9]]
+        kw["junctions"] = [[(0, "start"), (1, "start"), (2, "start"), (3, "start")]]
+        kw["feeds"] = [(0, 0.5 * h / n, 1.0)]
+    return kw
+
+
 def ground_parameters(seed):
     """(height / lambda, eps_r, sigma S/m, frequency Hz) of one grounded-array
     draw: height U(0.05, 1) lambda, eps_r U(5, 30), sigma log-U(1e-3, 5e-2),

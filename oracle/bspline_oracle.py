@@ -125,7 +125,13 @@ class OracleSolver:
         return np.repeat(self.radius, np.diff(geom["seg_off"]))
 
     # -- basis: bspline.py:1216-1414 ----------------------------------------
-    def endpoint_status(self):  # bspline.py:1108-1151 (free space)
+    @staticmethod
+    def touch_tol(pl):  # _ground_spec.py:127-153
+        return 1e-6 * max(float(np.sum(np.linalg.norm(np.diff(pl, axis=0), axis=1))), 1e-30)
+
+    def endpoint_status(self):  # bspline.py:1108-1151
+        """'free', a junction index, or 'ground' (an unjunctioned end in
+        the plane)."""
         n_w = len(self.wires)
         st, en = ["free"] * n_w, ["free"] * n_w
         for j, jw in enumerate(self.junctions):
@@ -134,7 +140,26 @@ class OracleSolver:
                     st[w] = j
                 else:
                     en[w] = j
+        if self.ground_z is not None:
+            for w, pl in enumerate(self.wires):
+                at = np.abs(pl[:, 2] - self.ground_z) <= self.touch_tol(pl)
+                if st[w] == "free" and at[0]:
+                    st[w] = "ground"
+                if en[w] == "free" and at[-1]:
+                    en[w] = "ground"
         return st, en
+
+    def grounded_junctions(self):  # bspline.py:1152-1169
+        if self.ground_z is None:
+            return set()
+        out = set()
+        for j, jw in enumerate(self.junctions):
+            w, end = jw[0]
+            pl = self.wires[w]
+            pt = pl[0] if end == "start" else pl[-1]
+            if abs(pt[2] - self.ground_z) <= self.touch_tol(pl):
+                out.add(j)
+        return out
 
     def basis(self, geom):
         d = self.degree
@@ -148,11 +173,15 @@ class OracleSolver:
             knots_all.append(knots)
             nb_w = len(knots) - d - 1
             kept = []
-            if st[w] != "free":
+            if st[w] == "ground":
+                kept.append((0, "gnd", None, "start"))
+            elif st[w] != "free":
                 kept.append((0, "dir", st[w], "start"))
             for j in range(1, nb_w - 1):
                 kept.append((j, "int", None, None))
-            if en[w] != "free":
+            if en[w] == "ground":
+                kept.append((nb_w - 1, "gnd", None, "end"))
+            elif en[w] != "free":
                 kept.append((nb_w - 1, "dir", en[w], "end"))
             n_w = pw["n"]
             unit = np.linspace(0.0, 1.0, d + 1)
@@ -427,7 +456,8 @@ class OracleSolver:
             for kidx, (j, *_r) in enumerate(kept):
                 B[l2g[kidx], f] = DM[j]
         port_rows = [j for j, _v in self.junction_ports]
-        keep = [r for r in range(kcl.shape[0]) if r not in set(port_rows)]
+        grounded = self.grounded_junctions()
+        keep = [r for r in range(kcl.shape[0]) if r not in set(port_rows) and r not in grounded]
         kcl_con, port_A = kcl[keep], kcl[port_rows]
         gap = np.zeros((nb, len(self.node_gaps)))
         end_to_j = {tuple(mm): j for j, jw in enumerate(self.junctions) for mm in jw}

@@ -374,7 +374,26 @@ def case_loading(kind):
     return dict(y=sol.y, coeffs=sol.coeffs, loss=p_tot, loss_per_wire=p_wire), dt
 
 
+def case_contact(kind, base_feed):
+    """Ground contact (reference bspline.py:1108-1169, 1296-1316): a base-
+    or segment-fed quarter-wave vertical over PEC and over the Sommerfeld
+    ground, and a vertical with three radials meeting it at a grounded
+    junction; Y, coeffs and (PEC) Z."""
+    kw = workloads.grounded_monopole(kind, base_feed)
+    s, sol, dt = _solve(kw)
+    out = dict(y=sol.y, coeffs=sol.coeffs)
+    if kind == "pec":
+        geom = s._build_geometry()
+        supp, polys, _kcl, _wk, _wb = s._build_basis_polynomials(geom)
+        out["Z"] = np.asarray(s._compute_Z_operator(geom, supp, polys))
+    return out, dt
+
+
 CASES = {
+    "contact_pec_base": lambda: case_contact("pec", True),
+    "contact_pec_seg": lambda: case_contact("pec", False),
+    "contact_somm_base": lambda: case_contact("sommerfeld", True),
+    "contact_radials": lambda: case_contact("radials", False),
     "loading_dipole": lambda: case_loading("dipole"),
     "loading_tee": lambda: case_loading("tee"),
     "loading_array2x2": lambda: case_loading("array"),

@@ -33,8 +33,8 @@ def test_ground_options_validation():
     lam = _kw()["wavelength"]
     with pytest.raises(ValueError):
         BSplineSolver(**_kw(ground_z=0.5 * lam))  # wire below the plane
-    with pytest.raises(NotImplementedError):
-        BSplineSolver(**_kw(ground_z=0.3 * lam))  # wire on the plane
+    with pytest.raises(ValueError, match="edge lying"):
+        BSplineSolver(**_kw(ground_z=0.3 * lam))  # horizontal wire lying in the plane
     s = BSplineSolver(**_kw())
     assert s.ground_model == "sommerfeld" and s.ground_z == 0.0
 
