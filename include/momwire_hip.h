@@ -48,7 +48,8 @@ typedef struct mw_mesh {
   int32_t n_basis;   /* B-spline unknowns                                     */
   int32_t n_tile;    /* basis tiles partitioning [0, n_basis)                 */
   int32_t n_se;      /* same-edge table entries: sum over edges of 2 n_e - 1  */
-  int32_t degree;    /* B-spline degree (2)                                   */
+  int32_t degree;    /* B-spline degree, 1 or 2 (tables padded to 3 wings,
+                        band slots and powers)                              */
   int32_t n_qp;      /* Gauss-Legendre points per segment per axis (4)        */
   int32_t ek;        /* 1: extended thin-wire kernel on coaxial equal-radius
                         pairs (labels below), 0: the reduced kernel

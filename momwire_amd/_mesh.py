@@ -164,9 +164,15 @@ def build_mesh(wires, npe, radius, degree, junctions, n_qp=4, ground_ends=()):
         per_wire.append(dict(seg_off=seg_off, n=len(h), h=h, arc=arc))
 
     # bases
+    # The device tables hold NM = 3 wings / band slots / powers per basis and
+    # segment; a degree-1 (tent) basis fills two of each, the rest stays empty
+    # (wing -1, zero coefficients).
+    NM = 3
+    if not 1 <= d <= 2:
+        raise ValueError(f"degree {d} is not built (1 or 2)")
     supp, polys, slots = [], [], []
-    band_poly = np.zeros((n_seg, d + 1, d + 1))
-    band_basis = np.full((n_seg, d + 1), -1, dtype=np.int32)
+    band_poly = np.zeros((n_seg, NM, NM))
+    band_basis = np.full((n_seg, NM), -1, dtype=np.int32)
     kcl_dirs = {j: [] for j in range(len(junctions))}
     m = 0
     for w, pw in enumerate(per_wire):
@@ -187,13 +193,13 @@ def build_mesh(wires, npe, radius, degree, junctions, n_qp=4, ground_ends=()):
         l2g = {}
         for kidx, (j, kind, jidx, endp) in enumerate(kept):
             lo, hi = max(0, j - d), min(n_sw, j + 1)
-            s_m = np.full(d + 1, -1, dtype=np.int32)
-            p_m = np.zeros((d + 1, d + 1))
-            sl_m = np.full(d + 1, -1, dtype=np.int32)
+            s_m = np.full(NM, -1, dtype=np.int32)
+            p_m = np.zeros((NM, NM))
+            sl_m = np.full(NM, -1, dtype=np.int32)
             for a, s in enumerate(range(lo, hi)):
                 slot = j - s  # row of basis j among the span's bases s..s+d
                 s_m[a] = pw["seg_off"] + s
-                p_m[a] = span_p[s][slot]
+                p_m[a, :d + 1] = span_p[s][slot]
                 sl_m[a] = slot
                 band_poly[pw["seg_off"] + s, slot] = p_m[a]
                 band_basis[pw["seg_off"] + s, slot] = m
@@ -206,9 +212,9 @@ def build_mesh(wires, npe, radius, degree, junctions, n_qp=4, ground_ends=()):
             m += 1
         pw.update(knots=knots, kept=kept, l2g=l2g, nb_w=nb_w)
     n_b = m
-    supp = np.array(supp, dtype=np.int32).reshape(n_b, d + 1)
-    polys = np.array(polys).reshape(n_b, d + 1, d + 1)
-    slots = np.array(slots, dtype=np.int32).reshape(n_b, d + 1)
+    supp = np.array(supp, dtype=np.int32).reshape(n_b, NM)
+    polys = np.array(polys).reshape(n_b, NM, NM)
+    slots = np.array(slots, dtype=np.int32).reshape(n_b, NM)
 
     # basis tiles: consecutive bases, <= TB per tile, <= TSMAX distinct segments
     tile_b0, tile_segs = [0], []
@@ -225,13 +231,13 @@ def build_mesh(wires, npe, radius, degree, junctions, n_qp=4, ground_ends=()):
     n_t = len(tile_segs)
     tile_seg = np.full((n_t, TSMAX), -1, dtype=np.int32)
     tile_nseg = np.zeros(n_t, dtype=np.int32)
-    bw_loc = np.full((n_b, d + 1), -1, dtype=np.int32)
+    bw_loc = np.full((n_b, NM), -1, dtype=np.int32)
     for t_i, segs in enumerate(tile_segs):
         tile_seg[t_i, :len(segs)] = segs
         tile_nseg[t_i] = len(segs)
         pos = {s: i for i, s in enumerate(segs)}
         for mm in range(tile_b0[t_i], tile_b0[t_i + 1]):
-            for a in range(d + 1):
+            for a in range(NM):
                 if supp[mm, a] >= 0:
                     bw_loc[mm, a] = pos[int(supp[mm, a])]
     ti, tj = np.triu_indices(n_t)

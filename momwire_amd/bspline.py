@@ -14,14 +14,15 @@ weighted per segment pair by Fresnel coefficients, any ``ground_phi_mode``)
 and the Sommerfeld ground (``ground_model="sommerfeld"``: C2-scaled exact
 image plus the interpolated remainder); ground contact (wire ends and
 junctions in the plane, PEC or Sommerfeld; refused under the reflection-
-coefficient ground as the reference refuses it); degree 2; n_qp_pair 4;
+coefficient ground as the reference refuses it); degree 2 and 1 (tents);
+n_qp_pair 4;
 point delta-gap, segment-gap and cos^2-smoothed feeds; K-wire junctions (KCL
 rows, any number), junction ports and node gaps; per-wire radii;
 distributed wire loading (conductor loss, insulation).
 The extended thin-wire kernel (``extended_kernel=True``) on coaxial
 equal-radius pairs, in free space and over every ground (the Sommerfeld
 remainder stays reduced, as in the reference).
-Refused with NotImplementedError: singular enrichment, degree != 2,
+Refused with NotImplementedError: singular enrichment, degree > 2,
 n_qp_pair != 4.
 """
 
@@ -195,8 +196,8 @@ class BSplineSolver(_SweptPortSolutions):
             raise NotImplementedError(f"options not served on the GPU path: {sorted(unsupported)}")
         if degree < 1:
             raise ValueError(f"degree must be >= 1, got {degree}")
-        if degree != 2:
-            raise NotImplementedError("the GPU path is built for degree 2")
+        if degree > 2:
+            raise NotImplementedError("the GPU path is built for degrees 1 and 2")
         if not wires:
             raise ValueError("wires must be non-empty")
         if int(n_qp_pair) != 4:
@@ -773,9 +774,32 @@ class BSplineSolver(_SweptPortSolutions):
     def compute_y_matrix(self):
         return self.compute_port_solution().y
 
-    def compute_impedance(self):
-        """Driving-point impedance of the (single) feed: 1 / Y[0, 0]."""
-        return 1.0 / self.compute_y_matrix()[0, 0]
+    def _port_voltages(self):
+        """Configured port voltages in port order: [gap feeds..., junction
+        ports..., node gaps...] (reference bspline.py:3881-3904)."""
+        return np.array([v for _w, _s, v in self.feeds] + [v for _j, v in self.junction_ports]
+                        + [v for _w, _e, v in self.node_gaps], dtype=np.complex128)
+
+    def compute_impedance(self, same_edge_prep=None):
+        """Driving-point impedance under the configured voltages, and the
+        coefficient vector of that excitation: (V_i / I_i per port -- a
+        scalar for one port --, coeffs), I = Y V and coeffs = W V from the one
+        all-port solve (reference bspline.py:3919-4017)."""
+        sol = self.compute_port_solution()
+        V = self._port_voltages()
+        z = V / (sol.y @ V)
+        return (z[0] if len(V) == 1 else z), sol.coeffs @ V
+
+    def compute_impedance_swept(self, k_array):
+        """Driving-point impedance at each wavenumber (reference
+        bspline.py:4479-4520): an array over k (of per-port arrays when there
+        are several ports)."""
+        with self._k_restored():
+            out = []
+            for kk in np.asarray(k_array, dtype=float):
+                self._set_k(kk)
+                out.append(self.compute_impedance()[0])
+        return np.array(out)
 
     def _port_solutions_swept(self, k_array):
         with self._k_restored():

@@ -114,7 +114,8 @@ int hip_status(hipError_t e, const char* where) {
 
 int check_mesh(const mw_mesh* M) {
   if (!M) return fail(MW_EINVAL, "%smesh is NULL");
-  if (M->degree != 2) return fail(MW_EINVAL, "%sdegree %lld not built (degree 2 only)", "", M->degree);
+  if (M->degree < 1 || M->degree > 2)
+    return fail(MW_EINVAL, "%sdegree %lld not built (1 or 2)", "", M->degree);
   if (M->n_qp != 4) return fail(MW_EINVAL, "%sn_qp_pair %lld not built (4 only)", "", M->n_qp);
   if (M->n_seg <= 0 || M->n_basis <= 0 || M->n_tile <= 0)
     return fail(MW_EINVAL, "%sempty mesh (n_seg=%lld, n_basis=%lld)", "", M->n_seg, M->n_basis);

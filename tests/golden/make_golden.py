@@ -389,7 +389,31 @@ def case_contact(kind, base_feed):
     return out, dt
 
 
+def case_d1(kind):
+    """Degree-1 (tent) bases (reference bspline.py:1216-1414 at d = 1): a
+    dipole at the tent-parity pin (half 5.29 m, 20 segments), the junction
+    tee with a junction port, and a grounded vertical over PEC."""
+    if kind == "dipole":
+        kw = dict(wires=[[(0.0, 0.0, -5.29), (0.0, 0.0, 5.29)]], nsegs=20)
+    elif kind == "tee":
+        kw = workloads.junction_tee("junction_port")
+    else:
+        kw = workloads.grounded_monopole("pec", True)
+    kw["degree"] = 1
+    s, sol, dt = _solve(kw)
+    out = dict(y=sol.y, coeffs=sol.coeffs)
+    if kind == "dipole":
+        geom = s._build_geometry()
+        supp, polys, _kcl, _wk, _wb = s._build_basis_polynomials(geom)
+        out["Z"] = np.asarray(s._compute_Z_operator(geom, supp, polys))
+        out["z_in"] = s.compute_impedance()[0]
+    return out, dt
+
+
 CASES = {
+    "d1_dipole": lambda: case_d1("dipole"),
+    "d1_tee": lambda: case_d1("tee"),
+    "d1_contact": lambda: case_d1("contact"),
     "contact_pec_base": lambda: case_contact("pec", True),
     "contact_pec_seg": lambda: case_contact("pec", False),
     "contact_somm_base": lambda: case_contact("sommerfeld", True),
