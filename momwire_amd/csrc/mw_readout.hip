@@ -83,14 +83,16 @@ __global__ __launch_bounds__(256) void k_element_moments(
 // projections need no exchange beyond the (re, im) lane pair.
 // ---------------------------------------------------------------------------
 constexpr int FD = 64;   // directions per workgroup
-constexpr int FE = 32;   // elements per k chunk (64 real k)
-constexpr int FC = 16;   // port columns per workgroup
+constexpr int FE = 16;   // elements per k chunk (32 real k)
+constexpr int FC = 32;   // port columns per workgroup
 constexpr int FB = 3 * FC;
+constexpr int HT = FC / 8;  // 16-wide real column tiles per component
 
 struct FarLds {
   double A[FD][2 * FE + 1];   // phase tile, (re, im) pairs along the row
   double2 B[FE][FB + 1];      // moments: [element][component * FC + column]
   double r[FD][3];            // r^ of the tile's directions
+  double m[FE][3];            // the chunk's element midpoints (z mirrored for the image)
 };
 
 MW_DEV double2 csqrt_principal(double x, double y) {
@@ -110,14 +112,19 @@ MW_DEV void far_stage(FarLds& L, int e0, bool image, int n_el, int ndir_t, const
                       double k, double gz2, const double2* mom, int64_t ldm, int p, int col0,
                       bool stage_b) {
   const int tid = threadIdx.x;
+  if (tid < FE * 3) {
+    const int e = tid / 3, c = tid % 3, eg = e0 + e;
+    double v = 0.0;
+    if (eg < n_el) v = (image && c == 2) ? gz2 - mid[3 * eg + 2] : mid[3 * eg + c];
+    L.m[e][c] = v;
+  }
+  __syncthreads();
 #pragma unroll 1
   for (int i = tid; i < FD * FE; i += 256) {
     const int r = i / FE, e = i % FE, eg = e0 + e;
     double cs = 0.0, sn = 0.0;
     if (r < ndir_t && eg < n_el) {
-      const double x = mid[3 * eg], y = mid[3 * eg + 1];
-      const double z = image ? gz2 - mid[3 * eg + 2] : mid[3 * eg + 2];
-      const double ph = k * (L.r[r][0] * x + L.r[r][1] * y + L.r[r][2] * z);
+      const double ph = k * (L.r[r][0] * L.m[e][0] + L.r[r][1] * L.m[e][1] + L.r[r][2] * L.m[e][2]);
       sincos(ph, &sn, &cs);
     }
     L.A[r][2 * e] = cs;
@@ -198,7 +205,7 @@ __global__ __launch_bounds__(256, 2) void k_far_moments(
       far_mfma(L, acc);
     }
   };
-  // Lane l holds real column 16 tc + l % 16 of tile tc = 2 comp + h: complex
+  // Lane l holds real column 16 tc + l % 16 of tile tc = HT comp + h: complex
   // column 8 h + (l % 16) / 2 of component comp, the real part on even lanes
   // and the imaginary part on odd ones; rows Mfma::row(l, rr) of the wave's
   // 16 directions.
@@ -219,11 +226,11 @@ __global__ __launch_bounds__(256, 2) void k_far_moments(
   // Image pass first; only this lane's part of the reflected contribution
   //   rho_v theta^ . S P_i,  -rho_h phi^ . S P_i
   // is kept across the direct pass.
-  double img_t[4][2], img_p[4][2];
+  double img_t[4][HT], img_p[4][HT];
 #pragma unroll
   for (int rr = 0; rr < 4; ++rr)
 #pragma unroll
-    for (int h = 0; h < 2; ++h) { img_t[rr][h] = 0.0; img_p[rr][h] = 0.0; }
+    for (int h = 0; h < HT; ++h) { img_t[rr][h] = 0.0; img_p[rr][h] = 0.0; }
   if (ground) {
     pass(true);
 #pragma unroll
@@ -240,11 +247,11 @@ __global__ __launch_bounds__(256, 2) void k_far_moments(
         mrho_h = make_double2(-rh.x, -rh.y);
       }
 #pragma unroll
-      for (int h = 0; h < 2; ++h) {
+      for (int h = 0; h < HT; ++h) {
         double ti = 0.0, pi = 0.0;
 #pragma unroll
         for (int comp = 0; comp < 3; ++comp) {
-          const double vi = (comp == 2 ? 1.0 : -1.0) * acc[2 * comp + h][rr];
+          const double vi = (comp == 2 ? 1.0 : -1.0) * acc[HT * comp + h][rr];
           ti += th[comp] * vi;
           pi += ph[comp] * vi;
         }
@@ -268,13 +275,13 @@ __global__ __launch_bounds__(256, 2) void k_far_moments(
     const int rloc = 16 * wave + Mfma<double>::row(lane, rr);
     const int64_t dg = d0 + rloc;
 #pragma unroll
-    for (int h = 0; h < 2; ++h) {
+    for (int h = 0; h < HT; ++h) {
       const int cc = col0 + 8 * h + ((lane & 15) >> 1);
       double mt = 0.0, mp = 0.0;
 #pragma unroll
       for (int comp = 0; comp < 3; ++comp) {
-        mt += th[comp] * acc[2 * comp + h][rr];
-        mp += ph[comp] * acc[2 * comp + h][rr];
+        mt += th[comp] * acc[HT * comp + h][rr];
+        mp += ph[comp] * acc[HT * comp + h][rr];
       }
       mt += img_t[rr][h];
       mp += img_p[rr][h];
