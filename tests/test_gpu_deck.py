@@ -1,0 +1,33 @@
+"""GPU parity on the deck path: every NEC-2 deck of the reference's fixture
+corpus, built by the reference's deck front end (parse + build_solver,
+constructor keywords recorded in tests/golden/decks/kwargs.json), solved by
+this package's BSplineSolver on the GPU; Y against the reference solver's Y
+for the same deck (tests/golden/decks/y.npz) within 1e-10."""
+
+import json
+import os
+
+import numpy as np
+import pytest
+
+from conftest import GOLDEN, rel_err
+from momwire_amd import _ground
+from momwire_amd import deck as gpu_deck
+
+pytestmark = pytest.mark.gpu
+pytest.importorskip("torch")
+
+RECORD = json.load(open(os.path.join(GOLDEN, "decks", "kwargs.json")))
+Y = np.load(os.path.join(GOLDEN, "decks", "y.npz"))
+
+
+def test_deck_corpus_matches_reference():
+    worst = 0.0
+    for name in sorted(RECORD["kwargs"]):
+        print("[gpu-test] deck", name, flush=True)
+        _ground.clear_cache()
+        y = gpu_deck.solver_from_kwargs(RECORD["kwargs"][name]).compute_port_solution().y
+        err = rel_err(y, Y[name])
+        worst = max(worst, err)
+        assert err < 1e-10, (name, err)
+    print("[gpu-test]   worst Y err over the corpus", worst, flush=True)
