@@ -76,6 +76,23 @@ def bspline_values(knots, d, n_basis, x):
     return vals
 
 
+def bspline_slopes(knots, d, n_basis, x):
+    """Arc-length derivatives of all n_basis B-splines at one point x: the
+    derivative of the span's polynomial (the span chosen as bspline_values
+    chooses it, right-continuous, the last span at the right end)."""
+    t = knots
+    x = float(np.clip(x, t[d], t[n_basis]))
+    span = int(np.searchsorted(t, x, side="right") - 1)
+    span = min(max(span, d), n_basis - 1)
+    P = span_polynomials(t, d, span, t[span])
+    u = x - t[span]
+    powers = np.arange(d + 1)
+    du = np.where(powers > 0, powers * u ** np.maximum(powers - 1, 0), 0.0)
+    vals = np.zeros(n_basis)
+    vals[span - d:span + 1] = P @ du
+    return vals
+
+
 # ---------------------------------------------------------------------------
 # The mesh
 # ---------------------------------------------------------------------------

@@ -32,7 +32,8 @@ import numpy as np
 
 from . import _ground, _lib
 from . import _loading, _mesh
-from ._mesh import EPS0, MU0, build_mesh, feed_column, sparse_pack, bspline_values
+from ._mesh import (EPS0, MU0, build_mesh, bspline_slopes, bspline_values, feed_column,
+                    sparse_pack)
 from ._port_solution import PortSolution, _SweptPortSolutions
 from ._lib import SolveAborted  # noqa: F401  (re-exported, reference _cancel.py)
 
@@ -54,8 +55,18 @@ class CancelToken:
     def cancelled(self):
         return bool(self._flag[0])
 
+    def raise_if_cancelled(self):
+        """Raise SolveAborted once cancel() has been called."""
+        if self._flag[0]:
+            raise _lib.SolveAborted()
+
     @property
     def address(self):
+        return self._flag.ctypes.data
+
+    @property
+    def ptr(self):
+        """Raw address of the flag's int32 (the kernels poll it)."""
         return self._flag.ctypes.data
 
 
@@ -889,6 +900,25 @@ class BSplineSolver(_SweptPortSolutions):
                                      float(np.min(self._radius_per_wire)), magnetic,
                                      image_z=self.ground_z)
         return out.cpu().numpy() if as_numpy else out
+
+    def current_slopes(self, coeffs, s_array=None):
+        """Per-wire dI/ds at the mesh knots or at s_array per wire: the
+        B-spline sum's exact arc-length derivative (reference
+        bspline.py:4523-4584)."""
+        coeffs = np.asarray(coeffs)
+        mesh = self._build()
+        d = self.degree
+        out = []
+        for w, pw in enumerate(mesh.per_wire):
+            kv = pw["knots"]
+            s_eval = pw["arc"] if s_array is None else np.asarray(s_array[w], dtype=float)
+            s_eval = np.clip(s_eval, kv[0], kv[-1])
+            c = np.zeros(pw["nb_w"], dtype=np.complex128)
+            for kidx, (j, *_r) in enumerate(pw["kept"]):
+                c[j] = coeffs[pw["l2g"][kidx]]
+            vals = np.array([bspline_slopes(kv, d, pw["nb_w"], x) for x in s_eval])
+            out.append(vals @ c if len(s_eval) else np.zeros(0, dtype=np.complex128))
+        return out
 
     def element_currents(self, coeffs, subdiv: int = 1):
         subdiv = int(subdiv)

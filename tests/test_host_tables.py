@@ -122,3 +122,33 @@ def test_c_abi_exports_every_declared_symbol():
     lib2 = _lib.load()
     assert lib2.mw_abi_version() == 1
     assert lib2.mw_getrf_workspace_bytes(2624) > 0
+
+
+@pytest.mark.parametrize("degree", [1, 2])
+def test_current_slopes_match_spline_derivative(degree):
+    """current_slopes against scipy's exact B-spline derivative (the
+    reference's route, bspline.py:4523-4584) at the knots, and against a
+    central difference of currents_at_knots at element centres."""
+    from scipy.interpolate import BSpline
+
+    from momwire_amd.bspline import BSplineSolver
+    kw = W.junction_tee("junction_port")
+    kw["degree"] = degree
+    s = BSplineSolver(**kw)
+    m = s._build()
+    c = np.random.default_rng(degree).standard_normal(m.n_basis) * (1 + 0.5j)
+    slopes = s.current_slopes(c)
+    for w, pw in enumerate(m.per_wire):
+        cb = np.zeros(pw["nb_w"], dtype=complex)
+        for kidx, (j, *_r) in enumerate(pw["kept"]):
+            cb[j] = c[pw["l2g"][kidx]]
+        ref = BSpline(pw["knots"], cb, degree, extrapolate=False).derivative(1)(
+            np.clip(pw["arc"], pw["knots"][0], pw["knots"][-1]))
+        assert np.abs(slopes[w] - ref).max() <= 1e-13 * np.abs(ref).max()
+        mids = 0.5 * (pw["arc"][1:] + pw["arc"][:-1])
+        eps = 1e-6 * pw["arc"][-1]
+        hi = s.currents_at_knots(c, [mids + eps if i == w else [] for i in range(len(m.per_wire))])
+        lo = s.currents_at_knots(c, [mids - eps if i == w else [] for i in range(len(m.per_wire))])
+        fd = (hi[w] - lo[w]) / (2 * eps)
+        mid_sl = s.current_slopes(c, [mids if i == w else [] for i in range(len(m.per_wire))])[w]
+        assert np.abs(fd - mid_sl).max() <= 1e-6 * np.abs(mid_sl).max()
