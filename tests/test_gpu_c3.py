@@ -1,0 +1,61 @@
+"""The 32x32 array at its full size (15,360 unknowns, 1,024 ports): no
+oracle finishes it in seconds, so the checks are the properties the
+formulation fixes at any size --
+
+* reciprocity: Galerkin testing with the drive columns makes Y complex
+  symmetric, Y = Y^T;
+* the FP64 solve's backward error: R = B - Z X against a second, independent
+  fill of Z, relative to |Z| |X|;
+* the mixed form (FP32 LU on MFMA, FP64 refinement) against FP64.
+"""
+
+import numpy as np
+import pytest
+
+from conftest import rel_err
+from momwire_amd import _lib, workloads as W
+from momwire_amd.bspline import BSplineSolver
+
+pytestmark = pytest.mark.gpu
+torch = pytest.importorskip("torch")
+
+
+def _say(*a):
+    print("[gpu-test]", *a, flush=True)
+
+
+def test_c3_full_size_properties():
+    kw = W.dipole_array(32, 32, 15, seed=2)
+    s = BSplineSolver(**kw)
+    _say("C3: fill + LU + 1,024-port solve (FP64)")
+    Y, X = s._solve_device()
+    torch.cuda.synchronize()
+    s._check_info()
+    D = s._dev
+    nb, npt = D.nb, D.n_ports
+    assert (nb, npt) == (15360, 1024)
+    y = Y.cpu().numpy()
+    _say("  reciprocity |Y - Y^T| / |Y|", rel_err(y, y.T))
+    assert rel_err(y, y.T) < 1e-10
+    # backward error against a fresh fill
+    lib = _lib.load()
+    st = torch.cuda.current_stream().cuda_stream
+    Z2 = torch.empty((nb, nb), dtype=torch.complex128, device="cuda")
+    _lib.check(lib.mw_same_edge_table(D.mw, s.k, st), "same-edge")
+    s._fill(D, s.k, s.omega, Z2.data_ptr(), nb, st)
+    R = torch.empty((nb, npt), dtype=torch.complex128, device="cuda")
+    _lib.check(lib.mw_zscatter_cols(nb, npt, D.col_max, D.col_idx.data_ptr(),
+                                    D.col_val.data_ptr(), R.data_ptr(), npt, st), "B")
+    _lib.check(lib.mw_zgemm_sub(nb, npt, nb, Z2.data_ptr(), nb, X.data_ptr(), X.stride(0),
+                                R.data_ptr(), npt, st), "B - Z X")
+    torch.cuda.synchronize()
+    back = float(torch.linalg.matrix_norm(R, ord=1) / (torch.linalg.matrix_norm(Z2, ord=1)
+                                                       * torch.linalg.matrix_norm(X, ord=1)))
+    _say("  backward error |B - Z X|_1 / (|Z|_1 |X|_1)", back)
+    assert back < 1e-14
+    del Z2, R
+    torch.cuda.empty_cache()
+    _say("C3: mixed precision")
+    ym = BSplineSolver(**kw, precision="mixed").compute_port_solution().y
+    _say("  mixed vs FP64", rel_err(ym, y))
+    assert rel_err(ym, y) < 1e-8
