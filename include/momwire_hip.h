@@ -50,7 +50,8 @@ typedef struct mw_mesh {
   int32_t n_se;      /* same-edge table entries: sum over edges of 2 n_e - 1  */
   int32_t degree;    /* B-spline degree, 1 or 2 (tables padded to 3 wings,
                         band slots and powers)                              */
-  int32_t n_qp;      /* Gauss-Legendre points per segment per axis (4)        */
+  int32_t n_qp;      /* Gauss-Legendre points per segment per axis, 1 .. 8
+                        (the batched sweep: 4)                              */
   int32_t ek;        /* 1: extended thin-wire kernel on coaxial equal-radius
                         pairs (labels below), 0: the reduced kernel
                         (reference _bspline_kernels.py:28-80, 198-322)     */

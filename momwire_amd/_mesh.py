@@ -118,6 +118,7 @@ class HostMesh:
     tile_pairs: np.ndarray   # (P, 2) int32, ti <= tj
     symmetric: bool
     gl: np.ndarray           # (2*n_qp + 32,)
+    n_qp: int                # Gauss-Legendre points per segment per axis
     per_wire: list = field(default_factory=list)  # arc, knots, kept, l2g, seg_off
     kcl: list = field(default_factory=list)       # [(junction, [(m, sign)])]
 
@@ -273,7 +274,7 @@ def build_mesh(wires, npe, radius, degree, junctions, n_qp=4, ground_ends=()):
         band_poly=band_poly, band_basis=band_basis, supp=supp, polys=polys,
         bw_slot=slots, bw_loc=bw_loc, tile_b0=np.array(tile_b0, dtype=np.int32),
         tile_seg=tile_seg, tile_nseg=tile_nseg, tile_pairs=tile_pairs,
-        symmetric=bool(np.all(np.asarray(radius) == radius[0])), gl=gl,
+        symmetric=bool(np.all(np.asarray(radius) == radius[0])), gl=gl, n_qp=int(n_qp),
         per_wire=per_wire, kcl=kcl)
 
 

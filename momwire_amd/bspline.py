@@ -15,7 +15,7 @@ and the Sommerfeld ground (``ground_model="sommerfeld"``: C2-scaled exact
 image plus the interpolated remainder); ground contact (wire ends and
 junctions in the plane, PEC or Sommerfeld; refused under the reflection-
 coefficient ground as the reference refuses it); degree 2 and 1 (tents);
-n_qp_pair 4;
+n_qp_pair 1 .. 8;
 point delta-gap, segment-gap and cos^2-smoothed feeds; K-wire junctions (KCL
 rows, any number), junction ports and node gaps; per-wire radii;
 distributed wire loading (conductor loss, insulation).
@@ -23,7 +23,7 @@ The extended thin-wire kernel (``extended_kernel=True``) on coaxial
 equal-radius pairs, in free space and over every ground (the Sommerfeld
 remainder stays reduced, as in the reference).
 Refused with NotImplementedError: singular enrichment, degree > 2,
-n_qp_pair != 4.
+n_qp_pair > 8.
 """
 
 import ctypes
@@ -111,7 +111,7 @@ class _Device:
         k = self.keep
         self.mw = _lib.MwMesh(
             n_seg=mesh.n_seg, n_edge=len(mesh.edge), n_basis=mesh.n_basis,
-            n_tile=len(mesh.tile_nseg), n_se=len(mesh.se_edge), degree=mesh.degree, n_qp=4,
+            n_tile=len(mesh.tile_nseg), n_se=len(mesh.se_edge), degree=mesh.degree, n_qp=mesh.n_qp,
             ek=int(ek is not None),
             seg=k["seg"].data_ptr(), seg_edge=k["seg_edge"].data_ptr(), edge=k["edge"].data_ptr(),
             edge_i=k["edge_i"].data_ptr(), se_edge=k["se_edge"].data_ptr(),
@@ -211,8 +211,10 @@ class BSplineSolver(_SweptPortSolutions):
             raise NotImplementedError("the GPU path is built for degrees 1 and 2")
         if not wires:
             raise ValueError("wires must be non-empty")
-        if int(n_qp_pair) != 4:
-            raise NotImplementedError("the GPU fill is built for n_qp_pair=4")
+        if int(n_qp_pair) < 1:
+            raise ValueError(f"n_qp_pair must be >= 1, got {n_qp_pair}")
+        if int(n_qp_pair) > 8:
+            raise NotImplementedError("the GPU fill is built for n_qp_pair 1 .. 8")
         if feed_model not in ("point", "segment"):
             raise ValueError(f"feed_model must be 'point' or 'segment', got {feed_model!r}")
         if feed_model == "segment" and feed_smoothing_factor is not None:
@@ -244,7 +246,7 @@ class BSplineSolver(_SweptPortSolutions):
         self.precision = precision
         self.refine_steps = None if refine_steps is None else int(refine_steps)
         self.degree = int(degree)
-        self.n_qp_pair = 4
+        self.n_qp_pair = int(n_qp_pair)
         self.feed_model = feed_model
         self.feed_smoothing_factor = feed_smoothing_factor
         self.n_qp_source = int(n_qp_source)
@@ -431,6 +433,7 @@ class BSplineSolver(_SweptPortSolutions):
             jends = {(w, e) for jw in self.junctions for w, e in jw}
             self._mesh = build_mesh(self.wires_polylines, self.n_per_edge_per_wire,
                                     self._radius_per_wire, self.degree, self.junctions,
+                                    n_qp=self.n_qp_pair,
                                     ground_ends=[g for g in self._ground_ends if g not in jends])
         return self._mesh
 

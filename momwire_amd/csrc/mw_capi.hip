@@ -116,7 +116,8 @@ int check_mesh(const mw_mesh* M) {
   if (!M) return fail(MW_EINVAL, "%smesh is NULL");
   if (M->degree < 1 || M->degree > 2)
     return fail(MW_EINVAL, "%sdegree %lld not built (1 or 2)", "", M->degree);
-  if (M->n_qp != 4) return fail(MW_EINVAL, "%sn_qp_pair %lld not built (4 only)", "", M->n_qp);
+  if (M->n_qp < 1 || M->n_qp > 8)
+    return fail(MW_EINVAL, "%sn_qp_pair %lld not built (1 .. 8)", "", M->n_qp);
   if (M->n_seg <= 0 || M->n_basis <= 0 || M->n_tile <= 0)
     return fail(MW_EINVAL, "%sempty mesh (n_seg=%lld, n_basis=%lld)", "", M->n_seg, M->n_basis);
   if (!M->seg || !M->seg_edge || !M->edge || !M->edge_i || !M->se_tab || !M->band_poly ||
@@ -431,6 +432,8 @@ int mw_sweep_plan_create(int32_t n_mesh, const mw_mesh* meshes, const mw_ports* 
   for (int i = 0; i < n_mesh; ++i) {
     int rc = check_mesh(&meshes[i]);
     if (rc) return rc;
+    if (meshes[i].n_qp != 4)
+      return fail(MW_EINVAL, "%smesh %lld: the batched fill is built for n_qp_pair = 4", "", i);
     if (ports[i].n_ports < 1 || ports[i].n_ports > 32 || ports[i].n_con < 0 ||
         ports[i].n_con > 32 || ports[i].n_ports + ports[i].n_con > 64)
       return fail(MW_ERANGE, "%smesh %lld: ports/constraints beyond the batched limits", "", i);

@@ -23,7 +23,7 @@
 namespace {
 
 constexpr int NM = 3;       // moments per axis (degree 2)
-constexpr int NQ = 4;       // quadrature points per axis
+constexpr int NQ_MAX = 8;   // Gauss-Legendre points per segment per axis: 1 .. NQ_MAX
 constexpr double INV4PI = 0.079577471545947667884441881686257181017229822870228;
 
 // ---------------------------------------------------------------------------
@@ -141,6 +141,7 @@ MW_DEV double static_unit(int p, int q, int delta, double ah, const double* glx,
   return total;
 }
 
+template <int NQ>
 MW_DEV void same_edge_body(int e, const mw_mesh& M, double k) {
   if (e >= M.n_se) return;
   const int ed = M.se_edge[e];
@@ -204,14 +205,16 @@ MW_DEV void same_edge_body(int e, const mw_mesh& M, double k) {
     for (int q = 0; q < NM; ++q) out[p * NM + q] = make_double2(jr[p][q], ji[p][q]);
 }
 
+template <int NQ>
 __global__ __launch_bounds__(256) void k_same_edge(mw_mesh M, double k) {
-  same_edge_body(blockIdx.x * blockDim.x + threadIdx.x, M, k);
+  same_edge_body<NQ>(blockIdx.x * blockDim.x + threadIdx.x, M, k);
 }
 
 // ---------------------------------------------------------------------------
 // Fused tile fill
 // ---------------------------------------------------------------------------
 
+template <int NQ>
 struct SegStage {
   double pos[NQ][3];   // quadrature points
   double wu[NM][NQ];   // w_q L (t_q L)^p
@@ -222,7 +225,8 @@ struct SegStage {
   int ekl;             // coaxial label (extended kernel), -1 when off
 };
 
-MW_DEV void stage_segment(const mw_mesh& M, int s, SegStage& S) {
+template <int NQ>
+MW_DEV void stage_segment(const mw_mesh& M, int s, SegStage<NQ>& S) {
   const double* g = M.seg + (size_t)s * 12;
   const double L = g[9];
   for (int q = 0; q < NQ; ++q) {
@@ -240,29 +244,32 @@ MW_DEV void stage_segment(const mw_mesh& M, int s, SegStage& S) {
   S.ekl = M.ek ? M.ek_label[s] : -1;
 }
 
-template <bool SYM>
+template <bool SYM, int NQ>
 MW_DEV void fill_body(int ti, int tj, const mw_mesh& M, double k, double wmu, double inv_weps,
                       double2* __restrict__ Z, int64_t ldz) {
-  __shared__ SegStage rs[MW_TSMAX], cs[MW_TSMAX];
+  __shared__ SegStage<NQ> rs[MW_TSMAX], cs[MW_TSMAX];
   __shared__ double2 Bt[MW_TSMAX * MW_TSMAX * 9];
   __shared__ int nr_s, nc_s;
 
   const int tid = threadIdx.x;
   const int nr = M.tile_nseg[ti], nc = M.tile_nseg[tj];
-  if (tid < nr) stage_segment(M, M.tile_seg[ti * MW_TSMAX + tid], rs[tid]);
-  if (tid >= 64 && tid - 64 < nc) stage_segment(M, M.tile_seg[tj * MW_TSMAX + tid - 64], cs[tid - 64]);
+  if (tid < nr) stage_segment<NQ>(M, M.tile_seg[ti * MW_TSMAX + tid], rs[tid]);
+  if (tid >= 64 && tid - 64 < nc)
+    stage_segment<NQ>(M, M.tile_seg[tj * MW_TSMAX + tid - 64], cs[tid - 64]);
   if (tid == 0) { nr_s = nr; nc_s = nc; }
   __syncthreads();
 
   const double2* se = reinterpret_cast<const double2*>(M.se_tab);
   const int npair = nr * nc;
-  // Two lanes per pair: half 0 takes q = 0,1; half 1 takes q = 2,3.
+  // Two lanes per pair: half 0 takes the first QH observer points, half 1
+  // the rest (q = 0,1 and 2,3 at the default four).
+  constexpr int QH = (NQ + 1) / 2;
   for (int item = tid; item < 2 * ((npair + 127) / 128) * 128; item += 256) {
     const int pair = item >> 1, half = item & 1;
     const bool live = pair < npair;
     const int li = live ? pair / nc : 0, lj = live ? pair % nc : 0;
-    const SegStage& I = rs[li];
-    const SegStage& J = cs[lj];
+    const SegStage<NQ>& I = rs[li];
+    const SegStage<NQ>& J = cs[lj];
     double jr[NM][NM], jim[NM][NM];
 #pragma unroll
     for (int p = 0; p < NM; ++p)
@@ -286,8 +293,9 @@ MW_DEV void fill_body(int ti, int tj, const mw_mesh& M, double k, double wmu, do
       const double a2 = I.a * I.a;
       const bool ext = I.ekl >= 0 && I.ekl == J.ekl;  // extended kernel on this pair
 #pragma unroll
-      for (int qq = 0; qq < 2; ++qq) {
-        const int q = 2 * half + qq;
+      for (int qq = 0; qq < QH; ++qq) {
+        const int q = QH * half + qq;
+        if (q >= NQ) break;
         const double x = I.pos[q][0], y = I.pos[q][1], z = I.pos[q][2];
         const double wq0 = I.wu[0][q], wq1 = I.wu[1][q], wq2 = I.wu[2][q];
 #pragma unroll
@@ -389,14 +397,15 @@ MW_DEV void fill_body(int ti, int tj, const mw_mesh& M, double k, double wmu, do
   }
 }
 
-template <bool SYM>
+template <bool SYM, int NQ>
 __global__ __launch_bounds__(256) void k_fill(mw_mesh M, double k, double wmu,
                                               double inv_weps, double2* __restrict__ Z,
                                               int64_t ldz, const int32_t* __restrict__ pairs) {
   if (SYM)
-    fill_body<true>(pairs[2 * blockIdx.x], pairs[2 * blockIdx.x + 1], M, k, wmu, inv_weps, Z, ldz);
+    fill_body<true, NQ>(pairs[2 * blockIdx.x], pairs[2 * blockIdx.x + 1], M, k, wmu, inv_weps, Z,
+                        ldz);
   else
-    fill_body<false>(blockIdx.y, blockIdx.x, M, k, wmu, inv_weps, Z, ldz);
+    fill_body<false, NQ>(blockIdx.y, blockIdx.x, M, k, wmu, inv_weps, Z, ldz);
 }
 
 // ---------------------------------------------------------------------------
@@ -406,7 +415,7 @@ __global__ __launch_bounds__(256) void kb_same_edge(const mw_sweep_prob* P, cons
   const mw_sweep_prob& p = P[blockIdx.y];
   mw_mesh M = meshes[p.mesh];
   M.se_tab = p.se_tab;
-  same_edge_body(blockIdx.x * blockDim.x + threadIdx.x, M, p.k);
+  same_edge_body<4>(blockIdx.x * blockDim.x + threadIdx.x, M, p.k);
 }
 
 // one workgroup per (problem, tile pair) work item
@@ -421,37 +430,63 @@ __global__ __launch_bounds__(256) void kb_fill(const mw_sweep_prob* P, const mw_
   const int q = item_pair[blockIdx.x];
   const double wmu = p.omega * mu, inv_weps = 1.0 / (p.omega * eps);
   if (p.symmetric)
-    fill_body<true>(pairs[2 * q], pairs[2 * q + 1], M, p.k, wmu, inv_weps,
-                    reinterpret_cast<double2*>(p.A), p.lda);
+    fill_body<true, 4>(pairs[2 * q], pairs[2 * q + 1], M, p.k, wmu, inv_weps,
+                       reinterpret_cast<double2*>(p.A), p.lda);
   else
-    fill_body<false>(pairs[2 * q], pairs[2 * q + 1], M, p.k, wmu, inv_weps,
-                     reinterpret_cast<double2*>(p.A), p.lda);
+    fill_body<false, 4>(pairs[2 * q], pairs[2 * q + 1], M, p.k, wmu, inv_weps,
+                        reinterpret_cast<double2*>(p.A), p.lda);
+}
+
+template <int NQ>
+hipError_t launch_fill_nq(const mw_mesh& M, double k, double wmu, double inv_weps, double2* Z,
+                          int64_t ldz, const int32_t* pairs, int32_t n_pairs, bool symmetric,
+                          hipStream_t st) {
+  if (symmetric) {
+    if (n_pairs == 0) return hipSuccess;
+    hipLaunchKernelGGL((k_fill<true, NQ>), dim3(n_pairs), dim3(256), 0, st, M, k, wmu, inv_weps,
+                       Z, ldz, pairs);
+  } else {
+    hipLaunchKernelGGL((k_fill<false, NQ>), dim3(M.n_tile, M.n_tile), dim3(256), 0, st, M, k, wmu,
+                       inv_weps, Z, ldz, pairs);
+  }
+  return hipGetLastError();
 }
 
 }  // namespace
+
+// Dispatch on the mesh's point count (the mesh check holds it to 1 .. NQ_MAX).
+#define MW_NQ_SWITCH(nq, CALL) \
+  switch (nq) {                \
+    case 1: CALL(1);           \
+    case 2: CALL(2);           \
+    case 3: CALL(3);           \
+    case 4: CALL(4);           \
+    case 5: CALL(5);           \
+    case 6: CALL(6);           \
+    case 7: CALL(7);           \
+    case 8: CALL(8);           \
+    default: return hipErrorInvalidValue; \
+  }
 
 // ---------------------------------------------------------------------------
 // Host launchers (called from mw_capi.cpp)
 // ---------------------------------------------------------------------------
 hipError_t mw_launch_same_edge(const mw_mesh& M, double k, hipStream_t st) {
   if (M.n_se == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_same_edge, dim3((M.n_se + 255) / 256), dim3(256), 0, st, M, k);
+#define MW_SE(NQ)                                                                        \
+  hipLaunchKernelGGL(k_same_edge<NQ>, dim3((M.n_se + 255) / 256), dim3(256), 0, st, M, k); \
   return hipGetLastError();
+  MW_NQ_SWITCH(M.n_qp, MW_SE)
+#undef MW_SE
 }
 
 hipError_t mw_launch_fill(const mw_mesh& M, double k, double omega, double eps, double mu,
                           double2* Z, int64_t ldz, const int32_t* pairs, int32_t n_pairs,
                           bool symmetric, hipStream_t st) {
   const double wmu = omega * mu, inv_weps = 1.0 / (omega * eps);
-  if (symmetric) {
-    if (n_pairs == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_fill<true>, dim3(n_pairs), dim3(256), 0, st, M, k, wmu, inv_weps, Z,
-                       ldz, pairs);
-  } else {
-    hipLaunchKernelGGL(k_fill<false>, dim3(M.n_tile, M.n_tile), dim3(256), 0, st, M, k, wmu,
-                       inv_weps, Z, ldz, pairs);
-  }
-  return hipGetLastError();
+#define MW_FILL(NQ) return launch_fill_nq<NQ>(M, k, wmu, inv_weps, Z, ldz, pairs, n_pairs, symmetric, st);
+  MW_NQ_SWITCH(M.n_qp, MW_FILL)
+#undef MW_FILL
 }
 
 hipError_t mw_launch_batch_fill(const mw_sweep_prob* P, const mw_mesh* meshes, int nprob,

@@ -626,12 +626,13 @@ __global__ __launch_bounds__(256) void k_somm_eval(mw_somm_grid G, const double*
 // ---------------------------------------------------------------------------
 // grounded fill: Z -= C2 * Z_image + Q
 // ---------------------------------------------------------------------------
-constexpr int NM = 3, NQ = 4, NQS = 3;
+constexpr int NM = 3, NQS = 3;   // NQ (image moments): the mesh's n_qp, a template parameter
 constexpr double INV4PI = 0.079577471545947667884441881686257181017229822870228;
 
+template <int NQ>
 struct GStage {
   double cen[3];         // segment midpoint (never mirrored)
-  double pos[NQ][3];     // 4-point nodes (image moments; mirrored for sources)
+  double pos[NQ][3];     // NQ-point nodes (image moments; mirrored for sources)
   double wu[NM][NQ];
   double sp[NQS][3];     // 3-point remainder nodes
   double sw[NM][NQS];    // w_q u_q^p of the remainder rule
@@ -652,7 +653,9 @@ struct GroundArgs {
   double er, ei, p0r, p0i, p1r, p1i;
 };
 
-MW_DEV void stage_ground(const mw_mesh& M, int s, bool source, const GroundArgs& A, GStage& S) {
+template <int NQ>
+MW_DEV void stage_ground(const mw_mesh& M, int s, bool source, const GroundArgs& A,
+                         GStage<NQ>& S) {
   const double* g = M.seg + (size_t)s * 12;
   const double L = g[9];
   for (int c = 0; c < 3; ++c) S.cen[c] = 0.5 * (g[c] + g[3 + c]);
@@ -684,8 +687,9 @@ MW_DEV void stage_ground(const mw_mesh& M, int s, bool source, const GroundArgs&
 // normal to the plane of incidence (x for a vertical ray), P = (t_i.p)(t_j.p);
 // w_A = rho_v (td - P) - rho_h P, w_Phi = c0 + c1 rho_v.  Symmetric in the
 // pair, so the symmetric tile fill stays valid.
-MW_DEV void refl_weights(const GStage& I, const GStage& J, const GroundArgs& A, double td, cd& wa,
-                         cd& wp) {
+template <int NQ>
+MW_DEV void refl_weights(const GStage<NQ>& I, const GStage<NQ>& J, const GroundArgs& A, double td,
+                         cd& wa, cd& wp) {
   constexpr double TINY = 1e-30;
   const double dx = I.cen[0] - J.cen[0], dy = I.cen[1] - J.cen[1];
   const double dz = I.cen[2] + J.cen[2] - 2.0 * A.gz;
@@ -706,17 +710,18 @@ MW_DEV void refl_weights(const GStage& I, const GStage& J, const GroundArgs& A, 
   wp = zadd(cz(A.p0r, A.p0i), zmul(cz(A.p1r, A.p1i), rv));
 }
 
-template <bool SYM>
+template <bool SYM, int NQ>
 MW_DEV void ground_body(int ti, int tj, const mw_mesh& M, const GroundArgs& A,
                         const mw_somm_grid& G, double2* __restrict__ Z, int64_t ldz) {
-  __shared__ GStage rs[MW_TSMAX], cs[MW_TSMAX];
+  __shared__ GStage<NQ> rs[MW_TSMAX], cs[MW_TSMAX];
   __shared__ double2 Bt[MW_TSMAX * MW_TSMAX * 9];
 
   const int tid = threadIdx.x;
   const int nr = M.tile_nseg[ti], nc = M.tile_nseg[tj];
-  if (tid < nr) stage_ground(M, M.tile_seg[ti * MW_TSMAX + tid], false, A, rs[tid]);
+  if (tid < nr) stage_ground<NQ>(M, M.tile_seg[ti * MW_TSMAX + tid], false, A, rs[tid]);
   if (tid >= 64 && tid - 64 < nc)
-    stage_ground(M, M.tile_seg[tj * MW_TSMAX + tid - 64], true, A, cs[tid - 64]);
+    stage_ground<NQ>(M, M.tile_seg[tj * MW_TSMAX + tid - 64], true, A, cs[tid - 64]);
+  constexpr int QH = (NQ + 1) / 2;  // image-moment observer points per half
   __syncthreads();
 
   const int npair = nr * nc;
@@ -724,12 +729,12 @@ MW_DEV void ground_body(int ti, int tj, const mw_mesh& M, const GroundArgs& A,
     const int pair = item >> 1, half = item & 1;
     const bool live = pair < npair;
     const int li = live ? pair / nc : 0, lj = live ? pair % nc : 0;
-    const GStage& I = rs[li];
-    const GStage& J = cs[lj];
+    const GStage<NQ>& I = rs[li];
+    const GStage<NQ>& J = cs[lj];
     const int be0 = half == 0 ? 0 : 2, be1 = half == 0 ? 2 : 3;
     double* bt = reinterpret_cast<double*>(Bt + (li * nc + lj) * 9);
     {
-      // image moments: observer points 2*half, 2*half+1 against the mirrored source
+      // image moments: observer points QH*half .. against the mirrored source
       double jr[NM][NM], jm[NM][NM];
 #pragma unroll
       for (int p = 0; p < NM; ++p)
@@ -741,8 +746,9 @@ MW_DEV void ground_body(int ti, int tj, const mw_mesh& M, const GroundArgs& A,
         // wire and its own image); the remainder below stays reduced
         const bool ext = I.ekl >= 0 && I.ekl == J.ekl;
 #pragma unroll
-        for (int qq = 0; qq < 2; ++qq) {
-          const int q = 2 * half + qq;
+        for (int qq = 0; qq < QH; ++qq) {
+          const int q = QH * half + qq;
+          if (q >= NQ) break;
           const double x = I.pos[q][0], y = I.pos[q][1], z = I.pos[q][2];
 #pragma unroll
           for (int r = 0; r < NQ; ++r) {
@@ -890,14 +896,29 @@ MW_DEV void ground_body(int ti, int tj, const mw_mesh& M, const GroundArgs& A,
   }
 }
 
-template <bool SYM>
+template <bool SYM, int NQ>
 __global__ __launch_bounds__(256) void k_fill_ground(mw_mesh M, GroundArgs A, mw_somm_grid G,
                                                      double2* __restrict__ Z, int64_t ldz,
                                                      const int32_t* __restrict__ pairs) {
   if (SYM)
-    ground_body<true>(pairs[2 * blockIdx.x], pairs[2 * blockIdx.x + 1], M, A, G, Z, ldz);
+    ground_body<true, NQ>(pairs[2 * blockIdx.x], pairs[2 * blockIdx.x + 1], M, A, G, Z, ldz);
   else
-    ground_body<false>(blockIdx.y, blockIdx.x, M, A, G, Z, ldz);
+    ground_body<false, NQ>(blockIdx.y, blockIdx.x, M, A, G, Z, ldz);
+}
+
+template <int NQ>
+hipError_t launch_ground_nq(const mw_mesh& M, const GroundArgs& A, const mw_somm_grid& g,
+                            double2* Z, int64_t ldz, const int32_t* pairs, int32_t n_pairs,
+                            bool symmetric, hipStream_t st) {
+  if (symmetric) {
+    if (n_pairs == 0) return hipSuccess;
+    hipLaunchKernelGGL((k_fill_ground<true, NQ>), dim3(n_pairs), dim3(256), 0, st, M, A, g, Z,
+                       ldz, pairs);
+  } else {
+    hipLaunchKernelGGL((k_fill_ground<false, NQ>), dim3(M.n_tile, M.n_tile), dim3(256), 0, st, M,
+                       A, g, Z, ldz, pairs);
+  }
+  return hipGetLastError();
 }
 
 }  // namespace
@@ -967,13 +988,15 @@ hipError_t mw_launch_fill_ground(const mw_mesh& M, double k, double omega, doubl
   A.remainder = G != nullptr;
   mw_somm_grid g{};
   if (G) g = *G;
-  if (symmetric) {
-    if (n_pairs == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_fill_ground<true>, dim3(n_pairs), dim3(256), 0, st, M, A, g, Z, ldz,
-                       pairs);
-  } else {
-    hipLaunchKernelGGL(k_fill_ground<false>, dim3(M.n_tile, M.n_tile), dim3(256), 0, st, M, A, g,
-                       Z, ldz, pairs);
+  switch (M.n_qp) {  // the mesh check holds n_qp to 1 .. 8
+    case 1: return launch_ground_nq<1>(M, A, g, Z, ldz, pairs, n_pairs, symmetric, st);
+    case 2: return launch_ground_nq<2>(M, A, g, Z, ldz, pairs, n_pairs, symmetric, st);
+    case 3: return launch_ground_nq<3>(M, A, g, Z, ldz, pairs, n_pairs, symmetric, st);
+    case 4: return launch_ground_nq<4>(M, A, g, Z, ldz, pairs, n_pairs, symmetric, st);
+    case 5: return launch_ground_nq<5>(M, A, g, Z, ldz, pairs, n_pairs, symmetric, st);
+    case 6: return launch_ground_nq<6>(M, A, g, Z, ldz, pairs, n_pairs, symmetric, st);
+    case 7: return launch_ground_nq<7>(M, A, g, Z, ldz, pairs, n_pairs, symmetric, st);
+    case 8: return launch_ground_nq<8>(M, A, g, Z, ldz, pairs, n_pairs, symmetric, st);
+    default: return hipErrorInvalidValue;
   }
-  return hipGetLastError();
 }

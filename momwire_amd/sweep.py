@@ -39,6 +39,8 @@ class BatchedSweep:
         self.cancel = cancel
         self.solvers = [BSplineSolver(**kw) for kw in structures]
         for i, s in enumerate(self.solvers):
+            if s.n_qp_pair != 4:
+                raise NotImplementedError(f"structure {i}: n_qp_pair != 4 in the batched sweep")
             if s._loading_active:
                 raise NotImplementedError(f"structure {i}: wire loading in the batched sweep "
                                           "(solve it with BSplineSolver one frequency at a time)")

@@ -410,7 +410,28 @@ def case_d1(kind):
     return out, dt
 
 
+def case_nqp(kind, nqp):
+    """Other Gauss orders per segment pair (reference bspline.py:390-394,
+    781): the off-edge, same-edge remainder and image moments at n_qp_pair
+    points -- the junction tee in free space, the grounded tee over PEC."""
+    kw = workloads.junction_tee("junction_port")
+    if kind == "pec":
+        kw = workloads.grounded_tee()
+        kw.pop("ground_eps", None)
+        kw.pop("ground_model", None)
+    kw["n_qp_pair"] = nqp
+    s, sol, dt = _solve(kw)
+    geom = s._build_geometry()
+    supp, polys, _kcl, _wk, _wb = s._build_basis_polynomials(geom)
+    return dict(y=sol.y, coeffs=sol.coeffs,
+                Z=np.asarray(s._compute_Z_operator(geom, supp, polys))), dt
+
+
 CASES = {
+    "nqp2_tee": lambda: case_nqp("free", 2),
+    "nqp3_tee": lambda: case_nqp("free", 3),
+    "nqp7_tee": lambda: case_nqp("free", 7),
+    "nqp5_pec_tee": lambda: case_nqp("pec", 5),
     "d1_dipole": lambda: case_d1("dipole"),
     "d1_tee": lambda: case_d1("tee"),
     "d1_contact": lambda: case_d1("contact"),

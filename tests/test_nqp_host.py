@@ -1,0 +1,45 @@
+"""Other Gauss orders per segment pair (n_qp_pair, reference
+bspline.py:390-394, 781) on the CPU: the oracle against the reference's Z
+and Y at 2, 3 and 7 points on the junction tee and at 5 over a PEC ground
+(tests/golden/nqp*.npz); the refusals outside 1 .. 8."""
+
+import pytest
+
+from conftest import golden, rel_err
+from momwire_amd import workloads as W
+from momwire_amd.bspline import BSplineSolver
+from oracle.bspline_oracle import OracleSolver
+
+
+def nqp_case(name):
+    nqp = int(name[3])
+    if "pec" in name:
+        kw = W.grounded_tee()
+        kw.pop("ground_eps", None)
+        kw.pop("ground_model", None)
+    else:
+        kw = W.junction_tee("junction_port")
+    kw["n_qp_pair"] = nqp
+    return kw
+
+
+CASES = ("nqp2_tee", "nqp3_tee", "nqp7_tee", "nqp5_pec_tee")
+
+
+@pytest.mark.parametrize("case", CASES)
+def test_nqp_oracle_matches_reference(case):
+    d = golden(case)
+    o = OracleSolver(**nqp_case(case))
+    assert rel_err(o.Z(), d["Z"]) < 1e-10
+    y, X = o.compute_port_solution()
+    assert rel_err(y, d["y"]) < 1e-10 and rel_err(X, d["coeffs"]) < 1e-10
+
+
+def test_nqp_validation():
+    kw = W.junction_tee("feed")
+    with pytest.raises(ValueError):
+        BSplineSolver(**kw, n_qp_pair=0)
+    with pytest.raises(NotImplementedError):
+        BSplineSolver(**kw, n_qp_pair=9)
+    s = BSplineSolver(**kw, n_qp_pair=6)
+    assert s._build().n_qp == 6 and len(s._build().gl) == 2 * 6 + 32
