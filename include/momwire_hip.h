@@ -325,8 +325,9 @@ int mw_somm_eval(const mw_somm_grid* grid, const double* R1, const double* theta
  * Grounded fill: Z -= C2 * Z_image + Q, accumulated into a Z already holding
  * the free-space matrix (row-major, ld ldz).  Z_image is the exact image
  * (mirrored sources, tangent dot t_m . (t_x, t_y, -t_z), charge weight 1);
- * Q the Galerkin remainder block of the tabulated field (3-point Gauss per
- * segment).  grid = NULL drops Q (image only).  Same tiling and symmetric
+ * Q the Galerkin remainder block of the tabulated field (n_qp_sommerfeld-
+ * point Gauss per segment, 1 .. 8; 3 is the reference's default).  grid =
+ * NULL drops Q (image only).  Same tiling and symmetric
  * tile-pair list as mw_fill_z.  Replaces the weighted image assembly and the
  * fused remainder kernel: assemble_Z_bspline_weighted_windowed and
  * sommerfeld_remainder_bspline_Q (reference _accelerators.cpp:1498-1608,
@@ -334,8 +335,8 @@ int mw_somm_eval(const mw_somm_grid* grid, const double* R1, const double* theta
  */
 int mw_fill_ground(const mw_mesh* mesh, double k, double omega, double eps, double mu,
                    double ground_z, double c2_re, double c2_im, const mw_somm_grid* grid,
-                   void* Z, int64_t ldz, const int32_t* tile_pairs, int32_t n_pairs,
-                   int32_t symmetric, void* stream);
+                   int32_t n_qp_sommerfeld, void* Z, int64_t ldz, const int32_t* tile_pairs,
+                   int32_t n_pairs, int32_t symmetric, void* stream);
 
 /*
  * Reflection-coefficient finite ground (NEC gn 0 style): Z -= Z_image with

@@ -23,7 +23,7 @@ The extended thin-wire kernel (``extended_kernel=True``) on coaxial
 equal-radius pairs, in free space and over every ground (the Sommerfeld
 remainder stays reduced, as in the reference).
 Refused with NotImplementedError: singular enrichment, degree > 2,
-n_qp_pair > 8.
+n_qp_pair or n_qp_sommerfeld > 8.
 """
 
 import ctypes
@@ -231,8 +231,10 @@ class BSplineSolver(_SweptPortSolutions):
         if ground_phi_mode not in _ground.PHI_MODES:
             raise ValueError(f"ground_phi_mode must be one of {_ground.PHI_MODES}, "
                              f"got {ground_phi_mode!r}")
-        if ground_model == "sommerfeld" and int(n_qp_sommerfeld) != 3:
-            raise NotImplementedError("the remainder block is built for n_qp_sommerfeld=3")
+        if int(n_qp_sommerfeld) < 1:
+            raise ValueError(f"n_qp_sommerfeld must be >= 1, got {n_qp_sommerfeld}")
+        if int(n_qp_sommerfeld) > 8:
+            raise NotImplementedError("the remainder block is built for n_qp_sommerfeld 1 .. 8")
         if int(swept_mem_mb) < 1:
             raise ValueError(f"swept_mem_mb must be >= 1, got {swept_mem_mb}")
 
@@ -567,7 +569,8 @@ class BSplineSolver(_SweptPortSolutions):
         c2 = _ground.image_coefficient(eps_t)
         gref = ctypes.byref(grid) if grid is not None else None
         _lib.check(lib.mw_fill_ground(D.mw, k, omega, self.eps, self.mu, self.ground_z, c2.real,
-                                      c2.imag, gref, zptr, ld, pairs.data_ptr(),
+                                      c2.imag, gref, self.n_qp_sommerfeld, zptr, ld,
+                                      pairs.data_ptr(),
                                       int(pairs.shape[0]), int(mesh.symmetric), st),
                    "ground fill")
 

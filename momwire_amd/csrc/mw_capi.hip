@@ -61,7 +61,7 @@ hipError_t mw_launch_somm_eval(const mw_somm_grid& G, const double* R1, const do
                                double2* out, hipStream_t st);
 hipError_t mw_launch_fill_ground(const mw_mesh& M, double k, double omega, double eps, double mu,
                                  double gz, double c2r, double c2i, const mw_somm_grid* G,
-                                 const double* refl, double2* Z, int64_t ldz,
+                                 const double* refl, int nqs, double2* Z, int64_t ldz,
                                  const int32_t* pairs, int32_t n_pairs, bool symmetric,
                                  hipStream_t st);
 
@@ -573,19 +573,22 @@ int mw_somm_eval(const mw_somm_grid* grid, const double* R1, const double* theta
 
 int mw_fill_ground(const mw_mesh* mesh, double k, double omega, double eps, double mu,
                    double ground_z, double c2_re, double c2_im, const mw_somm_grid* grid,
-                   void* Z, int64_t ldz, const int32_t* tile_pairs, int32_t n_pairs,
-                   int32_t symmetric, void* stream) {
+                   int32_t n_qp_sommerfeld, void* Z, int64_t ldz, const int32_t* tile_pairs,
+                   int32_t n_pairs, int32_t symmetric, void* stream) {
   g_err[0] = 0;
   int rc = check_mesh_ground(mesh);
   if (rc) return rc;
   if (grid && (rc = check_grid(grid))) return rc;
+  if (n_qp_sommerfeld < 1 || n_qp_sommerfeld > 8)
+    return fail(MW_EINVAL, "%sn_qp_sommerfeld %lld not built (1 .. 8)", "", n_qp_sommerfeld);
   if (!Z) return fail(MW_EINVAL, "%sZ is NULL");
   if (ldz < mesh->n_basis) return fail(MW_EINVAL, "%sldz %lld < n_basis %lld", "", ldz, mesh->n_basis);
   if (!(k > 0.0) || !(omega > 0.0)) return fail(MW_EINVAL, "%sk and omega must be positive");
   if (symmetric && (!tile_pairs || n_pairs <= 0))
     return fail(MW_EINVAL, "%ssymmetric fill needs the tile-pair list");
   return hip_status(mw_launch_fill_ground(*mesh, k, omega, eps, mu, ground_z, c2_re, c2_im, grid,
-                                          nullptr, (double2*)Z, ldz, tile_pairs, n_pairs,
+                                          nullptr, n_qp_sommerfeld, (double2*)Z, ldz, tile_pairs,
+                                          n_pairs,
                                           symmetric != 0, (hipStream_t)stream),
                     "mw_fill_ground");
 }
@@ -607,7 +610,7 @@ int mw_fill_ground_refl(const mw_mesh* mesh, double k, double omega, double eps,
     return fail(MW_EINVAL, "%ssymmetric fill needs the tile-pair list");
   const double refl[6] = {eps_t_re, eps_t_im, phi_c0_re, phi_c0_im, phi_c1_re, phi_c1_im};
   return hip_status(mw_launch_fill_ground(*mesh, k, omega, eps, mu, ground_z, 1.0, 0.0, nullptr,
-                                          refl, (double2*)Z, ldz, tile_pairs, n_pairs,
+                                          refl, 3, (double2*)Z, ldz, tile_pairs, n_pairs,
                                           symmetric != 0, (hipStream_t)stream),
                     "mw_fill_ground_refl");
 }

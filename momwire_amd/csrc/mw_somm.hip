@@ -626,7 +626,9 @@ __global__ __launch_bounds__(256) void k_somm_eval(mw_somm_grid G, const double*
 // ---------------------------------------------------------------------------
 // grounded fill: Z -= C2 * Z_image + Q
 // ---------------------------------------------------------------------------
-constexpr int NM = 3, NQS = 3;   // NQ (image moments): the mesh's n_qp, a template parameter
+// NQ (image moments): the mesh's n_qp, a template parameter; the remainder
+// rule's order (n_qp_sommerfeld, default 3) is a runtime count <= NQS_MAX.
+constexpr int NM = 3, NQS_MAX = 8;
 constexpr double INV4PI = 0.079577471545947667884441881686257181017229822870228;
 
 template <int NQ>
@@ -634,8 +636,8 @@ struct GStage {
   double cen[3];         // segment midpoint (never mirrored)
   double pos[NQ][3];     // NQ-point nodes (image moments; mirrored for sources)
   double wu[NM][NQ];
-  double sp[NQS][3];     // 3-point remainder nodes
-  double sw[NM][NQS];    // w_q u_q^p of the remainder rule
+  double sp[NQS_MAX][3];  // remainder-rule nodes
+  double sw[NM][NQS_MAX]; // w_q u_q^p of the remainder rule
   double tan[3];         // (mirrored for sources)
   double a;
   double poly[NM][NM];
@@ -646,7 +648,8 @@ struct GStage {
 struct GroundArgs {
   double k, wmu, inv_weps, gz;
   double c2r, c2i;
-  double t3[NQS], w3[NQS];  // remainder rule on [0, 1]
+  double t3[NQS_MAX], w3[NQS_MAX];  // remainder rule on [0, 1]
+  int nqs;                          // its order
   int remainder;
   // reflection-coefficient ground: eps~ and the charge weight c0 + c1 rho_v
   int refl;
@@ -666,7 +669,7 @@ MW_DEV void stage_ground(const mw_mesh& M, int s, bool source, const GroundArgs&
     const double w = M.gl[NQ + q] * L, u = t * L;
     S.wu[0][q] = w; S.wu[1][q] = w * u; S.wu[2][q] = w * u * u;
   }
-  for (int q = 0; q < NQS; ++q) {
+  for (int q = 0; q < A.nqs; ++q) {
     const double t = A.t3[q];
     for (int c = 0; c < 3; ++c) S.sp[q][c] = g[c] + t * (g[3 + c] - g[c]);
     const double w = A.w3[q] * L, u = t * L;
@@ -819,16 +822,18 @@ MW_DEV void ground_body(int ti, int tj, const mw_mesh& M, const GroundArgs& A,
       }
     }
     if (A.remainder) {
-      // remainder moments: node pairs t = qi*3 + rj, 5 on lane half 0, 4 on half 1
+      // remainder moments: node pairs t = qi nqs + rj, the first half (rounded
+      // up) on lane half 0, the rest on half 1
       double fr[NM][NM], fm[NM][NM];
 #pragma unroll
       for (int p = 0; p < NM; ++p)
 #pragma unroll
         for (int P = 0; P < NM; ++P) fr[p][P] = fm[p][P] = 0.0;
       if (live) {
-        const int t0 = half ? 5 : 0, t1 = half ? 9 : 5;
+        const int nt = A.nqs * A.nqs, mid = (nt + 1) / 2;
+        const int t0 = half ? mid : 0, t1 = half ? nt : mid;
         for (int t = t0; t < t1; ++t) {
-          const int qi = t / 3, rj = t % 3;
+          const int qi = t / A.nqs, rj = t % A.nqs;
           const cd v = proj_one(G, A.gz, A.k, I.sp[qi], I.tan, J.sp[rj], J.dir);
 #pragma unroll
           for (int p = 0; p < NM; ++p)
@@ -962,11 +967,37 @@ hipError_t mw_launch_somm_eval(const mw_somm_grid& G, const double* R1, const do
   return hipGetLastError();
 }
 
+// n-point Gauss-Legendre on [0, 1], n <= NQS_MAX: Newton on P_n from the
+// Tricomi starting values, P_n and P_n' by the three-term recurrence.
+static void gauss_legendre01(int n, double* t, double* w) {
+  for (int i = 0; i < n; ++i) {
+    double x = cos(3.14159265358979323846 * (i + 0.75) / (n + 0.5));
+    double dp = 1.0;
+    for (int it = 0; it < 100; ++it) {
+      double p0 = 1.0, p1 = x;
+      for (int j = 2; j <= n; ++j) {
+        const double p2 = ((2 * j - 1) * x * p1 - (j - 1) * p0) / j;
+        p0 = p1;
+        p1 = p2;
+      }
+      if (n == 1) { p0 = 1.0; p1 = x; }
+      dp = n * (x * p1 - p0) / (x * x - 1.0);
+      const double dx = p1 / dp;
+      x -= dx;
+      if (fabs(dx) < 1e-16) break;
+    }
+    // x descends with i; t ascends
+    t[i] = 0.5 * (1.0 - x);
+    w[i] = 1.0 / ((1.0 - x * x) * dp * dp);
+  }
+}
+
 hipError_t mw_launch_fill_ground(const mw_mesh& M, double k, double omega, double eps, double mu,
                                  double gz, double c2r, double c2i, const mw_somm_grid* G,
-                                 const double* refl, double2* Z, int64_t ldz,
+                                 const double* refl, int nqs, double2* Z, int64_t ldz,
                                  const int32_t* pairs, int32_t n_pairs, bool symmetric,
                                  hipStream_t st) {
+  if (nqs < 1 || nqs > NQS_MAX) return hipErrorInvalidValue;
   GroundArgs A;
   A.refl = refl != nullptr;
   A.er = A.ei = A.p0r = A.p0i = A.p1r = A.p1i = 0.0;
@@ -981,10 +1012,9 @@ hipError_t mw_launch_fill_ground(const mw_mesh& M, double k, double omega, doubl
   A.gz = gz;
   A.c2r = c2r;
   A.c2i = c2i;
-  // 3-point Gauss-Legendre mapped to [0, 1]: nodes (1 -+ sqrt(3/5))/2, 1/2;
-  // weights 5/18, 8/18, 5/18
-  A.t3[0] = 0.1127016653792583; A.t3[1] = 0.5; A.t3[2] = 0.8872983346207417;
-  A.w3[0] = 0.27777777777777785; A.w3[1] = 0.4444444444444444; A.w3[2] = 0.27777777777777785;
+  A.nqs = nqs;
+  for (int q = nqs; q < NQS_MAX; ++q) A.t3[q] = A.w3[q] = 0.0;
+  gauss_legendre01(nqs, A.t3, A.w3);
   A.remainder = G != nullptr;
   mw_somm_grid g{};
   if (G) g = *G;

@@ -427,7 +427,19 @@ def case_nqp(kind, nqp):
                 Z=np.asarray(s._compute_Z_operator(geom, supp, polys))), dt
 
 
+def case_nqs(nqs):
+    """The Sommerfeld remainder at another Gauss order per segment
+    (n_qp_sommerfeld, reference bspline.py:600-616, 1791-1874): the grounded
+    tee's Y and coeffs."""
+    kw = workloads.grounded_tee()
+    kw["n_qp_sommerfeld"] = nqs
+    s, sol, dt = _solve(kw)
+    return dict(y=sol.y, coeffs=sol.coeffs), dt
+
+
 CASES = {
+    "nqs2_ground_tee": lambda: case_nqs(2),
+    "nqs5_ground_tee": lambda: case_nqs(5),
     "nqp2_tee": lambda: case_nqp("free", 2),
     "nqp3_tee": lambda: case_nqp("free", 3),
     "nqp7_tee": lambda: case_nqp("free", 7),

@@ -149,7 +149,7 @@ def test_c_abi_rejects_ek_without_labels():
     rc = lib.mw_same_edge_table(ctypes.byref(m), 1.0, None)
     assert rc == _lib.MW_EINVAL and b"coaxial labels" in lib.mw_last_error()
     m.ek_label = 64
-    rc = lib.mw_fill_ground(ctypes.byref(m), 1.0, 1.0, 1.0, 1.0, 0.0, 1.0, 0.0, None, 64, 5,
+    rc = lib.mw_fill_ground(ctypes.byref(m), 1.0, 1.0, 1.0, 1.0, 0.0, 1.0, 0.0, None, 3, 64, 5,
                             None, 0, 1, None)
     assert rc == _lib.MW_EINVAL and b"image block" in lib.mw_last_error()
     rc = lib.mw_fill_ground_refl(ctypes.byref(m), 1.0, 1.0, 1.0, 1.0, 0.0, 4.0, -1.0, 0.3, 0.0,

@@ -27,3 +27,15 @@ def test_nqp_matches_reference(case):
     assert rel_err(D.Z[:, :D.nb].cpu().numpy(), d["Z"]) < 1e-10
     sol = s.compute_port_solution()
     assert rel_err(sol.y, d["y"]) < 1e-10 and rel_err(sol.coeffs, d["coeffs"]) < 1e-10
+
+
+@pytest.mark.parametrize("nqs", [2, 5])
+def test_nqs_matches_reference(nqs):
+    from momwire_amd import _ground, workloads as W
+    print("[gpu-test] n_qp_sommerfeld", nqs, flush=True)
+    d = golden(f"nqs{nqs}_ground_tee")
+    kw = W.grounded_tee()
+    kw["n_qp_sommerfeld"] = nqs
+    _ground.clear_cache()
+    sol = BSplineSolver(**kw).compute_port_solution()
+    assert rel_err(sol.y, d["y"]) < 1e-10 and rel_err(sol.coeffs, d["coeffs"]) < 1e-10
