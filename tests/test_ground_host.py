@@ -29,7 +29,9 @@ def test_ground_options_validation():
     s = BSplineSolver(**_kw(ground_model="refl-coef"))
     assert s.ground_phi_mode == "normal"
     with pytest.raises(NotImplementedError):
-        BSplineSolver(**_kw(n_qp_sommerfeld=5))
+        BSplineSolver(**_kw(n_qp_sommerfeld=9))
+    with pytest.raises(ValueError):
+        BSplineSolver(**_kw(n_qp_sommerfeld=0))
     lam = _kw()["wavelength"]
     with pytest.raises(ValueError):
         BSplineSolver(**_kw(ground_z=0.5 * lam))  # wire below the plane
