@@ -12,7 +12,18 @@ GOLDEN = os.path.join(ROOT, "tests", "golden")
 
 def pytest_configure(config):
     config.addinivalue_line("markers", "gpu: needs an MI355X (runs the HIP path)")
-    config.addinivalue_line("markers", "slow: long CPU-side check")
+    config.addinivalue_line("markers", "slow: long CPU-side check (run with MW_RUN_SLOW=1)")
+
+
+def pytest_collection_modifyitems(config, items):
+    """slow tests (minutes of numpy on the full 8x8 array) run on request:
+    MW_RUN_SLOW=1; the same golden vectors are checked on the GPU path."""
+    if os.environ.get("MW_RUN_SLOW"):
+        return
+    skip = pytest.mark.skip(reason="slow CPU check; set MW_RUN_SLOW=1 to run it")
+    for item in items:
+        if "slow" in item.keywords:
+            item.add_marker(skip)
 
 
 def golden(name):

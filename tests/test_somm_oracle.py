@@ -109,7 +109,6 @@ def test_grid_tables_and_eval_match_reference(case):
     assert _scale_err(ev, d["grid_eval"]) < 1e-12
 
 
-@pytest.mark.slow
 def test_far_zone_grid_matches_reference():
     d = golden("somm_grid_far")
     g = so.get_grid(complex(15, -20), K2, 6.0, OMEGA, so.MU0_EXACT)
@@ -155,7 +154,6 @@ def test_remainder_is_small_but_not_negligible():
     assert rel_err(y0, d["y"]) > 1e-6
 
 
-@pytest.mark.slow
 def test_c5_array_matches_reference():
     d = golden("c5_ground4x4")
     o = OracleSolver(**workloads.grounded_array(4, 4, 21, seed=4))
