@@ -27,6 +27,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 PEAK_FP64_TFLOPS = 78.6     # MI355X dense FP64 (vector and matrix)
+PEAK_FP32_TFLOPS = 157.3    # MI355X dense FP32 matrix
 METRIC = "port-solves/s & fill Gentries/s; mixed/learned vs FP64 GPU at Y rel err ≤1e-8"
 
 WORKLOADS = {
@@ -107,6 +108,13 @@ def sweep_main(args, wl):
         "gemm_variant": bs.gemm_variant,
         "solve_tflops_equiv": flops / dt / 1e12,
         "y_rel_err_vs_single_path_sampled": max(errs),
+        "roofline": {"kernel": "the whole sweep graph (batched fills, panel factors, MFMA "
+                               "updates and solves of every problem)",
+                     "bound": "mfma", "achieved": flops / dt / 1e12, "peak": PEAK_FP64_TFLOPS,
+                     "unit": "TFLOP/s", "frac": flops / dt / 1e12 / PEAK_FP64_TFLOPS,
+                     "traffic": None,
+                     "algorithmic": "sum over problems of (8/3) n^3 + 8 n^2 p "
+                                    f"= {flops:.3e} flop per sweep (fill not counted)"},
     }
     print(json.dumps(out))
 
@@ -172,6 +180,14 @@ def mixed_main(args, wl):
         "fp64_solve_tflops": (lu_flops(nb) + 8.0 * nb * nb * s64._port_count())
                              / (res["fp64"]["stage_ms"]["solve"] * 1e-3) / 1e12,
     }
+    # the mixed form's dominant kernel set: the FP32 factor + all-port solve
+    solve_flops = lu_flops(nb) + 8.0 * nb * nb * s64._port_count()
+    tf32 = solve_flops / (res["mixed"]["stage_ms"]["solve32"] * 1e-3) / 1e12
+    out["roofline"] = {"kernel": "mw_cgesv (FP32 factor + all-port solve on FP32 MFMA)",
+                       "bound": "mfma", "achieved": tf32, "peak": PEAK_FP32_TFLOPS,
+                       "unit": "TFLOP/s", "frac": tf32 / PEAK_FP32_TFLOPS, "traffic": None,
+                       "algorithmic": f"(8/3) n^3 + 8 n^2 p = {solve_flops:.3e} flop per solve "
+                                      f"(n={nb}, p={s64._port_count()})"}
     print(json.dumps(out))
 
 
