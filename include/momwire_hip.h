@@ -251,10 +251,15 @@ int mw_zsmall_solve(int32_t ns, int32_t nt, void* S, int64_t lds, void* T,
  * one structure at a time (bspline.py:4181-4439); per problem it computes
  * what compute_port_solution computes.  gemm_variant: 0 = 64x32 tiles,
  * 1 = 128x64 tiles for the batched updates (chosen by the autotuner).
+ * pivot_bound: the plan's own threshold-pivoting multiplier bound, meant as
+ * in mw_set_pivot_threshold (<= 0 pivots every panel).  32 (tau = 1/32)
+ * is the Python default: the unpivoted multipliers of Galerkin wire
+ * matrices stay in the tens, and a panel that falls back to the tournament
+ * holds up the next step of every problem in the batch.
  */
 int mw_sweep_plan_create(int32_t n_mesh, const mw_mesh* meshes, const mw_ports* ports,
                          int32_t nprob, const mw_sweep_prob* probs, int32_t gemm_variant,
-                         mw_sweep_plan** plan);
+                         double pivot_bound, mw_sweep_plan** plan);
 int mw_sweep_run(mw_sweep_plan* plan, double eps, double mu,
                  const volatile int32_t* cancel, void* stream);
 int mw_sweep_plan_destroy(mw_sweep_plan* plan);

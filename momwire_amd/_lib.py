@@ -114,7 +114,7 @@ SIGNATURES = {
     "mw_zsparse_rows": (ctypes.c_int, [_I64, _I32, _P, _P, _P, _I64, _I64, _P, _I64, _P]),
     "mw_zscatter_cols": (ctypes.c_int, [_I64, _I64, _I32, _P, _P, _P, _I64, _P]),
     "mw_zsmall_solve": (ctypes.c_int, [_I32, _I32, _P, _I64, _P, _I64, _P, _P]),
-    "mw_sweep_plan_create": (ctypes.c_int, [_I32, _P, _P, _I32, _P, _I32,
+    "mw_sweep_plan_create": (ctypes.c_int, [_I32, _P, _P, _I32, _P, _I32, _D,
                                             ctypes.POINTER(ctypes.c_void_p)]),
     "mw_sweep_run": (ctypes.c_int, [_P, _D, _D, _P, _P]),
     "mw_sweep_plan_destroy": (ctypes.c_int, [_P]),

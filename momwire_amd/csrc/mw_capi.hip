@@ -44,8 +44,11 @@ hipError_t mw_launch_batch_fill(const mw_sweep_prob* P, const mw_mesh* meshes, i
                                 int64_t n_items, double eps, double mu, hipStream_t st);
 int mw_sweep_lu_run(const mw_sweep_prob* dP, const mw_ports* dports, const mw_sweep_prob* hp,
                     int nprob, int max_ports, int max_con, int64_t max_scatter,
-                    int gemm_variant, const volatile int32_t* cancel, hipStream_t st,
+                    int gemm_variant, double lbound, const uint32_t* items,
+                    const int64_t* off, const volatile int32_t* cancel, hipStream_t st,
                     hipError_t* herr);
+void mw_sweep_work(const mw_sweep_prob* hp, int nprob, int gemm_variant,
+                   std::vector<uint32_t>& items, std::vector<int64_t>& off);
 hipError_t mw_launch_add_sparse(int64_t nnz, const int32_t* rows, const int32_t* cols,
                                 const void* vals, void* Z, int64_t ldz, hipStream_t st);
 hipError_t mw_launch_small_solve(int ns, int nt, void* S, int64_t lds, void* T, int64_t ldt,
@@ -403,11 +406,14 @@ int mw_c2z(int64_t rows, int64_t cols, const void* src, int64_t lds, void* dst, 
 struct mw_sweep_plan {
   int nprob = 0, n_mesh = 0, gemm_variant = 0, max_ports = 0, max_con = 0, max_nse = 0;
   int64_t max_scatter = 0, n_items = 0;
+  double pivot_bound = 32.0;
   mw_sweep_prob* dP = nullptr;
   mw_ports* dports = nullptr;
   mw_mesh* dmeshes = nullptr;
   int32_t* d_item_prob = nullptr;
   int32_t* d_item_pair = nullptr;
+  uint32_t* d_work = nullptr;      // per-step work lists of the factorisation
+  std::vector<int64_t> work_off;
   std::vector<mw_sweep_prob> hp;
 };
 
@@ -418,17 +424,20 @@ static void plan_free(mw_sweep_plan* p) {
   hipFree(p->dmeshes);
   hipFree(p->d_item_prob);
   hipFree(p->d_item_pair);
+  hipFree(p->d_work);
   delete p;
 }
 
 int mw_sweep_plan_create(int32_t n_mesh, const mw_mesh* meshes, const mw_ports* ports,
                          int32_t nprob, const mw_sweep_prob* probs, int32_t gemm_variant,
-                         mw_sweep_plan** out) {
+                         double pivot_bound, mw_sweep_plan** out) {
   g_err[0] = 0;
   if (!out) return fail(MW_EINVAL, "%splan output is NULL");
   *out = nullptr;
+  if (!(pivot_bound == pivot_bound)) return fail(MW_EINVAL, "%spivot threshold is NaN");
   if (n_mesh <= 0 || nprob <= 0 || !meshes || !ports || !probs)
     return fail(MW_EINVAL, "%sempty sweep");
+  if (nprob > 65536) return fail(MW_ERANGE, "%smore than 65536 problems in one sweep");
   for (int i = 0; i < n_mesh; ++i) {
     int rc = check_mesh(&meshes[i]);
     if (rc) return rc;
@@ -442,6 +451,7 @@ int mw_sweep_plan_create(int32_t n_mesh, const mw_mesh* meshes, const mw_ports* 
   p->nprob = nprob;
   p->n_mesh = n_mesh;
   p->gemm_variant = gemm_variant;
+  p->pivot_bound = pivot_bound;
   p->hp.assign(probs, probs + nprob);
   for (const auto& q : p->hp) {
     if (q.mesh < 0 || q.mesh >= n_mesh || q.n != meshes[q.mesh].n_basis ||
@@ -459,6 +469,8 @@ int mw_sweep_plan_create(int32_t n_mesh, const mw_mesh* meshes, const mw_ports* 
   }
   std::stable_sort(p->hp.begin(), p->hp.end(),
                    [](const mw_sweep_prob& a, const mw_sweep_prob& b) { return a.n > b.n; });
+  std::vector<uint32_t> work;
+  mw_sweep_work(p->hp.data(), nprob, gemm_variant, work, p->work_off);
   std::vector<int32_t> ip, iq;
   ip.reserve(p->n_items);
   iq.reserve(p->n_items);
@@ -477,7 +489,10 @@ int mw_sweep_plan_create(int32_t n_mesh, const mw_mesh* meshes, const mw_ports* 
       (e = hipMemcpy(p->dports, ports, sizeof(mw_ports) * n_mesh, hipMemcpyHostToDevice)) != hipSuccess ||
       (e = hipMemcpy(p->dmeshes, meshes, sizeof(mw_mesh) * n_mesh, hipMemcpyHostToDevice)) != hipSuccess ||
       (e = hipMemcpy(p->d_item_prob, ip.data(), sizeof(int32_t) * ip.size(), hipMemcpyHostToDevice)) != hipSuccess ||
-      (e = hipMemcpy(p->d_item_pair, iq.data(), sizeof(int32_t) * iq.size(), hipMemcpyHostToDevice)) != hipSuccess) {
+      (e = hipMemcpy(p->d_item_pair, iq.data(), sizeof(int32_t) * iq.size(), hipMemcpyHostToDevice)) != hipSuccess ||
+      (e = hipMalloc(&p->d_work, sizeof(uint32_t) * std::max<size_t>(work.size(), 1))) != hipSuccess ||
+      (!work.empty() &&
+       (e = hipMemcpy(p->d_work, work.data(), sizeof(uint32_t) * work.size(), hipMemcpyHostToDevice)) != hipSuccess)) {
     plan_free(p);
     return hip_status(e, "mw_sweep_plan_create");
   }
@@ -494,7 +509,8 @@ int mw_sweep_run(mw_sweep_plan* p, double eps, double mu, const volatile int32_t
                                          p->d_item_pair, p->n_items, eps, mu, st);
   if (herr != hipSuccess) return hip_status(herr, "sweep fill");
   int rc = mw_sweep_lu_run(p->dP, p->dports, p->hp.data(), p->nprob, p->max_ports, p->max_con,
-                           p->max_scatter, p->gemm_variant, cancel, st, &herr);
+                           p->max_scatter, p->gemm_variant, p->pivot_bound, p->d_work,
+                           p->work_off.data(), cancel, st, &herr);
   if (rc == MW_EHIP) return hip_status(herr, "sweep solve");
   if (rc == MW_EABORT) return fail(MW_EABORT, "%ssweep cancelled");
   return rc;

@@ -244,9 +244,13 @@ MW_DEV void stage_segment(const mw_mesh& M, int s, SegStage<NQ>& S) {
   S.ekl = M.ek ? M.ek_label[s] : -1;
 }
 
-template <bool SYM, int NQ>
-MW_DEV void fill_body(int ti, int tj, const mw_mesh& M, double k, double wmu, double inv_weps,
-                      double2* __restrict__ Z, int64_t ldz) {
+// `sym` is a run-time flag (a uniform branch at the store) rather than a
+// template parameter: the static LDS arrays below exist once per
+// instantiation, so a kernel holding both forms would carry twice the LDS
+// and lose half its occupancy.
+template <int NQ>
+MW_DEV void fill_body(bool sym, int ti, int tj, const mw_mesh& M, double k, double wmu,
+                      double inv_weps, double2* __restrict__ Z, int64_t ldz) {
   __shared__ SegStage<NQ> rs[MW_TSMAX], cs[MW_TSMAX];
   __shared__ double2 Bt[MW_TSMAX * MW_TSMAX * 9];
   __shared__ int nr_s, nc_s;
@@ -393,7 +397,7 @@ MW_DEV void fill_body(int ti, int tj, const mw_mesh& M, double k, double wmu, do
       }
     }
     Z[(int64_t)m * ldz + n] = make_double2(zr, zi);
-    if (SYM && ti != tj) Z[(int64_t)n * ldz + m] = make_double2(zr, zi);
+    if (sym && ti != tj) Z[(int64_t)n * ldz + m] = make_double2(zr, zi);
   }
 }
 
@@ -402,10 +406,10 @@ __global__ __launch_bounds__(256) void k_fill(mw_mesh M, double k, double wmu,
                                               double inv_weps, double2* __restrict__ Z,
                                               int64_t ldz, const int32_t* __restrict__ pairs) {
   if (SYM)
-    fill_body<true, NQ>(pairs[2 * blockIdx.x], pairs[2 * blockIdx.x + 1], M, k, wmu, inv_weps, Z,
-                        ldz);
+    fill_body<NQ>(true, pairs[2 * blockIdx.x], pairs[2 * blockIdx.x + 1], M, k, wmu, inv_weps, Z,
+                  ldz);
   else
-    fill_body<false, NQ>(blockIdx.y, blockIdx.x, M, k, wmu, inv_weps, Z, ldz);
+    fill_body<NQ>(false, blockIdx.y, blockIdx.x, M, k, wmu, inv_weps, Z, ldz);
 }
 
 // ---------------------------------------------------------------------------
@@ -429,12 +433,8 @@ __global__ __launch_bounds__(256) void kb_fill(const mw_sweep_prob* P, const mw_
   const int32_t* pairs = p.tile_pairs;
   const int q = item_pair[blockIdx.x];
   const double wmu = p.omega * mu, inv_weps = 1.0 / (p.omega * eps);
-  if (p.symmetric)
-    fill_body<true, 4>(pairs[2 * q], pairs[2 * q + 1], M, p.k, wmu, inv_weps,
-                       reinterpret_cast<double2*>(p.A), p.lda);
-  else
-    fill_body<false, 4>(pairs[2 * q], pairs[2 * q + 1], M, p.k, wmu, inv_weps,
-                        reinterpret_cast<double2*>(p.A), p.lda);
+  fill_body<4>(p.symmetric != 0, pairs[2 * q], pairs[2 * q + 1], M, p.k, wmu, inv_weps,
+               reinterpret_cast<double2*>(p.A), p.lda);
 }
 
 template <int NQ>

@@ -5,6 +5,7 @@
 //
 // Storage: row-major, leading dimension in complex elements.
 #include <algorithm>
+#include <vector>
 
 #include "mw_common.h"
 #include "mw_mfma.h"
@@ -986,8 +987,11 @@ MW_DEV void panel_pivot_body(int node0, typename cplx_t<T>::type* A, int64_t lda
       }
     }
   }
+  // final placement of the rows this panel moves: position pos_s[t] receives
+  // the row that stood at val_s[t] (a permutation of at most 2 bw rows)
+  __shared__ int32_t pos_s[2 * NB], val_s[2 * NB];
+  __shared__ int nt_s;
   if (tid == 0) {
-    __shared__ int32_t pos_s[2 * NB], val_s[2 * NB];
     int nt = 0;
     for (int i = 0; i < steps; ++i) {
       const int32_t g = L.rid[L.piv[i]];
@@ -1010,6 +1014,7 @@ MW_DEV void panel_pivot_body(int node0, typename cplx_t<T>::type* A, int64_t lda
       }
     }
     if (steps < bw && *info == 0) *info = (int32_t)(k0 + steps + 1);
+    nt_s = nt;
   }
   __syncthreads();
   wave_inverses(P);
@@ -1021,16 +1026,26 @@ MW_DEV void panel_pivot_body(int node0, typename cplx_t<T>::type* A, int64_t lda
   for (int e = tid; e < total; e += blockDim.x) ws.counter[e] = 0;
   // interchanges on columns [c_lo, nc): from the start of the current outer
   // block, so the block's multiplier columns end up in the block's final row
-  // order; nobody else reads A any more
+  // order; nobody else reads A any more.  The table applies them as one
+  // gather and one scatter per 32-column strip (a thread holds 8 of the
+  // strip's <= 64 x 32 entries between the two), not as bw dependent swaps.
   __syncthreads();
-  for (int64_t c = c_lo + tid; c < nc; c += blockDim.x) {
-    for (int i = 0; i < bw; ++i) {
-      const int64_t p1 = k0 + i, p2 = ipiv[k0 + i];
-      if (p2 != p1) {
-        const CT t = A[p1 * lda + c];
-        A[p1 * lda + c] = A[p2 * lda + c];
-        A[p2 * lda + c] = t;
-      }
+  const int nt = nt_s;
+  if (nt == 0) return;
+  for (int64_t c0 = c_lo; c0 < nc; c0 += 32) {
+    CT v[8];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      const int e = tid + 256 * j, t = e >> 5;
+      const int64_t c = c0 + (e & 31);
+      if (t < nt && c < nc) v[j] = A[(int64_t)val_s[t] * lda + c];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      const int e = tid + 256 * j, t = e >> 5;
+      const int64_t c = c0 + (e & 31);
+      if (t < nt && c < nc) A[(int64_t)pos_s[t] * lda + c] = v[j];
     }
   }
 }
@@ -1888,53 +1903,114 @@ __global__ void kb_rhs(const mw_sweep_prob* P, const mw_ports* ports, int scatte
   }
 }
 
-__global__ __launch_bounds__(256, 2) void kb_panel_nopiv(const mw_sweep_prob* P, int64_t k0,
-                                                      double lbound) {
-  const mw_sweep_prob& p = P[blockIdx.y];
+// Batched threshold check, in two launches.  The single-problem kernel lets
+// every workgroup refactor A11 so that no workgroup waits for another; over
+// thousands of problems that repeated 32-step chain is the whole cost, so
+// here one workgroup per problem factors A11 (storing L11\U11, the inverses
+// and identity interchanges, A11's own multipliers checked on the way) ...
+__global__ __launch_bounds__(256, 2) void kb_panel_a11(const mw_sweep_prob* P, int64_t k0,
+                                                    double lbound) {
+  const mw_sweep_prob& p = P[blockIdx.x];
   if (k0 >= p.n) return;
   const int bw = (int)((p.n - k0) < NB ? (p.n - k0) : NB);
-  const int nrb = (int)((p.n - k0 - bw + AW - 1) / AW);
-  if ((int)blockIdx.x > nrb) return;
   __shared__ PanelLds<double2> L;
-  panel_nopiv_body<double>(L, blockIdx.x, p.n, (const double2*)p.A, p.lda, k0, bw, (double2*)p.work,
+  panel_nopiv_body<double>(L, 0, p.n, (const double2*)p.A, p.lda, k0, bw, (double2*)p.work,
                            ws_inv(p.work, k0), p.ipiv, ws_flags(p.work, p.n) + k0 / NB, lbound);
 }
 
-// One workgroup per problem: problems whose panel passed the threshold test
-// (nearly all) exit at once; a flagged one runs its tournament leaves in
-// sequence -- the arrival counters make the last leaf of each group the
-// merger, exactly as when the leaves run in parallel.
-__global__ __launch_bounds__(256) void kb_panel_pivot(const mw_sweep_prob* P, int64_t k0) {
-  const mw_sweep_prob& p = P[blockIdx.x];
+// ... and then each workgroup forms 64 rows of L21 = A21 U11^{-1} from the
+// stored inverse (a product, no chain) and flags the panel if a multiplier
+// exceeds lbound or is not finite.  Nothing in A is written.
+__global__ __launch_bounds__(256) void kb_panel_check(const mw_sweep_prob* P, int64_t k0,
+                                                      double lbound, const uint32_t* items) {
+  const uint32_t it = items[blockIdx.x];
+  const mw_sweep_prob& p = P[it >> 16];
+  if (k0 >= p.n) return;
+  const int bw = (int)((p.n - k0) < NB ? (p.n - k0) : NB);
+  const int64_t r0 = k0 + bw + (int64_t)(it & 0xffff) * 64;
+  if (r0 >= p.n) return;
+  int32_t* flag = ws_flags(p.work, p.n) + k0 / NB;
+  if (*flag) return;  // A11 already failed: the tournament runs anyway
+  __shared__ double2 a[64][NB + 1];
+  __shared__ double2 u[NB][NB + 1];
+  const double2* A = (const double2*)p.A;
+  const double2* ui = ws_inv(p.work, k0) + NB * NB;
+  const int tid = threadIdx.x;
+  for (int e = tid; e < 64 * NB; e += blockDim.x) {
+    const int r = e >> 5, c = e & 31;
+    double2 v = make_double2(0.0, 0.0);
+    if (r0 + r < p.n && c < bw) v = A[(r0 + r) * p.lda + k0 + c];
+    a[r][c] = v;
+  }
+  for (int e = tid; e < NB * NB; e += blockDim.x) u[e >> 5][e & 31] = ui[e];
+  __syncthreads();
+  // thread: row tid / 4, columns 8 (tid % 4) .. + 7; U11^{-1} is upper
+  // triangular, so column c sums k = 0 .. c
+  const int r = tid >> 2, cb = (tid & 3) * 8;
+  bool over = false;
+  if (r0 + r < p.n) {
+    for (int c = cb; c < cb + 8 && c < bw; ++c) {
+      double sr = 0.0, si = 0.0;
+      for (int k = 0; k <= c; ++k) {
+        const double2 x = a[r][k], y = u[k][c];
+        sr += x.x * y.x - x.y * y.y;
+        si += x.x * y.y + x.y * y.x;
+      }
+      if (!(fabs(sr) + fabs(si) <= lbound)) over = true;
+    }
+  }
+  if (__any(over) && (tid & 63) == 0) atomicOr(flag, 1);
+}
+
+// One workgroup per (tournament leaf, problem): problems whose panel passed
+// the threshold test (nearly all) exit at once; the leaves of a flagged one
+// run side by side, and the arrival counters make the last leaf of each
+// group the merger, as in the single-problem factorisation.
+__global__ __launch_bounds__(256) void kb_panel_pivot(const mw_sweep_prob* P, int64_t k0,
+                                                      const uint32_t* items) {
+  const uint32_t it = items[blockIdx.x];
+  const mw_sweep_prob& p = P[it >> 16];
   if (k0 >= p.n) return;
   const int32_t* flag = ws_flags(p.work, p.n) + k0 / NB;
   if (*flag == 0) return;
   const int bw = (int)((p.n - k0) < NB ? (p.n - k0) : NB);
   const int nleaf = (int)((p.n - k0 + TROWS - 1) / TROWS);
-  for (int leaf = 0; leaf < nleaf; ++leaf) {
-    panel_pivot_body<double>(leaf, (double2*)p.A, p.lda, p.n, k0, p.n + p.ncols, k0, bw, nleaf,
-                             ws_tour(p.work, p.n), flag, (double2*)p.work, ws_inv(p.work, k0),
-                             p.ipiv, p.info);
-    __syncthreads();
-  }
+  const int leaf = (int)(it & 0xffff);
+  if (leaf >= nleaf) return;
+  panel_pivot_body<double>(leaf, (double2*)p.A, p.lda, p.n, k0, p.n + p.ncols, k0, bw, nleaf,
+                           ws_tour(p.work, p.n), flag, (double2*)p.work, ws_inv(p.work, k0),
+                           p.ipiv, p.info);
 }
 
-__global__ __launch_bounds__(256) void kb_panel_apply(const mw_sweep_prob* P, int64_t k0) {
-  const mw_sweep_prob& p = P[blockIdx.y];
+__global__ __launch_bounds__(256) void kb_panel_apply(const mw_sweep_prob* P, int64_t k0,
+                                                      const uint32_t* items) {
+  const uint32_t it = items[blockIdx.x];
+  const mw_sweep_prob& p = P[it >> 16];
   if (k0 >= p.n) return;
   const int bw = (int)((p.n - k0) < NB ? (p.n - k0) : NB);
   const int64_t nc = p.n + p.ncols;
   const int nrb = (int)((p.n - k0 - bw + AW - 1) / AW);
   const int ncb = (int)((nc - k0 - bw + AW - 1) / AW);
-  if ((int)blockIdx.x > nrb + ncb) return;
-  panel_apply_body<double>(blockIdx.x, p.n, nc, (double2*)p.A, p.lda, k0, bw, ws_inv(p.work, k0),
+  const int b = (int)(it & 0xffff);
+  if (b > nrb + ncb) return;
+  panel_apply_body<double>(b, p.n, nc, (double2*)p.A, p.lda, k0, bw, ws_inv(p.work, k0),
                            (const double2*)p.work, nrb, ncb);
 }
 
 // mode 0: trailing update after panel k0; mode 1: back-substitution update above panel k0
 template <bool BIG>
-__global__ __launch_bounds__(256) void kb_gemm(const mw_sweep_prob* P, int64_t k0, int mode) {
-  const mw_sweep_prob& p = P[blockIdx.z];
+// items (mode 0): one entry per workgroup, problem << 16 | tile row << 8 |
+// tile column; NULL (mode 1): grid (tile column, tile row, problem).
+__global__ __launch_bounds__(256) void kb_gemm(const mw_sweep_prob* P, int64_t k0, int mode,
+                                               const uint32_t* items) {
+  uint32_t pi = blockIdx.z, bx = blockIdx.x, by = blockIdx.y;
+  if (items) {
+    const uint32_t it = items[blockIdx.x];
+    pi = it >> 16;
+    by = (it >> 8) & 0xff;
+    bx = it & 0xff;
+  }
+  const mw_sweep_prob& p = P[pi];
   if (k0 >= p.n) return;
   const int bw = (int)((p.n - k0) < NB ? (p.n - k0) : NB);
   double2* A = reinterpret_cast<double2*>(p.A);
@@ -1956,12 +2032,11 @@ __global__ __launch_bounds__(256) void kb_gemm(const mw_sweep_prob* P, int64_t k
     c = A + p.n;
   }
   const int tm = BIG ? GBM : SBM, tn = BIG ? GBN : SBN;
-  if (m <= 0 || ncol <= 0 || (int64_t)blockIdx.y * tm >= m || (int64_t)blockIdx.x * tn >= ncol)
-    return;
+  if (m <= 0 || ncol <= 0 || (int64_t)by * tm >= m || (int64_t)bx * tn >= ncol) return;
   if (BIG)
-    gemm_big_body<double, false>(blockIdx.x, blockIdx.y, m, ncol, bw, a, lda, b, lda, c, lda);
+    gemm_big_body<double, false>(bx, by, m, ncol, bw, a, lda, b, lda, c, lda);
   else
-    gemm_small_body<double, false>(blockIdx.x, blockIdx.y, m, ncol, bw, a, lda, b, lda, c, lda);
+    gemm_small_body<double, false>(bx, by, m, ncol, bw, a, lda, b, lda, c, lda);
 }
 
 __global__ __launch_bounds__(256) void kb_bwd_apply(const mw_sweep_prob* P, int64_t k0) {
@@ -2429,12 +2504,58 @@ hipError_t mw_launch_c2z(int64_t rows, int64_t cols, const void* src, int64_t ld
 }
 
 // ---------------------------------------------------------------------------
+// Work lists of the batched factorisation.  A grid sized for the largest
+// problem would carry, at every panel step, workgroups that only find their
+// problem too small and exit (most of them, for uneven sizes); problem sizes
+// are fixed per plan, so the (problem, block) pairs that have work are listed
+// once on the host: per step, list 0 = threshold-check row blocks, 1 =
+// tournament leaves, 2 = panel-apply blocks, 3 = trailing-update tiles
+// (problem << 16 | tile row << 8 | tile column).  off[4 s + l] .. off[4 s + l
+// + 1] delimits list l of step s.  hp is sorted by n, largest first; needs
+// nprob <= 65536 and n <= 4096.
+// ---------------------------------------------------------------------------
+void mw_sweep_work(const mw_sweep_prob* hp, int nprob, int gemm_variant,
+                   std::vector<uint32_t>& items, std::vector<int64_t>& off) {
+  const int64_t nmax = hp[0].n;
+  const int64_t tm = gemm_variant ? GBM : SBM, tn = gemm_variant ? GBN : SBN;
+  items.clear();
+  off.clear();
+  for (int64_t k0 = 0; k0 < nmax; k0 += NB) {
+    for (int list = 0; list < 4; ++list) {
+      off.push_back((int64_t)items.size());
+      for (int i = 0; i < nprob && hp[i].n > k0; ++i) {
+        const int64_t n = hp[i].n, nc = n + hp[i].ncols;
+        const int64_t bw = (n - k0) < NB ? (n - k0) : NB;
+        const int64_t rest = n - k0 - bw;
+        const uint32_t key = (uint32_t)i << 16;
+        if (list == 0) {
+          for (int64_t b = 0; b < (rest + AW - 1) / AW; ++b) items.push_back(key | (uint32_t)b);
+        } else if (list == 1) {
+          for (int64_t l = 0; l < (n - k0 + TROWS - 1) / TROWS; ++l)
+            items.push_back(key | (uint32_t)l);
+        } else if (list == 2) {
+          const int64_t nblk = (rest + AW - 1) / AW + (nc - k0 - bw + AW - 1) / AW + 1;
+          for (int64_t b = 0; b < nblk; ++b) items.push_back(key | (uint32_t)b);
+        } else if (rest > 0) {
+          const int64_t ncol = nc - k0 - bw;
+          for (int64_t y = 0; y < (rest + tm - 1) / tm; ++y)
+            for (int64_t x = 0; x < (ncol + tn - 1) / tn; ++x)
+              items.push_back(key | (uint32_t)(y << 8) | (uint32_t)x);
+        }
+      }
+    }
+  }
+  off.push_back((int64_t)items.size());
+}
+
+// ---------------------------------------------------------------------------
 // Batched sweep driver.  `nact(k0)` = number of problems (sorted by n,
 // largest first) with n > k0; `hp` is the host copy of the sorted table.
 // ---------------------------------------------------------------------------
 int mw_sweep_lu_run(const mw_sweep_prob* dP, const mw_ports* dports, const mw_sweep_prob* hp,
                     int nprob, int max_ports, int max_con, int64_t max_scatter,
-                    int gemm_variant, const volatile int32_t* cancel, hipStream_t st,
+                    int gemm_variant, double lbound, const uint32_t* items,
+                    const int64_t* off, const volatile int32_t* cancel, hipStream_t st,
                     hipError_t* herr) {
   if (nprob <= 0) return MW_OK;
   const int64_t nmax = hp[0].n;
@@ -2457,27 +2578,26 @@ int mw_sweep_lu_run(const mw_sweep_prob* dP, const mw_ports* dports, const mw_sw
     }
     return lo;
   };
-  const double lbound = g_lbound;
   for (int64_t k0 = 0; k0 < nmax; k0 += NB) {
     if (cancel && *cancel) return MW_EABORT;
     const int na = nact(k0);
-    const int bw = (int)((nmax - k0) < NB ? (nmax - k0) : NB);
-    const int64_t rest = nmax - k0 - bw;
-    const int nrb = (int)((rest + AW - 1) / AW);
-    const int ncb = (int)((rest + maxcols + AW - 1) / AW);
-    hipLaunchKernelGGL(kb_panel_nopiv, dim3(nrb + 1, na), dim3(256), 0, st, dP, k0, lbound);
-    hipLaunchKernelGGL(kb_panel_pivot, dim3(na), dim3(256), 0, st, dP, k0);
-    hipLaunchKernelGGL(kb_panel_apply, dim3(nrb + ncb + 1, na), dim3(256), 0, st, dP, k0);
-    if (rest > 0) {
-      const int64_t ncol = rest + maxcols;
+    // the step's work lists: check, pivot, apply, update (mw_sweep_work)
+    const int64_t* o = off + 4 * (k0 / NB);
+    auto count = [&](int l) { return (unsigned)(o[l + 1] - o[l]); };
+    hipLaunchKernelGGL(kb_panel_a11, dim3(na), dim3(256), 0, st, dP, k0, lbound);
+    if (count(0))
+      hipLaunchKernelGGL(kb_panel_check, dim3(count(0)), dim3(256), 0, st, dP, k0, lbound,
+                         items + o[0]);
+    if (count(1))
+      hipLaunchKernelGGL(kb_panel_pivot, dim3(count(1)), dim3(256), 0, st, dP, k0, items + o[1]);
+    if (count(2))
+      hipLaunchKernelGGL(kb_panel_apply, dim3(count(2)), dim3(256), 0, st, dP, k0, items + o[2]);
+    if (count(3)) {
       if (gemm_variant)
-        hipLaunchKernelGGL(kb_gemm<true>, dim3((unsigned)((ncol + GBN - 1) / GBN),
-                                               (unsigned)((rest + GBM - 1) / GBM), na),
-                           dim3(256), 0, st, dP, k0, 0);
+        hipLaunchKernelGGL(kb_gemm<true>, dim3(count(3)), dim3(256), 0, st, dP, k0, 0, items + o[3]);
       else
-        hipLaunchKernelGGL(kb_gemm<false>, dim3((unsigned)((ncol + SBN - 1) / SBN),
-                                                (unsigned)((rest + SBM - 1) / SBM), na),
-                           dim3(256), 0, st, dP, k0, 0);
+        hipLaunchKernelGGL(kb_gemm<false>, dim3(count(3)), dim3(256), 0, st, dP, k0, 0,
+                           items + o[3]);
     }
   }
   const int64_t nlast = ((nmax + NB - 1) / NB - 1) * NB;
@@ -2489,11 +2609,11 @@ int mw_sweep_lu_run(const mw_sweep_prob* dP, const mw_ports* dports, const mw_sw
       if (gemm_variant)
         hipLaunchKernelGGL(kb_gemm<true>, dim3((unsigned)((maxcols + GBN - 1) / GBN),
                                                (unsigned)((k0 + GBM - 1) / GBM), na),
-                           dim3(256), 0, st, dP, k0, 1);
+                           dim3(256), 0, st, dP, k0, 1, (const uint32_t*)nullptr);
       else
         hipLaunchKernelGGL(kb_gemm<false>, dim3((unsigned)((maxcols + SBN - 1) / SBN),
                                                 (unsigned)((k0 + SBM - 1) / SBM), na),
-                           dim3(256), 0, st, dP, k0, 1);
+                           dim3(256), 0, st, dP, k0, 1, (const uint32_t*)nullptr);
     }
   }
   if (max_con > 0) {

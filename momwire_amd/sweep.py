@@ -13,6 +13,14 @@ bspline.py:4181-4439).
 The tile shape of the batched updates is chosen by a small autotuner: every
 candidate is timed on the sweep itself and the fastest is kept (per process,
 keyed by the sweep's size profile).
+
+Threshold pivoting keeps a panel's diagonal pivots while every multiplier
+stays within ``pivot_threshold`` (|re| + |im|); 32 by default here, against
+8 for one structure at a time.  In a batch, one problem whose panel falls
+back to the tournament holds up the next step of all of them, and the
+unpivoted multipliers of Galerkin wire matrices sit in the tens (on the
+sweep's random structures, 50 % of panels exceed 8 and 7 % exceed 32, with
+the same solution error).
 """
 
 import ctypes
@@ -30,13 +38,14 @@ _TUNED = {}
 class BatchedSweep:
     """All port solutions of ``structures`` x ``k_array`` on the GPU."""
 
-    def __init__(self, structures, k_array, gemm_variant=None, cancel=None):
+    def __init__(self, structures, k_array, gemm_variant=None, cancel=None, pivot_threshold=32.0):
         torch = _torch()
         lib = _lib.load()
         self.lib = lib
         self.torch = torch
         self.k_array = np.asarray(k_array, dtype=float)
         self.cancel = cancel
+        self.pivot_threshold = float(pivot_threshold)
         self.solvers = [BSplineSolver(**kw) for kw in structures]
         for i, s in enumerate(self.solvers):
             if s.n_qp_pair != 4:
@@ -128,7 +137,8 @@ class BatchedSweep:
             _lib.check(self.lib.mw_sweep_plan_create(len(self.solvers), ctypes.addressof(self._meshes),
                                                      ctypes.addressof(self._ports), len(self.problems),
                                                      ctypes.addressof(self._probs), int(variant),
-                                                     ctypes.byref(h)), "sweep plan")
+                                                     self.pivot_threshold, ctypes.byref(h)),
+                        "sweep plan")
             self._plan[variant] = h
         return self._plan[variant]
 

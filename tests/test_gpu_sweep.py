@@ -72,3 +72,25 @@ def test_single_path_matches_oracle_on_long_edges():
     sol = BSplineSolver(**kw).compute_port_solution()
     assert rel_err(sol.y, y_o) < 1e-10
     assert rel_err(sol.coeffs, x_o) < 1e-10
+
+
+@pytest.mark.parametrize("bound", [0.0, 8.0])
+def test_batched_sweep_pivot_thresholds(bound):
+    """bound 0 sends every panel of every problem through the tournament
+    (leaves of one problem side by side, two tree levels for the 827-unknown
+    structure); bound 8 is the single-structure default.  Either way each
+    problem's Y is the single-problem path's."""
+    from momwire_amd.sweep import BatchedSweep
+
+    idx = [89, 77, 4]
+    structs = [W.random_structure(3 * 100003 + i) for i in idx]
+    ks = W.sweep_wavenumbers()[[3, 12]]
+    _say("batched sweep, pivot bound", bound)
+    bs = BatchedSweep(structs, ks, gemm_variant=0, pivot_threshold=bound)
+    bs.run(graph=False)
+    torch.cuda.synchronize()
+    for (i, kk, y) in bs.y_matrices():
+        s = BSplineSolver(**structs[i])
+        s._set_k(kk)
+        assert rel_err(y, s.compute_port_solution().y) < 1e-12, (i, kk)
+    bs.close()

@@ -152,3 +152,14 @@ def test_current_slopes_match_spline_derivative(degree):
         fd = (hi[w] - lo[w]) / (2 * eps)
         mid_sl = s.current_slopes(c, [mids if i == w else [] for i in range(len(m.per_wire))])[w]
         assert np.abs(fd - mid_sl).max() <= 1e-6 * np.abs(mid_sl).max()
+
+
+def test_sweep_plan_rejects_nan_pivot_bound():
+    """The sweep plan refuses a NaN pivot threshold before touching anything
+    (so no GPU is needed)."""
+    if not os.path.exists(_lib.LIB_PATH):
+        pytest.skip("libmomwire_hip.so not built")
+    lib = _lib.load()
+    h = ctypes.c_void_p()
+    rc = lib.mw_sweep_plan_create(1, None, None, 1, None, 0, float("nan"), ctypes.byref(h))
+    assert rc == _lib.MW_EINVAL and b"NaN" in lib.mw_last_error() and not h.value
