@@ -327,6 +327,23 @@ int mw_somm_eval(const mw_somm_grid* grid, const double* R1, const double* theta
                  void* out, void* stream);
 
 /*
+ * Ground remainder of E at points in space for p sets of source moments:
+ * the reflected-wave combination of eqs 143-147 kept as a vector and summed
+ * over the sources, E_rem(point) = sum_s F(point, src_s) . p_s.  Replaces
+ * reflected_field_at's ("above", "above") regime (reference
+ * src/momwire/_field_point.py:292-331, with _reflected_wave_surfaces and
+ * _reflected_wave_combination at :145-233), batched over port columns.
+ * pts [n_pts][3], mid [n_src][3] (device); moment [n_src][ldm] complex with
+ * column c of component d at d * p + c; out [n_pts][ldo] likewise, written
+ * (accumulate = 0) or added to.  Only the remainder: the direct and
+ * C2-scaled image terms are the caller's, as in the reference.
+ */
+int mw_ground_fields(const mw_somm_grid* grid, double ground_z, double k, int32_t n_pts,
+                     const double* pts, int32_t n_src, const double* mid, const void* moment,
+                     int64_t ldm, int32_t p, int32_t accumulate, void* out, int64_t ldo,
+                     void* stream);
+
+/*
  * Grounded fill: Z -= C2 * Z_image + Q, accumulated into a Z already holding
  * the free-space matrix (row-major, ld ldz).  Z_image is the exact image
  * (mirrored sources, tangent dot t_m . (t_x, t_y, -t_z), charge weight 1);

@@ -121,6 +121,8 @@ SIGNATURES = {
     "mw_somm_six_integrals": (ctypes.c_int, [_D, _D, _D, _P, _P, _I64, _D, _I32, _P, _P, _P]),
     "mw_somm_surfaces": (ctypes.c_int, [_D, _D, _D, _D, _D, _P, _P, _I64, _D, _P, _P, _P]),
     "mw_somm_eval": (ctypes.c_int, [_P, _P, _P, _I64, _P, _P]),
+    "mw_ground_fields": (ctypes.c_int, [_P, _D, _D, _I32, _P, _I32, _P, _P, _I64, _I32, _I32,
+                                        _P, _I64, _P]),
     "mw_fill_ground": (ctypes.c_int, [ctypes.POINTER(MwMesh), _D, _D, _D, _D, _D, _D, _D, _P,
                                       _I32, _P, _I64, _P, _I32, _I32, _P]),
     "mw_fill_ground_refl": (ctypes.c_int, [ctypes.POINTER(MwMesh), _D, _D, _D, _D, _D, _D, _D, _D,

@@ -907,6 +907,27 @@ class BSplineSolver(_SweptPortSolutions):
                                      image_z=self.ground_z)
         return out.cpu().numpy() if as_numpy else out
 
+    def port_ground_remainder(self, points, magnetic=False, coeffs=None, subdiv: int = 8,
+                              as_numpy=True):
+        """The Sommerfeld ground's REMAINDER of E (or H, by the reference's
+        central-difference curl) at ``points`` for every port column, (n, 3,
+        p): ``reflected_field_at`` / ``reflected_h_field_at`` (reference
+        _field_point.py:292-432) fed with this solve's element moments at
+        ``subdiv`` elements per segment.  The full field over the ground is
+        E_direct + C2 E_image + this remainder, C2 = (eps~ - 1)/(eps~ + 1);
+        composing it is the caller's, as in the reference."""
+        from . import _ground, readout
+        if self.ground_model != "sommerfeld" or self.ground_eps is None:
+            raise ValueError("the ground remainder needs a Sommerfeld ground")
+        rm = self.readout_mesh(subdiv)
+        mom, _delta = readout.element_moments(rm, self._device_coeffs(coeffs))
+        eps_t = _ground.eps_tilde(self.ground_eps, self.omega, self.eps)
+        fn = readout.reflected_h_field_at if magnetic else readout.reflected_field_at
+        out = fn(points, rm.mid, mom, eps_t, self.ground_z, self.k, self.omega, mu=self.mu)
+        if as_numpy:
+            return out if isinstance(out, np.ndarray) else out.cpu().numpy()
+        return out
+
     def current_slopes(self, coeffs, s_array=None):
         """Per-wire dI/ds at the mesh knots or at s_array per wire: the
         B-spline sum's exact arc-length derivative (reference

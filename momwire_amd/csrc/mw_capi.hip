@@ -60,6 +60,10 @@ hipError_t mw_launch_somm_six(double er, double ei, double k2, const double* rho
 hipError_t mw_launch_somm_surfaces(double er, double ei, double k2, double omega, double mu,
                                    const double* R1, const double* th, int64_t n, double rtol,
                                    double2* out, const volatile int32_t* cancel, hipStream_t st);
+hipError_t mw_launch_ground_fields(const mw_somm_grid& G, double gz, double k, int n_pts,
+                                   const double* pts, int n_src, const double* mid,
+                                   const double2* mom, int64_t ldm, int p, int accumulate,
+                                   double2* out, int64_t ldo, hipStream_t st);
 hipError_t mw_launch_somm_eval(const mw_somm_grid& G, const double* R1, const double* th, int64_t n,
                                double2* out, hipStream_t st);
 hipError_t mw_launch_fill_ground(const mw_mesh& M, double k, double omega, double eps, double mu,
@@ -585,6 +589,25 @@ int mw_somm_eval(const mw_somm_grid* grid, const double* R1, const double* theta
   if (n > 0 && (!R1 || !theta || !out)) return fail(MW_EINVAL, "%spoint or output array is NULL");
   return hip_status(mw_launch_somm_eval(*grid, R1, theta, n, (double2*)out, (hipStream_t)stream),
                     "mw_somm_eval");
+}
+
+int mw_ground_fields(const mw_somm_grid* grid, double ground_z, double k, int32_t n_pts,
+                     const double* pts, int32_t n_src, const double* mid, const void* moment,
+                     int64_t ldm, int32_t p, int32_t accumulate, void* out, int64_t ldo,
+                     void* stream) {
+  g_err[0] = 0;
+  int rc = check_grid(grid);
+  if (rc) return rc;
+  if (n_pts < 0 || n_src < 0 || p < 1) return fail(MW_EINVAL, "%sbad point, source or column count");
+  if (!(k > 0.0)) return fail(MW_EINVAL, "%sk must be positive");
+  if (ldm < 3LL * p || ldo < 3LL * p)
+    return fail(MW_EINVAL, "%sldm %lld / ldo %lld below 3 p", "", ldm, ldo);
+  if ((n_pts > 0 && (!pts || !out)) || (n_src > 0 && (!mid || !moment)))
+    return fail(MW_EINVAL, "%sNULL array");
+  return hip_status(mw_launch_ground_fields(*grid, ground_z, k, n_pts, pts, n_src, mid,
+                                            (const double2*)moment, ldm, p, accumulate,
+                                            (double2*)out, ldo, (hipStream_t)stream),
+                    "mw_ground_fields");
 }
 
 int mw_fill_ground(const mw_mesh* mesh, double k, double omega, double eps, double mu,

@@ -624,6 +624,97 @@ __global__ __launch_bounds__(256) void k_somm_eval(mw_somm_grid G, const double*
 }
 
 // ---------------------------------------------------------------------------
+// Ground remainder of E at points in space (reference _field_point.py:145-233,
+// 292-331): the eqs 143-147 combination kept as a vector and summed over the
+// sources.  A complex moment p enters through cph = p_x dx + p_y dy and
+// sph = p_x dy - p_y dx (d = horizontal unit vector from the source's ground
+// projection to the point's; (1, 0) when they coincide, where
+// I_rho^H = -I_phi^H makes any direction serve).  One thread per point, GF_NCB
+// port columns per workgroup, sources staged through LDS.  Layouts as in the
+// near-field kernel: mom [n_src][ldm] with column c of component d at
+// d * p + c; out [n_pts][ldo] likewise.
+// ---------------------------------------------------------------------------
+constexpr int GF_NCB = 4, GF_NSRC = 64;
+
+struct GfLds {
+  double pos[GF_NSRC][3];
+  double2 val[GF_NSRC][3 * GF_NCB];
+};
+
+__global__ __launch_bounds__(256) void k_ground_fields(
+    mw_somm_grid G, double gz, double k, int n_pts, const double* __restrict__ pts, int n_src,
+    const double* __restrict__ mid, const double2* __restrict__ mom, int64_t ldm, int p,
+    int accumulate, double2* __restrict__ out, int64_t ldo) {
+  __shared__ GfLds L;
+  const int tid = threadIdx.x;
+  const int ip = blockIdx.x * 256 + tid;
+  const int col0 = blockIdx.y * GF_NCB;
+  const bool live = ip < n_pts;
+  double px = 0.0, py = 0.0, pz = 0.0;
+  if (live) { px = pts[3 * ip]; py = pts[3 * ip + 1]; pz = pts[3 * ip + 2]; }
+  cd acc[GF_NCB][3];
+#pragma unroll
+  for (int c = 0; c < GF_NCB; ++c)
+#pragma unroll
+    for (int d = 0; d < 3; ++d) acc[c][d] = cz(0.0, 0.0);
+  const double tiny = 1e-12 * G.r1_max;
+  for (int s0 = 0; s0 < n_src; s0 += GF_NSRC) {
+    __syncthreads();
+    for (int i = tid; i < GF_NSRC * 3 * GF_NCB; i += 256) {
+      const int s = i / (3 * GF_NCB), j = i % (3 * GF_NCB), d = j / GF_NCB, c = col0 + j % GF_NCB;
+      double2 v = make_double2(0.0, 0.0);
+      if (s0 + s < n_src && c < p) v = mom[(int64_t)(s0 + s) * ldm + (int64_t)d * p + c];
+      L.val[s][j] = v;
+    }
+    for (int i = tid; i < GF_NSRC * 3; i += 256)
+      L.pos[i / 3][i % 3] = (s0 + i / 3 < n_src) ? mid[3 * s0 + i] : 0.0;
+    __syncthreads();
+    if (!live) continue;
+    const int ns = min(GF_NSRC, n_src - s0);
+    for (int s = 0; s < ns; ++s) {
+      const double dx = px - L.pos[s][0], dy = py - L.pos[s][1];
+      const double rho = hypot(dx, dy);
+      const double hh = (pz - gz) + (L.pos[s][2] - gz);
+      const double r1 = sqrt(rho * rho + hh * hh);
+      cd f[4];
+      grid_eval(G, r1, atan2(hh, rho), f);
+      double gs, gc;
+      sincos(k * r1, &gs, &gc);
+      const cd g = cz(gc / r1, -gs / r1);
+      const bool safe = rho > tiny;
+      const double inv = safe ? 1.0 / rho : 0.0;
+      const double dhx = safe ? dx * inv : 1.0, dhy = safe ? dy * inv : 0.0;
+      const cd irv = zmul(g, f[0]), izv = zmul(g, f[1]), irh = zmul(g, f[2]), iph = zmul(g, f[3]);
+#pragma unroll
+      for (int c = 0; c < GF_NCB; ++c) {
+        const double2 mx = L.val[s][c], my = L.val[s][GF_NCB + c], mz = L.val[s][2 * GF_NCB + c];
+        const cd cph = cz(mx.x * dhx + my.x * dhy, mx.y * dhx + my.y * dhy);
+        const cd sph = cz(mx.x * dhy - my.x * dhx, mx.y * dhy - my.y * dhx);
+        const cd pzc = cz(mz.x, mz.y);
+        const cd e_rho = zadd(zmul(pzc, irv), zmul(cph, irh));
+        const cd e_phi = zmul(sph, iph);
+        const cd e_z = zsub(zmul(pzc, izv), zmul(cph, irv));
+        acc[c][0] = zadd(acc[c][0], zsub(zsc(e_rho, dhx), zsc(e_phi, dhy)));
+        acc[c][1] = zadd(acc[c][1], zadd(zsc(e_rho, dhy), zsc(e_phi, dhx)));
+        acc[c][2] = zadd(acc[c][2], e_z);
+      }
+    }
+  }
+  if (!live) return;
+#pragma unroll
+  for (int c = 0; c < GF_NCB; ++c) {
+    if (col0 + c >= p) break;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      double2* o = out + (int64_t)ip * ldo + (int64_t)d * p + col0 + c;
+      double2 v = acc[c][d];
+      if (accumulate) { v.x += o->x; v.y += o->y; }
+      *o = v;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
 // grounded fill: Z -= C2 * Z_image + Q
 // ---------------------------------------------------------------------------
 // NQ (image moments): the mesh's n_qp, a template parameter; the remainder
@@ -964,6 +1055,17 @@ hipError_t mw_launch_somm_eval(const mw_somm_grid& G, const double* R1, const do
   if (n <= 0) return hipSuccess;
   hipLaunchKernelGGL(k_somm_eval, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, G, R1, th,
                      n, out);
+  return hipGetLastError();
+}
+
+hipError_t mw_launch_ground_fields(const mw_somm_grid& G, double gz, double k, int n_pts,
+                                   const double* pts, int n_src, const double* mid,
+                                   const double2* mom, int64_t ldm, int p, int accumulate,
+                                   double2* out, int64_t ldo, hipStream_t st) {
+  if (n_pts <= 0 || p <= 0) return hipSuccess;
+  dim3 grid((unsigned)((n_pts + 255) / 256), (unsigned)((p + GF_NCB - 1) / GF_NCB));
+  hipLaunchKernelGGL(k_ground_fields, grid, dim3(256), 0, st, G, gz, k, n_pts, pts, n_src, mid,
+                     mom, ldm, p, accumulate, out, ldo);
   return hipGetLastError();
 }
 

@@ -14,7 +14,9 @@ solve, with the solution left where the solver put it:
   the elements arriving and leaving.
 * ``element_moments`` -> ``mw_element_moments``;
   ``far_moments`` -> ``mw_far_moments`` (FP64 MFMA, phase tiles generated in
-  LDS); ``element_fields`` -> ``mw_near_fields``.
+  LDS); ``element_fields`` -> ``mw_near_fields``;
+  ``reflected_field_at`` / ``reflected_h_field_at`` (the Sommerfeld ground
+  remainder at points, _field_point.py) -> ``mw_ground_fields``.
 
 ``far_moments`` and ``element_fields`` take the reference's arguments (one
 column: ``moment`` (n, 3), ``delta`` (n,)) or a trailing column axis
@@ -25,6 +27,7 @@ missing library or GPU raises.
 
 from __future__ import annotations
 
+import ctypes
 import math
 
 import numpy as np
@@ -287,3 +290,118 @@ def solver_ground(solver):
     g = _Finite()
     g.eps_r, g.sigma = eps_r, sigma
     return g
+
+
+# ---------------------------------------------------------------------------
+# Ground remainder at points in space (reference _field_point.py)
+# ---------------------------------------------------------------------------
+
+CURL_STEP_WAVELENGTHS = 1e-3   # _field_point.py:94
+_REGIMES = {
+    ("above", "above"): None,
+    ("below", "below"): "a source below the interface read at an observer below it",
+    ("below", "above"): "a source below the interface read at an observer above it",
+    ("above", "below"): "a source above the interface read at an observer below it",
+}
+
+
+def _check_regime(source_medium, observer_medium):
+    """The ("above", "above") reflected wave is the one regime with an
+    evaluator; the others raise ValueError naming themselves
+    (_field_point.py:266-289)."""
+    key = (source_medium, observer_medium)
+    if key not in _REGIMES:
+        raise ValueError(f"unknown ground regime {key!r}: source_medium and observer_medium "
+                         "must each be 'above' or 'below'")
+    if _REGIMES[key] is not None:
+        raise ValueError(f"the {key!r} ground regime -- {_REGIMES[key]} -- has no field "
+                         "evaluator here: its field is carried by the transmitted-field "
+                         "Sommerfeld surfaces, a different integral family. Only the "
+                         "('above', 'above') reflected-wave regime is implemented.")
+
+
+def ground_grid_for(points, src_mid, eps_t, ground_z, k, omega, mu=None, cancel=None,
+                    filler=None):
+    """(device master, physical view) of the Sommerfeld grid for these
+    observer/source pairs, sized as the reference sizes it: the largest
+    observer-to-image distance over points and sources together
+    (_field_point.py:122-142, _sommerfeld.py:1081-1116)."""
+    from . import _ground
+
+    mu = _ground.MU0_EXACT if mu is None else float(mu)
+    ex = np.concatenate([np.asarray(points, dtype=float).reshape(-1, 3),
+                         np.asarray(src_mid, dtype=float).reshape(-1, 3)])
+    r1 = _ground.max_image_distance(ex, ex, float(ground_z))
+    master = _ground.get_master(complex(eps_t), float(k), r1, cancel=cancel, filler=filler)
+    return master, master.struct(float(k), float(omega), mu)
+
+
+def reflected_field_at(points, src_mid, src_moment, eps_t, ground_z, k, omega, *,
+                       source_medium="above", observer_medium="above", mu=None, grid=None):
+    """The ground REMAINDER of E at ``points``: (n_points, 3) complex for
+    ``src_moment`` (n_src, 3), or (n_points, 3, p) for (n_src, 3, p) -- the
+    reference's ``reflected_field_at`` (_field_point.py:292-331) batched over
+    port columns, on the GPU (``mw_ground_fields``).  ``eps_t`` is the complex
+    relative permittivity (e^{+jwt}, Im <= 0), ``omega`` the radian frequency
+    the surfaces are normalised at, ``grid`` an optional (master, view) pair
+    from ``ground_grid_for`` so that related evaluations share one fill.
+    E(point) = E_direct + C2 E_image + this; composing is the caller's."""
+    _check_regime(source_medium, observer_medium)
+    torch = _torch()
+    lib = _lib.load()
+    pts_d, _ = _as_device(np.asarray(points, dtype=float).reshape(-1, 3)
+                          if not isinstance(points, torch.Tensor) else points, torch, torch.float64)
+    mid_d, _ = _as_device(src_mid, torch, torch.float64)
+    mom_d, on_dev = _as_device(src_moment, torch, torch.complex128)
+    single = mom_d.dim() == 2
+    if single:
+        mom_d = mom_d.unsqueeze(-1)
+    n_src, p, n_pts = int(mom_d.shape[0]), int(mom_d.shape[2]), int(pts_d.shape[0])
+    if tuple(mid_d.shape) != (n_src, 3) or mom_d.shape[1] != 3 or pts_d.shape[-1] != 3:
+        raise ValueError("expected points (n, 3), src_mid (s, 3), src_moment (s, 3[, p])")
+    if grid is None:
+        grid = ground_grid_for(pts_d.cpu().numpy(), mid_d.cpu().numpy(), eps_t, ground_z, k,
+                               omega, mu)
+    view = grid[1]
+    out = torch.empty((n_pts, 3, p), dtype=torch.complex128, device="cuda")
+    _lib.check(lib.mw_ground_fields(ctypes.byref(view), float(ground_z), float(k), n_pts,
+                                    pts_d.data_ptr(), n_src, mid_d.data_ptr(), mom_d.data_ptr(),
+                                    3 * p, p, 0, out.data_ptr(), 3 * p, _stream(torch)),
+               "ground remainder field")
+    if single:
+        out = out[..., 0]
+    return out if on_dev else out.cpu().numpy()
+
+
+def reflected_h_field_at(points, src_mid, src_moment, eps_t, ground_z, k, omega, *,
+                         source_medium="above", observer_medium="above", mu=None, step=None):
+    """The ground REMAINDER of H: curl E_rem / (-j omega mu) by central
+    differences at ``step`` (default 1e-3 wavelength), six evaluations of
+    ``reflected_field_at`` on ONE grid sized over all offset points, as the
+    reference does (_field_point.py:334-432)."""
+    from . import _ground
+
+    _check_regime(source_medium, observer_medium)
+    mu_v = _ground.MU0_EXACT if mu is None else float(mu)
+    pts = np.asarray(points, dtype=float).reshape(-1, 3)
+    if step is None:
+        step = CURL_STEP_WAVELENGTHS * (2.0 * math.pi / k)
+    offsets = []
+    for axis in range(3):
+        for sign in (1.0, -1.0):
+            shift = np.zeros(3)
+            shift[axis] = sign * step
+            offsets.append(pts + shift)
+    grid = ground_grid_for(np.concatenate(offsets), np.asarray(src_mid, dtype=float), eps_t,
+                           ground_z, k, omega, mu_v)
+    torch = _torch()
+    on_dev = isinstance(src_moment, torch.Tensor)
+    f = [reflected_field_at(o, src_mid, src_moment, eps_t, ground_z, k, omega, mu=mu_v, grid=grid)
+         for o in offsets]
+    if on_dev:
+        f = [x.cpu().numpy() for x in f]
+    d = [(f[2 * a] - f[2 * a + 1]) / (2.0 * step) for a in range(3)]
+    curl = np.stack([d[1][:, 2] - d[2][:, 1], d[2][:, 0] - d[0][:, 2], d[0][:, 1] - d[1][:, 0]],
+                    axis=1)
+    h = curl / (-1j * omega * mu_v)
+    return torch.from_numpy(h).cuda() if on_dev else h

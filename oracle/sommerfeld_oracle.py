@@ -8,6 +8,8 @@ manual eqs 136-172 as the reference implements them):
 * the four interpolation surfaces and their R1 -> 0 limits (:511-584),
 * the interpolation grid: layout, fill, 4x4 Lagrange evaluation, the cached
   normalised master and its rescaling (:642-914, 1016-1116),
+* the remainder field at points as a vector (_field_point.py:145-233) and its
+  central-difference curl (:334-432),
 * the projected remainder field (:940-1010) and the Galerkin remainder block Q
   (reference src/momwire/bspline.py:1791-1874, _accelerators.cpp:6081-6253),
 * the C2-scaled exact image (src/momwire/_ground_spec.py:90-125,
@@ -26,7 +28,13 @@ import it.  How it is pinned:
 * the R1 -> 0 limits by the limit of the direct evaluation at small R1;
 * the image coefficient by its eps -> 1 and eps -> infinity limits;
 * Q and the grounded Z and Y by the reference's golden values
-  (ground_dipole.npz, ground_tee.npz, c5_ground4x4.npz).
+  (ground_dipole.npz, ground_tee.npz, c5_ground4x4.npz);
+* the vector remainder at points by its projections: t_obs . E(p = t_src)
+  for unit Cartesian t_obs, t_src is the fill's remainder_proj (pinned
+  above), so the vector is fixed component by component (linearity in the
+  complex moment); and by the reference's own outputs (fieldpt_*.npz);
+* the curl by the free-space dipole: curl of the oracle's E equals
+  -j omega mu H of the same element fields (readout oracle).
 """
 
 import math
@@ -419,6 +427,47 @@ def remainder_proj(obs, t_obs, src, t_src, gz, k, grid):
     e_z = g * (tz[None, :] * IzV - th_s[None, :] * cphi * IrV)
     return (t_obs[:, 0][:, None] * (dhx * e_rho - dhy * e_phi)
             + t_obs[:, 1][:, None] * (dhy * e_rho + dhx * e_phi) + t_obs[:, 2][:, None] * e_z)
+
+
+def reflected_field_at(points, src_mid, src_moment, gz, k, grid):
+    """E remainder at points for complex moments (n_src, 3): the eqs 143-147
+    combination kept as a vector (reference _field_point.py:145-233).  The
+    horizontal unit vector from the source's ground projection to the point
+    falls back to (1, 0) when they coincide."""
+    points = np.asarray(points, dtype=float)
+    src_mid = np.asarray(src_mid, dtype=float)
+    p = np.asarray(src_moment, dtype=complex)
+    dx = points[:, 0][:, None] - src_mid[None, :, 0]
+    dy = points[:, 1][:, None] - src_mid[None, :, 1]
+    rho = np.hypot(dx, dy)
+    hh = (points[:, 2] - gz)[:, None] + (src_mid[None, :, 2] - gz)
+    r1 = np.sqrt(rho * rho + hh * hh)
+    IrV, IzV, IrH, IpH = grid.eval(r1.ravel(), np.arctan2(hh, rho).ravel()).reshape((4,) + r1.shape)
+    g = np.exp(-1j * k * r1) / r1
+    safe = rho > 1e-12 * grid.r1_max
+    inv = np.where(safe, 1.0 / np.where(safe, rho, 1.0), 0.0)
+    dhx = np.where(safe, dx * inv, 1.0)
+    dhy = np.where(safe, dy * inv, 0.0)
+    cph = p[None, :, 0] * dhx + p[None, :, 1] * dhy
+    sph = p[None, :, 0] * dhy - p[None, :, 1] * dhx
+    e_rho = g * (p[None, :, 2] * IrV + cph * IrH)
+    e_phi = g * sph * IpH
+    e_z = g * (p[None, :, 2] * IzV - cph * IrV)
+    return np.stack([np.sum(dhx * e_rho - dhy * e_phi, axis=1),
+                     np.sum(dhy * e_rho + dhx * e_phi, axis=1), np.sum(e_z, axis=1)], axis=1)
+
+
+def curl_fd(field, points, step):
+    """Central-difference curl of field(points) -> (n, 3) at ``step``
+    (the differencing of reference _field_point.py:408-432)."""
+    points = np.asarray(points, dtype=float)
+    d = []
+    for axis in range(3):
+        sh = np.zeros(3)
+        sh[axis] = step
+        d.append((field(points + sh) - field(points - sh)) / (2.0 * step))
+    return np.stack([d[1][:, 2] - d[2][:, 1], d[2][:, 0] - d[0][:, 2], d[0][:, 1] - d[1][:, 0]],
+                    axis=1)
 
 
 def remainder_block(geom, supp, polys, gz, k, grid, n_qp=3):

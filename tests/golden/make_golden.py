@@ -437,7 +437,36 @@ def case_nqs(nqs):
     return dict(y=sol.y, coeffs=sol.coeffs), dt
 
 
+def case_fieldpt():
+    """The Sommerfeld ground remainder at points in space (reference
+    src/momwire/_field_point.py:292-432): E by reflected_field_at and H by
+    reflected_h_field_at (central-difference curl) for seeded complex
+    moments on the grounded dipole's element midpoints, at points around it
+    (one directly above a source: the rho -> 0 branch)."""
+    from momwire import _field_point as F
+    from momwire import _ground_refl
+    t0 = time.time()
+    kw = workloads.grounded_array(1, 1, 21, params=(0.3, 12.0, 0.01, 10e6))
+    s = BSplineSolver(**kw)
+    eps_t = complex(_ground_refl.eps_tilde(s.ground_eps, s.omega, s.eps))
+    (p0, p1), = kw["wires"]
+    t = (np.arange(20) + 0.5) / 20
+    mid = np.asarray(p0)[None, :] + t[:, None] * (np.asarray(p1) - np.asarray(p0))[None, :]
+    rng = np.random.default_rng(21)
+    mom = rng.standard_normal((20, 3)) + 1j * rng.standard_normal((20, 3))
+    mom[:, 2] *= 0.5
+    pts = np.column_stack([rng.uniform(-15.0, 15.0, 24), rng.uniform(-10.0, 10.0, 24),
+                           rng.uniform(0.5, 15.0, 24)])
+    pts[0] = [mid[7, 0], mid[7, 1], 4.0]
+    e = F.reflected_field_at(pts, mid, mom, eps_t, s.ground_z, s.k, s.omega)
+    h = F.reflected_h_field_at(pts, mid, mom, eps_t, s.ground_z, s.k, s.omega)
+    return dict(points=pts, src_mid=mid, src_moment=mom, eps_t=np.array(eps_t),
+                ground_z=np.array(s.ground_z), k=np.array(s.k), omega=np.array(s.omega),
+                e=e, h=h), time.time() - t0
+
+
 CASES = {
+    "fieldpt_dipole": case_fieldpt,
     "nqs2_ground_tee": lambda: case_nqs(2),
     "nqs5_ground_tee": lambda: case_nqs(5),
     "nqp2_tee": lambda: case_nqp("free", 2),
