@@ -423,12 +423,12 @@ struct mw_sweep_plan {
 
 static void plan_free(mw_sweep_plan* p) {
   if (!p) return;
-  hipFree(p->dP);
-  hipFree(p->dports);
-  hipFree(p->dmeshes);
-  hipFree(p->d_item_prob);
-  hipFree(p->d_item_pair);
-  hipFree(p->d_work);
+  (void)hipFree(p->dP);
+  (void)hipFree(p->dports);
+  (void)hipFree(p->dmeshes);
+  (void)hipFree(p->d_item_prob);
+  (void)hipFree(p->d_item_pair);
+  (void)hipFree(p->d_work);
   delete p;
 }
 

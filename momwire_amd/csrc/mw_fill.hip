@@ -24,6 +24,7 @@ namespace {
 
 constexpr int NM = 3;       // moments per axis (degree 2)
 constexpr int NQ_MAX = 8;   // Gauss-Legendre points per segment per axis: 1 .. NQ_MAX
+static_assert(NQ_MAX == 8, "MW_NQ_SWITCH below instantiates NQ = 1 .. 8");
 constexpr double INV4PI = 0.079577471545947667884441881686257181017229822870228;
 
 // ---------------------------------------------------------------------------
@@ -253,14 +254,12 @@ MW_DEV void fill_body(bool sym, int ti, int tj, const mw_mesh& M, double k, doub
                       double inv_weps, double2* __restrict__ Z, int64_t ldz) {
   __shared__ SegStage<NQ> rs[MW_TSMAX], cs[MW_TSMAX];
   __shared__ double2 Bt[MW_TSMAX * MW_TSMAX * 9];
-  __shared__ int nr_s, nc_s;
 
   const int tid = threadIdx.x;
   const int nr = M.tile_nseg[ti], nc = M.tile_nseg[tj];
   if (tid < nr) stage_segment<NQ>(M, M.tile_seg[ti * MW_TSMAX + tid], rs[tid]);
   if (tid >= 64 && tid - 64 < nc)
     stage_segment<NQ>(M, M.tile_seg[tj * MW_TSMAX + tid - 64], cs[tid - 64]);
-  if (tid == 0) { nr_s = nr; nc_s = nc; }
   __syncthreads();
 
   const double2* se = reinterpret_cast<const double2*>(M.se_tab);

@@ -1070,7 +1070,6 @@ MW_DEV void panel_apply_body(int b, int64_t n, int64_t nc, typename cplx_t<T>::t
                              int64_t lda, int64_t k0, int bw,
                              const typename cplx_t<T>::type* inv,
                              const typename cplx_t<T>::type* lu11, int nrb, int ncb) {
-  typedef typename cplx_t<T>::type CT;
   __shared__ BlkLds<T> L;
   const int tid = threadIdx.x;
   if (b < nrb) {
@@ -2313,7 +2312,6 @@ static int solve_fwd(int64_t n, int64_t nrhs, const typename cplx_t<T>::type* A,
   typedef typename cplx_t<T>::type CT;
   const unsigned gx = (unsigned)((nrhs + AW - 1) / AW);
   const int64_t nbo = outer_block(n);
-  const bool one_level = nbo == NB;
   for (int64_t K0 = 0; K0 < n; K0 += nbo) {
     const int64_t kend = (n - K0) < nbo ? n : K0 + nbo;
     // a block's multiplier columns are stored in the block's final row order
