@@ -53,6 +53,11 @@ WORKLOADS = {
                             "moments M_theta, M_phi on a 1-degree grid (theta 0-180, phi "
                             "0-359: 65,160 directions), free space, FP64",
                        args=(8, 8, 41, 1), golden=None),
+    "groundfield4x4": dict(desc="Sommerfeld ground remainder of E (reflected_field_at) for all 16 "
+                                "port columns of the grounded 4x4 array (the ground4x4 structure, "
+                                "elements at 8 per segment: 2,688 sources) on a 64 x 64 grid of "
+                                "points 0.1-1.1 wavelength above the ground, FP64",
+                           args=None, golden=None),
     "sweep128": dict(desc="128 seeded random polyline structures (1-12 wires with shared "
                           "junction nodes, 43-827 unknowns) x 16 frequencies 0.8-1.2 x 300 MHz, "
                           "1-4 feeds, FP64; all fills and factorisations in one HIP graph",
@@ -412,6 +417,111 @@ def readout_main(args, wl):
     print(json.dumps(out))
 
 
+def groundfield_main(args, wl):
+    """Every port column of the grounded 4x4 solve read out as the Sommerfeld
+    remainder of E on a plane of points (mw_ground_fields), moments and grid
+    resident in HBM."""
+    import torch
+
+    from momwire_amd import _ground, readout, workloads
+    from momwire_amd.bspline import BSplineSolver
+    from oracle import sommerfeld_oracle as so
+
+    s = BSplineSolver(**workloads.grounded_array(4, 4, 21, seed=4))
+    s.compute_port_solution()
+    mid, mom, _n, _d = s.port_element_moments(subdiv=8)
+    lam = s.wavelength
+    gx = np.linspace(-0.5, 2.0, 64) * lam
+    gy = np.linspace(-0.5, 2.0, 64) * lam
+    xx, yy = np.meshgrid(gx, gy, indexing="ij")
+    zz = (0.1 + (np.arange(64 * 64) % 11) / 10.0) * lam
+    pts = np.column_stack([xx.ravel(), yy.ravel(), zz])
+    eps_t = _ground.eps_tilde(s.ground_eps, s.omega, s.eps)
+    grid = readout.ground_grid_for(pts, mid, eps_t, s.ground_z, s.k, s.omega, s.mu)
+    pts_d = torch.from_numpy(pts).cuda()
+    mid_d = torch.from_numpy(np.ascontiguousarray(mid)).cuda()
+    p, n_src, n_pts = int(mom.shape[2]), int(mid.shape[0]), len(pts)
+    print(f"[bench] {args.workload}: {n_pts} points, {n_src} sources, {p} columns; "
+          f"warmup {args.warmup}", file=sys.stderr, flush=True)
+
+    def step():
+        return readout.reflected_field_at(pts_d, mid_d, mom, eps_t, s.ground_z, s.k, s.omega,
+                                          mu=s.mu, grid=grid)
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+    ev[0].record()
+    for _ in range(args.steps):
+        out = step()
+    ev[1].record()
+    torch.cuda.synchronize()
+    ms = ev[0].elapsed_time(ev[1]) / args.steps
+    # parity of a sample of points and columns against the oracle on the
+    # device grid's own tables
+    m = grid[0]
+    vals = m.vals.cpu().numpy()
+    tabs = [vals[o:o + a * b].reshape(a, b, 4).transpose(2, 0, 1)
+            for (a, b), o in zip(m.shapes, m.off)]
+    og = so.Grid(m.regions, (m.r1_max, m.r_break, m.r_near, m.r0_fill), tabs, so.K2_REF,
+                 so.K2_REF * so.C_LIGHT, so.MU0_EXACT, m.eps_t).scaled(s.k, s.omega, s.mu)
+    sel = np.arange(0, n_pts, 97)
+    out_h = out.cpu().numpy()
+    mom_h = mom.cpu().numpy()
+    errs = []
+    for c in (0, 7, 15):
+        ref = so.reflected_field_at(pts[sel], mid, mom_h[:, :, c], s.ground_z, s.k, og)
+        errs.append(np.abs(out_h[sel, :, c] - ref).max() / np.abs(ref).max())
+    # interpolation 16 stencil nodes x 4 surfaces x complex FMA (256 flop) per
+    # point-source pair, plus 40 flop per column for the combination
+    flops = n_pts * n_src * (256 + 40 * p)
+    tf = flops / (ms * 1e-3) / 1e12
+    res = {
+        "metric": "Sommerfeld ground-remainder near field: port field maps/s",
+        "value": p / (ms * 1e-3),
+        "unit": "port field maps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": wl["desc"], "points": n_pts, "sources": n_src, "columns": p},
+        "field_rel_err_vs_oracle": float(max(errs)),
+        # FP64 vector work, priced against the FP64 compute peak (vector and
+        # matrix are both 78.6 TFLOP/s on MI355X)
+        "roofline": {"kernel": "k_ground_fields", "bound": "mfma", "achieved": tf,
+                     "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s", "frac": tf / PEAK_FP64_TFLOPS,
+                     "traffic": None,
+                     "algorithmic": f"({n_pts} points x {n_src} sources) x (256 + 40 x {p}) "
+                                    f"= {flops:.3e} flop per launch"},
+    }
+    if not args.no_cpu:
+        print("[bench] CPU baseline leg", file=sys.stderr, flush=True)
+        t0 = time.perf_counter()
+        reps = 0
+        chunk = pts[:64]
+        while True:
+            so.reflected_field_at(chunk, mid, mom_h[:, :, reps % p], s.ground_z, s.k, og)
+            reps += 1
+            if time.perf_counter() - t0 > 15.0:
+                break
+        per_chunk = (time.perf_counter() - t0) / reps
+        cores = len(os.sched_getaffinity(0))
+        if os.environ.get("OMP_NUM_THREADS"):
+            cores = min(cores, int(os.environ["OMP_NUM_THREADS"]))
+        res["cpu_baseline"] = {"value": 1.0 / (per_chunk * n_pts / 64), "unit": "port field maps/s",
+                               "cores": cores, "kind": "port",
+                               "sample": f"{reps} single-column evaluations at 64 points (numpy: "
+                                         f"the reference's reflected_field_at restated), "
+                                         f"scaled to {n_pts} points"}
+    print(json.dumps(res))
+
+
 def fill_flops(n_seg, n_b):
     """Algorithmic flops of the free-space fill: 16 quadrature points x 53 flops
     per segment pair plus 726 flops per Z entry (the reference's moment and
@@ -515,6 +625,8 @@ def main():
         return ground_main(args, wl)
     if args.workload == "readout8x8":
         return readout_main(args, wl)
+    if args.workload == "groundfield4x4":
+        return groundfield_main(args, wl)
     if wl["golden"] is None:
         return mixed_main(args, wl)
 

@@ -629,16 +629,22 @@ __global__ __launch_bounds__(256) void k_somm_eval(mw_somm_grid G, const double*
 // sources.  A complex moment p enters through cph = p_x dx + p_y dy and
 // sph = p_x dy - p_y dx (d = horizontal unit vector from the source's ground
 // projection to the point's; (1, 0) when they coincide, where
-// I_rho^H = -I_phi^H makes any direction serve).  One thread per point, GF_NCB
-// port columns per workgroup, sources staged through LDS.  Layouts as in the
+// I_rho^H = -I_phi^H makes any direction serve).  A workgroup takes GF_PTS
+// points and GF_NCB port columns; its threads split each LDS stage of sources
+// GF_SPLIT ways (thread: point tid % GF_PTS, sources tid / GF_PTS + GF_SPLIT
+// i) and the partial sums are added in a fixed order at the end, so a launch
+// has GF_SPLIT times the threads of one thread per point and the result does
+// not depend on scheduling.  Layouts as in the
 // near-field kernel: mom [n_src][ldm] with column c of component d at
 // d * p + c; out [n_pts][ldo] likewise.
 // ---------------------------------------------------------------------------
-constexpr int GF_NCB = 4, GF_NSRC = 64;
+constexpr int GF_NCB = 4, GF_NSRC = 64, GF_PTS = 64, GF_SPLIT = 4;
+static_assert(GF_PTS * GF_SPLIT == 256, "one 256-thread workgroup");
 
 struct GfLds {
   double pos[GF_NSRC][3];
   double2 val[GF_NSRC][3 * GF_NCB];
+  double2 red[GF_SPLIT][GF_PTS][3];
 };
 
 __global__ __launch_bounds__(256) void k_ground_fields(
@@ -646,8 +652,8 @@ __global__ __launch_bounds__(256) void k_ground_fields(
     const double* __restrict__ mid, const double2* __restrict__ mom, int64_t ldm, int p,
     int accumulate, double2* __restrict__ out, int64_t ldo) {
   __shared__ GfLds L;
-  const int tid = threadIdx.x;
-  const int ip = blockIdx.x * 256 + tid;
+  const int tid = threadIdx.x, lane = tid % GF_PTS, w = tid / GF_PTS;
+  const int ip = blockIdx.x * GF_PTS + lane;
   const int col0 = blockIdx.y * GF_NCB;
   const bool live = ip < n_pts;
   double px = 0.0, py = 0.0, pz = 0.0;
@@ -671,7 +677,7 @@ __global__ __launch_bounds__(256) void k_ground_fields(
     __syncthreads();
     if (!live) continue;
     const int ns = min(GF_NSRC, n_src - s0);
-    for (int s = 0; s < ns; ++s) {
+    for (int s = w; s < ns; s += GF_SPLIT) {
       const double dx = px - L.pos[s][0], dy = py - L.pos[s][1];
       const double rho = hypot(dx, dy);
       const double hh = (pz - gz) + (L.pos[s][2] - gz);
@@ -700,16 +706,26 @@ __global__ __launch_bounds__(256) void k_ground_fields(
       }
     }
   }
-  if (!live) return;
+  // the waves' partial sums, column by column, added in wave order
 #pragma unroll
   for (int c = 0; c < GF_NCB; ++c) {
-    if (col0 + c >= p) break;
+    __syncthreads();
 #pragma unroll
-    for (int d = 0; d < 3; ++d) {
-      double2* o = out + (int64_t)ip * ldo + (int64_t)d * p + col0 + c;
-      double2 v = acc[c][d];
-      if (accumulate) { v.x += o->x; v.y += o->y; }
-      *o = v;
+    for (int d = 0; d < 3; ++d) L.red[w][lane][d] = acc[c][d];
+    __syncthreads();
+    if (w == 0 && live && col0 + c < p) {
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        double2 v = L.red[0][lane][d];
+#pragma unroll
+        for (int q = 1; q < GF_SPLIT; ++q) {
+          v.x += L.red[q][lane][d].x;
+          v.y += L.red[q][lane][d].y;
+        }
+        double2* o = out + (int64_t)ip * ldo + (int64_t)d * p + col0 + c;
+        if (accumulate) { v.x += o->x; v.y += o->y; }
+        *o = v;
+      }
     }
   }
 }
@@ -1063,8 +1079,9 @@ hipError_t mw_launch_ground_fields(const mw_somm_grid& G, double gz, double k, i
                                    const double2* mom, int64_t ldm, int p, int accumulate,
                                    double2* out, int64_t ldo, hipStream_t st) {
   if (n_pts <= 0 || p <= 0) return hipSuccess;
-  dim3 grid((unsigned)((n_pts + 255) / 256), (unsigned)((p + GF_NCB - 1) / GF_NCB));
-  hipLaunchKernelGGL(k_ground_fields, grid, dim3(256), 0, st, G, gz, k, n_pts, pts, n_src, mid,
+  dim3 grid((unsigned)((n_pts + GF_PTS - 1) / GF_PTS), (unsigned)((p + GF_NCB - 1) / GF_NCB));
+  hipLaunchKernelGGL(k_ground_fields, grid, dim3(GF_PTS * GF_SPLIT), 0, st, G, gz, k, n_pts, pts,
+                     n_src, mid,
                      mom, ldm, p, accumulate, out, ldo);
   return hipGetLastError();
 }
