@@ -2317,17 +2317,25 @@ static int solve_fwd(int64_t n, int64_t nrhs, const typename cplx_t<T>::type* A,
     // a block's multiplier columns are stored in the block's final row order
     hipLaunchKernelGGL(k_laswp_rows<T>, dim3((unsigned)((nrhs + 255) / 256)), dim3(256), 0, st, X,
                        ldx, nrhs, ipiv, K0, kend);
+    // Rows inside the outer block take each panel's K = NB update; the rows
+    // below it, one K = (kend - K0) update after the block, in the same row
+    // order (the next block's interchanges come after it).
     for (int64_t k0 = K0; k0 < kend; k0 += NB) {
       const int bw = (int)((kend - k0) < NB ? (kend - k0) : NB);
       const CT* inv = reinterpret_cast<const CT*>(panel_inv(const_cast<void*>(work), k0));
       hipLaunchKernelGGL(k_solve_apply<T>, dim3(gx), dim3(256), 0, st, nrhs, inv, k0, bw, X, ldx,
                          1);
-      const int64_t rest = n - k0 - bw;
-      if (rest > 0) {
-        *herr = launch_gemm_sub<T>(rest, nrhs, bw, A + (k0 + bw) * lda + k0, lda, X + k0 * ldx,
+      const int64_t inner = kend - k0 - bw;
+      if (inner > 0) {
+        *herr = launch_gemm_sub<T>(inner, nrhs, bw, A + (k0 + bw) * lda + k0, lda, X + k0 * ldx,
                                    ldx, X + (k0 + bw) * ldx, ldx, st);
         if (*herr != hipSuccess) return MW_EHIP;
       }
+    }
+    if (n > kend) {
+      *herr = launch_gemm_sub<T>(n - kend, nrhs, kend - K0, A + kend * lda + K0, lda,
+                                 X + K0 * ldx, ldx, X + kend * ldx, ldx, st);
+      if (*herr != hipSuccess) return MW_EHIP;
     }
   }
   return MW_OK;
@@ -2355,13 +2363,28 @@ static int solve_bwd(int64_t n, int64_t nrhs, const typename cplx_t<T>::type* A,
                                  ldx, st);
       if (*herr != hipSuccess) return MW_EHIP;
     }
+    // Two levels, as the factorisation: rows inside the current outer block
+    // take each pair's K = 64 update; the rows above the block, one K = nbo
+    // update once the block is solved (nbo = NB: every pair updates all rows
+    // above, as one level).
+    const int64_t nbo = outer_block(n);
+    const int64_t pairs_end = (npan % 2 == 1) ? klone : n;
     for (int64_t i = npair - 1; i >= 0; --i) {
       const int64_t kq = 2 * i * NB;
       const int bwp = (int)((n - kq - NB) < NB ? (n - kq - NB) : NB);
       hipLaunchKernelGGL(k_bwd2_pair<T>, dim3(gx), dim3(256), 0, st, kq, bwp, nrhs, A, lda, work,
                          X, ldx);
-      *herr = launch_gemm_sub<T>(kq, nrhs, NB + bwp, A + kq, lda, X + kq * ldx, ldx, X, ldx, st);
-      if (*herr != hipSuccess) return MW_EHIP;
+      const int64_t lo = nbo > NB ? (kq / nbo) * nbo : 0;
+      if (kq > lo) {
+        *herr = launch_gemm_sub<T>(kq - lo, nrhs, NB + bwp, A + lo * lda + kq, lda, X + kq * ldx,
+                                   ldx, X + lo * ldx, ldx, st);
+        if (*herr != hipSuccess) return MW_EHIP;
+      }
+      if (kq == lo && lo > 0) {
+        const int64_t hi = (lo + nbo) < pairs_end ? lo + nbo : pairs_end;
+        *herr = launch_gemm_sub<T>(lo, nrhs, hi - lo, A + lo, lda, X + lo * ldx, ldx, X, ldx, st);
+        if (*herr != hipSuccess) return MW_EHIP;
+      }
     }
     *herr = hipGetLastError();
     return *herr == hipSuccess ? MW_OK : MW_EHIP;

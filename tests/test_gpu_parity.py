@@ -83,11 +83,13 @@ def test_lu_solve_random(n, p):
 
 
 @pytest.mark.parametrize("dtype", ["z", "c"])
-@pytest.mark.parametrize("n,p", [(1100, 300), (1024, 257), (33, 260)])
+@pytest.mark.parametrize("n,p", [(1100, 300), (1024, 257), (33, 260), (4500, 300), (4160, 260)])
 def test_lu_solve_many_rhs(dtype, n, p):
     """More than 256 right-hand sides: the back substitution forms each panel
     pair's solution once and updates the rows above with a plain GEMM (odd and
-    even panel counts, a ragged last panel)."""
+    even panel counts, a ragged last panel); past n = 4096 both triangular
+    solves run two levels (rows inside a 512-column block per panel, the rest
+    once per block)."""
     _say("lu many rhs", dtype, n, p)
     rng = np.random.default_rng(n + p)
     M = rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))
