@@ -9,7 +9,7 @@ trained on the GPU by the project's own forward/backward/Adam kernels from
 samples of the exact HIP engine, and it fills grid nodes inside its trained
 domain; the engine fills every other node (R1 = 0 limits, nodes beyond the
 trained extent, grounds outside the trained eps range), so the exact engine
-is both the fallback and the judge.
+is both the fallback and the yardstick of its error.
 
 There is no counterpart in the reference.  Use:
 

@@ -126,7 +126,7 @@ def test_engine_fallback_outside_trained_domain():
 @pytest.mark.parametrize("seed", [4, 5, 6])
 def test_trained_surrogate_gain(seed):
     """Grounded 4x4 arrays: the learned grid against the exact engine (the
-    judge), and against the method without it -- the C2 image alone."""
+    yardstick), and against the method without it -- the C2 image alone."""
     import time
 
     _say("trained surrogate, array seed", seed)

@@ -64,7 +64,7 @@ def test_single_path_matches_reference_sweep_golden():
 
 def test_single_path_matches_oracle_on_long_edges():
     """Where the reference is not self-consistent, the oracle (static moments
-    pinned to mpmath) is the judge at 1e-10."""
+    pinned to mpmath) is the yardstick at 1e-10."""
     from oracle.bspline_oracle import OracleSolver
 
     kw = W.random_structure(3 * 100003 + 4)
