@@ -1943,12 +1943,15 @@ __global__ __launch_bounds__(256) void kb_panel_check(const mw_sweep_prob* P, in
   }
   for (int e = tid; e < NB * NB; e += blockDim.x) u[e >> 5][e & 31] = ui[e];
   __syncthreads();
-  // thread: row tid / 4, columns 8 (tid % 4) .. + 7; U11^{-1} is upper
+  // thread: row tid / 4, columns tid % 4 + 4 j (interleaved, so the four
+  // threads of a row share the triangle's work evenly); U11^{-1} is upper
   // triangular, so column c sums k = 0 .. c
-  const int r = tid >> 2, cb = (tid & 3) * 8;
+  const int r = tid >> 2;
   bool over = false;
   if (r0 + r < p.n) {
-    for (int c = cb; c < cb + 8 && c < bw; ++c) {
+    for (int j = 0; j < 8; ++j) {
+      const int c = (tid & 3) + 4 * j;
+      if (c >= bw) break;
       double sr = 0.0, si = 0.0;
       for (int k = 0; k <= c; ++k) {
         const double2 x = a[r][k], y = u[k][c];
