@@ -1,18 +1,26 @@
 """Port-solve throughput of the MI355X B-spline path.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload array8x8]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload array32x32]
 
-One step = one full port solution of the workload held on the GPU: same-edge
-table, Galerkin fill of Z, LU factorisation, the all-port solve, KCL Schur
-step (when junctions exist) and the Y readout, leaving Y and the coefficient
-columns in HBM.  Geometry tables are uploaded once before timing, as the
-reference caches geometry and basis across solves.
+Default workload (the headline): the 32 x 32 dipole array, 15,360 unknowns
+and 1,024 ports, solved by the mixed form (FP64 fill, FP32 factor and
+all-port solve on MFMA, FP64-residual refinement, adjoint-corrected readout)
+against the FP64 GPU path.  One step = one full port solution held on the GPU:
+same-edge table, Galerkin fill of Z, factorisation, the all-port solve,
+refinement and the Y readout, leaving Y and the coefficient columns in HBM.
+Geometry tables are uploaded once before timing, as the reference caches
+geometry and basis across solves.  Beside it: the FP64 rate, their ratio, the
+Y error per correction, the graph-replay rate and the compute_port_solution()
+rate (Y and coeffs copied to the host).
 
-Prints ONE JSON line: the headline port-solves/s, the fill's Gentries/s, the
-per-stage split measured with HIP events on the launch stream inside the timed
-region, the roofline of the dominant kernel, the Y error against the
-reference's golden vectors, and a CPU baseline (the reference's own compiled
-accelerator + LAPACK on this host's cores, on a bounded sample).
+Other workloads: array8x8 (8 x 8 array, FP64, against the reference's golden
+Y), sweep128, ground4x4, readout8x8, groundfield4x4.
+
+Prints ONE JSON line: port-solves/s, the fill's Gentries/s, the per-stage
+split measured with HIP events on the launch stream inside the timed region,
+the roofline of the dominant kernel, the Y error, and a CPU baseline (the
+reference's own compiled accelerator + LAPACK on this host's cores, on a
+bounded sample).
 """
 
 import argparse
@@ -124,9 +132,34 @@ def sweep_main(args, wl):
     print(json.dumps(out))
 
 
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def host_cores():
+    cores = len(os.sched_getaffinity(0))
+    omp = os.environ.get("OMP_NUM_THREADS")
+    return min(cores, int(omp)) if omp else cores
+
+
+def stage_means(timers):
+    return {k: float(np.mean([a.elapsed_time(b) for a, b in v])) for k, v in timers.items()}
+
+
 def mixed_main(args, wl):
-    """Mixed precision vs the FP64 GPU path on one structure: solves/s of
-    each, and the mixed form's Y error against FP64 after every correction."""
+    """The headline: the 32x32 array solved by the mixed form against the FP64
+    GPU path.  Timed region: K eager mixed solves, each stage bracketed by HIP
+    events on the launch stream (the roofline's kernel durations come from
+    these events, inside the timed region); then the same for FP64; then the
+    graph-replay and the compute_port_solution() (Y and coeffs copied to the
+    host) rates of both, and the mixed form's Y error against FP64 after
+    every correction."""
     import torch
 
     from momwire_amd import workloads
@@ -137,35 +170,62 @@ def mixed_main(args, wl):
     smx = BSplineSolver(**kw, precision="mixed", refine_steps=args.refine_steps)
     mesh = s64._build()
     nb, ns = mesh.n_basis, mesh.n_seg
-    print(f"[bench] {args.workload}: {nb} unknowns, {s64._port_count()} ports", file=sys.stderr,
-          flush=True)
+    npt = s64._port_count()
+    print(f"[bench] {args.workload}: {nb} unknowns, {npt} ports; warmup {args.warmup}",
+          file=sys.stderr, flush=True)
     res = {}
-    for name, solver, fn in (("fp64", s64, s64._solve_device), ("mixed", smx, smx._solve_device_mixed)):
+    for name, solver, fn in (("mixed", smx, smx._solve_device_mixed),
+                             ("fp64", s64, s64._solve_device)):
         for _ in range(args.warmup):
             fn()
         torch.cuda.synchronize()
+        timers = {}
         t0 = time.perf_counter()
         for _ in range(args.steps):
-            fn()
+            fn(timers=timers)
         torch.cuda.synchronize()
         dt = time.perf_counter() - t0
-        timers = {}
-        fn(timers=timers)
-        torch.cuda.synchronize()
-        stage = {k: float(np.mean([a.elapsed_time(b) for a, b in v])) for k, v in timers.items()}
+        stage = stage_means(timers)
         res[name] = dict(rate=args.steps / dt, ms=1e3 * dt / args.steps, stage_ms=stage)
-        print(f"[bench] {name}: {1e3 * dt / args.steps:.1f} ms per solve {stage}", file=sys.stderr,
-              flush=True)
-    Y64 = s64._solve_device()[0].cpu().numpy()
-    hist = []
-    smx._solve_device_mixed(history=hist)
-    ynorm = np.abs(Y64).max()
-    per_step = [float(np.abs(h.cpu().numpy() - Y64).max() / ynorm) for h in hist
-                if not isinstance(h, tuple)]
-    adjoint = [float(np.abs(h[1].cpu().numpy() - Y64).max() / ynorm) for h in hist
-               if isinstance(h, tuple)]
-    s64._check_info()
+        print(f"[bench] {name}: {1e3 * dt / args.steps:.1f} ms per solve over {args.steps} "
+              f"steps {stage}", file=sys.stderr, flush=True)
+    # graph replays and the drop-in call, a few each
+    n_extra = args.extra_steps
+    for name, solver in (("mixed", smx), ("fp64", s64)) if n_extra > 0 else ():
+        solver.solve_graph()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(n_extra):
+            solver.solve_graph()
+        torch.cuda.synchronize()
+        res[name]["graph_rate"] = n_extra / (time.perf_counter() - t0)
+        solver.compute_port_solution()
+        t0 = time.perf_counter()
+        for _ in range(n_extra):
+            solver.compute_port_solution()
+        res[name]["api_rate"] = n_extra / (time.perf_counter() - t0)
+        print(f"[bench] {name}: graph {res[name]['graph_rate']:.2f}/s, compute_port_solution "
+              f"{res[name]['api_rate']:.2f}/s", file=sys.stderr, flush=True)
+    per_step, adjoint, final = [], [], None
+    if not args.no_parity:
+        print("[bench] Y error of the mixed form per correction", file=sys.stderr, flush=True)
+        from momwire_amd.bspline import _to_host
+        Y64, = _to_host(s64._solve_device()[0])
+        hist = []
+        smx._solve_device_mixed(history=hist)
+        ynorm = np.abs(Y64).max()
+        per_step = [float(np.abs(_to_host(h)[0] - Y64).max() / ynorm) for h in hist
+                    if not isinstance(h, tuple)]
+        adjoint = [float(np.abs(_to_host(h[1])[0] - Y64).max() / ynorm) for h in hist
+                   if isinstance(h, tuple)]
+        s64._check_info()
+        smx._check_info()
+        final = adjoint[-1] if adjoint else per_step[-1]
     fill_s = res["fp64"]["stage_ms"]["fill"] * 1e-3
+    solve_flops = lu_flops(nb) + 8.0 * nb * nb * npt
+    sm = res["mixed"]["stage_ms"]
+    tf32 = solve_flops / (sm["solve32"] * 1e-3) / 1e12
+    tf64 = solve_flops / (res["fp64"]["stage_ms"]["solve"] * 1e-3) / 1e12
     out = {
         "metric": METRIC,
         "value": res["mixed"]["rate"],
@@ -173,26 +233,41 @@ def mixed_main(args, wl):
         "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": res["mixed"]["ms"],
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "c64 factor / f64 residual", "data": "synthetic",
-        "config": {"workload": wl["desc"], "unknowns": nb, "segments": ns,
-                   "ports": s64._port_count(), "refine_steps": args.refine_steps},
+        "dtype": "c128 fill and residual, c64 factor and solves", "data": "synthetic",
+        "config": {"workload": wl["desc"], "unknowns": nb, "segments": ns, "ports": npt,
+                   "precision": "mixed (FP64 fill, FP32 MFMA factor + all-port solve, FP64 "
+                                "residual refinement, adjoint-corrected readout)",
+                   "refine_steps": args.refine_steps, "launch": "eager, HIP events per stage"},
         "fp64_port_solves_per_s": res["fp64"]["rate"],
         "mixed_over_fp64": res["mixed"]["rate"] / res["fp64"]["rate"],
+        "y_rel_err_vs_fp64": final,
         "y_rel_err_vs_fp64_per_step": per_step,
         "y_rel_err_adjoint_corrected_per_step": adjoint,
-        "stage_ms": {"fp64": res["fp64"]["stage_ms"], "mixed": res["mixed"]["stage_ms"]},
+        "graph_replay_port_solves_per_s": {k: res[k].get("graph_rate") for k in res},
+        "pcie_inclusive_port_solves_per_s": {k: res[k].get("api_rate") for k in res},
+        "stage_ms": {"mixed": sm, "fp64": res["fp64"]["stage_ms"]},
         "fill_gentries_per_s": nb * nb / fill_s / 1e9,
-        "fp64_solve_tflops": (lu_flops(nb) + 8.0 * nb * nb * s64._port_count())
-                             / (res["fp64"]["stage_ms"]["solve"] * 1e-3) / 1e12,
+        "fill_tflops": fill_flops(ns, nb) / fill_s / 1e12,
+        "fp64_solve_tflops": tf64,
     }
-    # the mixed form's dominant kernel set: the FP32 factor + all-port solve
-    solve_flops = lu_flops(nb) + 8.0 * nb * nb * s64._port_count()
-    tf32 = solve_flops / (res["mixed"]["stage_ms"]["solve32"] * 1e-3) / 1e12
-    out["roofline"] = {"kernel": "mw_cgesv (FP32 factor + all-port solve on FP32 MFMA)",
+    # the mixed form's dominant stage: the FP32 factor + all-port solve
+    out["roofline"] = {"kernel": "mw_cgesv: FP32 factor + all-port solve (k_gemm_sub<float> "
+                                 "trailing updates on FP32 MFMA 16x16x4, panels, back "
+                                 "substitution)",
                        "bound": "mfma", "achieved": tf32, "peak": PEAK_FP32_TFLOPS,
                        "unit": "TFLOP/s", "frac": tf32 / PEAK_FP32_TFLOPS, "traffic": None,
                        "algorithmic": f"(8/3) n^3 + 8 n^2 p = {solve_flops:.3e} flop per solve "
-                                      f"(n={nb}, p={s64._port_count()})"}
+                                      f"(n={nb}, p={npt})"}
+    tpath = os.path.join(ROOT, "profiles", "hbm_traffic.json")
+    if os.path.exists(tpath):
+        tr = json.load(open(tpath)).get(args.workload)
+        if tr:
+            out["roofline"]["traffic"] = tr["traffic_bytes_per_solve"]
+            out["roofline"]["traffic_source"] = tr["source"]
+            out["roofline"]["algorithmic_bytes"] = 2 * 8 * nb * (nb + npt)
+    if not args.no_cpu:
+        print("[bench] CPU baseline leg", file=sys.stderr, flush=True)
+        out["cpu_baseline"] = cpu_baseline(kw, budget_s=0.0)
     print(json.dumps(out))
 
 
@@ -533,32 +608,32 @@ def lu_flops(n):
     return 8.0 / 3.0 * n ** 3
 
 
-def cpu_baseline(kw, budget_s=20.0, cold_and_warm=False):
+def cpu_baseline(kw, budget_s=20.0, cold_and_warm=False, max_runs=5):
     """The reference CPU path (its compiled accelerator + scipy LAPACK) on this
-    host's cores: a bounded sample of whole port solves.  cold_and_warm: the
-    first solve fills the Sommerfeld grid (the reference's engine), the rest
-    reuse it, as the reference's grid cache does; the value is the cold rate."""
+    host's cores: a bounded sample of whole port solves.  The k-independent
+    set-up (geometry, basis, port columns) is built once outside the timed
+    solves, as the reference caches it; the fill takes the reference's own
+    route (tensor or chunked, by its swept_mem_mb budget).  budget_s <= 0:
+    one timed solve.  cold_and_warm: the first solve fills the Sommerfeld
+    grid (the reference's engine), the rest reuse it, as the reference's grid
+    cache does; the value is the cold rate."""
     from oracle.ref_cpu import RefCpuSolver, load_ref_accel
 
     acc = load_ref_accel()
     if acc is None:
-        from oracle.bspline_oracle import OracleSolver as Solver
-        kind = "port"
-        mk = lambda: Solver(**kw)  # noqa: E731
-    else:
-        kind = "reference"
-        mk = lambda: RefCpuSolver(acc, **kw)  # noqa: E731
-    cores = len(os.sched_getaffinity(0))
-    omp = os.environ.get("OMP_NUM_THREADS")
-    if omp:
-        cores = min(cores, int(omp))
+        raise RuntimeError("oracle/_ref is not built: the CPU baseline is the reference's own "
+                           "compiled accelerator (make -C oracle ref)")
+    kind = "reference"
+    cores = host_cores()
     os.environ.setdefault("OMP_NUM_THREADS", str(cores))
+    where = f"OMP threads={cores}, CPU {cpu_model()}"
+    s = RefCpuSolver(acc, **kw)
+    s.prepared()
+    route = ("chunked (assemble_Z_bspline_windowed)"
+             if not s.tensor_fits_budget(len(s.prepared()[0]["seg_l"])) else "tensor")
     times = []
-    t_start = time.perf_counter()
     if cold_and_warm:
-        from oracle.ref_cpu import RefCpuSolver as R
-        R._GRIDS.clear()
-        s = mk()
+        RefCpuSolver._GRIDS.clear()
         t0 = time.perf_counter()
         s.compute_port_solution()
         t_cold = time.perf_counter() - t0
@@ -568,30 +643,45 @@ def cpu_baseline(kw, budget_s=20.0, cold_and_warm=False):
             times.append(time.perf_counter() - t0)
         warm = float(np.median(times))
         return {"value": 1.0 / t_cold, "unit": "port-solves/s", "cores": cores, "kind": kind,
+                "cpu_model": cpu_model(),
                 "sample": f"1 cold port solve ({t_cold:.2f} s: grid filled by the reference's "
                           f"six-integral engine) and 3 warm ones (median {warm:.3f} s, "
-                          f"{1.0 / warm:.2f}/s) of the same workload; OMP threads={cores}"}
+                          f"{1.0 / warm:.2f}/s) of the same workload; {where}"}
+    if budget_s > 0:
+        s.compute_port_solution()  # warm-up
+    t_start = time.perf_counter()
+    parts = []
     while True:
-        s = mk()
+        tm = {}
         t0 = time.perf_counter()
-        s.compute_port_solution()
+        s.compute_port_solution(timings=tm)
         times.append(time.perf_counter() - t0)
-        if time.perf_counter() - t_start > budget_s or len(times) >= 5:
+        parts.append(tm)
+        if time.perf_counter() - t_start > budget_s or len(times) >= max_runs:
             break
     med = float(np.median(times))
+    fill = float(np.median([p["fill_s"] for p in parts]))
     return {"value": 1.0 / med, "unit": "port-solves/s", "cores": cores, "kind": kind,
-            "sample": f"{len(times)} full port solves of the same workload "
-                      f"(median {med:.2f} s each; fill in the reference's C++ accelerator, "
-                      f"solve in scipy LAPACK, OMP threads={cores})"}
+            "cpu_model": cpu_model(),
+            "sample": f"{len(times)} full port solve(s) of the same workload (median {med:.2f} s: "
+                      f"fill {fill:.2f} s on the reference's {route} route in its C++ "
+                      f"accelerator, solve {med - fill:.2f} s in scipy LAPACK); {where}"}
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="array8x8", choices=sorted(WORKLOADS))
+    ap.add_argument("--workload", default="array32x32", choices=sorted(WORKLOADS))
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-parity", action="store_true",
+                    help="array32x32: leave out the leg after the timed region that copies Y "
+                         "to the host and measures the mixed form's error (the line then "
+                         "carries no error figures); for hardware-counter runs")
+    ap.add_argument("--extra-steps", type=int, default=5,
+                    help="array32x32: solves timed for each of the graph-replay and the "
+                         "compute_port_solution() rates (0: not measured)")
     ap.add_argument("--eager", action="store_true",
                     help="launch kernels one by one instead of replaying the captured graph")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
@@ -674,14 +764,16 @@ def main():
     y_err = float(np.abs(y - g["y"]).max() / np.abs(g["y"]).max())
     s._check_info()
 
-    # PCIe-inclusive rate: Y and coeffs copied to host every solve
+    # PCIe-inclusive rate: the drop-in compute_port_solution(), Y and coeffs
+    # copied to the host every solve (it replays its captured graph from the
+    # second solve at a wavenumber on)
+    s.compute_port_solution()
+    s.compute_port_solution()
     torch.cuda.synchronize()
     t1 = time.perf_counter()
     n_pcie = max(3, args.steps // 4)
     for _ in range(n_pcie):
-        Y, W = s._solve_device()
-        Y.cpu().numpy()
-        W.cpu().numpy()
+        s.compute_port_solution()
     pcie_rate = n_pcie / (time.perf_counter() - t1)
 
     ncols = n_ports + len(s._port_plan()[1])

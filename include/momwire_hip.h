@@ -12,7 +12,21 @@
  * Complex values are interleaved (re, im) pairs: double2 for the FP64 entry
  * points (z*), float2 for the FP32 ones (c*).  Matrices are row-major with an
  * explicit leading dimension counted in complex elements.  Z is complex
- * symmetric, so row- and column-major storage describe the same matrix.
+ * symmetric only when every wire has the same radius (the fill uses the
+ * observer row's radius); with per-wire radii it is not, so a binding that
+ * hands over Fortran-order (column-major) storage solves with Z^T.  Pass
+ * row-major storage.
+ *
+ * Cancellation.  Entry points that take `cancel` say which side reads it:
+ *  - HOST-polled (mw_?getrf, mw_?gesv, mw_sweep_run): the host reads the
+ *    int32 between launches; any host int32 address works.
+ *  - DEVICE-polled (mw_somm_six_integrals, mw_somm_surfaces,
+ *    mw_somm_surfaces_varied): the kernels read it while they run, so it
+ *    must be device-accessible memory -- a flag from mw_cancel_create()
+ *    (mapped pinned host memory) or device memory.  An ordinary host
+ *    address is refused with MW_EINVAL before anything is launched.
+ * A flag from mw_cancel_create() serves both kinds.  Reference:
+ * _accelerators.cpp:79-113 (the polled flag), _cancel.py:20-79 (the token).
  */
 #ifndef MOMWIRE_HIP_H
 #define MOMWIRE_HIP_H
@@ -128,6 +142,16 @@ const char* mw_last_error(void);
 int mw_abi_version(void);
 
 /*
+ * A cancel flag both the host and the kernels can read: one int32 (zeroed)
+ * in mapped pinned host memory.  Write 1 through *flag to cancel, 0 to
+ * reset.  Replaces the token storage of CancelToken (reference
+ * src/momwire/_cancel.py:20-79, handed to the accelerators as an address,
+ * _accelerators.cpp:79-113).  mw_cancel_destroy frees it (NULL is a no-op).
+ */
+int mw_cancel_create(int32_t** flag);
+int mw_cancel_destroy(int32_t* flag);
+
+/*
  * Same-edge moment table: J_pq(delta) = static (closed form, 1/sqrt kernel)
  * + smooth remainder (exp(-jkR)-1)/(4 pi R) for every edge and offset delta.
  * Replaces seg_seg_static_moments_bspline_uniform (_accelerators.cpp:2822-2951)
@@ -157,8 +181,9 @@ int mw_fill_z(const mw_mesh* mesh, double k, double omega, double eps, double mu
  * scipy.linalg.solve (bspline.py:2971-2989).  ipiv: [n] device int32,
  * LAPACK-style sequential row interchanges (0-based).  work: device scratch of
  * mw_getrf_workspace_bytes(n) bytes.  info: device int32, set to 1 + the
- * first zero-pivot column (0 if none).  cancel: host int32 polled between
- * panels (may be NULL); a set flag stops the factorisation with MW_EABORT.
+ * first zero-pivot column (0 if none).  cancel: HOST-polled int32 read
+ * between panels (may be NULL); a set flag stops the factorisation with
+ * MW_EABORT.
  */
 int64_t mw_getrf_workspace_bytes(int64_t n);
 
@@ -166,7 +191,9 @@ int64_t mw_getrf_workspace_bytes(int64_t n);
  * Threshold pivoting: a panel keeps its diagonal pivots when every multiplier
  * of the panel satisfies |re| + |im| <= bound (checked on the GPU); otherwise
  * the panel is refactored with tournament partial pivoting.  bound <= 0
- * pivots every panel.  Default 8.  Process-wide setting.
+ * pivots every panel.  Default 8.  The setting belongs to the calling
+ * thread: it applies to the factorisations that thread starts afterwards,
+ * and threads driving other solvers keep their own (read once per call).
  */
 int mw_set_pivot_threshold(double bound);
 double mw_get_pivot_threshold(void);
@@ -305,6 +332,9 @@ typedef struct mw_somm_grid {
  * Replaces somm_six_integrals_batch (reference _accelerators.cpp:5771-5823,
  * python twin _sommerfeld.py:468-502).  Nodes must satisfy rho, h >= 0 and
  * rho + h > 0 (not checked on the device: the arrays are device memory).
+ * cancel: DEVICE-polled (see the top of this file), may be NULL; a node
+ * whose kernel sees it set stops early and leaves its output undefined, so
+ * the caller checks the flag after the stream has finished.
  */
 int mw_somm_six_integrals(double eps_re, double eps_im, double k2, const double* rho,
                           const double* h, int64_t n, double rtol, int32_t form, void* out,
@@ -313,7 +343,8 @@ int mw_somm_six_integrals(double eps_re, double eps_im, double k2, const double*
 /*
  * The four surfaces at n (R1, theta) nodes, out: [n][4] complex (R1 = 0 takes
  * the limits of eqs 169-172).  Replaces iv_surfaces_direct
- * (reference _sommerfeld.py:535-584), the grid's fill function.
+ * (reference _sommerfeld.py:535-584), the grid's fill function.  cancel as
+ * for mw_somm_six_integrals (DEVICE-polled).
  */
 int mw_somm_surfaces(double eps_re, double eps_im, double k2, double omega, double mu,
                      const double* R1, const double* theta, int64_t n, double rtol,
@@ -384,6 +415,7 @@ int mw_fill_ground_refl(const mw_mesh* mesh, double k, double omega, double eps,
  * The same surfaces with a complex permittivity per node (eps: [n][2]):
  * the sample generator of the learned surrogate.  No counterpart in the
  * reference (its engine takes one ground per call, _sommerfeld.py:468-502).
+ * cancel: DEVICE-polled, as for mw_somm_six_integrals.
  */
 int mw_somm_surfaces_varied(const double* eps, double k2, double omega, double mu,
                             const double* R1, const double* theta, int64_t n, double rtol,

@@ -205,8 +205,12 @@ class GridMaster:
                                             self.vals.data_ptr(), addr, st), "Sommerfeld grid")
         else:
             self.engine_nodes = filler(self, lib, st, addr)
-        if cancel is not None and cancel.cancelled:
-            raise _lib.SolveAborted("Sommerfeld grid fill cancelled")
+        if cancel is not None:
+            # the engine's nodes stop early once the flag is seen, leaving
+            # their values undefined: decide after the fill has finished
+            torch.cuda.current_stream().synchronize()
+            if cancel.cancelled:
+                raise _lib.SolveAborted("Sommerfeld grid fill cancelled")
         off = np.concatenate([[0], np.cumsum([a * b for a, b in self.shapes])])[:-1]
         self.off = off
 

@@ -96,6 +96,8 @@ _D = ctypes.c_double
 SIGNATURES = {
     "mw_last_error": (ctypes.c_char_p, []),
     "mw_abi_version": (ctypes.c_int, []),
+    "mw_cancel_create": (ctypes.c_int, [ctypes.POINTER(ctypes.POINTER(ctypes.c_int32))]),
+    "mw_cancel_destroy": (ctypes.c_int, [ctypes.POINTER(ctypes.c_int32)]),
     "mw_same_edge_table": (ctypes.c_int, [ctypes.POINTER(MwMesh), _D, _P]),
     "mw_fill_z": (ctypes.c_int, [ctypes.POINTER(MwMesh), _D, _D, _D, _D, _P, _I64, _P, _I32,
                                  _I32, _P]),

@@ -39,11 +39,29 @@ from ._lib import SolveAborted  # noqa: F401  (re-exported, reference _cancel.py
 
 
 class CancelToken:
-    """Cooperative cancel flag polled between LU panels and solve stages
-    (reference src/momwire/_cancel.py:20-79)."""
+    """Cooperative cancel flag (reference src/momwire/_cancel.py:20-79).
+
+    The host polls it between LU panels, sweep steps and refinement steps;
+    the Sommerfeld engine's kernels poll it while they run.  So the flag
+    lives in mapped pinned host memory (``mw_cancel_create``), which the host
+    and the device both read.  It is allocated when a solve first asks for
+    its address; until then a plain host int32 holds the state."""
 
     def __init__(self):
         self._flag = np.zeros(1, dtype=np.int32)
+        self._pinned = None
+
+    def _ensure_pinned(self):
+        if self._pinned is None:
+            import weakref
+
+            lib = _lib.load()
+            p = ctypes.POINTER(ctypes.c_int32)()
+            _lib.check(lib.mw_cancel_create(ctypes.byref(p)), "cancel flag")
+            arr = np.ctypeslib.as_array(p, shape=(1,))
+            arr[0] = self._flag[0]
+            self._flag, self._pinned = arr, p
+            weakref.finalize(self, lib.mw_cancel_destroy, p)
 
     def cancel(self):
         self._flag[0] = 1
@@ -62,12 +80,11 @@ class CancelToken:
 
     @property
     def address(self):
+        """Address of the flag's int32, readable by the host and the device."""
+        self._ensure_pinned()
         return self._flag.ctypes.data
 
-    @property
-    def ptr(self):
-        """Raw address of the flag's int32 (the kernels poll it)."""
-        return self._flag.ctypes.data
+    ptr = address
 
 
 def _torch():
@@ -76,6 +93,21 @@ def _torch():
     if not torch.cuda.is_available():
         raise _lib.AcceleratorUnavailable("no GPU visible to torch (HIP path needs an MI355X)")
     return torch
+
+
+def _to_host(*tensors):
+    """Device tensors to numpy through page-locked host buffers: one DMA
+    each at full PCIe rate, then one wait on the stream.  Each array is a
+    view of its own page-locked buffer (kept alive by the array); torch's
+    host allocator reuses the buffer once the array is released."""
+    torch = _torch()
+    out = []
+    for t in tensors:
+        h = torch.empty(tuple(t.shape), dtype=t.dtype, pin_memory=True)
+        h.copy_(t, non_blocking=True)
+        out.append(h)
+    torch.cuda.current_stream().synchronize()
+    return [h.numpy() for h in out]
 
 
 class _Device:
@@ -244,7 +276,7 @@ class BSplineSolver(_SweptPortSolutions):
             raise ValueError("refine_steps must be >= 0")
         # "mixed": FP64 fill, FP32 LU and solves on MFMA, iterative refinement
         # with the residual B - Z X in FP64 (refine_steps corrections; default
-        # 1 plus the adjoint-corrected readout, or 3 when KCL rows are present).
+        # 1 plus the adjoint-corrected readout when Z is symmetric, else 3).
         self.precision = precision
         self.refine_steps = None if refine_steps is None else int(refine_steps)
         self.degree = int(degree)
@@ -695,7 +727,12 @@ class BSplineSolver(_SweptPortSolutions):
             history.append(Yt[:D.n_ports, :D.n_ports])
 
         npt = D.n_ports
-        adjoint = D.n_con == 0
+        # The adjoint-corrected readout Y = B^T X + X^T (B - Z X) has an error
+        # quadratic in the iterate's only when Z = Z^T: every wire shares one
+        # radius (the fill uses the observer row's radius, so per-wire radii
+        # make Z unsymmetric) and no KCL rows are eliminated afterwards.
+        # Otherwise: the plain readout after three corrections.
+        adjoint = D.n_con == 0 and bool(mesh.symmetric)
         steps = self.refine_steps if self.refine_steps is not None else (1 if adjoint else 3)
 
         def residual():
@@ -707,7 +744,7 @@ class BSplineSolver(_SweptPortSolutions):
 
         def adjoint_readout(yptr):
             # Y + X^T R with R = B - Z X: the adjoint-corrected readout, whose
-            # error is quadratic in the iterate's (Z is complex symmetric)
+            # error is quadratic in the iterate's (taken only when Z = Z^T)
             self._post(D, D.X64.data_ptr(), nc, yptr, st)
             _lib.check(lib.mw_zgemm_tn_add(npt, npt, nb, D.X64.data_ptr(), nc, D.R64.data_ptr(),
                                            nc, yptr, max(npt, 1), st), "adjoint readout")
@@ -751,39 +788,63 @@ class BSplineSolver(_SweptPortSolutions):
             return self._solve_device_mixed(k, timers=timers)
         return self._solve_device(k, timers=timers)
 
-    def solve_graph(self):
-        """The whole device solve captured once as a HIP graph for the current
-        k and replayed on each call (launch-bound panels cost one graph launch
-        instead of ~10 kernel launches per 32 columns).  Returns the device
-        (Y, W) views the replay writes; cancellation is not polled inside a
-        replay.  Re-captured when k changes."""
+    def _capture(self, warm=True):
+        """Capture the whole device solve at the current k as a HIP graph
+        (``warm``: run it once eagerly first, unless a solve at this k has
+        already filled the grid and the loading values).
+
+        Everything the captured kernels read stays referenced by the entry:
+        the Sommerfeld master (the grid cache may evict it) and the loading
+        values of this frequency (replaced when the frequency changes)."""
         torch = _torch()
-        if getattr(self, "_graph_k", None) != self.k:
-            self._device()
+        self._device()
+        if warm:
             side = torch.cuda.Stream()
             side.wait_stream(torch.cuda.current_stream())
             with torch.cuda.stream(side):
                 self._solve()
             torch.cuda.current_stream().wait_stream(side)
-            g = torch.cuda.CUDAGraph()
-            with torch.cuda.graph(g):
-                out = self._solve()
-            self._graph, self._graph_out, self._graph_k = g, out, self.k
-        self._graph.replay()
-        return self._graph_out
+        g = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(g):
+            out = self._solve()
+        keep = (getattr(self, "_grid_master", None),
+                self._dev.__dict__.get("loading", {}).get("vals"))
+        self._graph_entry = (self.k, g, out, keep)
+
+    def solve_graph(self, warm=True):
+        """The whole device solve captured once as a HIP graph for the current
+        k and replayed on each call (launch-bound panels cost one graph launch
+        instead of ~10 kernel launches per 32 columns).  Returns the device
+        (Y, W) views the replay writes; cancellation is not polled inside a
+        replay.  Re-captured when k changes."""
+        entry = getattr(self, "_graph_entry", None)
+        if entry is None or entry[0] != self.k:
+            self._capture(warm)
+            entry = self._graph_entry
+        entry[1].replay()
+        return entry[2]
 
     def _check_info(self):
-        info = self._dev.info.cpu().numpy()
+        info, = _to_host(self._dev.info)
         if info[0] != 0:
             raise np.linalg.LinAlgError(f"Z is singular: zero pivot at column {info[0] - 1}")
         if info[1] != 0:
             raise np.linalg.LinAlgError("KCL Schur complement is singular")
 
     def compute_port_solution(self, same_edge_prep=None) -> PortSolution:
-        """Solve every port from ONE fill and ONE factorisation (bspline.py:4053-4179)."""
-        Y, W = self._solve()
-        y = Y.cpu().numpy()
-        coeffs = W.cpu().numpy()
+        """Solve every port from ONE fill and ONE factorisation (bspline.py:4053-4179).
+
+        The first solve at a wavenumber launches its kernels one by one; from
+        the second solve at the same k on, the captured graph of the whole
+        solve is replayed (one launch instead of thousands).  A solver with a
+        cancel token always launches eagerly, so that the flag is polled
+        between panels and refinement steps."""
+        if self._cancel is None and getattr(self, "_last_k", None) == self.k:
+            Y, W = self.solve_graph(warm=False)
+        else:
+            Y, W = self._solve()
+        self._last_k = self.k
+        y, coeffs = _to_host(Y, W)
         self._check_info()
         return PortSolution(y=y, coeffs=coeffs, port_currents=y,
                             basis=("bspline-gpu", self._mesh, id(self)))

@@ -141,12 +141,57 @@ int check_mesh_ground(const mw_mesh* M) {
     return fail(MW_EINVAL, "%sextended kernel without the image block's coaxial labels");
   return MW_OK;
 }
+
+// A cancel flag that a kernel polls must be memory the device can read: host
+// memory allocated mapped (mw_cancel_create, hipHostMalloc) or device memory.
+// An ordinary (pageable) host address is refused before anything is
+// launched; *dev receives the address the kernels dereference.
+int device_cancel(const volatile int32_t* cancel, const volatile int32_t** dev) {
+  *dev = nullptr;
+  if (!cancel) return MW_OK;
+  hipPointerAttribute_t a;
+  memset(&a, 0, sizeof a);
+  const hipError_t e = hipPointerGetAttributes(&a, (const void*)cancel);
+  if (e != hipSuccess) {
+    (void)hipGetLastError();  // the query's error must not surface at the next call
+    return fail(MW_EINVAL, "%scancel flag is not device-accessible memory (allocate it with "
+                           "mw_cancel_create)");
+  }
+  if (a.type == hipMemoryTypeHost && a.devicePointer) {
+    *dev = (const volatile int32_t*)a.devicePointer;
+    return MW_OK;
+  }
+  if (a.type == hipMemoryTypeDevice || a.type == hipMemoryTypeManaged) {
+    *dev = cancel;
+    return MW_OK;
+  }
+  return fail(MW_EINVAL, "%scancel flag is not device-accessible memory (allocate it with "
+                         "mw_cancel_create)");
+}
 }  // namespace
 
 extern "C" {
 
 const char* mw_last_error(void) { return g_err; }
 int mw_abi_version(void) { return 1; }
+
+int mw_cancel_create(int32_t** flag) {
+  g_err[0] = 0;
+  if (!flag) return fail(MW_EINVAL, "%sflag is NULL");
+  *flag = nullptr;
+  void* p = nullptr;
+  const hipError_t e = hipHostMalloc(&p, 64, hipHostMallocMapped);
+  if (e != hipSuccess) return hip_status(e, "mw_cancel_create");
+  *(volatile int32_t*)p = 0;
+  *flag = (int32_t*)p;
+  return MW_OK;
+}
+
+int mw_cancel_destroy(int32_t* flag) {
+  g_err[0] = 0;
+  if (!flag) return MW_OK;
+  return hip_status(hipHostFree(flag), "mw_cancel_destroy");
+}
 
 int mw_same_edge_table(const mw_mesh* mesh, double k, void* stream) {
   g_err[0] = 0;
@@ -559,10 +604,12 @@ int mw_somm_six_integrals(double eps_re, double eps_im, double k2, const double*
   if (!(rtol > 0.0)) return fail(MW_EINVAL, "%srtol must be positive");
   if (form < 0 || form > 2) return fail(MW_EINVAL, "%sform must be 0 (auto), 1 (J) or 2 (H)");
   if (!(eps_re == eps_re) || !(eps_im == eps_im)) return fail(MW_EINVAL, "%seps is NaN");
-  int rc = hip_status(mw_launch_somm_six(eps_re, eps_im, k2, rho, h, n, rtol, form,
-                                         (double2*)out, cancel, (hipStream_t)stream),
-                      "mw_somm_six_integrals");
-  return rc;
+  const volatile int32_t* dcancel = nullptr;
+  int rc = device_cancel(cancel, &dcancel);
+  if (rc) return rc;
+  return hip_status(mw_launch_somm_six(eps_re, eps_im, k2, rho, h, n, rtol, form,
+                                       (double2*)out, dcancel, (hipStream_t)stream),
+                    "mw_somm_six_integrals");
 }
 
 int mw_somm_surfaces(double eps_re, double eps_im, double k2, double omega, double mu,
@@ -575,8 +622,11 @@ int mw_somm_surfaces(double eps_re, double eps_im, double k2, double omega, doub
     return fail(MW_EINVAL, "%sk2, omega and mu must be positive");
   if (!(rtol > 0.0)) return fail(MW_EINVAL, "%srtol must be positive");
   if (!(eps_re == eps_re) || !(eps_im == eps_im)) return fail(MW_EINVAL, "%seps is NaN");
+  const volatile int32_t* dcancel = nullptr;
+  int rc = device_cancel(cancel, &dcancel);
+  if (rc) return rc;
   return hip_status(mw_launch_somm_surfaces(eps_re, eps_im, k2, omega, mu, R1, theta, n, rtol,
-                                            (double2*)out, cancel, (hipStream_t)stream),
+                                            (double2*)out, dcancel, (hipStream_t)stream),
                     "mw_somm_surfaces");
 }
 
@@ -664,8 +714,11 @@ int mw_somm_surfaces_varied(const double* eps, double k2, double omega, double m
   if (!(k2 > 0.0) || !(omega > 0.0) || !(mu > 0.0))
     return fail(MW_EINVAL, "%sk2, omega and mu must be positive");
   if (!(rtol > 0.0)) return fail(MW_EINVAL, "%srtol must be positive");
+  const volatile int32_t* dcancel = nullptr;
+  int rc = device_cancel(cancel, &dcancel);
+  if (rc) return rc;
   return hip_status(mw_launch_somm_surfaces_varied(eps, k2, omega, mu, R1, theta, n, rtol,
-                                                   (double2*)out, cancel, (hipStream_t)stream),
+                                                   (double2*)out, dcancel, (hipStream_t)stream),
                     "mw_somm_surfaces_varied");
 }
 

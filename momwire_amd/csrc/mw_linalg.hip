@@ -2130,7 +2130,10 @@ static void* panel_inv(void* work, int64_t k0) {
   return reinterpret_cast<char*>(work) + (int64_t)NB * NB * 16 * (1 + 2 * (k0 / NB));
 }
 
-static double g_lbound = 8.0;  // threshold-pivoting multiplier bound (1/tau); <= 0: always pivot
+// Threshold-pivoting multiplier bound (1/tau; <= 0: always pivot), one per
+// calling thread: solvers driven from different threads never see each
+// other's setting, and each factorisation reads it once when it starts.
+static thread_local double g_lbound = 8.0;
 
 // Outer block width: trailing updates use K = nbo.  Small systems are bound
 // by the panel chain rather than by the updates, so they keep one level.

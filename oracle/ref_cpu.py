@@ -9,12 +9,21 @@ seg_seg_reg_moments_bspline_swept, assemble_Z_bspline, then LAPACK
 (scipy.linalg.solve) over all port columns.  Geometry, basis and port columns
 come from the oracle's restatement of the reference's Python set-up.
 
+Where the reference itself would not hold the whole moment tensor -- its
+(d+1)^2 N^2 complex entries over the swept_mem_mb budget, 256 MiB by default
+(bspline.py:3695-3700) -- it takes the chunked route instead
+(bspline.py:2386-2550): observer-row chunks of the moments accumulated into a
+Fortran-order Z by assemble_Z_bspline_windowed, then per-edge corrections
+(static + smooth remainder minus the full-kernel block), and LAPACK factors Z
+in place.  ``Z`` follows the same dispatch.
+
 Used as bench.py's timed CPU baseline (kind "reference") and as a checker.
 """
 
 import glob
 import importlib.util
 import os
+import time
 
 import numpy as np
 import scipy.linalg
@@ -37,9 +46,10 @@ def load_ref_accel():
 class RefCpuSolver(OracleSolver):
     """OracleSolver whose fill and assembly run in the reference's C++."""
 
-    def __init__(self, acc, **kw):
+    def __init__(self, acc, swept_mem_mb=256, **kw):
         super().__init__(**kw)
         self.acc = acc
+        self.swept_mem_mb = int(swept_mem_mb)  # the reference's default (bspline.py:511-982)
 
     def J_blocks(self, geom, k):
         d = self.degree
@@ -65,6 +75,78 @@ class RefCpuSolver(OracleSolver):
                 A_reg = self.acc.seg_seg_reg_moments_bspline_swept(R, wu, np.array([float(k)]))[0]
                 J[:, :, sl, sl] = A_st + A_reg
         return J
+
+    def tensor_fits_budget(self, n_segs):
+        """The reference's route choice (bspline.py:3695-3700)."""
+        return (self.degree + 1) ** 2 * int(n_segs) ** 2 * 16 <= (self.swept_mem_mb << 20)
+
+    def Z(self, geom=None, basis=None, remainder=True):
+        geom = geom or self.geometry()
+        basis = basis or self.basis(geom)
+        if self.tensor_fits_budget(len(geom["seg_l"])):
+            return super().Z(geom, basis, remainder)
+        supp, polys = basis[:2]
+        Z = self.Z_chunked(geom, supp, polys)
+        if self.ground_z is not None:
+            Z -= self.ground_terms(geom, supp, polys, remainder)
+        if any(a is not None for a in self.loading_args):
+            Z += self.loading_term(geom, supp, polys)
+        return Z
+
+    def Z_chunked(self, geom, supp, polys):
+        """Free-space Z on the reference's chunked route (bspline.py:2386-2550
+        -> assemble_Z_bspline_windowed, _accelerators.cpp:1325-1443)."""
+        d = self.degree
+        t01, w01 = gl01(self.n_qp_pair)
+        a = float(self.radius[0])
+        if not np.all(self.radius == a):
+            raise NotImplementedError("uniform radius only")
+        seg_l, seg_r = geom["seg_l"], geom["seg_r"]
+        n_segs = len(seg_l)
+        n_b = supp.shape[0]
+        Z = np.zeros((n_b, n_b), dtype=np.complex128, order="F")
+        supp_c = np.ascontiguousarray(supp, dtype=np.int64)
+        polys_c = np.ascontiguousarray(polys, dtype=np.float64)
+        tan_c = np.ascontiguousarray(geom["tangents"], dtype=np.float64)
+        all_n = np.arange(n_b, dtype=np.int64)
+        om, acc = float(self.omega), self.acc
+
+        def touching(lo, hi):
+            return np.nonzero(((supp_c >= lo) & (supp_c < hi)).any(axis=1))[0].astype(np.int64)
+
+        def accumulate(Jw, i0, i1, j0, j1, m_idx, n_idx):
+            acc.assemble_Z_bspline_windowed(Jw, supp_c, polys_c, tan_c, m_idx, n_idx, int(i0),
+                                            int(i1), int(j0), int(j1), om, EPS0, MU0, Z, 0)
+
+        row_bytes = (d + 1) ** 2 * n_segs * 16
+        chunk = max(1, int(self.swept_mem_mb * 1024 * 1024 // row_bytes))
+        for i0 in range(0, n_segs, chunk):
+            i1 = min(i0 + chunk, n_segs)
+            Jc = acc.seg_seg_full_moments_bspline(seg_l[i0:i1], seg_r[i0:i1], seg_l, seg_r, a * a,
+                                                  float(self.k), d, t01, w01)
+            accumulate(Jc, i0, i1, 0, n_segs, touching(i0, i1), all_n)
+            del Jc
+        for w, pw in enumerate(geom["per_wire"]):
+            base = geom["seg_off"][w]
+            for e in range(len(pw["ed_off"]) - 1):
+                lo, hi = base + pw["ed_off"][e], base + pw["ed_off"][e + 1]
+                arc = pw["ed_arc"][e]
+                n_e = len(arc) - 1
+                h = float(arc[1] - arc[0])
+                A_st = acc.seg_seg_static_moments_bspline_uniform(h, a, n_e, d)
+                sq = (arc[:-1, None] + t01[None, :] * (arc[1:] - arc[:-1])[:, None]).ravel()
+                R = np.sqrt((sq[:, None] - sq[None, :]) ** 2 + a * a)
+                hh = (arc[1:] - arc[:-1])[:, None]
+                u, ww = t01[None, :] * hh, w01[None, :] * hh
+                wu = np.stack([ww * u ** p for p in range(d + 1)])
+                A_reg = acc.seg_seg_reg_moments_bspline_swept(R, wu, np.array([float(self.k)]))[0]
+                J_edge = acc.seg_seg_full_moments_bspline(seg_l[lo:hi], seg_r[lo:hi], seg_l[lo:hi],
+                                                          seg_r[lo:hi], a * a, float(self.k), d,
+                                                          t01, w01)
+                corr = np.ascontiguousarray((A_st + A_reg) - J_edge)
+                idx = touching(lo, hi)
+                accumulate(corr, lo, hi, lo, hi, idx, idx)
+        return Z
 
     def assemble(self, J, supp, polys, geom, td=None):
         if td is None:
@@ -152,13 +234,27 @@ class RefCpuSolver(OracleSolver):
             vals.append(S.reshape((4,) + rr.shape))
         return so.Grid(regions, scal, vals, k2, om, so.MU0_EXACT, eps_b)
 
-    def compute_port_solution(self):
-        geom = self.geometry()
-        basis = self.basis(geom)
+    def prepared(self):
+        """Geometry, basis and port columns: k-independent, built once and
+        reused across solves as the reference caches them (bspline.py:1004-1014)."""
+        if getattr(self, "_prep", None) is None:
+            geom = self.geometry()
+            basis = self.basis(geom)
+            self._prep = (geom, basis, self.port_columns(geom, basis))
+        return self._prep
+
+    def compute_port_solution(self, timings=None):
+        geom, basis, (B, kcl) = self.prepared()
+        t0 = time.perf_counter()
         Z = self.Z(geom, basis)
-        B, kcl = self.port_columns(geom, basis)
+        if timings is not None:
+            timings["fill_s"] = time.perf_counter() - t0
         if kcl.shape[0] == 0:
+            # the reference solves in place on its Fortran-order Z (bspline.py:2984)
             X = scipy.linalg.solve(Z, B, overwrite_a=True)
         else:
             X = self.solve_kcl(Z, B, kcl)
-        return B.T @ X, X
+        Y = B.T @ X
+        if timings is not None:
+            timings["solve_s"] = time.perf_counter() - t0 - timings["fill_s"]
+        return Y, X

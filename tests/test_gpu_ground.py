@@ -267,3 +267,49 @@ def test_grounded_graph_replay_matches_eager():
         torch.cuda.synchronize()
         assert rel_err(Y.cpu().numpy(), y_eager) < 1e-13
         s.compute_port_solution()
+
+
+def test_engine_refuses_pageable_cancel_flag():
+    """The engine's kernels poll the cancel flag on the device, so an
+    ordinary host address is refused on the host before any launch (the
+    call asks for zero nodes, so nothing would be launched either way)."""
+    _say("engine cancel: pageable host flag refused")
+    lib = _lib.load()
+    flag = np.zeros(1, dtype=np.int32)
+    out = torch.empty((1, 4), dtype=torch.complex128, device="cuda")
+    r = _dev(np.ones(1))
+    for call in (
+        lambda: lib.mw_somm_surfaces(12.0, -1.0, K2, K2 * 299792458.0, 4e-7 * np.pi,
+                                     r.data_ptr(), r.data_ptr(), 0, 1e-6, out.data_ptr(),
+                                     flag.ctypes.data, _st()),
+        lambda: lib.mw_somm_six_integrals(12.0, -1.0, K2, r.data_ptr(), r.data_ptr(), 0, 1e-6, 0,
+                                          out.data_ptr(), flag.ctypes.data, _st()),
+    ):
+        rc = call()
+        assert rc == _lib.MW_EINVAL, rc
+        assert b"device-accessible" in lib.mw_last_error()
+    torch.cuda.synchronize()
+
+
+def test_grounded_solve_with_cancel_token():
+    """A cold grounded solve carrying a (pinned, device-readable) token:
+    untripped it matches the reference's Y; tripped before the solve the
+    engine stops and the solve raises SolveAborted, caching no grid."""
+    from momwire_amd.bspline import CancelToken, SolveAborted
+    _say("engine cancel: grounded dipole with a token")
+    d = golden("ground_dipole")
+    tok = CancelToken()
+    _ground.clear_cache()
+    s = BSplineSolver(**_grid_case("ground_dipole"), cancel=tok)
+    sol = s.compute_port_solution()
+    assert rel_err(sol.y, d["y"]) < 1e-10
+    assert rel_err(sol.coeffs, d["coeffs"]) < 1e-10
+    _say("engine cancel: pre-tripped token")
+    _ground.clear_cache()
+    tok.cancel()
+    s2 = BSplineSolver(**_grid_case("ground_dipole"), cancel=tok)
+    with pytest.raises(SolveAborted):
+        s2.compute_port_solution()
+    assert not _ground._MASTERS
+    tok.reset()
+    assert rel_err(s2.compute_port_solution().y, d["y"]) < 1e-10

@@ -343,3 +343,25 @@ def test_gemm_tn_add():
                                                dC.data_ptr(), n, _stream()), "tn")
         torch.cuda.synchronize()
         assert rel_err(dC.cpu().numpy(), C + A.T @ B) < 1e-13
+
+
+def test_compute_port_solution_replays_graph_across_k():
+    """compute_port_solution: eager on the first solve at a k, the captured
+    graph from the second on.  Moving to another k and back replays the
+    graph captured before, whose per-frequency loading values must still be
+    alive; every call matches an eager solve at the same k."""
+    _say("compute_port_solution graph across k")
+    kw = dict(W.dipole_array(2, 2, 13, seed=3), wire_conductivity=5.8e7)
+    s = BSplineSolver(**kw)
+    ref = BSplineSolver(**kw)
+    k1 = s.k
+    k2 = 1.05 * k1
+    for kk in (k1, k1, k1, k2, k2, k1, k1, k2):
+        s._set_k(kk)
+        ref._set_k(kk)
+        sol = s.compute_port_solution()
+        Yr, Wr = ref._solve_device()
+        torch.cuda.synchronize()
+        assert rel_err(sol.y, Yr.cpu().numpy()) < 1e-13, kk
+        assert rel_err(sol.coeffs, Wr.cpu().numpy()) < 1e-13, kk
+    assert s._graph_entry is not None

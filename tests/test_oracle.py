@@ -157,3 +157,27 @@ def test_reference_spread_on_long_edges(index, name, ref_accel):
     assert 1e-11 < spread < 3e-8
     y_o, _ = OracleSolver(**kw).compute_port_solution()
     assert rel_err(y_o, g["y"]) < 3e-8
+
+
+def test_reference_chunked_route_equals_tensor_route(ref_accel):
+    """The CPU baseline's chunked route (the reference's own choice once the
+    moment tensor is over swept_mem_mb, bspline.py:2386-2550, 3695-3700)
+    against its tensor route on the same 3x3 array: the windowed assembly is
+    linear, so the two agree to rounding (reference
+    tests/test_momwire.py:2505-2545 holds them to 1e-10)."""
+    from momwire_amd import workloads as W
+    from oracle.ref_cpu import RefCpuSolver
+
+    kw = W.dipole_array(3, 3, 21, seed=1)
+    tensor = RefCpuSolver(ref_accel, **kw)
+    chunked = RefCpuSolver(ref_accel, swept_mem_mb=1, **kw)
+    geom = tensor.geometry()
+    basis = tensor.basis(geom)
+    assert tensor.tensor_fits_budget(len(geom["seg_l"]))
+    assert not chunked.tensor_fits_budget(len(geom["seg_l"]))
+    Zt, Zc = tensor.Z(geom, basis), chunked.Z(geom, basis)
+    assert np.abs(Zt - Zc).max() / np.abs(Zt).max() < 1e-13
+    # and the whole port solve through it
+    yt, _ = tensor.compute_port_solution()
+    yc, _ = chunked.compute_port_solution()
+    assert np.abs(yt - yc).max() / np.abs(yt).max() < 1e-11
