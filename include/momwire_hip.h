@@ -231,6 +231,21 @@ int mw_cgesv(int64_t n, int64_t nrhs, void* A, int64_t lda, int32_t* ipiv,
              void* work, int32_t* info, const volatile int32_t* cancel,
              void* stream);
 
+/*
+ * mw_zgesv without the workspace argument, for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  No counterpart in the reference: it
+ * exists for those callers only.  The LU workspace is a static device array
+ * of 17 MiB per GPU (held from the first call on that GPU until the process
+ * ends), so n <= 16384; larger n returns MW_ERANGE.  Everything else as
+ * mw_zgesv (ipiv and info are outputs the caller passes).  The workspace
+ * keeps the last call's contents and is shared by every call on the GPU:
+ * calls that overlap on one GPU (two streams or two threads) race, and the
+ * factors cannot be reused by mw_zgetrs afterwards.  Allocates nothing.
+ */
+int mw_zgesv_noalloc(int64_t n, int64_t nrhs, void* A, int64_t lda, int32_t* ipiv,
+                     int32_t* info, const volatile int32_t* cancel, void* stream);
+
 /* C <- C - A B (complex, MFMA).  A: m x k, B: k x n, C: m x n. */
 int mw_zgemm_sub(int64_t m, int64_t n, int64_t k, const void* A, int64_t lda,
                  const void* B, int64_t ldb, void* C, int64_t ldc, void* stream);

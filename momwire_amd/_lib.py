@@ -108,6 +108,7 @@ SIGNATURES = {
     "mw_cgetrf": (ctypes.c_int, [_I64, _P, _I64, _P, _P, _P, _P, _P]),
     "mw_zgesv": (ctypes.c_int, [_I64, _I64, _P, _I64, _P, _P, _P, _P, _P]),
     "mw_cgesv": (ctypes.c_int, [_I64, _I64, _P, _I64, _P, _P, _P, _P, _P]),
+    "mw_zgesv_noalloc": (ctypes.c_int, [_I64, _I64, _P, _I64, _P, _P, _P, _P]),
     "mw_zgetrs": (ctypes.c_int, [_I64, _I64, _P, _I64, _P, _P, _P, _I64, _P]),
     "mw_cgetrs": (ctypes.c_int, [_I64, _I64, _P, _I64, _P, _P, _P, _I64, _P]),
     "mw_zgemm_sub": (ctypes.c_int, [_I64, _I64, _I64, _P, _I64, _P, _I64, _P, _I64, _P]),

@@ -4,6 +4,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <mutex>
 #include <vector>
 
 #include "mw_common.h"
@@ -267,6 +268,65 @@ int mw_zgesv(int64_t n, int64_t nrhs, void* A, int64_t lda, int32_t* ipiv, void*
              int32_t* info, const volatile int32_t* cancel, void* stream) {
   return gesv_common(true, n, nrhs, A, lda, ipiv, work, info, cancel, stream);
 }
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// The factor-and-solve with no allocation: its LU workspace (diagonal-block
+// inverses, pivot flags, tournament counters and candidates) is a
+// file-scope device array instead of a caller-provided buffer.  Each GPU
+// holds its own copy of the array from the first call on that GPU until the
+// process ends; host code reaches it only through the address that
+// hipGetSymbolAddress returns for the current device.
+// ---------------------------------------------------------------------------
+namespace {
+constexpr int64_t kScratchBytes = 17LL << 20;  // mw_getrf_workspace_bytes(16384) fits
+constexpr int64_t kScratchMaxN = 16384;
+constexpr int kScratchDevs = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kScratchDevs] = {nullptr};
+}  // namespace
+
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+namespace {
+// The device address of this GPU's scratch array (looked up once per device,
+// with that device current).
+int scratch_for_current_device(void** out) {
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return hip_status(e, "mw_zgesv_noalloc: hipGetDevice");
+  if (dev < 0 || dev >= kScratchDevs)
+    return fail(MW_EINVAL, "%sdevice ordinal %lld beyond the scratch table", "", dev);
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return hip_status(e, "mw_zgesv_noalloc: hipGetSymbolAddress");
+    if (!p) return fail(MW_EHIP, "%shipGetSymbolAddress returned NULL for the scratch array");
+    g_scratch_ptr[dev] = p;
+  }
+  *out = g_scratch_ptr[dev];
+  return MW_OK;
+}
+}  // namespace
+
+extern "C" {
+
+int mw_zgesv_noalloc(int64_t n, int64_t nrhs, void* A, int64_t lda, int32_t* ipiv,
+                     int32_t* info, const volatile int32_t* cancel, void* stream) {
+  g_err[0] = 0;
+  if (n <= 0 || nrhs < 0) return fail(MW_EINVAL, "%sbad sizes n=%lld nrhs=%lld", "", n, nrhs);
+  if (n > kScratchMaxN || mw_getrf_ws_bytes(n) > kScratchBytes)
+    return fail(MW_ERANGE,
+                "%sn=%lld is above the static scratch of mw_zgesv_noalloc (17 MiB: "
+                "n <= %lld); use mw_zgesv with a workspace",
+                "", n, kScratchMaxN);
+  void* work = nullptr;
+  int rc = scratch_for_current_device(&work);
+  if (rc) return rc;
+  return gesv_common(true, n, nrhs, A, lda, ipiv, work, info, cancel, stream);
+}
+
 int mw_cgesv(int64_t n, int64_t nrhs, void* A, int64_t lda, int32_t* ipiv, void* work,
              int32_t* info, const volatile int32_t* cancel, void* stream) {
   return gesv_common(false, n, nrhs, A, lda, ipiv, work, info, cancel, stream);

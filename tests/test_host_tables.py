@@ -163,3 +163,15 @@ def test_sweep_plan_rejects_nan_pivot_bound():
     h = ctypes.c_void_p()
     rc = lib.mw_sweep_plan_create(1, None, None, 1, None, 0, float("nan"), ctypes.byref(h))
     assert rc == _lib.MW_EINVAL and b"NaN" in lib.mw_last_error() and not h.value
+
+
+def test_noalloc_gesv_refuses_oversize_on_the_host():
+    """mw_zgesv_noalloc's size limit is checked before any HIP call: a system
+    past its static workspace (n > 16384) and bad sizes are refused on a
+    machine with no GPU at all."""
+    from momwire_amd import _lib
+
+    lib = _lib.load()
+    assert lib.mw_zgesv_noalloc(16385, 1, None, 16386, None, None, None, None) == _lib.MW_ERANGE
+    assert b"n <= 16384" in lib.mw_last_error()
+    assert lib.mw_zgesv_noalloc(0, 1, None, 1, None, None, None, None) == _lib.MW_EINVAL
