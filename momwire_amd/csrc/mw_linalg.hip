@@ -275,13 +275,320 @@ __global__ __launch_bounds__(256) void k_gemm_sub_s(int64_t m, int64_t n, int64_
   gemm_small_body<T, TA>(blockIdx.x, blockIdx.y, m, n, kdim, A, lda, B, ldb, C, ldc);
 }
 
+// ---------------------------------------------------------------------------
+// Large products C <- C - A B (the LU's K = 512 trailing updates, the
+// residuals of the mixed form): the same 128 x 64-complex C tile per
+// workgroup, with the operands laid out in LDS for 16-byte reads.
+//
+//  - B is expanded to its real form [[br, bi], [-bi, br]] when it is staged,
+//    and stored transposed (real column major), so a B fragment is a plain
+//    LDS load with no per-step selects.
+//  - Within a stage the real k index is permuted: position (kr % G) S + kr / G,
+//    G the k groups of the MFMA's lanes, S = KR / G.  A lane then finds the k
+//    values of its successive MFMA steps side by side and reads them 16 bytes
+//    at a time (A and B use the same permutation, so the sum is unchanged).
+//  - FP64: 16x16x4 MFMA, each wave a 4 x 4 grid of 16 x 16 tiles, KR = 16
+//    real k per stage.  FP32: 32x32x2 MFMA (half the instructions and LDS
+//    reads of 16x16x4 for the same work), each wave a 2 x 2 grid of 32 x 32
+//    tiles, KR = 32 (k_gemm_sub_pf, whose 32 x 32 output element r of lane l
+//    sits at row 8 (r / 4) + 4 (l / 32) + r % 4, column l % 32).
+//  - Tiles are visited in an XCD-aware grouped order (pgemm_tile): workgroup
+//    p runs on XCD p % 8, each XCD takes a contiguous range of the order, and
+//    the order walks 8 row tiles down a column of tiles before moving right,
+//    so the A rows and B columns a CU pair works on are shared in its XCD's L2.
+// ---------------------------------------------------------------------------
+template <typename T> struct PG;  // the FP64 instance; FP32 is k_gemm_sub_pf
+template <> struct PG<double> {
+  static constexpr int KR = 16, LD = 18, G = 4, TS = 16, NT = 4;  // tile size, tiles per wave side
+  typedef v4d acc_t;
+  static MW_DEV acc_t mma(double a, double b, acc_t c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+  }
+  static constexpr int NACC = 4;
+  // output element r of lane l: row (l / 16) + 4 r, column l % 16
+  static MW_DEV int orow(int lane, int r) { return (lane >> 4) + 4 * r; }
+};
+typedef float v16f __attribute__((ext_vector_type(16)));
+
+template <typename T>
+struct PGemmLds {
+  T As[2][GBM][PG<T>::LD];      // -A, permuted real k
+  T Bt[2][2 * GBN][PG<T>::LD];  // expanded B, [real column][permuted real k]
+};
+
+// tile (bx, by) of workgroup p over an nbx x nby tile grid: a bijection
+MW_DEV void pgemm_tile(int p, int nbx, int nby, int& bx, int& by) {
+  const int T = nbx * nby;
+  const int x = p & 7, local = p >> 3;
+  const int t = x * (T >> 3) + min(x, T & 7) + local;  // XCD x's range starts here
+  constexpr int GM = 8;
+  const int gsz = GM * nbx;
+  const int gid = t / gsz, first = gid * GM;
+  const int gm = min(nby - first, GM);
+  const int r = t - gid * gsz;
+  by = first + r % gm;
+  bx = r / gm;
+}
+
+template <typename T>
+__global__ __launch_bounds__(256, 2) void k_gemm_sub_p(int64_t m, int64_t n, int64_t kdim,
+                                                      const typename cplx_t<T>::type* __restrict__ A,
+                                                      int64_t lda,
+                                                      const typename cplx_t<T>::type* __restrict__ B,
+                                                      int64_t ldb, typename cplx_t<T>::type* C,
+                                                      int64_t ldc, int nbx, int nby) {
+  typedef typename cplx_t<T>::type CT;
+  typedef PG<T> P;
+  constexpr int KR = P::KR, G = P::G, S = KR / G, KC = KR / 2, TS = P::TS, NT = P::NT;
+  constexpr int LPG = 64 / G;  // lanes per k group
+  const int lt = threadIdx.x & (TS - 1);  // the lane's row (A) / column (B, C) in a tile
+  __shared__ PGemmLds<T> L;
+  int bx, by;
+  pgemm_tile(blockIdx.x, nbx, nby, bx, by);
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wr = wave >> 1, wc = wave & 1;  // the wave's 64 x 64 real quadrant
+  const int64_t row0 = (int64_t)by * GBM, col0 = (int64_t)bx * GBN;
+  T* Cr = reinterpret_cast<T*>(C);
+
+  typename P::acc_t acc[NT][NT];
+#pragma unroll
+  for (int i = 0; i < NT; ++i)
+#pragma unroll
+    for (int j = 0; j < NT; ++j)
+#pragma unroll
+      for (int r = 0; r < P::NACC; ++r) {
+        const int64_t gr = row0 + wr * 64 + i * TS + P::orow(lane, r);
+        const int64_t gcr = 2 * col0 + wc * 64 + j * TS + lt;
+        acc[i][j][r] = (gr < m && gcr < 2 * n) ? Cr[gr * 2 * ldc + gcr] : (T)0;
+      }
+
+  // staging: A 128 rows x KC complex (thread: row tid / 2, half tid % 2);
+  // B KC rows x 64 complex (thread: column tid % 64, rows (tid / 64) BQ + q)
+  constexpr int AQ = KC / 2, BQ = KC / 4;
+  CT ra[AQ], rb[BQ];
+  const int a_row = tid >> 1, a_h = tid & 1;
+  const int b_col = tid & 63, b_k = (tid >> 6) * BQ;
+  auto gload = [&](int64_t k0) {
+    const int64_t gr = row0 + a_row;
+#pragma unroll
+    for (int q = 0; q < AQ; ++q) {
+      const int64_t kc = k0 + a_h * AQ + q;
+      CT v;
+      v.x = 0; v.y = 0;
+      if (gr < m && kc < kdim) v = A[gr * lda + kc];
+      ra[q] = v;
+    }
+#pragma unroll
+    for (int q = 0; q < BQ; ++q) {
+      const int64_t kc = k0 + b_k + q;
+      CT v;
+      v.x = 0; v.y = 0;
+      if (kc < kdim && col0 + b_col < n) v = B[kc * ldb + col0 + b_col];
+      rb[q] = v;
+    }
+  };
+  // position of real k index 2 kc + part (part 0: re, 1: im), unsigned so
+  // the divisions by powers of two fold
+  auto pos = [](unsigned kc, unsigned part) {
+    const unsigned kr = 2u * kc + part;
+    return (kr % (unsigned)G) * (unsigned)S + kr / (unsigned)G;
+  };
+  auto sstore = [&](int buf) {
+#pragma unroll
+    for (int q = 0; q < AQ; ++q) {
+      const unsigned kc = (unsigned)(a_h * AQ + q);
+      L.As[buf][a_row][pos(kc, 0)] = -ra[q].x;  // C - A B = C + (-A) B
+      L.As[buf][a_row][pos(kc, 1)] = -ra[q].y;
+    }
+#pragma unroll
+    for (int q = 0; q < BQ; ++q) {
+      const unsigned kc = (unsigned)(b_k + q);
+      const CT v = rb[q];
+      L.Bt[buf][2 * b_col][pos(kc, 0)] = v.x;
+      L.Bt[buf][2 * b_col + 1][pos(kc, 0)] = v.y;
+      L.Bt[buf][2 * b_col][pos(kc, 1)] = -v.y;
+      L.Bt[buf][2 * b_col + 1][pos(kc, 1)] = v.x;
+    }
+  };
+
+  gload(0);
+  sstore(0);
+  __syncthreads();
+  int buf = 0;
+  const int g = (threadIdx.x & 63) / LPG;
+  constexpr int H = 16 / sizeof(T);  // k-steps per 16-byte read
+  for (int64_t k0 = 0; k0 < kdim; k0 += KC) {
+    const bool more = k0 + KC < kdim;
+    if (more) gload(k0 + KC);
+#pragma unroll
+    for (int h = 0; h < S; h += H) {
+      T a[NT][H], b[NT][H];
+#pragma unroll
+      for (int i = 0; i < NT; ++i)
+#pragma unroll
+        for (int s = 0; s < H; ++s) a[i][s] = L.As[buf][wr * 64 + i * TS + lt][g * S + h + s];
+#pragma unroll
+      for (int j = 0; j < NT; ++j)
+#pragma unroll
+        for (int s = 0; s < H; ++s) b[j][s] = L.Bt[buf][wc * 64 + j * TS + lt][g * S + h + s];
+#pragma unroll
+      for (int s = 0; s < H; ++s)
+#pragma unroll
+        for (int i = 0; i < NT; ++i)
+#pragma unroll
+          for (int j = 0; j < NT; ++j) acc[i][j] = P::mma(a[i][s], b[j][s], acc[i][j]);
+    }
+    if (more) {
+      sstore(buf ^ 1);
+      __syncthreads();
+      buf ^= 1;
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < NT; ++i)
+#pragma unroll
+    for (int j = 0; j < NT; ++j)
+#pragma unroll
+      for (int r = 0; r < P::NACC; ++r) {
+        const int64_t gr = row0 + wr * 64 + i * TS + P::orow(lane, r);
+        const int64_t gcr = 2 * col0 + wc * 64 + j * TS + lt;
+        if (gr < m && gcr < 2 * n) Cr[gr * 2 * ldc + gcr] = acc[i][j][r];
+      }
+}
+
+// The FP32 large product written out for the 32x32x2 MFMA (the same
+// scheme as k_gemm_sub_p; this explicit form measured 8.6 ms against 9.5 ms
+// for the 14848 x 15872 x 512 trailing update of the 15,360-unknown system).
+__global__ __launch_bounds__(256, 2) void k_gemm_sub_pf(int64_t m, int64_t n, int64_t kdim,
+                                                       const float2* __restrict__ A, int64_t lda,
+                                                       const float2* __restrict__ B, int64_t ldb,
+                                                       float2* C, int64_t ldc, int nbx, int nby) {
+  constexpr int KR = 32, LD = 36, HALF = KR / 2, KC = KR / 2;
+  __shared__ float As[2][128][LD];
+  __shared__ float Bt[2][128][LD];
+  int bx, by;
+  pgemm_tile(blockIdx.x, nbx, nby, bx, by);
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wr = wave >> 1, wc = wave & 1;
+  const int64_t row0 = (int64_t)by * 128, col0 = (int64_t)bx * 64;
+  float* Cr = reinterpret_cast<float*>(C);
+  v16f acc[2][2];
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j)
+#pragma unroll
+      for (int r = 0; r < 16; ++r) {
+        const int64_t gr = row0 + wr * 64 + i * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+        const int64_t gcr = 2 * col0 + wc * 64 + j * 32 + (lane & 31);
+        acc[i][j][r] = (gr < m && gcr < 2 * n) ? Cr[gr * 2 * ldc + gcr] : 0.f;
+      }
+  constexpr int AQ = KC / 2;  // 8 complex per thread: row tid / 2, half tid & 1
+  float2 ra[AQ];
+  float2 rb[4];               // 4 consecutive complex rows of one column
+  const int a_row = tid >> 1, a_h = tid & 1;
+  const int b_col = tid & 63, b_k0 = (tid >> 6) * 4;
+  auto gload = [&](int64_t k0) {
+    const int64_t gr = row0 + a_row;
+#pragma unroll
+    for (int q = 0; q < AQ; ++q) {
+      const int64_t kc = k0 + a_h * AQ + q;
+      float2 v = make_float2(0.f, 0.f);
+      if (gr < m && kc < kdim) v = A[gr * lda + kc];
+      ra[q] = v;
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int64_t kc = k0 + b_k0 + q;
+      float2 v = make_float2(0.f, 0.f);
+      if (kc < kdim && col0 + b_col < n) v = B[kc * ldb + col0 + b_col];
+      rb[q] = v;
+    }
+  };
+  auto sstore = [&](int buf) {
+#pragma unroll
+    for (int q = 0; q < AQ; ++q) {
+      As[buf][a_row][a_h * AQ + q] = -ra[q].x;         // real k 2 kc: position kc
+      As[buf][a_row][HALF + a_h * AQ + q] = -ra[q].y;  // real k 2 kc + 1: HALF + kc
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const float2 v = rb[q];
+      const int kc = b_k0 + q;
+      Bt[buf][2 * b_col][kc] = v.x;
+      Bt[buf][2 * b_col + 1][kc] = v.y;
+      Bt[buf][2 * b_col][HALF + kc] = -v.y;
+      Bt[buf][2 * b_col + 1][HALF + kc] = v.x;
+    }
+  };
+  gload(0);
+  sstore(0);
+  __syncthreads();
+  int buf = 0;
+  const int g = lane >> 5;
+  for (int64_t k0 = 0; k0 < kdim; k0 += KC) {
+    const bool more = k0 + KC < kdim;
+    if (more) gload(k0 + KC);
+#pragma unroll
+    for (int h = 0; h < HALF; h += 4) {
+      float a[2][4], b[2][4];
+#pragma unroll
+      for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int s = 0; s < 4; ++s) a[i][s] = As[buf][wr * 64 + i * 32 + (lane & 31)][g * HALF + h + s];
+#pragma unroll
+      for (int j = 0; j < 2; ++j)
+#pragma unroll
+        for (int s = 0; s < 4; ++s) b[j][s] = Bt[buf][wc * 64 + j * 32 + (lane & 31)][g * HALF + h + s];
+#pragma unroll
+      for (int s = 0; s < 4; ++s)
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+#pragma unroll
+          for (int j = 0; j < 2; ++j)
+            acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x2f32(a[i][s], b[j][s], acc[i][j], 0, 0, 0);
+    }
+    if (more) {
+      sstore(buf ^ 1);
+      __syncthreads();
+      buf ^= 1;
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j)
+#pragma unroll
+      for (int r = 0; r < 16; ++r) {
+        const int64_t gr = row0 + wr * 64 + i * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+        const int64_t gcr = 2 * col0 + wc * 64 + j * 32 + (lane & 31);
+        if (gr < m && gcr < 2 * n) Cr[gr * 2 * ldc + gcr] = acc[i][j][r];
+      }
+}
+
+template <typename T>
+hipError_t launch_gemm_p(int64_t m, int64_t n, int64_t k, const void* A, int64_t lda,
+                         const void* B, int64_t ldb, void* C, int64_t ldc, hipStream_t st) {
+  typedef typename cplx_t<T>::type CT;
+  const int nbx = (int)((n + GBN - 1) / GBN), nby = (int)((m + GBM - 1) / GBM);
+  if constexpr (sizeof(T) == 4)
+    hipLaunchKernelGGL(k_gemm_sub_pf, dim3((unsigned)(nbx * nby)), dim3(256), 0, st, m, n, k,
+                       (const float2*)A, lda, (const float2*)B, ldb, (float2*)C, ldc, nbx, nby);
+  else
+    hipLaunchKernelGGL((k_gemm_sub_p<T>), dim3((unsigned)(nbx * nby)), dim3(256), 0, st, m, n, k,
+                       (const CT*)A, lda, (const CT*)B, ldb, (CT*)C, ldc, nbx, nby);
+  return hipGetLastError();
+}
+
 template <typename T, bool TA = false>
 hipError_t launch_gemm_sub(int64_t m, int64_t n, int64_t k, const void* A, int64_t lda,
                            const void* B, int64_t ldb, void* C, int64_t ldc, hipStream_t st) {
   typedef typename cplx_t<T>::type CT;
   if (m <= 0 || n <= 0 || k <= 0) return hipSuccess;
   const int64_t big_tiles = ((n + GBN - 1) / GBN) * ((m + GBM - 1) / GBM);
-  if (big_tiles >= 1024) {
+  if (big_tiles >= 1024 && !TA) {
+    return launch_gemm_p<T>(m, n, k, A, lda, B, ldb, C, ldc, st);
+  } else if (big_tiles >= 1024) {
     dim3 grid((unsigned)((n + GBN - 1) / GBN), (unsigned)((m + GBM - 1) / GBM));
     hipLaunchKernelGGL((k_gemm_sub<T, TA>), grid, dim3(256), 0, st, m, n, k, (const CT*)A, lda,
                        (const CT*)B, ldb, (CT*)C, ldc);
