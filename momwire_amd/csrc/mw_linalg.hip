@@ -287,34 +287,18 @@ __global__ __launch_bounds__(256) void k_gemm_sub_s(int64_t m, int64_t n, int64_
 //    G the k groups of the MFMA's lanes, S = KR / G.  A lane then finds the k
 //    values of its successive MFMA steps side by side and reads them 16 bytes
 //    at a time (A and B use the same permutation, so the sum is unchanged).
-//  - FP64: 16x16x4 MFMA, each wave a 4 x 4 grid of 16 x 16 tiles, KR = 16
-//    real k per stage.  FP32: 32x32x2 MFMA (half the instructions and LDS
-//    reads of 16x16x4 for the same work), each wave a 2 x 2 grid of 32 x 32
-//    tiles, KR = 32 (k_gemm_sub_pf, whose 32 x 32 output element r of lane l
-//    sits at row 8 (r / 4) + 4 (l / 32) + r % 4, column l % 32).
+//  - FP64 (k_gemm_sub_pd): 16x16x4 MFMA, each wave a 4 x 4 grid of 16 x 16
+//    tiles, KR = 16 real k per stage.  FP32 (k_gemm_sub_pf): 32x32x2 MFMA
+//    (half the instructions and LDS reads of 16x16x4 for the same work),
+//    each wave a 2 x 2 grid of 32 x 32 tiles, KR = 32; its 32 x 32 output
+//    element r of lane l sits at row 8 (r / 4) + 4 (l / 32) + r % 4, column
+//    l % 32.
 //  - Tiles are visited in an XCD-aware grouped order (pgemm_tile): workgroup
 //    p runs on XCD p % 8, each XCD takes a contiguous range of the order, and
 //    the order walks 8 row tiles down a column of tiles before moving right,
 //    so the A rows and B columns a CU pair works on are shared in its XCD's L2.
 // ---------------------------------------------------------------------------
-template <typename T> struct PG;  // the FP64 instance; FP32 is k_gemm_sub_pf
-template <> struct PG<double> {
-  static constexpr int KR = 16, LD = 18, G = 4, TS = 16, NT = 4;  // tile size, tiles per wave side
-  typedef v4d acc_t;
-  static MW_DEV acc_t mma(double a, double b, acc_t c) {
-    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
-  }
-  static constexpr int NACC = 4;
-  // output element r of lane l: row (l / 16) + 4 r, column l % 16
-  static MW_DEV int orow(int lane, int r) { return (lane >> 4) + 4 * r; }
-};
 typedef float v16f __attribute__((ext_vector_type(16)));
-
-template <typename T>
-struct PGemmLds {
-  T As[2][GBM][PG<T>::LD];      // -A, permuted real k
-  T Bt[2][2 * GBN][PG<T>::LD];  // expanded B, [real column][permuted real k]
-};
 
 // tile (bx, by) of workgroup p over an nbx x nby tile grid: a bijection
 MW_DEV void pgemm_tile(int p, int nbx, int nby, int& bx, int& by) {
@@ -330,113 +314,114 @@ MW_DEV void pgemm_tile(int p, int nbx, int nby, int& bx, int& by) {
   bx = r / gm;
 }
 
-template <typename T>
-__global__ __launch_bounds__(256, 2) void k_gemm_sub_p(int64_t m, int64_t n, int64_t kdim,
-                                                      const typename cplx_t<T>::type* __restrict__ A,
-                                                      int64_t lda,
-                                                      const typename cplx_t<T>::type* __restrict__ B,
-                                                      int64_t ldb, typename cplx_t<T>::type* C,
-                                                      int64_t ldc, int nbx, int nby) {
-  typedef typename cplx_t<T>::type CT;
-  typedef PG<T> P;
-  constexpr int KR = P::KR, G = P::G, S = KR / G, KC = KR / 2, TS = P::TS, NT = P::NT;
-  constexpr int LPG = 64 / G;  // lanes per k group
-  const int lt = threadIdx.x & (TS - 1);  // the lane's row (A) / column (B, C) in a tile
-  __shared__ PGemmLds<T> L;
+// FP64: 16x16x4 MFMA, KR = 16 real k per stage (8 complex), LDS rows of 18
+// doubles (144 bytes: 16 consecutive rows hit distinct banks with 16-byte
+// accesses).  Permutation: position (kr % 4) 4 + kr / 4.  A is staged by
+// row (thread: row tid / 2, complex k 4 (tid & 1) + q) and written as
+// 16-byte pairs; B by REAL column (thread: real column tid & 127, complex
+// rows par + 2 q, par = tid >> 7), so consecutive lanes write consecutive
+// LDS rows.  Output element r of lane l: row l / 16 + 4 r, column l % 16.
+__global__ __launch_bounds__(256, 2) void k_gemm_sub_pd(int64_t m, int64_t n, int64_t kdim,
+                                                       const double2* __restrict__ A, int64_t lda,
+                                                       const double2* __restrict__ B, int64_t ldb,
+                                                       double2* C, int64_t ldc, int nbx, int nby) {
+  constexpr int LD = 18, S = 4, KC = 8;
+  __shared__ double As[2][128][LD];
+  __shared__ double Bt[2][128][LD];
   int bx, by;
   pgemm_tile(blockIdx.x, nbx, nby, bx, by);
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wr = wave >> 1, wc = wave & 1;  // the wave's 64 x 64 real quadrant
-  const int64_t row0 = (int64_t)by * GBM, col0 = (int64_t)bx * GBN;
-  T* Cr = reinterpret_cast<T*>(C);
-
-  typename P::acc_t acc[NT][NT];
+  const int64_t row0 = (int64_t)by * 128, col0 = (int64_t)bx * 64;
+  double* Cr = reinterpret_cast<double*>(C);
+  v4d acc[4][4];
 #pragma unroll
-  for (int i = 0; i < NT; ++i)
+  for (int i = 0; i < 4; ++i)
 #pragma unroll
-    for (int j = 0; j < NT; ++j)
+    for (int j = 0; j < 4; ++j)
 #pragma unroll
-      for (int r = 0; r < P::NACC; ++r) {
-        const int64_t gr = row0 + wr * 64 + i * TS + P::orow(lane, r);
-        const int64_t gcr = 2 * col0 + wc * 64 + j * TS + lt;
-        acc[i][j][r] = (gr < m && gcr < 2 * n) ? Cr[gr * 2 * ldc + gcr] : (T)0;
+      for (int r = 0; r < 4; ++r) {
+        const int64_t gr = row0 + wr * 64 + i * 16 + (lane >> 4) + 4 * r;
+        const int64_t gcr = 2 * col0 + wc * 64 + j * 16 + (lane & 15);
+        acc[i][j][r] = (gr < m && gcr < 2 * n) ? Cr[gr * 2 * ldc + gcr] : 0.0;
       }
-
-  // staging: A 128 rows x KC complex (thread: row tid / 2, half tid % 2);
-  // B KC rows x 64 complex (thread: column tid % 64, rows (tid / 64) BQ + q)
-  constexpr int AQ = KC / 2, BQ = KC / 4;
-  CT ra[AQ], rb[BQ];
+  double2 ra[4], rb[4];
   const int a_row = tid >> 1, a_h = tid & 1;
-  const int b_col = tid & 63, b_k = (tid >> 6) * BQ;
+  const int cr = tid & 127, b_par = tid >> 7;
   auto gload = [&](int64_t k0) {
     const int64_t gr = row0 + a_row;
 #pragma unroll
-    for (int q = 0; q < AQ; ++q) {
-      const int64_t kc = k0 + a_h * AQ + q;
-      CT v;
-      v.x = 0; v.y = 0;
+    for (int q = 0; q < 4; ++q) {
+      const int64_t kc = k0 + a_h * 4 + q;
+      double2 v = make_double2(0.0, 0.0);
       if (gr < m && kc < kdim) v = A[gr * lda + kc];
       ra[q] = v;
     }
 #pragma unroll
-    for (int q = 0; q < BQ; ++q) {
-      const int64_t kc = k0 + b_k + q;
-      CT v;
-      v.x = 0; v.y = 0;
-      if (kc < kdim && col0 + b_col < n) v = B[kc * ldb + col0 + b_col];
+    for (int q = 0; q < 4; ++q) {
+      const int64_t kc = k0 + b_par + 2 * q;
+      double2 v = make_double2(0.0, 0.0);
+      if (kc < kdim && col0 + (cr >> 1) < n) v = B[kc * ldb + col0 + (cr >> 1)];
       rb[q] = v;
     }
   };
-  // position of real k index 2 kc + part (part 0: re, 1: im), unsigned so
-  // the divisions by powers of two fold
-  auto pos = [](unsigned kc, unsigned part) {
-    const unsigned kr = 2u * kc + part;
-    return (kr % (unsigned)G) * (unsigned)S + kr / (unsigned)G;
-  };
   auto sstore = [&](int buf) {
+    // A's reals kr = 8 a_h + 2 q + part sit at ((2q + part) % 4) 4 + 2 a_h +
+    // (2q + part) / 4: complex q and q + 2 of one part are neighbours.
+    // C - A B = C + (-A) B.
 #pragma unroll
-    for (int q = 0; q < AQ; ++q) {
-      const unsigned kc = (unsigned)(a_h * AQ + q);
-      L.As[buf][a_row][pos(kc, 0)] = -ra[q].x;  // C - A B = C + (-A) B
-      L.As[buf][a_row][pos(kc, 1)] = -ra[q].y;
+    for (int q = 0; q < 2; ++q) {
+      *reinterpret_cast<double2*>(&As[buf][a_row][(2 * q) * S + 2 * a_h]) =
+          make_double2(-ra[q].x, -ra[q + 2].x);
+      *reinterpret_cast<double2*>(&As[buf][a_row][(2 * q + 1) * S + 2 * a_h]) =
+          make_double2(-ra[q].y, -ra[q + 2].y);
     }
+    // B_r[2kc][cr] = cr even ? br : bi and B_r[2kc + 1][cr] = cr even ? -bi : br,
+    // at k groups 2 par and 2 par + 1, step q
+    const bool im = cr & 1;
+    double v0[4], v1[4];
 #pragma unroll
-    for (int q = 0; q < BQ; ++q) {
-      const unsigned kc = (unsigned)(b_k + q);
-      const CT v = rb[q];
-      L.Bt[buf][2 * b_col][pos(kc, 0)] = v.x;
-      L.Bt[buf][2 * b_col + 1][pos(kc, 0)] = v.y;
-      L.Bt[buf][2 * b_col][pos(kc, 1)] = -v.y;
-      L.Bt[buf][2 * b_col + 1][pos(kc, 1)] = v.x;
+    for (int q = 0; q < 4; ++q) {
+      v0[q] = im ? rb[q].y : rb[q].x;
+      v1[q] = im ? rb[q].x : -rb[q].y;
     }
+    *reinterpret_cast<double2*>(&Bt[buf][cr][(2 * b_par) * S]) = make_double2(v0[0], v0[1]);
+    *reinterpret_cast<double2*>(&Bt[buf][cr][(2 * b_par) * S + 2]) = make_double2(v0[2], v0[3]);
+    *reinterpret_cast<double2*>(&Bt[buf][cr][(2 * b_par + 1) * S]) = make_double2(v1[0], v1[1]);
+    *reinterpret_cast<double2*>(&Bt[buf][cr][(2 * b_par + 1) * S + 2]) = make_double2(v1[2], v1[3]);
   };
-
   gload(0);
   sstore(0);
   __syncthreads();
   int buf = 0;
-  const int g = (threadIdx.x & 63) / LPG;
-  constexpr int H = 16 / sizeof(T);  // k-steps per 16-byte read
+  const int g = lane >> 4;
   for (int64_t k0 = 0; k0 < kdim; k0 += KC) {
     const bool more = k0 + KC < kdim;
     if (more) gload(k0 + KC);
 #pragma unroll
-    for (int h = 0; h < S; h += H) {
-      T a[NT][H], b[NT][H];
+    for (int h = 0; h < S; h += 2) {
+      double a[4][2], b[4][2];
 #pragma unroll
-      for (int i = 0; i < NT; ++i)
+      for (int i = 0; i < 4; ++i) {
+        const double2 t =
+            *reinterpret_cast<const double2*>(&As[buf][wr * 64 + i * 16 + (lane & 15)][g * S + h]);
+        a[i][0] = t.x;
+        a[i][1] = t.y;
+      }
 #pragma unroll
-        for (int s = 0; s < H; ++s) a[i][s] = L.As[buf][wr * 64 + i * TS + lt][g * S + h + s];
+      for (int j = 0; j < 4; ++j) {
+        const double2 t =
+            *reinterpret_cast<const double2*>(&Bt[buf][wc * 64 + j * 16 + (lane & 15)][g * S + h]);
+        b[j][0] = t.x;
+        b[j][1] = t.y;
+      }
 #pragma unroll
-      for (int j = 0; j < NT; ++j)
+      for (int s = 0; s < 2; ++s)
 #pragma unroll
-        for (int s = 0; s < H; ++s) b[j][s] = L.Bt[buf][wc * 64 + j * TS + lt][g * S + h + s];
+        for (int i = 0; i < 4; ++i)
 #pragma unroll
-      for (int s = 0; s < H; ++s)
-#pragma unroll
-        for (int i = 0; i < NT; ++i)
-#pragma unroll
-          for (int j = 0; j < NT; ++j) acc[i][j] = P::mma(a[i][s], b[j][s], acc[i][j]);
+          for (int j = 0; j < 4; ++j)
+            acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i][s], b[j][s], acc[i][j], 0, 0, 0);
     }
     if (more) {
       sstore(buf ^ 1);
@@ -445,13 +430,13 @@ __global__ __launch_bounds__(256, 2) void k_gemm_sub_p(int64_t m, int64_t n, int
     }
   }
 #pragma unroll
-  for (int i = 0; i < NT; ++i)
+  for (int i = 0; i < 4; ++i)
 #pragma unroll
-    for (int j = 0; j < NT; ++j)
+    for (int j = 0; j < 4; ++j)
 #pragma unroll
-      for (int r = 0; r < P::NACC; ++r) {
-        const int64_t gr = row0 + wr * 64 + i * TS + P::orow(lane, r);
-        const int64_t gcr = 2 * col0 + wc * 64 + j * TS + lt;
+      for (int r = 0; r < 4; ++r) {
+        const int64_t gr = row0 + wr * 64 + i * 16 + (lane >> 4) + 4 * r;
+        const int64_t gcr = 2 * col0 + wc * 64 + j * 16 + (lane & 15);
         if (gr < m && gcr < 2 * n) Cr[gr * 2 * ldc + gcr] = acc[i][j][r];
       }
 }
@@ -569,14 +554,13 @@ __global__ __launch_bounds__(256, 2) void k_gemm_sub_pf(int64_t m, int64_t n, in
 template <typename T>
 hipError_t launch_gemm_p(int64_t m, int64_t n, int64_t k, const void* A, int64_t lda,
                          const void* B, int64_t ldb, void* C, int64_t ldc, hipStream_t st) {
-  typedef typename cplx_t<T>::type CT;
   const int nbx = (int)((n + GBN - 1) / GBN), nby = (int)((m + GBM - 1) / GBM);
   if constexpr (sizeof(T) == 4)
     hipLaunchKernelGGL(k_gemm_sub_pf, dim3((unsigned)(nbx * nby)), dim3(256), 0, st, m, n, k,
                        (const float2*)A, lda, (const float2*)B, ldb, (float2*)C, ldc, nbx, nby);
   else
-    hipLaunchKernelGGL((k_gemm_sub_p<T>), dim3((unsigned)(nbx * nby)), dim3(256), 0, st, m, n, k,
-                       (const CT*)A, lda, (const CT*)B, ldb, (CT*)C, ldc, nbx, nby);
+    hipLaunchKernelGGL(k_gemm_sub_pd, dim3((unsigned)(nbx * nby)), dim3(256), 0, st, m, n, k,
+                       (const double2*)A, lda, (const double2*)B, ldb, (double2*)C, ldc, nbx, nby);
   return hipGetLastError();
 }
 
