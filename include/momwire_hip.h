@@ -94,6 +94,9 @@ typedef struct mw_mesh {
   const int32_t* ek_img_src; /* mirrored sources in the ground image block,
                               from one scan over both (NULL if !ek or no
                               ground)                                        */
+  const double* se_static; /* [n_se][9] k-independent static part of the
+                              same-edge table (mw_same_edge_static), or NULL:
+                              mw_same_edge_table then computes it per call  */
 } mw_mesh;
 
 /*
@@ -159,6 +162,17 @@ int mw_cancel_destroy(int32_t* flag);
  * reference calls per edge (_bspline_kernels.py:333-517).
  */
 int mw_same_edge_table(const mw_mesh* mesh, double k, void* stream);
+
+/*
+ * The k-independent half of the same-edge table, once per mesh: the static
+ * closed-form moments h^{p+q+1} I(p, q, delta; a/h) / (4 pi) of every entry,
+ * out: [n_se][9] doubles (device).  Point mesh->se_static at it afterwards and
+ * every mw_same_edge_table call adds only the k-dependent smooth remainder.
+ * Replaces the reference's per-edge seg_seg_static_moments_bspline_uniform
+ * (_accelerators.cpp:2822-2951), which its swept path also computes once per
+ * geometry (_bspline_kernels.py:333-422).
+ */
+int mw_same_edge_static(const mw_mesh* mesh, double* out, void* stream);
 
 /*
  * Fused Galerkin fill Z[m][n] for all basis pairs, free space.  Replaces

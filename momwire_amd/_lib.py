@@ -55,6 +55,7 @@ class MwMesh(ctypes.Structure):
         ("ek_label", ctypes.c_void_p),
         ("ek_img_obs", ctypes.c_void_p),
         ("ek_img_src", ctypes.c_void_p),
+        ("se_static", ctypes.c_void_p),
     ]
 
 
@@ -99,6 +100,7 @@ SIGNATURES = {
     "mw_cancel_create": (ctypes.c_int, [ctypes.POINTER(ctypes.POINTER(ctypes.c_int32))]),
     "mw_cancel_destroy": (ctypes.c_int, [ctypes.POINTER(ctypes.c_int32)]),
     "mw_same_edge_table": (ctypes.c_int, [ctypes.POINTER(MwMesh), _D, _P]),
+    "mw_same_edge_static": (ctypes.c_int, [ctypes.POINTER(MwMesh), _P, _P]),
     "mw_fill_z": (ctypes.c_int, [ctypes.POINTER(MwMesh), _D, _D, _D, _D, _P, _I64, _P, _I32,
                                  _I32, _P]),
     "mw_getrf_workspace_bytes": (_I64, [_I64]),

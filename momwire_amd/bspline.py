@@ -152,6 +152,14 @@ class _Device:
             tile_b0=k["tile_b0"].data_ptr(), tile_seg=k["tile_seg"].data_ptr(),
             tile_nseg=k["tile_nseg"].data_ptr(), gl=k["gl"].data_ptr(),
             ek_label=ek_ptr[0], ek_img_obs=ek_ptr[1], ek_img_src=ek_ptr[2])
+        # the k-independent static half of the same-edge table, once per mesh
+        n_se = len(mesh.se_edge)
+        if n_se:
+            self.keep["se_static"] = torch.empty((n_se, 9), dtype=torch.float64, device=dev)
+            ptr = self.keep["se_static"].data_ptr()
+            _lib.check(_lib.load().mw_same_edge_static(
+                self.mw, ptr, torch.cuda.current_stream().cuda_stream), "same-edge static table")
+            self.mw.se_static = ptr
         nb = mesh.n_basis
         self.nb = nb
         self.n_ports = len(rows_port)

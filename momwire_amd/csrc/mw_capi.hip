@@ -10,6 +10,8 @@
 #include "mw_common.h"
 
 hipError_t mw_launch_same_edge(const mw_mesh& M, double k, hipStream_t st);
+hipError_t mw_launch_same_edge_static(const mw_mesh& M, double* out, hipStream_t st);
+int64_t mw_same_edge_blocks(int32_t n_se);
 hipError_t mw_launch_fill(const mw_mesh& M, double k, double omega, double eps, double mu,
                           double2* Z, int64_t ldz, const int32_t* pairs, int32_t n_pairs,
                           bool symmetric, hipStream_t st);
@@ -41,7 +43,8 @@ hipError_t mw_launch_c2z(int64_t rows, int64_t cols, const void* src, int64_t ld
 void mw_set_pivot_bound(double lbound);
 double mw_get_pivot_bound();
 hipError_t mw_launch_batch_fill(const mw_sweep_prob* P, const mw_mesh* meshes, int nprob,
-                                int max_nse, const int32_t* item_prob, const int32_t* item_pair,
+                                const int32_t* se_items, int64_t n_se_items,
+                                const int32_t* item_prob, const int32_t* item_pair,
                                 int64_t n_items, double eps, double mu, hipStream_t st);
 int mw_sweep_lu_run(const mw_sweep_prob* dP, const mw_ports* dports, const mw_sweep_prob* hp,
                     int nprob, int max_ports, int max_con, int64_t max_scatter,
@@ -200,6 +203,15 @@ int mw_same_edge_table(const mw_mesh* mesh, double k, void* stream) {
   if (rc) return rc;
   if (!(k > 0.0)) return fail(MW_EINVAL, "%swavenumber must be positive");
   return hip_status(mw_launch_same_edge(*mesh, k, (hipStream_t)stream), "mw_same_edge_table");
+}
+
+int mw_same_edge_static(const mw_mesh* mesh, double* out, void* stream) {
+  g_err[0] = 0;
+  int rc = check_mesh(mesh);
+  if (rc) return rc;
+  if (mesh->n_se > 0 && !out) return fail(MW_EINVAL, "%sout is NULL");
+  return hip_status(mw_launch_same_edge_static(*mesh, out, (hipStream_t)stream),
+                    "mw_same_edge_static");
 }
 
 int mw_fill_z(const mw_mesh* mesh, double k, double omega, double eps, double mu, void* Z,
@@ -514,13 +526,14 @@ int mw_c2z(int64_t rows, int64_t cols, const void* src, int64_t lds, void* dst, 
 
 struct mw_sweep_plan {
   int nprob = 0, n_mesh = 0, gemm_variant = 0, max_ports = 0, max_con = 0, max_nse = 0;
-  int64_t max_scatter = 0, n_items = 0;
+  int64_t max_scatter = 0, n_items = 0, n_se_items = 0;
   double pivot_bound = 32.0;
   mw_sweep_prob* dP = nullptr;
   mw_ports* dports = nullptr;
   mw_mesh* dmeshes = nullptr;
   int32_t* d_item_prob = nullptr;
   int32_t* d_item_pair = nullptr;
+  int32_t* d_se_items = nullptr;   // (problem, 256-thread block) of the same-edge tables
   uint32_t* d_work = nullptr;      // per-step work lists of the factorisation
   std::vector<int64_t> work_off;
   std::vector<mw_sweep_prob> hp;
@@ -533,6 +546,7 @@ static void plan_free(mw_sweep_plan* p) {
   (void)hipFree(p->dmeshes);
   (void)hipFree(p->d_item_prob);
   (void)hipFree(p->d_item_pair);
+  (void)hipFree(p->d_se_items);
   (void)hipFree(p->d_work);
   delete p;
 }
@@ -588,6 +602,15 @@ int mw_sweep_plan_create(int32_t n_mesh, const mw_mesh* meshes, const mw_ports* 
       ip.push_back(i);
       iq.push_back(j);
     }
+  std::vector<int32_t> se_items;
+  for (int i = 0; i < nprob; ++i) {
+    const int64_t nb = mw_same_edge_blocks(meshes[p->hp[i].mesh].n_se);
+    for (int64_t b = 0; b < nb; ++b) {
+      se_items.push_back(i);
+      se_items.push_back((int32_t)b);
+    }
+  }
+  p->n_se_items = (int64_t)se_items.size() / 2;
   hipError_t e = hipSuccess;
   if ((e = hipMalloc(&p->dP, sizeof(mw_sweep_prob) * nprob)) != hipSuccess ||
       (e = hipMalloc(&p->dports, sizeof(mw_ports) * n_mesh)) != hipSuccess ||
@@ -599,6 +622,9 @@ int mw_sweep_plan_create(int32_t n_mesh, const mw_mesh* meshes, const mw_ports* 
       (e = hipMemcpy(p->dmeshes, meshes, sizeof(mw_mesh) * n_mesh, hipMemcpyHostToDevice)) != hipSuccess ||
       (e = hipMemcpy(p->d_item_prob, ip.data(), sizeof(int32_t) * ip.size(), hipMemcpyHostToDevice)) != hipSuccess ||
       (e = hipMemcpy(p->d_item_pair, iq.data(), sizeof(int32_t) * iq.size(), hipMemcpyHostToDevice)) != hipSuccess ||
+      (e = hipMalloc(&p->d_se_items, sizeof(int32_t) * std::max<size_t>(se_items.size(), 2))) != hipSuccess ||
+      (!se_items.empty() &&
+       (e = hipMemcpy(p->d_se_items, se_items.data(), sizeof(int32_t) * se_items.size(), hipMemcpyHostToDevice)) != hipSuccess) ||
       (e = hipMalloc(&p->d_work, sizeof(uint32_t) * std::max<size_t>(work.size(), 1))) != hipSuccess ||
       (!work.empty() &&
        (e = hipMemcpy(p->d_work, work.data(), sizeof(uint32_t) * work.size(), hipMemcpyHostToDevice)) != hipSuccess)) {
@@ -614,8 +640,9 @@ int mw_sweep_run(mw_sweep_plan* p, double eps, double mu, const volatile int32_t
   g_err[0] = 0;
   if (!p) return fail(MW_EINVAL, "%splan is NULL");
   hipStream_t st = (hipStream_t)stream;
-  hipError_t herr = mw_launch_batch_fill(p->dP, p->dmeshes, p->nprob, p->max_nse, p->d_item_prob,
-                                         p->d_item_pair, p->n_items, eps, mu, st);
+  hipError_t herr = mw_launch_batch_fill(p->dP, p->dmeshes, p->nprob, p->d_se_items,
+                                         p->n_se_items, p->d_item_prob, p->d_item_pair,
+                                         p->n_items, eps, mu, st);
   if (herr != hipSuccess) return hip_status(herr, "sweep fill");
   int rc = mw_sweep_lu_run(p->dP, p->dports, p->hp.data(), p->nprob, p->max_ports, p->max_con,
                            p->max_scatter, p->gemm_variant, p->pivot_bound, p->d_work,

@@ -142,73 +142,110 @@ MW_DEV double static_unit(int p, int q, int delta, double ah, const double* glx,
   return total;
 }
 
-template <int NQ>
-MW_DEV void same_edge_body(int e, const mw_mesh& M, double k) {
-  if (e >= M.n_se) return;
+// The k-independent part of a same-edge entry: static moment (p, q) of entry e,
+// h^{p+q+1} I_unit(p, q, delta; a/h) / (4 pi).
+MW_DEV double same_edge_static(int e, int p, int q, const mw_mesh& M, int nq) {
   const int ed = M.se_edge[e];
   const int n_e = M.edge_i[2 * ed], off = M.edge_i[2 * ed + 1];
   const int delta = e - off - (n_e - 1);
   const double h = M.edge[2 * ed], a = M.edge[2 * ed + 1];
+  const double* glx = M.gl + 2 * nq;
+  const double* glw = glx + 16;
+  double hp = h;
+  for (int t = 0; t < p + q; ++t) hp *= h;
+  return hp * static_unit(p, q, delta, a / h, glx, glw, M.ek != 0) * INV4PI;
+}
+
+// One thread per (entry, p, q): the static table, once per mesh.
+__global__ __launch_bounds__(256) void k_same_edge_static(mw_mesh M, int nq, double* out) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (int64_t)M.n_se * 9) return;
+  const int e = (int)(t / 9), pq = (int)(t % 9);
+  out[t] = same_edge_static(e, pq / 3, pq % 3, M, nq);
+}
+
+// Same-edge entry e at wavenumber k: the static part (from M.se_static when
+// the mesh carries it, else computed here) plus the smooth remainder
+// (exp(-jkR) - 1)/(4 pi R) on the arc distance.  A group of LG lanes owns an
+// entry: lane j takes Gauss point pair j (q = j / NQ, r = j % NQ), the
+// group's nine complex partial moments are summed by shuffles, and lane
+// j < 9 writes moment (j / 3, j % 3).
+template <int NQ>
+struct SeLanes {
+  static constexpr int LG = NQ * NQ <= 16 ? 16 : 64;
+};
+
+template <int NQ>
+MW_DEV void same_edge_body(int64_t t, const mw_mesh& M, double k) {
+  constexpr int LG = SeLanes<NQ>::LG;
+  const int e = (int)(t / LG), j = (int)(t % LG);
+  const bool live = e < M.n_se;
+  const int ec = live ? e : 0;
+  const int ed = M.se_edge[ec];
+  const int n_e = M.edge_i[2 * ed], off = M.edge_i[2 * ed + 1];
+  const int delta = ec - off - (n_e - 1);
+  const double h = M.edge[2 * ed], a = M.edge[2 * ed + 1];
   const double* t01 = M.gl;
   const double* w01 = M.gl + NQ;
-  const double* glx = M.gl + 2 * NQ;
-  const double* glw = glx + 16;
-
-  double jr[NM][NM], ji[NM][NM];
-  // static part: h^{p+q+1} * I_unit / (4 pi)
-  const double ah = a / h;
-  for (int p = 0; p < NM; ++p)
-    for (int q = 0; q < NM; ++q) {
-      double hp = h;
-      for (int t = 0; t < p + q; ++t) hp *= h;
-      jr[p][q] = hp * static_unit(p, q, delta, ah, glx, glw, M.ek != 0) * INV4PI;
-      ji[p][q] = 0.0;
-    }
-  // smooth remainder (exp(-jkR)-1)/(4 pi R) on the arc distance
-  for (int q = 0; q < NQ; ++q) {
-    double wq[NM];
-    const double uq = t01[q] * h;
+  double jr[NM * NM], ji[NM * NM];
+#pragma unroll
+  for (int i = 0; i < NM * NM; ++i) jr[i] = ji[i] = 0.0;
+  if (live && j < NQ * NQ) {
+    const int q = j / NQ, r = j % NQ;
+    double wq[NM], wr[NM];
+    const double uq = t01[q] * h, ur = t01[r] * h;
     wq[0] = w01[q] * h; wq[1] = wq[0] * uq; wq[2] = wq[1] * uq;
-    for (int r = 0; r < NQ; ++r) {
-      double wr[NM];
-      const double ur = t01[r] * h;
-      wr[0] = w01[r] * h; wr[1] = wr[0] * ur; wr[2] = wr[1] * ur;
-      const double ds = (t01[q] - t01[r] - (double)delta) * h;
-      const double R = sqrt(ds * ds + a * a);
-      double sn, cs, sh, chh;
-      sincos(k * R, &sn, &cs);
-      sincos(0.5 * k * R, &sh, &chh);
-      const double inv = INV4PI / R;
-      double gr = -2.0 * sh * sh * inv;  // cos(kR) - 1
-      double gi = -sn * inv;
-      if (M.ek) {
-        // extended kernel: [(exp(-jkR) - 1) fac + extra] / (4 pi R),
-        // extra = fac - fac_static = T1 (3jkR - (kR)^2) - T2 jkR, term by term
-        // (reference _bspline_kernels.py:225-246, 464-477)
-        const double2 f = ek_factor(R, a, k);
-        const double r2 = R * R, kr = k * R;
-        const double t1 = 0.25 * a * a * a * a / (r2 * r2), t2 = 0.5 * a * a / r2;
-        const double er = -t1 * kr * kr, ei = kr * (3.0 * t1 - t2);
-        const double g0 = gr, g1 = gi;
-        gr = g0 * f.x - g1 * f.y + er * inv;
-        gi = g0 * f.y + g1 * f.x + ei * inv;
-      }
-      for (int p = 0; p < NM; ++p)
-        for (int P = 0; P < NM; ++P) {
-          const double w = wq[p] * wr[P];
-          jr[p][P] += w * gr;
-          ji[p][P] += w * gi;
-        }
+    wr[0] = w01[r] * h; wr[1] = wr[0] * ur; wr[2] = wr[1] * ur;
+    const double ds = (t01[q] - t01[r] - (double)delta) * h;
+    const double R = sqrt(ds * ds + a * a);
+    double sn, cs, sh, chh;
+    sincos(k * R, &sn, &cs);
+    sincos(0.5 * k * R, &sh, &chh);
+    const double inv = INV4PI / R;
+    double gr = -2.0 * sh * sh * inv;  // cos(kR) - 1
+    double gi = -sn * inv;
+    if (M.ek) {
+      // extended kernel: [(exp(-jkR) - 1) fac + extra] / (4 pi R),
+      // extra = fac - fac_static = T1 (3jkR - (kR)^2) - T2 jkR, term by term
+      // (reference _bspline_kernels.py:225-246, 464-477)
+      const double2 f = ek_factor(R, a, k);
+      const double r2 = R * R, kr = k * R;
+      const double t1 = 0.25 * a * a * a * a / (r2 * r2), t2 = 0.5 * a * a / r2;
+      const double er = -t1 * kr * kr, ei = kr * (3.0 * t1 - t2);
+      const double g0 = gr, g1 = gi;
+      gr = g0 * f.x - g1 * f.y + er * inv;
+      gi = g0 * f.y + g1 * f.x + ei * inv;
     }
+#pragma unroll
+    for (int p = 0; p < NM; ++p)
+#pragma unroll
+      for (int P = 0; P < NM; ++P) {
+        const double w = wq[p] * wr[P];
+        jr[p * NM + P] = w * gr;
+        ji[p * NM + P] = w * gi;
+      }
   }
-  double2* out = reinterpret_cast<double2*>(M.se_tab) + (size_t)e * 9;
-  for (int p = 0; p < NM; ++p)
-    for (int q = 0; q < NM; ++q) out[p * NM + q] = make_double2(jr[p][q], ji[p][q]);
+  // sum over the group's lanes (LG is a power of two; groups are aligned)
+#pragma unroll
+  for (int o = LG / 2; o > 0; o >>= 1)
+#pragma unroll
+    for (int i = 0; i < NM * NM; ++i) {
+      jr[i] += __shfl_xor(jr[i], o, 64);
+      ji[i] += __shfl_xor(ji[i], o, 64);
+    }
+  if (!live || j >= NM * NM) return;
+  double rr = 0.0, ri = 0.0;
+#pragma unroll
+  for (int i = 0; i < NM * NM; ++i)
+    if (i == j) { rr = jr[i]; ri = ji[i]; }
+  const double st = M.se_static ? M.se_static[(size_t)e * 9 + j]
+                                : same_edge_static(e, j / NM, j % NM, M, NQ);
+  reinterpret_cast<double2*>(M.se_tab)[(size_t)e * 9 + j] = make_double2(st + rr, ri);
 }
 
 template <int NQ>
 __global__ __launch_bounds__(256) void k_same_edge(mw_mesh M, double k) {
-  same_edge_body<NQ>(blockIdx.x * blockDim.x + threadIdx.x, M, k);
+  same_edge_body<NQ>((int64_t)blockIdx.x * blockDim.x + threadIdx.x, M, k);
 }
 
 // ---------------------------------------------------------------------------
@@ -414,11 +451,13 @@ __global__ __launch_bounds__(256) void k_fill(mw_mesh M, double k, double wmu,
 // ---------------------------------------------------------------------------
 // Batched fill over many (structure, wavenumber) problems
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void kb_same_edge(const mw_sweep_prob* P, const mw_mesh* meshes) {
-  const mw_sweep_prob& p = P[blockIdx.y];
+// one workgroup per (problem, block of entries) work item
+__global__ __launch_bounds__(256) void kb_same_edge(const mw_sweep_prob* P, const mw_mesh* meshes,
+                                                    const int32_t* __restrict__ items) {
+  const mw_sweep_prob& p = P[items[2 * blockIdx.x]];
   mw_mesh M = meshes[p.mesh];
   M.se_tab = p.se_tab;
-  same_edge_body<4>(blockIdx.x * blockDim.x + threadIdx.x, M, p.k);
+  same_edge_body<4>((int64_t)items[2 * blockIdx.x + 1] * blockDim.x + threadIdx.x, M, p.k);
 }
 
 // one workgroup per (problem, tile pair) work item
@@ -472,11 +511,23 @@ hipError_t launch_fill_nq(const mw_mesh& M, double k, double wmu, double inv_wep
 // ---------------------------------------------------------------------------
 hipError_t mw_launch_same_edge(const mw_mesh& M, double k, hipStream_t st) {
   if (M.n_se == 0) return hipSuccess;
-#define MW_SE(NQ)                                                                        \
-  hipLaunchKernelGGL(k_same_edge<NQ>, dim3((M.n_se + 255) / 256), dim3(256), 0, st, M, k); \
-  return hipGetLastError();
+#define MW_SE(NQ)                                                                          \
+  {                                                                                        \
+    const int64_t thr = (int64_t)M.n_se * SeLanes<NQ>::LG;                                 \
+    hipLaunchKernelGGL(k_same_edge<NQ>, dim3((unsigned)((thr + 255) / 256)), dim3(256), 0, \
+                       st, M, k);                                                          \
+    return hipGetLastError();                                                              \
+  }
   MW_NQ_SWITCH(M.n_qp, MW_SE)
 #undef MW_SE
+}
+
+hipError_t mw_launch_same_edge_static(const mw_mesh& M, double* out, hipStream_t st) {
+  if (M.n_se == 0) return hipSuccess;
+  const int64_t thr = (int64_t)M.n_se * 9;
+  hipLaunchKernelGGL(k_same_edge_static, dim3((unsigned)((thr + 255) / 256)), dim3(256), 0, st, M,
+                     (int)M.n_qp, out);
+  return hipGetLastError();
 }
 
 hipError_t mw_launch_fill(const mw_mesh& M, double k, double omega, double eps, double mu,
@@ -488,13 +539,19 @@ hipError_t mw_launch_fill(const mw_mesh& M, double k, double omega, double eps, 
 #undef MW_FILL
 }
 
+// 256-thread blocks of a batched same-edge table (n_qp_pair = 4)
+int64_t mw_same_edge_blocks(int32_t n_se) {
+  return ((int64_t)n_se * SeLanes<4>::LG + 255) / 256;
+}
+
 hipError_t mw_launch_batch_fill(const mw_sweep_prob* P, const mw_mesh* meshes, int nprob,
-                                int max_nse, const int32_t* item_prob, const int32_t* item_pair,
+                                const int32_t* se_items, int64_t n_se_items,
+                                const int32_t* item_prob, const int32_t* item_pair,
                                 int64_t n_items, double eps, double mu, hipStream_t st) {
   if (nprob <= 0) return hipSuccess;
-  if (max_nse > 0)
-    hipLaunchKernelGGL(kb_same_edge, dim3((max_nse + 255) / 256, nprob), dim3(256), 0, st, P,
-                       meshes);
+  if (n_se_items > 0)
+    hipLaunchKernelGGL(kb_same_edge, dim3((unsigned)n_se_items), dim3(256), 0, st, P, meshes,
+                       se_items);
   if (n_items > 0)
     hipLaunchKernelGGL(kb_fill, dim3((unsigned)n_items), dim3(256), 0, st, P, meshes, item_prob,
                        item_pair, eps, mu);
