@@ -582,37 +582,49 @@ class BSplineSolver(_SweptPortSolutions):
         """Free-space fill, then the ground terms: Z -= C2 * image + Q
         (PEC: C2 = 1, no Q), or Z -= the Fresnel-weighted image for the
         reflection-coefficient ground."""
+        self._fill_launcher(D, k, omega, zptr, ld, st)()
+
+    def _fill_launcher(self, D, k, omega, zptr, ld, st):
+        """Resolve everything the fill needs on the host (loading values,
+        ground grid, weights) and return a function that only launches its
+        kernels."""
         lib = _lib.load()
         mesh = self._mesh
         pairs = D.keep["pairs"]
-        _lib.check(lib.mw_fill_z(D.mw, k, omega, self.eps, self.mu, zptr, ld, pairs.data_ptr(),
-                                 int(pairs.shape[0]), int(mesh.symmetric), st), "fill")
+        sym = int(mesh.symmetric)
+        calls = [lambda: _lib.check(lib.mw_fill_z(D.mw, k, omega, self.eps, self.mu, zptr, ld,
+                                                  pairs.data_ptr(), int(pairs.shape[0]), sym,
+                                                  st), "fill")]
         if self._loading_active:
             rows, cols, vals = self._loading_device(D, omega)
-            _lib.check(lib.mw_zadd_sparse(D.nb, int(rows.shape[0]), rows.data_ptr(),
-                                          cols.data_ptr(), vals.data_ptr(), zptr, ld, st),
-                       "wire loading")
-        if self.ground_z is None:
-            return
-        if self.ground_eps is not None and self.ground_model == "refl-coef":
+            calls.append(lambda: _lib.check(lib.mw_zadd_sparse(
+                D.nb, int(rows.shape[0]), rows.data_ptr(), cols.data_ptr(), vals.data_ptr(), zptr,
+                ld, st), "wire loading"))
+        if self.ground_z is not None and self.ground_eps is not None \
+                and self.ground_model == "refl-coef":
             eps_t = _ground.eps_tilde(self.ground_eps, omega, self.eps)
             c0, c1 = _ground.phi_coefficients(self.ground_phi_mode, eps_t)
-            _lib.check(lib.mw_fill_ground_refl(D.mw, k, omega, self.eps, self.mu, self.ground_z,
-                                               eps_t.real, eps_t.imag, c0.real, c0.imag, c1.real,
-                                               c1.imag, zptr, ld, pairs.data_ptr(),
-                                               int(pairs.shape[0]), int(mesh.symmetric), st),
-                       "reflection-coefficient ground fill")
-            return
-        grid = self._ground_grid(k, omega)
-        eps_t = None if self.ground_eps is None else _ground.eps_tilde(self.ground_eps, omega,
-                                                                      self.eps)
-        c2 = _ground.image_coefficient(eps_t)
-        gref = ctypes.byref(grid) if grid is not None else None
-        _lib.check(lib.mw_fill_ground(D.mw, k, omega, self.eps, self.mu, self.ground_z, c2.real,
-                                      c2.imag, gref, self.n_qp_sommerfeld, zptr, ld,
-                                      pairs.data_ptr(),
-                                      int(pairs.shape[0]), int(mesh.symmetric), st),
-                   "ground fill")
+            calls.append(lambda: _lib.check(lib.mw_fill_ground_refl(
+                D.mw, k, omega, self.eps, self.mu, self.ground_z, eps_t.real, eps_t.imag, c0.real,
+                c0.imag, c1.real, c1.imag, zptr, ld, pairs.data_ptr(), int(pairs.shape[0]), sym,
+                st), "reflection-coefficient ground fill"))
+        elif self.ground_z is not None:
+            grid = self._ground_grid(k, omega)
+            eps_t = None if self.ground_eps is None else _ground.eps_tilde(self.ground_eps, omega,
+                                                                          self.eps)
+            c2 = _ground.image_coefficient(eps_t)
+            gref = ctypes.byref(grid) if grid is not None else None
+            calls.append(lambda: _lib.check(lib.mw_fill_ground(
+                D.mw, k, omega, self.eps, self.mu, self.ground_z, c2.real, c2.imag, gref,
+                self.n_qp_sommerfeld, zptr, ld, pairs.data_ptr(), int(pairs.shape[0]), sym, st),
+                "ground fill"))
+
+        def launch():
+            for c in calls:
+                c()
+        launch.grid = grid if self.ground_z is not None and self.ground_model == "sommerfeld" \
+            and self.ground_eps is not None else None  # kept alive with the launcher
+        return launch
 
     def _loading_device(self, D, omega):
         """Device (rows, cols, merged Z'_w S_w values) of the loading term at

@@ -89,6 +89,40 @@ def sample_inputs(n, seed):
     return x[perm]
 
 
+def sample_grid_nodes(n, seed):
+    """Nodes the grid fill actually queries: for random grounds over the
+    trained eps range and extents log-uniform in [0.35, 6] wavelengths, the
+    lattice nodes of each grid layout (_ground.grid_layout, the reference's
+    region table) that lie inside the trained (rho, h) box, drawn until n.
+    These weight the grazing (small theta) rows and the fine near-zone
+    lattices the way the fill weights them, which a uniform box does not."""
+    rng = np.random.default_rng(seed)
+    parts, have = [], 0
+    while have < n:
+        er = rng.uniform(*EPS_R)
+        s = np.exp(rng.uniform(np.log(SIGMA_RATIO[0]), np.log(SIGMA_RATIO[1])))
+        r1 = np.exp(rng.uniform(np.log(0.35), np.log(6.0)))
+        regions, scal = _ground.grid_layout(complex(er, -s), _ground.K2_REF, r1)
+        _shapes, R, T = _ground.grid_nodes(regions, scal[3])
+        rho, h = np.maximum(R * np.cos(T), 0.0), np.maximum(R * np.sin(T), 0.0)
+        ok = (R > 0.0) & (rho <= RHO_MAX) & (h <= H_MAX)
+        idx = np.nonzero(ok)[0]
+        take = rng.choice(idx, size=min(idx.size, 2048, n - have), replace=False)
+        parts.append(np.stack([rho[take], h[take], np.full(take.size, er),
+                               np.full(take.size, s)], axis=1))
+        have += take.size
+    x = np.concatenate(parts)
+    return x[rng.permutation(x.shape[0])]
+
+
+def sample_training_inputs(n, seed, grid_frac=0.5):
+    """The training set: grid_frac of the nodes from grid layouts
+    (sample_grid_nodes), the rest from sample_inputs' box and near-field mix."""
+    n_grid = int(round(grid_frac * n))
+    x = np.concatenate([sample_inputs(n - n_grid, seed), sample_grid_nodes(n_grid, seed + 1000)])
+    return x[np.random.default_rng(seed + 2000).permutation(n)]
+
+
 def engine_targets(x, chunk=1 << 16, log=None):
     """Exact surfaces of the nodes x (n, 4) from the HIP engine, as the
     network's 8 normalised outputs (device float32 tensor (n, 8))."""
@@ -320,6 +354,9 @@ def main(argv=None):
     ap.add_argument("--batch", type=int, default=4096)
     ap.add_argument("--lr", type=float, default=2e-3)
     ap.add_argument("--seed", type=int, default=7)
+    ap.add_argument("--grid-frac", type=float, default=0.0,
+                    help="share of training nodes drawn from grid layouts (the shipped "
+                         "weights: 0)")
     ap.add_argument("--out", default=WEIGHTS)
     a = ap.parse_args(argv)
 
@@ -328,9 +365,9 @@ def main(argv=None):
 
     torch = _torch()
     t0 = time.perf_counter()
-    x = sample_inputs(a.samples, a.seed)
+    x = sample_training_inputs(a.samples, a.seed, a.grid_frac)
     t = engine_targets(x, log=log)
-    xv = sample_inputs(a.val, a.seed + 1)
+    xv = sample_training_inputs(a.val, a.seed + 1, a.grid_frac)
     tv = engine_targets(xv)
     torch.cuda.synchronize()
     t_gen = time.perf_counter() - t0
@@ -346,6 +383,7 @@ def main(argv=None):
     err = np.abs(pv - tvh).max(axis=1) / np.abs(tvh).max(axis=1)
     rms = float(np.sqrt(np.mean(np.abs(pv - tvh) ** 2) / np.mean(np.abs(tvh) ** 2)))
     meta = dict(samples=a.samples, steps=a.steps, batch=a.batch, lr=a.lr, seed=a.seed,
+                grid_frac=a.grid_frac,
                 gen_s=round(t_gen, 2), train_s=round(t_train, 2), final_loss=hist[-1][1],
                 val_rms_rel=rms, val_median_rel=float(np.median(err)),
                 val_p99_rel=float(np.percentile(err, 99)))

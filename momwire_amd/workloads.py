@@ -139,6 +139,22 @@ def ground_parameters(seed):
     return height, eps_r, sigma, freq
 
 
+def ground_box_draws(n=16, seed=2026):
+    """n grounded-array draws spread over the whole C5 box: heights stratified
+    over [0.05, 1] wavelength (one per n-th of the range, jittered), eps_r
+    U(5, 30), sigma log-U(1e-3, 5e-2) S/m, frequency U(3, 30) MHz; each a
+    (height / lambda, eps_r, sigma, freq) tuple for grounded_array(params=...)."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(n):
+        h = 0.05 + 0.95 * (i + rng.uniform()) / n
+        er = float(rng.uniform(5.0, 30.0))
+        sig = float(np.exp(rng.uniform(np.log(1e-3), np.log(5e-2))))
+        f = float(rng.uniform(3e6, 30e6))
+        out.append((float(h), er, sig, f))
+    return out
+
+
 def grounded_array(nx=4, ny=4, n_seg=21, seed=4, length=0.48, radius=1e-3, pitch=0.5,
                    params=None, ground_model="sommerfeld", phi_mode=None):
     """Horizontal dipole array over finite ground (ground plane z = 0),
