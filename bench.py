@@ -210,6 +210,22 @@ def mixed_main(args, wl):
         res[name]["api_rate"] = n_extra / (time.perf_counter() - t0)
         print(f"[bench] {name}: graph {res[name]['graph_rate']:.2f}/s, compute_port_solution "
               f"{res[name]['api_rate']:.2f}/s", file=sys.stderr, flush=True)
+    # the mixed form with the FP32 far-interaction fill (pairs beyond 3 segment
+    # lengths), its fill time and its Y error per correction against FP64
+    far = None
+    if args.far_steps > 0:
+        sfar = BSplineSolver(**kw, precision="mixed", refine_steps=args.refine_steps,
+                             far_fill="fp32")
+        sfar._solve_device_mixed()
+        torch.cuda.synchronize()
+        timers = {}
+        t0 = time.perf_counter()
+        for _ in range(args.far_steps):
+            sfar._solve_device_mixed(timers=timers)
+        torch.cuda.synchronize()
+        far = {"port_solves_per_s": args.far_steps / (time.perf_counter() - t0),
+               "stage_ms": stage_means(timers)}
+        print(f"[bench] mixed + FP32 far fill: {far}", file=sys.stderr, flush=True)
     per_step, adjoint, final = [], [], None
     if not args.no_parity:
         print("[bench] Y error of the mixed form per correction", file=sys.stderr, flush=True)
@@ -225,6 +241,15 @@ def mixed_main(args, wl):
         s64._check_info()
         smx._check_info()
         final = adjoint[-1] if adjoint else per_step[-1]
+        if far is not None:
+            hf = []
+            sfar._solve_device_mixed(history=hf)
+            far["y_rel_err_vs_fp64_per_step"] = [
+                float(np.abs(_to_host(h)[0] - Y64).max() / ynorm) for h in hf
+                if not isinstance(h, tuple)]
+            far["y_rel_err_adjoint_corrected_per_step"] = [
+                float(np.abs(_to_host(h[1])[0] - Y64).max() / ynorm) for h in hf
+                if isinstance(h, tuple)]
     fill_s = res["fp64"]["stage_ms"]["fill"] * 1e-3
     solve_flops = lu_flops(nb) + 8.0 * nb * nb * npt
     sm = res["mixed"]["stage_ms"]
@@ -253,6 +278,7 @@ def mixed_main(args, wl):
         "fill_gentries_per_s": nb * nb / fill_s / 1e9,
         "fill_tflops": fill_flops(ns, nb) / fill_s / 1e12,
         "fp64_solve_tflops": tf64,
+        "mixed_with_fp32_far_fill": far,
     }
     # the mixed form's dominant stage: the FP32 factor + all-port solve
     out["roofline"] = {"kernel": "mw_cgesv: FP32 factor + all-port solve (k_gemm_sub<float> "
@@ -731,6 +757,9 @@ def main():
                     help="array32x32: leave out the leg after the timed region that copies Y "
                          "to the host and measures the mixed form's error (the line then "
                          "carries no error figures); for hardware-counter runs")
+    ap.add_argument("--far-steps", type=int, default=3,
+                    help="array32x32: mixed solves timed with the FP32 far-interaction fill "
+                         "(0: not measured)")
     ap.add_argument("--extra-steps", type=int, default=5,
                     help="array32x32: solves timed for each of the graph-replay and the "
                          "compute_port_solution() rates (0: not measured)")

@@ -189,6 +189,19 @@ int mw_fill_z(const mw_mesh* mesh, double k, double omega, double eps, double mu
               int32_t symmetric, void* stream);
 
 /*
+ * mw_fill_z with the far-interaction kernel in FP32: segment pairs whose
+ * midpoints are more than near_ratio x the longer segment apart take
+ * exp(-jkR)/(4 pi R) with its phase reduced in FP64 and sin, cos and the
+ * reciprocal in single precision; the near pairs, the same-edge table and the
+ * extended-kernel pairs stay FP64.  The mixed form's optional fill; no
+ * counterpart in the reference (which fills every pair in FP64,
+ * _accelerators.cpp:370-542).
+ */
+int mw_fill_z_far32(const mw_mesh* mesh, double k, double omega, double eps, double mu,
+                    void* Z, int64_t ldz, const int32_t* tile_pairs, int32_t n_pairs,
+                    int32_t symmetric, double near_ratio, void* stream);
+
+/*
  * LU factorisation with row pivoting (threshold pivoting per 32-column panel,
  * tournament partial pivoting where the threshold fails), in place:
  * A = P L U, L unit lower.  Replaces LAPACK zgetrf/cgetrf behind

@@ -103,6 +103,8 @@ SIGNATURES = {
     "mw_same_edge_static": (ctypes.c_int, [ctypes.POINTER(MwMesh), _P, _P]),
     "mw_fill_z": (ctypes.c_int, [ctypes.POINTER(MwMesh), _D, _D, _D, _D, _P, _I64, _P, _I32,
                                  _I32, _P]),
+    "mw_fill_z_far32": (ctypes.c_int, [ctypes.POINTER(MwMesh), _D, _D, _D, _D, _P, _I64, _P, _I32,
+                                       _I32, _D, _P]),
     "mw_getrf_workspace_bytes": (_I64, [_I64]),
     "mw_set_pivot_threshold": (ctypes.c_int, [_D]),
     "mw_get_pivot_threshold": (_D, []),

@@ -38,6 +38,11 @@ from ._port_solution import PortSolution, _SweptPortSolutions
 from ._lib import SolveAborted  # noqa: F401  (re-exported, reference _cancel.py)
 
 
+# far_fill="fp32": pairs within this many segment lengths stay FP64 (the
+# near-neighbour integrals of the mixed form)
+FAR_NEAR_RATIO = 3.0
+
+
 class CancelToken:
     """Cooperative cancel flag (reference src/momwire/_cancel.py:20-79).
 
@@ -242,7 +247,7 @@ class BSplineSolver(_SweptPortSolutions):
                  nsegs=101, ground_z=None, ground_eps=None, ground_phi_mode="normal",
                  ground_model="refl-coef", n_qp_sommerfeld=3, use_singular_enrichment=False,
                  swept_mem_mb=256, cancel=None, precision="fp64", refine_steps=None,
-                 somm_surrogate=None, **unsupported):
+                 somm_surrogate=None, far_fill="fp64", **unsupported):
         if unsupported:
             raise NotImplementedError(f"options not served on the GPU path: {sorted(unsupported)}")
         if degree < 1:
@@ -282,6 +287,13 @@ class BSplineSolver(_SweptPortSolutions):
             raise ValueError(f"precision must be 'fp64' or 'mixed', got {precision!r}")
         if refine_steps is not None and int(refine_steps) < 0:
             raise ValueError("refine_steps must be >= 0")
+        if far_fill not in ("fp64", "fp32"):
+            raise ValueError(f"far_fill must be 'fp64' or 'fp32', got {far_fill!r}")
+        if far_fill == "fp32" and precision != "mixed":
+            raise ValueError("far_fill='fp32' is an option of precision='mixed'")
+        # far_fill="fp32": segment pairs more than 3 segment lengths apart take
+        # the kernel's phase in FP32 (the rest of the fill stays FP64)
+        self.far_fill = far_fill
         # "mixed": FP64 fill, FP32 LU and solves on MFMA, iterative refinement
         # with the residual B - Z X in FP64 (refine_steps corrections; default
         # 1 plus the adjoint-corrected readout when Z is symmetric, else 3).
@@ -592,9 +604,14 @@ class BSplineSolver(_SweptPortSolutions):
         mesh = self._mesh
         pairs = D.keep["pairs"]
         sym = int(mesh.symmetric)
-        calls = [lambda: _lib.check(lib.mw_fill_z(D.mw, k, omega, self.eps, self.mu, zptr, ld,
-                                                  pairs.data_ptr(), int(pairs.shape[0]), sym,
-                                                  st), "fill")]
+        if self.far_fill == "fp32":
+            calls = [lambda: _lib.check(lib.mw_fill_z_far32(
+                D.mw, k, omega, self.eps, self.mu, zptr, ld, pairs.data_ptr(),
+                int(pairs.shape[0]), sym, FAR_NEAR_RATIO, st), "fill (FP32 far pairs)")]
+        else:
+            calls = [lambda: _lib.check(lib.mw_fill_z(D.mw, k, omega, self.eps, self.mu, zptr, ld,
+                                                      pairs.data_ptr(), int(pairs.shape[0]), sym,
+                                                      st), "fill")]
         if self._loading_active:
             rows, cols, vals = self._loading_device(D, omega)
             calls.append(lambda: _lib.check(lib.mw_zadd_sparse(

@@ -11,6 +11,9 @@
 
 hipError_t mw_launch_same_edge(const mw_mesh& M, double k, hipStream_t st);
 hipError_t mw_launch_same_edge_static(const mw_mesh& M, double* out, hipStream_t st);
+hipError_t mw_launch_fill_far32(const mw_mesh& M, double k, double omega, double eps, double mu,
+                                double2* Z, int64_t ldz, const int32_t* pairs, int32_t n_pairs,
+                                bool symmetric, double near_ratio, hipStream_t st);
 int64_t mw_same_edge_blocks(int32_t n_se);
 hipError_t mw_launch_fill(const mw_mesh& M, double k, double omega, double eps, double mu,
                           double2* Z, int64_t ldz, const int32_t* pairs, int32_t n_pairs,
@@ -227,6 +230,22 @@ int mw_fill_z(const mw_mesh* mesh, double k, double omega, double eps, double mu
   return hip_status(mw_launch_fill(*mesh, k, omega, eps, mu, (double2*)Z, ldz, tile_pairs,
                                    n_pairs, symmetric != 0, (hipStream_t)stream),
                     "mw_fill_z");
+}
+
+int mw_fill_z_far32(const mw_mesh* mesh, double k, double omega, double eps, double mu, void* Z,
+                    int64_t ldz, const int32_t* tile_pairs, int32_t n_pairs, int32_t symmetric,
+                    double near_ratio, void* stream) {
+  g_err[0] = 0;
+  int rc = check_mesh(mesh);
+  if (rc) return rc;
+  if (!Z) return fail(MW_EINVAL, "%sZ is NULL");
+  if (ldz < mesh->n_basis) return fail(MW_EINVAL, "%sldz %lld < n_basis %lld", "", ldz, mesh->n_basis);
+  if (symmetric && (!tile_pairs || n_pairs <= 0))
+    return fail(MW_EINVAL, "%ssymmetric fill needs the tile-pair list");
+  if (!(near_ratio >= 0.0)) return fail(MW_EINVAL, "%snear_ratio must be >= 0");
+  return hip_status(mw_launch_fill_far32(*mesh, k, omega, eps, mu, (double2*)Z, ldz, tile_pairs,
+                                         n_pairs, symmetric != 0, near_ratio, (hipStream_t)stream),
+                    "mw_fill_z_far32");
 }
 
 int64_t mw_getrf_workspace_bytes(int64_t n) { return n > 0 ? mw_getrf_ws_bytes(n) : 0; }

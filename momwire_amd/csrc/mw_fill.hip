@@ -259,6 +259,8 @@ struct SegStage {
   double tan[3];
   double a;            // observer-row radius
   double poly[NM][NM]; // band polynomials [slot][power]
+  double mid[3];       // segment midpoint
+  double len;          // segment length
   int edge, eidx;
   int ekl;             // coaxial label (extended kernel), -1 when off
 };
@@ -275,6 +277,8 @@ MW_DEV void stage_segment(const mw_mesh& M, int s, SegStage<NQ>& S) {
   }
   S.tan[0] = g[6]; S.tan[1] = g[7]; S.tan[2] = g[8];
   S.a = g[10];
+  for (int c = 0; c < 3; ++c) S.mid[c] = 0.5 * (g[c] + g[3 + c]);
+  S.len = L;
   const double* bp = M.band_poly + (size_t)s * 9;
   for (int i = 0; i < 9; ++i) S.poly[i / 3][i % 3] = bp[i];
   S.edge = M.seg_edge[2 * s];
@@ -286,9 +290,29 @@ MW_DEV void stage_segment(const mw_mesh& M, int s, SegStage<NQ>& S) {
 // template parameter: the static LDS arrays below exist once per
 // instantiation, so a kernel holding both forms would carry twice the LDS
 // and lose half its occupancy.
-template <int NQ>
+// The phase of a far pair in FP32: kR (up to ~10^2 rad) is reduced to
+// [-pi, pi] in FP64 (Cody-Waite with 2 pi split in two), then sin and cos are
+// taken in single precision and 1/(4 pi R) with an FP32 reciprocal.
+MW_DEV void far_phase32(double k, double R, double& gr, double& gi) {
+  constexpr double INV_2PI = 0.15915494309189533577;
+  constexpr double TWO_PI_HI = 6.28318530717958623200;  // 2 pi rounded to double
+  constexpr double TWO_PI_LO = 2.44929359829470635445e-16;
+  const double kr = k * R;
+  const double nn = rint(kr * INV_2PI);
+  const double t = fma(-nn, TWO_PI_LO, fma(-nn, TWO_PI_HI, kr));
+  float s32, c32;
+  sincosf((float)t, &s32, &c32);
+  const float inv = (float)INV4PI / (float)R;
+  gr = (double)(c32 * inv);
+  gi = -(double)(s32 * inv);
+}
+
+// FAR32: segment pairs whose midpoints lie more than sqrt(near2) x the longer
+// segment apart take the kernel's phase in FP32 (far_phase32); the others,
+// and every same-edge, extended-kernel or image term, stay FP64.
+template <int NQ, bool FAR32 = false>
 MW_DEV void fill_body(bool sym, int ti, int tj, const mw_mesh& M, double k, double wmu,
-                      double inv_weps, double2* __restrict__ Z, int64_t ldz) {
+                      double inv_weps, double2* __restrict__ Z, int64_t ldz, double near2 = 0.0) {
   __shared__ SegStage<NQ> rs[MW_TSMAX], cs[MW_TSMAX];
   __shared__ double2 Bt[MW_TSMAX * MW_TSMAX * 9];
 
@@ -332,6 +356,12 @@ MW_DEV void fill_body(bool sym, int ti, int tj, const mw_mesh& M, double k, doub
     } else if (live) {
       const double a2 = I.a * I.a;
       const bool ext = I.ekl >= 0 && I.ekl == J.ekl;  // extended kernel on this pair
+      bool far = false;
+      if (FAR32 && !ext) {
+        const double mx = I.mid[0] - J.mid[0], my = I.mid[1] - J.mid[1], mz = I.mid[2] - J.mid[2];
+        const double lm = fmax(I.len, J.len);
+        far = mx * mx + my * my + mz * mz > near2 * lm * lm;
+      }
 #pragma unroll
       for (int qq = 0; qq < QH; ++qq) {
         const int q = QH * half + qq;
@@ -342,10 +372,16 @@ MW_DEV void fill_body(bool sym, int ti, int tj, const mw_mesh& M, double k, doub
         for (int r = 0; r < NQ; ++r) {
           const double dx = x - J.pos[r][0], dy = y - J.pos[r][1], dz = z - J.pos[r][2];
           const double R = sqrt(dx * dx + dy * dy + dz * dz + a2);
-          double sn, cn;
-          sincos(k * R, &sn, &cn);
-          const double inv = INV4PI / R;
-          double gr = cn * inv, gi = -sn * inv;  // exp(-jkR)/(4 pi R)
+          double gr, gi;  // exp(-jkR)/(4 pi R)
+          if (FAR32 && far) {
+            far_phase32(k, R, gr, gi);
+          } else {
+            double sn, cn;
+            sincos(k * R, &sn, &cn);
+            const double inv = INV4PI / R;
+            gr = cn * inv;
+            gi = -sn * inv;
+          }
           if (ext) {
             const double2 f = ek_factor(R, I.a, k);
             const double g0 = gr;
@@ -437,15 +473,16 @@ MW_DEV void fill_body(bool sym, int ti, int tj, const mw_mesh& M, double k, doub
   }
 }
 
-template <bool SYM, int NQ>
+template <bool SYM, int NQ, bool FAR32 = false>
 __global__ __launch_bounds__(256) void k_fill(mw_mesh M, double k, double wmu,
                                               double inv_weps, double2* __restrict__ Z,
-                                              int64_t ldz, const int32_t* __restrict__ pairs) {
+                                              int64_t ldz, const int32_t* __restrict__ pairs,
+                                              double near2 = 0.0) {
   if (SYM)
-    fill_body<NQ>(true, pairs[2 * blockIdx.x], pairs[2 * blockIdx.x + 1], M, k, wmu, inv_weps, Z,
-                  ldz);
+    fill_body<NQ, FAR32>(true, pairs[2 * blockIdx.x], pairs[2 * blockIdx.x + 1], M, k, wmu,
+                         inv_weps, Z, ldz, near2);
   else
-    fill_body<NQ>(false, blockIdx.y, blockIdx.x, M, k, wmu, inv_weps, Z, ldz);
+    fill_body<NQ, FAR32>(false, blockIdx.y, blockIdx.x, M, k, wmu, inv_weps, Z, ldz, near2);
 }
 
 // ---------------------------------------------------------------------------
@@ -475,17 +512,17 @@ __global__ __launch_bounds__(256) void kb_fill(const mw_sweep_prob* P, const mw_
                reinterpret_cast<double2*>(p.A), p.lda);
 }
 
-template <int NQ>
+template <int NQ, bool FAR32 = false>
 hipError_t launch_fill_nq(const mw_mesh& M, double k, double wmu, double inv_weps, double2* Z,
                           int64_t ldz, const int32_t* pairs, int32_t n_pairs, bool symmetric,
-                          hipStream_t st) {
+                          hipStream_t st, double near2 = 0.0) {
   if (symmetric) {
     if (n_pairs == 0) return hipSuccess;
-    hipLaunchKernelGGL((k_fill<true, NQ>), dim3(n_pairs), dim3(256), 0, st, M, k, wmu, inv_weps,
-                       Z, ldz, pairs);
+    hipLaunchKernelGGL((k_fill<true, NQ, FAR32>), dim3(n_pairs), dim3(256), 0, st, M, k, wmu,
+                       inv_weps, Z, ldz, pairs, near2);
   } else {
-    hipLaunchKernelGGL((k_fill<false, NQ>), dim3(M.n_tile, M.n_tile), dim3(256), 0, st, M, k, wmu,
-                       inv_weps, Z, ldz, pairs);
+    hipLaunchKernelGGL((k_fill<false, NQ, FAR32>), dim3(M.n_tile, M.n_tile), dim3(256), 0, st, M,
+                       k, wmu, inv_weps, Z, ldz, pairs, near2);
   }
   return hipGetLastError();
 }
@@ -542,6 +579,18 @@ hipError_t mw_launch_fill(const mw_mesh& M, double k, double omega, double eps, 
 // 256-thread blocks of a batched same-edge table (n_qp_pair = 4)
 int64_t mw_same_edge_blocks(int32_t n_se) {
   return ((int64_t)n_se * SeLanes<4>::LG + 255) / 256;
+}
+
+// The mixed form's fill: far pairs with an FP32 phase (near_ratio: the
+// distance, in segment lengths, inside which pairs stay FP64).
+hipError_t mw_launch_fill_far32(const mw_mesh& M, double k, double omega, double eps, double mu,
+                                double2* Z, int64_t ldz, const int32_t* pairs, int32_t n_pairs,
+                                bool symmetric, double near_ratio, hipStream_t st) {
+  const double wmu = omega * mu, inv_weps = 1.0 / (omega * eps);
+  const double near2 = near_ratio * near_ratio;
+#define MW_FILL32(NQ) return launch_fill_nq<NQ, true>(M, k, wmu, inv_weps, Z, ldz, pairs, n_pairs, symmetric, st, near2);
+  MW_NQ_SWITCH(M.n_qp, MW_FILL32)
+#undef MW_FILL32
 }
 
 hipError_t mw_launch_batch_fill(const mw_sweep_prob* P, const mw_mesh* meshes, int nprob,
