@@ -331,6 +331,16 @@ int mw_sweep_plan_create(int32_t n_mesh, const mw_mesh* meshes, const mw_ports* 
                          double pivot_bound, mw_sweep_plan** plan);
 int mw_sweep_run(mw_sweep_plan* plan, double eps, double mu,
                  const volatile int32_t* cancel, void* stream);
+/*
+ * The fill half of mw_sweep_run alone: every problem's Z into the first n
+ * columns of its A (what mw_fill_z writes for that mesh and wavenumber,
+ * same-edge tables included), for callers that factor the matrices
+ * elsewhere.  Problems of one mesh whose wavenumbers step evenly share their
+ * geometry: one workgroup per tile pair walks up to 16 of them.  No
+ * counterpart in the reference, which fills one structure at one wavenumber
+ * at a time (bspline.py:2222-2550).
+ */
+int mw_sweep_fill(mw_sweep_plan* plan, double eps, double mu, void* stream);
 int mw_sweep_plan_destroy(mw_sweep_plan* plan);
 
 /*

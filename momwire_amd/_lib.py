@@ -124,6 +124,7 @@ SIGNATURES = {
     "mw_sweep_plan_create": (ctypes.c_int, [_I32, _P, _P, _I32, _P, _I32, _D,
                                             ctypes.POINTER(ctypes.c_void_p)]),
     "mw_sweep_run": (ctypes.c_int, [_P, _D, _D, _P, _P]),
+    "mw_sweep_fill": (ctypes.c_int, [_P, _D, _D, _P]),
     "mw_sweep_plan_destroy": (ctypes.c_int, [_P]),
     "mw_somm_six_integrals": (ctypes.c_int, [_D, _D, _D, _P, _P, _I64, _D, _I32, _P, _P, _P]),
     "mw_somm_surfaces": (ctypes.c_int, [_D, _D, _D, _D, _D, _P, _P, _I64, _D, _P, _P, _P]),

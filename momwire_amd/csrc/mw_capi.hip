@@ -47,8 +47,10 @@ void mw_set_pivot_bound(double lbound);
 double mw_get_pivot_bound();
 hipError_t mw_launch_batch_fill(const mw_sweep_prob* P, const mw_mesh* meshes, int nprob,
                                 const int32_t* se_items, int64_t n_se_items,
-                                const int32_t* item_prob, const int32_t* item_pair,
-                                int64_t n_items, double eps, double mu, hipStream_t st);
+                                const mw_kgroup* groups, const int32_t* group_prob,
+                                const int32_t* item_group, const int32_t* item_pair,
+                                int64_t n_items, int64_t n_plain, double eps, double mu,
+                                hipStream_t st);
 int mw_sweep_lu_run(const mw_sweep_prob* dP, const mw_ports* dports, const mw_sweep_prob* hp,
                     int nprob, int max_ports, int max_con, int64_t max_scatter,
                     int gemm_variant, double lbound, const uint32_t* items,
@@ -545,12 +547,15 @@ int mw_c2z(int64_t rows, int64_t cols, const void* src, int64_t lds, void* dst, 
 
 struct mw_sweep_plan {
   int nprob = 0, n_mesh = 0, gemm_variant = 0, max_ports = 0, max_con = 0, max_nse = 0;
-  int64_t max_scatter = 0, n_items = 0, n_se_items = 0;
+  int64_t max_scatter = 0, n_items = 0, n_se_items = 0, n_groups = 0;
+  int64_t n_plain = 0;             // fill items before those of extended-kernel meshes
   double pivot_bound = 32.0;
   mw_sweep_prob* dP = nullptr;
   mw_ports* dports = nullptr;
   mw_mesh* dmeshes = nullptr;
-  int32_t* d_item_prob = nullptr;
+  mw_kgroup* d_groups = nullptr;   // frequency groups of the fill
+  int32_t* d_group_prob = nullptr; // problems of each group, in wavenumber order
+  int32_t* d_item_group = nullptr; // (group, tile pair) work items of the fill
   int32_t* d_item_pair = nullptr;
   int32_t* d_se_items = nullptr;   // (problem, 256-thread block) of the same-edge tables
   uint32_t* d_work = nullptr;      // per-step work lists of the factorisation
@@ -563,11 +568,70 @@ static void plan_free(mw_sweep_plan* p) {
   (void)hipFree(p->dP);
   (void)hipFree(p->dports);
   (void)hipFree(p->dmeshes);
-  (void)hipFree(p->d_item_prob);
+  (void)hipFree(p->d_groups);
+  (void)hipFree(p->d_group_prob);
+  (void)hipFree(p->d_item_group);
   (void)hipFree(p->d_item_pair);
   (void)hipFree(p->d_se_items);
   (void)hipFree(p->d_work);
   delete p;
+}
+
+// Frequency groups of the batched fill: the problems of one mesh (with one
+// tile list and symmetry), in ascending wavenumber, cut into runs of at most
+// kMaxRun whose wavenumbers step evenly to within 4 ulp.  The fill advances
+// exp(-j k R) from one wavenumber of a run to the next by a fixed rotation,
+// so a run's length also bounds the rounding that the rotations accumulate.
+static constexpr int kMaxRun = 16;
+
+static void sweep_kgroups(const std::vector<mw_sweep_prob>& hp, std::vector<mw_kgroup>& groups,
+                          std::vector<int32_t>& gprob) {
+  std::vector<int32_t> order(hp.size());
+  for (size_t i = 0; i < hp.size(); ++i) order[i] = (int32_t)i;
+  auto key_less = [&](int32_t a, int32_t b) {
+    const mw_sweep_prob &x = hp[a], &y = hp[b];
+    if (x.mesh != y.mesh) return x.mesh < y.mesh;
+    if (x.tile_pairs != y.tile_pairs) return x.tile_pairs < y.tile_pairs;
+    if (x.symmetric != y.symmetric) return x.symmetric < y.symmetric;
+    if (x.n_pairs != y.n_pairs) return x.n_pairs < y.n_pairs;
+    return x.k < y.k;
+  };
+  std::stable_sort(order.begin(), order.end(), key_less);
+  auto same_mesh = [&](int32_t a, int32_t b) {
+    const mw_sweep_prob &x = hp[a], &y = hp[b];
+    return x.mesh == y.mesh && x.tile_pairs == y.tile_pairs && x.symmetric == y.symmetric &&
+           x.n_pairs == y.n_pairs;
+  };
+  const double tol = 4.0 * 2.220446049250313e-16;
+  size_t i = 0;
+  while (i < order.size()) {
+    mw_kgroup g;
+    g.mesh = hp[order[i]].mesh;
+    g.first = (int32_t)gprob.size();
+    g.pad = 0;
+    g.k0 = hp[order[i]].k;
+    g.dk = 0.0;
+    size_t j = i + 1;
+    if (j < order.size() && same_mesh(order[i], order[j])) {
+      g.dk = hp[order[j]].k - g.k0;
+      ++j;
+      while (j < order.size() && (int)(j - i) < kMaxRun && same_mesh(order[i], order[j])) {
+        const double kj = hp[order[j]].k;
+        if (std::fabs(kj - (g.k0 + (double)(j - i) * g.dk)) > tol * std::fabs(kj)) break;
+        ++j;
+      }
+    }
+    g.nk = (int32_t)(j - i);
+    for (size_t t = i; t < j; ++t) gprob.push_back(order[t]);
+    groups.push_back(g);
+    i = j;
+  }
+}
+
+static hipError_t sweep_fill(const mw_sweep_plan* p, double eps, double mu, hipStream_t st) {
+  return mw_launch_batch_fill(p->dP, p->dmeshes, p->nprob, p->d_se_items, p->n_se_items,
+                              p->d_groups, p->d_group_prob, p->d_item_group, p->d_item_pair,
+                              p->n_items, p->n_plain, eps, mu, st);
 }
 
 int mw_sweep_plan_create(int32_t n_mesh, const mw_mesh* meshes, const mw_ports* ports,
@@ -607,20 +671,27 @@ int mw_sweep_plan_create(int32_t n_mesh, const mw_mesh* meshes, const mw_ports* 
     p->max_con = std::max(p->max_con, pt.n_con);
     p->max_scatter = std::max<int64_t>(p->max_scatter, (int64_t)q.ncols * pt.col_max);
     p->max_nse = std::max(p->max_nse, meshes[q.mesh].n_se);
-    p->n_items += q.n_pairs;
   }
   std::stable_sort(p->hp.begin(), p->hp.end(),
                    [](const mw_sweep_prob& a, const mw_sweep_prob& b) { return a.n > b.n; });
   std::vector<uint32_t> work;
   mw_sweep_work(p->hp.data(), nprob, gemm_variant, work, p->work_off);
-  std::vector<int32_t> ip, iq;
-  ip.reserve(p->n_items);
-  iq.reserve(p->n_items);
-  for (int i = 0; i < nprob; ++i)
-    for (int j = 0; j < p->hp[i].n_pairs; ++j) {
-      ip.push_back(i);
-      iq.push_back(j);
+  std::vector<mw_kgroup> groups;
+  std::vector<int32_t> gprob, ig, iq;
+  sweep_kgroups(p->hp, groups, gprob);
+  for (int ek = 0; ek < 2; ++ek) {
+    if (ek == 1) p->n_plain = (int64_t)ig.size();
+    for (size_t g = 0; g < groups.size(); ++g) {
+      if ((meshes[groups[g].mesh].ek != 0) != (ek == 1)) continue;
+      const int32_t np = p->hp[gprob[groups[g].first]].n_pairs;
+      for (int32_t j = 0; j < np; ++j) {
+        ig.push_back((int32_t)g);
+        iq.push_back(j);
+      }
     }
+  }
+  p->n_groups = (int64_t)groups.size();
+  p->n_items = (int64_t)ig.size();
   std::vector<int32_t> se_items;
   for (int i = 0; i < nprob; ++i) {
     const int64_t nb = mw_same_edge_blocks(meshes[p->hp[i].mesh].n_se);
@@ -634,13 +705,19 @@ int mw_sweep_plan_create(int32_t n_mesh, const mw_mesh* meshes, const mw_ports* 
   if ((e = hipMalloc(&p->dP, sizeof(mw_sweep_prob) * nprob)) != hipSuccess ||
       (e = hipMalloc(&p->dports, sizeof(mw_ports) * n_mesh)) != hipSuccess ||
       (e = hipMalloc(&p->dmeshes, sizeof(mw_mesh) * n_mesh)) != hipSuccess ||
-      (e = hipMalloc(&p->d_item_prob, sizeof(int32_t) * std::max<int64_t>(p->n_items, 1))) != hipSuccess ||
+      (e = hipMalloc(&p->d_groups, sizeof(mw_kgroup) * std::max<size_t>(groups.size(), 1))) != hipSuccess ||
+      (e = hipMalloc(&p->d_group_prob, sizeof(int32_t) * std::max<size_t>(gprob.size(), 1))) != hipSuccess ||
+      (e = hipMalloc(&p->d_item_group, sizeof(int32_t) * std::max<int64_t>(p->n_items, 1))) != hipSuccess ||
       (e = hipMalloc(&p->d_item_pair, sizeof(int32_t) * std::max<int64_t>(p->n_items, 1))) != hipSuccess ||
       (e = hipMemcpy(p->dP, p->hp.data(), sizeof(mw_sweep_prob) * nprob, hipMemcpyHostToDevice)) != hipSuccess ||
       (e = hipMemcpy(p->dports, ports, sizeof(mw_ports) * n_mesh, hipMemcpyHostToDevice)) != hipSuccess ||
       (e = hipMemcpy(p->dmeshes, meshes, sizeof(mw_mesh) * n_mesh, hipMemcpyHostToDevice)) != hipSuccess ||
-      (e = hipMemcpy(p->d_item_prob, ip.data(), sizeof(int32_t) * ip.size(), hipMemcpyHostToDevice)) != hipSuccess ||
-      (e = hipMemcpy(p->d_item_pair, iq.data(), sizeof(int32_t) * iq.size(), hipMemcpyHostToDevice)) != hipSuccess ||
+      (e = hipMemcpy(p->d_groups, groups.data(), sizeof(mw_kgroup) * groups.size(), hipMemcpyHostToDevice)) != hipSuccess ||
+      (e = hipMemcpy(p->d_group_prob, gprob.data(), sizeof(int32_t) * gprob.size(), hipMemcpyHostToDevice)) != hipSuccess ||
+      (!ig.empty() &&
+       (e = hipMemcpy(p->d_item_group, ig.data(), sizeof(int32_t) * ig.size(), hipMemcpyHostToDevice)) != hipSuccess) ||
+      (!iq.empty() &&
+       (e = hipMemcpy(p->d_item_pair, iq.data(), sizeof(int32_t) * iq.size(), hipMemcpyHostToDevice)) != hipSuccess) ||
       (e = hipMalloc(&p->d_se_items, sizeof(int32_t) * std::max<size_t>(se_items.size(), 2))) != hipSuccess ||
       (!se_items.empty() &&
        (e = hipMemcpy(p->d_se_items, se_items.data(), sizeof(int32_t) * se_items.size(), hipMemcpyHostToDevice)) != hipSuccess) ||
@@ -659,9 +736,7 @@ int mw_sweep_run(mw_sweep_plan* p, double eps, double mu, const volatile int32_t
   g_err[0] = 0;
   if (!p) return fail(MW_EINVAL, "%splan is NULL");
   hipStream_t st = (hipStream_t)stream;
-  hipError_t herr = mw_launch_batch_fill(p->dP, p->dmeshes, p->nprob, p->d_se_items,
-                                         p->n_se_items, p->d_item_prob, p->d_item_pair,
-                                         p->n_items, eps, mu, st);
+  hipError_t herr = sweep_fill(p, eps, mu, st);
   if (herr != hipSuccess) return hip_status(herr, "sweep fill");
   int rc = mw_sweep_lu_run(p->dP, p->dports, p->hp.data(), p->nprob, p->max_ports, p->max_con,
                            p->max_scatter, p->gemm_variant, p->pivot_bound, p->d_work,
@@ -669,6 +744,12 @@ int mw_sweep_run(mw_sweep_plan* p, double eps, double mu, const volatile int32_t
   if (rc == MW_EHIP) return hip_status(herr, "sweep solve");
   if (rc == MW_EABORT) return fail(MW_EABORT, "%ssweep cancelled");
   return rc;
+}
+
+int mw_sweep_fill(mw_sweep_plan* p, double eps, double mu, void* stream) {
+  g_err[0] = 0;
+  if (!p) return fail(MW_EINVAL, "%splan is NULL");
+  return hip_status(sweep_fill(p, eps, mu, (hipStream_t)stream), "sweep fill");
 }
 
 int mw_sweep_plan_destroy(mw_sweep_plan* p) {

@@ -43,6 +43,25 @@ template <typename C> MW_DEV C crecip(C b) { C one; one.x = 1; one.y = 0; return
 // |re| + |im|: the pivot magnitude LAPACK's izamax uses.
 template <typename C> MW_DEV double cabs1(C a) { return fabs((double)a.x) + fabs((double)a.y); }
 
+// The value the neighbouring lane (lane ^ 1) holds: two DPP quad-permute
+// moves (quad_perm [1,0,3,2]) on the halves, plain VALU work where a
+// generic shuffle goes through the LDS crossbar.
+MW_DEV double lane_swap1(double x) {
+  const unsigned long long b = (unsigned long long)__double_as_longlong(x);
+  const unsigned lo = (unsigned)__builtin_amdgcn_mov_dpp((int)(unsigned)b, 0xB1, 0xF, 0xF, false);
+  const unsigned hi =
+      (unsigned)__builtin_amdgcn_mov_dpp((int)(unsigned)(b >> 32), 0xB1, 0xF, 0xF, false);
+  return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
+// A frequency group of a batched sweep: problems group_prob[first ..
+// first + nk) share one mesh, tile list and symmetry, and their wavenumbers
+// are k0 + f dk in that order (to a few ulp).
+struct mw_kgroup {
+  int32_t mesh, nk, first, pad;
+  double k0, dk;
+};
+
 // Extended thin-wire kernel factor of a coaxial equal-radius pair (NEC's
 // Eq 89 with the observer on its own wire's surface: rho = b = a):
 //   1 + T1 C2 - T2 C1,  T1 = a^4/(4R^4),  T2 = a^2/(2R^2),  C1 = 1 + jkR,
@@ -53,6 +72,12 @@ MW_DEV double2 ek_factor(double R, double a, double k) {
   const double r2 = R * R, kr = k * R, a2 = a * a;
   const double t1 = 0.25 * a2 * a2 / (r2 * r2), t2 = 0.5 * a2 / r2;
   return make_double2(1.0 + t1 * (3.0 - kr * kr) - t2, kr * (3.0 * t1 - t2));
+}
+// ek_factor - 1, without forming the 1
+MW_DEV double2 ek_factor_minus1(double R, double a, double k) {
+  const double r2 = R * R, kr = k * R, a2 = a * a;
+  const double t1 = 0.25 * a2 * a2 / (r2 * r2), t2 = 0.5 * a2 / r2;
+  return make_double2(t1 * (3.0 - kr * kr) - t2, kr * (3.0 * t1 - t2));
 }
 
 // Status codes (MW_OK ...) come from the public header.

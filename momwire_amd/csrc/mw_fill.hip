@@ -286,6 +286,50 @@ MW_DEV void stage_segment(const mw_mesh& M, int s, SegStage<NQ>& S) {
   S.ekl = M.ek ? M.ek_label[s] : -1;
 }
 
+// Contract a segment pair's moments J[p][P] (both halves' sums) to its band
+// block B[al][be] = j w mu td (C_i J C_j^T)[al][be] + (C_i' J C_j'^T)[al][be]
+// / (j w eps), the derivative polynomials being C'[p'][s] = (p'+1) C[s][p'+1].
+// The pair's two lanes share the 9 entries evenly: column be = half (3 rows
+// each), then rows 2 half .. 2 half + 1 of column 2.
+template <int NQ>
+MW_DEV void contract_band(const double (&jr)[NM][NM], const double (&jim)[NM][NM],
+                          const SegStage<NQ>& I, const SegStage<NQ>& J, double td, double wmu,
+                          double inv_weps, int half, double2* __restrict__ out) {
+#pragma unroll
+  for (int step = 0; step < 2; ++step) {
+    const int be = step == 0 ? half : 2;
+    const double cj0 = J.poly[be][0], cj1 = J.poly[be][1], cj2 = J.poly[be][2];
+    double tr[NM], tm[NM];
+#pragma unroll
+    for (int a = 0; a < NM; ++a) {
+      tr[a] = jr[a][0] * cj0 + jr[a][1] * cj1 + jr[a][2] * cj2;
+      tm[a] = jim[a][0] * cj0 + jim[a][1] * cj1 + jim[a][2] * cj2;
+    }
+    const double dj0 = cj1, dj1 = 2.0 * cj2;  // derivative coefficients
+    double fr[2], fi[2];
+#pragma unroll
+    for (int a = 0; a < 2; ++a) {
+      fr[a] = jr[a][0] * dj0 + jr[a][1] * dj1;
+      fi[a] = jim[a][0] * dj0 + jim[a][1] * dj1;
+    }
+#pragma unroll
+    for (int t = 0; t < (step == 0 ? NM : 2); ++t) {
+      const int al = step == 0 ? t : 2 * half + t;
+      if (al >= NM) break;
+      const double ci0 = I.poly[al][0], ci1 = I.poly[al][1], ci2 = I.poly[al][2];
+      const double Ar = ci0 * tr[0] + ci1 * tr[1] + ci2 * tr[2];
+      const double Ai = ci0 * tm[0] + ci1 * tm[1] + ci2 * tm[2];
+      const double di0 = ci1, di1 = 2.0 * ci2;
+      const double Pr = di0 * fr[0] + di1 * fr[1];
+      const double Pi = di0 * fi[0] + di1 * fi[1];
+      // j w mu td A + Phi / (j w eps)
+      const double Br = -wmu * td * Ai + Pi * inv_weps;
+      const double Bi = wmu * td * Ar - Pr * inv_weps;
+      out[al * 3 + be] = make_double2(Br, Bi);
+    }
+  }
+}
+
 // `sym` is a run-time flag (a uniform branch at the store) rather than a
 // template parameter: the static LDS arrays below exist once per
 // instantiation, so a kernel holding both forms would carry twice the LDS
@@ -405,43 +449,12 @@ MW_DEV void fill_body(bool sym, int ti, int tj, const mw_mesh& M, double k, doub
     for (int p = 0; p < NM; ++p)
 #pragma unroll
       for (int P = 0; P < NM; ++P) {
-        jr[p][P] += __shfl_xor(jr[p][P], 1);
-        jim[p][P] += __shfl_xor(jim[p][P], 1);
+        jr[p][P] += lane_swap1(jr[p][P]);
+        jim[p][P] += lane_swap1(jim[p][P]);
       }
     if (!live) continue;
-    // Contract to the band block B[al][be].  A-term T = J C_j, then C_i^T T;
-    // Phi-term uses derivative polynomials C'[p'][s] = (p'+1) C[s][p'+1].
     const double td = I.tan[0] * J.tan[0] + I.tan[1] * J.tan[1] + I.tan[2] * J.tan[2];
-    const int be0 = half == 0 ? 0 : 2, be1 = half == 0 ? 2 : 3;
-    for (int be = be0; be < be1; ++be) {
-      const double cj0 = J.poly[be][0], cj1 = J.poly[be][1], cj2 = J.poly[be][2];
-      double tr[NM], tim[NM];
-#pragma unroll
-      for (int p = 0; p < NM; ++p) {
-        tr[p] = jr[p][0] * cj0 + jr[p][1] * cj1 + jr[p][2] * cj2;
-        tim[p] = jim[p][0] * cj0 + jim[p][1] * cj1 + jim[p][2] * cj2;
-      }
-      const double dj0 = cj1, dj1 = 2.0 * cj2;  // derivative coefficients
-      double fr[2], fi[2];
-#pragma unroll
-      for (int p = 0; p < 2; ++p) {
-        fr[p] = jr[p][0] * dj0 + jr[p][1] * dj1;
-        fi[p] = jim[p][0] * dj0 + jim[p][1] * dj1;
-      }
-#pragma unroll
-      for (int al = 0; al < NM; ++al) {
-        const double ci0 = I.poly[al][0], ci1 = I.poly[al][1], ci2 = I.poly[al][2];
-        const double Ar = ci0 * tr[0] + ci1 * tr[1] + ci2 * tr[2];
-        const double Ai = ci0 * tim[0] + ci1 * tim[1] + ci2 * tim[2];
-        const double di0 = ci1, di1 = 2.0 * ci2;
-        const double Pr = di0 * fr[0] + di1 * fr[1];
-        const double Pi = di0 * fi[0] + di1 * fi[1];
-        // j w mu td A + Phi / (j w eps)
-        const double Br = -wmu * td * Ai + Pi * inv_weps;
-        const double Bi = wmu * td * Ar - Pr * inv_weps;
-        Bt[(li * nc + lj) * 9 + al * 3 + be] = make_double2(Br, Bi);
-      }
-    }
+    contract_band<NQ>(jr, jim, I, J, td, wmu, inv_weps, half, Bt + (li * nc + lj) * 9);
   }
   __syncthreads();
 
@@ -497,19 +510,208 @@ __global__ __launch_bounds__(256) void kb_same_edge(const mw_sweep_prob* P, cons
   same_edge_body<4>((int64_t)items[2 * blockIdx.x + 1] * blockDim.x + threadIdx.x, M, p.k);
 }
 
-// one workgroup per (problem, tile pair) work item
-__global__ __launch_bounds__(256) void kb_fill(const mw_sweep_prob* P, const mw_mesh* meshes,
-                                               const int32_t* __restrict__ item_prob,
-                                               const int32_t* __restrict__ item_pair,
-                                               double eps, double mu) {
-  const mw_sweep_prob& p = P[item_prob[blockIdx.x]];
-  mw_mesh M = meshes[p.mesh];
-  M.se_tab = p.se_tab;
-  const int32_t* pairs = p.tile_pairs;
-  const int q = item_pair[blockIdx.x];
-  const double wmu = p.omega * mu, inv_weps = 1.0 / (p.omega * eps);
-  fill_body<4>(p.symmetric != 0, pairs[2 * q], pairs[2 * q + 1], M, p.k, wmu, inv_weps,
-               reinterpret_cast<double2*>(p.A), p.lda);
+// Batched fill swept over wavenumber.  A frequency group is a run of
+// problems on one mesh whose wavenumbers step evenly, k_f = k0 + f dk.  One
+// workgroup per (group, tile pair) stages the two tiles' segments and each
+// segment pair's point-to-point distances once, then walks the group's
+// wavenumbers: exp(-j k_f R)/(4 pi R) advances by one complex product with
+// exp(-j dk R) per step, where the single-problem fill takes a sincos, a
+// square root and a division per point pair and wavenumber.  Pairs under the
+// extended kernel multiply in its factor, a polynomial in k R; same-edge
+// pairs read each problem's own table.  Two lanes per segment pair (half 0: observer
+// points 0-1, half 1: points 2-3), up to MW_TSMAX^2 pairs in one pass; the
+// band blocks of consecutive wavenumbers alternate between two LDS buffers,
+// so one barrier per wavenumber separates the gather of one from the
+// contraction of the next.
+constexpr int KF_THREADS = 2 * ((MW_TSMAX * MW_TSMAX + 127) / 128) * 128;
+static_assert(KF_THREADS == 768, "two lanes per segment pair of an 18 x 18 tile pair");
+
+// EXT: the groups' meshes carry the extended kernel (the plan launches the
+// two forms over separate item lists).
+template <bool EXT>
+__global__ __launch_bounds__(KF_THREADS) void kb_fill_k(const mw_sweep_prob* __restrict__ P,
+                                                        const mw_mesh* __restrict__ meshes,
+                                                        const mw_kgroup* __restrict__ groups,
+                                                        const int32_t* __restrict__ group_prob,
+                                                        const int32_t* __restrict__ item_group,
+                                                        const int32_t* __restrict__ item_pair,
+                                                        double eps, double mu) {
+  constexpr int NQ = 4, QH = 2;
+  __shared__ SegStage<NQ> rs[MW_TSMAX], cs[MW_TSMAX];
+  __shared__ double2 Bt[2][MW_TSMAX * MW_TSMAX * 9];
+
+  const mw_kgroup G = groups[item_group[blockIdx.x]];
+  const mw_mesh& M = meshes[G.mesh];
+  const mw_sweep_prob& p0 = P[group_prob[G.first]];
+  const bool sym = p0.symmetric != 0;
+  const int qp = item_pair[blockIdx.x];
+  const int ti = p0.tile_pairs[2 * qp], tj = p0.tile_pairs[2 * qp + 1];
+  const int tid = threadIdx.x;
+  const int nr = M.tile_nseg[ti], nc = M.tile_nseg[tj];
+  if (tid < nr) stage_segment<NQ>(M, M.tile_seg[ti * MW_TSMAX + tid], rs[tid]);
+  if (tid >= 64 && tid - 64 < nc)
+    stage_segment<NQ>(M, M.tile_seg[tj * MW_TSMAX + tid - 64], cs[tid - 64]);
+  __syncthreads();
+
+  const int npair = nr * nc;
+  const int pair = tid >> 1, half = tid & 1;
+  const bool live = pair < npair;
+  const int li = live ? pair / nc : 0, lj = live ? pair % nc : 0;
+  const SegStage<NQ>& I = rs[li];
+  const SegStage<NQ>& J = cs[lj];
+  const bool same_edge = live && I.edge == J.edge;
+  const bool ext = EXT && I.ekl >= 0 && I.ekl == J.ekl;
+  const bool smooth = live && !same_edge;
+
+  // the lane's QH x NQ point pairs: phasor exp(-j k R)/(4 pi R) at the
+  // current wavenumber and its step exp(-j dk R)
+  double er[QH][NQ], ei[QH][NQ], sr[QH][NQ], si[QH][NQ];
+#pragma unroll
+  for (int qq = 0; qq < QH; ++qq)
+#pragma unroll
+    for (int r = 0; r < NQ; ++r) { er[qq][r] = ei[qq][r] = 0.0; sr[qq][r] = 1.0; si[qq][r] = 0.0; }
+  if (smooth) {
+    const double a2 = I.a * I.a;
+#pragma unroll
+    for (int qq = 0; qq < QH; ++qq) {
+      const int q = QH * half + qq;
+#pragma unroll
+      for (int r = 0; r < NQ; ++r) {
+        const double dx = I.pos[q][0] - J.pos[r][0], dy = I.pos[q][1] - J.pos[r][1];
+        const double dz = I.pos[q][2] - J.pos[r][2];
+        const double R = sqrt(dx * dx + dy * dy + dz * dz + a2);
+        double sn, cn;
+        sincos(G.k0 * R, &sn, &cn);
+        const double inv = INV4PI / R;
+        er[qq][r] = cn * inv;
+        ei[qq][r] = -sn * inv;
+        if (G.nk > 1) {
+          sincos(G.dk * R, &sn, &cn);
+          sr[qq][r] = cn;
+          si[qq][r] = -sn;
+        }
+      }
+    }
+  }
+  int se_off = 0;
+  if (same_edge) {
+    const int ed = I.edge;
+    const int n_e = M.edge_i[2 * ed], off = M.edge_i[2 * ed + 1];
+    se_off = (off + (J.eidx - I.eidx) + n_e - 1) * 9;
+  }
+  const double td = I.tan[0] * J.tan[0] + I.tan[1] * J.tan[1] + I.tan[2] * J.tan[2];
+
+  // gather: output (m, n) of the tile pair per thread of the first MW_TB^2;
+  // the band-block offsets of its up to 9 wing pairs, 16 bits each
+  // (0xffff: absent), packed once for every wavenumber
+  const int m0 = M.tile_b0[ti], n0 = M.tile_b0[tj];
+  const int gm = m0 + tid / MW_TB, gn = n0 + tid % MW_TB;
+  const bool gather = tid < MW_TB * MW_TB && gm < M.tile_b0[ti + 1] && gn < M.tile_b0[tj + 1];
+  uint32_t goff[5] = {0xffffffffu, 0xffffffffu, 0xffffffffu, 0xffffffffu, 0xffffffffu};
+  if (gather) {
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      const int la = M.bw_loc[3 * gm + a], sa = M.bw_slot[3 * gm + a];
+#pragma unroll
+      for (int b = 0; b < 3; ++b) {
+        const int lb = M.bw_loc[3 * gn + b], sb = M.bw_slot[3 * gn + b];
+        const uint32_t o = (la < 0 || lb < 0) ? 0xffffu
+                                               : (uint32_t)((la * nc + lb) * 9 + sa * 3 + sb);
+        const int i = 3 * a + b, sh = 16 * (i & 1);
+        goff[i >> 1] = (goff[i >> 1] & ~(0xffffu << sh)) | (o << sh);
+      }
+    }
+  }
+
+#pragma unroll 2
+  for (int f = 0; f < G.nk; ++f) {
+    const mw_sweep_prob& p = P[group_prob[G.first + f]];
+    const double wmu = p.omega * mu, inv_weps = 1.0 / (p.omega * eps);
+    double2* __restrict__ bt = Bt[f & 1];
+    double jr[NM][NM], jim[NM][NM];
+    if (smooth) {
+      // T[b] = sum_r w_b(r) g(q, r) per observer point, then J += w_a(q) T[b];
+      // each phasor is used at this wavenumber, then rotated to the next
+#pragma unroll
+      for (int qq = 0; qq < QH; ++qq) {
+        const int q = QH * half + qq;
+        double tr[NM], tm[NM];
+#pragma unroll
+        for (int r = 0; r < NQ; ++r) {
+          const double gr = er[qq][r], gi = ei[qq][r];
+#pragma unroll
+          for (int b = 0; b < NM; ++b) {
+            const double w = J.wu[b][r];
+            tr[b] = r == 0 ? w * gr : fma(w, gr, tr[b]);
+            tm[b] = r == 0 ? w * gi : fma(w, gi, tm[b]);
+          }
+          if (ext) {
+            // the extended kernel's factor, a polynomial in k R: its
+            // departure from 1 is added on
+            const double dx = I.pos[q][0] - J.pos[r][0], dy = I.pos[q][1] - J.pos[r][1];
+            const double dz = I.pos[q][2] - J.pos[r][2];
+            const double R = sqrt(dx * dx + dy * dy + dz * dz + I.a * I.a);
+            const double2 d = ek_factor_minus1(R, I.a, p.k);
+            const double hr = gr * d.x - gi * d.y, hi = gr * d.y + gi * d.x;
+#pragma unroll
+            for (int b = 0; b < NM; ++b) {
+              tr[b] = fma(J.wu[b][r], hr, tr[b]);
+              tm[b] = fma(J.wu[b][r], hi, tm[b]);
+            }
+          }
+          er[qq][r] = gr * sr[qq][r] - gi * si[qq][r];
+          ei[qq][r] = gr * si[qq][r] + gi * sr[qq][r];
+        }
+#pragma unroll
+        for (int a = 0; a < NM; ++a) {
+          const double w = I.wu[a][q];
+#pragma unroll
+          for (int b = 0; b < NM; ++b) {
+            jr[a][b] = qq == 0 ? w * tr[b] : fma(w, tr[b], jr[a][b]);
+            jim[a][b] = qq == 0 ? w * tm[b] : fma(w, tm[b], jim[a][b]);
+          }
+        }
+      }
+    } else {
+#pragma unroll
+      for (int a = 0; a < NM; ++a)
+#pragma unroll
+        for (int b = 0; b < NM; ++b) { jr[a][b] = 0.0; jim[a][b] = 0.0; }
+      if (same_edge && half == 0) {
+        const double2* tab = reinterpret_cast<const double2*>(p.se_tab) + se_off;
+#pragma unroll
+        for (int a = 0; a < NM; ++a)
+#pragma unroll
+          for (int b = 0; b < NM; ++b) {
+            const double2 v = tab[a * NM + b];
+            jr[a][b] = v.x; jim[a][b] = v.y;
+          }
+      }
+    }
+#pragma unroll
+    for (int a = 0; a < NM; ++a)
+#pragma unroll
+      for (int b = 0; b < NM; ++b) {
+        jr[a][b] += lane_swap1(jr[a][b]);
+        jim[a][b] += lane_swap1(jim[a][b]);
+      }
+    if (live) contract_band<NQ>(jr, jim, I, J, td, wmu, inv_weps, half, bt + (li * nc + lj) * 9);
+    __syncthreads();
+    if (gather) {
+      double zr = 0.0, zi = 0.0;
+#pragma unroll
+      for (int i = 0; i < 9; ++i) {
+        const uint32_t o = (goff[i >> 1] >> (16 * (i & 1))) & 0xffffu;
+        if (o != 0xffffu) {
+          const double2 v = bt[o];
+          zr += v.x; zi += v.y;
+        }
+      }
+      double2* Z = reinterpret_cast<double2*>(p.A);
+      Z[(int64_t)gm * p.lda + gn] = make_double2(zr, zi);
+      if (sym && ti != tj) Z[(int64_t)gn * p.lda + gm] = make_double2(zr, zi);
+    }
+  }
 }
 
 template <int NQ, bool FAR32 = false>
@@ -593,16 +795,24 @@ hipError_t mw_launch_fill_far32(const mw_mesh& M, double k, double omega, double
 #undef MW_FILL32
 }
 
+// Items [0, n_plain) are tile pairs of meshes without the extended kernel,
+// [n_plain, n_items) of meshes with it.
 hipError_t mw_launch_batch_fill(const mw_sweep_prob* P, const mw_mesh* meshes, int nprob,
                                 const int32_t* se_items, int64_t n_se_items,
-                                const int32_t* item_prob, const int32_t* item_pair,
-                                int64_t n_items, double eps, double mu, hipStream_t st) {
+                                const mw_kgroup* groups, const int32_t* group_prob,
+                                const int32_t* item_group, const int32_t* item_pair,
+                                int64_t n_items, int64_t n_plain, double eps, double mu,
+                                hipStream_t st) {
   if (nprob <= 0) return hipSuccess;
   if (n_se_items > 0)
     hipLaunchKernelGGL(kb_same_edge, dim3((unsigned)n_se_items), dim3(256), 0, st, P, meshes,
                        se_items);
-  if (n_items > 0)
-    hipLaunchKernelGGL(kb_fill, dim3((unsigned)n_items), dim3(256), 0, st, P, meshes, item_prob,
-                       item_pair, eps, mu);
+  if (n_plain > 0)
+    hipLaunchKernelGGL(kb_fill_k<false>, dim3((unsigned)n_plain), dim3(KF_THREADS), 0, st, P,
+                       meshes, groups, group_prob, item_group, item_pair, eps, mu);
+  if (n_items > n_plain)
+    hipLaunchKernelGGL(kb_fill_k<true>, dim3((unsigned)(n_items - n_plain)), dim3(KF_THREADS), 0,
+                       st, P, meshes, groups, group_prob, item_group + n_plain,
+                       item_pair + n_plain, eps, mu);
   return hipGetLastError();
 }

@@ -39,13 +39,6 @@ class BatchedSweep:
     """All port solutions of ``structures`` x ``k_array`` on the GPU."""
 
     def __init__(self, structures, k_array, gemm_variant=None, cancel=None, pivot_threshold=32.0):
-        torch = _torch()
-        lib = _lib.load()
-        self.lib = lib
-        self.torch = torch
-        self.k_array = np.asarray(k_array, dtype=float)
-        self.cancel = cancel
-        self.pivot_threshold = float(pivot_threshold)
         self.solvers = [BSplineSolver(**kw) for kw in structures]
         for i, s in enumerate(self.solvers):
             if s.n_qp_pair != 4:
@@ -53,6 +46,17 @@ class BatchedSweep:
             if s._loading_active:
                 raise NotImplementedError(f"structure {i}: wire loading in the batched sweep "
                                           "(solve it with BSplineSolver one frequency at a time)")
+            if s.ground_z is not None:
+                raise NotImplementedError(f"structure {i}: a ground plane in the batched sweep "
+                                          "(its fill is free space only; solve it with "
+                                          "BSplineSolver one frequency at a time)")
+        torch = _torch()
+        lib = _lib.load()
+        self.lib = lib
+        self.torch = torch
+        self.k_array = np.asarray(k_array, dtype=float)
+        self.cancel = cancel
+        self.pivot_threshold = float(pivot_threshold)
         dev = torch.device("cuda")
         n_mesh = len(self.solvers)
         meshes = (_lib.MwMesh * n_mesh)()
@@ -182,6 +186,19 @@ class BatchedSweep:
                 self._launch(self.gemm_variant)
             self._graph = g
         self._graph.replay()
+
+    def fill(self):
+        """Only the fill: every problem's Z into its buffer (what the
+        single-structure fill writes at that wavenumber); see z_matrix."""
+        st = self.torch.cuda.current_stream().cuda_stream
+        _lib.check(self.lib.mw_sweep_fill(self._plan_for(self.gemm_variant), self.eps, self.mu,
+                                          st), "sweep fill")
+
+    def z_matrix(self, index):
+        """Z of problem ``index`` (after fill(); run() factors it in place)."""
+        i, kk, buf = self.problems[index]
+        n = self.devs[i].nb
+        return buf["A"][:, :n].cpu().numpy()
 
     def y_matrices(self):
         """[(structure index, k, Y as numpy)] in problem order; raises on a

@@ -175,3 +175,12 @@ def test_noalloc_gesv_refuses_oversize_on_the_host():
     assert lib.mw_zgesv_noalloc(16385, 1, None, 16386, None, None, None, None) == _lib.MW_ERANGE
     assert b"n <= 16384" in lib.mw_last_error()
     assert lib.mw_zgesv_noalloc(0, 1, None, 1, None, None, None, None) == _lib.MW_EINVAL
+
+
+def test_batched_sweep_refuses_ground():
+    """The batched fill is free space only: a grounded structure is refused
+    before anything touches the GPU, never solved without its ground."""
+    from momwire_amd.sweep import BatchedSweep
+
+    with pytest.raises(NotImplementedError, match="ground"):
+        BatchedSweep([W.straight_dipole(), W.grounded_monopole("pec")], [1.0])
