@@ -139,6 +139,27 @@ typedef struct mw_ports {
 
 typedef struct mw_sweep_plan mw_sweep_plan;
 
+/*
+ * A leaf block of the block-low-rank (H-matrix) operator.  Its rows and
+ * columns are cluster members (basis ids, ascending) listed from row_off /
+ * col_off in two index pools.  Storage sits in one complex double store at
+ * data_off: a dense block as D (m x n, row-major); a compressed block as U^T
+ * (cap x m) then V (cap x n), U V ~ the block.
+ */
+typedef struct mw_hblock {
+  int32_t m, n;        /* rows, columns                                       */
+  int32_t rank;        /* compressed: rank of U V; -1: stored dense           */
+  int32_t cap;         /* compressed: factor capacity in ranks; dense: 0      */
+  int64_t row_off;     /* first row member in the row pool                    */
+  int64_t col_off;     /* first column member in the column pool              */
+  int64_t data_off;    /* complex offset of the block's storage in the store  */
+  int64_t aux_off;     /* mw_hm_aca: byte offset of its m + n used flags in the
+                          workspace; mw_hm_invert: complex offset of its
+                          2 m^2 scratch in the workspace                      */
+  int64_t t_off;       /* compressed: first row of V X[cols] in mw_hm_apply's
+                          scratch T (rows of nrhs complex)                    */
+} mw_hblock;
+
 /* Last error message of the calling thread ("" if none). */
 const char* mw_last_error(void);
 /* ABI version of this library. */
@@ -300,6 +321,68 @@ int mw_zsparse_rows(int64_t nrows, int32_t maxnz, const int32_t* idx,
 /* X[:, :] = 0, then X[idx[c][t]][c] = val[c][t]: dense right-hand sides. */
 int mw_zscatter_cols(int64_t n, int64_t ncols, int32_t maxnz, const int32_t* idx,
                      const double* val, void* X, int64_t ldx, void* stream);
+
+/*
+ * Block-low-rank operator of Z (H-matrix / ACA), free space.  Replaces the
+ * reference's HMatrixSolver operator build and apply (hmatrix.py:982-1083,
+ * _aca.py:137-248) and its off-edge block assembler for ACA sampling
+ * (_accelerators.cpp:1949-2208).  wing_seg / wing_slot: [n_basis][3]
+ * device tables, the global segment and band slot of every wing of every
+ * basis (-1: absent wing).
+ *
+ * mw_zentries: out[t] = Z[rows[t]][cols[t]], t < n.  same_edge_table != 0
+ * takes same-edge segment pairs from the mesh's same-edge table (the dense
+ * fill's entries; call mw_same_edge_table for this k first), 0 integrates
+ * every pair by quadrature (the admissible blocks' off-edge kernel,
+ * hmatrix.py:1097-1155).
+ */
+int mw_zentries(const mw_mesh* mesh, const int32_t* wing_seg, const int32_t* wing_slot, double k,
+                double omega, double eps, double mu, const int32_t* rows, const int32_t* cols,
+                int64_t n, int32_t same_edge_table, void* out, void* stream);
+/*
+ * mw_hm_fill_dense: D of the dense blocks, work items[t] = (block index << 32
+ * | first entry), 256 row-major entries each (near blocks, hmatrix.py:1010-
+ * 1022, with same_edge_table = 1; admissible blocks that did not compress,
+ * hmatrix.py:1054-1058, with 0).
+ */
+int mw_hm_fill_dense(const mw_mesh* mesh, const int32_t* wing_seg, const int32_t* wing_slot,
+                     double k, double omega, double eps, double mu, const int32_t* rows_pool,
+                     const int32_t* cols_pool, const mw_hblock* blocks, const int64_t* items,
+                     int64_t n_items, int32_t same_edge_table, void* store, void* stream);
+/*
+ * mw_hm_aca: partial-pivoted ACA of the blocks list[0 .. n_list), one
+ * workgroup each, all at once: the reference's aca_partial (_aca.py:137-211)
+ * with its pivoting, hop and stopping rules at tolerance tol.  Writes U^T, V
+ * and rank into each block (rank -1: the factors would not compress, or the
+ * rank outgrew cap; the caller then stores the block dense).  work: the used
+ * flags (aux_off bytes per block).
+ */
+int mw_hm_aca(const mw_mesh* mesh, const int32_t* wing_seg, const int32_t* wing_slot, double k,
+              double omega, double eps, double mu, const int32_t* rows_pool,
+              const int32_t* cols_pool, mw_hblock* blocks, const int32_t* list, int32_t n_list,
+              double tol, void* store, void* work, void* stream);
+/*
+ * mw_hm_apply: Y = H X for nrhs columns (X, Y: n x nrhs, row strides ldx,
+ * ldy), the batched product of the reference's HMatrix.matmat
+ * (_aca.py:238-248).  lr_list: the compressed blocks (T = V X[cols] for
+ * each, T sized sum of ranks x nrhs); row_ptr / row_ent: per row of Y, the
+ * (block, position of the row among the block's rows) pairs that hold it,
+ * summed in that order.
+ */
+int mw_hm_apply(int64_t n, const int32_t* cols_pool, const mw_hblock* blocks,
+                const int32_t* lr_list, int32_t n_lr, const int64_t* row_ptr,
+                const int32_t* row_ent, const void* store, const void* X, int64_t ldx,
+                int32_t nrhs, void* T, void* Y, int64_t ldy, void* stream);
+/*
+ * mw_hm_invert: each square dense block of list replaced by its inverse
+ * (Gauss-Jordan, partial pivoting; info[t] = 1 for an exactly singular
+ * block, left unchanged).  The leaf diagonal blocks' inverses precondition
+ * the Krylov solve (block Jacobi).  No counterpart in the reference, which
+ * factors the whole near field with a sparse LU (hmatrix.py:95-112,
+ * 1468-1493).
+ */
+int mw_hm_invert(const mw_hblock* blocks, const int32_t* list, int32_t n_list, void* store,
+                 void* work, int32_t* info, void* stream);
 
 /*
  * Small dense solve T <- S^{-1} T in one workgroup (S: ns x ns, T: ns x nt,

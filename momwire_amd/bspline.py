@@ -193,6 +193,14 @@ class _Device:
             lib = _lib.load()
             self.work = torch.empty(int(lib.mw_getrf_workspace_bytes(nb)), dtype=torch.uint8,
                                     device=self.dev)
+            self.post_buffers()
+        return self
+
+    def post_buffers(self):
+        """What the KCL Schur step and the readout need (no Z)."""
+        torch = self.torch
+        if getattr(self, "Y", None) is None:
+            lib = _lib.load()
             self.info = torch.zeros(2, dtype=torch.int32, device=self.dev)
             self.Y = torch.empty((max(self.n_ports, 1), max(self.n_ports, 1)),
                                  dtype=torch.complex128, device=self.dev)
@@ -539,13 +547,17 @@ class BSplineSolver(_SweptPortSolutions):
             self._ports = (cols, con)
         return self._ports
 
-    def _device(self):
+    def _device_base(self):
+        """The device mesh tables and port columns (no solve buffers)."""
         if self._dev is None:
             cols, con = self._port_plan()
             mesh = self._build()
             ek = _mesh.ek_tables(mesh, self.ground_z) if self.extended_kernel else None
             self._dev = _Device(mesh, cols + con, con, cols, ek=ek)
-        return self._dev.buffers()
+        return self._dev
+
+    def _device(self):
+        return self._device_base().buffers()
 
     # ------------------------------------------------------------------
     def _solve_device(self, k=None, timers=None):

@@ -43,6 +43,27 @@ hipError_t mw_launch_z2c(int64_t rows, int64_t cols, const void* src, int64_t ld
                          int64_t ldd, hipStream_t st);
 hipError_t mw_launch_c2z(int64_t rows, int64_t cols, const void* src, int64_t lds, void* dst,
                          int64_t ldd, int accumulate, hipStream_t st);
+hipError_t mw_launch_zentries(const mw_mesh& M, const int32_t* wing_seg, const int32_t* wing_slot,
+                              double k, double omega, double eps, double mu, const int32_t* rows,
+                              const int32_t* cols, int64_t n, int table, void* out,
+                              hipStream_t st);
+hipError_t mw_launch_hm_dense(const mw_mesh& M, const int32_t* wing_seg, const int32_t* wing_slot,
+                              double k, double omega, double eps, double mu,
+                              const int32_t* rows_pool, const int32_t* cols_pool,
+                              const mw_hblock* blk, const int64_t* items, int64_t n_items,
+                              int table, void* store, hipStream_t st);
+hipError_t mw_launch_hm_aca(const mw_mesh& M, const int32_t* wing_seg, const int32_t* wing_slot,
+                            double k, double omega, double eps, double mu,
+                            const int32_t* rows_pool, const int32_t* cols_pool, mw_hblock* blk,
+                            const int32_t* list, int32_t n_list, double tol, void* store,
+                            void* flags, hipStream_t st);
+hipError_t mw_launch_hm_apply(int64_t nrows, const int32_t* cols_pool, const mw_hblock* blk,
+                              const int32_t* lr_list, int32_t n_lr, const int64_t* row_ptr,
+                              const int32_t* row_ent, const void* store, const void* X,
+                              int64_t ldx, int32_t s, void* T, void* Y, int64_t ldy,
+                              hipStream_t st);
+hipError_t mw_launch_hm_invert(const mw_hblock* blk, const int32_t* list, int32_t n_list,
+                               void* store, void* work, int32_t* info, hipStream_t st);
 void mw_set_pivot_bound(double lbound);
 double mw_get_pivot_bound();
 hipError_t mw_launch_batch_fill(const mw_sweep_prob* P, const mw_mesh* meshes, int nprob,
@@ -750,6 +771,92 @@ int mw_sweep_fill(mw_sweep_plan* p, double eps, double mu, void* stream) {
   g_err[0] = 0;
   if (!p) return fail(MW_EINVAL, "%splan is NULL");
   return hip_status(sweep_fill(p, eps, mu, (hipStream_t)stream), "sweep fill");
+}
+
+// --- block-low-rank operator --------------------------------------------
+
+static int check_entry_args(const mw_mesh* mesh, const int32_t* wing_seg,
+                            const int32_t* wing_slot, double omega, double eps, double mu) {
+  int rc = check_mesh(mesh);
+  if (rc) return rc;
+  if (!wing_seg || !wing_slot) return fail(MW_EINVAL, "%swing tables are NULL");
+  if (!(omega > 0.0) || !(eps > 0.0) || !(mu > 0.0))
+    return fail(MW_EINVAL, "%somega, eps and mu must be positive");
+  return MW_OK;
+}
+
+int mw_zentries(const mw_mesh* mesh, const int32_t* wing_seg, const int32_t* wing_slot, double k,
+                double omega, double eps, double mu, const int32_t* rows, const int32_t* cols,
+                int64_t n, int32_t same_edge_table, void* out, void* stream) {
+  g_err[0] = 0;
+  int rc = check_entry_args(mesh, wing_seg, wing_slot, omega, eps, mu);
+  if (rc) return rc;
+  if (n < 0) return fail(MW_EINVAL, "%snegative entry count");
+  if (n > 0 && (!rows || !cols || !out)) return fail(MW_EINVAL, "%sNULL argument");
+  return hip_status(mw_launch_zentries(*mesh, wing_seg, wing_slot, k, omega, eps, mu, rows, cols, n,
+                                       same_edge_table, out, (hipStream_t)stream),
+                    "mw_zentries");
+}
+
+int mw_hm_fill_dense(const mw_mesh* mesh, const int32_t* wing_seg, const int32_t* wing_slot,
+                     double k, double omega, double eps, double mu, const int32_t* rows_pool,
+                     const int32_t* cols_pool, const mw_hblock* blocks, const int64_t* items,
+                     int64_t n_items, int32_t same_edge_table, void* store, void* stream) {
+  g_err[0] = 0;
+  int rc = check_entry_args(mesh, wing_seg, wing_slot, omega, eps, mu);
+  if (rc) return rc;
+  if (n_items < 0 || n_items > 0x7fffffffLL) return fail(MW_ERANGE, "%sbad work item count");
+  if (n_items > 0 && (!rows_pool || !cols_pool || !blocks || !items || !store))
+    return fail(MW_EINVAL, "%sNULL argument");
+  return hip_status(mw_launch_hm_dense(*mesh, wing_seg, wing_slot, k, omega, eps, mu, rows_pool,
+                                       cols_pool, blocks, items, n_items, same_edge_table, store,
+                                       (hipStream_t)stream),
+                    "mw_hm_fill_dense");
+}
+
+int mw_hm_aca(const mw_mesh* mesh, const int32_t* wing_seg, const int32_t* wing_slot, double k,
+              double omega, double eps, double mu, const int32_t* rows_pool,
+              const int32_t* cols_pool, mw_hblock* blocks, const int32_t* list, int32_t n_list,
+              double tol, void* store, void* work, void* stream) {
+  g_err[0] = 0;
+  int rc = check_entry_args(mesh, wing_seg, wing_slot, omega, eps, mu);
+  if (rc) return rc;
+  if (n_list < 0) return fail(MW_EINVAL, "%snegative block count");
+  if (!(tol > 0.0)) return fail(MW_EINVAL, "%sACA tolerance must be positive");
+  if (n_list > 0 && (!rows_pool || !cols_pool || !blocks || !list || !store || !work))
+    return fail(MW_EINVAL, "%sNULL argument");
+  return hip_status(mw_launch_hm_aca(*mesh, wing_seg, wing_slot, k, omega, eps, mu, rows_pool,
+                                     cols_pool, blocks, list, n_list, tol, store, work,
+                                     (hipStream_t)stream),
+                    "mw_hm_aca");
+}
+
+int mw_hm_apply(int64_t n, const int32_t* cols_pool, const mw_hblock* blocks,
+                const int32_t* lr_list, int32_t n_lr, const int64_t* row_ptr,
+                const int32_t* row_ent, const void* store, const void* X, int64_t ldx,
+                int32_t nrhs, void* T, void* Y, int64_t ldy, void* stream) {
+  g_err[0] = 0;
+  if (n < 0 || nrhs < 0 || n_lr < 0) return fail(MW_EINVAL, "%snegative size");
+  if (n > 0 && nrhs > 0) {
+    if (!cols_pool || !blocks || !row_ptr || !row_ent || !store || !X || !Y ||
+        (n_lr > 0 && (!lr_list || !T)))
+      return fail(MW_EINVAL, "%sNULL argument");
+    if (ldx < nrhs || ldy < nrhs) return fail(MW_EINVAL, "%sbad leading dimension");
+  }
+  return hip_status(mw_launch_hm_apply(n, cols_pool, blocks, lr_list, n_lr, row_ptr, row_ent,
+                                       store, X, ldx, nrhs, T, Y, ldy, (hipStream_t)stream),
+                    "mw_hm_apply");
+}
+
+int mw_hm_invert(const mw_hblock* blocks, const int32_t* list, int32_t n_list, void* store,
+                 void* work, int32_t* info, void* stream) {
+  g_err[0] = 0;
+  if (n_list < 0) return fail(MW_EINVAL, "%snegative block count");
+  if (n_list > 0 && (!blocks || !list || !store || !work || !info))
+    return fail(MW_EINVAL, "%sNULL argument");
+  return hip_status(mw_launch_hm_invert(blocks, list, n_list, store, work, info,
+                                        (hipStream_t)stream),
+                    "mw_hm_invert");
 }
 
 int mw_sweep_plan_destroy(mw_sweep_plan* p) {
