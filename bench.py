@@ -66,6 +66,16 @@ WORKLOADS = {
                                 "elements at 8 per segment: 2,688 sources) on a 64 x 64 grid of "
                                 "points 0.1-1.1 wavelength above the ground, FP64",
                            args=None, golden=None),
+    "hm16x16": dict(desc="16x16 array of z-directed 0.48-lambda dipoles, 15 segments each (3,840 "
+                         "unknowns), 0.6-lambda pitch, +-0.02-lambda seeded jitter (seed 2), "
+                         "radius 1e-3 lambda, free space, 300 MHz, 256 centre feeds; "
+                         "block-low-rank operator (aca_eta 1, leaf 32, aca_tol 1e-4) and block "
+                         "GMRES to solve_tol 1e-6, FP64",
+                    args=(16, 16, 15, 2), golden="hm_array16x16"),
+    "hm32x32": dict(desc="the array32x32 structure (15,360 unknowns, 1024 centre feeds) on the "
+                         "block-low-rank operator (aca_eta 1, leaf 32, aca_tol 1e-4) and block "
+                         "GMRES to solve_tol 1e-6, FP64",
+                    args=(32, 32, 15, 2), golden=None),
     "sweep128": dict(desc="128 seeded random polyline structures (1-12 wires with shared "
                           "junction nodes, 43-827 unknowns) x 16 frequencies 0.8-1.2 x 300 MHz, "
                           "1-4 feeds, FP64; all fills and factorisations in one HIP graph",
@@ -746,6 +756,103 @@ def cpu_baseline(kw, budget_s=20.0, cold_and_warm=False, max_runs=5):
                       f"accelerator, solve {med - fill:.2f} s in scipy LAPACK); {where}"}
 
 
+ENTRY_FLOPS = 9 * 16 * 53  # one Z entry: 9 wing pairs x 16 point pairs x 53 flop
+
+
+def hm_main(args, wl):
+    """The block-low-rank port solve: one step = the operator build (near
+    blocks, ACA of every far block, dense fallbacks) plus the block-GMRES
+    solve of every port column and the readout, device-resident."""
+    import torch
+
+    from momwire_amd import _lib, workloads
+    from momwire_amd.bspline import BSplineSolver
+    from momwire_amd.hmatrix import HMatrixSolver, block_gmres
+
+    kw = workloads.dipole_array(*wl["args"][:3], seed=wl["args"][3])
+    s = HMatrixSolver(**kw)
+    mesh = s._build()
+    nb = mesh.n_basis
+    t0 = time.perf_counter()
+    part = s.build_partition()
+    t_part = time.perf_counter() - t0
+    print(f"[bench] {args.workload}: {nb} unknowns, {s._port_count()} ports, partition "
+          f"{part.n_far} far + {part.n_near} near blocks in {t_part:.2f} s (host, once per "
+          f"geometry)", file=sys.stderr, flush=True)
+    for _ in range(args.warmup):
+        s._solve()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        Y, W = s._solve()
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / args.steps
+    # stage split (HIP events on the launch stream): operator build, solve
+    st = {"build": [], "solve": []}
+    for _ in range(3):
+        e0, e1, e2 = (torch.cuda.Event(enable_timing=True) for _ in range(3))
+        e0.record()
+        H = s.build_hmatrix()
+        e1.record()
+        D = s._dev
+        B = torch.zeros((nb, D.ncols), dtype=torch.complex128, device="cuda")
+        _lib.check(_lib.load().mw_zscatter_cols(nb, D.ncols, D.col_max, D.col_idx.data_ptr(),
+                                                D.col_val.data_ptr(), B.data_ptr(), D.ncols,
+                                                torch.cuda.current_stream().cuda_stream), "B")
+        X, iters = block_gmres(H.apply, H.precond, B, s.solve_tol)
+        e2.record()
+        torch.cuda.synchronize()
+        st["build"].append(e0.elapsed_time(e1))
+        st["solve"].append(e1.elapsed_time(e2))
+    stage = {k: float(np.mean(v)) for k, v in st.items()}
+    stats = H.stats()
+    y = Y.cpu().numpy()
+    # dense FP64 GPU solve of the same structure, for time and the approximation error
+    d = BSplineSolver(**kw)
+    d.solve_graph()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(3):
+        Yd, _Wd = d.solve_graph(warm=False)
+    torch.cuda.synchronize()
+    dense_ms = 1e3 * (time.perf_counter() - t0) / 3
+    yd = Yd.cpu().numpy()
+    err_dense = float(np.abs(y - yd).max() / np.abs(yd).max())
+    err_ref = None
+    if wl["golden"]:
+        g = np.load(os.path.join(ROOT, "tests", "golden", wl["golden"] + ".npz"))
+        err_ref = float(np.abs(y - g["y"]).max() / np.abs(g["y"]).max())
+    b = H.blocks
+    far = part.blocks[:, 4] == 1
+    lr = b["rank"] >= 0
+    fl = far & lr
+    samples = int((b["rank"][fl].astype(np.int64) * (b["m"][fl] + b["n"][fl])).sum())
+    dense_entries = int((b["m"][~lr].astype(np.int64) * b["n"][~lr]).sum())
+    entries = samples + dense_entries
+    build_flops = entries * ENTRY_FLOPS
+    achieved = build_flops / (stage["build"] * 1e-3) / 1e12
+    out = {
+        "metric": METRIC, "value": 1.0 / dt, "unit": "port-solves/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": 1e3 * dt,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": wl["desc"], "unknowns": nb, "ports": int(D.n_ports),
+                   "step": "operator build + block GMRES + readout, device-resident"},
+        "stage_ms": stage, "gmres_iterations": int(iters), "hmatrix_stats": stats,
+        "operator_bytes": 16 * stats["storage"], "dense_z_bytes": 16 * nb * nb,
+        "dense_fp64_gpu_ms": dense_ms,
+        "y_rel_err_vs_dense_gpu": err_dense,
+        "y_rel_err_vs_reference_hmatrix": err_ref,
+        "roofline": {"kernel": "the operator build (k_hm_dense + k_hm_aca: Z entries at the "
+                               "near blocks and the ACA samples)",
+                     "bound": "valu", "achieved": achieved, "peak": PEAK_FP64_TFLOPS,
+                     "unit": "TFLOP/s", "frac": achieved / PEAK_FP64_TFLOPS, "traffic": None,
+                     "algorithmic": f"{entries} entries (dense blocks + ACA rows/columns "
+                                    f"sum r (m + n)) x {ENTRY_FLOPS} flop"},
+    }
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -798,6 +905,8 @@ def main():
         return readout_main(args, wl)
     if args.workload == "groundfield4x4":
         return groundfield_main(args, wl)
+    if args.workload.startswith("hm"):
+        return hm_main(args, wl)
     if wl["golden"] is None:
         return mixed_main(args, wl)
 

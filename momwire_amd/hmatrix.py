@@ -114,8 +114,9 @@ class HOperator:
         _lib.check(lib.mw_same_edge_table(D.mw, k, st), "same-edge table")
         near_ids = np.flatnonzero(~far)
         self._fill_dense(args, d_blocks, blocks, near_ids, 1, store, st)
-        # far blocks: ACA, all at once
+        # far blocks: ACA, all at once, the largest first (they finish last)
         far_ids = np.flatnonzero(far).astype(np.int32)
+        far_ids = far_ids[np.argsort(-(m64 + n64)[far_ids], kind="stable")]
         if far_ids.size:
             flags = torch.empty(int(flag_sz.sum()), dtype=torch.uint8, device=dev)
             fl = up(far_ids)
@@ -279,12 +280,36 @@ class HOperator:
         return Y
 
 
+def _qr_tall(W):
+    """W = Q R for a tall block (N x s): CholeskyQR2 -- two passes of the
+    Gram matrix's Cholesky factor, GEMMs on the MFMA units instead of a
+    column-by-column Householder sweep.  A Gram matrix that is not
+    numerically positive definite (nearly dependent columns) falls back to
+    Householder QR."""
+    import torch
+
+    Q, R = W, None
+    for _ in range(2):
+        L, info = torch.linalg.cholesky_ex(Q.conj().T @ Q)
+        if int(info) != 0:
+            return torch.linalg.qr(W)
+        Rk = L.conj().T
+        Q = torch.linalg.solve_triangular(Rk, Q, upper=True, left=False)
+        R = Rk if R is None else Rk @ R
+    return Q, R
+
+
 def block_gmres(op, prec, B, rtol, restart=None, maxiter=2000):
     """Left-preconditioned restarted block GMRES for op(X) = B, every column
     in one block Krylov space (the reference's _block_gmres,
-    hmatrix.py:177-273: CGS2 orthogonalisation, incremental QR of the block
-    Hessenberg, stopping on the per-column preconditioned residual relative
-    to |M^{-1} b_j|).  Device tensors throughout.  Returns (X, iterations)."""
+    hmatrix.py:177-273): preconditioned initial guess M^{-1} B, CGS2
+    orthogonalisation, stopping on the per-column preconditioned residual
+    relative to |M^{-1} b_j|.  The small least-squares problem over the block
+    Hessenberg is solved afresh at each step from a thin QR of the Hessenberg
+    (the reference updates one incrementally with block rotations; the
+    solution and the residual test are the same), so every factorisation is
+    a CholeskyQR of a tall block.  Device tensors throughout.  Returns (X,
+    iterations)."""
     import torch
 
     N, s = B.shape
@@ -296,20 +321,19 @@ def block_gmres(op, prec, B, rtol, restart=None, maxiter=2000):
     X = Bt.clone()
     m = min(restart, N)
     total = 0
-    dt = B.dtype
+    dt, dev = B.dtype, B.device
     for _ in range(maxiter // m + 1):
         R = Bt - prec(op(X))
         if bool((torch.linalg.vector_norm(R, dim=0) <= rtol * bnorms).all()):
             break
-        Q, beta = torch.linalg.qr(R)
-        V = torch.empty((N, (m + 1) * s), dtype=dt, device=B.device)
+        Q, beta = _qr_tall(R)
+        V = torch.empty((N, (m + 1) * s), dtype=dt, device=dev)
         V[:, :s] = Q
-        Rbar = torch.zeros(((m + 1) * s, m * s), dtype=dt, device=B.device)
-        g = torch.zeros(((m + 1) * s, s), dtype=dt, device=B.device)
-        g[:s] = beta
-        rots = []
-        kb = 0
-        converged = False
+        Hs = torch.zeros(((m + 1) * s, m * s), dtype=dt, device=dev)
+        E = torch.zeros(((m + 1) * s, s), dtype=dt, device=dev)
+        E[:s] = beta
+        Yk = None
+        psz = 0
         for k in range(m):
             total += 1
             p = (k + 1) * s
@@ -319,34 +343,25 @@ def block_gmres(op, prec, B, rtol, restart=None, maxiter=2000):
             W = W - Vp @ H1
             H2 = Vp.conj().T @ W
             W = W - Vp @ H2
-            Qk, Hkk = torch.linalg.qr(W)
+            Qk, Hkk = _qr_tall(W)
             V[:, p:p + s] = Qk
-            c = torch.empty((p + s, s), dtype=dt, device=B.device)
-            c[:p] = H1 + H2
-            c[p:] = Hkk
-            for j, Qh in enumerate(rots):
-                r0 = j * s
-                c[r0:r0 + 2 * s] = Qh.conj().T @ c[r0:r0 + 2 * s]
-            Qh, Rh = torch.linalg.qr(c[p - s:p + s], mode="complete")
-            rots.append(Qh)
-            c[p - s:p] = Rh[:s]
-            Rbar[:p, p - s:p] = c[:p]
-            g[p - s:p + s] = Qh.conj().T @ g[p - s:p + s]
-            kb = k + 1
-            rk = torch.linalg.vector_norm(g[p:p + s], dim=0)
+            Hs[:p, p - s:p] = H1 + H2
+            Hs[p:p + s, p - s:p] = Hkk
+            Hk = Hs[:p + s, :p]
+            Qh, Rh = _qr_tall(Hk)
+            Ek = E[:p + s]
+            diag = torch.abs(torch.diagonal(Rh))
+            if bool((diag > 1e-14 * max(float(diag.max()), 1.0)).all()):
+                Yk = torch.linalg.solve_triangular(Rh, Qh.conj().T @ Ek, upper=True)
+            else:  # near-breakdown: least squares on the host
+                Yk = torch.from_numpy(np.linalg.lstsq(Hk.cpu().numpy(), Ek.cpu().numpy(),
+                                                      rcond=None)[0]).to(dev)
+            psz = p
+            rk = torch.linalg.vector_norm(Ek - Hk @ Yk, dim=0)
             if bool((rk <= rtol * bnorms).all()):
-                converged = True
                 break
-        psz = kb * s
-        Rtop = Rbar[:psz, :psz]
-        diag = torch.abs(torch.diagonal(Rtop))
-        if bool((diag > 1e-14 * max(float(diag.max()), 1.0)).all()):
-            Yk = torch.linalg.solve_triangular(Rtop, g[:psz], upper=True)
-        else:  # near-breakdown: least squares on the host
-            Yk = torch.from_numpy(np.linalg.lstsq(Rtop.cpu().numpy(), g[:psz].cpu().numpy(),
-                                                  rcond=None)[0]).to(B.device)
         X = X + V[:, :psz] @ Yk
-        if converged:
+        if bool((rk <= rtol * bnorms).all()):
             break
     return X, total
 
