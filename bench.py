@@ -767,7 +767,7 @@ def hm_main(args, wl):
 
     from momwire_amd import _lib, workloads
     from momwire_amd.bspline import BSplineSolver
-    from momwire_amd.hmatrix import HMatrixSolver, block_gmres
+    from momwire_amd.hmatrix import HMatrixSolver
 
     kw = workloads.dipole_array(*wl["args"][:3], seed=wl["args"][3])
     s = HMatrixSolver(**kw)
@@ -799,7 +799,7 @@ def hm_main(args, wl):
         _lib.check(_lib.load().mw_zscatter_cols(nb, D.ncols, D.col_max, D.col_idx.data_ptr(),
                                                 D.col_val.data_ptr(), B.data_ptr(), D.ncols,
                                                 torch.cuda.current_stream().cuda_stream), "B")
-        X, iters = block_gmres(H.apply, H.precond, B, s.solve_tol)
+        X, iters = H.solve(B, s.solve_tol)
         e2.record()
         torch.cuda.synchronize()
         st["build"].append(e0.elapsed_time(e1))

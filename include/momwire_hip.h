@@ -158,7 +158,25 @@ typedef struct mw_hblock {
                           2 m^2 scratch in the workspace                      */
   int64_t t_off;       /* compressed: first row of V X[cols] in mw_hm_apply's
                           scratch T (rows of nrhs complex)                    */
+  int64_t row_start;   /* leaf-order position of the first row (the cluster
+                          tree's leaves one after another: every cluster is
+                          one contiguous range)                               */
+  int64_t col_start;   /* leaf-order position of the first column            */
 } mw_hblock;
+
+/* mw_hm_apply work items: 64 V rows of the compressed blocks that share a
+   column cluster (their (block, rank) pairs from t_rows[row0]); k columns
+   whose leaf-order positions are colpos[col_off ..]. */
+typedef struct mw_hm_titem {
+  int32_t row0, nrows, k, pad;
+  int64_t col_off;
+} mw_hm_titem;
+/* mw_hm_apply work items: up to 32 leaf-order rows from row0, summed over
+   the (block, row offset) pairs r_ent[ent0 .. ent0 + nent). */
+typedef struct mw_hm_ritem {
+  int64_t row0;
+  int32_t nrows, ent0, nent, pad;
+} mw_hm_ritem;
 
 /* Last error message of the calling thread ("" if none). */
 const char* mw_last_error(void);
@@ -362,17 +380,20 @@ int mw_hm_aca(const mw_mesh* mesh, const int32_t* wing_seg, const int32_t* wing_
               const int32_t* cols_pool, mw_hblock* blocks, const int32_t* list, int32_t n_list,
               double tol, void* store, void* work, void* stream);
 /*
- * mw_hm_apply: Y = H X for nrhs columns (X, Y: n x nrhs, row strides ldx,
- * ldy), the batched product of the reference's HMatrix.matmat
- * (_aca.py:238-248).  lr_list: the compressed blocks (T = V X[cols] for
- * each, T sized sum of ranks x nrhs); row_ptr / row_ent: per row of Y, the
- * (block, position of the row among the block's rows) pairs that hold it,
- * summed in that order.
+ * mw_hm_apply: Y = H X for nrhs columns in leaf order (X, Y: n x nrhs, row
+ * strides ldx, ldy), the batched product of the reference's HMatrix.matmat
+ * (_aca.py:238-248).  Dense blocks are stored in leaf order; compressed
+ * blocks keep the ACA's member order: sigma[row_off + t] is the position of
+ * leaf-order row t among the block's rows, colpos[col_off + j] the
+ * leaf-order position of column member j.  Phase 1 forms T = V X[cols]
+ * (t_items; T: sum of ranks x nrhs, rows from t_off), phase 2 the rows of Y
+ * (r_items), each row's terms summed in block order.
  */
-int mw_hm_apply(int64_t n, const int32_t* cols_pool, const mw_hblock* blocks,
-                const int32_t* lr_list, int32_t n_lr, const int64_t* row_ptr,
-                const int32_t* row_ent, const void* store, const void* X, int64_t ldx,
-                int32_t nrhs, void* T, void* Y, int64_t ldy, void* stream);
+int mw_hm_apply(const mw_hblock* blocks, const void* store, const mw_hm_titem* t_items,
+                int32_t n_t, const int32_t* t_rows, const int32_t* colpos,
+                const mw_hm_ritem* r_items, int32_t n_r, const int32_t* r_ent,
+                const int32_t* sigma, const void* X, int64_t ldx, int32_t nrhs, void* T,
+                void* Y, int64_t ldy, void* stream);
 /*
  * mw_hm_invert: each square dense block of list replaced by its inverse
  * (Gauss-Jordan, partial pivoting; info[t] = 1 for an exactly singular

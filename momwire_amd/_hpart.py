@@ -151,6 +151,55 @@ class Partition:
         self.leaves = [c.indices for c in self.root.leaves()]
         self.n_far = int(self.blocks[:, 4].sum())
         self.n_near = len(self.blocks) - self.n_far
+        # leaf order: the leaves' members one after another; every cluster is
+        # then one contiguous range of it (a leaf's members keep their
+        # ascending order, so a leaf block reads the same in both orders)
+        self.perm = np.concatenate(self.leaves).astype(np.int64)
+        self.pos = np.empty(self.n, dtype=np.int64)
+        self.pos[self.perm] = np.arange(self.n)
+        # per block: first leaf-order position of its row and column clusters
+        self.row_start = np.array([self.pos[self.block_rows(b)].min() for b in
+                                   range(len(self.blocks))], dtype=np.int64)
+        self.col_start = np.array([self.pos[self.block_cols(b)].min() for b in
+                                   range(len(self.blocks))], dtype=np.int64)
+        nb = len(self.blocks)
+        m = self.blocks[:, 1]
+        n = self.blocks[:, 3]
+        # every block's members in leaf order (dense blocks are stored so)
+        self.rows_lpool = np.concatenate(
+            [self.perm[self.row_start[b]:self.row_start[b] + m[b]] for b in range(nb)]
+        ).astype(np.int32)
+        self.cols_lpool = np.concatenate(
+            [self.perm[self.col_start[b]:self.col_start[b] + n[b]] for b in range(nb)]
+        ).astype(np.int32)
+        # compressed blocks keep ascending member order: sigma[row_off + t] is
+        # leaf-order row t's position among the block's rows, colpos the
+        # leaf-order position of each column member
+        self.sigma = np.concatenate(
+            [np.searchsorted(self.block_rows(b), self.rows_lpool[self.blocks[b, 0]:
+                                                                 self.blocks[b, 0] + m[b]])
+             for b in range(nb)]).astype(np.int32)
+        self.colpos = self.pos[self.cols_pool].astype(np.int32)
+        # row chunks of at most 32 leaf-order rows, never across a leaf
+        starts, sizes = [], []
+        at = 0
+        for leaf in self.leaves:
+            for o in range(0, leaf.size, 32):
+                starts.append(at + o)
+                sizes.append(min(32, leaf.size - o))
+            at += leaf.size
+        self.chunk_start = np.array(starts, dtype=np.int64)
+        self.chunk_rows = np.array(sizes, dtype=np.int32)
+        # (chunk, block, row offset) for every block over every chunk it holds,
+        # in block order
+        lo = np.searchsorted(self.chunk_start, self.row_start)
+        hi = np.searchsorted(self.chunk_start, self.row_start + m)
+        cnt = hi - lo
+        self.ent_block = np.repeat(np.arange(nb), cnt).astype(np.int32)
+        first = np.repeat(lo - np.concatenate([[0], np.cumsum(cnt)[:-1]]), cnt)
+        self.ent_chunk = (np.arange(cnt.sum()) + first).astype(np.int64)
+        self.ent_ro = (self.chunk_start[self.ent_chunk]
+                       - self.row_start[self.ent_block]).astype(np.int32)
 
     def block_rows(self, b):
         o, m = self.blocks[b, 0], self.blocks[b, 1]

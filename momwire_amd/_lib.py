@@ -131,8 +131,8 @@ SIGNATURES = {
                                         _P, _P, _I64, _I32, _P, _P]),
     "mw_hm_aca": (ctypes.c_int, [ctypes.POINTER(MwMesh), _P, _P, _D, _D, _D, _D, _P, _P, _P, _P,
                                  _I32, _D, _P, _P, _P]),
-    "mw_hm_apply": (ctypes.c_int, [_I64, _P, _P, _P, _I32, _P, _P, _P, _P, _I64, _I32, _P, _P,
-                                   _I64, _P]),
+    "mw_hm_apply": (ctypes.c_int, [_P, _P, _P, _I32, _P, _P, _P, _I32, _P, _P, _P, _I64, _I32,
+                                   _P, _P, _I64, _P]),
     "mw_hm_invert": (ctypes.c_int, [_P, _P, _I32, _P, _P, _P, _P]),
     "mw_sweep_plan_destroy": (ctypes.c_int, [_P]),
     "mw_somm_six_integrals": (ctypes.c_int, [_D, _D, _D, _P, _P, _I64, _D, _I32, _P, _P, _P]),

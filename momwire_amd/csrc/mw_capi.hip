@@ -57,11 +57,11 @@ hipError_t mw_launch_hm_aca(const mw_mesh& M, const int32_t* wing_seg, const int
                             const int32_t* rows_pool, const int32_t* cols_pool, mw_hblock* blk,
                             const int32_t* list, int32_t n_list, double tol, void* store,
                             void* flags, hipStream_t st);
-hipError_t mw_launch_hm_apply(int64_t nrows, const int32_t* cols_pool, const mw_hblock* blk,
-                              const int32_t* lr_list, int32_t n_lr, const int64_t* row_ptr,
-                              const int32_t* row_ent, const void* store, const void* X,
-                              int64_t ldx, int32_t s, void* T, void* Y, int64_t ldy,
-                              hipStream_t st);
+hipError_t mw_launch_hm_apply(const mw_hblock* blk, const void* store, const mw_hm_titem* t_items,
+                              int32_t n_t, const int32_t* t_rows, const int32_t* colpos,
+                              const mw_hm_ritem* r_items, int32_t n_r, const int32_t* r_ent,
+                              const int32_t* sigma, const void* X, int64_t ldx, int32_t s,
+                              void* T, void* Y, int64_t ldy, hipStream_t st);
 hipError_t mw_launch_hm_invert(const mw_hblock* blk, const int32_t* list, int32_t n_list,
                                void* store, void* work, int32_t* info, hipStream_t st);
 void mw_set_pivot_bound(double lbound);
@@ -774,6 +774,8 @@ int mw_sweep_fill(mw_sweep_plan* p, double eps, double mu, void* stream) {
 }
 
 // --- block-low-rank operator --------------------------------------------
+static_assert(sizeof(mw_hblock) == 72, "mw_hblock layout (the Python binding mirrors it)");
+static_assert(sizeof(mw_hm_titem) == 24 && sizeof(mw_hm_ritem) == 24, "work item layouts");
 
 static int check_entry_args(const mw_mesh* mesh, const int32_t* wing_seg,
                             const int32_t* wing_slot, double omega, double eps, double mu) {
@@ -831,20 +833,21 @@ int mw_hm_aca(const mw_mesh* mesh, const int32_t* wing_seg, const int32_t* wing_
                     "mw_hm_aca");
 }
 
-int mw_hm_apply(int64_t n, const int32_t* cols_pool, const mw_hblock* blocks,
-                const int32_t* lr_list, int32_t n_lr, const int64_t* row_ptr,
-                const int32_t* row_ent, const void* store, const void* X, int64_t ldx,
-                int32_t nrhs, void* T, void* Y, int64_t ldy, void* stream) {
+int mw_hm_apply(const mw_hblock* blocks, const void* store, const mw_hm_titem* t_items,
+                int32_t n_t, const int32_t* t_rows, const int32_t* colpos,
+                const mw_hm_ritem* r_items, int32_t n_r, const int32_t* r_ent,
+                const int32_t* sigma, const void* X, int64_t ldx, int32_t nrhs, void* T,
+                void* Y, int64_t ldy, void* stream) {
   g_err[0] = 0;
-  if (n < 0 || nrhs < 0 || n_lr < 0) return fail(MW_EINVAL, "%snegative size");
-  if (n > 0 && nrhs > 0) {
-    if (!cols_pool || !blocks || !row_ptr || !row_ent || !store || !X || !Y ||
-        (n_lr > 0 && (!lr_list || !T)))
+  if (nrhs < 0 || n_t < 0 || n_r < 0) return fail(MW_EINVAL, "%snegative size");
+  if (nrhs > 0) {
+    if (!blocks || !store || !X || !Y || (n_r > 0 && (!r_items || !r_ent)) ||
+        (n_t > 0 && (!t_items || !t_rows || !colpos || !T)))
       return fail(MW_EINVAL, "%sNULL argument");
     if (ldx < nrhs || ldy < nrhs) return fail(MW_EINVAL, "%sbad leading dimension");
   }
-  return hip_status(mw_launch_hm_apply(n, cols_pool, blocks, lr_list, n_lr, row_ptr, row_ent,
-                                       store, X, ldx, nrhs, T, Y, ldy, (hipStream_t)stream),
+  return hip_status(mw_launch_hm_apply(blocks, store, t_items, n_t, t_rows, colpos, r_items, n_r,
+                                       r_ent, sigma, X, ldx, nrhs, T, Y, ldy, (hipStream_t)stream),
                     "mw_hm_apply");
 }
 

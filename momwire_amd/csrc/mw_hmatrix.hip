@@ -360,73 +360,173 @@ __global__ __launch_bounds__(AT) void k_hm_aca(EntryCtx C, const int32_t* __rest
 }
 
 // ---------------------------------------------------------------------------
-// Operator product Y = H X (X, Y: n_basis x s, row stride ldx / ldy)
+// Operator product Y = H X in leaf order (X, Y: n x s, row strides ldx/ldy)
 // ---------------------------------------------------------------------------
-// T_b = V_b X[J_b] for every low-rank block: (block, 64 columns) per
-// workgroup, one rank row per wave at a time.
-__global__ __launch_bounds__(256) void k_hm_vx(const int32_t* __restrict__ cols_pool,
-                                               const mw_hblock* __restrict__ blk,
-                                               const int32_t* __restrict__ list,
-                                               const double2* __restrict__ store,
-                                               const double2* __restrict__ X, int64_t ldx, int s,
-                                               double2* __restrict__ T) {
-  const mw_hblock B = blk[list[blockIdx.x]];
-  if (B.rank <= 0) return;
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int c = blockIdx.y * 64 + lane;
-  if (c >= s) return;
-  const int32_t* Jc = cols_pool + B.col_off;
-  const double2* V = store + B.data_off + (int64_t)B.cap * B.m;
-  for (int kk = wave; kk < B.rank; kk += 4) {
-    double2 acc = make_double2(0.0, 0.0);
-    const double2* vk = V + (int64_t)kk * B.n;
-    for (int j = 0; j < B.n; ++j) {
-      const double2 v = vk[j], x = X[(int64_t)Jc[j] * ldx + c];
-      acc.x += v.x * x.x - v.y * x.y;
-      acc.y += v.x * x.y + v.y * x.x;
+// Leaf order lists the cluster tree's leaves one after another, so every
+// cluster is a contiguous range of rows.  Dense blocks are stored in leaf
+// order; compressed blocks keep the ACA's member order (ascending basis id),
+// reached through two maps: sigma (a row's position among the block's rows
+// in that order) and colpos (a column member's leaf-order position).
+//
+// Phase 1, T = V X[cols]: the V rows of every compressed block that shares
+// a column cluster form one product; a workgroup computes 64 of those rows
+// x 64 columns, 4 x 4 complex per thread, K in chunks of 16 through LDS.
+constexpr int TR = 64, TC = 64, TKC = 16;
+
+__global__ __launch_bounds__(256) void k_hm_t(const mw_hblock* __restrict__ blk,
+                                              const mw_hm_titem* __restrict__ items,
+                                              const int32_t* __restrict__ t_rows,
+                                              const int32_t* __restrict__ colpos,
+                                              const double2* __restrict__ store,
+                                              const double2* __restrict__ X, int64_t ldx, int s,
+                                              double2* __restrict__ T) {
+  __shared__ double2 As[TKC][TR];
+  __shared__ double2 Bs[TKC][TC];
+  __shared__ int64_t vrow[TR];  // complex offset of the V row, -1: padding
+  __shared__ int64_t trow[TR];  // row of T
+  const mw_hm_titem it = items[blockIdx.x];
+  const int c0 = blockIdx.y * TC;
+  const int tid = threadIdx.x, rg = tid >> 4, cg = tid & 15;
+  if (tid < TR) {
+    vrow[tid] = -1;
+    if (tid < it.nrows) {
+      const int b = t_rows[2 * (it.row0 + tid)], k = t_rows[2 * (it.row0 + tid) + 1];
+      const mw_hblock B = blk[b];
+      vrow[tid] = B.data_off + (int64_t)B.cap * B.m + (int64_t)k * B.n;
+      trow[tid] = B.t_off + k;
     }
-    T[(B.t_off + kk) * (int64_t)s + c] = acc;
+  }
+  __syncthreads();
+  double2 acc[4][4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) acc[i][j] = make_double2(0.0, 0.0);
+  const int K = it.k;
+  for (int k0 = 0; k0 < K; k0 += TKC) {
+    for (int idx = tid; idx < TR * TKC; idx += 256) {
+      const int r = idx / TKC, kk = idx % TKC;
+      const int64_t v = vrow[r];
+      As[kk][r] = (v >= 0 && k0 + kk < K) ? store[v + k0 + kk] : make_double2(0.0, 0.0);
+    }
+    for (int idx = tid; idx < TKC * TC; idx += 256) {
+      const int kk = idx / TC, c = idx % TC;
+      Bs[kk][c] = (k0 + kk < K && c0 + c < s)
+                      ? X[(int64_t)colpos[it.col_off + k0 + kk] * ldx + c0 + c]
+                      : make_double2(0.0, 0.0);
+    }
+    __syncthreads();
+#pragma unroll 4
+    for (int kk = 0; kk < TKC; ++kk) {
+      double2 a[4], bv[4];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) a[i] = As[kk][rg + 16 * i];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) bv[j] = Bs[kk][cg + 16 * j];
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          acc[i][j].x = fma(a[i].x, bv[j].x, fma(-a[i].y, bv[j].y, acc[i][j].x));
+          acc[i][j].y = fma(a[i].x, bv[j].y, fma(a[i].y, bv[j].x, acc[i][j].y));
+        }
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const int r = rg + 16 * i;
+    if (r >= it.nrows) continue;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int c = c0 + cg + 16 * j;
+      if (c < s) T[trow[r] * (int64_t)s + c] = acc[i][j];
+    }
   }
 }
 
-// Y[i] = sum over the blocks holding row i (row_ptr / row_ent: (block,
-// position of i among the block's rows), in block order): dense D[p] X[J],
-// low-rank U^T[:, p] . T_b.  One wave per row, 64 columns per wave.
-__global__ __launch_bounds__(256) void k_hm_rows(int64_t nrows,
-                                                 const int32_t* __restrict__ cols_pool,
-                                                 const mw_hblock* __restrict__ blk,
-                                                 const int64_t* __restrict__ row_ptr,
-                                                 const int32_t* __restrict__ row_ent,
-                                                 const double2* __restrict__ store,
-                                                 const double2* __restrict__ X, int64_t ldx,
-                                                 const double2* __restrict__ T, int s,
-                                                 double2* __restrict__ Y, int64_t ldy) {
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int64_t i = (int64_t)blockIdx.x * 4 + wave;
-  const int c = blockIdx.y * 64 + lane;
-  if (i >= nrows || c >= s) return;
-  double2 acc = make_double2(0.0, 0.0);
-  for (int64_t e = row_ptr[i]; e < row_ptr[i + 1]; ++e) {
-    const mw_hblock B = blk[row_ent[2 * e]];
-    const int p = row_ent[2 * e + 1];
-    if (B.rank < 0) {
-      const double2* D = store + B.data_off + (int64_t)p * B.n;
-      const int32_t* Jc = cols_pool + B.col_off;
-      for (int j = 0; j < B.n; ++j) {
-        const double2 d = D[j], x = X[(int64_t)Jc[j] * ldx + c];
-        acc.x += d.x * x.x - d.y * x.y;
-        acc.y += d.x * x.y + d.y * x.x;
+// Phase 2: 32 leaf-order rows x 64 columns per workgroup, summed over every
+// block whose rows hold them (the item's entries: block, offset of the
+// item's first row among the block's leaf-order rows), in block order:
+// dense D[rows][:] X[cols], compressed U[rows][:] T_b.  4 x 4 complex per
+// thread.
+constexpr int RR = 32, RC = 64, RKC = 16;
+
+__global__ __launch_bounds__(128) void k_hm_r(const mw_hblock* __restrict__ blk,
+                                              const mw_hm_ritem* __restrict__ items,
+                                              const int32_t* __restrict__ ent,
+                                              const int32_t* __restrict__ sigma,
+                                              const double2* __restrict__ store,
+                                              const double2* __restrict__ X, int64_t ldx,
+                                              const double2* __restrict__ T, int s,
+                                              double2* __restrict__ Y, int64_t ldy) {
+  __shared__ double2 As[RKC][RR];
+  __shared__ double2 Bs[RKC][RC];
+  const mw_hm_ritem it = items[blockIdx.x];
+  const int c0 = blockIdx.y * RC;
+  const int tid = threadIdx.x, rg = tid >> 4, cg = tid & 15;
+  double2 acc[4][4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) acc[i][j] = make_double2(0.0, 0.0);
+  for (int e = it.ent0; e < it.ent0 + it.nent; ++e) {
+    const mw_hblock B = blk[ent[2 * e]];
+    const int ro = ent[2 * e + 1];
+    const bool dense = B.rank < 0;
+    const int K = dense ? B.n : B.rank;
+    const double2* D = store + B.data_off;
+    for (int k0 = 0; k0 < K; k0 += RKC) {
+      for (int idx = tid; idx < RR * RKC; idx += 128) {
+        double2 v = make_double2(0.0, 0.0);
+        if (dense) {
+          const int t = idx / RKC, kk = idx % RKC;
+          if (t < it.nrows && k0 + kk < K) v = D[(int64_t)(ro + t) * B.n + k0 + kk];
+          As[kk][t] = v;
+        } else {
+          const int kk = idx / RR, t = idx % RR;
+          if (t < it.nrows && k0 + kk < K)
+            v = D[(int64_t)(k0 + kk) * B.m + sigma[B.row_off + ro + t]];
+          As[kk][t] = v;
+        }
       }
-    } else {
-      const double2* Ut = store + B.data_off + p;
-      for (int kk = 0; kk < B.rank; ++kk) {
-        const double2 u = Ut[(int64_t)kk * B.m], t = T[(B.t_off + kk) * (int64_t)s + c];
-        acc.x += u.x * t.x - u.y * t.y;
-        acc.y += u.x * t.y + u.y * t.x;
+      for (int idx = tid; idx < RKC * RC; idx += 128) {
+        const int kk = idx / RC, c = idx % RC;
+        double2 v = make_double2(0.0, 0.0);
+        if (k0 + kk < K && c0 + c < s)
+          v = dense ? X[(B.col_start + k0 + kk) * ldx + c0 + c]
+                    : T[(B.t_off + k0 + kk) * (int64_t)s + c0 + c];
+        Bs[kk][c] = v;
       }
+      __syncthreads();
+#pragma unroll 4
+      for (int kk = 0; kk < RKC; ++kk) {
+        double2 a[4], bv[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) a[i] = As[kk][rg + 8 * i];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) bv[j] = Bs[kk][cg + 16 * j];
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+#pragma unroll
+          for (int j = 0; j < 4; ++j) {
+            acc[i][j].x = fma(a[i].x, bv[j].x, fma(-a[i].y, bv[j].y, acc[i][j].x));
+            acc[i][j].y = fma(a[i].x, bv[j].y, fma(a[i].y, bv[j].x, acc[i][j].y));
+          }
+      }
+      __syncthreads();
     }
   }
-  Y[i * ldy + c] = acc;
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const int t = rg + 8 * i;
+    if (t >= it.nrows) continue;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int c = c0 + cg + 16 * j;
+      if (c < s) Y[(it.row0 + t) * ldy + c] = acc[i][j];
+    }
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -564,23 +664,23 @@ hipError_t mw_launch_hm_aca(const mw_mesh& M, const int32_t* wing_seg, const int
   return hipGetLastError();
 }
 
-hipError_t mw_launch_hm_apply(int64_t nrows, const int32_t* cols_pool, const mw_hblock* blk,
-                              const int32_t* lr_list, int32_t n_lr, const int64_t* row_ptr,
-                              const int32_t* row_ent, const void* store, const void* X,
-                              int64_t ldx, int32_t s, void* T, void* Y, int64_t ldy,
-                              hipStream_t st) {
-  if (nrows <= 0 || s <= 0) return hipSuccess;
-  const unsigned cy = (unsigned)((s + 63) / 64);
-  if (n_lr > 0)
-    hipLaunchKernelGGL(k_hm_vx, dim3((unsigned)n_lr, cy), dim3(256), 0, st, cols_pool, blk,
-                       lr_list, reinterpret_cast<const double2*>(store),
+hipError_t mw_launch_hm_apply(const mw_hblock* blk, const void* store, const mw_hm_titem* t_items,
+                              int32_t n_t, const int32_t* t_rows, const int32_t* colpos,
+                              const mw_hm_ritem* r_items, int32_t n_r, const int32_t* r_ent,
+                              const int32_t* sigma, const void* X, int64_t ldx, int32_t s,
+                              void* T, void* Y, int64_t ldy, hipStream_t st) {
+  if (s <= 0) return hipSuccess;
+  if (n_t > 0)
+    hipLaunchKernelGGL(k_hm_t, dim3((unsigned)n_t, (unsigned)((s + TC - 1) / TC)), dim3(256), 0,
+                       st, blk, t_items, t_rows, colpos, reinterpret_cast<const double2*>(store),
                        reinterpret_cast<const double2*>(X), ldx, (int)s,
                        reinterpret_cast<double2*>(T));
-  hipLaunchKernelGGL(k_hm_rows, dim3((unsigned)((nrows + 3) / 4), cy), dim3(256), 0, st, nrows,
-                     cols_pool, blk, row_ptr, row_ent, reinterpret_cast<const double2*>(store),
-                     reinterpret_cast<const double2*>(X), ldx,
-                     reinterpret_cast<const double2*>(T), (int)s, reinterpret_cast<double2*>(Y),
-                     ldy);
+  if (n_r > 0)
+    hipLaunchKernelGGL(k_hm_r, dim3((unsigned)n_r, (unsigned)((s + RC - 1) / RC)), dim3(128), 0,
+                       st, blk, r_items, r_ent, sigma, reinterpret_cast<const double2*>(store),
+                       reinterpret_cast<const double2*>(X), ldx,
+                       reinterpret_cast<const double2*>(T), (int)s, reinterpret_cast<double2*>(Y),
+                       ldy);
   return hipGetLastError();
 }
 

@@ -38,33 +38,35 @@ from .bspline import BSplineSolver, _to_host, _torch
 
 HBLOCK = np.dtype([("m", np.int32), ("n", np.int32), ("rank", np.int32), ("cap", np.int32),
                    ("row_off", np.int64), ("col_off", np.int64), ("data_off", np.int64),
-                   ("aux_off", np.int64), ("t_off", np.int64)])
-assert HBLOCK.itemsize == 56
+                   ("aux_off", np.int64), ("t_off", np.int64), ("row_start", np.int64),
+                   ("col_start", np.int64)])
+TITEM = np.dtype([("row0", np.int32), ("nrows", np.int32), ("k", np.int32), ("pad", np.int32),
+                  ("col_off", np.int64)])
+RITEM = np.dtype([("row0", np.int64), ("nrows", np.int32), ("ent0", np.int32), ("nent", np.int32),
+                  ("pad", np.int32)])
+assert HBLOCK.itemsize == 72 and TITEM.itemsize == 24 and RITEM.itemsize == 24
 
 
-def _row_csr(blocks, rows_pool, nrows, select=None):
-    """Per row, the (block, position) pairs of the blocks that hold it, in
-    block order: (row_ptr int64 [nrows + 1], row_ent int32 [nnz][2])."""
-    ids = np.arange(len(blocks)) if select is None else np.asarray(select)
-    rows, bid, pos = [], [], []
-    for b in ids:
-        o, m = int(blocks[b]["row_off"]), int(blocks[b]["m"])
-        rows.append(rows_pool[o:o + m])
-        bid.append(np.full(m, b, dtype=np.int32))
-        pos.append(np.arange(m, dtype=np.int32))
-    rows = np.concatenate(rows)
-    bid = np.concatenate(bid)
-    pos = np.concatenate(pos)
-    order = np.argsort(rows, kind="stable")
-    ptr = np.zeros(nrows + 1, dtype=np.int64)
-    np.cumsum(np.bincount(rows, minlength=nrows), out=ptr[1:])
-    ent = np.stack([bid[order], pos[order]], axis=1).astype(np.int32)
-    return ptr, np.ascontiguousarray(ent)
+def _r_items(part, keep_block):
+    """Phase-2 work of the product: per row chunk, its (block, row offset)
+    entries in block order, blocks with keep_block False left out."""
+    sel = keep_block[part.ent_block]
+    chunk = part.ent_chunk[sel]
+    order = np.argsort(chunk, kind="stable")
+    ent = np.stack([part.ent_block[sel][order], part.ent_ro[sel][order]], axis=1)
+    counts = np.bincount(chunk, minlength=len(part.chunk_start))
+    items = np.zeros(len(part.chunk_start), dtype=RITEM)
+    items["row0"] = part.chunk_start
+    items["nrows"] = part.chunk_rows
+    items["ent0"] = np.concatenate([[0], np.cumsum(counts)[:-1]])
+    items["nent"] = counts
+    return items, np.ascontiguousarray(ent.astype(np.int32))
 
 
 class HOperator:
-    """The compressed operator on the device: block table, index pools,
-    store, and the row lists of its product (and of the preconditioner)."""
+    """The compressed operator on the device: block table, index pools and
+    store, and the work lists of its product (and of the preconditioner).
+    The product runs in leaf order (the partition's perm)."""
 
     def __init__(self, solver, part, tol, k):
         torch = _torch()
@@ -82,6 +84,8 @@ class HOperator:
         blocks["n"] = part.blocks[:, 3]
         blocks["row_off"] = part.blocks[:, 0]
         blocks["col_off"] = part.blocks[:, 2]
+        blocks["row_start"] = part.row_start
+        blocks["col_start"] = part.col_start
         far = part.blocks[:, 4] == 1
         m64 = blocks["m"].astype(np.int64)
         n64 = blocks["n"].astype(np.int64)
@@ -100,9 +104,18 @@ class HOperator:
         np.cumsum(flag_sz[:-1], out=aux[1:])
         blocks["aux_off"] = aux
 
+        cache = D.keep.setdefault("hm_pools", {})
+        key = id(part)
+        if key not in cache:
+            up0 = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dev)  # noqa: E731
+            cache.clear()
+            cache[key] = dict(rows=up0(part.rows_pool), cols=up0(part.cols_pool),
+                              lrows=up0(part.rows_lpool), lcols=up0(part.cols_lpool),
+                              sigma=up0(part.sigma), colpos=up0(part.colpos),
+                              perm=up0(part.perm), pos=up0(part.pos))
+        pools = cache[key]
+        self.pools = pools
         up = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dev)  # noqa: E731
-        self.rows_pool = up(part.rows_pool)
-        self.cols_pool = up(part.cols_pool)
         self.wing_seg = D.keep["hm_wing_seg"]
         self.wing_slot = D.keep["hm_wing_slot"]
         store = torch.empty(int(size.sum()), dtype=torch.complex128, device=dev)
@@ -112,15 +125,14 @@ class HOperator:
 
         # near blocks: dense, same-edge pairs from the analytic table
         _lib.check(lib.mw_same_edge_table(D.mw, k, st), "same-edge table")
-        near_ids = np.flatnonzero(~far)
-        self._fill_dense(args, d_blocks, blocks, near_ids, 1, store, st)
+        self._fill_dense(args, d_blocks, blocks, np.flatnonzero(~far), 1, store)
         # far blocks: ACA, all at once, the largest first (they finish last)
         far_ids = np.flatnonzero(far).astype(np.int32)
         far_ids = far_ids[np.argsort(-(m64 + n64)[far_ids], kind="stable")]
         if far_ids.size:
             flags = torch.empty(int(flag_sz.sum()), dtype=torch.uint8, device=dev)
             fl = up(far_ids)
-            _lib.check(lib.mw_hm_aca(*args, self.rows_pool.data_ptr(), self.cols_pool.data_ptr(),
+            _lib.check(lib.mw_hm_aca(*args, pools["rows"].data_ptr(), pools["cols"].data_ptr(),
                                      d_blocks.data_ptr(), fl.data_ptr(), int(far_ids.size),
                                      float(tol), store.data_ptr(), flags.data_ptr(), st), "ACA")
             got, = _to_host(d_blocks)
@@ -131,28 +143,54 @@ class HOperator:
         if dense_far.size:
             extra = (m64 * n64)[dense_far]
             base = store.numel()
-            offs = base + np.concatenate([[0], np.cumsum(extra)[:-1]])
-            blocks["data_off"][dense_far] = offs
+            blocks["data_off"][dense_far] = base + np.concatenate([[0], np.cumsum(extra)[:-1]])
             blocks["cap"][dense_far] = 0
             store = torch.cat([store, torch.empty(int(extra.sum()), dtype=torch.complex128,
                                                   device=dev)])
             d_blocks = up(blocks.view(np.uint8))
-            self._fill_dense(args, d_blocks, blocks, dense_far, 0, store, st)
-        lr = np.flatnonzero(far & (blocks["rank"] > 0)).astype(np.int32)
+            self._fill_dense(args, d_blocks, blocks, dense_far, 0, store)
+        lr = far & (blocks["rank"] > 0)
+        lr_ids = np.flatnonzero(lr)
         t_off = np.zeros(len(blocks), dtype=np.int64)
-        t_off[lr] = np.concatenate([[0], np.cumsum(blocks["rank"][lr])[:-1]]) if lr.size else []
+        if lr_ids.size:
+            t_off[lr_ids] = np.concatenate([[0], np.cumsum(blocks["rank"][lr_ids])[:-1]])
         blocks["t_off"] = t_off
-        self.t_rows = int(blocks["rank"][lr].sum()) if lr.size else 0
+        self.t_rows = int(blocks["rank"][lr_ids].sum()) if lr_ids.size else 0
+        # phase 1 of the product: the V rows of blocks sharing a column
+        # cluster, 64 at a time
+        titems, trows = [], []
+        base = 0
+        if lr_ids.size:
+            keyc = part.col_start[lr_ids] * (n + 1) + blocks["n"][lr_ids]
+            order = np.argsort(keyc, kind="stable")
+            grp = lr_ids[order]
+            bounds = np.flatnonzero(np.diff(keyc[order])) + 1
+            for g in np.split(grp, bounds):
+                rk = blocks["rank"][g]
+                rows = np.stack([np.repeat(g, rk), np.concatenate([np.arange(r) for r in rk])],
+                                axis=1)
+                trows.append(rows)
+                for r0 in range(0, len(rows), 64):
+                    titems.append((base + r0, min(64, len(rows) - r0), int(blocks["n"][g[0]]), 0,
+                                   int(blocks["col_off"][g[0]])))
+                base += len(rows)
+        self.t_items = up(np.array(titems, dtype=TITEM).view(np.uint8)) if titems else None
+        self.n_t = len(titems)
+        self.t_rowlist = up(np.concatenate(trows).astype(np.int32)) if trows else None
+        # phase 2: every row chunk over the blocks that hold it (compressed
+        # blocks of rank 0 add nothing)
+        keep = ~(far & (blocks["rank"] == 0))
+        r_items, r_ent = _r_items(part, keep)
+        self.r_items, self.r_ent = up(r_items.view(np.uint8)), up(r_ent)
+        self.n_r = len(r_items)
         self.blocks = blocks
         self.d_blocks = up(blocks.view(np.uint8))
         self.store = store
-        self.lr = up(lr) if lr.size else None
-        self.n_lr = int(lr.size)
-        ptr, ent = _row_csr(blocks, part.rows_pool, n)
-        self.row_ptr, self.row_ent = up(ptr), up(ent)
         self._precond_built = False
 
-    def _fill_dense(self, args, d_blocks, blocks, ids, table, store, st):
+    def _fill_dense(self, args, d_blocks, blocks, ids, table, store):
+        """Dense blocks in leaf order (a leaf's members keep ascending order,
+        so near blocks read the same either way)."""
         if not len(ids):
             return
         items = []
@@ -161,8 +199,8 @@ class HOperator:
             items.append((np.int64(b) << 32) | np.arange(0, ne, 256, dtype=np.int64))
         items = np.concatenate(items)
         d_items = self.torch.from_numpy(items).to(store.device)
-        _lib.check(self.lib.mw_hm_fill_dense(*args, self.rows_pool.data_ptr(),
-                                             self.cols_pool.data_ptr(), d_blocks.data_ptr(),
+        _lib.check(self.lib.mw_hm_fill_dense(*args, self.pools["lrows"].data_ptr(),
+                                             self.pools["lcols"].data_ptr(), d_blocks.data_ptr(),
                                              d_items.data_ptr(), int(items.size), table,
                                              store.data_ptr(), self.torch.cuda.current_stream()
                                              .cuda_stream), "dense blocks")
@@ -186,67 +224,77 @@ class HOperator:
         }
 
     def to_dense(self):
-        """The operator as a dense numpy matrix (validation, small n): dense
-        blocks as stored, compressed blocks as U V (_aca.py:280-287)."""
+        """The operator as a dense numpy matrix in basis order (validation,
+        small n): dense blocks as stored, compressed blocks as U V
+        (_aca.py:280-287)."""
         store, = _to_host(self.store)
+        part = self.part
         Z = np.zeros((self.n, self.n), dtype=np.complex128)
         for b, B in enumerate(self.blocks):
-            I, J = self.part.block_rows(b), self.part.block_cols(b)
             m, n, o = int(B["m"]), int(B["n"]), int(B["data_off"])
+            ro, co = int(B["row_off"]), int(B["col_off"])
             if B["rank"] < 0:
+                I, J = part.rows_lpool[ro:ro + m], part.cols_lpool[co:co + n]
                 Z[np.ix_(I, J)] = store[o:o + m * n].reshape(m, n)
             else:
+                I, J = part.rows_pool[ro:ro + m], part.cols_pool[co:co + n]
                 r, cap = int(B["rank"]), int(B["cap"])
                 Ut = store[o:o + cap * m].reshape(cap, m)[:r]
                 V = store[o + cap * m:o + cap * (m + n)].reshape(cap, n)[:r]
                 Z[np.ix_(I, J)] = Ut.T @ V
         return Z
 
-    def apply(self, X, out=None):
-        """H X for a (n, s) complex128 device tensor."""
+    def apply_leaf(self, X):
+        """H X for a (n, s) complex128 device tensor in leaf order."""
         torch = self.torch
         X = X.contiguous()
         s = X.shape[1]
-        Y = torch.empty_like(X) if out is None else out
+        Y = torch.empty_like(X)
         T = (torch.empty((max(self.t_rows, 1), s), dtype=torch.complex128, device=X.device)
-             if self.n_lr else None)
+             if self.n_t else None)
         st = torch.cuda.current_stream().cuda_stream
         _lib.check(self.lib.mw_hm_apply(
-            self.n, self.cols_pool.data_ptr(), self.d_blocks.data_ptr(),
-            self.lr.data_ptr() if self.n_lr else None, self.n_lr, self.row_ptr.data_ptr(),
-            self.row_ent.data_ptr(), self.store.data_ptr(), X.data_ptr(), s, s,
-            T.data_ptr() if T is not None else None, Y.data_ptr(), Y.stride(0), st), "H X")
+            self.d_blocks.data_ptr(), self.store.data_ptr(),
+            self.t_items.data_ptr() if self.n_t else None, self.n_t,
+            self.t_rowlist.data_ptr() if self.n_t else None, self.pools["colpos"].data_ptr(),
+            self.r_items.data_ptr(), self.n_r, self.r_ent.data_ptr(),
+            self.pools["sigma"].data_ptr(), X.data_ptr(), s, s,
+            T.data_ptr() if T is not None else None, Y.data_ptr(), s, st), "H X")
         return Y
 
+    def apply(self, X):
+        """H X for a (n, s) complex128 device tensor in basis order."""
+        perm, pos = self.pools["perm"], self.pools["pos"]
+        return self.apply_leaf(X[perm])[pos]
+
     def _build_precond(self):
-        """Block Jacobi: the inverses of the leaf diagonal blocks."""
+        """Block Jacobi: the inverses of the leaf diagonal blocks, in leaf
+        order."""
         torch = self.torch
         part, blocks = self.part, self.blocks
         dev = self.store.device
-        diag = []
-        for b in range(len(blocks)):
-            if (part.blocks[b, 4] == 0 and blocks["m"][b] == blocks["n"][b]
-                    and part.blocks[b, 0] >= 0):
-                I = part.block_rows(b)
-                J = part.block_cols(b)
-                if I[0] == J[0] and np.array_equal(I, J):
-                    diag.append(b)
+        diag = [b for b in range(len(blocks))
+                if part.blocks[b, 4] == 0 and part.row_start[b] == part.col_start[b]
+                and blocks["m"][b] == blocks["n"][b]]
+        diag.sort(key=lambda b: part.row_start[b])  # leaf order
         pb = np.zeros(len(diag), dtype=HBLOCK)
         pb["m"] = blocks["m"][diag]
         pb["n"] = blocks["n"][diag]
         pb["rank"] = -1
-        pb["row_off"] = blocks["row_off"][diag]
-        pb["col_off"] = blocks["col_off"][diag]
+        pb["row_start"] = part.row_start[diag]
+        pb["col_start"] = part.col_start[diag]
         sz = pb["m"].astype(np.int64) * pb["n"]
         pb["data_off"] = np.concatenate([[0], np.cumsum(sz)[:-1]])
         pb["aux_off"] = np.concatenate([[0], np.cumsum(2 * sz)[:-1]])
-        covered = np.concatenate([part.block_rows(b) for b in diag])
-        if covered.size != self.n or np.unique(covered).size != self.n:
+        cover = np.zeros(self.n, dtype=np.int32)
+        for b in diag:
+            cover[part.row_start[b]:part.row_start[b] + blocks["m"][b]] += 1
+        if not (cover == 1).all():
             raise RuntimeError("the leaf diagonal blocks do not cover every basis once")
-        pstore = torch.empty(int(sz.sum()), dtype=torch.complex128, device=dev)
-        for i, b in enumerate(diag):
-            o, e = int(blocks["data_off"][b]), int(sz[i])
-            pstore[int(pb["data_off"][i]):int(pb["data_off"][i]) + e] = self.store[o:o + e]
+        src = np.concatenate([np.arange(int(blocks["data_off"][b]),
+                                        int(blocks["data_off"][b]) + int(sz[i]))
+                              for i, b in enumerate(diag)])
+        pstore = self.store[torch.from_numpy(src).to(dev)].contiguous()
         work = torch.empty(int(2 * sz.sum()), dtype=torch.complex128, device=dev)
         info = torch.zeros(len(diag), dtype=torch.int32, device=dev)
         d_pb = torch.from_numpy(pb.view(np.uint8).copy()).to(dev)
@@ -258,14 +306,21 @@ class HOperator:
         bad, = _to_host(info)
         if bad.any():
             raise np.linalg.LinAlgError("a leaf diagonal block of Z is singular")
-        ptr, ent = _row_csr(pb, part.rows_pool, self.n)
+        # one entry per row chunk: the inverse of its leaf
+        leaf_of = np.searchsorted(pb["row_start"], part.chunk_start, side="right") - 1
+        items = np.zeros(len(part.chunk_start), dtype=RITEM)
+        items["row0"] = part.chunk_start
+        items["nrows"] = part.chunk_rows
+        items["ent0"] = np.arange(len(part.chunk_start))
+        items["nent"] = 1
+        ent = np.stack([leaf_of, part.chunk_start - pb["row_start"][leaf_of]], axis=1)
         self.p_blocks, self.p_store = d_pb, pstore
-        self.p_ptr = torch.from_numpy(ptr).to(dev)
-        self.p_ent = torch.from_numpy(ent).to(dev)
+        self.p_items = torch.from_numpy(items.view(np.uint8).copy()).to(dev)
+        self.p_ent = torch.from_numpy(np.ascontiguousarray(ent.astype(np.int32))).to(dev)
         self._precond_built = True
 
-    def precond(self, R):
-        """M^{-1} R, M = the leaf diagonal blocks of Z."""
+    def precond_leaf(self, R):
+        """M^{-1} R in leaf order, M = the leaf diagonal blocks of Z."""
         torch = self.torch
         if not self._precond_built:
             self._build_precond()
@@ -274,10 +329,21 @@ class HOperator:
         Y = torch.empty_like(R)
         st = torch.cuda.current_stream().cuda_stream
         _lib.check(self.lib.mw_hm_apply(
-            self.n, self.cols_pool.data_ptr(), self.p_blocks.data_ptr(), None, 0,
-            self.p_ptr.data_ptr(), self.p_ent.data_ptr(), self.p_store.data_ptr(), R.data_ptr(),
-            s, s, None, Y.data_ptr(), s, st), "M^-1 R")
+            self.p_blocks.data_ptr(), self.p_store.data_ptr(), None, 0, None, None,
+            self.p_items.data_ptr(), len(self.part.chunk_start), self.p_ent.data_ptr(), None,
+            R.data_ptr(), s, s, None, Y.data_ptr(), s, st), "M^-1 R")
         return Y
+
+    def precond(self, R):
+        """M^{-1} R in basis order."""
+        perm, pos = self.pools["perm"], self.pools["pos"]
+        return self.precond_leaf(R[perm])[pos]
+
+    def solve(self, B, rtol):
+        """Block GMRES for H X = B (basis order), run in leaf order."""
+        perm, pos = self.pools["perm"], self.pools["pos"]
+        X, iters = block_gmres(self.apply_leaf, self.precond_leaf, B[perm].contiguous(), rtol)
+        return X[pos].contiguous(), iters
 
 
 def _qr_tall(W):
@@ -446,7 +512,7 @@ class HMatrixSolver(BSplineSolver):
         _lib.check(lib.mw_zscatter_cols(nb, ncols, D.col_max, D.col_idx.data_ptr(),
                                         D.col_val.data_ptr(), B.data_ptr(), B.stride(0), st),
                    "right-hand sides")
-        X, iters = block_gmres(H.apply, H.precond, B[:, :ncols], self.solve_tol)
+        X, iters = H.solve(B[:, :ncols], self.solve_tol)
         self._last_solve_iters = [iters] * D.n_ports
         X = X.contiguous()
         D.info.zero_()
