@@ -373,12 +373,13 @@ int mw_hm_fill_dense(const mw_mesh* mesh, const int32_t* wing_seg, const int32_t
  * with its pivoting, hop and stopping rules at tolerance tol.  Writes U^T, V
  * and rank into each block (rank -1: the factors would not compress, or the
  * rank outgrew cap; the caller then stores the block dense).  work: the used
- * flags (aux_off bytes per block).
+ * flags (aux_off bytes per block).  list[0 .. n_large) run in 512-thread
+ * workgroups (large blocks), the rest in 256-thread ones.
  */
 int mw_hm_aca(const mw_mesh* mesh, const int32_t* wing_seg, const int32_t* wing_slot, double k,
               double omega, double eps, double mu, const int32_t* rows_pool,
               const int32_t* cols_pool, mw_hblock* blocks, const int32_t* list, int32_t n_list,
-              double tol, void* store, void* work, void* stream);
+              int32_t n_large, double tol, void* store, void* work, void* stream);
 /*
  * mw_hm_apply: Y = H X for nrhs columns in leaf order (X, Y: n x nrhs, row
  * strides ldx, ldy), the batched product of the reference's HMatrix.matmat

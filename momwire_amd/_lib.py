@@ -130,7 +130,7 @@ SIGNATURES = {
     "mw_hm_fill_dense": (ctypes.c_int, [ctypes.POINTER(MwMesh), _P, _P, _D, _D, _D, _D, _P, _P,
                                         _P, _P, _I64, _I32, _P, _P]),
     "mw_hm_aca": (ctypes.c_int, [ctypes.POINTER(MwMesh), _P, _P, _D, _D, _D, _D, _P, _P, _P, _P,
-                                 _I32, _D, _P, _P, _P]),
+                                 _I32, _I32, _D, _P, _P, _P]),
     "mw_hm_apply": (ctypes.c_int, [_P, _P, _P, _I32, _P, _P, _P, _I32, _P, _P, _P, _I64, _I32,
                                    _P, _P, _I64, _P]),
     "mw_hm_invert": (ctypes.c_int, [_P, _P, _I32, _P, _P, _P, _P]),

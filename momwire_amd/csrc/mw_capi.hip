@@ -55,8 +55,8 @@ hipError_t mw_launch_hm_dense(const mw_mesh& M, const int32_t* wing_seg, const i
 hipError_t mw_launch_hm_aca(const mw_mesh& M, const int32_t* wing_seg, const int32_t* wing_slot,
                             double k, double omega, double eps, double mu,
                             const int32_t* rows_pool, const int32_t* cols_pool, mw_hblock* blk,
-                            const int32_t* list, int32_t n_list, double tol, void* store,
-                            void* flags, hipStream_t st);
+                            const int32_t* list, int32_t n_list, int32_t n_large, double tol,
+                            void* store, void* flags, hipStream_t st);
 hipError_t mw_launch_hm_apply(const mw_hblock* blk, const void* store, const mw_hm_titem* t_items,
                               int32_t n_t, const int32_t* t_rows, const int32_t* colpos,
                               const mw_hm_ritem* r_items, int32_t n_r, const int32_t* r_ent,
@@ -819,16 +819,17 @@ int mw_hm_fill_dense(const mw_mesh* mesh, const int32_t* wing_seg, const int32_t
 int mw_hm_aca(const mw_mesh* mesh, const int32_t* wing_seg, const int32_t* wing_slot, double k,
               double omega, double eps, double mu, const int32_t* rows_pool,
               const int32_t* cols_pool, mw_hblock* blocks, const int32_t* list, int32_t n_list,
-              double tol, void* store, void* work, void* stream) {
+              int32_t n_large, double tol, void* store, void* work, void* stream) {
   g_err[0] = 0;
   int rc = check_entry_args(mesh, wing_seg, wing_slot, omega, eps, mu);
   if (rc) return rc;
-  if (n_list < 0) return fail(MW_EINVAL, "%snegative block count");
+  if (n_list < 0 || n_large < 0 || n_large > n_list)
+    return fail(MW_EINVAL, "%sbad block counts");
   if (!(tol > 0.0)) return fail(MW_EINVAL, "%sACA tolerance must be positive");
   if (n_list > 0 && (!rows_pool || !cols_pool || !blocks || !list || !store || !work))
     return fail(MW_EINVAL, "%sNULL argument");
   return hip_status(mw_launch_hm_aca(*mesh, wing_seg, wing_slot, k, omega, eps, mu, rows_pool,
-                                     cols_pool, blocks, list, n_list, tol, store, work,
+                                     cols_pool, blocks, list, n_list, n_large, tol, store, work,
                                      (hipStream_t)stream),
                     "mw_hm_aca");
 }

@@ -170,9 +170,9 @@ __global__ __launch_bounds__(256) void k_hm_dense(EntryCtx C, const int32_t* __r
 // ---------------------------------------------------------------------------
 // Batched ACA
 // ---------------------------------------------------------------------------
-constexpr int AT = 256;  // threads per ACA workgroup
-
-// argmax of key over the workgroup (ties: smallest index); key < 0 never wins
+// argmax of key over the workgroup of AT threads (ties: smallest index);
+// key < 0 never wins
+template <int AT>
 MW_DEV void wg_argmax(double key, int idx, double* sv, int* si, double& best, int& bi) {
   const int tid = threadIdx.x;
   sv[tid] = key;
@@ -194,6 +194,7 @@ MW_DEV void wg_argmax(double key, int idx, double* sv, int* si, double& best, in
   __syncthreads();
 }
 
+template <int AT>
 MW_DEV double wg_sum(double v, double* sv) {
   const int tid = threadIdx.x;
   sv[tid] = v;
@@ -207,6 +208,7 @@ MW_DEV double wg_sum(double v, double* sv) {
   return r;
 }
 
+template <int AT>
 MW_DEV double2 wg_csum(double2 v, double* sv, double* sw) {
   const int tid = threadIdx.x;
   sv[tid] = v.x;
@@ -234,7 +236,10 @@ MW_DEV double cabs(double2 z) { return hypot(z.x, z.y); }
 // column among unused rows; stop when |u| |v| <= tol |U V|_F.  The factors
 // land in the block's store as U^T (cap x m) and V (cap x n).  rank on
 // return: the factors' rank, or -1 when the block is better stored dense
-// (rank (m + n) >= m n) or outgrew its capacity.
+// (rank (m + n) >= m n) or outgrew its capacity.  AT threads per workgroup:
+// 512 for the large blocks, whose sequential rank steps set the launch's
+// length, 256 for the rest.
+template <int AT>
 __global__ __launch_bounds__(AT) void k_hm_aca(EntryCtx C, const int32_t* __restrict__ rows_pool,
                                                const int32_t* __restrict__ cols_pool,
                                                mw_hblock* __restrict__ blk,
@@ -284,7 +289,7 @@ __global__ __launch_bounds__(AT) void k_hm_aca(EntryCtx C, const int32_t* __rest
     if (tid == 0) used_r[istar] = 1;
     double best;
     int jb;
-    wg_argmax(key, kidx, sv, si, best, jb);
+    wg_argmax<AT>(key, kidx, sv, si, best, jb);
     const double2 delta = jb < n ? vr[jb] : make_double2(0.0, 0.0);
     if (jb >= n || cabs(delta) < 1e-300 || best <= 0.0) {
       // the residual row vanished: the first unused row, else done
@@ -293,7 +298,7 @@ __global__ __launch_bounds__(AT) void k_hm_aca(EntryCtx C, const int32_t* __rest
         if (!used_r[i] && i < first) first = i;
       double dummy;
       int fi;
-      wg_argmax(first < m ? (double)(m - first) : -1.0, first, sv, si, dummy, fi);
+      wg_argmax<AT>(first < m ? (double)(m - first) : -1.0, first, sv, si, dummy, fi);
       if (dummy <= 0.0) break;
       istar = fi;
       continue;
@@ -313,8 +318,8 @@ __global__ __launch_bounds__(AT) void k_hm_aca(EntryCtx C, const int32_t* __rest
     for (int j = tid; j < n; j += AT) vn2 += vr[j].x * vr[j].x + vr[j].y * vr[j].y;
     __syncthreads();
     if (tid == 0) used_c[jb] = 1;
-    un2 = wg_sum(un2, sv);
-    vn2 = wg_sum(vn2, sv);
+    un2 = wg_sum<AT>(un2, sv);
+    vn2 = wg_sum<AT>(vn2, sv);
     double cross = 0.0;
     for (int kk = 0; kk < rank; ++kk) {
       double2 a = make_double2(0.0, 0.0), b = make_double2(0.0, 0.0);
@@ -328,8 +333,8 @@ __global__ __launch_bounds__(AT) void k_hm_aca(EntryCtx C, const int32_t* __rest
         b.x += x.x * y.x + x.y * y.y;
         b.y += x.x * y.y - x.y * y.x;
       }
-      a = wg_csum(a, sv, sw);
-      b = wg_csum(b, sv, sw);
+      a = wg_csum<AT>(a, sv, sw);
+      b = wg_csum<AT>(b, sv, sw);
       cross += a.x * b.x - a.y * b.y;
     }
     const double unvn = sqrt(un2) * sqrt(vn2);
@@ -349,7 +354,7 @@ __global__ __launch_bounds__(AT) void k_hm_aca(EntryCtx C, const int32_t* __rest
     __syncthreads();
     double bc;
     int ib;
-    wg_argmax(key, kidx, sv, si, bc, ib);
+    wg_argmax<AT>(key, kidx, sv, si, bc, ib);
     if (ib >= m) break;  // every row used
     istar = ib;
   }
@@ -651,16 +656,22 @@ hipError_t mw_launch_hm_dense(const mw_mesh& M, const int32_t* wing_seg, const i
   return hipGetLastError();
 }
 
+// list[0 .. n_large) go to 512-thread workgroups, the rest to 256-thread ones.
 hipError_t mw_launch_hm_aca(const mw_mesh& M, const int32_t* wing_seg, const int32_t* wing_slot,
                             double k, double omega, double eps, double mu,
                             const int32_t* rows_pool, const int32_t* cols_pool, mw_hblock* blk,
-                            const int32_t* list, int32_t n_list, double tol, void* store,
-                            void* flags, hipStream_t st) {
+                            const int32_t* list, int32_t n_list, int32_t n_large, double tol,
+                            void* store, void* flags, hipStream_t st) {
   if (n_list <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_hm_aca, dim3((unsigned)n_list), dim3(AT), 0, st,
-                     entry_ctx(M, wing_seg, wing_slot, k, omega, eps, mu), rows_pool, cols_pool,
-                     blk, list, tol, reinterpret_cast<double2*>(store),
-                     reinterpret_cast<uint8_t*>(flags));
+  const EntryCtx C = entry_ctx(M, wing_seg, wing_slot, k, omega, eps, mu);
+  if (n_large > 0)
+    hipLaunchKernelGGL(k_hm_aca<512>, dim3((unsigned)n_large), dim3(512), 0, st, C, rows_pool,
+                       cols_pool, blk, list, tol, reinterpret_cast<double2*>(store),
+                       reinterpret_cast<uint8_t*>(flags));
+  if (n_list > n_large)
+    hipLaunchKernelGGL(k_hm_aca<256>, dim3((unsigned)(n_list - n_large)), dim3(256), 0, st, C,
+                       rows_pool, cols_pool, blk, list + n_large, tol,
+                       reinterpret_cast<double2*>(store), reinterpret_cast<uint8_t*>(flags));
   return hipGetLastError();
 }
 

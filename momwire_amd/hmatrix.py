@@ -63,6 +63,36 @@ def _r_items(part, keep_block):
     return items, np.ascontiguousarray(ent.astype(np.int32))
 
 
+def _t_items(part, blocks, lr_ids, n):
+    """Phase-1 work of the product: the compressed blocks grouped by column
+    cluster, their (block, rank) rows listed group after group, and one item
+    per 64 rows of a group.  Returns (items TITEM, rows int32 [R][2])."""
+    if not len(lr_ids):
+        return np.zeros(0, dtype=TITEM), np.zeros((0, 2), dtype=np.int32)
+    keyc = part.col_start[lr_ids] * (n + 1) + blocks["n"][lr_ids]
+    order = np.argsort(keyc, kind="stable")
+    grp = lr_ids[order]
+    rk = blocks["rank"][grp].astype(np.int64)
+    # rows: every block's ranks 0 .. r-1, blocks in group order
+    first = np.concatenate([[0], np.cumsum(rk)[:-1]])
+    rows = np.stack([np.repeat(grp, rk), np.arange(rk.sum()) - np.repeat(first, rk)], axis=1)
+    # groups: runs of equal key; their row spans and their 64-row tiles
+    newg = np.concatenate([[True], np.diff(keyc[order]) != 0])
+    gstart_blk = np.flatnonzero(newg)
+    g_rows = np.add.reduceat(rk, gstart_blk)
+    g_base = np.concatenate([[0], np.cumsum(g_rows)[:-1]])
+    ntile = -(-g_rows // 64)
+    tg = np.repeat(np.arange(len(g_rows)), ntile)
+    t_in = np.arange(ntile.sum()) - np.repeat(np.concatenate([[0], np.cumsum(ntile)[:-1]]), ntile)
+    items = np.zeros(len(tg), dtype=TITEM)
+    items["row0"] = g_base[tg] + 64 * t_in
+    items["nrows"] = np.minimum(64, g_rows[tg] - 64 * t_in)
+    lead = grp[gstart_blk]
+    items["k"] = blocks["n"][lead][tg]
+    items["col_off"] = blocks["col_off"][lead][tg]
+    return items, np.ascontiguousarray(rows.astype(np.int32))
+
+
 class HOperator:
     """The compressed operator on the device: block table, index pools and
     store, and the work lists of its product (and of the preconditioner).
@@ -132,9 +162,21 @@ class HOperator:
         if far_ids.size:
             flags = torch.empty(int(flag_sz.sum()), dtype=torch.uint8, device=dev)
             fl = up(far_ids)
-            _lib.check(lib.mw_hm_aca(*args, pools["rows"].data_ptr(), pools["cols"].data_ptr(),
-                                     d_blocks.data_ptr(), fl.data_ptr(), int(far_ids.size),
-                                     float(tol), store.data_ptr(), flags.data_ptr(), st), "ACA")
+            # the large blocks (512-thread workgroups) on a second stream, so
+            # that they run beside the rest instead of ahead of them
+            n_large = int(((m64 + n64)[far_ids] >= 1024).sum())
+            aca = lambda lst, cnt, large, stream: _lib.check(lib.mw_hm_aca(  # noqa: E731
+                *args, pools["rows"].data_ptr(), pools["cols"].data_ptr(), d_blocks.data_ptr(),
+                lst, cnt, large, float(tol), store.data_ptr(), flags.data_ptr(), stream), "ACA")
+            if n_large:
+                cur = torch.cuda.current_stream()
+                side = torch.cuda.Stream()
+                side.wait_stream(cur)
+                aca(fl.data_ptr(), n_large, n_large, side.cuda_stream)
+            if far_ids.size > n_large:
+                aca(fl.data_ptr() + 4 * n_large, int(far_ids.size) - n_large, 0, st)
+            if n_large:
+                cur.wait_stream(side)
             got, = _to_host(d_blocks)
             blocks["rank"] = got.view(HBLOCK)["rank"]
             del flags
@@ -158,25 +200,10 @@ class HOperator:
         self.t_rows = int(blocks["rank"][lr_ids].sum()) if lr_ids.size else 0
         # phase 1 of the product: the V rows of blocks sharing a column
         # cluster, 64 at a time
-        titems, trows = [], []
-        base = 0
-        if lr_ids.size:
-            keyc = part.col_start[lr_ids] * (n + 1) + blocks["n"][lr_ids]
-            order = np.argsort(keyc, kind="stable")
-            grp = lr_ids[order]
-            bounds = np.flatnonzero(np.diff(keyc[order])) + 1
-            for g in np.split(grp, bounds):
-                rk = blocks["rank"][g]
-                rows = np.stack([np.repeat(g, rk), np.concatenate([np.arange(r) for r in rk])],
-                                axis=1)
-                trows.append(rows)
-                for r0 in range(0, len(rows), 64):
-                    titems.append((base + r0, min(64, len(rows) - r0), int(blocks["n"][g[0]]), 0,
-                                   int(blocks["col_off"][g[0]])))
-                base += len(rows)
-        self.t_items = up(np.array(titems, dtype=TITEM).view(np.uint8)) if titems else None
-        self.n_t = len(titems)
-        self.t_rowlist = up(np.concatenate(trows).astype(np.int32)) if trows else None
+        ti, tr = _t_items(part, blocks, lr_ids, n)
+        self.t_items = up(ti.view(np.uint8)) if len(ti) else None
+        self.n_t = len(ti)
+        self.t_rowlist = up(tr) if len(ti) else None
         # phase 2: every row chunk over the blocks that hold it (compressed
         # blocks of rank 0 add nothing)
         keep = ~(far & (blocks["rank"] == 0))
@@ -193,11 +220,10 @@ class HOperator:
         so near blocks read the same either way)."""
         if not len(ids):
             return
-        items = []
-        for b in ids:
-            ne = int(blocks["m"][b]) * int(blocks["n"][b])
-            items.append((np.int64(b) << 32) | np.arange(0, ne, 256, dtype=np.int64))
-        items = np.concatenate(items)
+        ids = np.asarray(ids, dtype=np.int64)
+        nit = -(-(blocks["m"][ids].astype(np.int64) * blocks["n"][ids]) // 256)
+        first = np.arange(nit.sum()) - np.repeat(np.concatenate([[0], np.cumsum(nit)[:-1]]), nit)
+        items = (np.repeat(ids, nit) << 32) | (256 * first)
         d_items = self.torch.from_numpy(items).to(store.device)
         _lib.check(self.lib.mw_hm_fill_dense(*args, self.pools["lrows"].data_ptr(),
                                              self.pools["lcols"].data_ptr(), d_blocks.data_ptr(),
@@ -273,10 +299,9 @@ class HOperator:
         torch = self.torch
         part, blocks = self.part, self.blocks
         dev = self.store.device
-        diag = [b for b in range(len(blocks))
-                if part.blocks[b, 4] == 0 and part.row_start[b] == part.col_start[b]
-                and blocks["m"][b] == blocks["n"][b]]
-        diag.sort(key=lambda b: part.row_start[b])  # leaf order
+        diag = np.flatnonzero((part.blocks[:, 4] == 0) & (part.row_start == part.col_start)
+                              & (blocks["m"] == blocks["n"]))
+        diag = diag[np.argsort(part.row_start[diag], kind="stable")]  # leaf order
         pb = np.zeros(len(diag), dtype=HBLOCK)
         pb["m"] = blocks["m"][diag]
         pb["n"] = blocks["n"][diag]
@@ -286,14 +311,12 @@ class HOperator:
         sz = pb["m"].astype(np.int64) * pb["n"]
         pb["data_off"] = np.concatenate([[0], np.cumsum(sz)[:-1]])
         pb["aux_off"] = np.concatenate([[0], np.cumsum(2 * sz)[:-1]])
-        cover = np.zeros(self.n, dtype=np.int32)
-        for b in diag:
-            cover[part.row_start[b]:part.row_start[b] + blocks["m"][b]] += 1
-        if not (cover == 1).all():
+        ends = pb["row_start"] + pb["m"]
+        if (pb["row_start"][0] != 0 or ends[-1] != self.n
+                or not np.array_equal(pb["row_start"][1:], ends[:-1])):
             raise RuntimeError("the leaf diagonal blocks do not cover every basis once")
-        src = np.concatenate([np.arange(int(blocks["data_off"][b]),
-                                        int(blocks["data_off"][b]) + int(sz[i]))
-                              for i, b in enumerate(diag)])
+        src = (np.repeat(blocks["data_off"][diag] - pb["data_off"], sz)
+               + np.arange(int(sz.sum())))
         pstore = self.store[torch.from_numpy(src).to(dev)].contiguous()
         work = torch.empty(int(2 * sz.sum()), dtype=torch.complex128, device=dev)
         info = torch.zeros(len(diag), dtype=torch.int32, device=dev)
