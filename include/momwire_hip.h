@@ -423,9 +423,16 @@ int mw_zsmall_solve(int32_t ns, int32_t nt, void* S, int64_t lds, void* T,
  * 32-column panel of the largest problem), so the whole sweep can be
  * captured as one HIP graph.  No counterpart in the reference, which sweeps
  * one structure at a time (bspline.py:4181-4439); per problem it computes
- * what compute_port_solution computes.  gemm_variant: 0 = 64x32 tiles,
- * 1 = 128x64 tiles for the batched updates (chosen by the autotuner).
- * pivot_bound: the plan's own threshold-pivoting multiplier bound, meant as
+ * what compute_port_solution computes.  gemm_variant (chosen by the
+ * autotuner): 0 = per-step launches with 64x32 update tiles, 1 = the same
+ * with 128x64 tiles; 2..5 = after the fill, ONE persistent launch in which a
+ * workgroup takes one problem at a time (largest first) through its whole
+ * solve -- 2: 64x32 tiles, 3: 128x64 tiles, two workgroups per compute unit;
+ * 4, 5: the same with one per compute unit.  No workgroup of that launch
+ * waits for another.  Its cancel flag is read by the kernel when it is
+ * device-readable (mw_cancel_create): problems claimed after it is set are
+ * left unsolved with info[0] = -1.  An ordinary host flag is read once,
+ * before the launches.  pivot_bound: the plan's own threshold-pivoting multiplier bound, meant as
  * in mw_set_pivot_threshold (<= 0 pivots every panel).  32 (tau = 1/32)
  * is the Python default: the unpivoted multipliers of Galerkin wire
  * matrices stay in the tens, and a panel that falls back to the tournament

@@ -79,6 +79,9 @@ int mw_sweep_lu_run(const mw_sweep_prob* dP, const mw_ports* dports, const mw_sw
                     hipError_t* herr);
 void mw_sweep_work(const mw_sweep_prob* hp, int nprob, int gemm_variant,
                    std::vector<uint32_t>& items, std::vector<int64_t>& off);
+int mw_sweep_wg_run(const mw_sweep_prob* dP, const mw_ports* dports, int nprob, int big,
+                    int per_cu, double lbound, int32_t* d_next, const volatile int32_t* cancel,
+                    hipStream_t st, hipError_t* herr);
 hipError_t mw_launch_add_sparse(int64_t nnz, const int32_t* rows, const int32_t* cols,
                                 const void* vals, void* Z, int64_t ldz, hipStream_t st);
 hipError_t mw_launch_small_solve(int ns, int nt, void* S, int64_t lds, void* T, int64_t ldt,
@@ -197,6 +200,22 @@ int device_cancel(const volatile int32_t* cancel, const volatile int32_t** dev) 
   }
   return fail(MW_EINVAL, "%scancel flag is not device-accessible memory (allocate it with "
                          "mw_cancel_create)");
+}
+
+// The address a kernel may read `cancel` through (mapped pinned or device
+// memory), or NULL for an ordinary host address; sets no error.
+const volatile int32_t* device_view(const volatile int32_t* cancel) {
+  if (!cancel) return nullptr;
+  hipPointerAttribute_t a;
+  memset(&a, 0, sizeof a);
+  if (hipPointerGetAttributes(&a, (const void*)cancel) != hipSuccess) {
+    (void)hipGetLastError();
+    return nullptr;
+  }
+  if (a.type == hipMemoryTypeHost && a.devicePointer)
+    return (const volatile int32_t*)a.devicePointer;
+  if (a.type == hipMemoryTypeDevice || a.type == hipMemoryTypeManaged) return cancel;
+  return nullptr;
 }
 }  // namespace
 
@@ -580,6 +599,7 @@ struct mw_sweep_plan {
   int32_t* d_item_pair = nullptr;
   int32_t* d_se_items = nullptr;   // (problem, 256-thread block) of the same-edge tables
   uint32_t* d_work = nullptr;      // per-step work lists of the factorisation
+  int32_t* d_next = nullptr;       // claim counter of the persistent form
   std::vector<int64_t> work_off;
   std::vector<mw_sweep_prob> hp;
 };
@@ -595,6 +615,7 @@ static void plan_free(mw_sweep_plan* p) {
   (void)hipFree(p->d_item_pair);
   (void)hipFree(p->d_se_items);
   (void)hipFree(p->d_work);
+  (void)hipFree(p->d_next);
   delete p;
 }
 
@@ -665,6 +686,8 @@ int mw_sweep_plan_create(int32_t n_mesh, const mw_mesh* meshes, const mw_ports* 
   if (n_mesh <= 0 || nprob <= 0 || !meshes || !ports || !probs)
     return fail(MW_EINVAL, "%sempty sweep");
   if (nprob > 65536) return fail(MW_ERANGE, "%smore than 65536 problems in one sweep");
+  if (gemm_variant < 0 || gemm_variant > 5)
+    return fail(MW_EINVAL, "%sgemm_variant %lld: expected 0..5", "", (long long)gemm_variant);
   for (int i = 0; i < n_mesh; ++i) {
     int rc = check_mesh(&meshes[i]);
     if (rc) return rc;
@@ -696,7 +719,7 @@ int mw_sweep_plan_create(int32_t n_mesh, const mw_mesh* meshes, const mw_ports* 
   std::stable_sort(p->hp.begin(), p->hp.end(),
                    [](const mw_sweep_prob& a, const mw_sweep_prob& b) { return a.n > b.n; });
   std::vector<uint32_t> work;
-  mw_sweep_work(p->hp.data(), nprob, gemm_variant, work, p->work_off);
+  if (gemm_variant < 2) mw_sweep_work(p->hp.data(), nprob, gemm_variant, work, p->work_off);
   std::vector<mw_kgroup> groups;
   std::vector<int32_t> gprob, ig, iq;
   sweep_kgroups(p->hp, groups, gprob);
@@ -743,6 +766,7 @@ int mw_sweep_plan_create(int32_t n_mesh, const mw_mesh* meshes, const mw_ports* 
       (!se_items.empty() &&
        (e = hipMemcpy(p->d_se_items, se_items.data(), sizeof(int32_t) * se_items.size(), hipMemcpyHostToDevice)) != hipSuccess) ||
       (e = hipMalloc(&p->d_work, sizeof(uint32_t) * std::max<size_t>(work.size(), 1))) != hipSuccess ||
+      (e = hipMalloc(&p->d_next, sizeof(int32_t) * 2)) != hipSuccess ||
       (!work.empty() &&
        (e = hipMemcpy(p->d_work, work.data(), sizeof(uint32_t) * work.size(), hipMemcpyHostToDevice)) != hipSuccess)) {
     plan_free(p);
@@ -757,11 +781,21 @@ int mw_sweep_run(mw_sweep_plan* p, double eps, double mu, const volatile int32_t
   g_err[0] = 0;
   if (!p) return fail(MW_EINVAL, "%splan is NULL");
   hipStream_t st = (hipStream_t)stream;
+  if (cancel && *cancel) return fail(MW_EABORT, "%ssweep cancelled");
   hipError_t herr = sweep_fill(p, eps, mu, st);
   if (herr != hipSuccess) return hip_status(herr, "sweep fill");
-  int rc = mw_sweep_lu_run(p->dP, p->dports, p->hp.data(), p->nprob, p->max_ports, p->max_con,
-                           p->max_scatter, p->gemm_variant, p->pivot_bound, p->d_work,
-                           p->work_off.data(), cancel, st, &herr);
+  int rc;
+  if (p->gemm_variant >= 2) {
+    // one launch: the kernel reads the flag itself when it can (mapped or
+    // device memory); an ordinary host flag is only read here, before it
+    const int v = p->gemm_variant - 2;
+    rc = mw_sweep_wg_run(p->dP, p->dports, p->nprob, v & 1, (v & 2) ? 1 : 2, p->pivot_bound,
+                         p->d_next, device_view(cancel), st, &herr);
+  } else {
+    rc = mw_sweep_lu_run(p->dP, p->dports, p->hp.data(), p->nprob, p->max_ports, p->max_con,
+                         p->max_scatter, p->gemm_variant, p->pivot_bound, p->d_work,
+                         p->work_off.data(), cancel, st, &herr);
+  }
   if (rc == MW_EHIP) return hip_status(herr, "sweep solve");
   if (rc == MW_EABORT) return fail(MW_EABORT, "%ssweep cancelled");
   return rc;

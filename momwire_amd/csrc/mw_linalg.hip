@@ -1192,16 +1192,28 @@ struct TourWs {
   int32_t* counter;  // [nodes], zero on entry, reset by the root
 };
 
+// LDS of the tournament fallback (from the caller, so that a kernel running
+// several phases can share one buffer between them).
+template <typename CT>
+struct PivotLds {
+  TourLds<CT> tour;
+  PanelLds<CT> panel;
+  int last;
+  int32_t pos[2 * NB], val[2 * NB];
+  int nt;
+};
+
 template <typename T>
-MW_DEV void panel_pivot_body(int node0, typename cplx_t<T>::type* A, int64_t lda, int64_t n,
+MW_DEV void panel_pivot_body(PivotLds<typename cplx_t<T>::type>& SL, int node0,
+                             typename cplx_t<T>::type* A, int64_t lda, int64_t n,
                              int64_t c_lo, int64_t nc, int64_t k0, int bw, int nleaf, TourWs ws,
                              const int32_t* flag, typename cplx_t<T>::type* lu11,
                              typename cplx_t<T>::type* inv, int32_t* ipiv, int32_t* info) {
   typedef typename cplx_t<T>::type CT;
   if (*flag == 0) return;
-  __shared__ TourLds<CT> L;
-  __shared__ PanelLds<CT> P;
-  __shared__ int last_s;
+  TourLds<CT>& L = SL.tour;
+  PanelLds<CT>& P = SL.panel;
+  int& last_s = SL.last;
   const int tid = threadIdx.x;
   int node = node0, nodes = nleaf, lvl_off = 0;
   if (tid == 0) {
@@ -1280,8 +1292,9 @@ MW_DEV void panel_pivot_body(int node0, typename cplx_t<T>::type* A, int64_t lda
   }
   // final placement of the rows this panel moves: position pos_s[t] receives
   // the row that stood at val_s[t] (a permutation of at most 2 bw rows)
-  __shared__ int32_t pos_s[2 * NB], val_s[2 * NB];
-  __shared__ int nt_s;
+  int32_t* pos_s = SL.pos;
+  int32_t* val_s = SL.val;
+  int& nt_s = SL.nt;
   if (tid == 0) {
     int nt = 0;
     for (int i = 0; i < steps; ++i) {
@@ -1349,19 +1362,19 @@ __global__ __launch_bounds__(256) void k_panel_pivot(typename cplx_t<T>::type* A
                                                      typename cplx_t<T>::type* lu11,
                                                      typename cplx_t<T>::type* inv,
                                                      int32_t* ipiv, int32_t* info) {
-  panel_pivot_body<T>(blockIdx.x, A, lda, n, c_lo, nc, k0, bw, nleaf, ws, flag, lu11, inv, ipiv,
-                      info);
+  __shared__ PivotLds<typename cplx_t<T>::type> S;
+  panel_pivot_body<T>(S, blockIdx.x, A, lda, n, c_lo, nc, k0, bw, nleaf, ws, flag, lu11, inv,
+                      ipiv, info);
 }
 
 // After the interchanges: A11 <- L11\U11, L21 <- A21 U11^{-1}, U12 <- L11^{-1} A12.
 // Blocks [0, nrb): 64-row blocks of L21; [nrb, nrb+ncb): 64-column blocks of
 // U12; last: A11.
 template <typename T>
-MW_DEV void panel_apply_body(int b, int64_t n, int64_t nc, typename cplx_t<T>::type* A,
-                             int64_t lda, int64_t k0, int bw,
+MW_DEV void panel_apply_body(BlkLds<T>& L, int b, int64_t n, int64_t nc,
+                             typename cplx_t<T>::type* A, int64_t lda, int64_t k0, int bw,
                              const typename cplx_t<T>::type* inv,
                              const typename cplx_t<T>::type* lu11, int nrb, int ncb) {
-  __shared__ BlkLds<T> L;
   const int tid = threadIdx.x;
   if (b < nrb) {
     const int64_t r0 = k0 + bw + (int64_t)b * AW;
@@ -1385,7 +1398,8 @@ __global__ __launch_bounds__(256) void k_panel_apply(int64_t n, int64_t nc, type
                                                      const typename cplx_t<T>::type* inv,
                                                      const typename cplx_t<T>::type* lu11,
                                                      int nrb, int ncb) {
-  panel_apply_body<T>(blockIdx.x, n, nc, A, lda, k0, bw, inv, lu11, nrb, ncb);
+  __shared__ BlkLds<T> L;
+  panel_apply_body<T>(L, blockIdx.x, n, nc, A, lda, k0, bw, inv, lu11, nrb, ncb);
 }
 
 // ---------------------------------------------------------------------------
@@ -1649,10 +1663,9 @@ __global__ __launch_bounds__(256, 2) void k_lookahead(int64_t n, int64_t nc,
 // Solve blocks on AW right-hand-side columns per workgroup:
 // lower != 0: X_p <- L11^{-1} X_p; lower == 0: X_p <- U11^{-1} X_p.
 template <typename T>
-MW_DEV void solve_apply_body(int bx, int64_t nrhs, const typename cplx_t<T>::type* inv,
-                             int64_t k0, int bw, typename cplx_t<T>::type* X, int64_t ldx,
-                             int lower) {
-  __shared__ BlkLds<T> L;
+MW_DEV void solve_apply_body(BlkLds<T>& L, int bx, int64_t nrhs,
+                             const typename cplx_t<T>::type* inv, int64_t k0, int bw,
+                             typename cplx_t<T>::type* X, int64_t ldx, int lower) {
   const int64_t c0 = (int64_t)bx * AW;
   const int cols = (int)((nrhs - c0) < AW ? (nrhs - c0) : AW);
   block_product<T, 32, 64>(L.As, &L.Bs[0][0], 65, lower ? inv : inv + NB * NB, NB,
@@ -1664,7 +1677,8 @@ __global__ __launch_bounds__(256) void k_solve_apply(int64_t nrhs, const typenam
                                                      int64_t k0, int bw,
                                                      typename cplx_t<T>::type* X, int64_t ldx,
                                                      int lower) {
-  solve_apply_body<T>(blockIdx.x, nrhs, inv, k0, bw, X, ldx, lower);
+  __shared__ BlkLds<T> L;
+  solve_apply_body<T>(L, blockIdx.x, nrhs, inv, k0, bw, X, ldx, lower);
 }
 
 // One step of the back substitution for a single panel p (an odd last panel
@@ -2087,10 +2101,15 @@ __global__ void k_c2z(int64_t rows, int64_t cols, const float2* src, int64_t lds
 
 // T <- S^{-1} T, one workgroup, Gauss-Jordan with partial pivoting in LDS
 // (ns <= 32, nt <= 64).
-MW_DEV void small_solve_body(int ns, int nt, double2* S, int64_t lds, double2* Tm, int64_t ldt,
-                             int32_t* info) {
-  __shared__ double2 M[32][97];
-  __shared__ int piv_s;
+struct SmallLds {
+  double2 M[32][97];
+  int piv;
+};
+
+MW_DEV void small_solve_body(SmallLds& SL, int ns, int nt, double2* S, int64_t lds, double2* Tm,
+                             int64_t ldt, int32_t* info) {
+  double2 (*M)[97] = SL.M;
+  int& piv_s = SL.piv;
   const int tid = threadIdx.x, lane = tid & 63;
   const int w = ns + nt;
   for (int e = tid; e < ns * w; e += blockDim.x) {
@@ -2138,7 +2157,8 @@ MW_DEV void small_solve_body(int ns, int nt, double2* S, int64_t lds, double2* T
 
 __global__ __launch_bounds__(256) void k_small_solve(int ns, int nt, double2* S, int64_t lds,
                                                      double2* Tm, int64_t ldt, int32_t* info) {
-  small_solve_body(ns, nt, S, lds, Tm, ldt, info);
+  __shared__ SmallLds SL;
+  small_solve_body(SL, ns, nt, S, lds, Tm, ldt, info);
 }
 
 // ---------------------------------------------------------------------------
@@ -2270,7 +2290,8 @@ __global__ __launch_bounds__(256) void kb_panel_pivot(const mw_sweep_prob* P, in
   const int nleaf = (int)((p.n - k0 + TROWS - 1) / TROWS);
   const int leaf = (int)(it & 0xffff);
   if (leaf >= nleaf) return;
-  panel_pivot_body<double>(leaf, (double2*)p.A, p.lda, p.n, k0, p.n + p.ncols, k0, bw, nleaf,
+  __shared__ PivotLds<double2> S;
+  panel_pivot_body<double>(S, leaf, (double2*)p.A, p.lda, p.n, k0, p.n + p.ncols, k0, bw, nleaf,
                            ws_tour(p.work, p.n), flag, (double2*)p.work, ws_inv(p.work, k0),
                            p.ipiv, p.info);
 }
@@ -2286,7 +2307,8 @@ __global__ __launch_bounds__(256) void kb_panel_apply(const mw_sweep_prob* P, in
   const int ncb = (int)((nc - k0 - bw + AW - 1) / AW);
   const int b = (int)(it & 0xffff);
   if (b > nrb + ncb) return;
-  panel_apply_body<double>(b, p.n, nc, (double2*)p.A, p.lda, k0, bw, ws_inv(p.work, k0),
+  __shared__ BlkLds<double> L;
+  panel_apply_body<double>(L, b, p.n, nc, (double2*)p.A, p.lda, k0, bw, ws_inv(p.work, k0),
                            (const double2*)p.work, nrb, ncb);
 }
 
@@ -2336,7 +2358,8 @@ __global__ __launch_bounds__(256) void kb_bwd_apply(const mw_sweep_prob* P, int6
   const mw_sweep_prob& p = P[blockIdx.y];
   if (k0 >= p.n || (int64_t)blockIdx.x * AW >= p.ncols) return;
   const int bw = (int)((p.n - k0) < NB ? (p.n - k0) : NB);
-  solve_apply_body<double>(blockIdx.x, p.ncols, ws_inv(p.work, k0), k0, bw,
+  __shared__ BlkLds<double> L;
+  solve_apply_body<double>(L, blockIdx.x, p.ncols, ws_inv(p.work, k0), k0, bw,
                            reinterpret_cast<double2*>(p.A) + p.n, p.lda, 0);
 }
 
@@ -2367,7 +2390,8 @@ __global__ __launch_bounds__(256) void kb_small_solve(const mw_sweep_prob* P, co
   const mw_sweep_prob& p = P[blockIdx.x];
   const mw_ports& q = ports[p.mesh];
   if (q.n_con == 0) return;
-  small_solve_body(q.n_con, q.n_ports, (double2*)p.S, q.n_con, (double2*)p.T, q.n_ports,
+  __shared__ SmallLds SL;
+  small_solve_body(SL, q.n_con, q.n_ports, (double2*)p.S, q.n_con, (double2*)p.T, q.n_ports,
                    p.info + 1);
 }
 
@@ -2394,6 +2418,267 @@ __global__ void kb_readout(const mw_sweep_prob* P, const mw_ports* ports) {
   sparse_rows_body(i, j, q.n_ports, q.port_max, q.port_idx, q.port_val,
                    reinterpret_cast<const double2*>(p.A) + p.n, p.lda, q.n_ports,
                    reinterpret_cast<double2*>(p.Y), q.n_ports);
+}
+
+// ---------------------------------------------------------------------------
+// Persistent batched solve: one workgroup takes one problem at a time and
+// runs its whole port solve -- right-hand sides, the threshold-pivoted
+// factorisation with its tournament fallback, the back substitution, the KCL
+// Schur step and the Y readout.  Problems are sorted largest first and
+// claimed through one counter, so the largest start first and the small ones
+// fill in behind them.  No workgroup ever waits for another: the launch needs
+// no co-residency and no cross-workgroup synchronisation.  Within a
+// workgroup the phases are ordered by barriers (one phase's global writes are
+// read by other threads of the same workgroup in the next).
+//
+// Each phase is a separate (not inlined) function, so each gets the whole
+// register budget: inlined into one kernel body, the phases' uniform values
+// overflowed the scalar registers into vector lanes and spilled.  They share
+// one LDS buffer declared at namespace scope, which every phase addresses
+// directly (LDS instructions, not generic ones).
+// ---------------------------------------------------------------------------
+constexpr size_t kWgLds =
+    sizeof(BlkLds<double>) > sizeof(PivotLds<double2>) ? sizeof(BlkLds<double>)
+                                                       : sizeof(PivotLds<double2>);
+static_assert(kWgLds >= sizeof(PanelLds<double2>) && kWgLds >= sizeof(SmallLds) &&
+                  kWgLds >= sizeof(K32Lds<double>) &&
+                  kWgLds >= sizeof(GemmLds<double>) && kWgLds >= sizeof(GemmSmallLds<double>),
+              "the shared buffer holds every phase's LDS");
+__shared__ __align__(16) unsigned char g_wg_lds[kWgLds];
+__shared__ int g_wg_flag;
+
+#define MW_PHASE __device__ __attribute__((noinline))
+
+// Workspace flags, tournament counters and info; the right-hand sides.
+MW_PHASE void wg_setup(const mw_sweep_prob* pp, const mw_ports* qq) {
+  const mw_sweep_prob& p = *pp;
+  const mw_ports& q = *qq;
+  const int tid = threadIdx.x;
+  const int64_t n = p.n, ncols = p.ncols, lda = p.lda;
+  double2* X = reinterpret_cast<double2*>(p.A) + n;
+  int32_t* flags = ws_flags(p.work, n);
+  const int64_t cnt = (n + NB - 1) / NB + d_tour_nodes(n);
+  for (int64_t e = tid; e < cnt; e += blockDim.x) flags[e] = 0;
+  if (tid < 2) p.info[tid] = 0;
+  for (int64_t e = tid; e < n * ncols; e += blockDim.x)
+    X[(e / ncols) * lda + e % ncols] = make_double2(0.0, 0.0);
+  __syncthreads();
+  for (int64_t e = tid; e < ncols * q.col_max; e += blockDim.x) {
+    const int32_t r = q.col_idx[e];
+    if (r >= 0) X[(int64_t)r * lda + e / q.col_max] = make_double2(q.col_val[e], 0.0);
+  }
+  __syncthreads();
+}
+
+// A11 of the panel at k0 without row exchanges, both inverses, A11's own
+// multipliers checked, then L21 = A21 U11^{-1} checked against the bound
+// (nothing written into A).  Returns whether the panel needs the fallback.
+MW_PHASE int wg_factor_check(const mw_sweep_prob* pp, int64_t k0, double lbound) {
+  const mw_sweep_prob& p = *pp;
+  const int64_t n = p.n, lda = p.lda;
+  const int bw = (int)((n - k0) < NB ? (n - k0) : NB);
+  double2* A = reinterpret_cast<double2*>(p.A);
+  double2* inv = ws_inv(p.work, k0);
+  int32_t* flag = ws_flags(p.work, n) + k0 / NB;
+  PanelLds<double2>& P = *reinterpret_cast<PanelLds<double2>*>(g_wg_lds);
+  panel_nopiv_body<double>(P, 0, n, A, lda, k0, bw, reinterpret_cast<double2*>(p.work), inv,
+                           p.ipiv, flag, lbound);
+  __syncthreads();
+  const int bad = P.bad;
+  __syncthreads();
+  if (bad) return 1;
+  BlkLds<double>& B = *reinterpret_cast<BlkLds<double>*>(g_wg_lds);
+  const int nrb = (int)((n - k0 - bw + AW - 1) / AW);
+  bool over = false;
+  for (int b = 0; b < nrb; ++b) {
+    const int64_t r0 = k0 + bw + (int64_t)b * AW;
+    const int rows = (int)((n - r0) < AW ? (n - r0) : AW);
+    over |= block_product<double, 64, 32>(B.As, &B.Bs[0][0], 65, inv + NB * NB, NB,
+                                          A + r0 * lda + k0, lda, rows, bw, true, lbound);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) g_wg_flag = 0;
+  __syncthreads();
+  if (over) g_wg_flag = 1;
+  __syncthreads();
+  return g_wg_flag;
+}
+
+// Tournament fallback of the panel at k0: the leaves one after the other
+// (the last child of each merge group, as the arrival counters count, merges).
+MW_PHASE void wg_pivot(const mw_sweep_prob* pp, int64_t k0) {
+  const mw_sweep_prob& p = *pp;
+  const int64_t n = p.n, nc = n + p.ncols;
+  const int bw = (int)((n - k0) < NB ? (n - k0) : NB);
+  int32_t* flag = ws_flags(p.work, n) + k0 / NB;
+  if (threadIdx.x == 0) *flag = 1;
+  __syncthreads();
+  const int nleaf = (int)((n - k0 + TROWS - 1) / TROWS);
+  PivotLds<double2>& S = *reinterpret_cast<PivotLds<double2>*>(g_wg_lds);
+  for (int leaf = 0; leaf < nleaf; ++leaf) {
+    panel_pivot_body<double>(S, leaf, reinterpret_cast<double2*>(p.A), p.lda, n, k0, nc, k0, bw,
+                             nleaf, ws_tour(p.work, n), flag, reinterpret_cast<double2*>(p.work),
+                             ws_inv(p.work, k0), p.ipiv, p.info);
+    __syncthreads();
+  }
+}
+
+// L21, U12 (right-hand sides included) and A11 of the panel at k0 into A.
+MW_PHASE void wg_apply(const mw_sweep_prob* pp, int64_t k0) {
+  const mw_sweep_prob& p = *pp;
+  const int64_t n = p.n, nc = n + p.ncols;
+  const int bw = (int)((n - k0) < NB ? (n - k0) : NB);
+  const int nrb = (int)((n - k0 - bw + AW - 1) / AW);
+  const int ncb = (int)((nc - k0 - bw + AW - 1) / AW);
+  BlkLds<double>& B = *reinterpret_cast<BlkLds<double>*>(g_wg_lds);
+  for (int b = 0; b <= nrb + ncb; ++b) {
+    panel_apply_body<double>(B, b, n, nc, reinterpret_cast<double2*>(p.A), p.lda, k0, bw,
+                             ws_inv(p.work, k0), reinterpret_cast<const double2*>(p.work), nrb,
+                             ncb);
+    __syncthreads();
+  }
+}
+
+// C -= A B over an m x ncol block with K = kdim <= 32, one tile after the
+// other.
+template <bool BIG>
+MW_PHASE void wg_gemm(int64_t m, int64_t ncol, int kdim, const double2* a, const double2* b,
+                      double2* c, int64_t ld) {
+  if (!BIG) {
+    // 64 x 32 tiles walked in order, the next tile's operands loaded while
+    // the current one's MFMAs run
+    const int gxn = (int)((ncol + NB - 1) / NB);
+    const int ntiles = gxn * (int)((m + 63) / 64);
+    gemm_k32_tiles<double>(*reinterpret_cast<K32Lds<double>*>(g_wg_lds), 0, 1, ntiles, gxn, m,
+                           ncol, kdim, a, ld, b, ld, c, ld);
+    __syncthreads();
+    return;
+  }
+  const int tm = BIG ? GBM : SBM, tn = BIG ? GBN : SBN;
+  const int ny = (int)((m + tm - 1) / tm), nx = (int)((ncol + tn - 1) / tn);
+  for (int t = 0; t < nx * ny; ++t) {
+    if (BIG)
+      gemm_big_tile<double, false>(*reinterpret_cast<GemmLds<double>*>(g_wg_lds), t % nx, t / nx,
+                                   m, ncol, kdim, a, ld, b, ld, c, ld);
+    else
+      gemm_small_tile<double, false>(*reinterpret_cast<GemmSmallLds<double>*>(g_wg_lds), t % nx,
+                                     t / nx, m, ncol, kdim, a, ld, b, ld, c, ld);
+    __syncthreads();  // LDS free for the next tile
+  }
+}
+
+// X_k <- U_kk^{-1} X_k for the panel at k0 (all right-hand-side columns).
+MW_PHASE void wg_back_step(const mw_sweep_prob* pp, int64_t k0) {
+  const mw_sweep_prob& p = *pp;
+  const int64_t n = p.n, ncols = p.ncols;
+  const int bw = (int)((n - k0) < NB ? (n - k0) : NB);
+  BlkLds<double>& B = *reinterpret_cast<BlkLds<double>*>(g_wg_lds);
+  for (int bx = 0; bx < (int)((ncols + AW - 1) / AW); ++bx) {
+    solve_apply_body<double>(B, bx, ncols, ws_inv(p.work, k0), k0, bw,
+                             reinterpret_cast<double2*>(p.A) + n, p.lda, 0);
+    __syncthreads();
+  }
+}
+
+// KCL Schur step (S = A Xc, T = A W; Lambda = S^{-1} T; W -= Xc Lambda) and
+// the Y readout.
+MW_PHASE void wg_finish(const mw_sweep_prob* pp, const mw_ports* qq) {
+  const mw_sweep_prob& p = *pp;
+  const mw_ports& q = *qq;
+  const int tid = threadIdx.x;
+  const int64_t n = p.n, lda = p.lda;
+  double2* X = reinterpret_cast<double2*>(p.A) + n;
+  if (q.n_con > 0) {
+    const int nsc = q.n_con + q.n_ports;
+    for (int e = tid; e < q.n_con * nsc; e += blockDim.x) {
+      const int i = e / nsc, j = e % nsc;
+      const int col = j < q.n_con ? q.n_ports + j : j - q.n_con;
+      double sr = 0.0, si = 0.0;
+      for (int t = 0; t < q.con_max; ++t) {
+        const int32_t r = q.con_idx[i * q.con_max + t];
+        if (r < 0) continue;
+        const double v = q.con_val[i * q.con_max + t];
+        const double2 x = X[(int64_t)r * lda + col];
+        sr += v * x.x;
+        si += v * x.y;
+      }
+      double2* out = j < q.n_con ? reinterpret_cast<double2*>(p.S) + i * q.n_con + j
+                                 : reinterpret_cast<double2*>(p.T) + i * q.n_ports + (j - q.n_con);
+      *out = make_double2(sr, si);
+    }
+    __syncthreads();
+    small_solve_body(*reinterpret_cast<SmallLds*>(g_wg_lds), q.n_con, q.n_ports,
+                     reinterpret_cast<double2*>(p.S), q.n_con, reinterpret_cast<double2*>(p.T),
+                     q.n_ports, p.info + 1);
+    __syncthreads();
+    const double2* Lm = reinterpret_cast<const double2*>(p.T);
+    for (int64_t r = tid; r < n; r += blockDim.x) {
+      double2* Xr = X + r * lda;
+      for (int j = 0; j < q.n_ports; ++j) {
+        double2 acc = Xr[j];
+        for (int c = 0; c < q.n_con; ++c) acc = cfms(acc, Xr[q.n_ports + c], Lm[c * q.n_ports + j]);
+        Xr[j] = acc;
+      }
+    }
+    __syncthreads();
+  }
+  for (int e = tid; e < q.n_ports * q.n_ports; e += blockDim.x)
+    sparse_rows_body(e / q.n_ports, e % q.n_ports, q.n_ports, q.port_max, q.port_idx, q.port_val,
+                     X, lda, q.n_ports, reinterpret_cast<double2*>(p.Y), q.n_ports);
+  __syncthreads();
+}
+
+template <bool BIG>
+MW_DEV void wg_port_solve(const mw_sweep_prob* pp, const mw_ports* qq, double lbound) {
+  wg_setup(pp, qq);
+  const int64_t n = pp->n, nc = n + pp->ncols, lda = pp->lda;
+  double2* A = reinterpret_cast<double2*>(pp->A);
+  for (int64_t k0 = 0; k0 < n; k0 += NB) {
+    const int bw = (int)((n - k0) < NB ? (n - k0) : NB);
+    if (wg_factor_check(pp, k0, lbound)) wg_pivot(pp, k0);
+    wg_apply(pp, k0);
+    const int64_t rest = n - k0 - bw;
+    if (rest > 0)
+      wg_gemm<BIG>(rest, nc - k0 - bw, bw, A + (k0 + bw) * lda + k0, A + k0 * lda + k0 + bw,
+                   A + (k0 + bw) * lda + k0 + bw, lda);
+  }
+  double2* X = A + n;
+  for (int64_t k0 = ((n + NB - 1) / NB - 1) * NB; k0 >= 0; k0 -= NB) {
+    wg_back_step(pp, k0);
+    const int bw = (int)((n - k0) < NB ? (n - k0) : NB);
+    if (k0 > 0) wg_gemm<BIG>(k0, pp->ncols, bw, A + k0, X + k0 * lda, X, lda);
+  }
+  wg_finish(pp, qq);
+}
+
+// next[0]: the claim counter (zero on entry).  A set cancel flag (device-
+// readable) stops the solves: every problem claimed after it is seen gets
+// info[0] = -1 instead.
+template <bool BIG>
+__global__ __launch_bounds__(256, 2) void kb_lu_wg(const mw_sweep_prob* P, const mw_ports* ports,
+                                                   int nprob, int32_t* next, double lbound,
+                                                   const volatile int32_t* cancel) {
+  __shared__ int claim_s, stop_s;
+  // Every branch after the claim is on wave-uniform (scalar) values and no
+  // path back to the barrier depends on the thread index: with a divergent
+  // "stop" branch and a thread-0-only write before the back edge, the
+  // compiler once sent lanes 1-63 of wave 0 straight back to the barrier
+  // while lane 0, the only one that claims, waited behind them.
+  while (true) {
+    if (threadIdx.x == 0) {
+      claim_s = atomicAdd(next, 1);
+      stop_s = (cancel != nullptr && *cancel != 0) ? 1 : 0;
+    }
+    __syncthreads();
+    const int pi = __builtin_amdgcn_readfirstlane(claim_s);
+    const int stop = __builtin_amdgcn_readfirstlane(stop_s);
+    __syncthreads();
+    if (pi >= nprob) return;
+    if (stop)
+      P[pi].info[0] = -1;  // every thread stores the same value
+    else
+      wg_port_solve<BIG>(P + pi, ports + P[pi].mesh, lbound);
+  }
 }
 
 }  // namespace
@@ -2941,6 +3226,35 @@ int mw_sweep_lu_run(const mw_sweep_prob* dP, const mw_ports* dports, const mw_sw
                        st, dP, dports);
   }
   hipLaunchKernelGGL(kb_readout, dim3(max_ports, nprob), dim3(64), 0, st, dP, dports);
+  *herr = hipGetLastError();
+  return *herr == hipSuccess ? MW_OK : MW_EHIP;
+}
+
+// ---------------------------------------------------------------------------
+// Persistent batched solve (kb_lu_wg): the claim counter zeroed, then one
+// launch of `per_cu` workgroups per compute unit (at most the occupancy
+// limit, at most one per problem).  `cancel`: a device-readable flag or NULL.
+// ---------------------------------------------------------------------------
+int mw_sweep_wg_run(const mw_sweep_prob* dP, const mw_ports* dports, int nprob, int big,
+                    int per_cu, double lbound, int32_t* d_next, const volatile int32_t* cancel,
+                    hipStream_t st, hipError_t* herr) {
+  if (nprob <= 0) return MW_OK;
+  int dev = 0, cus = 0, occ = 0;
+  if ((*herr = hipGetDevice(&dev)) != hipSuccess) return MW_EHIP;
+  if ((*herr = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev)) !=
+      hipSuccess)
+    return MW_EHIP;
+  *herr = big ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, kb_lu_wg<true>, 256, 0)
+              : hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, kb_lu_wg<false>, 256, 0);
+  if (*herr != hipSuccess) return MW_EHIP;
+  const int grid = std::max(1, std::min(nprob, cus * std::max(1, std::min(per_cu, occ))));
+  hipLaunchKernelGGL(k_zero_i32, dim3(1), dim3(64), 0, st, d_next, (int64_t)1);
+  if (big)
+    hipLaunchKernelGGL(kb_lu_wg<true>, dim3(grid), dim3(256), 0, st, dP, dports, nprob, d_next,
+                       lbound, cancel);
+  else
+    hipLaunchKernelGGL(kb_lu_wg<false>, dim3(grid), dim3(256), 0, st, dP, dports, nprob, d_next,
+                       lbound, cancel);
   *herr = hipGetLastError();
   return *herr == hipSuccess ? MW_OK : MW_EHIP;
 }

@@ -10,9 +10,16 @@ wavenumber (reference src/momwire/bspline.py:4053-4179); the reference has
 no cross-structure batching (it sweeps one structure at a time,
 bspline.py:4181-4439).
 
-The tile shape of the batched updates is chosen by a small autotuner: every
-candidate is timed on the sweep itself and the fastest is kept (per process,
-keyed by the sweep's size profile).
+Two forms of the factorisations exist, and a small autotuner picks the form
+and its tile shape: every candidate is timed on the sweep itself and the
+fastest is kept (per process, keyed by the sweep's size profile).
+  0, 1  per-step launches over every active problem (about 4 per 32-column
+        panel of the largest problem), with 64 x 32 or 128 x 64 update tiles;
+  2..5  after the fill, ONE persistent launch in which each workgroup takes
+        one problem at a time (largest first) through its whole solve:
+        64 x 32 or 128 x 64 update tiles (bit 0), two or one workgroups per
+        compute unit (bit 1).  Its cancel token is read by the kernel between
+        problems.
 
 Threshold pivoting keeps a panel's diagonal pivots while every multiplier
 stays within ``pivot_threshold`` (|re| + |im|); 32 by default here, against
@@ -31,6 +38,8 @@ import numpy as np
 from . import _lib, _mesh
 from ._lib import MwPorts, MwSweepProb
 from .bspline import BSplineSolver, _Device, _torch
+
+VARIANTS = (0, 1, 2, 3, 4, 5)
 
 _TUNED = {}
 
@@ -153,10 +162,10 @@ class BatchedSweep:
                    "sweep")
 
     def autotune(self, reps=2):
-        """Time each tile variant of the batched updates on this sweep; keep the fastest."""
+        """Time each form and tile variant on this sweep; keep the fastest."""
         torch = self.torch
         best, self.tuning = None, {}
-        for variant in (0, 1):
+        for variant in VARIANTS:
             self._launch(variant)  # warm
             torch.cuda.synchronize()
             t0 = time.perf_counter()
@@ -206,6 +215,8 @@ class BatchedSweep:
         out = []
         for i, kk, buf in self.problems:
             info = buf["info"].cpu().numpy()
+            if info[0] < 0:
+                raise _lib.SolveAborted(f"structure {i} at k={kk}: sweep cancelled before it")
             if info[0] or info[1]:
                 raise np.linalg.LinAlgError(f"structure {i} at k={kk}: singular system")
             npt = self.devs[i].n_ports

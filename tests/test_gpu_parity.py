@@ -102,7 +102,7 @@ def test_lu_solve_many_rhs(dtype, n, p):
 
 
 @pytest.mark.parametrize("dtype", ["z", "c"])
-@pytest.mark.parametrize("n,p", [(1, 1), (40, 3), (300, 64), (1111, 9)])
+@pytest.mark.parametrize("n,p", [(1, 1), (40, 3), (300, 64), (1111, 9), (4500, 70), (5000, 300)])
 def test_gesv_augmented(dtype, n, p):
     """mw_?gesv: the right-hand sides ride through the factorisation as extra
     columns of [A | B]; the answer must match LAPACK's."""
@@ -125,6 +125,48 @@ def test_gesv_augmented(dtype, n, p):
     tol = eps * np.linalg.cond(M) * 200
     assert int(info.cpu()[0]) == 0
     assert rel_err(X, ref) < tol
+
+
+def _gesv_dev(fn, n, p, aug, ipiv, work, info):
+    _lib.check(fn(n, p, aug.data_ptr(), n + p, ipiv.data_ptr(), work.data_ptr(), info.data_ptr(),
+                  None, _stream()), "gesv")
+
+
+def test_gesv_two_level_graph_replay():
+    """Past n = 4096 the next outer block's panel chain runs on a side stream
+    beside the trailing update (forked and joined with events).  Captured in
+    a graph, every replay (the matrix re-uploaded between replays, eager work
+    and synchronisations in between) must equal the eager solve bit for bit;
+    the random matrix takes the tournament fallback on many panels."""
+    _say("gesv two-level graph replay")
+    n, p = 4200, 40
+    rng = np.random.default_rng(77)
+    M = rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))
+    RHS = rng.standard_normal((n, p)) + 1j * rng.standard_normal((n, p))
+    lib = _lib.load()
+    host = torch.from_numpy(np.hstack([M, RHS])).cuda()
+    aug = host.clone()
+    ipiv = torch.empty(n, dtype=torch.int32, device="cuda")
+    work = torch.empty(int(lib.mw_getrf_workspace_bytes(n)), dtype=torch.uint8, device="cuda")
+    info = torch.zeros(1, dtype=torch.int32, device="cuda")
+    _gesv_dev(lib.mw_zgesv, n, p, aug, ipiv, work, info)
+    torch.cuda.synchronize()
+    X_eager = aug[:, n:].cpu().numpy()
+    assert int(info.cpu()[0]) == 0
+    ref = np.linalg.solve(M, RHS)
+    assert rel_err(X_eager, ref) < 1.1e-16 * np.linalg.cond(M) * 200
+    aug.copy_(host)
+    torch.cuda.synchronize()
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g):
+        _gesv_dev(lib.mw_zgesv, n, p, aug, ipiv, work, info)
+    for _ in range(3):
+        aug.copy_(host)
+        torch.cuda.synchronize()
+        g.replay()
+        torch.cuda.synchronize()
+        assert np.array_equal(aug[:, n:].cpu().numpy(), X_eager)
+        assert int(info.cpu()[0]) == 0
 
 
 def test_lu_needs_pivoting():

@@ -136,3 +136,65 @@ def test_batched_fill_walks_wavenumbers():
         assert err < 1e-13, (i, kk, err)
     _say("  worst Z rel err", worst)
     bs.close()
+
+
+@pytest.mark.parametrize("variant", [2, 3, 4, 5])
+@pytest.mark.parametrize("bound", [32.0, 0.0])
+def test_persistent_form_matches_single(variant, bound):
+    """The persistent form: after the fill, one launch in which each
+    workgroup takes one problem at a time through its whole solve (tiles and
+    workgroups per compute unit by variant).  bound 0 sends every panel
+    through the tournament, run leaf by leaf inside the problem's workgroup
+    (two tree levels for the 827-unknown structure).  Each problem's Y must
+    be the single-problem path's, KCL junction structures included."""
+    from momwire_amd.sweep import BatchedSweep
+
+    idx = [89, 77, 4, 0]
+    structs = [W.random_structure(3 * 100003 + i) for i in idx]
+    ks = W.sweep_wavenumbers()[[2, 13]]
+    _say("persistent sweep, variant", variant, "bound", bound)
+    bs = BatchedSweep(structs, ks, gemm_variant=variant, pivot_threshold=bound)
+    n_con = [D.n_con for D in bs.devs]
+    assert max(n_con) > 0  # the KCL Schur step runs inside the workgroup too
+    for graph in (False, True, True):
+        bs.run(graph=graph)
+        torch.cuda.synchronize()
+        for (i, kk, y) in bs.y_matrices():
+            s = BSplineSolver(**structs[i])
+            s._set_k(kk)
+            assert rel_err(y, s.compute_port_solution().y) < 1e-12, (i, kk)
+    bs.close()
+
+
+def test_persistent_form_cancelled_on_the_device():
+    """A cancel token set while persistent launches are still queued: each
+    run's host-side check passes at enqueue time, the flag is set from the
+    host afterwards, and the launches that start later read it when they
+    claim a problem, leave every problem unsolved (info[0] = -1) and
+    y_matrices raises SolveAborted.  After reset the same plan solves
+    normally.  (Sixty runs are queued, so the flag is set long before the
+    queue drains; every kernel still exits normally.)"""
+    from momwire_amd.bspline import CancelToken, SolveAborted
+    from momwire_amd.sweep import BatchedSweep
+
+    structs = [W.random_structure(3 * 100003 + i) for i in (89, 77, 4, 0)]
+    ks = W.sweep_wavenumbers()
+    tok = CancelToken()
+    bs = BatchedSweep(structs, ks, gemm_variant=2, cancel=tok)
+    _ = tok.address  # the flag in mapped memory before the runs
+    bs.run(graph=False)
+    torch.cuda.synchronize()
+    for _ in range(60):
+        bs.run(graph=False)         # host check passes: the flag is still clear
+    tok.cancel()                    # set while most of the runs are still queued
+    torch.cuda.synchronize()
+    with pytest.raises(SolveAborted):
+        bs.y_matrices()
+    tok.reset()
+    bs.run(graph=False)
+    torch.cuda.synchronize()
+    for (i, kk, y) in bs.y_matrices()[::7]:
+        s = BSplineSolver(**structs[i])
+        s._set_k(kk)
+        assert rel_err(y, s.compute_port_solution().y) < 1e-12, (i, kk)
+    bs.close()
