@@ -26,6 +26,13 @@ template <typename C> MW_DEV C cfms(C a, C b, C c) {
   z.y = fma(-b.x, c.y, fma(-b.y, c.x, a.y));
   return z;
 }
+// a + b*c, as two fused multiply-adds per component
+template <typename C> MW_DEV C cfma(C a, C b, C c) {
+  C z;
+  z.x = fma(b.x, c.x, fma(-b.y, c.y, a.x));
+  z.y = fma(b.x, c.y, fma(b.y, c.x, a.y));
+  return z;
+}
 template <typename C> MW_DEV C cdiv(C a, C b) {
   // Smith's algorithm: no overflow for well-scaled inputs.
   typedef decltype(a.x) T;

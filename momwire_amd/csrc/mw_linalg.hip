@@ -773,13 +773,13 @@ template <typename CT> MW_DEV CT lane_bcast(CT v, int k) {
 
 template <typename CT>
 struct PanelLds {
-  CT M[NB][NB + 1];   // L11\U11 (rows workgroups reuse M and Li as a 64-row buffer)
+  CT M[NB][NB + 1];   // L11\U11 (checking workgroups reuse M and Li for 64 rows of A21)
   CT Li[NB][NB + 1];
   CT Ui[NB][NB + 1];
-  CT dinv[NB];
+  CT prow[2][NB];     // the elimination's pivot rows (double-buffered)
+  CT T[2][16 * NB];   // products of the inverses' doubling steps (L, U)
   int bad;
   int piv_s;
-  int progress;       // columns of M published by wave_lu_inv's elimination
 };
 
 // 1/d for a complex d != 0: conj(d) / |d|^2 with the real reciprocal from
@@ -801,218 +801,156 @@ template <typename CT> MW_DEV CT crecip_fast(CT d) {
   return o;
 }
 
-// LU without row exchanges of the NB x NB block in P.M (identity-padded) with
-// both inverses formed alongside it.  Wave 0 eliminates: lane r holds row r
-// in registers and the pivot row is broadcast with v_readlane, so the 31
-// steps need no barrier.  Rows at or above the pivot get a zero multiplier
-// instead of a predicated update (x - 0*u is x exactly for finite u).  A zero
-// pivot zeroes the step's multipliers; it, or a multiplier beyond lbound,
-// sets P.bad (the panel is then refactored with pivoting from A).  After
-// elimination step k every row's column k is final: wave 0 publishes it to
-// P.M and advances P.progress.  Other waves consume the columns as they
-// appear -- L^{-1} a row at a time, U^{-1} or L21 = A21 U^{-1} a column at a
-// time -- each waiting only for the columns it reads, so the chains overlap
-// instead of following each other.  The waits poll a counter that wave 0 of
-// the same workgroup advances and that wave never waits on the others, so
-// every wait ends.  A zero pivot leaves a zero diagonal inverse (and P.bad
-// set).
-template <typename CT>
-MW_DEV void publish_progress(PanelLds<CT>& P, int v) {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-  if ((threadIdx.x & 63) == 0)
-    __hip_atomic_store(&P.progress, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+// Lane (lane & ~7) | K of a group of 8 lanes (K constant), for every lane:
+// ds_swizzle in bit mode (and 0x18, or K) within each half-wave.
+template <int K> MW_DEV int swz8(int v) {
+  return __builtin_amdgcn_ds_swizzle(v, 0x18 | (K << 5));
+}
+template <int K> MW_DEV double swz8(double v) {
+  return __hiloint2double(swz8<K>(__double2hiint(v)), swz8<K>(__double2loint(v)));
+}
+template <int K> MW_DEV float swz8(float v) { return __int_as_float(swz8<K>(__float_as_int(v))); }
+template <int K, typename CT> MW_DEV CT swz8c(CT v) {
+  CT o;
+  o.x = swz8<K>(v.x);
+  o.y = swz8<K>(v.y);
+  return o;
 }
 
-template <typename CT>
-MW_DEV void wait_progress(PanelLds<CT>& P, int v) {
-  while (__hip_atomic_load(&P.progress, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) < v)
-    __builtin_amdgcn_s_sleep(1);
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
-}
-
-template <typename CT>
-MW_DEV void wave_lu_publish(PanelLds<CT>& P, double lbound) {
-  const int lane = threadIdx.x & 63;
-  const int r = lane & 31;
-  CT R[NB];
+// One elimination step k of lu32_eliminate (K = k & 7 and J = k >> 3 as
+// template constants, so the slot and the swizzle are static).
+template <int k, typename CT>
+MW_DEV void lu32_step(PanelLds<CT>& P, CT (&a)[4], int r, int g, double lbound, bool& bad) {
+  constexpr int K = k & 7, J = k >> 3;
+  const CT* pr = P.prow[k & 1];
+  const CT d = pr[k];
+  const bool zero = d.x == 0 && d.y == 0;
+  CT inv = crecip_fast(d);
+  inv.x = zero ? 0 : inv.x;
+  inv.y = zero ? 0 : inv.y;
+  bad |= zero;
+  // A[r][k] lives in lane (r, K) of this row's group of 8, slot J
+  const CT ark = swz8c<K>(a[J]);
+  if (r > k) {
+    const CT l = cmul(ark, inv);
+    if (!(cabs1(l) <= lbound)) bad = true;
 #pragma unroll
-  for (int j = 0; j < NB; ++j) R[j] = P.M[r][j];
-  bool bad = false;
-  // 1 / pivot (0 for a zero pivot, which marks the panel)
-  auto pivot_inverse = [&](int k) {
-    const CT d = lane_bcast(R[k], k);
-    const bool zero = d.x == 0 && d.y == 0;
-    bad |= zero;
-    CT inv = crecip_fast(d);
-    inv.x = zero ? 0 : inv.x;
-    inv.y = zero ? 0 : inv.y;
-    return inv;
-  };
-  CT inv = pivot_inverse(0);
-#pragma unroll
-  for (int k = 0; k < NB - 1; ++k) {
-    const bool below = r > k;
-    CT l = cmul(R[k], inv);
-    if (below && !(cabs1(l) <= lbound)) bad = true;
-    l.x = below ? l.x : 0;
-    l.y = below ? l.y : 0;
-    R[k].x = below ? l.x : R[k].x;
-    R[k].y = below ? l.y : R[k].y;
-    if (lane < 32) P.M[r][k] = R[k];
-    // column k + 1 first: the next pivot's reciprocal (a chain of dependent
-    // operations) then issues while the rest of this step's updates do
-    R[k + 1] = cfms(R[k + 1], l, lane_bcast(R[k + 1], k));
-    if (k + 1 < NB - 1) inv = pivot_inverse(k + 1);
-#pragma unroll
-    for (int j = k + 2; j < NB; ++j) R[j] = cfms(R[j], l, lane_bcast(R[j], k));
-    // after the step's updates: the column's LDS write has landed meanwhile
-    publish_progress(P, k + 1);
+    for (int j = 0; j < 4; ++j)
+      if (g + 8 * j > k) a[j] = cfms(a[j], l, pr[g + 8 * j]);
+    if (g == K) a[J] = l;
   }
-  const CT last = lane_bcast(R[NB - 1], NB - 1);
-  if (last.x == 0 && last.y == 0) bad = true;
-  if (lane < 32) P.M[r][NB - 1] = R[NB - 1];
-  publish_progress(P, NB);
-  if (__any(bad) && lane == 0) P.bad = 1;
+  if (r == k + 1) {
+#pragma unroll
+    for (int j = 0; j < 4; ++j) P.prow[(k + 1) & 1][g + 8 * j] = a[j];
+  }
+  __syncthreads();
 }
 
-// L^{-1}, a row at a time as the columns of L are published: lane c solves
-// L x = e_c (row r reads columns < r).
-template <typename CT>
-MW_DEV void wave_linv(PanelLds<CT>& P) {
-  const int c = threadIdx.x & 31;
-  CT x[NB];
-  x[0].x = (c == 0) ? 1 : 0;
-  x[0].y = 0;
-  if ((threadIdx.x & 63) < 32) P.Li[0][c] = x[0];
-#pragma unroll
-  for (int r = 1; r < NB; ++r) {
-    __builtin_amdgcn_sched_barrier(0);  // keep each row's LDS reads after its wait
-    wait_progress(P, r);
-    CT s4[4];
-#pragma unroll
-    for (int q = 0; q < 4; ++q) { s4[q].x = 0; s4[q].y = 0; }
-    s4[0].x = (r == c) ? 1 : 0;
-#pragma unroll
-    for (int j = 0; j < r; ++j) {
-      // at most four operands in flight (registers)
-      if ((j & 3) == 3) __builtin_amdgcn_sched_barrier(0);
-      s4[j & 3] = cfms(s4[j & 3], P.M[r][j], x[j]);
-    }
-    x[r].x = (s4[0].x + s4[1].x) + (s4[2].x + s4[3].x);
-    x[r].y = (s4[0].y + s4[1].y) + (s4[2].y + s4[3].y);
-    // stored at once: the store pins the row's arithmetic before the next
-    // wait (left to the end, it would be sunk there, holding every loaded
-    // operand live)
-    if ((threadIdx.x & 63) < 32) P.Li[r][c] = x[r];
+template <int k, typename CT>
+MW_DEV void lu32_steps(PanelLds<CT>& P, CT (&a)[4], int r, int g, double lbound, bool& bad) {
+  if constexpr (k < NB - 1) {
+    lu32_step<k>(P, a, r, g, lbound, bad);
+    lu32_steps<k + 1>(P, a, r, g, lbound, bad);
   }
 }
 
-// Rows of X with X U = I (rhs == nullptr; lane r: row r of U^{-1}, into
-// P.Ui), or of L21 with L21 U = A21 (lane r: row r of A21 at rhs, LDS rows
-// of NB + 1, overwritten by L21), a column at a time as the columns of U are
-// published (column c reads column c of U).  Returns, per lane, whether an
-// entry of a checked row (check) exceeds lbound or is not finite.
+// LU without row exchanges of the NB x NB block in P.M (identity-padded),
+// all 256 threads: thread t holds row t / 8, columns t % 8 + 8 j (j < 4, a
+// cyclic split so the shrinking trailing block stays spread over every
+// thread).  Step k: the pivot row comes from LDS (written by its 8 threads
+// at the end of step k - 1, double-buffered, so one barrier per step), each
+// row's multiplier from the group's lane holding column k (ds_swizzle, no
+// LDS round trip).  The entries and their update order are the textbook
+// right-looking elimination's.  A zero pivot zeroes the step's multipliers;
+// it, or a multiplier beyond lbound or not finite, sets P.bad.  Ends with a
+// barrier; P.bad must be initialised before the call.
 template <typename CT>
-MW_DEV bool wave_right_solve(PanelLds<CT>& P, const CT* rhs, bool check, double lbound) {
-  const int r = threadIdx.x & (rhs ? 63 : 31);
-  CT x[NB];
-  bool over = false;
-#pragma unroll
-  for (int c = 0; c < NB; ++c) {
-    __builtin_amdgcn_sched_barrier(0);  // keep each column's LDS reads after its wait
-    wait_progress(P, c + 1);
-    const CT u = P.M[c][c];
-    const bool zero = u.x == 0 && u.y == 0;
-    CT d = crecip_fast(u);
-    d.x = zero ? 0 : d.x;
-    d.y = zero ? 0 : d.y;
-    CT s4[4];
-#pragma unroll
-    for (int q = 0; q < 4; ++q) { s4[q].x = 0; s4[q].y = 0; }
-    if (rhs)
-      s4[0] = rhs[r * (NB + 1) + c];
-    else
-      s4[0].x = (r == c) ? 1 : 0;
-#pragma unroll
-    for (int j = 0; j < c; ++j) {
-      if ((j & 3) == 3) __builtin_amdgcn_sched_barrier(0);
-      s4[j & 3] = cfms(s4[j & 3], x[j], P.M[j][c]);
-    }
-    CT t;
-    t.x = (s4[0].x + s4[1].x) + (s4[2].x + s4[3].x);
-    t.y = (s4[0].y + s4[1].y) + (s4[2].y + s4[3].y);
-    x[c] = cmul(t, d);
-    // each result is stored at once: the store pins the column's arithmetic
-    // before the next wait (see wave_linv)
-    if (check) {
-      over |= !(cabs1(x[c]) <= lbound);
-      const_cast<CT*>(rhs)[r * (NB + 1) + c] = x[c];
-    } else if ((threadIdx.x & 63) < 32) {
-      P.Ui[r][c] = x[c];
-    }
-  }
-  return over;
-}
-
-// Inverses of the factors held in P.M (L\U, padded to NB with the
-// identity): P.Li = L^{-1}, P.Ui = U^{-1}.  One column per lane of wave 0 by
-// forward substitution: lanes 0-31 solve L x = e_c; lanes 32-63 solve the
-// index-reversed unit-upper system of U~ = D^{-1} U, so both halves run the
-// same instructions; U^{-1} = U~^{-1} D^{-1}.  Each row's
-// sum runs as four independent partial sums.  A zero pivot (reported
-// elsewhere) leaves a zero diagonal inverse.  Called by every thread; ends
-// with a barrier.
-template <typename CT>
-MW_DEV void wave_inverses(PanelLds<CT>& P) {
+MW_DEV void lu32_eliminate(PanelLds<CT>& P, double lbound) {
   const int tid = threadIdx.x;
-  if (tid < NB) {
-    CT d = P.M[tid][tid];
-    if (d.x != 0 || d.y != 0) d = crecip(d);
-    P.dinv[tid] = d;
+  const int r = tid >> 3, g = tid & 7;
+  CT a[4];
+#pragma unroll
+  for (int j = 0; j < 4; ++j) a[j] = P.M[r][g + 8 * j];
+  if (r == 0) {
+#pragma unroll
+    for (int j = 0; j < 4; ++j) P.prow[0][g + 8 * j] = a[j];
   }
   __syncthreads();
-  // the two unit-lower systems side by side in Li, Ui (contiguous):
-  // Li[i][j] = L[i][j], Ui[i][j] = U~[31-i][31-j] for j < i
+  bool bad = false;
+  lu32_steps<0>(P, a, r, g, lbound, bad);
+  const CT last = P.prow[(NB - 1) & 1][NB - 1];
+  bad |= (last.x == 0 && last.y == 0);
+#pragma unroll
+  for (int j = 0; j < 4; ++j) P.M[r][g + 8 * j] = a[j];
+  if (__any(bad) && (tid & 63) == 0) P.bad = 1;
+  __syncthreads();
+}
+
+// P.Li = L^{-1} and P.Ui = U^{-1} of the factors in P.M (L\U, padded to NB
+// with the identity), by recursive doubling over diagonal blocks of size s =
+// 1, 2, 4, 8, 16: for each pair of s-blocks,
+//   [[A, 0], [C, B]]^{-1} = [[A^{-1}, 0], [-B^{-1} (C A^{-1}), B^{-1}]]   (L)
+//   [[A, C], [0, B]]^{-1} = [[A^{-1}, -A^{-1} (C B^{-1})], [0, B^{-1}]]   (U)
+// two dependent products per level (the bracket into P.T, then the block),
+// so five short levels replace the 32-step substitutions.  A zero pivot
+// leaves a zero diagonal inverse.  Called by every thread; ends with a
+// barrier.
+template <typename CT>
+MW_DEV void lu32_inverses(PanelLds<CT>& P) {
+  const int tid = threadIdx.x;
   for (int e = tid; e < NB * NB; e += blockDim.x) {
-    const int i = e / NB, j = e % NB;
-    CT v, w;
-    v.x = 0; v.y = 0; w = v;
-    if (j < i) {
-      v = P.M[i][j];
-      w = cmul(P.M[NB - 1 - i][NB - 1 - j], P.dinv[NB - 1 - i]);
+    const int i = e >> 5, j = e & 31;
+    CT li, ui;
+    li.x = (i == j) ? 1 : 0;
+    li.y = 0;
+    ui.x = 0;
+    ui.y = 0;
+    if (i == j) {
+      const CT d = P.M[i][i];
+      if (d.x != 0 || d.y != 0) ui = crecip(d);
     }
-    P.Li[i][j] = v;
-    P.Ui[i][j] = w;
+    P.Li[i][j] = li;
+    P.Ui[i][j] = ui;
   }
   __syncthreads();
-  if (tid < 64) {
-    const int upper = tid >> 5;
-    const int c = tid & 31;
-    // one address per lane: no select between two arrays
-    const CT* A = &P.Li[0][0] + upper * (NB * (NB + 1));
-    CT x[NB];
 #pragma unroll
-    for (int r = 0; r < NB; ++r) {
-      CT s[4];
-#pragma unroll
-      for (int q = 0; q < 4; ++q) { s[q].x = 0; s[q].y = 0; }
-      s[0].x = (r == c) ? 1 : 0;
-#pragma unroll
-      for (int j = 0; j < r; ++j) s[j & 3] = cfms(s[j & 3], A[r * (NB + 1) + j], x[j]);
-      x[r].x = (s[0].x + s[1].x) + (s[2].x + s[3].x);
-      x[r].y = (s[0].y + s[1].y) + (s[2].y + s[3].y);
+  for (int s = 1; s < NB; s *= 2) {
+    const int cnt = (NB / 2) * s;  // entries of one bracket matrix, all pairs
+    for (int e = tid; e < 2 * cnt; e += blockDim.x) {
+      const int upper = e >= cnt, q = e - upper * cnt;
+      const int p = q / (s * s), i = (q / s) % s, j = q % s;
+      const int c0 = 2 * s * p, r0 = c0 + s;
+      CT acc;
+      acc.x = 0;
+      acc.y = 0;
+      if (!upper) {
+        for (int m = j; m < s; ++m) acc = cfma(acc, P.M[r0 + i][c0 + m], P.Li[c0 + m][c0 + j]);
+      } else {
+        for (int m = 0; m <= j; ++m) acc = cfma(acc, P.M[c0 + i][r0 + m], P.Ui[r0 + m][r0 + j]);
+      }
+      P.T[upper][q] = acc;
     }
-    if (!upper) {
-#pragma unroll
-      for (int r = 0; r < NB; ++r) P.Li[r][c] = x[r];
-    } else {
-      const int co = NB - 1 - c;
-      const CT dc = P.dinv[co];
-#pragma unroll
-      for (int r = 0; r < NB; ++r) P.Ui[NB - 1 - r][co] = cmul(x[r], dc);
+    __syncthreads();
+    for (int e = tid; e < 2 * cnt; e += blockDim.x) {
+      const int upper = e >= cnt, q = e - upper * cnt;
+      const int p = q / (s * s), i = (q / s) % s, j = q % s;
+      const int c0 = 2 * s * p, r0 = c0 + s;
+      const CT* Tp = &P.T[upper][p * s * s];
+      CT acc;
+      acc.x = 0;
+      acc.y = 0;
+      if (!upper) {
+        for (int m = 0; m <= i; ++m) acc = cfma(acc, P.Li[r0 + i][r0 + m], Tp[m * s + j]);
+        P.Li[r0 + i][c0 + j].x = -acc.x;
+        P.Li[r0 + i][c0 + j].y = -acc.y;
+      } else {
+        for (int m = i; m < s; ++m) acc = cfma(acc, P.Ui[c0 + i][c0 + m], Tp[m * s + j]);
+        P.Ui[c0 + i][r0 + j].x = -acc.x;
+        P.Ui[c0 + i][r0 + j].y = -acc.y;
+      }
     }
+    __syncthreads();
   }
-  __syncthreads();
 }
 
 constexpr int AW = 64;  // rows (or columns) per workgroup in the block products
@@ -1113,15 +1051,17 @@ MW_DEV void store_factors(const PanelLds<CT>& P, int bw, CT* lu11, CT* inv) {
 // of the panel -- those of A11 and L21 = A21 U11^{-1} -- stays within lbound
 // (threshold pivoting with tau = 1/lbound).  Nothing in A is written; a
 // violated bound (or a zero pivot) sets flag for the tournament fallback.
-// Block 0 stores L11\U11, the inverses and identity interchanges.
+// Block 0 stores L11\U11, the inverses and identity interchanges; block b > 0
+// checks 64 rows of L21, formed on MFMA from U11^{-1}.
 template <typename T>
 MW_DEV void panel_nopiv_body(PanelLds<typename cplx_t<T>::type>& P, int b, int64_t n,
                              const typename cplx_t<T>::type* A, int64_t lda, int64_t k0, int bw,
                              typename cplx_t<T>::type* lu11, typename cplx_t<T>::type* inv,
                              int32_t* ipiv, int32_t* flag, double lbound) {
   typedef typename cplx_t<T>::type CT;
-  const int tid = threadIdx.x;
-  // the elimination is a latency chain: let it issue ahead of update waves
+  typedef typename Mfma<T>::acc_t acc_t;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  // the factor is a latency chain: let it issue ahead of update waves
   // sharing its SIMD (when it overlaps a trailing update)
   __builtin_amdgcn_s_setprio(3);
   for (int e = tid; e < NB * NB; e += blockDim.x) {
@@ -1131,43 +1071,57 @@ MW_DEV void panel_nopiv_body(PanelLds<typename cplx_t<T>::type>& P, int b, int64
     if (i < bw && j < bw) v = A[(k0 + i) * lda + k0 + j];
     P.M[i][j] = v;
   }
-  if (tid == 0) {
-    P.bad = (lbound <= 0.0) ? 1 : 0;
-    P.progress = 0;
-  }
-  // blocks b > 0 check 64 rows of L21 = A21 U11^{-1}, one row per lane of
-  // wave 1, solved against U11 as its columns appear (no inverse needed);
-  // A21 is staged where the inverses would go (Li, Ui: 64 rows of NB + 1)
-  CT* a21 = &P.Li[0][0];
+  if (tid == 0) P.bad = (lbound <= 0.0) ? 1 : 0;
+  // a checking block's 64 rows of A21, loaded now so that they arrive while
+  // the factor runs
   const int64_t r0 = k0 + bw + (int64_t)(b - 1) * 64;
-  if (b > 0)
-    for (int e = tid; e < 64 * NB; e += blockDim.x) {
-      const int r = e >> 5, c = e & 31;
-      CT v;
-      v.x = 0; v.y = 0;
-      if (r0 + r < n && c < bw) v = A[(r0 + r) * lda + k0 + c];
-      a21[r * (NB + 1) + c] = v;
-    }
-  // wave-uniform (readfirstlane): the roles below are scalar branches
-  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  CT a21[8];
+#pragma unroll
+  for (int q = 0; q < 8; ++q) {
+    const int e = tid + 256 * q, r = e >> 5, c = e & 31;
+    a21[q].x = 0;
+    a21[q].y = 0;
+    if (b > 0 && r0 + r < n && c < bw) a21[q] = A[(r0 + r) * lda + k0 + c];
+  }
   __syncthreads();
-  bool over = false;
-  if (wave == 0)
-    wave_lu_publish(P, lbound);
-  else if (b == 0 && wave == 1)
-    wave_linv(P);
-  else if (b == 0 && wave == 2)
-    wave_right_solve(P, (const CT*)nullptr, false, 0.0);
-  else if (b > 0 && wave == 1)
-    over = wave_right_solve(P, a21, true, lbound) && r0 + (tid & 63) < n;
-  __syncthreads();
+  lu32_eliminate(P, lbound);
+  lu32_inverses(P);
   if (b == 0) {
     store_factors(P, bw, lu11, inv);
     if (tid < bw) ipiv[k0 + tid] = (int32_t)(k0 + tid);
     if (tid == 0 && P.bad) atomicOr(flag, 1);
     return;
   }
-  if (__any(over) && (tid & 63) == 0) atomicOr(flag, 1);
+  // L21 = A21 U11^{-1} for the 64 rows; A21 as reals over M and Li (64 rows
+  // of 65), U11^{-1} read from Ui
+  static_assert(offsetof(PanelLds<CT>, Li) == sizeof(P.M) &&
+                    sizeof(P.M) + sizeof(P.Li) >= sizeof(T) * 64 * 65,
+                "64 rows of 65 reals fit over M and Li");
+  T* As = reinterpret_cast<T*>(&P.M[0][0]);
+#pragma unroll
+  for (int q = 0; q < 8; ++q) {
+    const int e = tid + 256 * q, r = e >> 5, c = e & 31;
+    As[r * 65 + 2 * c] = a21[q].x;
+    As[r * 65 + 2 * c + 1] = a21[q].y;
+  }
+  __syncthreads();
+  constexpr int TC = TileGrid<64, 32>::TC, TPW = TileGrid<64, 32>::TPW;
+  acc_t acc[TPW];
+  lds_mfma<T, 64, 32>(As, &P.Ui[0][0], NB + 1, acc);
+  bool over = false;
+#pragma unroll
+  for (int j = 0; j < TPW; ++j) {
+    const int t = TPW * wave + j, tr = t / TC, tc = t % TC;
+#pragma unroll
+    for (int rr = 0; rr < 4; ++rr) {
+      const int row = 16 * tr + Mfma<T>::row(lane, rr);
+      const int cr = 16 * tc + (lane & 15);
+      // |re| + |im| of the complex entry: its two parts sit in lanes l, l ^ 1
+      const double mag = fabs((double)acc[j][rr]) + fabs((double)__shfl_xor(acc[j][rr], 1));
+      if (r0 + row < n && (cr >> 1) < bw && !(mag <= lbound)) over = true;
+    }
+  }
+  if (__any(over) && lane == 0) atomicOr(flag, 1);
 }
 
 template <typename T>
@@ -1321,7 +1275,7 @@ MW_DEV void panel_pivot_body(PivotLds<typename cplx_t<T>::type>& SL, int node0,
     nt_s = nt;
   }
   __syncthreads();
-  wave_inverses(P);
+  lu32_inverses(P);
   store_factors(P, bw, lu11, inv);
   // reset the arrival counters for the next fallback
   int total = 0;
