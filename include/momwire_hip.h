@@ -427,8 +427,10 @@ int mw_zsmall_solve(int32_t ns, int32_t nt, void* S, int64_t lds, void* T,
  * autotuner): 0 = per-step launches with 64x32 update tiles, 1 = the same
  * with 128x64 tiles; 2..5 = after the fill, ONE persistent launch in which a
  * workgroup takes one problem at a time (largest first) through its whole
- * solve -- 2: 64x32 tiles, 3: 128x64 tiles, two workgroups per compute unit;
- * 4, 5: the same with one per compute unit.  No workgroup of that launch
+ * solve -- 2: one-level blocking (a K = 32 update per 32-column panel),
+ * 3: two levels (128-column outer blocks, one K = 128 update of the rest
+ * per block), both with two workgroups per compute unit; 4, 5: the same
+ * with one per compute unit.  No workgroup of that launch
  * waits for another.  Its cancel flag is read by the kernel when it is
  * device-readable (mw_cancel_create): problems claimed after it is set are
  * left unsolved with info[0] = -1.  An ordinary host flag is read once,

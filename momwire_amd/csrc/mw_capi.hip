@@ -79,7 +79,7 @@ int mw_sweep_lu_run(const mw_sweep_prob* dP, const mw_ports* dports, const mw_sw
                     hipError_t* herr);
 void mw_sweep_work(const mw_sweep_prob* hp, int nprob, int gemm_variant,
                    std::vector<uint32_t>& items, std::vector<int64_t>& off);
-int mw_sweep_wg_run(const mw_sweep_prob* dP, const mw_ports* dports, int nprob, int big,
+int mw_sweep_wg_run(const mw_sweep_prob* dP, const mw_ports* dports, int nprob, int two,
                     int per_cu, double lbound, int32_t* d_next, const volatile int32_t* cancel,
                     hipStream_t st, hipError_t* herr);
 hipError_t mw_launch_add_sparse(int64_t nnz, const int32_t* rows, const int32_t* cols,

@@ -1569,6 +1569,95 @@ MW_DEV void gemm_k32_tiles(K32Lds<T>& L, int b, int G, int ntiles, int gxn, int6
   }
 }
 
+// C -= A B on 64 x 32 (complex) tiles with any kdim, by chunks of 32: one
+// tile's C stays in registers while its K chunks pass through LDS, and the
+// next chunk's operands (and the next tile's C) load while the current
+// chunk's MFMAs run.  One workgroup walks every tile.
+template <typename T>
+MW_DEV void gemm_kc_tiles(K32Lds<T>& L, int ntiles, int gxn, int64_t m, int64_t n, int kdim,
+                          const typename cplx_t<T>::type* __restrict__ A, int64_t lda,
+                          const typename cplx_t<T>::type* __restrict__ B, int64_t ldb,
+                          typename cplx_t<T>::type* C, int64_t ldc) {
+  typedef typename cplx_t<T>::type CT;
+  typedef typename Mfma<T>::acc_t acc_t;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int nch = (kdim + NB - 1) / NB;
+  const int total = ntiles * nch;
+  if (total <= 0) return;
+  T* Cr = reinterpret_cast<T*>(C);
+  acc_t cn[4], acc[4];
+  CT ra[8], rb[4];
+  // item it: tile it / nch, K chunk it % nch; a tile's first chunk also
+  // brings its C
+  auto load = [&](int it) {
+    const int t = it / nch, q = it % nch;
+    const int64_t row0 = (int64_t)(t / gxn) * 64, col0 = (int64_t)(t % gxn) * NB;
+    const int64_t k0 = (int64_t)q * NB;
+    if (q == 0) {
+#pragma unroll
+      for (int j = 0; j < 4; ++j)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int64_t gr = row0 + 16 * wave + Mfma<T>::row(lane, r);
+          const int cr = 16 * j + (lane & 15);
+          cn[j][r] = (gr < m && col0 + (cr >> 1) < n) ? Cr[gr * 2 * ldc + 2 * col0 + cr] : (T)0;
+        }
+    }
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const int e = tid + 256 * u, r = e >> 5, c = e & 31;
+      CT v;
+      v.x = 0; v.y = 0;
+      if (row0 + r < m && k0 + c < kdim) v = A[(row0 + r) * lda + k0 + c];
+      ra[u] = v;
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int e = tid + 256 * u, r = e >> 5, c = e & 31;
+      CT v;
+      v.x = 0; v.y = 0;
+      if (k0 + r < kdim && col0 + c < n) v = B[(k0 + r) * ldb + col0 + c];
+      rb[u] = v;
+    }
+  };
+  load(0);
+  for (int it = 0;;) {
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const int e = tid + 256 * u, r = e >> 5, c = e & 31;
+      L.As[r][2 * c] = -ra[u].x;
+      L.As[r][2 * c + 1] = -ra[u].y;
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int e = tid + 256 * u;
+      L.Bs[e >> 5][e & 31] = rb[u];
+    }
+    if (it % nch == 0) {
+#pragma unroll
+      for (int j = 0; j < 4; ++j) acc[j] = cn[j];
+    }
+    const int cur = it++;
+    __syncthreads();
+    if (it < total) load(it);
+    lds_mfma<T, 64, 32, false>(&L.As[0][0], &L.Bs[0][0], NB + 1, acc);
+    if (cur % nch == nch - 1) {
+      const int t = cur / nch;
+      const int64_t row0 = (int64_t)(t / gxn) * 64, col0 = (int64_t)(t % gxn) * NB;
+#pragma unroll
+      for (int j = 0; j < 4; ++j)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int64_t gr = row0 + 16 * wave + Mfma<T>::row(lane, r);
+          const int cr = 16 * j + (lane & 15);
+          if (gr < m && col0 + (cr >> 1) < n) Cr[gr * 2 * ldc + 2 * col0 + cr] = acc[j][r];
+        }
+    }
+    if (it >= total) break;
+    __syncthreads();  // LDS free for the next chunk's operands
+  }
+}
+
 template <typename T>
 struct NextLds {
   typedef typename cplx_t<T>::type CT;
@@ -1958,12 +2047,21 @@ template <typename T>
 __global__ void k_laswp_rows(typename cplx_t<T>::type* X, int64_t ldx, int64_t ncols,
                              const int32_t* ipiv, int64_t r0, int64_t r1) {
   typedef typename cplx_t<T>::type CT;
+  // the interchanges staged in LDS 1024 at a time (one read each, not one
+  // per thread and row)
+  __shared__ int32_t ps[1024];
   const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= ncols) return;
-  for (int64_t i = r0; i < r1; ++i) {
-    const int64_t p = ipiv[i];
-    if (p != i) {
-      const CT t = X[i * ldx + c]; X[i * ldx + c] = X[p * ldx + c]; X[p * ldx + c] = t;
+  for (int64_t i0 = r0; i0 < r1; i0 += 1024) {
+    const int cnt = (int)((r1 - i0) < 1024 ? (r1 - i0) : 1024);
+    __syncthreads();
+    for (int e = threadIdx.x; e < cnt; e += blockDim.x) ps[e] = ipiv[i0 + e];
+    __syncthreads();
+    if (c >= ncols) continue;
+    for (int e = 0; e < cnt; ++e) {
+      const int64_t i = i0 + e, p = ps[e];
+      if (p != i) {
+        const CT t = X[i * ldx + c]; X[i * ldx + c] = X[p * ldx + c]; X[p * ldx + c] = t;
+      }
     }
   }
 }
@@ -2395,8 +2493,7 @@ constexpr size_t kWgLds =
     sizeof(BlkLds<double>) > sizeof(PivotLds<double2>) ? sizeof(BlkLds<double>)
                                                        : sizeof(PivotLds<double2>);
 static_assert(kWgLds >= sizeof(PanelLds<double2>) && kWgLds >= sizeof(SmallLds) &&
-                  kWgLds >= sizeof(K32Lds<double>) &&
-                  kWgLds >= sizeof(GemmLds<double>) && kWgLds >= sizeof(GemmSmallLds<double>),
+                  kWgLds >= sizeof(K32Lds<double>),
               "the shared buffer holds every phase's LDS");
 __shared__ __align__(16) unsigned char g_wg_lds[kWgLds];
 __shared__ int g_wg_flag;
@@ -2459,8 +2556,9 @@ MW_PHASE int wg_factor_check(const mw_sweep_prob* pp, int64_t k0, double lbound)
 }
 
 // Tournament fallback of the panel at k0: the leaves one after the other
-// (the last child of each merge group, as the arrival counters count, merges).
-MW_PHASE void wg_pivot(const mw_sweep_prob* pp, int64_t k0) {
+// (the last child of each merge group, as the arrival counters count, merges);
+// its interchanges apply to columns [c_lo, n + ncols).
+MW_PHASE void wg_pivot(const mw_sweep_prob* pp, int64_t k0, int64_t c_lo) {
   const mw_sweep_prob& p = *pp;
   const int64_t n = p.n, nc = n + p.ncols;
   const int bw = (int)((n - k0) < NB ? (n - k0) : NB);
@@ -2470,17 +2568,17 @@ MW_PHASE void wg_pivot(const mw_sweep_prob* pp, int64_t k0) {
   const int nleaf = (int)((n - k0 + TROWS - 1) / TROWS);
   PivotLds<double2>& S = *reinterpret_cast<PivotLds<double2>*>(g_wg_lds);
   for (int leaf = 0; leaf < nleaf; ++leaf) {
-    panel_pivot_body<double>(S, leaf, reinterpret_cast<double2*>(p.A), p.lda, n, k0, nc, k0, bw,
+    panel_pivot_body<double>(S, leaf, reinterpret_cast<double2*>(p.A), p.lda, n, c_lo, nc, k0, bw,
                              nleaf, ws_tour(p.work, n), flag, reinterpret_cast<double2*>(p.work),
                              ws_inv(p.work, k0), p.ipiv, p.info);
     __syncthreads();
   }
 }
 
-// L21, U12 (right-hand sides included) and A11 of the panel at k0 into A.
-MW_PHASE void wg_apply(const mw_sweep_prob* pp, int64_t k0) {
+// L21, U12 for columns [k0 + bw, nc) and A11 of the panel at k0 into A.
+MW_PHASE void wg_apply(const mw_sweep_prob* pp, int64_t k0, int64_t nc) {
   const mw_sweep_prob& p = *pp;
-  const int64_t n = p.n, nc = n + p.ncols;
+  const int64_t n = p.n;
   const int bw = (int)((n - k0) < NB ? (n - k0) : NB);
   const int nrb = (int)((n - k0 - bw + AW - 1) / AW);
   const int ncb = (int)((nc - k0 - bw + AW - 1) / AW);
@@ -2493,31 +2591,26 @@ MW_PHASE void wg_apply(const mw_sweep_prob* pp, int64_t k0) {
   }
 }
 
-// C -= A B over an m x ncol block with K = kdim <= 32, one tile after the
-// other.
-template <bool BIG>
+// C -= A B over an m x ncol block (any K), 64 x 32 tiles one after the other.
 MW_PHASE void wg_gemm(int64_t m, int64_t ncol, int kdim, const double2* a, const double2* b,
                       double2* c, int64_t ld) {
-  if (!BIG) {
-    // 64 x 32 tiles walked in order, the next tile's operands loaded while
-    // the current one's MFMAs run
-    const int gxn = (int)((ncol + NB - 1) / NB);
-    const int ntiles = gxn * (int)((m + 63) / 64);
-    gemm_k32_tiles<double>(*reinterpret_cast<K32Lds<double>*>(g_wg_lds), 0, 1, ntiles, gxn, m,
-                           ncol, kdim, a, ld, b, ld, c, ld);
+  const int gxn = (int)((ncol + NB - 1) / NB);
+  const int ntiles = gxn * (int)((m + 63) / 64);
+  gemm_kc_tiles<double>(*reinterpret_cast<K32Lds<double>*>(g_wg_lds), ntiles, gxn, m, ncol, kdim,
+                        a, ld, b, ld, c, ld);
+  __syncthreads();
+}
+
+// X[k0 .. k0 + bw) <- L11^{-1} X[k0 .. k0 + bw) over ncols columns (the
+// panel at k0; X rows of ldx).
+MW_PHASE void wg_lower_rows(const mw_sweep_prob* pp, int64_t k0, double2* X, int64_t ldx,
+                            int64_t ncols) {
+  const mw_sweep_prob& p = *pp;
+  const int bw = (int)((p.n - k0) < NB ? (p.n - k0) : NB);
+  BlkLds<double>& B = *reinterpret_cast<BlkLds<double>*>(g_wg_lds);
+  for (int bx = 0; bx < (int)((ncols + AW - 1) / AW); ++bx) {
+    solve_apply_body<double>(B, bx, ncols, ws_inv(p.work, k0), k0, bw, X, ldx, 1);
     __syncthreads();
-    return;
-  }
-  const int tm = BIG ? GBM : SBM, tn = BIG ? GBN : SBN;
-  const int ny = (int)((m + tm - 1) / tm), nx = (int)((ncol + tn - 1) / tn);
-  for (int t = 0; t < nx * ny; ++t) {
-    if (BIG)
-      gemm_big_tile<double, false>(*reinterpret_cast<GemmLds<double>*>(g_wg_lds), t % nx, t / nx,
-                                   m, ncol, kdim, a, ld, b, ld, c, ld);
-    else
-      gemm_small_tile<double, false>(*reinterpret_cast<GemmSmallLds<double>*>(g_wg_lds), t % nx,
-                                     t / nx, m, ncol, kdim, a, ld, b, ld, c, ld);
-    __syncthreads();  // LDS free for the next tile
   }
 }
 
@@ -2582,25 +2675,54 @@ MW_PHASE void wg_finish(const mw_sweep_prob* pp, const mw_ports* qq) {
   __syncthreads();
 }
 
-template <bool BIG>
+// One level (TWO = false): per 32-column panel the check (tournament if it
+// fails), the apply and the K = 32 update of everything right of it.  Two
+// levels: a 128-column outer block is factored panel by panel with the
+// updates kept inside the block, then the block's U rows for the columns
+// right of it, then one K = 128 update of the rest -- the trailing matrix is
+// read and written once per outer block instead of once per panel.
+constexpr int kWgOuter = 128;
+
+template <bool TWO>
 MW_DEV void wg_port_solve(const mw_sweep_prob* pp, const mw_ports* qq, double lbound) {
   wg_setup(pp, qq);
   const int64_t n = pp->n, nc = n + pp->ncols, lda = pp->lda;
   double2* A = reinterpret_cast<double2*>(pp->A);
-  for (int64_t k0 = 0; k0 < n; k0 += NB) {
-    const int bw = (int)((n - k0) < NB ? (n - k0) : NB);
-    if (wg_factor_check(pp, k0, lbound)) wg_pivot(pp, k0);
-    wg_apply(pp, k0);
-    const int64_t rest = n - k0 - bw;
-    if (rest > 0)
-      wg_gemm<BIG>(rest, nc - k0 - bw, bw, A + (k0 + bw) * lda + k0, A + k0 * lda + k0 + bw,
-                   A + (k0 + bw) * lda + k0 + bw, lda);
+  const int64_t nbo = TWO ? kWgOuter : NB;
+  for (int64_t K0 = 0; K0 < n; K0 += nbo) {
+    const int64_t kend = (n - K0) < nbo ? n : K0 + nbo;
+    // one level: the apply and the update reach every column at once
+    const int64_t cblk = TWO ? kend : nc;
+    for (int64_t k0 = K0; k0 < kend; k0 += NB) {
+      const int bw = (int)((n - k0) < NB ? (n - k0) : NB);
+      if (wg_factor_check(pp, k0, lbound)) wg_pivot(pp, k0, K0);
+      wg_apply(pp, k0, cblk);
+      const int64_t rest = n - k0 - bw, restc = cblk - k0 - bw;
+      if (rest > 0 && restc > 0)
+        wg_gemm(rest, restc, bw, A + (k0 + bw) * lda + k0, A + k0 * lda + k0 + bw,
+                A + (k0 + bw) * lda + k0 + bw, lda);
+    }
+    const int64_t ncr = nc - kend;
+    if (TWO && ncr > 0) {
+      double2* X = A + kend;  // the block's U rows, columns [kend, nc)
+      for (int64_t k0 = K0; k0 < kend; k0 += NB) {
+        const int bw = (int)((n - k0) < NB ? (n - k0) : NB);
+        wg_lower_rows(pp, k0, X, lda, ncr);
+        const int64_t below = kend - k0 - bw;
+        if (below > 0)
+          wg_gemm(below, ncr, bw, A + (k0 + bw) * lda + k0, X + k0 * lda, X + (k0 + bw) * lda,
+                  lda);
+      }
+      if (n > kend)
+        wg_gemm(n - kend, ncr, (int)(kend - K0), A + kend * lda + K0, A + K0 * lda + kend,
+                A + kend * lda + kend, lda);
+    }
   }
   double2* X = A + n;
   for (int64_t k0 = ((n + NB - 1) / NB - 1) * NB; k0 >= 0; k0 -= NB) {
     wg_back_step(pp, k0);
     const int bw = (int)((n - k0) < NB ? (n - k0) : NB);
-    if (k0 > 0) wg_gemm<BIG>(k0, pp->ncols, bw, A + k0, X + k0 * lda, X, lda);
+    if (k0 > 0) wg_gemm(k0, pp->ncols, bw, A + k0, X + k0 * lda, X, lda);
   }
   wg_finish(pp, qq);
 }
@@ -2608,7 +2730,7 @@ MW_DEV void wg_port_solve(const mw_sweep_prob* pp, const mw_ports* qq, double lb
 // next[0]: the claim counter (zero on entry).  A set cancel flag (device-
 // readable) stops the solves: every problem claimed after it is seen gets
 // info[0] = -1 instead.
-template <bool BIG>
+template <bool TWO>
 __global__ __launch_bounds__(256, 2) void kb_lu_wg(const mw_sweep_prob* P, const mw_ports* ports,
                                                    int nprob, int32_t* next, double lbound,
                                                    const volatile int32_t* cancel) {
@@ -2631,7 +2753,7 @@ __global__ __launch_bounds__(256, 2) void kb_lu_wg(const mw_sweep_prob* P, const
     if (stop)
       P[pi].info[0] = -1;  // every thread stores the same value
     else
-      wg_port_solve<BIG>(P + pi, ports + P[pi].mesh, lbound);
+      wg_port_solve<TWO>(P + pi, ports + P[pi].mesh, lbound);
   }
 }
 
@@ -3187,9 +3309,10 @@ int mw_sweep_lu_run(const mw_sweep_prob* dP, const mw_ports* dports, const mw_sw
 // ---------------------------------------------------------------------------
 // Persistent batched solve (kb_lu_wg): the claim counter zeroed, then one
 // launch of `per_cu` workgroups per compute unit (at most the occupancy
-// limit, at most one per problem).  `cancel`: a device-readable flag or NULL.
+// limit, at most one per problem); `two`: 128-column outer blocks.
+// `cancel`: a device-readable flag or NULL.
 // ---------------------------------------------------------------------------
-int mw_sweep_wg_run(const mw_sweep_prob* dP, const mw_ports* dports, int nprob, int big,
+int mw_sweep_wg_run(const mw_sweep_prob* dP, const mw_ports* dports, int nprob, int two,
                     int per_cu, double lbound, int32_t* d_next, const volatile int32_t* cancel,
                     hipStream_t st, hipError_t* herr) {
   if (nprob <= 0) return MW_OK;
@@ -3198,12 +3321,12 @@ int mw_sweep_wg_run(const mw_sweep_prob* dP, const mw_ports* dports, int nprob, 
   if ((*herr = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev)) !=
       hipSuccess)
     return MW_EHIP;
-  *herr = big ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, kb_lu_wg<true>, 256, 0)
+  *herr = two ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, kb_lu_wg<true>, 256, 0)
               : hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, kb_lu_wg<false>, 256, 0);
   if (*herr != hipSuccess) return MW_EHIP;
   const int grid = std::max(1, std::min(nprob, cus * std::max(1, std::min(per_cu, occ))));
   hipLaunchKernelGGL(k_zero_i32, dim3(1), dim3(64), 0, st, d_next, (int64_t)1);
-  if (big)
+  if (two)
     hipLaunchKernelGGL(kb_lu_wg<true>, dim3(grid), dim3(256), 0, st, dP, dports, nprob, d_next,
                        lbound, cancel);
   else

@@ -17,9 +17,9 @@ fastest is kept (per process, keyed by the sweep's size profile).
         panel of the largest problem), with 64 x 32 or 128 x 64 update tiles;
   2..5  after the fill, ONE persistent launch in which each workgroup takes
         one problem at a time (largest first) through its whole solve:
-        64 x 32 or 128 x 64 update tiles (bit 0), two or one workgroups per
-        compute unit (bit 1).  Its cancel token is read by the kernel between
-        problems.
+        one-level or two-level (128-column outer blocks) blocking (bit 0),
+        two or one workgroups per compute unit (bit 1).  Its cancel token is
+        read by the kernel between problems.
 
 Threshold pivoting keeps a panel's diagonal pivots while every multiplier
 stays within ``pivot_threshold`` (|re| + |im|); 32 by default here, against
