@@ -61,6 +61,50 @@ def test_c3_full_size_properties():
     assert rel_err(ym, y) < 1e-8
 
 
+def test_c3_41_segments_per_element():
+    """The 32x32 array with the 8x8 array's element geometry (41 segments per
+    dipole): 41,984 unknowns and 1,024 ports, the other reading of "the same
+    element geometry".  FP64: reciprocity and the backward error against an
+    independent fill; the mixed form within 1e-8 of FP64."""
+    kw = W.dipole_array(32, 32, 41, seed=2)
+    s = BSplineSolver(**kw)
+    _say("C3 at 41 segments per element: fill + LU + 1,024-port solve (FP64)")
+    Y, X = s._solve_device()
+    torch.cuda.synchronize()
+    s._check_info()
+    D = s._dev
+    nb, npt = D.nb, D.n_ports
+    assert (nb, npt) == (41984, 1024)
+    y = Y.cpu().numpy()
+    _say("  reciprocity |Y - Y^T| / |Y|", rel_err(y, y.T))
+    assert rel_err(y, y.T) < 1e-10
+    lib = _lib.load()
+    st = torch.cuda.current_stream().cuda_stream
+    Z2 = torch.empty((nb, nb), dtype=torch.complex128, device="cuda")
+    _lib.check(lib.mw_same_edge_table(D.mw, s.k, st), "same-edge")
+    s._fill(D, s.k, s.omega, Z2.data_ptr(), nb, st)
+    R = torch.empty((nb, npt), dtype=torch.complex128, device="cuda")
+    _lib.check(lib.mw_zscatter_cols(nb, npt, D.col_max, D.col_idx.data_ptr(),
+                                    D.col_val.data_ptr(), R.data_ptr(), npt, st), "B")
+    _lib.check(lib.mw_zgemm_sub(nb, npt, nb, Z2.data_ptr(), nb, X.data_ptr(), X.stride(0),
+                                R.data_ptr(), npt, st), "B - Z X")
+    torch.cuda.synchronize()
+    back = float(torch.linalg.matrix_norm(R, ord=1) / (torch.linalg.matrix_norm(Z2, ord=1)
+                                                       * torch.linalg.matrix_norm(X, ord=1)))
+    _say("  backward error |B - Z X|_1 / (|Z|_1 |X|_1)", back)
+    assert back < 1e-14
+    del Z2, R, Y, X
+    s._dev = None
+    torch.cuda.empty_cache()
+    _say("  mixed precision")
+    sm = BSplineSolver(**kw, precision="mixed")
+    ym = sm.compute_port_solution().y
+    _say("  mixed vs FP64", rel_err(ym, y))
+    assert rel_err(ym, y) < 1e-8
+    sm._dev = None
+    torch.cuda.empty_cache()
+
+
 def test_fp32_far_fill():
     """far_fill="fp32" (the mixed form's far-interaction fill): on an 8x8
     array, pairs within 3 segment lengths are untouched, Z as a whole stays

@@ -133,11 +133,11 @@ def _gesv_dev(fn, n, p, aug, ipiv, work, info):
 
 
 def test_gesv_two_level_graph_replay():
-    """Past n = 4096 the next outer block's panel chain runs on a side stream
-    beside the trailing update (forked and joined with events).  Captured in
-    a graph, every replay (the matrix re-uploaded between replays, eager work
-    and synchronisations in between) must equal the eager solve bit for bit;
-    the random matrix takes the tournament fallback on many panels."""
+    """Past n = 4096 (two-level blocking: 512-column outer blocks), the solve
+    captured in a graph must replay to the eager answer bit for bit, replay
+    after replay (the matrix re-uploaded between replays, synchronisations in
+    between); the random matrix takes the tournament fallback on many
+    panels."""
     _say("gesv two-level graph replay")
     n, p = 4200, 40
     rng = np.random.default_rng(77)
@@ -295,6 +295,46 @@ def test_c2_array_8x8_matches_reference_golden():
     assert rel_err(sol.coeffs[:, g["coeffs_cols"]], g["coeffs"]) < 1e-10
     assert rel_err(sol.coeffs.sum(axis=1), g["coeffs_colsum"]) < 1e-10
     assert rel_err(sol.y, sol.y.T) < 1e-12  # reciprocity
+
+
+def test_c2_every_coefficient_column():
+    """The golden holds 4 of C2's 64 coefficient columns and the column sums;
+    every column is pinned here by the system itself: Z (pinned to the
+    oracle at 1e-11 by the fill tests) times the GPU's coefficients gives
+    each port's delta-gap column back to a backward error of a few ulps
+    times n, and Y = B^T X reads the same coefficients out."""
+    _say("c2 every coefficient column")
+    from momwire_amd import _lib
+
+    s = BSplineSolver(**W.dipole_array(8, 8, 41, seed=1))
+    sol = s.compute_port_solution()
+    D = s._device()
+    lib = _lib.load()
+    sym = int(s._mesh.symmetric)
+    pairs = D.keep["pairs"]
+    _lib.check(lib.mw_same_edge_table(D.mw, s.k, _stream()), "same-edge table")
+    _lib.check(lib.mw_fill_z(D.mw, s.k, s.omega, s.eps, s.mu, D.Z.data_ptr(), D.ld,
+                             pairs.data_ptr(), int(pairs.shape[0]), sym, _stream()), "fill")
+    torch.cuda.synchronize()
+    Z = D.Z[:, :D.nb].cpu().numpy()
+    X = sol.coeffs
+    assert X.shape == (2624, 64)
+    B = np.zeros_like(X)
+    cols = D.col_idx.cpu().numpy().reshape(D.ncols, -1)
+    vals = D.col_val.cpu().numpy().reshape(D.ncols, -1)
+    for j in range(64):
+        for r, v in zip(cols[j], vals[j]):
+            if r >= 0:
+                B[r, j] += v
+    # backward error per column, no worse than LAPACK's on the same system
+    scale = np.abs(Z).max() * np.abs(X).max(axis=0)
+    res = np.abs(Z @ X - B).max(axis=0) / scale
+    X_ref = np.linalg.solve(Z, B)
+    res_ref = np.abs(Z @ X_ref - B).max(axis=0) / scale
+    assert (res <= 20 * res_ref + 1e-15).all(), (res.max(), res_ref.max())
+    assert rel_err(X, X_ref) < 1e-11
+    # Y is B^T X for every column pair
+    assert rel_err(sol.y, B.T @ X) < 1e-12
 
 
 def test_graph_replay_matches_eager():
