@@ -298,6 +298,27 @@ int mw_cgesv(int64_t n, int64_t nrhs, void* A, int64_t lda, int32_t* ipiv,
              void* stream);
 
 /*
+ * mw_?gesv for a complex SYMMETRIC A (A[:, :n] = A[:, :n]^T exactly: the
+ * Galerkin Z when every wire has the same radius), factored WITHOUT row
+ * exchanges, so U = D L^T: past n = 4096 each 512-column block's U rows come
+ * from its L columns by a transpose-scale and the trailing update covers only
+ * the blocks on or below the block diagonal (half the factorisation's work).
+ * Same arguments and outputs as mw_?gesv (ipiv returned as the identity;
+ * the factors serve mw_?getrs).  When a panel's threshold test fails (a
+ * multiplier beyond the bound of mw_set_pivot_threshold, or a zero pivot)
+ * the panel is NOT pivoted and info[0] = -1 on return: the result is not
+ * valid and the caller refactors the original matrix with mw_?gesv.
+ * n <= 4096: mw_?gesv itself.  The reference has no counterpart: it solves
+ * Z with scipy.linalg.solve (bspline.py:2971-2989) whatever its symmetry.
+ */
+int mw_zsysv(int64_t n, int64_t nrhs, void* A, int64_t lda, int32_t* ipiv,
+             void* work, int32_t* info, const volatile int32_t* cancel,
+             void* stream);
+int mw_csysv(int64_t n, int64_t nrhs, void* A, int64_t lda, int32_t* ipiv,
+             void* work, int32_t* info, const volatile int32_t* cancel,
+             void* stream);
+
+/*
  * mw_zgesv without the workspace argument, for callers that cannot allocate
  * device memory (code under HIP graph capture, tight serving loops,
  * harnesses that forbid hipMalloc).  No counterpart in the reference: it

@@ -469,6 +469,10 @@ class BSplineSolver(_SweptPortSolutions):
         self._mesh = None
         self._ports = None
         self._dev = None
+        # the symmetric factorisation (mw_?sysv) used by the last solve, and
+        # whether one has needed row exchanges (then the pivoting path only)
+        self._sym_used = False
+        self._sym_failed = False
 
     # ------------------------------------------------------------------
     def _check_junction_coincidence(self):
@@ -595,12 +599,36 @@ class BSplineSolver(_SweptPortSolutions):
                                         D.col_val.data_ptr(), D.X.data_ptr(), ld, st),
                    "right-hand sides")
         mark("solve", True)
-        _lib.check(lib.mw_zgesv(nb, ncols, D.Z.data_ptr(), ld, D.ipiv.data_ptr(),
-                                D.work.data_ptr(), D.info.data_ptr(), cancel, st),
-                   "factorisation and solve")
+        self._sym_used = self._sym_ok(D)
+        gesv = lib.mw_zsysv if self._sym_used else lib.mw_zgesv
+        _lib.check(gesv(nb, ncols, D.Z.data_ptr(), ld, D.ipiv.data_ptr(), D.work.data_ptr(),
+                        D.info.data_ptr(), cancel, st), "factorisation and solve")
         mark("solve", False)
         self._post(D, D.X.data_ptr(), ld, D.Y.data_ptr(), st)
         return D.Y[:D.n_ports, :D.n_ports], D.X[:, :D.n_ports]
+
+    def _sym_ok(self, D):
+        """Whether the solve takes the symmetric factorisation (mw_?sysv, no
+        row exchanges, half the trailing update): Z = Z^T exactly (one radius
+        for every wire, free space, no wire loading: the fill computes the
+        upper tiles and mirrors them) at two-level sizes, unless a solve of
+        this solver has already needed row exchanges."""
+        return (bool(self._mesh.symmetric) and self.ground_z is None
+                and not self._loading_active and not self.extended_kernel
+                and D.nb > 4096 and not self._sym_failed)
+
+    def _solve_valid(self, graph=False):
+        """The device solve (a graph replay when ``graph``), repeated with the
+        pivoting factorisation when the symmetric one met a panel that needs
+        row exchanges (info = -1): that result is not valid."""
+        Y, W = self.solve_graph(warm=False) if graph else self._solve()
+        if self._sym_used:
+            info, = _to_host(self._dev.info)
+            if info[0] == -1:
+                self._sym_failed = True
+                self._graph_entry = None
+                Y, W = self._solve()
+        return Y, W
 
     def _fill(self, D, k, omega, zptr, ld, st):
         """Free-space fill, then the ground terms: Z -= C2 * image + Q
@@ -759,8 +787,10 @@ class BSplineSolver(_SweptPortSolutions):
         _lib.check(lib.mw_z2c(nb, nb, D.Z.data_ptr(), nb, D.Z32.data_ptr(), D.ld32, st), "Z to FP32")
         _lib.check(lib.mw_z2c(nb, ncols, D.R64.data_ptr(), nc, D.X32.data_ptr(), D.ld32, st),
                    "B to FP32")
-        _lib.check(lib.mw_cgesv(nb, ncols, D.Z32.data_ptr(), D.ld32, D.ipiv32.data_ptr(),
-                                D.work32.data_ptr(), D.info.data_ptr(), cancel, st),
+        self._sym_used = self._sym_ok(D)
+        gesv = lib.mw_csysv if self._sym_used else lib.mw_cgesv
+        _lib.check(gesv(nb, ncols, D.Z32.data_ptr(), D.ld32, D.ipiv32.data_ptr(),
+                        D.work32.data_ptr(), D.info.data_ptr(), cancel, st),
                    "FP32 factorisation and solve")
         _lib.check(lib.mw_c2z(nb, ncols, D.X32.data_ptr(), D.ld32, D.X64.data_ptr(), nc, 0, st),
                    "X to FP64")
@@ -875,6 +905,10 @@ class BSplineSolver(_SweptPortSolutions):
 
     def _check_info(self):
         info, = _to_host(self._dev.info)
+        if info[0] == -1:
+            raise np.linalg.LinAlgError("the symmetric factorisation met a panel that needs row "
+                                        "exchanges; its result is not valid (compute_port_"
+                                        "solution repeats such a solve with pivoting)")
         if info[0] != 0:
             raise np.linalg.LinAlgError(f"Z is singular: zero pivot at column {info[0] - 1}")
         if info[1] != 0:
@@ -888,10 +922,8 @@ class BSplineSolver(_SweptPortSolutions):
         solve is replayed (one launch instead of thousands).  A solver with a
         cancel token always launches eagerly, so that the flag is polled
         between panels and refinement steps."""
-        if self._cancel is None and getattr(self, "_last_k", None) == self.k:
-            Y, W = self.solve_graph(warm=False)
-        else:
-            Y, W = self._solve()
+        graph = self._cancel is None and getattr(self, "_last_k", None) == self.k
+        Y, W = self._solve_valid(graph)
         self._last_k = self.k
         y, coeffs = _to_host(Y, W)
         self._check_info()
@@ -967,7 +999,8 @@ class BSplineSolver(_SweptPortSolutions):
         (None) or the given (n_b, p) array."""
         import torch
         if coeffs is None:
-            _Y, W = self._solve()
+            _Y, W = self._solve_valid()
+            self._check_info()
             return W
         if isinstance(coeffs, torch.Tensor):
             return coeffs.to(device="cuda", dtype=torch.complex128)

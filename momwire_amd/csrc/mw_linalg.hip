@@ -314,6 +314,25 @@ MW_DEV void pgemm_tile(int p, int nbx, int nby, int& bx, int& by) {
   bx = r / gm;
 }
 
+// Tile (bx, by) of workgroup p when only the blocks on or below the block
+// diagonal are wanted (the symmetric factorisation's trailing update): the
+// matrix in 512 x 512 blocks, block row g holding blocks h = 0 .. g, each
+// block 4 x 8 tiles of 128 x 64.  The blocks are numbered row by row and
+// their tiles consecutively, and the XCD ranges split that numbering as in
+// pgemm_tile.  nbr: block rows.
+MW_DEV void pgemm_stair_tile(int p, int nbr, int& bx, int& by) {
+  const int T = 16 * nbr * (nbr + 1);  // 32 tiles per block, nbr (nbr + 1) / 2 blocks
+  const int x = p & 7, local = p >> 3;
+  const int t = x * (T >> 3) + min(x, T & 7) + local;
+  const int b = t >> 5, r = t & 31;
+  int g = (int)((sqrt(8.0 * b + 1.0) - 1.0) * 0.5);
+  while ((g + 1) * (g + 2) / 2 <= b) ++g;
+  while (g * (g + 1) / 2 > b) --g;
+  const int h = b - g * (g + 1) / 2;
+  by = 4 * g + (r & 3);
+  bx = 8 * h + (r >> 2);
+}
+
 // FP64: 16x16x4 MFMA, KR = 16 real k per stage (8 complex), LDS rows of 18
 // doubles (144 bytes: 16 consecutive rows hit distinct banks with 16-byte
 // accesses).  Permutation: position (kr % 4) 4 + kr / 4.  A is staged by
@@ -324,12 +343,16 @@ MW_DEV void pgemm_tile(int p, int nbx, int nby, int& bx, int& by) {
 __global__ __launch_bounds__(256, 2) void k_gemm_sub_pd(int64_t m, int64_t n, int64_t kdim,
                                                        const double2* __restrict__ A, int64_t lda,
                                                        const double2* __restrict__ B, int64_t ldb,
-                                                       double2* C, int64_t ldc, int nbx, int nby) {
+                                                       double2* C, int64_t ldc, int nbx, int nby,
+                                                       int stair) {
   constexpr int LD = 18, S = 4, KC = 8;
   __shared__ double As[2][128][LD];
   __shared__ double Bt[2][128][LD];
   int bx, by;
-  pgemm_tile(blockIdx.x, nbx, nby, bx, by);
+  if (stair)
+    pgemm_stair_tile(blockIdx.x, stair, bx, by);
+  else
+    pgemm_tile(blockIdx.x, nbx, nby, bx, by);
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wr = wave >> 1, wc = wave & 1;  // the wave's 64 x 64 real quadrant
   const int64_t row0 = (int64_t)by * 128, col0 = (int64_t)bx * 64;
@@ -447,12 +470,16 @@ __global__ __launch_bounds__(256, 2) void k_gemm_sub_pd(int64_t m, int64_t n, in
 __global__ __launch_bounds__(256, 2) void k_gemm_sub_pf(int64_t m, int64_t n, int64_t kdim,
                                                        const float2* __restrict__ A, int64_t lda,
                                                        const float2* __restrict__ B, int64_t ldb,
-                                                       float2* C, int64_t ldc, int nbx, int nby) {
+                                                       float2* C, int64_t ldc, int nbx, int nby,
+                                                       int stair) {
   constexpr int KR = 32, LD = 36, HALF = KR / 2, KC = KR / 2;
   __shared__ float As[2][128][LD];
   __shared__ float Bt[2][128][LD];
   int bx, by;
-  pgemm_tile(blockIdx.x, nbx, nby, bx, by);
+  if (stair)
+    pgemm_stair_tile(blockIdx.x, stair, bx, by);
+  else
+    pgemm_tile(blockIdx.x, nbx, nby, bx, by);
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wr = wave >> 1, wc = wave & 1;
   const int64_t row0 = (int64_t)by * 128, col0 = (int64_t)bx * 64;
@@ -557,10 +584,28 @@ hipError_t launch_gemm_p(int64_t m, int64_t n, int64_t k, const void* A, int64_t
   const int nbx = (int)((n + GBN - 1) / GBN), nby = (int)((m + GBM - 1) / GBM);
   if constexpr (sizeof(T) == 4)
     hipLaunchKernelGGL(k_gemm_sub_pf, dim3((unsigned)(nbx * nby)), dim3(256), 0, st, m, n, k,
-                       (const float2*)A, lda, (const float2*)B, ldb, (float2*)C, ldc, nbx, nby);
+                       (const float2*)A, lda, (const float2*)B, ldb, (float2*)C, ldc, nbx, nby, 0);
   else
     hipLaunchKernelGGL(k_gemm_sub_pd, dim3((unsigned)(nbx * nby)), dim3(256), 0, st, m, n, k,
-                       (const double2*)A, lda, (const double2*)B, ldb, (double2*)C, ldc, nbx, nby);
+                       (const double2*)A, lda, (const double2*)B, ldb, (double2*)C, ldc, nbx, nby,
+                       0);
+  return hipGetLastError();
+}
+
+// C -= A B over the blocks of an m x m product (512 x 512 blocks) on or below
+// the block diagonal.
+template <typename T>
+hipError_t launch_gemm_stair(int64_t m, int64_t k, const void* A, int64_t lda, const void* B,
+                             int64_t ldb, void* C, int64_t ldc, hipStream_t st) {
+  if (m <= 0 || k <= 0) return hipSuccess;
+  const int nbr = (int)((m + 511) / 512);
+  const unsigned grid = (unsigned)(16 * nbr * (nbr + 1));
+  if constexpr (sizeof(T) == 4)
+    hipLaunchKernelGGL(k_gemm_sub_pf, dim3(grid), dim3(256), 0, st, m, m, k, (const float2*)A, lda,
+                       (const float2*)B, ldb, (float2*)C, ldc, 0, 0, nbr);
+  else
+    hipLaunchKernelGGL(k_gemm_sub_pd, dim3(grid), dim3(256), 0, st, m, m, k, (const double2*)A,
+                       lda, (const double2*)B, ldb, (double2*)C, ldc, 0, 0, nbr);
   return hipGetLastError();
 }
 
@@ -2760,6 +2805,50 @@ __global__ __launch_bounds__(256, 2) void kb_lu_wg(const mw_sweep_prob* P, const
 }  // namespace
 
 // ---------------------------------------------------------------------------
+// Symmetric factorisation helpers.  For Z = Z^T factored without row
+// exchanges, U = D L^T (D: U's diagonal), so an outer block's U rows right of
+// it follow from its L columns below it:
+//   A[K0 + i][c] = A[K0 + i][K0 + i] A[c][K0 + i],  c in [kend, n).
+// 64 x 64 tiles transposed through LDS (coalesced reads along i, writes
+// along c).  blockIdx.x: 64 columns i of the block, blockIdx.y: 64 rows c.
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ __launch_bounds__(256) void k_sym_u_rows(int64_t K0, int bs, int64_t kend, int64_t n,
+                                                   typename cplx_t<T>::type* A, int64_t lda) {
+  typedef typename cplx_t<T>::type CT;
+  __shared__ CT tile[64][65];
+  __shared__ CT dg[64];
+  const int tid = threadIdx.x;
+  const int i0 = blockIdx.x * 64;
+  const int64_t c0 = kend + (int64_t)blockIdx.y * 64;
+  const int ni = (bs - i0) < 64 ? (bs - i0) : 64;
+  const int nc = (int)((n - c0) < 64 ? (n - c0) : 64);
+  if (tid < ni) dg[tid] = A[(K0 + i0 + tid) * lda + K0 + i0 + tid];
+  for (int e = tid; e < 64 * 64; e += 256) {
+    const int r = e >> 6, j = e & 63;  // row c0 + r, column K0 + i0 + j
+    if (r < nc && j < ni) tile[r][j] = A[(c0 + r) * lda + K0 + i0 + j];
+  }
+  __syncthreads();
+  for (int e = tid; e < 64 * 64; e += 256) {
+    const int j = e >> 6, r = e & 63;  // row K0 + i0 + j, column c0 + r
+    if (r < nc && j < ni) A[(K0 + i0 + j) * lda + c0 + r] = cmul(dg[j], tile[r][j]);
+  }
+}
+
+// info[0] <- -1 (unless already set) when any panel's threshold check failed:
+// the symmetric factorisation does not pivot, so its result is not valid.
+__global__ void k_sym_info(const int32_t* flags, int64_t npan, int32_t* info) {
+  __shared__ int any;
+  if (threadIdx.x == 0) any = 0;
+  __syncthreads();
+  int mine = 0;
+  for (int64_t e = threadIdx.x; e < npan; e += blockDim.x) mine |= flags[e];
+  if (mine) any = 1;
+  __syncthreads();
+  if (threadIdx.x == 0 && any && info[0] == 0) info[0] = -1;
+}
+
+// ---------------------------------------------------------------------------
 // Host drivers (called from mw_capi.hip)
 // ---------------------------------------------------------------------------
 // workspace: lu11 [NB*NB complex double] | diagonal-block inverses
@@ -3101,6 +3190,101 @@ int mw_gesv_t(int64_t n, int64_t nrhs, void* Av, int64_t lda, int32_t* ipiv, voi
   return *herr == hipSuccess ? MW_OK : MW_EHIP;
 }
 
+// Z = Z^T, factored and solved without row exchanges (the right-hand sides
+// riding along as in mw_gesv_t).  Two levels only (n > 4096): per outer block
+// the panel chain without its tournament fallback, the block's U rows right
+// of it by symmetry (k_sym_u_rows) and for the right-hand sides by
+// substitution, then the trailing update on the blocks on or below the block
+// diagonal (the part the later blocks read; the rest of the upper triangle is
+// overwritten by their U rows) and on the right-hand sides.  A panel whose
+// threshold check fails is not pivoted: info[0] = -1 and the result is not
+// valid (refactor the original matrix with mw_gesv_t).  One level: mw_gesv_t.
+template <typename T>
+int mw_sysv_t(int64_t n, int64_t nrhs, void* Av, int64_t lda, int32_t* ipiv, void* work,
+              int32_t* info, const volatile int32_t* cancel, hipStream_t st, hipError_t* herr) {
+  typedef typename cplx_t<T>::type CT;
+  const int64_t nbo = outer_block(n);
+  if (nbo == NB) return mw_gesv_t<T>(n, nrhs, Av, lda, ipiv, work, info, cancel, st, herr);
+  CT* A = reinterpret_cast<CT*>(Av);
+  const int64_t nc = n + nrhs;
+  const int64_t npan = (n + NB - 1) / NB;
+  const int64_t nodes = tour_nodes(n);
+  char* wp = reinterpret_cast<char*>(work);
+  CT* lu11 = reinterpret_cast<CT*>(wp);
+  int32_t* flags = reinterpret_cast<int32_t*>(wp + (int64_t)NB * NB * 16 * (1 + 2 * npan));
+  hipLaunchKernelGGL(k_zero_i32, dim3((unsigned)((npan + nodes + 255) / 256)), dim3(256), 0, st,
+                     flags, npan + nodes);
+  hipLaunchKernelGGL(k_zero_i32, dim3(1), dim3(64), 0, st, info, (int64_t)1);
+  const double lbound = g_lbound;
+  for (int64_t K0 = 0; K0 < n; K0 += nbo) {
+    const int64_t kend = (n - K0) < nbo ? n : K0 + nbo;
+    const int64_t bs = kend - K0;
+    for (int64_t k0 = K0; k0 < kend; k0 += NB) {
+      if (cancel && *cancel) return MW_EABORT;
+      const int bw = (int)((kend - k0) < NB ? (kend - k0) : NB);
+      const int64_t rest = n - k0 - bw, restc = kend - k0 - bw;
+      const int nrb = (int)((rest + AW - 1) / AW);
+      const int ncb = (int)((kend - k0 - bw + AW - 1) / AW);
+      CT* inv = reinterpret_cast<CT*>(panel_inv(work, k0));
+      hipLaunchKernelGGL(k_panel_nopiv<T>, dim3(nrb + 1), dim3(256), 0, st, n, (const CT*)A, lda,
+                         k0, bw, lu11, inv, ipiv, flags + k0 / NB, lbound);
+      hipLaunchKernelGGL(k_panel_apply<T>, dim3(nrb + ncb + 1), dim3(256), 0, st, n, kend, A, lda,
+                         k0, bw, (const CT*)inv, (const CT*)lu11, nrb, ncb);
+      if (rest > 0 && restc > 0) {
+        *herr = launch_gemm_sub<T>(rest, restc, bw, A + (k0 + bw) * lda + k0, lda,
+                                   A + k0 * lda + k0 + bw, lda, A + (k0 + bw) * lda + k0 + bw,
+                                   lda, st);
+        if (*herr != hipSuccess) return MW_EHIP;
+      }
+    }
+    if (n > kend) {
+      hipLaunchKernelGGL(k_sym_u_rows<T>, dim3((unsigned)((bs + 63) / 64),
+                                               (unsigned)((n - kend + 63) / 64)),
+                         dim3(256), 0, st, K0, (int)bs, kend, n, A, lda);
+    }
+    if (nrhs > 0) {
+      // the right-hand sides' U rows: L_blk^{-1} B_blk, panel by panel
+      CT* X = A + n;
+      const unsigned gx = (unsigned)((nrhs + AW - 1) / AW);
+      for (int64_t k0 = K0; k0 < kend; k0 += NB) {
+        const int bw = (int)((kend - k0) < NB ? (kend - k0) : NB);
+        const CT* inv = reinterpret_cast<const CT*>(panel_inv(work, k0));
+        hipLaunchKernelGGL(k_solve_apply<T>, dim3(gx), dim3(256), 0, st, nrhs, inv, k0, bw, X, lda,
+                           1);
+        const int64_t below = kend - k0 - bw;
+        if (below > 0) {
+          *herr = launch_gemm_sub<T>(below, nrhs, bw, A + (k0 + bw) * lda + k0, lda, X + k0 * lda,
+                                     lda, X + (k0 + bw) * lda, lda, st);
+          if (*herr != hipSuccess) return MW_EHIP;
+        }
+      }
+    }
+    if (n > kend) {
+      *herr = launch_gemm_stair<T>(n - kend, bs, A + kend * lda + K0, lda, A + K0 * lda + kend,
+                                   lda, A + kend * lda + kend, lda, st);
+      if (*herr != hipSuccess) return MW_EHIP;
+      if (nrhs > 0) {
+        *herr = launch_gemm_sub<T>(n - kend, nrhs, bs, A + kend * lda + K0, lda,
+                                   A + K0 * lda + n, lda, A + kend * lda + n, lda, st);
+        if (*herr != hipSuccess) return MW_EHIP;
+      }
+    }
+    *herr = hipGetLastError();
+    if (*herr != hipSuccess) return MW_EHIP;
+  }
+  hipLaunchKernelGGL(k_sym_info, dim3(1), dim3(256), 0, st, (const int32_t*)flags, npan, info);
+  if (nrhs > 0) {
+    int rc = solve_bwd<T>(n, nrhs, A, lda, work, A + n, lda, st, herr);
+    if (rc) return rc;
+  }
+  *herr = hipGetLastError();
+  return *herr == hipSuccess ? MW_OK : MW_EHIP;
+}
+
+template int mw_sysv_t<double>(int64_t, int64_t, void*, int64_t, int32_t*, void*, int32_t*,
+                               const volatile int32_t*, hipStream_t, hipError_t*);
+template int mw_sysv_t<float>(int64_t, int64_t, void*, int64_t, int32_t*, void*, int32_t*,
+                              const volatile int32_t*, hipStream_t, hipError_t*);
 template int mw_getrf_t<double>(int64_t, int64_t, void*, int64_t, int32_t*, void*, int32_t*,
                                 const volatile int32_t*, hipStream_t, hipError_t*);
 template int mw_getrf_t<float>(int64_t, int64_t, void*, int64_t, int32_t*, void*, int32_t*,

@@ -447,3 +447,65 @@ def test_compute_port_solution_replays_graph_across_k():
         assert rel_err(sol.y, Yr.cpu().numpy()) < 1e-13, kk
         assert rel_err(sol.coeffs, Wr.cpu().numpy()) < 1e-13, kk
     assert s._graph_entry is not None
+
+
+@pytest.mark.parametrize("dtype", ["z", "c"])
+def test_sysv_symmetric_two_level(dtype):
+    """mw_?sysv on a complex symmetric system past n = 4096: no row exchanges,
+    each block's U rows from its L columns, the trailing update on the blocks
+    on or below the block diagonal.  A well-scaled symmetric system matches
+    LAPACK; a symmetric system whose unpivoted multipliers exceed the
+    threshold bound reports info = -1 (result not valid)."""
+    _say("sysv", dtype)
+    n, p = 4500, 70
+    rng = np.random.default_rng(11)
+    R = rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))
+    lib = _lib.load()
+    ct = torch.complex128 if dtype == "z" else torch.complex64
+    fn = lib.mw_zsysv if dtype == "z" else lib.mw_csysv
+    RHS = rng.standard_normal((n, p)) + 1j * rng.standard_normal((n, p))
+    for dominant in (True, False):
+        M = R + R.T + (4.0 * np.sqrt(n) * np.eye(n) if dominant else 0.0)
+        aug = torch.from_numpy(np.hstack([M, RHS])).to(ct).cuda()
+        ipiv = torch.empty(n, dtype=torch.int32, device="cuda")
+        work = torch.empty(int(lib.mw_getrf_workspace_bytes(n)), dtype=torch.uint8,
+                           device="cuda")
+        info = torch.zeros(1, dtype=torch.int32, device="cuda")
+        _gesv_dev(fn, n, p, aug, ipiv, work, info)
+        torch.cuda.synchronize()
+        if dominant:
+            assert int(info.cpu()[0]) == 0
+            X = aug[:, n:].cpu().numpy().astype(np.complex128)
+            ref = np.linalg.solve(M, RHS)
+            eps = 1.1e-16 if dtype == "z" else 6e-8
+            assert rel_err(X, ref) < eps * np.linalg.cond(M) * 200
+            assert np.array_equal(ipiv.cpu().numpy(), np.arange(n))
+        else:
+            assert int(info.cpu()[0]) == -1
+
+
+def test_symmetric_solver_path_and_fallback():
+    """A 15 x 20 array (4,500 unknowns, one radius): the solver takes the
+    symmetric factorisation and gives the pivoting path's Y; with a pivot
+    bound of 0 (every panel fails the threshold test) compute_port_solution
+    falls back to the pivoting path and still returns the right Y."""
+    _say("symmetric solver path")
+    kw = W.dipole_array(15, 20, 15, seed=3)
+    s_sym = BSplineSolver(**kw)
+    y_sym = s_sym.compute_port_solution().y
+    assert s_sym._sym_used and not s_sym._sym_failed
+    s_piv = BSplineSolver(**kw)
+    s_piv._sym_failed = True  # the pivoting path only
+    y_piv = s_piv.compute_port_solution().y
+    assert not s_piv._sym_used
+    assert rel_err(y_sym, y_piv) < 1e-12
+    lib = _lib.load()
+    default = lib.mw_get_pivot_threshold()
+    try:
+        _lib.check(lib.mw_set_pivot_threshold(0.0), "threshold")
+        s_fb = BSplineSolver(**kw)
+        sol = s_fb.compute_port_solution()
+        assert s_fb._sym_failed and not s_fb._sym_used
+        assert rel_err(sol.y, y_piv) < 1e-12
+    finally:
+        lib.mw_set_pivot_threshold(default)
