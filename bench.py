@@ -267,10 +267,13 @@ def mixed_main(args, wl):
                 float(np.abs(_to_host(h[1])[0] - Y64).max() / ynorm) for h in hf
                 if isinstance(h, tuple)]
     fill_s = res["fp64"]["stage_ms"]["fill"] * 1e-3
-    solve_flops = lu_flops(nb) + 8.0 * nb * nb * npt
+    # the factorisation each form ran: symmetric (mw_?sysv) when Z = Z^T
+    sym32, sym64 = bool(smx._sym_used), bool(s64._sym_used)
+    solve_flops = lu_flops(nb, sym32) + 8.0 * nb * nb * npt
+    solve_flops64 = lu_flops(nb, sym64) + 8.0 * nb * nb * npt
     sm = res["mixed"]["stage_ms"]
     tf32 = solve_flops / (sm["solve32"] * 1e-3) / 1e12
-    tf64 = solve_flops / (res["fp64"]["stage_ms"]["solve"] * 1e-3) / 1e12
+    tf64 = solve_flops64 / (res["fp64"]["stage_ms"]["solve"] * 1e-3) / 1e12
     out = {
         "metric": METRIC,
         "value": res["mixed"]["rate"],
@@ -280,8 +283,12 @@ def mixed_main(args, wl):
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "c128 fill and residual, c64 factor and solves", "data": "synthetic",
         "config": {"workload": wl["desc"], "unknowns": nb, "segments": ns, "ports": npt,
-                   "precision": "mixed (FP64 fill, FP32 MFMA factor + all-port solve, FP64 "
-                                "residual refinement, adjoint-corrected readout)",
+                   "precision": "mixed (FP64 fill, FP32 MFMA factor + all-port solve, one "
+                                "FP64 residual, one FP32 correction, second-order readout "
+                                "checked on the device)" if args.refine_steps is None else
+                                f"mixed (FP64 fill, FP32 MFMA factor + all-port solve, "
+                                f"{args.refine_steps} FP64-residual corrections, "
+                                f"adjoint-corrected readout)",
                    "refine_steps": args.refine_steps, "launch": "eager, HIP events per stage"},
         "fp64_port_solves_per_s": res["fp64"]["rate"],
         "mixed_over_fp64": res["mixed"]["rate"] / res["fp64"]["rate"],
@@ -294,15 +301,22 @@ def mixed_main(args, wl):
         "fill_gentries_per_s": nb * nb / fill_s / 1e9,
         "fill_tflops": fill_flops(ns, nb) / fill_s / 1e12,
         "fp64_solve_tflops": tf64,
+        "factorisation": {"mixed": "symmetric (mw_csysv)" if sym32 else "LU (mw_cgesv)",
+                          "fp64": "symmetric (mw_zsysv)" if sym64 else "LU (mw_zgesv)"},
         "mixed_with_fp32_far_fill": far,
+        "mixed_readout_check": getattr(smx, "mixed_readout", None),
     }
     # the mixed form's dominant stage: the FP32 factor + all-port solve
-    out["roofline"] = {"kernel": "mw_cgesv: FP32 factor + all-port solve (k_gemm_sub_pf "
-                                 "K = 512 trailing updates on the FP32 32x32x2 MFMA, panels, "
-                                 "back substitution)",
+    fac = "(4/3) n^3" if sym32 else "(8/3) n^3"
+    out["roofline"] = {"kernel": ("mw_csysv: FP32 symmetric factor + all-port solve (k_gemm_sub_pf "
+                                  "K = 512 updates of the block-lower staircase on the FP32 "
+                                  "32x32x2 MFMA, panels, back substitution)") if sym32 else
+                                 ("mw_cgesv: FP32 factor + all-port solve (k_gemm_sub_pf K = 512 "
+                                  "trailing updates on the FP32 32x32x2 MFMA, panels, back "
+                                  "substitution)"),
                        "bound": "mfma", "achieved": tf32, "peak": PEAK_FP32_TFLOPS,
                        "unit": "TFLOP/s", "frac": tf32 / PEAK_FP32_TFLOPS, "traffic": None,
-                       "algorithmic": f"(8/3) n^3 + 8 n^2 p = {solve_flops:.3e} flop per solve "
+                       "algorithmic": f"{fac} + 8 n^2 p = {solve_flops:.3e} flop per solve "
                                       f"(n={nb}, p={npt})"}
     tpath = os.path.join(ROOT, "profiles", "hbm_traffic.json")
     if os.path.exists(tpath):
@@ -701,8 +715,10 @@ def fill_flops(n_seg, n_b):
     return 16 * 53 * n_seg ** 2 + 726 * n_b ** 2
 
 
-def lu_flops(n):
-    return 8.0 / 3.0 * n ** 3
+def lu_flops(n, symmetric=False):
+    """Factorisation flops: (8/3) n^3 for LU; (4/3) n^3 for the symmetric
+    factorisation (Z = Z^T, U = D L^T: half the trailing update)."""
+    return (4.0 if symmetric else 8.0) / 3.0 * n ** 3
 
 
 def cpu_baseline(kw, budget_s=20.0, cold_and_warm=False, max_runs=5):

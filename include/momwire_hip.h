@@ -272,6 +272,7 @@ int mw_cgetrf(int64_t n, void* A, int64_t lda, int32_t* ipiv, void* work,
  * Multi-RHS solve with the factors of mw_?getrf: X <- A^{-1} X for nrhs
  * columns (X: n x nrhs, row stride ldx).  work: the same workspace the
  * factorisation wrote (it keeps the inverted 32x32 diagonal blocks).
+ * ipiv NULL: factors without row exchanges (mw_?sysv's).
  * Replaces LAPACK zgetrs.
  */
 int mw_zgetrs(int64_t n, int64_t nrhs, const void* A, int64_t lda,
@@ -486,6 +487,25 @@ int mw_z2c(int64_t rows, int64_t cols, const void* src, int64_t lds, void* dst,
            int64_t ldd, void* stream);
 int mw_c2z(int64_t rows, int64_t cols, const void* src, int64_t lds, void* dst,
            int64_t ldd, int32_t accumulate, void* stream);
+
+/*
+ * The mixed form's one-residual port readout and its check.  Inputs, all
+ * FP64 complex and row-major on the device: X0 (n x p, the FP32 solve of
+ * Z X = B), C (n x p, the FP32 solve of Z C = R with R = B - Z X0 in FP64),
+ * T = C^T R (p x p) and Y = B^T X0 + X0^T R (p x p).  On return
+ * Y <- Y + T, the readout B^T X0 + X0^T R + C^T R of B^T Z^{-1} B for
+ * Z = Z^T, whose error is (Z^{-1} R - C)^T R; stats (4 doubles, device)
+ * = {delta, max|T|, max|Y|, delta max|T|} with delta = max_j |C_j| / |X0_j|
+ * (the FP32 solve's relative error, which C shares), and gate (1 int32,
+ * device) = 0 when delta max|T| <= tol max|Y|, else 1: the caller then
+ * reads out again from a fresh FP64 residual.  scratch: 16 (p + 1) bytes
+ * of device memory (cleared here).  No counterpart in the reference, which
+ * solves in FP64 only (bspline.py:2971-2989).
+ */
+int mw_zreadout_check(int64_t n, int64_t p, const void* X0, int64_t ldx, const void* C,
+                      int64_t ldc, const void* T, int64_t ldt, void* Y, int64_t ldy,
+                      double tol, void* scratch, double* stats, int32_t* gate,
+                      void* stream);
 
 /* ------------------------------------------------------------------------
  * Sommerfeld ground (ground_model="sommerfeld")

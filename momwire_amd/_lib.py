@@ -166,6 +166,8 @@ SIGNATURES = {
                                           _D, _P, _P]),
     "mw_z2c": (ctypes.c_int, [_I64, _I64, _P, _I64, _P, _I64, _P]),
     "mw_c2z": (ctypes.c_int, [_I64, _I64, _P, _I64, _P, _I64, _I32, _P]),
+    "mw_zreadout_check": (ctypes.c_int, [_I64, _I64, _P, _I64, _P, _I64, _P, _I64, _P, _I64,
+                                         ctypes.c_double, _P, _P, _P, _P]),
 }
 
 _lib = None

@@ -46,6 +46,10 @@ hipError_t mw_launch_z2c(int64_t rows, int64_t cols, const void* src, int64_t ld
                          int64_t ldd, hipStream_t st);
 hipError_t mw_launch_c2z(int64_t rows, int64_t cols, const void* src, int64_t lds, void* dst,
                          int64_t ldd, int accumulate, hipStream_t st);
+hipError_t mw_launch_readout_check(int64_t n, int64_t p, const void* X0, int64_t ldx,
+                                   const void* C, int64_t ldc, const void* T, int64_t ldt, void* Y,
+                                   int64_t ldy, double tol, void* scratch, double* stats,
+                                   int32_t* gate, hipStream_t st);
 hipError_t mw_launch_zentries(const mw_mesh& M, const int32_t* wing_seg, const int32_t* wing_slot,
                               double k, double omega, double eps, double mu, const int32_t* rows,
                               const int32_t* cols, int64_t n, int table, void* out,
@@ -436,7 +440,7 @@ static int getrs_common(bool dbl, int64_t n, int64_t nrhs, const void* A, int64_
                         void* stream) {
   g_err[0] = 0;
   if (n <= 0 || nrhs < 0) return fail(MW_EINVAL, "%sbad sizes n=%lld nrhs=%lld", "", n, nrhs);
-  if (!A || !ipiv || !X || !work) return fail(MW_EINVAL, "%sNULL argument");
+  if (!A || !X || !work) return fail(MW_EINVAL, "%sNULL argument");
   if (lda < n || ldx < nrhs) return fail(MW_EINVAL, "%sbad leading dimension (lda=%lld ldx=%lld)", "", lda, ldx);
   if (nrhs == 0) return MW_OK;
   hipError_t herr = hipSuccess;
@@ -609,6 +613,20 @@ int mw_c2z(int64_t rows, int64_t cols, const void* src, int64_t lds, void* dst, 
   if (lds < cols || ldd < cols) return fail(MW_EINVAL, "%sbad leading dimension");
   return hip_status(mw_launch_c2z(rows, cols, src, lds, dst, ldd, accumulate, (hipStream_t)stream),
                     "mw_c2z");
+}
+
+int mw_zreadout_check(int64_t n, int64_t p, const void* X0, int64_t ldx, const void* C,
+                      int64_t ldc, const void* T, int64_t ldt, void* Y, int64_t ldy, double tol,
+                      void* scratch, double* stats, int32_t* gate, void* stream) {
+  g_err[0] = 0;
+  if (n <= 0 || p <= 0) return fail(MW_EINVAL, "%sbad sizes n=%lld p=%lld", "", n, p);
+  if (!X0 || !C || !T || !Y || !scratch || !stats || !gate)
+    return fail(MW_EINVAL, "%sNULL argument");
+  if (ldx < p || ldc < p || ldt < p || ldy < p) return fail(MW_EINVAL, "%sbad leading dimension");
+  if (!(tol >= 0.0)) return fail(MW_EINVAL, "%stolerance must be >= 0 (not NaN)");
+  return hip_status(mw_launch_readout_check(n, p, X0, ldx, C, ldc, T, ldt, Y, ldy, tol, scratch,
+                                            stats, gate, (hipStream_t)stream),
+                    "mw_zreadout_check");
 }
 
 struct mw_sweep_plan {

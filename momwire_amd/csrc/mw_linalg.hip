@@ -4,6 +4,7 @@
 // (double: FP64 path, float: the FP32 factor of the mixed path).
 //
 // Storage: row-major, leading dimension in complex elements.
+#include <cfloat>
 #include <algorithm>
 #include <vector>
 
@@ -2196,6 +2197,102 @@ __global__ void k_c2z(int64_t rows, int64_t cols, const float2* src, int64_t lds
   dst[r * ldd + c] = o;
 }
 
+// The mixed form's one-residual readout, Y = B^T X0 + X0^T R + C^T R with
+// R = B - Z X0 (FP64) and C = Z32^{-1} R (the FP32 factors): its error is
+// (Z^{-1} R - C)^T R, about delta |C^T R| with delta = max_j |C_j| / |X0_j|
+// (the FP32 solve's relative error, the same for C as for X0).  Three
+// kernels: per-column sums of squares, Y += T with max |T| and max |Y|, and
+// the verdict.  Maxima of non-negative doubles are kept as their bit
+// patterns (ordered as the values), so they can be combined by atomicMax.
+__global__ __launch_bounds__(256) void k_colsq2(int64_t n, int64_t p, const double2* X,
+                                               int64_t ldx, const double2* C, int64_t ldc,
+                                               int64_t rows_per_blk, double* acc) {
+  __shared__ double red[2][4][64];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int64_t j = (int64_t)blockIdx.x * 64 + lane;
+  const int64_t r0 = (int64_t)blockIdx.y * rows_per_blk;
+  const int64_t r1 = (r0 + rows_per_blk) < n ? r0 + rows_per_blk : n;
+  double sx = 0.0, sc = 0.0;
+  if (j < p) {
+    for (int64_t r = r0 + w; r < r1; r += 4) {
+      const double2 x = X[r * ldx + j], c = C[r * ldc + j];
+      sx += x.x * x.x + x.y * x.y;
+      sc += c.x * c.x + c.y * c.y;
+    }
+  }
+  red[0][w][lane] = sx;
+  red[1][w][lane] = sc;
+  __syncthreads();
+  if (w == 0 && j < p) {
+    sx = (red[0][0][lane] + red[0][1][lane]) + (red[0][2][lane] + red[0][3][lane]);
+    sc = (red[1][0][lane] + red[1][1][lane]) + (red[1][2][lane] + red[1][3][lane]);
+    atomicAdd(acc + 2 * j, sx);
+    atomicAdd(acc + 2 * j + 1, sc);
+  }
+}
+
+__global__ __launch_bounds__(256) void k_add_maxabs(int64_t p, const double2* T, int64_t ldt,
+                                                   double2* Y, int64_t ldy,
+                                                   unsigned long long* mx) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  double t2 = 0.0, y2 = 0.0;
+  if (e < p * p) {
+    const int64_t r = e / p, c = e % p;
+    const double2 t = T[r * ldt + c];
+    double2 y = Y[r * ldy + c];
+    y.x += t.x;
+    y.y += t.y;
+    Y[r * ldy + c] = y;
+    t2 = t.x * t.x + t.y * t.y;
+    y2 = y.x * y.x + y.y * y.y;
+  }
+  // the quiet-NaN pattern orders above every finite value and infinity, so a
+  // NaN survives the maxima and fails the verdict's comparison
+  const unsigned long long qnan = 0x7FF8000000000000ULL;
+  unsigned long long tb = (t2 == t2) ? (unsigned long long)__double_as_longlong(t2) : qnan;
+  unsigned long long yb = (y2 == y2) ? (unsigned long long)__double_as_longlong(y2) : qnan;
+  for (int off = 32; off > 0; off >>= 1) {
+    const unsigned long long to = __shfl_xor(tb, off), yo = __shfl_xor(yb, off);
+    tb = to > tb ? to : tb;
+    yb = yo > yb ? yo : yb;
+  }
+  if ((threadIdx.x & 63) == 0) {
+    atomicMax(mx, tb);
+    atomicMax(mx + 1, yb);
+  }
+}
+
+__global__ __launch_bounds__(256) void k_readout_verdict(int64_t p, const double* acc,
+                                                        const unsigned long long* mx, double tol,
+                                                        double* stats, int32_t* gate) {
+  __shared__ double red[256];
+  double q = 0.0;
+  for (int64_t j = threadIdx.x; j < p; j += 256) {
+    const double x = acc[2 * j], c = acc[2 * j + 1];
+    double r = x > 0.0 ? c / x : (c == 0.0 ? 0.0 : INFINITY);
+    if (!(r <= DBL_MAX)) r = INFINITY;  // NaN or overflow: no estimate
+    q = fmax(q, r);
+  }
+  red[threadIdx.x] = q;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + s]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    const double delta = sqrt(red[0]);
+    const double tmax = sqrt(__longlong_as_double((long long)mx[0]));
+    const double ymax = sqrt(__longlong_as_double((long long)mx[1]));
+    const double est = delta * tmax;
+    stats[0] = delta;
+    stats[1] = tmax;
+    stats[2] = ymax;
+    stats[3] = est;
+    // NaN anywhere fails the comparison: the caller then takes a fresh residual
+    gate[0] = (est <= tol * ymax) ? 0 : 1;
+  }
+}
+
 // T <- S^{-1} T, one workgroup, Gauss-Jordan with partial pivoting in LDS
 // (ns <= 32, nt <= 64).
 struct SmallLds {
@@ -3062,8 +3159,10 @@ static int solve_fwd(int64_t n, int64_t nrhs, const typename cplx_t<T>::type* A,
   for (int64_t K0 = 0; K0 < n; K0 += nbo) {
     const int64_t kend = (n - K0) < nbo ? n : K0 + nbo;
     // a block's multiplier columns are stored in the block's final row order
-    hipLaunchKernelGGL(k_laswp_rows<T>, dim3((unsigned)((nrhs + 255) / 256)), dim3(256), 0, st, X,
-                       ldx, nrhs, ipiv, K0, kend);
+    // (no ipiv: factors without row exchanges, as the symmetric ones)
+    if (ipiv)
+      hipLaunchKernelGGL(k_laswp_rows<T>, dim3((unsigned)((nrhs + 255) / 256)), dim3(256), 0, st,
+                         X, ldx, nrhs, ipiv, K0, kend);
     // Rows inside the outer block take each panel's K = NB update; the rows
     // below it, one K = (kend - K0) update after the block, in the same row
     // order (the next block's interchanges come after it).
@@ -3363,6 +3462,27 @@ hipError_t mw_launch_c2z(int64_t rows, int64_t cols, const void* src, int64_t ld
   const int64_t tot = rows * cols;
   hipLaunchKernelGGL(k_c2z, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st, rows, cols,
                      (const float2*)src, lds, (double2*)dst, ldd, accumulate);
+  return hipGetLastError();
+}
+
+// scratch: 2 p column sums then the two maxima (16 (p + 1) bytes), cleared here
+hipError_t mw_launch_readout_check(int64_t n, int64_t p, const void* X0, int64_t ldx,
+                                   const void* C, int64_t ldc, const void* T, int64_t ldt, void* Y,
+                                   int64_t ldy, double tol, void* scratch, double* stats,
+                                   int32_t* gate, hipStream_t st) {
+  double* acc = reinterpret_cast<double*>(scratch);
+  unsigned long long* mx = reinterpret_cast<unsigned long long*>(acc + 2 * p);
+  const int64_t words = 4 * (p + 1);
+  hipLaunchKernelGGL(k_zero_i32, dim3((unsigned)((words + 255) / 256)), dim3(256), 0, st,
+                     reinterpret_cast<int32_t*>(scratch), words);
+  const int64_t rpb = 256;
+  hipLaunchKernelGGL(k_colsq2, dim3((unsigned)((p + 63) / 64), (unsigned)((n + rpb - 1) / rpb)),
+                     dim3(256), 0, st, n, p, (const double2*)X0, ldx, (const double2*)C, ldc, rpb,
+                     acc);
+  hipLaunchKernelGGL(k_add_maxabs, dim3((unsigned)((p * p + 255) / 256)), dim3(256), 0, st, p,
+                     (const double2*)T, ldt, (double2*)Y, ldy, mx);
+  hipLaunchKernelGGL(k_readout_verdict, dim3(1), dim3(256), 0, st, p, (const double*)acc,
+                     (const unsigned long long*)mx, tol, stats, gate);
   return hipGetLastError();
 }
 

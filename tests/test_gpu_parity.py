@@ -413,6 +413,90 @@ def test_mixed_precision_refines_to_fp64(name, kw):
     assert rel_err(sol.y, sol.y.T) < 1e-9
 
 
+def test_readout_check_kernel():
+    """mw_zreadout_check against numpy: Y + T, delta = max_j |C_j| / |X0_j|,
+    max|T|, max|Y + T|, the estimate delta max|T| and the gate on both sides
+    of the tolerance; a NaN in T sets the gate."""
+    _say("readout check kernel")
+    lib = _lib.load()
+    rng = np.random.default_rng(11)
+    for n, p in [(700, 5), (3000, 70), (257, 300)]:
+        X0 = rng.standard_normal((n, p)) + 1j * rng.standard_normal((n, p))
+        C = 1e-3 * (rng.standard_normal((n, p)) + 1j * rng.standard_normal((n, p)))
+        C[:, p // 2] *= 7.0
+        T = 1e-6 * (rng.standard_normal((p, p)) + 1j * rng.standard_normal((p, p)))
+        Y = rng.standard_normal((p, p)) + 1j * rng.standard_normal((p, p))
+        delta = np.max(np.linalg.norm(C, axis=0) / np.linalg.norm(X0, axis=0))
+        est = delta * np.abs(T).max()
+        ymax = np.abs(Y + T).max()
+        for tol, want in ((0.5 * est / ymax, 1), (2.0 * est / ymax, 0)):
+            dX, dC, dT, dY = _dev(X0), _dev(C), _dev(T), _dev(Y)
+            scratch = torch.empty(16 * (p + 1), dtype=torch.uint8, device="cuda")
+            stats = torch.zeros(4, dtype=torch.float64, device="cuda")
+            gate = torch.full((1,), 7, dtype=torch.int32, device="cuda")
+            _lib.check(lib.mw_zreadout_check(n, p, dX.data_ptr(), p, dC.data_ptr(), p,
+                                             dT.data_ptr(), p, dY.data_ptr(), p, tol,
+                                             scratch.data_ptr(), stats.data_ptr(),
+                                             gate.data_ptr(), _stream()), "check")
+            torch.cuda.synchronize()
+            assert rel_err(dY.cpu().numpy(), Y + T) < 1e-15
+            st = stats.cpu().numpy()
+            assert abs(st[0] - delta) < 1e-12 * delta
+            assert abs(st[1] - np.abs(T).max()) < 1e-14 * st[1]
+            assert abs(st[2] - ymax) < 1e-14 * ymax
+            assert abs(st[3] - est) < 1e-12 * est
+            assert int(gate.cpu()[0]) == want, (n, p, tol)
+    T[1, 2] = np.nan
+    dX, dC, dT, dY = _dev(X0), _dev(C), _dev(T), _dev(Y)
+    _lib.check(lib.mw_zreadout_check(n, p, dX.data_ptr(), p, dC.data_ptr(), p, dT.data_ptr(), p,
+                                     dY.data_ptr(), p, 1.0, scratch.data_ptr(), stats.data_ptr(),
+                                     gate.data_ptr(), _stream()), "check NaN")
+    torch.cuda.synchronize()
+    assert int(gate.cpu()[0]) == 1
+
+
+@pytest.mark.parametrize("name,kw", [("array2x2", W.dipole_array(2, 2, 11, seed=5)),
+                                     ("c2_array8x8", W.dipole_array(8, 8, 41, seed=1))])
+def test_mixed_one_residual_readout(name, kw, monkeypatch):
+    """The default mixed form on a symmetric Z: one FP64 residual, one FP32
+    correction and the readout B^T X0 + X0^T R + C^T R, whose error the device
+    estimates as delta max|C^T R|.  The estimate must not fall short of the
+    readout's actual error (max-entry, against the reference's golden Y) by
+    more than 2x; past READOUT_TOL a fresh residual reads out again, and the
+    default Y is within 1e-8 of the golden either way.  Forced past the check
+    (tolerance 0), graph replays (compute_port_solution from its second
+    call) equal the eager solve."""
+    from momwire_amd import bspline as B
+    _say("mixed one-residual readout", name)
+    g = golden(name)
+    gmax = np.abs(g["y"]).max()
+    s = BSplineSolver(**kw, precision="mixed")
+    y1 = s._solve_device_mixed()[0].cpu().numpy()
+    info = s.mixed_readout
+    _say("  check", info, "Y error", rel_err(y1, g["y"]))
+    assert info["second_residual"] == (info["estimate"] > B.READOUT_TOL * info["max_y"])
+    assert rel_err(y1, g["y"]) < 1e-8
+    monkeypatch.setattr(B, "READOUT_TOL", 1.0)
+    s1 = BSplineSolver(**kw, precision="mixed")
+    y_one = s1._solve_device_mixed()[0].cpu().numpy()
+    est = s1.mixed_readout["estimate"] / s1.mixed_readout["max_y"]
+    err = np.abs(y_one - g["y"]).max() / gmax
+    _say("  one-residual error %.3e, estimate %.3e" % (err, est))
+    assert not s1.mixed_readout["second_residual"]
+    assert err <= 2.0 * est + 1e-11
+    monkeypatch.setattr(B, "READOUT_TOL", 0.0)
+    s2 = BSplineSolver(**kw, precision="mixed")
+    y2 = s2._solve_device_mixed()[0].cpu().numpy()
+    assert s2.mixed_readout["second_residual"]
+    assert rel_err(y2, g["y"]) < 1e-8
+    sols = [s2.compute_port_solution() for _ in range(3)]
+    assert s2._graph_entry is not None and s2._graph_entry[4]
+    for sol in sols:
+        assert rel_err(sol.y, y2) < 1e-13
+        assert rel_err(sol.coeffs, sols[0].coeffs) < 1e-13
+    assert s2.mixed_readout["second_residual"]
+
+
 def test_gemm_tn_add():
     _say("gemm tn add")
     rng = np.random.default_rng(4)
