@@ -18,8 +18,12 @@
  * row-major storage.
  *
  * Cancellation.  Entry points that take `cancel` say which side reads it:
- *  - HOST-polled (mw_?getrf, mw_?gesv, mw_sweep_run): the host reads the
- *    int32 between launches; any host int32 address works.
+ *  - HOST-polled (mw_?getrf, mw_?gesv, mw_?sysv, mw_sweep_run): the host
+ *    reads the int32 between launches; any host int32 address works.
+ *    mw_sweep_run's persistent forms (gemm_variant 2-5) run as one launch:
+ *    they also read the flag on the device before each problem claim when
+ *    it is device-accessible mapped host memory (mw_cancel_create); an
+ *    ordinary host address is then read only before the launch.
  *  - DEVICE-polled (mw_somm_six_integrals, mw_somm_surfaces,
  *    mw_somm_surfaces_varied): the kernels read it while they run, so it
  *    must be device-accessible memory -- a flag from mw_cancel_create()
@@ -304,8 +308,12 @@ int mw_cgesv(int64_t n, int64_t nrhs, void* A, int64_t lda, int32_t* ipiv,
  * exchanges, so U = D L^T: past n = 4096 each 512-column block's U rows come
  * from its L columns by a transpose-scale and the trailing update covers only
  * the blocks on or below the block diagonal (half the factorisation's work).
- * Same arguments and outputs as mw_?gesv (ipiv returned as the identity;
- * the factors serve mw_?getrs).  When a panel's threshold test fails (a
+ * The inverses of the 512 x 512 diagonal blocks of L and U are then formed
+ * (recursive doubling from the 32 x 32 panel inverses) and the right-hand
+ * sides solved by one block product per 512 rows in each direction.  Same
+ * arguments and outputs as mw_?gesv except work, which holds
+ * mw_sysv_workspace_bytes(n) bytes (the block inverses stay there for
+ * mw_?sytrs; ipiv is returned as the identity past n = 4096).  When a panel's threshold test fails (a
  * multiplier beyond the bound of mw_set_pivot_threshold, or a zero pivot)
  * the panel is NOT pivoted and info[0] = -1 on return: the result is not
  * valid and the caller refactors the original matrix with mw_?gesv.
@@ -318,6 +326,26 @@ int mw_zsysv(int64_t n, int64_t nrhs, void* A, int64_t lda, int32_t* ipiv,
 int mw_csysv(int64_t n, int64_t nrhs, void* A, int64_t lda, int32_t* ipiv,
              void* work, int32_t* info, const volatile int32_t* cancel,
              void* stream);
+
+/*
+ * Workspace of mw_?sysv in bytes: mw_getrf_workspace_bytes(n) plus, past
+ * n = 4096, the 512 x 512 diagonal-block inverses of L and U and a
+ * 512 x 1024 staging block (2 x 8192 n + 8 MiB more).
+ */
+int64_t mw_sysv_workspace_bytes(int64_t n);
+
+/*
+ * X <- A^{-1} X with the factors and workspace of mw_?sysv (X: n x nrhs,
+ * row stride ldx): past n = 4096 one block product per 512 rows each way;
+ * n <= 4096, where mw_?sysv is mw_?gesv, mw_?getrs with ipiv.  No
+ * counterpart in the reference (see mw_?sysv).
+ */
+int mw_zsytrs(int64_t n, int64_t nrhs, const void* A, int64_t lda,
+              const int32_t* ipiv, const void* work, void* X, int64_t ldx,
+              void* stream);
+int mw_csytrs(int64_t n, int64_t nrhs, const void* A, int64_t lda,
+              const int32_t* ipiv, const void* work, void* X, int64_t ldx,
+              void* stream);
 
 /*
  * mw_zgesv without the workspace argument, for callers that cannot allocate

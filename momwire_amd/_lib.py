@@ -106,6 +106,7 @@ SIGNATURES = {
     "mw_fill_z_far32": (ctypes.c_int, [ctypes.POINTER(MwMesh), _D, _D, _D, _D, _P, _I64, _P, _I32,
                                        _I32, _D, _P]),
     "mw_getrf_workspace_bytes": (_I64, [_I64]),
+    "mw_sysv_workspace_bytes": (_I64, [_I64]),
     "mw_set_pivot_threshold": (ctypes.c_int, [_D]),
     "mw_get_pivot_threshold": (_D, []),
     "mw_zgetrf": (ctypes.c_int, [_I64, _P, _I64, _P, _P, _P, _P, _P]),
@@ -117,6 +118,8 @@ SIGNATURES = {
     "mw_zgesv_noalloc": (ctypes.c_int, [_I64, _I64, _P, _I64, _P, _P, _P, _P]),
     "mw_zgetrs": (ctypes.c_int, [_I64, _I64, _P, _I64, _P, _P, _P, _I64, _P]),
     "mw_cgetrs": (ctypes.c_int, [_I64, _I64, _P, _I64, _P, _P, _P, _I64, _P]),
+    "mw_zsytrs": (ctypes.c_int, [_I64, _I64, _P, _I64, _P, _P, _P, _I64, _P]),
+    "mw_csytrs": (ctypes.c_int, [_I64, _I64, _P, _I64, _P, _P, _P, _I64, _P]),
     "mw_zgemm_sub": (ctypes.c_int, [_I64, _I64, _I64, _P, _I64, _P, _I64, _P, _I64, _P]),
     "mw_cgemm_sub": (ctypes.c_int, [_I64, _I64, _I64, _P, _I64, _P, _I64, _P, _I64, _P]),
     "mw_zgemm_tn_add": (ctypes.c_int, [_I64, _I64, _I64, _P, _I64, _P, _I64, _P, _I64, _P]),

@@ -197,7 +197,9 @@ class _Device:
             self.X = self.Z[:, nb:]
             self.ipiv = torch.empty(nb, dtype=torch.int32, device=self.dev)
             lib = _lib.load()
-            self.work = torch.empty(int(lib.mw_getrf_workspace_bytes(nb)), dtype=torch.uint8,
+            # the symmetric factorisation's workspace (its block inverses) past
+            # n = 4096; the LU's below
+            self.work = torch.empty(int(lib.mw_sysv_workspace_bytes(nb)), dtype=torch.uint8,
                                     device=self.dev)
             self.post_buffers()
         return self
@@ -248,7 +250,7 @@ class _Device:
             self.gate = torch.zeros(1, dtype=torch.int32, device=self.dev)
             self.ipiv32 = torch.empty(nb, dtype=torch.int32, device=self.dev)
             lib = _lib.load()
-            self.work32 = torch.empty(int(lib.mw_getrf_workspace_bytes(nb)), dtype=torch.uint8,
+            self.work32 = torch.empty(int(lib.mw_sysv_workspace_bytes(nb)), dtype=torch.uint8,
                                       device=self.dev)
         return self
 
@@ -827,7 +829,7 @@ class BSplineSolver(_SweptPortSolutions):
         adjoint = D.n_con == 0 and bool(mesh.symmetric)
         one_residual = adjoint and self.refine_steps is None
         steps = self.refine_steps if self.refine_steps is not None else (1 if adjoint else 3)
-        ipiv32 = None if self._sym_used else D.ipiv32.data_ptr()
+        trs32 = lib.mw_csytrs if self._sym_used else lib.mw_cgetrs
 
         def record_adjoint():
             if history is None or not adjoint:
@@ -842,9 +844,8 @@ class BSplineSolver(_SweptPortSolutions):
             record_adjoint()
             _lib.check(lib.mw_z2c(nb, ncols, D.R64.data_ptr(), nc, D.D32.data_ptr(), nc, st),
                        "residual to FP32")
-            _lib.check(lib.mw_cgetrs(nb, ncols, D.Z32.data_ptr(), D.ld32, ipiv32,
-                                     D.work32.data_ptr(), D.D32.data_ptr(), nc, st),
-                       "FP32 correction")
+            _lib.check(trs32(nb, ncols, D.Z32.data_ptr(), D.ld32, D.ipiv32.data_ptr(),
+                             D.work32.data_ptr(), D.D32.data_ptr(), nc, st), "FP32 correction")
             if accumulate_into is not None:
                 _lib.check(lib.mw_c2z(nb, ncols, D.D32.data_ptr(), nc, accumulate_into, nc, 1, st),
                            "X += correction")

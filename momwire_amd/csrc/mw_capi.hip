@@ -19,6 +19,7 @@ hipError_t mw_launch_fill(const mw_mesh& M, double k, double omega, double eps, 
                           double2* Z, int64_t ldz, const int32_t* pairs, int32_t n_pairs,
                           bool symmetric, hipStream_t st);
 int64_t mw_getrf_ws_bytes(int64_t n);
+int64_t mw_sysv_ws_bytes(int64_t n);
 template <typename T>
 int mw_getrf_t(int64_t n, int64_t nc, void* A, int64_t lda, int32_t* ipiv, void* work,
                int32_t* info, const volatile int32_t* cancel, hipStream_t st, hipError_t* herr);
@@ -30,6 +31,9 @@ int mw_sysv_t(int64_t n, int64_t nrhs, void* A, int64_t lda, int32_t* ipiv, void
               int32_t* info, const volatile int32_t* cancel, hipStream_t st, hipError_t* herr);
 template <typename T>
 int mw_getrs_t(int64_t n, int64_t nrhs, const void* A, int64_t lda, const int32_t* ipiv,
+               const void* work, void* X, int64_t ldx, hipStream_t st, hipError_t* herr);
+template <typename T>
+int mw_sytrs_t(int64_t n, int64_t nrhs, const void* A, int64_t lda, const int32_t* ipiv,
                const void* work, void* X, int64_t ldx, hipStream_t st, hipError_t* herr);
 hipError_t mw_gemm_sub_z(int64_t m, int64_t n, int64_t k, const void* A, int64_t lda,
                          const void* B, int64_t ldb, void* C, int64_t ldc, hipStream_t st);
@@ -298,6 +302,9 @@ int mw_fill_z_far32(const mw_mesh* mesh, double k, double omega, double eps, dou
 }
 
 int64_t mw_getrf_workspace_bytes(int64_t n) { return n > 0 ? mw_getrf_ws_bytes(n) : 0; }
+int64_t mw_sysv_workspace_bytes(int64_t n) {
+  return n > 4096 ? mw_sysv_ws_bytes(n) : mw_getrf_workspace_bytes(n);
+}
 
 int mw_set_pivot_threshold(double bound) {
   g_err[0] = 0;
@@ -453,6 +460,30 @@ static int getrs_common(bool dbl, int64_t n, int64_t nrhs, const void* A, int64_
 int mw_zgetrs(int64_t n, int64_t nrhs, const void* A, int64_t lda, const int32_t* ipiv,
               const void* work, void* X, int64_t ldx, void* stream) {
   return getrs_common(true, n, nrhs, A, lda, ipiv, work, X, ldx, stream);
+}
+
+static int sytrs_common(bool dbl, int64_t n, int64_t nrhs, const void* A, int64_t lda,
+                        const int32_t* ipiv, const void* work, void* X, int64_t ldx,
+                        void* stream) {
+  g_err[0] = 0;
+  if (n <= 0 || nrhs < 0) return fail(MW_EINVAL, "%sbad sizes n=%lld nrhs=%lld", "", n, nrhs);
+  if (!A || !X || !work || (n <= 4096 && !ipiv)) return fail(MW_EINVAL, "%sNULL argument");
+  if (lda < n || ldx < nrhs) return fail(MW_EINVAL, "%sbad leading dimension (lda=%lld ldx=%lld)", "", lda, ldx);
+  if (nrhs == 0) return MW_OK;
+  hipError_t herr = hipSuccess;
+  int rc = dbl ? mw_sytrs_t<double>(n, nrhs, A, lda, ipiv, work, X, ldx, (hipStream_t)stream, &herr)
+               : mw_sytrs_t<float>(n, nrhs, A, lda, ipiv, work, X, ldx, (hipStream_t)stream, &herr);
+  if (rc == MW_EHIP) return hip_status(herr, "sytrs");
+  return rc;
+}
+
+int mw_zsytrs(int64_t n, int64_t nrhs, const void* A, int64_t lda, const int32_t* ipiv,
+              const void* work, void* X, int64_t ldx, void* stream) {
+  return sytrs_common(true, n, nrhs, A, lda, ipiv, work, X, ldx, stream);
+}
+int mw_csytrs(int64_t n, int64_t nrhs, const void* A, int64_t lda, const int32_t* ipiv,
+              const void* work, void* X, int64_t ldx, void* stream) {
+  return sytrs_common(false, n, nrhs, A, lda, ipiv, work, X, ldx, stream);
 }
 int mw_cgetrs(int64_t n, int64_t nrhs, const void* A, int64_t lda, const int32_t* ipiv,
               const void* work, void* X, int64_t ldx, void* stream) {

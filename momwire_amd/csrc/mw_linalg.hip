@@ -2932,6 +2932,133 @@ __global__ __launch_bounds__(256) void k_sym_u_rows(int64_t K0, int bs, int64_t 
   }
 }
 
+// ---------------------------------------------------------------------------
+// Inverses of the symmetric factorisation's 512 x 512 diagonal blocks, for
+// triangular solves by block products.  Per outer block b (rows K0 = 512 b,
+// bs = min(512, n - K0)): N_b = -L_bb^{-1} (L_bb the unit lower diagonal
+// block of the factors) and V_b = -U_bb^{-1} = N_b^T D_b^{-1} (U = D L^T),
+// both stored 512 x 512 (zero beyond bs).  N_b is built by recursive
+// doubling from the panels' 32 x 32 inverses: for a 2s x 2s diagonal block
+// [[P, 0], [C, Q]], the lower-left block of its inverse is -Q^{-1} C P^{-1};
+// with the negated blocks N_P = -P^{-1} and N_Q = -Q^{-1} it is
+// N_Q C N_P, formed as T = 0 - C N_P, then 0 - N_Q T.  T lives in V_b (the
+// lower-left quadrants of different levels do not overlap) until V_b is
+// written at the end.
+// ---------------------------------------------------------------------------
+constexpr int SYB = 512;  // outer block of the symmetric factorisation
+
+// N_b <- the negated panel inverses on its 32 x 32 diagonal blocks, zero
+// elsewhere; V_b <- 0.  Grid: (512 rows / 1, nblk); 256 threads, 2 columns each.
+template <typename T>
+__global__ __launch_bounds__(256) void k_blkinv_init(int64_t n, const void* work,
+                                                    typename cplx_t<T>::type* N,
+                                                    typename cplx_t<T>::type* V) {
+  typedef typename cplx_t<T>::type CT;
+  const int r = blockIdx.x, b = blockIdx.y;
+  const int64_t K0 = (int64_t)b * SYB;
+  const int bs = (int)((n - K0) < SYB ? (n - K0) : SYB);
+  CT* Nr = N + ((int64_t)b * SYB + r) * SYB;
+  CT* Vr = V + ((int64_t)b * SYB + r) * SYB;
+  const CT* li = reinterpret_cast<const CT*>(reinterpret_cast<const char*>(work) +
+                                             (int64_t)NB * NB * 16 * (1 + 2 * ((K0 + r) / NB)));
+  for (int c = threadIdx.x; c < SYB; c += 256) {
+    CT v;
+    v.x = 0;
+    v.y = 0;
+    if (r < bs && c < bs && (c / NB) == (r / NB) && c <= r) {
+      const CT l = li[(r % NB) * NB + (c % NB)];
+      v.x = -l.x;
+      v.y = -l.y;
+    }
+    Nr[c] = v;
+    CT z;
+    z.x = 0;
+    z.y = 0;
+    Vr[c] = z;
+  }
+}
+
+// One doubling level (s = 32, 64, 128, 256) over every block and pair:
+// PHASE 1: T = 0 - C N_P into V_b; PHASE 2: N_b's lower-left block = 0 - N_Q T.
+// Grid: nblk * (256 / s) pairs * tiles of the s x s output (64 x 32 tiles).
+template <typename T, int PHASE>
+__global__ __launch_bounds__(256) void k_blkinv_level(int s, int64_t n,
+                                                     const typename cplx_t<T>::type* A,
+                                                     int64_t lda, typename cplx_t<T>::type* N,
+                                                     typename cplx_t<T>::type* V) {
+  __shared__ GemmSmallLds<T> L;
+  const int tx = (s + SBN - 1) / SBN, ty = (s + SBM - 1) / SBM;
+  const int pairs = SYB / (2 * s);
+  int id = blockIdx.x;
+  const int tile = id % (tx * ty);
+  id /= tx * ty;
+  const int p = id % pairs, b = id / pairs;
+  const int64_t K0 = (int64_t)b * SYB;
+  const int bs = (int)((n - K0) < SYB ? (n - K0) : SYB);
+  const int a0 = 2 * p * s, d0 = a0 + s;
+  if (d0 >= bs) return;
+  const int mD = (bs - d0) < s ? (bs - d0) : s;
+  const int bx = tile % tx, by = tile / tx;
+  auto* Nb = N + (int64_t)b * SYB * SYB;
+  auto* Vb = V + (int64_t)b * SYB * SYB;
+  if (PHASE == 1)
+    gemm_small_tile<T, false>(L, bx, by, mD, s, s, A + (K0 + d0) * lda + K0 + a0, lda,
+                              Nb + (int64_t)a0 * SYB + a0, SYB, Vb + (int64_t)d0 * SYB + a0, SYB);
+  else
+    gemm_small_tile<T, false>(L, bx, by, mD, s, mD, Nb + (int64_t)d0 * SYB + d0, SYB,
+                              Vb + (int64_t)d0 * SYB + a0, SYB, Nb + (int64_t)d0 * SYB + a0, SYB);
+}
+
+// V_b[i][j] = N_b[j][i] / D_j (D_j = U's diagonal, A[K0 + j][K0 + j]): 64 x 64
+// tiles through LDS.  Grid: (8, 8, nblk).
+template <typename T>
+__global__ __launch_bounds__(256) void k_blkinv_vt(int64_t n, const typename cplx_t<T>::type* A,
+                                                  int64_t lda, const typename cplx_t<T>::type* N,
+                                                  typename cplx_t<T>::type* V) {
+  typedef typename cplx_t<T>::type CT;
+  __shared__ CT tile[64][65];
+  __shared__ CT dr[64];
+  const int tid = threadIdx.x;
+  const int j0 = blockIdx.x * 64, i0 = blockIdx.y * 64, b = blockIdx.z;
+  const int64_t K0 = (int64_t)b * SYB;
+  const int bs = (int)((n - K0) < SYB ? (n - K0) : SYB);
+  const CT* Nb = N + (int64_t)b * SYB * SYB;
+  CT* Vb = V + (int64_t)b * SYB * SYB;
+  if (tid < 64) {
+    CT d;
+    d.x = 0;
+    d.y = 0;
+    if (j0 + tid < bs) d = crecip(A[(K0 + j0 + tid) * lda + K0 + j0 + tid]);
+    dr[tid] = d;
+  }
+  for (int e = tid; e < 64 * 64; e += 256) {
+    const int jj = e >> 6, ii = e & 63;  // N_b[j0 + jj][i0 + ii]
+    tile[jj][ii] = Nb[(int64_t)(j0 + jj) * SYB + i0 + ii];
+  }
+  __syncthreads();
+  for (int e = tid; e < 64 * 64; e += 256) {
+    const int ii = e >> 6, jj = e & 63;  // V_b[i0 + ii][j0 + jj]
+    Vb[(int64_t)(i0 + ii) * SYB + j0 + jj] = cmul(tile[jj][ii], dr[jj]);
+  }
+}
+
+// tmp[r][c] <- X[r][c], X[r][c] <- 0 over rows x cols (the right-hand sides
+// of one block, moved aside for an out-of-place block product).
+template <typename T>
+__global__ void k_move_zero(int64_t rows, int64_t cols, typename cplx_t<T>::type* X, int64_t ldx,
+                            typename cplx_t<T>::type* tmp, int64_t ldt) {
+  typedef typename cplx_t<T>::type CT;
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= rows * cols) return;
+  const int64_t r = e / cols, c = e % cols;
+  CT* x = X + r * ldx + c;
+  tmp[r * ldt + c] = *x;
+  CT z;
+  z.x = 0;
+  z.y = 0;
+  *x = z;
+}
+
 // info[0] <- -1 (unless already set) when any panel's threshold check failed:
 // the symmetric factorisation does not pivot, so its result is not valid.
 __global__ void k_sym_info(const int32_t* flags, int64_t npan, int32_t* info) {
@@ -3289,23 +3416,114 @@ int mw_gesv_t(int64_t n, int64_t nrhs, void* Av, int64_t lda, int32_t* ipiv, voi
   return *herr == hipSuccess ? MW_OK : MW_EHIP;
 }
 
-// Z = Z^T, factored and solved without row exchanges (the right-hand sides
-// riding along as in mw_gesv_t).  Two levels only (n > 4096): per outer block
-// the panel chain without its tournament fallback, the block's U rows right
-// of it by symmetry (k_sym_u_rows) and for the right-hand sides by
-// substitution, then the trailing update on the blocks on or below the block
-// diagonal (the part the later blocks read; the rest of the upper triangle is
-// overwritten by their U rows) and on the right-hand sides.  A panel whose
-// threshold check fails is not pivoted: info[0] = -1 and the result is not
-// valid (refactor the original matrix with mw_gesv_t).  One level: mw_gesv_t.
+// The symmetric factorisation's workspace: mw_getrf_ws_bytes(n), then the
+// diagonal-block inverses N (nblk x 512 x 512), V (the same) and a
+// 512 x SYW staging block for the out-of-place block products (16-byte slots,
+// so one size serves FP64 and FP32).
+constexpr int64_t SYW = 1024;  // right-hand-side columns per block product
+int64_t mw_sysv_ws_bytes(int64_t n) {
+  const int64_t nblk = (n + SYB - 1) / SYB;
+  return mw_getrf_ws_bytes(n) + 2 * nblk * SYB * SYB * 16 + SYB * SYW * 16;
+}
+
+template <typename T>
+struct SymWs {
+  typedef typename cplx_t<T>::type CT;
+  CT *N, *V, *tmp;
+  SymWs(void* work, int64_t n) {
+    const int64_t nblk = (n + SYB - 1) / SYB;
+    char* base = reinterpret_cast<char*>(work) + mw_getrf_ws_bytes(n);
+    N = reinterpret_cast<CT*>(base);
+    V = reinterpret_cast<CT*>(base + nblk * SYB * SYB * 16);
+    tmp = reinterpret_cast<CT*>(base + 2 * nblk * SYB * SYB * 16);
+  }
+};
+
+// N_b and V_b of every outer block (see k_blkinv_init): ten launches in all.
+template <typename T>
+static hipError_t sym_block_inverses(int64_t n, const typename cplx_t<T>::type* A, int64_t lda,
+                                     void* work, hipStream_t st) {
+  const int64_t nblk = (n + SYB - 1) / SYB;
+  SymWs<T> w(work, n);
+  hipLaunchKernelGGL(k_blkinv_init<T>, dim3(SYB, (unsigned)nblk), dim3(256), 0, st, n,
+                     (const void*)work, w.N, w.V);
+  for (int s = NB; s < SYB; s *= 2) {
+    const int tiles = ((s + SBN - 1) / SBN) * ((s + SBM - 1) / SBM);
+    const unsigned grid = (unsigned)(nblk * (SYB / (2 * s)) * tiles);
+    hipLaunchKernelGGL((k_blkinv_level<T, 1>), dim3(grid), dim3(256), 0, st, s, n, A, lda, w.N,
+                       w.V);
+    hipLaunchKernelGGL((k_blkinv_level<T, 2>), dim3(grid), dim3(256), 0, st, s, n, A, lda, w.N,
+                       w.V);
+  }
+  hipLaunchKernelGGL(k_blkinv_vt<T>, dim3(SYB / 64, SYB / 64, (unsigned)nblk), dim3(256), 0, st,
+                     n, A, lda, (const typename cplx_t<T>::type*)w.N, w.V);
+  return hipGetLastError();
+}
+
+// X <- Z^{-1} X with the symmetric factors (Z = L D L^T, U = D L^T) and the
+// block inverses: per 512-row block one product with -N_b (forward) or -V_b
+// (backward) on the block's rows, moved aside first, and one K = 512 update
+// of the rows below (forward) or above (backward).
+template <typename T>
+static int sym_solve(int64_t n, int64_t nrhs, const typename cplx_t<T>::type* A, int64_t lda,
+                     void* work, typename cplx_t<T>::type* X, int64_t ldx, hipStream_t st,
+                     hipError_t* herr) {
+  typedef typename cplx_t<T>::type CT;
+  if (nrhs <= 0) return MW_OK;
+  SymWs<T> w(work, n);
+  const int64_t nblk = (n + SYB - 1) / SYB;
+  auto diag = [&](const CT* M, int64_t K0, int64_t bs) -> hipError_t {
+    for (int64_t c0 = 0; c0 < nrhs; c0 += SYW) {
+      const int64_t cw = (nrhs - c0) < SYW ? (nrhs - c0) : SYW;
+      const int64_t tot = bs * cw;
+      hipLaunchKernelGGL(k_move_zero<T>, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st, bs,
+                         cw, X + K0 * ldx + c0, ldx, w.tmp, cw);
+      hipError_t e = launch_gemm_sub<T>(bs, cw, bs, M, SYB, w.tmp, cw, X + K0 * ldx + c0, ldx, st);
+      if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
+  };
+  for (int64_t b = 0; b < nblk; ++b) {
+    const int64_t K0 = b * SYB, kend = (n - K0) < SYB ? n : K0 + SYB, bs = kend - K0;
+    *herr = diag(w.N + b * SYB * SYB, K0, bs);
+    if (*herr != hipSuccess) return MW_EHIP;
+    if (n > kend) {
+      *herr = launch_gemm_sub<T>(n - kend, nrhs, bs, A + kend * lda + K0, lda, X + K0 * ldx, ldx,
+                                 X + kend * ldx, ldx, st);
+      if (*herr != hipSuccess) return MW_EHIP;
+    }
+  }
+  for (int64_t b = nblk - 1; b >= 0; --b) {
+    const int64_t K0 = b * SYB, kend = (n - K0) < SYB ? n : K0 + SYB, bs = kend - K0;
+    *herr = diag(w.V + b * SYB * SYB, K0, bs);
+    if (*herr != hipSuccess) return MW_EHIP;
+    if (K0 > 0) {
+      *herr = launch_gemm_sub<T>(K0, nrhs, bs, A + K0, lda, X + K0 * ldx, ldx, X, ldx, st);
+      if (*herr != hipSuccess) return MW_EHIP;
+    }
+  }
+  *herr = hipGetLastError();
+  return *herr == hipSuccess ? MW_OK : MW_EHIP;
+}
+
+// Z = Z^T, factored without row exchanges.  Two levels only (n > 4096): per
+// outer block the panel chain without its tournament fallback, the block's U
+// rows right of it by symmetry (k_sym_u_rows), then the trailing update on
+// the blocks on or below the block diagonal (the part the later blocks read;
+// the rest of the upper triangle is overwritten by their U rows).  Then the
+// 512 x 512 diagonal blocks' inverses (sym_block_inverses) and the
+// right-hand sides (columns [n, n + nrhs) of A) by block products.  A panel
+// whose threshold check fails is not pivoted: info[0] = -1 and the result is
+// not valid (refactor the original matrix with mw_gesv_t).  One level:
+// mw_gesv_t.  work: mw_sysv_ws_bytes(n).
 template <typename T>
 int mw_sysv_t(int64_t n, int64_t nrhs, void* Av, int64_t lda, int32_t* ipiv, void* work,
               int32_t* info, const volatile int32_t* cancel, hipStream_t st, hipError_t* herr) {
   typedef typename cplx_t<T>::type CT;
   const int64_t nbo = outer_block(n);
   if (nbo == NB) return mw_gesv_t<T>(n, nrhs, Av, lda, ipiv, work, info, cancel, st, herr);
+  static_assert(SYB == 512, "the symmetric path's blocks are the two-level outer blocks");
   CT* A = reinterpret_cast<CT*>(Av);
-  const int64_t nc = n + nrhs;
   const int64_t npan = (n + NB - 1) / NB;
   const int64_t nodes = tour_nodes(n);
   char* wp = reinterpret_cast<char*>(work);
@@ -3340,46 +3558,34 @@ int mw_sysv_t(int64_t n, int64_t nrhs, void* Av, int64_t lda, int32_t* ipiv, voi
       hipLaunchKernelGGL(k_sym_u_rows<T>, dim3((unsigned)((bs + 63) / 64),
                                                (unsigned)((n - kend + 63) / 64)),
                          dim3(256), 0, st, K0, (int)bs, kend, n, A, lda);
-    }
-    if (nrhs > 0) {
-      // the right-hand sides' U rows: L_blk^{-1} B_blk, panel by panel
-      CT* X = A + n;
-      const unsigned gx = (unsigned)((nrhs + AW - 1) / AW);
-      for (int64_t k0 = K0; k0 < kend; k0 += NB) {
-        const int bw = (int)((kend - k0) < NB ? (kend - k0) : NB);
-        const CT* inv = reinterpret_cast<const CT*>(panel_inv(work, k0));
-        hipLaunchKernelGGL(k_solve_apply<T>, dim3(gx), dim3(256), 0, st, nrhs, inv, k0, bw, X, lda,
-                           1);
-        const int64_t below = kend - k0 - bw;
-        if (below > 0) {
-          *herr = launch_gemm_sub<T>(below, nrhs, bw, A + (k0 + bw) * lda + k0, lda, X + k0 * lda,
-                                     lda, X + (k0 + bw) * lda, lda, st);
-          if (*herr != hipSuccess) return MW_EHIP;
-        }
-      }
-    }
-    if (n > kend) {
       *herr = launch_gemm_stair<T>(n - kend, bs, A + kend * lda + K0, lda, A + K0 * lda + kend,
                                    lda, A + kend * lda + kend, lda, st);
       if (*herr != hipSuccess) return MW_EHIP;
-      if (nrhs > 0) {
-        *herr = launch_gemm_sub<T>(n - kend, nrhs, bs, A + kend * lda + K0, lda,
-                                   A + K0 * lda + n, lda, A + kend * lda + n, lda, st);
-        if (*herr != hipSuccess) return MW_EHIP;
-      }
     }
     *herr = hipGetLastError();
     if (*herr != hipSuccess) return MW_EHIP;
   }
   hipLaunchKernelGGL(k_sym_info, dim3(1), dim3(256), 0, st, (const int32_t*)flags, npan, info);
-  if (nrhs > 0) {
-    int rc = solve_bwd<T>(n, nrhs, A, lda, work, A + n, lda, st, herr);
-    if (rc) return rc;
-  }
-  *herr = hipGetLastError();
-  return *herr == hipSuccess ? MW_OK : MW_EHIP;
+  *herr = sym_block_inverses<T>(n, A, lda, work, st);
+  if (*herr != hipSuccess) return MW_EHIP;
+  return sym_solve<T>(n, nrhs, A, lda, work, A + n, lda, st, herr);
 }
 
+// X <- Z^{-1} X with the factors of mw_sysv_t (its workspace); one level
+// (n <= 4096: mw_gesv_t's pivoted factors): mw_getrs_t with ipiv.
+template <typename T>
+int mw_sytrs_t(int64_t n, int64_t nrhs, const void* Av, int64_t lda, const int32_t* ipiv,
+               const void* work, void* Xv, int64_t ldx, hipStream_t st, hipError_t* herr) {
+  typedef typename cplx_t<T>::type CT;
+  if (outer_block(n) == NB) return mw_getrs_t<T>(n, nrhs, Av, lda, ipiv, work, Xv, ldx, st, herr);
+  return sym_solve<T>(n, nrhs, reinterpret_cast<const CT*>(Av), lda, const_cast<void*>(work),
+                      reinterpret_cast<CT*>(Xv), ldx, st, herr);
+}
+
+template int mw_sytrs_t<double>(int64_t, int64_t, const void*, int64_t, const int32_t*,
+                                const void*, void*, int64_t, hipStream_t, hipError_t*);
+template int mw_sytrs_t<float>(int64_t, int64_t, const void*, int64_t, const int32_t*,
+                               const void*, void*, int64_t, hipStream_t, hipError_t*);
 template int mw_sysv_t<double>(int64_t, int64_t, void*, int64_t, int32_t*, void*, int32_t*,
                                const volatile int32_t*, hipStream_t, hipError_t*);
 template int mw_sysv_t<float>(int64_t, int64_t, void*, int64_t, int32_t*, void*, int32_t*,

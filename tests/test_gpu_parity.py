@@ -534,25 +534,30 @@ def test_compute_port_solution_replays_graph_across_k():
 
 
 @pytest.mark.parametrize("dtype", ["z", "c"])
-def test_sysv_symmetric_two_level(dtype):
+@pytest.mark.parametrize("n,p", [(4500, 70), (5000, 1100)])
+def test_sysv_symmetric_two_level(dtype, n, p):
     """mw_?sysv on a complex symmetric system past n = 4096: no row exchanges,
     each block's U rows from its L columns, the trailing update on the blocks
-    on or below the block diagonal.  A well-scaled symmetric system matches
-    LAPACK; a symmetric system whose unpivoted multipliers exceed the
-    threshold bound reports info = -1 (result not valid)."""
-    _say("sysv", dtype)
-    n, p = 4500, 70
+    on or below the block diagonal, the right-hand sides by block products
+    with the 512 x 512 diagonal blocks' inverses (a partial last block; more
+    than 1,024 right-hand sides in two column chunks).  A well-scaled
+    symmetric system matches LAPACK, and mw_?sytrs with the same factors
+    solves a second set of right-hand sides to the same accuracy; a symmetric
+    system whose unpivoted multipliers exceed the threshold bound reports
+    info = -1 (result not valid)."""
+    _say("sysv", dtype, n, p)
     rng = np.random.default_rng(11)
     R = rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))
     lib = _lib.load()
     ct = torch.complex128 if dtype == "z" else torch.complex64
-    fn = lib.mw_zsysv if dtype == "z" else lib.mw_csysv
+    fn, trs = (lib.mw_zsysv, lib.mw_zsytrs) if dtype == "z" else (lib.mw_csysv, lib.mw_csytrs)
     RHS = rng.standard_normal((n, p)) + 1j * rng.standard_normal((n, p))
+    eps = 1.1e-16 if dtype == "z" else 6e-8
     for dominant in (True, False):
         M = R + R.T + (4.0 * np.sqrt(n) * np.eye(n) if dominant else 0.0)
         aug = torch.from_numpy(np.hstack([M, RHS])).to(ct).cuda()
         ipiv = torch.empty(n, dtype=torch.int32, device="cuda")
-        work = torch.empty(int(lib.mw_getrf_workspace_bytes(n)), dtype=torch.uint8,
+        work = torch.empty(int(lib.mw_sysv_workspace_bytes(n)), dtype=torch.uint8,
                            device="cuda")
         info = torch.zeros(1, dtype=torch.int32, device="cuda")
         _gesv_dev(fn, n, p, aug, ipiv, work, info)
@@ -561,9 +566,16 @@ def test_sysv_symmetric_two_level(dtype):
             assert int(info.cpu()[0]) == 0
             X = aug[:, n:].cpu().numpy().astype(np.complex128)
             ref = np.linalg.solve(M, RHS)
-            eps = 1.1e-16 if dtype == "z" else 6e-8
-            assert rel_err(X, ref) < eps * np.linalg.cond(M) * 200
+            # the dominant system is well conditioned: |M| |M^-1| is O(10)
+            assert rel_err(X, ref) < eps * 2e4, rel_err(X, ref)
             assert np.array_equal(ipiv.cpu().numpy(), np.arange(n))
+            RHS2 = rng.standard_normal((n, 37)) + 1j * rng.standard_normal((n, 37))
+            dX2 = torch.from_numpy(RHS2).to(ct).cuda()
+            _lib.check(trs(n, 37, aug.data_ptr(), n + p, ipiv.data_ptr(), work.data_ptr(),
+                           dX2.data_ptr(), 37, _stream()), "sytrs")
+            torch.cuda.synchronize()
+            X2 = dX2.cpu().numpy().astype(np.complex128)
+            assert rel_err(X2, np.linalg.solve(M, RHS2)) < eps * 2e4
         else:
             assert int(info.cpu()[0]) == -1
 
