@@ -616,9 +616,13 @@ hipError_t launch_gemm_sub(int64_t m, int64_t n, int64_t k, const void* A, int64
   typedef typename cplx_t<T>::type CT;
   if (m <= 0 || n <= 0 || k <= 0) return hipSuccess;
   const int64_t big_tiles = ((n + GBN - 1) / GBN) * ((m + GBM - 1) / GBM);
-  if (big_tiles >= 1024 && !TA) {
+  // the 128 x 64 tiles when there are enough of them: 1024 at any K; at
+  // K >= 256, 256 (one per CU), since each tile's K loop is then long
+  // enough to hide its own loads
+  const bool big = big_tiles >= 1024 || (k >= 256 && big_tiles >= 256);
+  if (big && !TA) {
     return launch_gemm_p<T>(m, n, k, A, lda, B, ldb, C, ldc, st);
-  } else if (big_tiles >= 1024) {
+  } else if (big) {
     dim3 grid((unsigned)((n + GBN - 1) / GBN), (unsigned)((m + GBM - 1) / GBM));
     hipLaunchKernelGGL((k_gemm_sub<T, TA>), grid, dim3(256), 0, st, m, n, k, (const CT*)A, lda,
                        (const CT*)B, ldb, (CT*)C, ldc);
@@ -3059,6 +3063,28 @@ __global__ void k_move_zero(int64_t rows, int64_t cols, typename cplx_t<T>::type
   *x = z;
 }
 
+// X_b <- X_b - M_b B for one block's rows (bs <= 512) and cw <= SYW columns,
+// M_b a 512 x 512 block inverse: lower triangular (N_b, LOWER) or upper
+// (V_b), so a 64-row tile's K range stops at its last row or starts at its
+// first.  One 64 x 32 tile per workgroup, K in 32-wide chunks with the next
+// chunk's operands prefetched into registers (gemm_kc_tiles).
+template <typename T, bool LOWER>
+__global__ __launch_bounds__(256) void k_blk_trmm(int bs, int cw,
+                                                 const typename cplx_t<T>::type* __restrict__ M,
+                                                 const typename cplx_t<T>::type* __restrict__ B,
+                                                 typename cplx_t<T>::type* X, int64_t ldx) {
+  __shared__ K32Lds<T> L;
+  const int gxn = (cw + NB - 1) / NB;
+  const int r0 = (int)(blockIdx.x / gxn) * 64, c0 = (int)(blockIdx.x % gxn) * NB;
+  if (r0 >= bs) return;
+  const int kb = LOWER ? 0 : r0;
+  const int ke = LOWER ? ((r0 + 64) < bs ? r0 + 64 : bs) : bs;
+  const int m = (bs - r0) < 64 ? bs - r0 : 64;
+  const int n = (cw - c0) < NB ? cw - c0 : NB;
+  gemm_kc_tiles<T>(L, 1, 1, m, n, ke - kb, M + (int64_t)r0 * SYB + kb, SYB,
+                   B + (int64_t)kb * cw + c0, cw, X + (int64_t)r0 * ldx + c0, ldx);
+}
+
 // info[0] <- -1 (unless already set) when any panel's threshold check failed:
 // the symmetric factorisation does not pivot, so its result is not valid.
 __global__ void k_sym_info(const int32_t* flags, int64_t npan, int32_t* info) {
@@ -3472,20 +3498,27 @@ static int sym_solve(int64_t n, int64_t nrhs, const typename cplx_t<T>::type* A,
   if (nrhs <= 0) return MW_OK;
   SymWs<T> w(work, n);
   const int64_t nblk = (n + SYB - 1) / SYB;
-  auto diag = [&](const CT* M, int64_t K0, int64_t bs) -> hipError_t {
+  auto diag = [&](const CT* M, bool lower, int64_t K0, int64_t bs) -> hipError_t {
     for (int64_t c0 = 0; c0 < nrhs; c0 += SYW) {
       const int64_t cw = (nrhs - c0) < SYW ? (nrhs - c0) : SYW;
       const int64_t tot = bs * cw;
       hipLaunchKernelGGL(k_move_zero<T>, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st, bs,
                          cw, X + K0 * ldx + c0, ldx, w.tmp, cw);
-      hipError_t e = launch_gemm_sub<T>(bs, cw, bs, M, SYB, w.tmp, cw, X + K0 * ldx + c0, ldx, st);
+      const unsigned grid = (unsigned)(((bs + 63) / 64) * ((cw + NB - 1) / NB));
+      if (lower)
+        hipLaunchKernelGGL((k_blk_trmm<T, true>), dim3(grid), dim3(256), 0, st, (int)bs, (int)cw, M,
+                           (const CT*)w.tmp, X + K0 * ldx + c0, ldx);
+      else
+        hipLaunchKernelGGL((k_blk_trmm<T, false>), dim3(grid), dim3(256), 0, st, (int)bs, (int)cw,
+                           M, (const CT*)w.tmp, X + K0 * ldx + c0, ldx);
+      hipError_t e = hipGetLastError();
       if (e != hipSuccess) return e;
     }
     return hipSuccess;
   };
   for (int64_t b = 0; b < nblk; ++b) {
     const int64_t K0 = b * SYB, kend = (n - K0) < SYB ? n : K0 + SYB, bs = kend - K0;
-    *herr = diag(w.N + b * SYB * SYB, K0, bs);
+    *herr = diag(w.N + b * SYB * SYB, true, K0, bs);
     if (*herr != hipSuccess) return MW_EHIP;
     if (n > kend) {
       *herr = launch_gemm_sub<T>(n - kend, nrhs, bs, A + kend * lda + K0, lda, X + K0 * ldx, ldx,
@@ -3495,7 +3528,7 @@ static int sym_solve(int64_t n, int64_t nrhs, const typename cplx_t<T>::type* A,
   }
   for (int64_t b = nblk - 1; b >= 0; --b) {
     const int64_t K0 = b * SYB, kend = (n - K0) < SYB ? n : K0 + SYB, bs = kend - K0;
-    *herr = diag(w.V + b * SYB * SYB, K0, bs);
+    *herr = diag(w.V + b * SYB * SYB, false, K0, bs);
     if (*herr != hipSuccess) return MW_EHIP;
     if (K0 > 0) {
       *herr = launch_gemm_sub<T>(K0, nrhs, bs, A + K0, lda, X + K0 * ldx, ldx, X, ldx, st);
@@ -3533,25 +3566,43 @@ int mw_sysv_t(int64_t n, int64_t nrhs, void* Av, int64_t lda, int32_t* ipiv, voi
                      flags, npan + nodes);
   hipLaunchKernelGGL(k_zero_i32, dim3(1), dim3(64), 0, st, info, (int64_t)1);
   const double lbound = g_lbound;
+  int resident = 0;
+  if ((*herr = lookahead_resident<T>(&resident)) != hipSuccess) return MW_EHIP;
   for (int64_t K0 = 0; K0 < n; K0 += nbo) {
     const int64_t kend = (n - K0) < nbo ? n : K0 + nbo;
     const int64_t bs = kend - K0;
+    // Inside the block, the one-level look-ahead chain with the block's last
+    // column as its right edge: the block's first panel factored alone, then
+    // per panel k_panel_apply_la (L21, the block's U12, the next panel's
+    // columns updated) and k_lookahead (the next panel's factor and threshold
+    // check beside the rest of the block's K = 32 update, rows down to n).
+    {
+      const int bw0 = (int)(bs < NB ? bs : NB);
+      const int nrb0 = (int)((n - K0 - bw0 + AW - 1) / AW);
+      hipLaunchKernelGGL(k_panel_nopiv<T>, dim3(nrb0 + 1), dim3(256), 0, st, n, (const CT*)A, lda,
+                         K0, bw0, lu11, reinterpret_cast<CT*>(panel_inv(work, K0)), ipiv,
+                         flags + K0 / NB, lbound);
+    }
     for (int64_t k0 = K0; k0 < kend; k0 += NB) {
       if (cancel && *cancel) return MW_EABORT;
       const int bw = (int)((kend - k0) < NB ? (kend - k0) : NB);
-      const int64_t rest = n - k0 - bw, restc = kend - k0 - bw;
-      const int nrb = (int)((rest + AW - 1) / AW);
-      const int ncb = (int)((kend - k0 - bw + AW - 1) / AW);
+      const int64_t kn = k0 + bw, rest = n - kn, restc = kend - kn;
+      const int bwn = (int)(restc < NB ? restc : NB);
+      const int nrb = (int)((rest + LAW - 1) / LAW);
+      const int ncb = (int)((restc - bwn + LAW - 1) / LAW);
       CT* inv = reinterpret_cast<CT*>(panel_inv(work, k0));
-      hipLaunchKernelGGL(k_panel_nopiv<T>, dim3(nrb + 1), dim3(256), 0, st, n, (const CT*)A, lda,
-                         k0, bw, lu11, inv, ipiv, flags + k0 / NB, lbound);
-      hipLaunchKernelGGL(k_panel_apply<T>, dim3(nrb + ncb + 1), dim3(256), 0, st, n, kend, A, lda,
-                         k0, bw, (const CT*)inv, (const CT*)lu11, nrb, ncb);
-      if (rest > 0 && restc > 0) {
-        *herr = launch_gemm_sub<T>(rest, restc, bw, A + (k0 + bw) * lda + k0, lda,
-                                   A + k0 * lda + k0 + bw, lda, A + (k0 + bw) * lda + k0 + bw,
-                                   lda, st);
-        if (*herr != hipSuccess) return MW_EHIP;
+      hipLaunchKernelGGL(k_panel_apply_la<T>, dim3(nrb + ncb + 1), dim3(256), 0, st, n, kend, A,
+                         lda, k0, bw, bwn, (const CT*)inv, (const CT*)lu11, nrb, ncb);
+      if (bwn > 0) {
+        const int nnp = (int)((rest - bwn + AW - 1) / AW) + 1;
+        const int64_t ncols = restc - bwn;
+        const int gxn = (int)((ncols + NB - 1) / NB);
+        const int ntiles = gxn * (int)((rest + 63) / 64);
+        const int G = std::max(1, std::min(ntiles, resident - nnp - 1));
+        hipLaunchKernelGGL(k_lookahead<T>, dim3((unsigned)(nnp + 1 + (ntiles > 0 ? G : 0))),
+                           dim3(256), 0, st, n, kend, A, lda, k0, bw, bwn, (const CT*)inv, lu11,
+                           reinterpret_cast<CT*>(panel_inv(work, kn)), ipiv, flags + kn / NB,
+                           lbound, nnp, gxn > 0 ? gxn : 1, ntiles, G);
       }
     }
     if (n > kend) {
