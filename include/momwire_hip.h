@@ -313,7 +313,13 @@ int mw_cgesv(int64_t n, int64_t nrhs, void* A, int64_t lda, int32_t* ipiv,
  * sides solved by one block product per 512 rows in each direction.  Same
  * arguments and outputs as mw_?gesv except work, which holds
  * mw_sysv_workspace_bytes(n) bytes (the block inverses stay there for
- * mw_?sytrs; ipiv is returned as the identity past n = 4096).  When a panel's threshold test fails (a
+ * mw_?sytrs; ipiv is returned as the identity past n = 4096).
+ * first_row: NULL, or a HOST array of nrhs int32 giving the first row at
+ * which each right-hand side may be nonzero (its rows above are zero on
+ * entry: a port's drive column starts at its feed row); the forward pass
+ * then skips, per 512-row block, the columns that have not started by the
+ * block's end, which takes a third of its work on an array whose ports are
+ * numbered in feed-row order.  Read during the call.  When a panel's threshold test fails (a
  * multiplier beyond the bound of mw_set_pivot_threshold, or a zero pivot)
  * the panel is NOT pivoted and info[0] = -1 on return: the result is not
  * valid and the caller refactors the original matrix with mw_?gesv.
@@ -322,10 +328,10 @@ int mw_cgesv(int64_t n, int64_t nrhs, void* A, int64_t lda, int32_t* ipiv,
  */
 int mw_zsysv(int64_t n, int64_t nrhs, void* A, int64_t lda, int32_t* ipiv,
              void* work, int32_t* info, const volatile int32_t* cancel,
-             void* stream);
+             const int32_t* first_row, void* stream);
 int mw_csysv(int64_t n, int64_t nrhs, void* A, int64_t lda, int32_t* ipiv,
              void* work, int32_t* info, const volatile int32_t* cancel,
-             void* stream);
+             const int32_t* first_row, void* stream);
 
 /*
  * Workspace of mw_?sysv in bytes: mw_getrf_workspace_bytes(n) plus, past

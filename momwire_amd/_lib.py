@@ -113,8 +113,8 @@ SIGNATURES = {
     "mw_cgetrf": (ctypes.c_int, [_I64, _P, _I64, _P, _P, _P, _P, _P]),
     "mw_zgesv": (ctypes.c_int, [_I64, _I64, _P, _I64, _P, _P, _P, _P, _P]),
     "mw_cgesv": (ctypes.c_int, [_I64, _I64, _P, _I64, _P, _P, _P, _P, _P]),
-    "mw_zsysv": (ctypes.c_int, [_I64, _I64, _P, _I64, _P, _P, _P, _P, _P]),
-    "mw_csysv": (ctypes.c_int, [_I64, _I64, _P, _I64, _P, _P, _P, _P, _P]),
+    "mw_zsysv": (ctypes.c_int, [_I64, _I64, _P, _I64, _P, _P, _P, _P, _P, _P]),
+    "mw_csysv": (ctypes.c_int, [_I64, _I64, _P, _I64, _P, _P, _P, _P, _P, _P]),
     "mw_zgesv_noalloc": (ctypes.c_int, [_I64, _I64, _P, _I64, _P, _P, _P, _P]),
     "mw_zgetrs": (ctypes.c_int, [_I64, _I64, _P, _I64, _P, _P, _P, _I64, _P]),
     "mw_cgetrs": (ctypes.c_int, [_I64, _I64, _P, _I64, _P, _P, _P, _I64, _P]),

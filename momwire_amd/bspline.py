@@ -178,6 +178,11 @@ class _Device:
         self.ncols = len(cols)
         ci, cv, cmax = sparse_pack(cols)
         self.col_idx, self.col_val, self.col_max = up(ci), up(cv), cmax
+        # first row of each right-hand side (host): the symmetric solve's
+        # forward pass skips the columns that have not started yet
+        self.first_row = np.ascontiguousarray(
+            np.where(ci >= 0, ci, nb).min(axis=1).astype(np.int32) if self.ncols
+            else np.zeros(0, dtype=np.int32))
         pi, pv, pmax = sparse_pack(rows_port)
         self.port_idx, self.port_val, self.port_max = up(pi), up(pv), pmax
         if self.n_con:
@@ -615,9 +620,14 @@ class BSplineSolver(_SweptPortSolutions):
                    "right-hand sides")
         mark("solve", True)
         self._sym_used = self._sym_ok(D)
-        gesv = lib.mw_zsysv if self._sym_used else lib.mw_zgesv
-        _lib.check(gesv(nb, ncols, D.Z.data_ptr(), ld, D.ipiv.data_ptr(), D.work.data_ptr(),
-                        D.info.data_ptr(), cancel, st), "factorisation and solve")
+        if self._sym_used:
+            _lib.check(lib.mw_zsysv(nb, ncols, D.Z.data_ptr(), ld, D.ipiv.data_ptr(),
+                                    D.work.data_ptr(), D.info.data_ptr(), cancel,
+                                    D.first_row.ctypes.data, st), "factorisation and solve")
+        else:
+            _lib.check(lib.mw_zgesv(nb, ncols, D.Z.data_ptr(), ld, D.ipiv.data_ptr(),
+                                    D.work.data_ptr(), D.info.data_ptr(), cancel, st),
+                       "factorisation and solve")
         mark("solve", False)
         self._post(D, D.X.data_ptr(), ld, D.Y.data_ptr(), st)
         return D.Y[:D.n_ports, :D.n_ports], D.X[:, :D.n_ports]
@@ -803,10 +813,14 @@ class BSplineSolver(_SweptPortSolutions):
         _lib.check(lib.mw_z2c(nb, ncols, D.R64.data_ptr(), nc, D.X32.data_ptr(), D.ld32, st),
                    "B to FP32")
         self._sym_used = self._sym_ok(D)
-        gesv = lib.mw_csysv if self._sym_used else lib.mw_cgesv
-        _lib.check(gesv(nb, ncols, D.Z32.data_ptr(), D.ld32, D.ipiv32.data_ptr(),
-                        D.work32.data_ptr(), D.info.data_ptr(), cancel, st),
-                   "FP32 factorisation and solve")
+        if self._sym_used:
+            _lib.check(lib.mw_csysv(nb, ncols, D.Z32.data_ptr(), D.ld32, D.ipiv32.data_ptr(),
+                                    D.work32.data_ptr(), D.info.data_ptr(), cancel,
+                                    D.first_row.ctypes.data, st), "FP32 factorisation and solve")
+        else:
+            _lib.check(lib.mw_cgesv(nb, ncols, D.Z32.data_ptr(), D.ld32, D.ipiv32.data_ptr(),
+                                    D.work32.data_ptr(), D.info.data_ptr(), cancel, st),
+                       "FP32 factorisation and solve")
         _lib.check(lib.mw_c2z(nb, ncols, D.X32.data_ptr(), D.ld32, D.X64.data_ptr(), nc, 0, st),
                    "X to FP64")
         mark("solve32", False)

@@ -28,7 +28,8 @@ int mw_gesv_t(int64_t n, int64_t nrhs, void* A, int64_t lda, int32_t* ipiv, void
               int32_t* info, const volatile int32_t* cancel, hipStream_t st, hipError_t* herr);
 template <typename T>
 int mw_sysv_t(int64_t n, int64_t nrhs, void* A, int64_t lda, int32_t* ipiv, void* work,
-              int32_t* info, const volatile int32_t* cancel, hipStream_t st, hipError_t* herr);
+              int32_t* info, const volatile int32_t* cancel, const int32_t* first_row,
+              hipStream_t st, hipError_t* herr);
 template <typename T>
 int mw_getrs_t(int64_t n, int64_t nrhs, const void* A, int64_t lda, const int32_t* ipiv,
                const void* work, void* X, int64_t ldx, hipStream_t st, hipError_t* herr);
@@ -420,26 +421,31 @@ int mw_cgesv(int64_t n, int64_t nrhs, void* A, int64_t lda, int32_t* ipiv, void*
 }
 
 static int sysv_common(bool dbl, int64_t n, int64_t nrhs, void* A, int64_t lda, int32_t* ipiv,
-                       void* work, int32_t* info, const volatile int32_t* cancel, void* stream) {
+                       void* work, int32_t* info, const volatile int32_t* cancel,
+                       const int32_t* first_row, void* stream) {
   g_err[0] = 0;
   if (n <= 0 || nrhs < 0) return fail(MW_EINVAL, "%sbad sizes n=%lld nrhs=%lld", "", n, nrhs);
   if (!A || !ipiv || !work || !info) return fail(MW_EINVAL, "%sNULL argument");
   if (lda < n + nrhs) return fail(MW_EINVAL, "%slda %lld < n + nrhs %lld", "", lda, n + nrhs);
   hipError_t herr = hipSuccess;
-  int rc = dbl ? mw_sysv_t<double>(n, nrhs, A, lda, ipiv, work, info, cancel, (hipStream_t)stream, &herr)
-               : mw_sysv_t<float>(n, nrhs, A, lda, ipiv, work, info, cancel, (hipStream_t)stream, &herr);
+  int rc = dbl ? mw_sysv_t<double>(n, nrhs, A, lda, ipiv, work, info, cancel, first_row,
+                                   (hipStream_t)stream, &herr)
+               : mw_sysv_t<float>(n, nrhs, A, lda, ipiv, work, info, cancel, first_row,
+                                  (hipStream_t)stream, &herr);
   if (rc == MW_EHIP) return hip_status(herr, "sysv");
   if (rc == MW_EABORT) return fail(MW_EABORT, "%sfactorisation cancelled");
   return rc;
 }
 
 int mw_zsysv(int64_t n, int64_t nrhs, void* A, int64_t lda, int32_t* ipiv, void* work,
-             int32_t* info, const volatile int32_t* cancel, void* stream) {
-  return sysv_common(true, n, nrhs, A, lda, ipiv, work, info, cancel, stream);
+             int32_t* info, const volatile int32_t* cancel, const int32_t* first_row,
+             void* stream) {
+  return sysv_common(true, n, nrhs, A, lda, ipiv, work, info, cancel, first_row, stream);
 }
 int mw_csysv(int64_t n, int64_t nrhs, void* A, int64_t lda, int32_t* ipiv, void* work,
-             int32_t* info, const volatile int32_t* cancel, void* stream) {
-  return sysv_common(false, n, nrhs, A, lda, ipiv, work, info, cancel, stream);
+             int32_t* info, const volatile int32_t* cancel, const int32_t* first_row,
+             void* stream) {
+  return sysv_common(false, n, nrhs, A, lda, ipiv, work, info, cancel, first_row, stream);
 }
 
 static int getrs_common(bool dbl, int64_t n, int64_t nrhs, const void* A, int64_t lda,

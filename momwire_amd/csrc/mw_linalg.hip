@@ -3489,18 +3489,23 @@ static hipError_t sym_block_inverses(int64_t n, const typename cplx_t<T>::type* 
 // X <- Z^{-1} X with the symmetric factors (Z = L D L^T, U = D L^T) and the
 // block inverses: per 512-row block one product with -N_b (forward) or -V_b
 // (backward) on the block's rows, moved aside first, and one K = 512 update
-// of the rows below (forward) or above (backward).
+// of the rows below (forward) or above (backward).  first_row (host, nrhs
+// entries, or NULL): the first row where each right-hand side may be
+// nonzero.  The forward pass then works, in block b, on the columns up to the
+// last one that has started by the block's end: the others are still zero
+// there, and L^{-1} keeps them so (a port's drive column starts at its feed
+// row, so the forward pass does a third of the dense work on an array).
 template <typename T>
 static int sym_solve(int64_t n, int64_t nrhs, const typename cplx_t<T>::type* A, int64_t lda,
-                     void* work, typename cplx_t<T>::type* X, int64_t ldx, hipStream_t st,
-                     hipError_t* herr) {
+                     void* work, typename cplx_t<T>::type* X, int64_t ldx,
+                     const int32_t* first_row, hipStream_t st, hipError_t* herr) {
   typedef typename cplx_t<T>::type CT;
   if (nrhs <= 0) return MW_OK;
   SymWs<T> w(work, n);
   const int64_t nblk = (n + SYB - 1) / SYB;
-  auto diag = [&](const CT* M, bool lower, int64_t K0, int64_t bs) -> hipError_t {
-    for (int64_t c0 = 0; c0 < nrhs; c0 += SYW) {
-      const int64_t cw = (nrhs - c0) < SYW ? (nrhs - c0) : SYW;
+  auto diag = [&](const CT* M, bool lower, int64_t K0, int64_t bs, int64_t ncols) -> hipError_t {
+    for (int64_t c0 = 0; c0 < ncols; c0 += SYW) {
+      const int64_t cw = (ncols - c0) < SYW ? (ncols - c0) : SYW;
       const int64_t tot = bs * cw;
       hipLaunchKernelGGL(k_move_zero<T>, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st, bs,
                          cw, X + K0 * ldx + c0, ldx, w.tmp, cw);
@@ -3518,17 +3523,24 @@ static int sym_solve(int64_t n, int64_t nrhs, const typename cplx_t<T>::type* A,
   };
   for (int64_t b = 0; b < nblk; ++b) {
     const int64_t K0 = b * SYB, kend = (n - K0) < SYB ? n : K0 + SYB, bs = kend - K0;
-    *herr = diag(w.N + b * SYB * SYB, true, K0, bs);
+    int64_t act = nrhs;
+    if (first_row) {
+      act = 0;
+      for (int64_t c = 0; c < nrhs; ++c)
+        if (first_row[c] < kend) act = c + 1;
+    }
+    if (act == 0) continue;
+    *herr = diag(w.N + b * SYB * SYB, true, K0, bs, act);
     if (*herr != hipSuccess) return MW_EHIP;
     if (n > kend) {
-      *herr = launch_gemm_sub<T>(n - kend, nrhs, bs, A + kend * lda + K0, lda, X + K0 * ldx, ldx,
+      *herr = launch_gemm_sub<T>(n - kend, act, bs, A + kend * lda + K0, lda, X + K0 * ldx, ldx,
                                  X + kend * ldx, ldx, st);
       if (*herr != hipSuccess) return MW_EHIP;
     }
   }
   for (int64_t b = nblk - 1; b >= 0; --b) {
     const int64_t K0 = b * SYB, kend = (n - K0) < SYB ? n : K0 + SYB, bs = kend - K0;
-    *herr = diag(w.V + b * SYB * SYB, false, K0, bs);
+    *herr = diag(w.V + b * SYB * SYB, false, K0, bs, nrhs);
     if (*herr != hipSuccess) return MW_EHIP;
     if (K0 > 0) {
       *herr = launch_gemm_sub<T>(K0, nrhs, bs, A + K0, lda, X + K0 * ldx, ldx, X, ldx, st);
@@ -3548,10 +3560,11 @@ static int sym_solve(int64_t n, int64_t nrhs, const typename cplx_t<T>::type* A,
 // right-hand sides (columns [n, n + nrhs) of A) by block products.  A panel
 // whose threshold check fails is not pivoted: info[0] = -1 and the result is
 // not valid (refactor the original matrix with mw_gesv_t).  One level:
-// mw_gesv_t.  work: mw_sysv_ws_bytes(n).
+// mw_gesv_t.  work: mw_sysv_ws_bytes(n).  first_row: see sym_solve.
 template <typename T>
 int mw_sysv_t(int64_t n, int64_t nrhs, void* Av, int64_t lda, int32_t* ipiv, void* work,
-              int32_t* info, const volatile int32_t* cancel, hipStream_t st, hipError_t* herr) {
+              int32_t* info, const volatile int32_t* cancel, const int32_t* first_row,
+              hipStream_t st, hipError_t* herr) {
   typedef typename cplx_t<T>::type CT;
   const int64_t nbo = outer_block(n);
   if (nbo == NB) return mw_gesv_t<T>(n, nrhs, Av, lda, ipiv, work, info, cancel, st, herr);
@@ -3619,7 +3632,7 @@ int mw_sysv_t(int64_t n, int64_t nrhs, void* Av, int64_t lda, int32_t* ipiv, voi
   hipLaunchKernelGGL(k_sym_info, dim3(1), dim3(256), 0, st, (const int32_t*)flags, npan, info);
   *herr = sym_block_inverses<T>(n, A, lda, work, st);
   if (*herr != hipSuccess) return MW_EHIP;
-  return sym_solve<T>(n, nrhs, A, lda, work, A + n, lda, st, herr);
+  return sym_solve<T>(n, nrhs, A, lda, work, A + n, lda, first_row, st, herr);
 }
 
 // X <- Z^{-1} X with the factors of mw_sysv_t (its workspace); one level
@@ -3630,7 +3643,7 @@ int mw_sytrs_t(int64_t n, int64_t nrhs, const void* Av, int64_t lda, const int32
   typedef typename cplx_t<T>::type CT;
   if (outer_block(n) == NB) return mw_getrs_t<T>(n, nrhs, Av, lda, ipiv, work, Xv, ldx, st, herr);
   return sym_solve<T>(n, nrhs, reinterpret_cast<const CT*>(Av), lda, const_cast<void*>(work),
-                      reinterpret_cast<CT*>(Xv), ldx, st, herr);
+                      reinterpret_cast<CT*>(Xv), ldx, nullptr, st, herr);
 }
 
 template int mw_sytrs_t<double>(int64_t, int64_t, const void*, int64_t, const int32_t*,
@@ -3638,9 +3651,11 @@ template int mw_sytrs_t<double>(int64_t, int64_t, const void*, int64_t, const in
 template int mw_sytrs_t<float>(int64_t, int64_t, const void*, int64_t, const int32_t*,
                                const void*, void*, int64_t, hipStream_t, hipError_t*);
 template int mw_sysv_t<double>(int64_t, int64_t, void*, int64_t, int32_t*, void*, int32_t*,
-                               const volatile int32_t*, hipStream_t, hipError_t*);
+                               const volatile int32_t*, const int32_t*, hipStream_t,
+                               hipError_t*);
 template int mw_sysv_t<float>(int64_t, int64_t, void*, int64_t, int32_t*, void*, int32_t*,
-                              const volatile int32_t*, hipStream_t, hipError_t*);
+                               const volatile int32_t*, const int32_t*, hipStream_t,
+                               hipError_t*);
 template int mw_getrf_t<double>(int64_t, int64_t, void*, int64_t, int32_t*, void*, int32_t*,
                                 const volatile int32_t*, hipStream_t, hipError_t*);
 template int mw_getrf_t<float>(int64_t, int64_t, void*, int64_t, int32_t*, void*, int32_t*,

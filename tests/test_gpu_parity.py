@@ -127,9 +127,9 @@ def test_gesv_augmented(dtype, n, p):
     assert rel_err(X, ref) < tol
 
 
-def _gesv_dev(fn, n, p, aug, ipiv, work, info):
+def _gesv_dev(fn, n, p, aug, ipiv, work, info, *extra):
     _lib.check(fn(n, p, aug.data_ptr(), n + p, ipiv.data_ptr(), work.data_ptr(), info.data_ptr(),
-                  None, _stream()), "gesv")
+                  None, *extra, _stream()), "gesv")
 
 
 def test_gesv_two_level_graph_replay():
@@ -541,7 +541,8 @@ def test_sysv_symmetric_two_level(dtype, n, p):
     on or below the block diagonal, the right-hand sides by block products
     with the 512 x 512 diagonal blocks' inverses (a partial last block; more
     than 1,024 right-hand sides in two column chunks).  A well-scaled
-    symmetric system matches LAPACK, and mw_?sytrs with the same factors
+    symmetric system matches LAPACK, with and without the first-row hint of
+    staircase right-hand sides, and mw_?sytrs with the same factors
     solves a second set of right-hand sides to the same accuracy; a symmetric
     system whose unpivoted multipliers exceed the threshold bound reports
     info = -1 (result not valid)."""
@@ -553,14 +554,26 @@ def test_sysv_symmetric_two_level(dtype, n, p):
     fn, trs = (lib.mw_zsysv, lib.mw_zsytrs) if dtype == "z" else (lib.mw_csysv, lib.mw_csytrs)
     RHS = rng.standard_normal((n, p)) + 1j * rng.standard_normal((n, p))
     eps = 1.1e-16 if dtype == "z" else 6e-8
+    # a staircase of first rows (the first three columns dense), as a port
+    # solve's drive columns: zero above each column's first row
+    first = np.sort(rng.integers(0, n, p)).astype(np.int32)
+    first[:3] = 0
+    RHS_st = np.where(np.arange(n)[:, None] >= first[None, :], RHS, 0)
     for dominant in (True, False):
         M = R + R.T + (4.0 * np.sqrt(n) * np.eye(n) if dominant else 0.0)
-        aug = torch.from_numpy(np.hstack([M, RHS])).to(ct).cuda()
         ipiv = torch.empty(n, dtype=torch.int32, device="cuda")
         work = torch.empty(int(lib.mw_sysv_workspace_bytes(n)), dtype=torch.uint8,
                            device="cuda")
         info = torch.zeros(1, dtype=torch.int32, device="cuda")
-        _gesv_dev(fn, n, p, aug, ipiv, work, info)
+        if dominant:
+            aug = torch.from_numpy(np.hstack([M, RHS_st])).to(ct).cuda()
+            _gesv_dev(fn, n, p, aug, ipiv, work, info, first.ctypes.data)
+            torch.cuda.synchronize()
+            assert int(info.cpu()[0]) == 0
+            X = aug[:, n:].cpu().numpy().astype(np.complex128)
+            assert rel_err(X, np.linalg.solve(M, RHS_st)) < eps * 2e4
+        aug = torch.from_numpy(np.hstack([M, RHS])).to(ct).cuda()
+        _gesv_dev(fn, n, p, aug, ipiv, work, info, None)
         torch.cuda.synchronize()
         if dominant:
             assert int(info.cpu()[0]) == 0
