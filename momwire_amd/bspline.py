@@ -897,7 +897,7 @@ class BSplineSolver(_SweptPortSolutions):
             if torch.cuda.is_current_stream_capturing():
                 self._capture_needs_check = True   # solve_graph checks after each replay
             else:
-                self._mixed_check(D, st)
+                self._mixed_check(D, st, mark)
             if history is not None:
                 history.append(("adjoint", D.Y[:npt, :npt].clone()))
             return D.Y[:npt, :npt], D.X64[:, :npt]
@@ -938,17 +938,22 @@ class BSplineSolver(_SweptPortSolutions):
         _lib.check(lib.mw_zgemm_tn_add(npt, npt, D.nb, D.X64.data_ptr(), nc, D.R64.data_ptr(),
                                        nc, yptr, max(npt, 1), st), "adjoint readout")
 
-    def _mixed_check(self, D, st):
+    def _mixed_check(self, D, st, mark=None):
         """Read the one-residual readout's check (a host synchronisation);
         when it failed, read out again from a fresh FP64 residual at the
-        corrected coefficients (an error quadratic in theirs)."""
+        corrected coefficients (an error quadratic in theirs; timed as the
+        stage "readout2" when ``mark`` is the solve's stage marker)."""
         (gate,), stats = _to_host(D.gate, D.gstats)
         self.mixed_readout = {"delta": float(stats[0]), "max_ctr": float(stats[1]),
                               "max_y": float(stats[2]), "estimate": float(stats[3]),
                               "tol": READOUT_TOL, "second_residual": bool(gate)}
         if gate:
+            if mark is not None:
+                mark("readout2", True)
             self._fp64_residual(D, st)
             self._adjoint_readout(D, D.Y.data_ptr(), st)
+            if mark is not None:
+                mark("readout2", False)
 
     def _solve(self, k=None, timers=None):
         if self.precision == "mixed":

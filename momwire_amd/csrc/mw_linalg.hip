@@ -3611,7 +3611,10 @@ int mw_sysv_t(int64_t n, int64_t nrhs, void* Av, int64_t lda, int32_t* ipiv, voi
         const int64_t ncols = restc - bwn;
         const int gxn = (int)((ncols + NB - 1) / NB);
         const int ntiles = gxn * (int)((rest + 63) / 64);
-        const int G = std::max(1, std::min(ntiles, resident - nnp - 1));
+        // update blocks beside the factor's: what stays resident with them, and
+        // at least half the device's worth when the factor's blocks alone fill
+        // it (tall panels: they are then dispatched as those finish)
+        const int G = std::max(1, std::min(ntiles, std::max(resident - nnp - 1, resident / 2)));
         hipLaunchKernelGGL(k_lookahead<T>, dim3((unsigned)(nnp + 1 + (ntiles > 0 ? G : 0))),
                            dim3(256), 0, st, n, kend, A, lda, k0, bw, bwn, (const CT*)inv, lu11,
                            reinterpret_cast<CT*>(panel_inv(work, kn)), ipiv, flags + kn / NB,
