@@ -636,7 +636,7 @@ class BSplineSolver(_SweptPortSolutions):
         """Whether the solve takes the symmetric factorisation (mw_?sysv, no
         row exchanges, half the trailing update): Z = Z^T exactly (one radius
         for every wire, free space, no wire loading: the fill computes the
-        upper tiles and mirrors them) at two-level sizes, unless a solve of
+        upper tiles and mirrors them) past 4,096 unknowns, unless a solve of
         this solver has already needed row exchanges."""
         return (bool(self._mesh.symmetric) and self.ground_z is None
                 and not self._loading_active and not self.extended_kernel

@@ -303,9 +303,7 @@ int mw_fill_z_far32(const mw_mesh* mesh, double k, double omega, double eps, dou
 }
 
 int64_t mw_getrf_workspace_bytes(int64_t n) { return n > 0 ? mw_getrf_ws_bytes(n) : 0; }
-int64_t mw_sysv_workspace_bytes(int64_t n) {
-  return n > 4096 ? mw_sysv_ws_bytes(n) : mw_getrf_workspace_bytes(n);
-}
+int64_t mw_sysv_workspace_bytes(int64_t n) { return n > 0 ? mw_sysv_ws_bytes(n) : 0; }
 
 int mw_set_pivot_threshold(double bound) {
   g_err[0] = 0;

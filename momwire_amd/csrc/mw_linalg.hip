@@ -2949,18 +2949,24 @@ __global__ __launch_bounds__(256) void k_sym_u_rows(int64_t K0, int bs, int64_t 
 // lower-left quadrants of different levels do not overlap) until V_b is
 // written at the end.
 // ---------------------------------------------------------------------------
-constexpr int SYB = 512;  // outer block of the symmetric factorisation
+constexpr int SYB = 512;  // largest outer block of the symmetric factorisation (storage slot)
+
+// Outer block of the symmetric factorisation: 512 past 4096 unknowns (the
+// LU's two-level block); none below, where the one-level look-ahead LU is
+// faster (2,624 unknowns: 5.4 ms per port solve against 6.7 ms for the
+// symmetric path with 256-column blocks).
+static int64_t sym_block(int64_t n) { return n > 4096 ? 512 : 0; }
 
 // N_b <- the negated panel inverses on its 32 x 32 diagonal blocks, zero
 // elsewhere; V_b <- 0.  Grid: (512 rows / 1, nblk); 256 threads, 2 columns each.
 template <typename T>
-__global__ __launch_bounds__(256) void k_blkinv_init(int64_t n, const void* work,
+__global__ __launch_bounds__(256) void k_blkinv_init(int64_t n, int bz, const void* work,
                                                     typename cplx_t<T>::type* N,
                                                     typename cplx_t<T>::type* V) {
   typedef typename cplx_t<T>::type CT;
   const int r = blockIdx.x, b = blockIdx.y;
-  const int64_t K0 = (int64_t)b * SYB;
-  const int bs = (int)((n - K0) < SYB ? (n - K0) : SYB);
+  const int64_t K0 = (int64_t)b * bz;
+  const int bs = (int)((n - K0) < bz ? (n - K0) : bz);
   CT* Nr = N + ((int64_t)b * SYB + r) * SYB;
   CT* Vr = V + ((int64_t)b * SYB + r) * SYB;
   const CT* li = reinterpret_cast<const CT*>(reinterpret_cast<const char*>(work) +
@@ -2986,19 +2992,19 @@ __global__ __launch_bounds__(256) void k_blkinv_init(int64_t n, const void* work
 // PHASE 1: T = 0 - C N_P into V_b; PHASE 2: N_b's lower-left block = 0 - N_Q T.
 // Grid: nblk * (256 / s) pairs * tiles of the s x s output (64 x 32 tiles).
 template <typename T, int PHASE>
-__global__ __launch_bounds__(256) void k_blkinv_level(int s, int64_t n,
+__global__ __launch_bounds__(256) void k_blkinv_level(int s, int64_t n, int bz,
                                                      const typename cplx_t<T>::type* A,
                                                      int64_t lda, typename cplx_t<T>::type* N,
                                                      typename cplx_t<T>::type* V) {
   __shared__ GemmSmallLds<T> L;
   const int tx = (s + SBN - 1) / SBN, ty = (s + SBM - 1) / SBM;
-  const int pairs = SYB / (2 * s);
+  const int pairs = bz / (2 * s);
   int id = blockIdx.x;
   const int tile = id % (tx * ty);
   id /= tx * ty;
   const int p = id % pairs, b = id / pairs;
-  const int64_t K0 = (int64_t)b * SYB;
-  const int bs = (int)((n - K0) < SYB ? (n - K0) : SYB);
+  const int64_t K0 = (int64_t)b * bz;
+  const int bs = (int)((n - K0) < bz ? (n - K0) : bz);
   const int a0 = 2 * p * s, d0 = a0 + s;
   if (d0 >= bs) return;
   const int mD = (bs - d0) < s ? (bs - d0) : s;
@@ -3016,16 +3022,17 @@ __global__ __launch_bounds__(256) void k_blkinv_level(int s, int64_t n,
 // V_b[i][j] = N_b[j][i] / D_j (D_j = U's diagonal, A[K0 + j][K0 + j]): 64 x 64
 // tiles through LDS.  Grid: (8, 8, nblk).
 template <typename T>
-__global__ __launch_bounds__(256) void k_blkinv_vt(int64_t n, const typename cplx_t<T>::type* A,
-                                                  int64_t lda, const typename cplx_t<T>::type* N,
+__global__ __launch_bounds__(256) void k_blkinv_vt(int64_t n, int bz,
+                                                  const typename cplx_t<T>::type* A, int64_t lda,
+                                                  const typename cplx_t<T>::type* N,
                                                   typename cplx_t<T>::type* V) {
   typedef typename cplx_t<T>::type CT;
   __shared__ CT tile[64][65];
   __shared__ CT dr[64];
   const int tid = threadIdx.x;
   const int j0 = blockIdx.x * 64, i0 = blockIdx.y * 64, b = blockIdx.z;
-  const int64_t K0 = (int64_t)b * SYB;
-  const int bs = (int)((n - K0) < SYB ? (n - K0) : SYB);
+  const int64_t K0 = (int64_t)b * bz;
+  const int bs = (int)((n - K0) < bz ? (n - K0) : bz);
   const CT* Nb = N + (int64_t)b * SYB * SYB;
   CT* Vb = V + (int64_t)b * SYB * SYB;
   if (tid < 64) {
@@ -3448,7 +3455,9 @@ int mw_gesv_t(int64_t n, int64_t nrhs, void* Av, int64_t lda, int32_t* ipiv, voi
 // so one size serves FP64 and FP32).
 constexpr int64_t SYW = 1024;  // right-hand-side columns per block product
 int64_t mw_sysv_ws_bytes(int64_t n) {
-  const int64_t nblk = (n + SYB - 1) / SYB;
+  const int64_t bz = sym_block(n);
+  if (bz == 0) return mw_getrf_ws_bytes(n);
+  const int64_t nblk = (n + bz - 1) / bz;
   return mw_getrf_ws_bytes(n) + 2 * nblk * SYB * SYB * 16 + SYB * SYW * 16;
 }
 
@@ -3457,7 +3466,7 @@ struct SymWs {
   typedef typename cplx_t<T>::type CT;
   CT *N, *V, *tmp;
   SymWs(void* work, int64_t n) {
-    const int64_t nblk = (n + SYB - 1) / SYB;
+    const int64_t bz = sym_block(n), nblk = (n + bz - 1) / bz;
     char* base = reinterpret_cast<char*>(work) + mw_getrf_ws_bytes(n);
     N = reinterpret_cast<CT*>(base);
     V = reinterpret_cast<CT*>(base + nblk * SYB * SYB * 16);
@@ -3469,20 +3478,21 @@ struct SymWs {
 template <typename T>
 static hipError_t sym_block_inverses(int64_t n, const typename cplx_t<T>::type* A, int64_t lda,
                                      void* work, hipStream_t st) {
-  const int64_t nblk = (n + SYB - 1) / SYB;
+  const int bz = (int)sym_block(n);
+  const int64_t nblk = (n + bz - 1) / bz;
   SymWs<T> w(work, n);
-  hipLaunchKernelGGL(k_blkinv_init<T>, dim3(SYB, (unsigned)nblk), dim3(256), 0, st, n,
+  hipLaunchKernelGGL(k_blkinv_init<T>, dim3(bz, (unsigned)nblk), dim3(256), 0, st, n, bz,
                      (const void*)work, w.N, w.V);
-  for (int s = NB; s < SYB; s *= 2) {
+  for (int s = NB; s < bz; s *= 2) {
     const int tiles = ((s + SBN - 1) / SBN) * ((s + SBM - 1) / SBM);
-    const unsigned grid = (unsigned)(nblk * (SYB / (2 * s)) * tiles);
-    hipLaunchKernelGGL((k_blkinv_level<T, 1>), dim3(grid), dim3(256), 0, st, s, n, A, lda, w.N,
-                       w.V);
-    hipLaunchKernelGGL((k_blkinv_level<T, 2>), dim3(grid), dim3(256), 0, st, s, n, A, lda, w.N,
-                       w.V);
+    const unsigned grid = (unsigned)(nblk * (bz / (2 * s)) * tiles);
+    hipLaunchKernelGGL((k_blkinv_level<T, 1>), dim3(grid), dim3(256), 0, st, s, n, bz, A, lda,
+                       w.N, w.V);
+    hipLaunchKernelGGL((k_blkinv_level<T, 2>), dim3(grid), dim3(256), 0, st, s, n, bz, A, lda,
+                       w.N, w.V);
   }
-  hipLaunchKernelGGL(k_blkinv_vt<T>, dim3(SYB / 64, SYB / 64, (unsigned)nblk), dim3(256), 0, st,
-                     n, A, lda, (const typename cplx_t<T>::type*)w.N, w.V);
+  hipLaunchKernelGGL(k_blkinv_vt<T>, dim3(bz / 64, bz / 64, (unsigned)nblk), dim3(256), 0, st,
+                     n, bz, A, lda, (const typename cplx_t<T>::type*)w.N, w.V);
   return hipGetLastError();
 }
 
@@ -3502,7 +3512,7 @@ static int sym_solve(int64_t n, int64_t nrhs, const typename cplx_t<T>::type* A,
   typedef typename cplx_t<T>::type CT;
   if (nrhs <= 0) return MW_OK;
   SymWs<T> w(work, n);
-  const int64_t nblk = (n + SYB - 1) / SYB;
+  const int64_t bz = sym_block(n), nblk = (n + bz - 1) / bz;
   auto diag = [&](const CT* M, bool lower, int64_t K0, int64_t bs, int64_t ncols) -> hipError_t {
     for (int64_t c0 = 0; c0 < ncols; c0 += SYW) {
       const int64_t cw = (ncols - c0) < SYW ? (ncols - c0) : SYW;
@@ -3522,7 +3532,7 @@ static int sym_solve(int64_t n, int64_t nrhs, const typename cplx_t<T>::type* A,
     return hipSuccess;
   };
   for (int64_t b = 0; b < nblk; ++b) {
-    const int64_t K0 = b * SYB, kend = (n - K0) < SYB ? n : K0 + SYB, bs = kend - K0;
+    const int64_t K0 = b * bz, kend = (n - K0) < bz ? n : K0 + bz, bs = kend - K0;
     int64_t act = nrhs;
     if (first_row) {
       act = 0;
@@ -3539,7 +3549,7 @@ static int sym_solve(int64_t n, int64_t nrhs, const typename cplx_t<T>::type* A,
     }
   }
   for (int64_t b = nblk - 1; b >= 0; --b) {
-    const int64_t K0 = b * SYB, kend = (n - K0) < SYB ? n : K0 + SYB, bs = kend - K0;
+    const int64_t K0 = b * bz, kend = (n - K0) < bz ? n : K0 + bz, bs = kend - K0;
     *herr = diag(w.V + b * SYB * SYB, false, K0, bs, nrhs);
     if (*herr != hipSuccess) return MW_EHIP;
     if (K0 > 0) {
@@ -3566,9 +3576,8 @@ int mw_sysv_t(int64_t n, int64_t nrhs, void* Av, int64_t lda, int32_t* ipiv, voi
               int32_t* info, const volatile int32_t* cancel, const int32_t* first_row,
               hipStream_t st, hipError_t* herr) {
   typedef typename cplx_t<T>::type CT;
-  const int64_t nbo = outer_block(n);
-  if (nbo == NB) return mw_gesv_t<T>(n, nrhs, Av, lda, ipiv, work, info, cancel, st, herr);
-  static_assert(SYB == 512, "the symmetric path's blocks are the two-level outer blocks");
+  const int64_t nbo = sym_block(n);
+  if (nbo == 0) return mw_gesv_t<T>(n, nrhs, Av, lda, ipiv, work, info, cancel, st, herr);
   CT* A = reinterpret_cast<CT*>(Av);
   const int64_t npan = (n + NB - 1) / NB;
   const int64_t nodes = tour_nodes(n);
@@ -3644,7 +3653,7 @@ template <typename T>
 int mw_sytrs_t(int64_t n, int64_t nrhs, const void* Av, int64_t lda, const int32_t* ipiv,
                const void* work, void* Xv, int64_t ldx, hipStream_t st, hipError_t* herr) {
   typedef typename cplx_t<T>::type CT;
-  if (outer_block(n) == NB) return mw_getrs_t<T>(n, nrhs, Av, lda, ipiv, work, Xv, ldx, st, herr);
+  if (sym_block(n) == 0) return mw_getrs_t<T>(n, nrhs, Av, lda, ipiv, work, Xv, ldx, st, herr);
   return sym_solve<T>(n, nrhs, reinterpret_cast<const CT*>(Av), lda, const_cast<void*>(work),
                       reinterpret_cast<CT*>(Xv), ldx, nullptr, st, herr);
 }

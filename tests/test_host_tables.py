@@ -154,6 +154,22 @@ def test_current_slopes_match_spline_derivative(degree):
         assert np.abs(fd - mid_sl).max() <= 1e-6 * np.abs(mid_sl).max()
 
 
+def test_sysv_workspace_holds_the_block_inverses():
+    """mw_sysv_workspace_bytes covers the LU workspace plus, for every size
+    the symmetric path factors (past 4,096 unknowns, 512-column outer
+    blocks), two 512 x 512 complex slots per outer block and the 512 x 1024
+    staging block; a workspace sized for the LU alone would let the
+    block-inverse kernels write past its end."""
+    if not os.path.exists(_lib.LIB_PATH):
+        pytest.skip("libmomwire_hip.so not built")
+    lib = _lib.load()
+    for n in (100, 1024, 1025, 2000, 2624, 4096, 4097, 15360, 41984):
+        getrf = lib.mw_getrf_workspace_bytes(n)
+        bz = 512 if n > 4096 else 0
+        need = getrf + ((2 * -(-n // bz) * 512 * 512 * 16 + 512 * 1024 * 16) if bz else 0)
+        assert lib.mw_sysv_workspace_bytes(n) >= need, n
+
+
 def test_sweep_plan_rejects_nan_pivot_bound():
     """The sweep plan refuses a NaN pivot threshold before touching anything
     (so no GPU is needed)."""
