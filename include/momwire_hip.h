@@ -319,10 +319,11 @@ int mw_cgesv(int64_t n, int64_t nrhs, void* A, int64_t lda, int32_t* ipiv,
  * entry: a port's drive column starts at its feed row); the forward pass
  * then skips, per 512-row block, the columns that have not started by the
  * block's end, which takes a third of its work on an array whose ports are
- * numbered in feed-row order.  Read during the call.  When a panel's threshold test fails (a
- * multiplier beyond the bound of mw_set_pivot_threshold, or a zero pivot)
- * the panel is NOT pivoted and info[0] = -1 on return: the result is not
- * valid and the caller refactors the original matrix with mw_?gesv.
+ * numbered in feed-row order.  Read during the call.
+ * When a panel's threshold test fails (a multiplier beyond the bound of
+ * mw_set_pivot_threshold, or a zero pivot) the panel is NOT pivoted and
+ * info[0] = -1 on return: the result is not valid and the caller refactors
+ * the original matrix with mw_?gesv.
  * n <= 4096: mw_?gesv itself.  The reference has no counterpart: it solves
  * Z with scipy.linalg.solve (bspline.py:2971-2989) whatever its symmetry.
  */
