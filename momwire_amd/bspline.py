@@ -636,11 +636,11 @@ class BSplineSolver(_SweptPortSolutions):
         """Whether the solve takes the symmetric factorisation (mw_?sysv, no
         row exchanges, half the trailing update): Z = Z^T exactly (one radius
         for every wire, free space, no wire loading: the fill computes the
-        upper tiles and mirrors them) past 4,096 unknowns, unless a solve of
-        this solver has already needed row exchanges."""
+        upper tiles and mirrors them), unless a solve of this solver has
+        already needed row exchanges."""
         return (bool(self._mesh.symmetric) and self.ground_z is None
                 and not self._loading_active and not self.extended_kernel
-                and D.nb > 4096 and not self._sym_failed)
+                and not self._sym_failed)
 
     def _solve_valid(self, graph=False):
         """The device solve (a graph replay when ``graph``), repeated with the

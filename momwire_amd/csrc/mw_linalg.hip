@@ -1426,13 +1426,22 @@ template <typename T>
 struct LaLds {
   T As[LAW][65];                               // A21, then L21, as reals
   typename cplx_t<T>::type Bs[2][NB][NB + 1];  // U11^{-1} then U12n | A12n
-  T Ls[NB][65];                                // L11^{-1} as reals
+  T Ls[NB][65];                                // L11^{-1} (symmetric: A21 top) as reals
+  typename cplx_t<T>::type D[NB];              // symmetric: U11's diagonal
 };
 
-template <typename T>
+//
+// SYM (Z = Z^T, no row exchanges: U = D L^T): the trailing update keeps only
+// the lower part, the panel-diagonal blocks and the block right of each (the
+// next panel's A12) up to date; so every block but the first, whose rows are
+// the next panel's, also stores the U12 columns of its own rows, D L21^T, in
+// place of the stale upper part (the first block's columns are the next
+// panel's U12, which k_lookahead forms from A12 as without symmetry).
+template <typename T, bool SYM = false>
 MW_DEV void l21_lookahead_body(LaLds<T>& L, int b, int64_t n, typename cplx_t<T>::type* A,
                                int64_t lda, int64_t k0, int bw, int bwn,
-                               const typename cplx_t<T>::type* inv) {
+                               const typename cplx_t<T>::type* inv,
+                               const typename cplx_t<T>::type* lu11 = nullptr) {
   typedef typename cplx_t<T>::type CT;
   typedef typename Mfma<T>::acc_t acc_t;
   // 32 x 32 products: tiles t = 2 wave + j of a 2 x 4 grid
@@ -1470,6 +1479,12 @@ MW_DEV void l21_lookahead_body(LaLds<T>& L, int b, int64_t n, typename cplx_t<T>
     if (r < bw && c < bwn) w = A[(k0 + r) * lda + kn + c];
     L.Bs[1][r][c] = w;
   }
+  if (SYM && tid < NB) {
+    CT d;
+    d.x = 0; d.y = 0;
+    if (tid < bw) d = lu11[tid * NB + tid];
+    L.D[tid] = d;
+  }
   __syncthreads();
   acc_t l21[2], u12[2];
   lds_mfma<T, LAW, 32>(&L.As[0][0], &L.Bs[0][0][0], NB + 1, l21);
@@ -1489,6 +1504,18 @@ MW_DEV void l21_lookahead_body(LaLds<T>& L, int b, int64_t n, typename cplx_t<T>
     }
   }
   __syncthreads();
+  if (SYM && b > 0) {
+    // this block's U12 columns: U[k0 + i][r0 + row] = D_i L21[row][i]
+    for (int e = tid; e < NB * LAW; e += 256) {
+      const int i = e / LAW, row = e % LAW;
+      if (i < bw && row < rows) {
+        CT l;
+        l.x = L.As[row][2 * i];
+        l.y = L.As[row][2 * i + 1];
+        A[(k0 + i) * lda + r0 + row] = cmul(L.D[i], l);
+      }
+    }
+  }
   acc_t upd[2];
   lds_mfma<T, LAW, 32>(&L.As[0][0], &L.Bs[0][0][0], NB + 1, upd);
 #pragma unroll
@@ -1508,7 +1535,9 @@ MW_DEV void l21_lookahead_body(LaLds<T>& L, int b, int64_t n, typename cplx_t<T>
 // Blocks [0, nrb): L21 with the look-ahead update, LAW rows each;
 // [nrb, nrb + ncb): U12 on LAW-column blocks from kn + bwn; last: A11.  Small
 // blocks: each one's MFMA work is on the panel chain's critical path.
-template <typename T>
+// SYM: the L21 blocks also store U12 (D L21^T) for the columns of their rows,
+// so the U12 blocks cover only the right-hand-side columns [n, nc).
+template <typename T, bool SYM = false>
 __global__ __launch_bounds__(256) void k_panel_apply_la(int64_t n, int64_t nc,
                                                         typename cplx_t<T>::type* A, int64_t lda,
                                                         int64_t k0, int bw, int bwn,
@@ -1518,9 +1547,10 @@ __global__ __launch_bounds__(256) void k_panel_apply_la(int64_t n, int64_t nc,
   __shared__ LaLds<T> L;
   const int b = blockIdx.x;
   if (b < nrb) {
-    l21_lookahead_body<T>(L, b, n, A, lda, k0, bw, bwn, inv);
+    l21_lookahead_body<T, SYM>(L, b, n, A, lda, k0, bw, bwn, inv, lu11);
   } else if (b < nrb + ncb) {
-    const int64_t c0 = k0 + bw + bwn + (int64_t)(b - nrb) * LAW;
+    const int64_t cfirst = SYM ? ((k0 + bw + bwn) > n ? (k0 + bw + bwn) : n) : k0 + bw + bwn;
+    const int64_t c0 = cfirst + (int64_t)(b - nrb) * LAW;
     const int cols = (int)((nc - c0) < LAW ? (nc - c0) : LAW);
     block_product<T, 32, LAW, false>(L.As, &L.Bs[0][0][0], NB + 1, inv, NB, A + k0 * lda + c0, lda,
                                      bw, cols, false, 0.0);
@@ -1708,6 +1738,137 @@ MW_DEV void gemm_kc_tiles(K32Lds<T>& L, int ntiles, int gxn, int64_t m, int64_t 
   }
 }
 
+// The symmetric look-ahead's trailing update: rows [kn, n) against the
+// columns [kn + bwn, n) of the square part, only the 64 x 32 tiles whose first
+// column is at most 32 past their last row (so the lower part, the
+// panel-diagonal blocks and the block right of each stay up to date; the
+// rest lies further above and is replaced by U rows later), then every tile
+// of the right-hand-side columns [n, nc).  Square row tr holds
+// min(gxs, 2 tr + q + 1) tiles (q = floor((95 - bwn) / 32): 1 for a full
+// panel, 2 for a narrower one), so the first tstar rows hold tr^2 + q tr
+// tiles before row tr and the rest gxs each.
+struct SymTiles {
+  int64_t kn, c0s, n, nc;
+  int64_t gxs, gxr, nrt, q, tstar, s_star, s_sq;
+  __host__ __device__ SymTiles(int64_t n_, int64_t nc_, int64_t kn_, int bwn) {
+    n = n_;
+    nc = nc_;
+    kn = kn_;
+    c0s = kn_ + bwn;
+    const int64_t ns = n_ - c0s;
+    gxs = ns > 0 ? (ns + NB - 1) / NB : 0;
+    gxr = nc_ > n_ ? (nc_ - n_ + NB - 1) / NB : 0;
+    nrt = (n_ - kn_ + 63) / 64;
+    q = (95 - bwn) / 32;
+    tstar = gxs > q ? (gxs - q) / 2 : 0;
+    if (tstar > nrt) tstar = nrt;
+    s_star = tstar * tstar + q * tstar;
+    s_sq = gxs > 0 ? s_star + (nrt - tstar) * gxs : 0;
+  }
+  __host__ __device__ int64_t total() const { return s_sq + nrt * gxr; }
+  // tile t: rows row0 .. row0 + 64, columns col0 .. col0 + 32, below colend
+  __device__ void tile(int64_t t, int64_t& row0, int64_t& col0, int64_t& colend) const {
+    int64_t tr, tc;
+    if (t < s_star) {
+      tr = (int64_t)((sqrt((double)(q * q + 4 * t)) - (double)q) * 0.5);
+      while ((tr + 1) * (tr + 1) + q * (tr + 1) <= t) ++tr;
+      while (tr > 0 && tr * tr + q * tr > t) --tr;
+      tc = t - (tr * tr + q * tr);
+      col0 = c0s + NB * tc;
+      colend = n;
+    } else if (t < s_sq) {
+      const int64_t u = t - s_star;
+      tr = tstar + u / gxs;
+      tc = u % gxs;
+      col0 = c0s + NB * tc;
+      colend = n;
+    } else {
+      const int64_t u = t - s_sq;
+      tr = u / gxr;
+      tc = u % gxr;
+      col0 = n + NB * tc;
+      colend = nc;
+    }
+    row0 = kn + 64 * tr;
+  }
+};
+
+// gemm_k32_tiles over SymTiles: C = A[row0.., col0..] -= A[row0.., k0..k0+bw)
+// A[k0.., col0..) for each tile, workgroup b walking tiles b, b + G, ...
+template <typename T>
+MW_DEV void gemm_k32_sym(K32Lds<T>& L, int b, int G, const SymTiles& S, int kdim,
+                         typename cplx_t<T>::type* A, int64_t lda, int64_t k0) {
+  typedef typename cplx_t<T>::type CT;
+  typedef typename Mfma<T>::acc_t acc_t;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  T* Ar = reinterpret_cast<T*>(A);
+  const int64_t ntiles = S.total();
+  acc_t cn[4];
+  CT ra[8], rb[4];
+  int64_t crow = 0, ccol = 0, cend = 0;  // the tile whose operands are loaded
+  auto load = [&](int64_t t) {
+    S.tile(t, crow, ccol, cend);
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int64_t gr = crow + 16 * wave + Mfma<T>::row(lane, r);
+        const int cr = 16 * j + (lane & 15);
+        cn[j][r] = (gr < S.n && ccol + (cr >> 1) < cend) ? Ar[gr * 2 * lda + 2 * ccol + cr] : (T)0;
+      }
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      const int e = tid + 256 * q, r = e >> 5, c = e & 31;
+      CT v;
+      v.x = 0; v.y = 0;
+      if (crow + r < S.n && c < kdim) v = A[(crow + r) * lda + k0 + c];
+      ra[q] = v;
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int e = tid + 256 * q, r = e >> 5, c = e & 31;
+      CT v;
+      v.x = 0; v.y = 0;
+      if (r < kdim && ccol + c < cend) v = A[(k0 + r) * lda + ccol + c];
+      rb[q] = v;
+    }
+  };
+  int64_t t = b;
+  if (t >= ntiles) return;
+  load(t);
+  while (true) {
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      const int e = tid + 256 * q, r = e >> 5, c = e & 31;
+      L.As[r][2 * c] = -ra[q].x;
+      L.As[r][2 * c + 1] = -ra[q].y;
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int e = tid + 256 * q;
+      L.Bs[e >> 5][e & 31] = rb[q];
+    }
+    acc_t acc[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) acc[j] = cn[j];
+    const int64_t row0 = crow, col0 = ccol, colend = cend;
+    t += G;
+    __syncthreads();
+    if (t < ntiles) load(t);
+    lds_mfma<T, 64, 32, false>(&L.As[0][0], &L.Bs[0][0], NB + 1, acc);
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int64_t gr = row0 + 16 * wave + Mfma<T>::row(lane, r);
+        const int cr = 16 * j + (lane & 15);
+        if (gr < S.n && col0 + (cr >> 1) < colend) Ar[gr * 2 * lda + 2 * col0 + cr] = acc[j][r];
+      }
+    if (t >= ntiles) break;
+    __syncthreads();  // LDS free for the next tile's operands
+  }
+}
+
 template <typename T>
 struct NextLds {
   typedef typename cplx_t<T>::type CT;
@@ -1721,7 +1882,8 @@ struct NextLds {
 // (ntiles 64 x 32 tiles, gxn per row of tiles).  One LDS buffer serves
 // whichever part a block runs.  The factor's blocks come first so that they
 // are dispatched first.
-template <typename T>
+// SYM: the update over SymTiles.
+template <typename T, bool SYM = false>
 __global__ __launch_bounds__(256, 2) void k_lookahead(int64_t n, int64_t nc,
                                                    typename cplx_t<T>::type* A, int64_t lda,
                                                    int64_t k0, int bw, int bwn,
@@ -1748,6 +1910,11 @@ __global__ __launch_bounds__(256, 2) void k_lookahead(int64_t n, int64_t nc,
     return;
   }
   b -= 1;
+  if (SYM) {
+    gemm_k32_sym<T>(*reinterpret_cast<K32Lds<T>*>(smem), b, G, SymTiles(n, nc, kn, bwn), bw, A,
+                    lda, k0);
+    return;
+  }
   const int64_t c0 = kn + bwn;
   gemm_k32_tiles<T>(*reinterpret_cast<K32Lds<T>*>(smem), b, G, ntiles, gxn, n - kn, nc - c0, bw,
                     A + kn * lda + k0, lda, A + k0 * lda + c0, lda, A + kn * lda + c0, lda);
@@ -3168,7 +3335,10 @@ static hipError_t lookahead_resident(int* out) {
 // launch.  (The same overlap from two streams joined by events ran slower:
 // the update's workgroups, dispatched first, held the CUs and the factor's
 // blocks waited behind them.)
-template <typename T>
+// SYM: Z = Z^T without row exchanges (no tournament fallback: a failed
+// threshold check leaves its flag for k_sym_info), the update on the lower
+// part and the panel-diagonal blocks only (k_lookahead<T, true>).
+template <typename T, bool SYM = false>
 static int getrf_lookahead(int64_t n, int64_t nc, typename cplx_t<T>::type* A, int64_t lda,
                            int32_t* ipiv, void* work, int32_t* flags, const TourWs& tw,
                            int32_t* info, const volatile int32_t* cancel, hipStream_t st,
@@ -3192,24 +3362,31 @@ static int getrf_lookahead(int64_t n, int64_t nc, typename cplx_t<T>::type* A, i
     const int64_t rest = n - kn;
     const int bwn = (int)(rest < NB ? rest : NB);
     const int nrb = (int)((rest + LAW - 1) / LAW);
-    const int ncb = (int)((nc - kn - bwn + LAW - 1) / LAW);
+    // U12 blocks: every column right of the next panel (SYM: the right-hand
+    // sides only; the L21 blocks store the rest)
+    const int64_t cu = SYM ? std::max(kn + bwn, n) : kn + bwn;
+    const int ncb = (int)((nc - cu + LAW - 1) / LAW);
     CT* inv = reinterpret_cast<CT*>(panel_inv(work, k0));
     int32_t* flag = flags + k0 / NB;
-    const int nleaf = (int)((n - k0 + TROWS - 1) / TROWS);
-    // interchanges from column k0 on: earlier multiplier columns keep their
-    // order, as the per-panel forward solve expects
-    hipLaunchKernelGGL(k_panel_pivot<T>, dim3(nleaf), dim3(256), 0, st, A, lda, n, k0, nc, k0, bw,
-                       nleaf, tw, (const int32_t*)flag, lu11, inv, ipiv, info);
-    hipLaunchKernelGGL(k_panel_apply_la<T>, dim3(nrb + ncb + 1), dim3(256), 0, st, n, nc, A, lda,
-                       k0, bw, bwn, (const CT*)inv, (const CT*)lu11, nrb, ncb);
+    if (!SYM) {
+      const int nleaf = (int)((n - k0 + TROWS - 1) / TROWS);
+      // interchanges from column k0 on: earlier multiplier columns keep their
+      // order, as the per-panel forward solve expects
+      hipLaunchKernelGGL(k_panel_pivot<T>, dim3(nleaf), dim3(256), 0, st, A, lda, n, k0, nc, k0,
+                         bw, nleaf, tw, (const int32_t*)flag, lu11, inv, ipiv, info);
+    }
+    hipLaunchKernelGGL((k_panel_apply_la<T, SYM>), dim3(nrb + (ncb > 0 ? ncb : 0) + 1), dim3(256),
+                       0, st, n, nc, A, lda, k0, bw, bwn, (const CT*)inv, (const CT*)lu11, nrb,
+                       ncb > 0 ? ncb : 0);
     if (bwn > 0) {
       const int nnp = (int)((rest - bwn + AW - 1) / AW) + 1;
       const int64_t ncols = nc - kn - bwn;
       const int gxn = (int)((ncols + NB - 1) / NB);  // 0: no update left (no RHS)
-      const int ntiles = gxn * (int)((rest + 63) / 64);
+      const int ntiles = SYM ? (int)SymTiles(n, nc, kn, bwn).total()
+                             : gxn * (int)((rest + 63) / 64);
       // update blocks: what stays resident beside the factor's blocks
       const int G = std::max(1, std::min(ntiles, resident - nnp - 1));
-      hipLaunchKernelGGL(k_lookahead<T>, dim3((unsigned)(nnp + 1 + (ntiles > 0 ? G : 0))),
+      hipLaunchKernelGGL((k_lookahead<T, SYM>), dim3((unsigned)(nnp + 1 + (ntiles > 0 ? G : 0))),
                          dim3(256), 0, st, n, nc, A, lda, k0, bw, bwn, (const CT*)inv, lu11,
                          reinterpret_cast<CT*>(panel_inv(work, kn)), ipiv, flag + 1, lbound, nnp,
                          gxn > 0 ? gxn : 1, ntiles, G);
@@ -3577,7 +3754,6 @@ int mw_sysv_t(int64_t n, int64_t nrhs, void* Av, int64_t lda, int32_t* ipiv, voi
               hipStream_t st, hipError_t* herr) {
   typedef typename cplx_t<T>::type CT;
   const int64_t nbo = sym_block(n);
-  if (nbo == 0) return mw_gesv_t<T>(n, nrhs, Av, lda, ipiv, work, info, cancel, st, herr);
   CT* A = reinterpret_cast<CT*>(Av);
   const int64_t npan = (n + NB - 1) / NB;
   const int64_t nodes = tour_nodes(n);
@@ -3587,6 +3763,24 @@ int mw_sysv_t(int64_t n, int64_t nrhs, void* Av, int64_t lda, int32_t* ipiv, voi
   hipLaunchKernelGGL(k_zero_i32, dim3((unsigned)((npan + nodes + 255) / 256)), dim3(256), 0, st,
                      flags, npan + nodes);
   hipLaunchKernelGGL(k_zero_i32, dim3(1), dim3(64), 0, st, info, (int64_t)1);
+  if (nbo == 0) {
+    // one level: the look-ahead chain over [A | B] with the update on the
+    // lower part and the panel-diagonal blocks, then the back substitution
+    TourWs tw;
+    tw.counter = flags + npan;
+    tw.ncand = tw.counter + nodes;
+    tw.cand = tw.ncand + nodes;
+    int rc = getrf_lookahead<T, true>(n, n + nrhs, A, lda, ipiv, work, flags, tw, info, cancel,
+                                      st, herr);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_sym_info, dim3(1), dim3(256), 0, st, (const int32_t*)flags, npan, info);
+    if (nrhs > 0) {
+      rc = solve_bwd<T>(n, nrhs, A, lda, work, A + n, lda, st, herr);
+      if (rc) return rc;
+    }
+    *herr = hipGetLastError();
+    return *herr == hipSuccess ? MW_OK : MW_EHIP;
+  }
   const double lbound = g_lbound;
   int resident = 0;
   if ((*herr = lookahead_resident<T>(&resident)) != hipSuccess) return MW_EHIP;
