@@ -992,19 +992,25 @@ def main():
     pcie_rate = n_pcie / (time.perf_counter() - t1)
 
     ncols = n_ports + len(s._port_plan()[1])
-    sv_flops = lu_flops(nb) + 8.0 * nb * nb * ncols
+    sym = bool(s._sym_used)
+    sv_flops = lu_flops(nb, sym) + 8.0 * nb * nb * ncols
     lu_tf = sv_flops / solve_s / 1e12
     fill_tf = fill_flops(ns, nb) / fill_s / 1e12
     dominant = "solve" if solve_s >= fill_s else "fill"
     if dominant == "solve":
-        roof = {"kernel": "factor + all-port solve (mw_zgesv: per 32-column panel, threshold-"
-                          "pivot fallback, apply with the next panel's update, then the next "
-                          "panel's factor in one launch with the MFMA trailing update over "
-                          "[Z | B]; then back substitution two panels per step)",
+        roof = {"kernel": ("factor + all-port solve (mw_zsysv at one level: Z = Z^T factored "
+                           "without row exchanges; per 32-column panel, apply with the next "
+                           "panel's update, then the next panel's factor in one launch with "
+                           "the MFMA trailing update on the lower part and [B]; then back "
+                           "substitution two panels per step)") if sym else
+                          ("factor + all-port solve (mw_zgesv: per 32-column panel, threshold-"
+                           "pivot fallback, apply with the next panel's update, then the next "
+                           "panel's factor in one launch with the MFMA trailing update over "
+                           "[Z | B]; then back substitution two panels per step)"),
                 "bound": "mfma", "achieved": lu_tf, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
                 "frac": lu_tf / PEAK_FP64_TFLOPS, "traffic": None,
-                "algorithmic": f"(8/3) n^3 + 8 n^2 p = {sv_flops:.3e} flop per solve "
-                               f"(n={nb}, p={ncols})"}
+                "algorithmic": f"{'(4/3)' if sym else '(8/3)'} n^3 + 8 n^2 p = {sv_flops:.3e} "
+                               f"flop per solve (n={nb}, p={ncols})"}
     else:
         roof = {"kernel": "fill (k_same_edge + k_fill)", "bound": "valu", "achieved": fill_tf,
                 "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s", "frac": fill_tf / PEAK_FP64_TFLOPS,
