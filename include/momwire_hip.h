@@ -313,7 +313,7 @@ int mw_cgesv(int64_t n, int64_t nrhs, void* A, int64_t lda, int32_t* ipiv,
  * sides solved by one block product per 512 rows in each direction.  Same
  * arguments and outputs as mw_?gesv except work, which holds
  * mw_sysv_workspace_bytes(n) bytes (the block inverses stay there for
- * mw_?sytrs; ipiv is returned as the identity past n = 4096).
+ * mw_?sytrs; ipiv is returned as the identity).
  * first_row: NULL, or a HOST array of nrhs int32 giving the first row at
  * which each right-hand side may be nonzero (its rows above are zero on
  * entry: a port's drive column starts at its feed row); the forward pass
@@ -324,8 +324,12 @@ int mw_cgesv(int64_t n, int64_t nrhs, void* A, int64_t lda, int32_t* ipiv,
  * mw_set_pivot_threshold, or a zero pivot) the panel is NOT pivoted and
  * info[0] = -1 on return: the result is not valid and the caller refactors
  * the original matrix with mw_?gesv.
- * n <= 4096: mw_?gesv itself.  The reference has no counterpart: it solves
- * Z with scipy.linalg.solve (bspline.py:2971-2989) whatever its symmetry.
+ * n <= 4096: one level, the look-ahead chain of mw_?gesv without row
+ * exchanges, its trailing update on the lower part and the panel-diagonal
+ * blocks only (the U12 rows formed as D L21^T); first_row is not used and
+ * work is mw_getrf_workspace_bytes(n).  The reference has no counterpart:
+ * it solves Z with scipy.linalg.solve (bspline.py:2971-2989) whatever its
+ * symmetry.
  */
 int mw_zsysv(int64_t n, int64_t nrhs, void* A, int64_t lda, int32_t* ipiv,
              void* work, int32_t* info, const volatile int32_t* cancel,
@@ -344,7 +348,7 @@ int64_t mw_sysv_workspace_bytes(int64_t n);
 /*
  * X <- A^{-1} X with the factors and workspace of mw_?sysv (X: n x nrhs,
  * row stride ldx): past n = 4096 one block product per 512 rows each way;
- * n <= 4096, where mw_?sysv is mw_?gesv, mw_?getrs with ipiv.  No
+ * n <= 4096 (one level), mw_?getrs with ipiv (the identity).  No
  * counterpart in the reference (see mw_?sysv).
  */
 int mw_zsytrs(int64_t n, int64_t nrhs, const void* A, int64_t lda,
