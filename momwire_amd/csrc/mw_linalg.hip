@@ -1869,6 +1869,119 @@ MW_DEV void gemm_k32_sym(K32Lds<T>& L, int b, int G, const SymTiles& S, int kdim
   }
 }
 
+// C -= A B on 64 x 64 (complex) tiles, FP64, any kdim by chunks of 32, for
+// one workgroup walking every tile (the persistent sweep's updates).  Waves
+// 2 x 2: wave (wr, wc) holds rows 32 wr .. + 32 and real columns 64 wc .. +
+// 64, so a k step reads 2 A and 4 B fragments for 8 MFMAs (the 64 x 32 tile
+// of gemm_kc_tiles reads 1 and 4 for 4).  Operands of the next chunk load
+// into registers while a chunk's MFMAs run; a tile's C loads into the
+// accumulators as soon as the previous tile is stored.
+struct K64Lds {
+  double As[64][65];     // -A as reals: row r, real k
+  double2 Bs[NB][65];    // k row, complex column
+};
+
+MW_DEV void gemm_kc64_tiles(K64Lds& L, int ntiles, int gxn, int64_t m, int64_t n, int kdim,
+                            const double2* __restrict__ A, int64_t lda,
+                            const double2* __restrict__ B, int64_t ldb, double2* C,
+                            int64_t ldc) {
+  typedef Mfma<double>::acc_t acc_t;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wr = wave >> 1, wc = wave & 1;
+  const int nch = (kdim + NB - 1) / NB;
+  const int total = ntiles * nch;
+  if (total <= 0) return;
+  double* Cr = reinterpret_cast<double*>(C);
+  acc_t acc[2][4];
+  double2 ra[8], rb[8];
+  auto load_c = [&](int t) {
+    const int64_t row0 = (int64_t)(t / gxn) * 64, col0 = (int64_t)(t % gxn) * 64;
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+      for (int j = 0; j < 4; ++j)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int64_t gr = row0 + 32 * wr + 16 * i + Mfma<double>::row(lane, r);
+          const int cr = 64 * wc + 16 * j + (lane & 15);
+          acc[i][j][r] =
+              (gr < m && col0 + (cr >> 1) < n) ? Cr[gr * 2 * ldc + 2 * col0 + cr] : 0.0;
+        }
+  };
+  auto load = [&](int it) {
+    const int t = it / nch, q = it % nch;
+    const int64_t row0 = (int64_t)(t / gxn) * 64, col0 = (int64_t)(t % gxn) * 64;
+    const int64_t k0 = (int64_t)q * NB;
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const int e = tid + 256 * u, r = e >> 5, c = e & 31;
+      double2 v = make_double2(0.0, 0.0);
+      if (row0 + r < m && k0 + c < kdim) v = A[(row0 + r) * lda + k0 + c];
+      ra[u] = v;
+    }
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const int e = tid + 256 * u, r = e >> 6, c = e & 63;
+      double2 v = make_double2(0.0, 0.0);
+      if (k0 + r < kdim && col0 + c < n) v = B[(k0 + r) * ldb + col0 + c];
+      rb[u] = v;
+    }
+  };
+  load_c(0);
+  load(0);
+  for (int it = 0;;) {
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const int e = tid + 256 * u, r = e >> 5, c = e & 31;
+      L.As[r][2 * c] = -ra[u].x;
+      L.As[r][2 * c + 1] = -ra[u].y;
+    }
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const int e = tid + 256 * u;
+      L.Bs[e >> 6][e & 63] = rb[u];
+    }
+    const int cur = it++;
+    __syncthreads();
+    if (it < total) load(it);
+#pragma unroll
+    for (int ks = 0; ks < 2 * NB; ks += 4) {
+      const int kr = ks + (lane >> 4), kpar = kr & 1;
+      double a[2], b[4];
+#pragma unroll
+      for (int i = 0; i < 2; ++i) a[i] = L.As[32 * wr + 16 * i + (lane & 15)][kr];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        // real row kr, real column cr of [[br, bi], [-bi, br]]: one double
+        const int cr = 64 * wc + 16 * j + (lane & 15), odd = cr & 1;
+        const double v = reinterpret_cast<const double*>(&L.Bs[kr >> 1][cr >> 1])[odd ^ kpar];
+        b[j] = (kpar && !odd) ? -v : v;
+      }
+#pragma unroll
+      for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc[i][j] = Mfma<double>::mma(a[i], b[j], acc[i][j]);
+    }
+    if (cur % nch == nch - 1) {
+      const int t = cur / nch;
+      const int64_t row0 = (int64_t)(t / gxn) * 64, col0 = (int64_t)(t % gxn) * 64;
+#pragma unroll
+      for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            const int64_t gr = row0 + 32 * wr + 16 * i + Mfma<double>::row(lane, r);
+            const int cr = 64 * wc + 16 * j + (lane & 15);
+            if (gr < m && col0 + (cr >> 1) < n) Cr[gr * 2 * ldc + 2 * col0 + cr] = acc[i][j][r];
+          }
+      if (it < total) load_c(t + 1);
+    }
+    if (it >= total) break;
+    __syncthreads();  // LDS free for the next chunk's operands
+  }
+}
+
 template <typename T>
 struct NextLds {
   typedef typename cplx_t<T>::type CT;
@@ -2806,7 +2919,7 @@ constexpr size_t kWgLds =
     sizeof(BlkLds<double>) > sizeof(PivotLds<double2>) ? sizeof(BlkLds<double>)
                                                        : sizeof(PivotLds<double2>);
 static_assert(kWgLds >= sizeof(PanelLds<double2>) && kWgLds >= sizeof(SmallLds) &&
-                  kWgLds >= sizeof(K32Lds<double>),
+                  kWgLds >= sizeof(K32Lds<double>) && kWgLds >= sizeof(K64Lds),
               "the shared buffer holds every phase's LDS");
 __shared__ __align__(16) unsigned char g_wg_lds[kWgLds];
 __shared__ int g_wg_flag;
@@ -2907,10 +3020,10 @@ MW_PHASE void wg_apply(const mw_sweep_prob* pp, int64_t k0, int64_t nc) {
 // C -= A B over an m x ncol block (any K), 64 x 32 tiles one after the other.
 MW_PHASE void wg_gemm(int64_t m, int64_t ncol, int kdim, const double2* a, const double2* b,
                       double2* c, int64_t ld) {
-  const int gxn = (int)((ncol + NB - 1) / NB);
+  const int gxn = (int)((ncol + 63) / 64);
   const int ntiles = gxn * (int)((m + 63) / 64);
-  gemm_kc_tiles<double>(*reinterpret_cast<K32Lds<double>*>(g_wg_lds), ntiles, gxn, m, ncol, kdim,
-                        a, ld, b, ld, c, ld);
+  gemm_kc64_tiles(*reinterpret_cast<K64Lds*>(g_wg_lds), ntiles, gxn, m, ncol, kdim, a, ld, b, ld,
+                  c, ld);
   __syncthreads();
 }
 
