@@ -446,9 +446,9 @@ __global__ __launch_bounds__(256, 2) void k_gemm_sub_pd(int64_t m, int64_t n, in
 #pragma unroll
           for (int j = 0; j < 4; ++j)
             acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i][s], b[j][s], acc[i][j], 0, 0, 0);
+      if (h == 0 && more) sstore(buf ^ 1);  // between this stage's MFMAs
     }
     if (more) {
-      sstore(buf ^ 1);
       __syncthreads();
       buf ^= 1;
     }
@@ -466,16 +466,21 @@ __global__ __launch_bounds__(256, 2) void k_gemm_sub_pd(int64_t m, int64_t n, in
 }
 
 // The FP32 large product written out for the 32x32x2 MFMA (the same
-// scheme as k_gemm_sub_p; this explicit form measured 8.6 ms against 9.5 ms
-// for the 14848 x 15872 x 512 trailing update of the 15,360-unknown system).
+// scheme as k_gemm_sub_pd).  Within a stage, a wave reads the LDS fragments
+// of k step h + 1 before it issues the MFMAs of step h, and writes the next
+// stage's operands to the other buffer after the second of its four steps;
+// tiles wholly inside the matrix load their operands and C with no bounds
+// checks (one branch per stage instead of one per element).
+typedef float v4f __attribute__((ext_vector_type(4)));
+
 __global__ __launch_bounds__(256, 2) void k_gemm_sub_pf(int64_t m, int64_t n, int64_t kdim,
                                                        const float2* __restrict__ A, int64_t lda,
                                                        const float2* __restrict__ B, int64_t ldb,
                                                        float2* C, int64_t ldc, int nbx, int nby,
                                                        int stair) {
   constexpr int KR = 32, LD = 36, HALF = KR / 2, KC = KR / 2;
-  __shared__ float As[2][128][LD];
-  __shared__ float Bt[2][128][LD];
+  __shared__ __attribute__((aligned(16))) float As[2][128][LD];
+  __shared__ __attribute__((aligned(16))) float Bt[2][128][LD];
   int bx, by;
   if (stair)
     pgemm_stair_tile(blockIdx.x, stair, bx, by);
@@ -484,6 +489,7 @@ __global__ __launch_bounds__(256, 2) void k_gemm_sub_pf(int64_t m, int64_t n, in
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wr = wave >> 1, wc = wave & 1;
   const int64_t row0 = (int64_t)by * 128, col0 = (int64_t)bx * 64;
+  const bool inside = row0 + 128 <= m && col0 + 64 <= n;
   float* Cr = reinterpret_cast<float*>(C);
   v16f acc[2][2];
 #pragma unroll
@@ -494,7 +500,7 @@ __global__ __launch_bounds__(256, 2) void k_gemm_sub_pf(int64_t m, int64_t n, in
       for (int r = 0; r < 16; ++r) {
         const int64_t gr = row0 + wr * 64 + i * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
         const int64_t gcr = 2 * col0 + wc * 64 + j * 32 + (lane & 31);
-        acc[i][j][r] = (gr < m && gcr < 2 * n) ? Cr[gr * 2 * ldc + gcr] : 0.f;
+        acc[i][j][r] = (inside || (gr < m && gcr < 2 * n)) ? Cr[gr * 2 * ldc + gcr] : 0.f;
       }
   constexpr int AQ = KC / 2;  // 8 complex per thread: row tid / 2, half tid & 1
   float2 ra[AQ];
@@ -503,6 +509,15 @@ __global__ __launch_bounds__(256, 2) void k_gemm_sub_pf(int64_t m, int64_t n, in
   const int b_col = tid & 63, b_k0 = (tid >> 6) * 4;
   auto gload = [&](int64_t k0) {
     const int64_t gr = row0 + a_row;
+    if (inside && k0 + KC <= kdim) {
+      const float2* pa = A + gr * lda + k0 + a_h * AQ;
+      const float2* pb = B + (k0 + b_k0) * ldb + col0 + b_col;
+#pragma unroll
+      for (int q = 0; q < AQ; ++q) ra[q] = pa[q];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) rb[q] = pb[q * ldb];
+      return;
+    }
 #pragma unroll
     for (int q = 0; q < AQ; ++q) {
       const int64_t kc = k0 + a_h * AQ + q;
@@ -534,37 +549,43 @@ __global__ __launch_bounds__(256, 2) void k_gemm_sub_pf(int64_t m, int64_t n, in
       Bt[buf][2 * b_col + 1][HALF + kc] = v.x;
     }
   };
+  const int g = lane >> 5;
+  // fragments of four consecutive (permuted) real k of one k step
+  auto lread = [&](int buf, int h, v4f (&fa)[2], v4f (&fb)[2]) {
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+      fa[i] = *reinterpret_cast<const v4f*>(&As[buf][wr * 64 + i * 32 + (lane & 31)][g * HALF + h]);
+#pragma unroll
+    for (int j = 0; j < 2; ++j)
+      fb[j] = *reinterpret_cast<const v4f*>(&Bt[buf][wc * 64 + j * 32 + (lane & 31)][g * HALF + h]);
+  };
   gload(0);
   sstore(0);
   __syncthreads();
   int buf = 0;
-  const int g = lane >> 5;
+  v4f fa[2][2], fb[2][2];  // [set][tile]: two k steps in flight
+  lread(0, 0, fa[0], fb[0]);
   for (int64_t k0 = 0; k0 < kdim; k0 += KC) {
     const bool more = k0 + KC < kdim;
     if (more) gload(k0 + KC);
 #pragma unroll
-    for (int h = 0; h < HALF; h += 4) {
-      float a[2][4], b[2][4];
-#pragma unroll
-      for (int i = 0; i < 2; ++i)
-#pragma unroll
-        for (int s = 0; s < 4; ++s) a[i][s] = As[buf][wr * 64 + i * 32 + (lane & 31)][g * HALF + h + s];
-#pragma unroll
-      for (int j = 0; j < 2; ++j)
-#pragma unroll
-        for (int s = 0; s < 4; ++s) b[j][s] = Bt[buf][wc * 64 + j * 32 + (lane & 31)][g * HALF + h + s];
+    for (int hh = 0; hh < HALF / 4; ++hh) {
+      const int cs = hh & 1;
+      if (hh + 1 < HALF / 4) lread(buf, 4 * (hh + 1), fa[cs ^ 1], fb[cs ^ 1]);
 #pragma unroll
       for (int s = 0; s < 4; ++s)
 #pragma unroll
         for (int i = 0; i < 2; ++i)
 #pragma unroll
           for (int j = 0; j < 2; ++j)
-            acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x2f32(a[i][s], b[j][s], acc[i][j], 0, 0, 0);
+            acc[i][j] =
+                __builtin_amdgcn_mfma_f32_32x32x2f32(fa[cs][i][s], fb[cs][j][s], acc[i][j], 0, 0, 0);
+      if (hh == 1 && more) sstore(buf ^ 1);
     }
     if (more) {
-      sstore(buf ^ 1);
       __syncthreads();
       buf ^= 1;
+      lread(buf, 0, fa[0], fb[0]);
     }
   }
 #pragma unroll
@@ -575,7 +596,7 @@ __global__ __launch_bounds__(256, 2) void k_gemm_sub_pf(int64_t m, int64_t n, in
       for (int r = 0; r < 16; ++r) {
         const int64_t gr = row0 + wr * 64 + i * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
         const int64_t gcr = 2 * col0 + wc * 64 + j * 32 + (lane & 31);
-        if (gr < m && gcr < 2 * n) Cr[gr * 2 * ldc + gcr] = acc[i][j][r];
+        if (inside || (gr < m && gcr < 2 * n)) Cr[gr * 2 * ldc + gcr] = acc[i][j][r];
       }
 }
 
