@@ -473,12 +473,13 @@ __global__ __launch_bounds__(256, 2) void k_gemm_sub_pd(int64_t m, int64_t n, in
 // checks (one branch per stage instead of one per element).
 typedef float v4f __attribute__((ext_vector_type(4)));
 
-__global__ __launch_bounds__(256, 2) void k_gemm_sub_pf(int64_t m, int64_t n, int64_t kdim,
-                                                       const float2* __restrict__ A, int64_t lda,
-                                                       const float2* __restrict__ B, int64_t ldb,
-                                                       float2* C, int64_t ldc, int nbx, int nby,
-                                                       int stair) {
-  constexpr int KR = 32, LD = 36, HALF = KR / 2, KC = KR / 2;
+template <int KR, int MINB>
+__global__ __launch_bounds__(256, MINB) void k_gemm_sub_pf(int64_t m, int64_t n, int64_t kdim,
+                                                          const float2* __restrict__ A, int64_t lda,
+                                                          const float2* __restrict__ B, int64_t ldb,
+                                                          float2* C, int64_t ldc, int nbx, int nby,
+                                                          int stair) {
+  constexpr int LD = KR + 4, HALF = KR / 2, KC = KR / 2, BQ = KC / 4;
   __shared__ __attribute__((aligned(16))) float As[2][128][LD];
   __shared__ __attribute__((aligned(16))) float Bt[2][128][LD];
   int bx, by;
@@ -504,9 +505,9 @@ __global__ __launch_bounds__(256, 2) void k_gemm_sub_pf(int64_t m, int64_t n, in
       }
   constexpr int AQ = KC / 2;  // 8 complex per thread: row tid / 2, half tid & 1
   float2 ra[AQ];
-  float2 rb[4];               // 4 consecutive complex rows of one column
+  float2 rb[BQ];              // BQ consecutive complex rows of one column
   const int a_row = tid >> 1, a_h = tid & 1;
-  const int b_col = tid & 63, b_k0 = (tid >> 6) * 4;
+  const int b_col = tid & 63, b_k0 = (tid >> 6) * BQ;
   auto gload = [&](int64_t k0) {
     const int64_t gr = row0 + a_row;
     if (inside && k0 + KC <= kdim) {
@@ -515,7 +516,7 @@ __global__ __launch_bounds__(256, 2) void k_gemm_sub_pf(int64_t m, int64_t n, in
 #pragma unroll
       for (int q = 0; q < AQ; ++q) ra[q] = pa[q];
 #pragma unroll
-      for (int q = 0; q < 4; ++q) rb[q] = pb[q * ldb];
+      for (int q = 0; q < BQ; ++q) rb[q] = pb[q * ldb];
       return;
     }
 #pragma unroll
@@ -526,7 +527,7 @@ __global__ __launch_bounds__(256, 2) void k_gemm_sub_pf(int64_t m, int64_t n, in
       ra[q] = v;
     }
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
+    for (int q = 0; q < BQ; ++q) {
       const int64_t kc = k0 + b_k0 + q;
       float2 v = make_float2(0.f, 0.f);
       if (kc < kdim && col0 + b_col < n) v = B[kc * ldb + col0 + b_col];
@@ -540,7 +541,7 @@ __global__ __launch_bounds__(256, 2) void k_gemm_sub_pf(int64_t m, int64_t n, in
       As[buf][a_row][HALF + a_h * AQ + q] = -ra[q].y;  // real k 2 kc + 1: HALF + kc
     }
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
+    for (int q = 0; q < BQ; ++q) {
       const float2 v = rb[q];
       const int kc = b_k0 + q;
       Bt[buf][2 * b_col][kc] = v.x;
@@ -580,7 +581,7 @@ __global__ __launch_bounds__(256, 2) void k_gemm_sub_pf(int64_t m, int64_t n, in
           for (int j = 0; j < 2; ++j)
             acc[i][j] =
                 __builtin_amdgcn_mfma_f32_32x32x2f32(fa[cs][i][s], fb[cs][j][s], acc[i][j], 0, 0, 0);
-      if (hh == 1 && more) sstore(buf ^ 1);
+      if (hh == HALF / 8 - 1 && more) sstore(buf ^ 1);
     }
     if (more) {
       __syncthreads();
@@ -600,13 +601,23 @@ __global__ __launch_bounds__(256, 2) void k_gemm_sub_pf(int64_t m, int64_t n, in
       }
 }
 
+// FP32 large products: 16 complex k per stage, two workgroups per CU (8
+// complex per stage measured 2-4 % slower, with or without a third workgroup
+// per CU).
+static void launch_pf(dim3 grid, hipStream_t st, int64_t m, int64_t n, int64_t k, const float2* A,
+                      int64_t lda, const float2* B, int64_t ldb, float2* C, int64_t ldc, int nbx,
+                      int nby, int stair) {
+  hipLaunchKernelGGL((k_gemm_sub_pf<32, 2>), grid, dim3(256), 0, st, m, n, k, A, lda, B, ldb, C,
+                     ldc, nbx, nby, stair);
+}
+
 template <typename T>
 hipError_t launch_gemm_p(int64_t m, int64_t n, int64_t k, const void* A, int64_t lda,
                          const void* B, int64_t ldb, void* C, int64_t ldc, hipStream_t st) {
   const int nbx = (int)((n + GBN - 1) / GBN), nby = (int)((m + GBM - 1) / GBM);
   if constexpr (sizeof(T) == 4)
-    hipLaunchKernelGGL(k_gemm_sub_pf, dim3((unsigned)(nbx * nby)), dim3(256), 0, st, m, n, k,
-                       (const float2*)A, lda, (const float2*)B, ldb, (float2*)C, ldc, nbx, nby, 0);
+    launch_pf(dim3((unsigned)(nbx * nby)), st, m, n, k, (const float2*)A, lda, (const float2*)B,
+              ldb, (float2*)C, ldc, nbx, nby, 0);
   else
     hipLaunchKernelGGL(k_gemm_sub_pd, dim3((unsigned)(nbx * nby)), dim3(256), 0, st, m, n, k,
                        (const double2*)A, lda, (const double2*)B, ldb, (double2*)C, ldc, nbx, nby,
@@ -623,8 +634,8 @@ hipError_t launch_gemm_stair(int64_t m, int64_t k, const void* A, int64_t lda, c
   const int nbr = (int)((m + 511) / 512);
   const unsigned grid = (unsigned)(16 * nbr * (nbr + 1));
   if constexpr (sizeof(T) == 4)
-    hipLaunchKernelGGL(k_gemm_sub_pf, dim3(grid), dim3(256), 0, st, m, m, k, (const float2*)A, lda,
-                       (const float2*)B, ldb, (float2*)C, ldc, 0, 0, nbr);
+    launch_pf(dim3(grid), st, m, m, k, (const float2*)A, lda, (const float2*)B, ldb, (float2*)C,
+              ldc, 0, 0, nbr);
   else
     hipLaunchKernelGGL(k_gemm_sub_pd, dim3(grid), dim3(256), 0, st, m, m, k, (const double2*)A,
                        lda, (const double2*)B, ldb, (double2*)C, ldc, 0, 0, nbr);
