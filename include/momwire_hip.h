@@ -386,6 +386,10 @@ int mw_cgemm_sub(int64_t m, int64_t n, int64_t k, const void* A, int64_t lda,
  */
 int mw_zgemm_tn_add(int64_t m, int64_t n, int64_t k, const void* A, int64_t lda,
                     const void* B, int64_t ldb, void* C, int64_t ldc, void* stream);
+/* The same in FP32 (complex64): the mixed form's C^T R, a term ~1e-6 of Y,
+ * for which single precision leaves ~1e-11 of Y. */
+int mw_cgemm_tn_add(int64_t m, int64_t n, int64_t k, const void* A, int64_t lda,
+                    const void* B, int64_t ldb, void* C, int64_t ldc, void* stream);
 
 /*
  * out[i][j] = sum_t val[i][t] * X[idx[i][t]][j]  (sparse rows times X).

@@ -123,6 +123,7 @@ SIGNATURES = {
     "mw_zgemm_sub": (ctypes.c_int, [_I64, _I64, _I64, _P, _I64, _P, _I64, _P, _I64, _P]),
     "mw_cgemm_sub": (ctypes.c_int, [_I64, _I64, _I64, _P, _I64, _P, _I64, _P, _I64, _P]),
     "mw_zgemm_tn_add": (ctypes.c_int, [_I64, _I64, _I64, _P, _I64, _P, _I64, _P, _I64, _P]),
+    "mw_cgemm_tn_add": (ctypes.c_int, [_I64, _I64, _I64, _P, _I64, _P, _I64, _P, _I64, _P]),
     "mw_zsparse_rows": (ctypes.c_int, [_I64, _I32, _P, _P, _P, _I64, _I64, _P, _I64, _P]),
     "mw_zscatter_cols": (ctypes.c_int, [_I64, _I64, _I32, _P, _P, _P, _I64, _P]),
     "mw_zsmall_solve": (ctypes.c_int, [_I32, _I32, _P, _I64, _P, _I64, _P, _P]),

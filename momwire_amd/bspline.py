@@ -250,6 +250,7 @@ class _Device:
             npt = max(self.n_ports, 1)
             self.C64 = torch.empty((nb, nc), dtype=torch.complex128, device=self.dev)
             self.T64 = torch.empty((npt, npt), dtype=torch.complex128, device=self.dev)
+            self.T32 = torch.empty((npt, npt), dtype=torch.complex64, device=self.dev)
             self.gscratch = torch.empty(16 * (npt + 1), dtype=torch.uint8, device=self.dev)
             self.gstats = torch.zeros(4, dtype=torch.float64, device=self.dev)
             self.gate = torch.zeros(1, dtype=torch.int32, device=self.dev)
@@ -882,9 +883,16 @@ class BSplineSolver(_SweptPortSolutions):
             mark("refine", False)
             mark("readout", True)
             self._adjoint_readout(D, D.Y.data_ptr(), st)
-            D.T64.zero_()
-            _lib.check(lib.mw_zgemm_tn_add(npt, npt, nb, D.C64.data_ptr(), nc, D.R64.data_ptr(),
-                                           nc, D.T64.data_ptr(), max(npt, 1), st), "C^T R")
+            # T = C^T R in FP32 (C from the FP32 solve, R rounded into the
+            # factor block's right-hand-side columns, free after the solve):
+            # T is ~1e-6 of Y, so single precision leaves ~1e-11 of Y
+            _lib.check(lib.mw_z2c(nb, npt, D.R64.data_ptr(), nc, D.X32.data_ptr(), D.ld32, st),
+                       "R to FP32")
+            D.T32.zero_()
+            _lib.check(lib.mw_cgemm_tn_add(npt, npt, nb, D.D32.data_ptr(), nc, D.X32.data_ptr(),
+                                           D.ld32, D.T32.data_ptr(), max(npt, 1), st), "C^T R")
+            _lib.check(lib.mw_c2z(npt, npt, D.T32.data_ptr(), max(npt, 1), D.T64.data_ptr(),
+                                  max(npt, 1), 0, st), "T to FP64")
             _lib.check(lib.mw_zreadout_check(nb, npt, D.X64.data_ptr(), nc, D.C64.data_ptr(), nc,
                                              D.T64.data_ptr(), max(npt, 1), D.Y.data_ptr(),
                                              max(npt, 1), READOUT_TOL, D.gscratch.data_ptr(),

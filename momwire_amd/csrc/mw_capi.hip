@@ -42,6 +42,8 @@ hipError_t mw_gemm_sub_c(int64_t m, int64_t n, int64_t k, const void* A, int64_t
                          const void* B, int64_t ldb, void* C, int64_t ldc, hipStream_t st);
 hipError_t mw_gemm_tn_add_z(int64_t m, int64_t n, int64_t k, const void* A, int64_t lda,
                             const void* B, int64_t ldb, void* C, int64_t ldc, hipStream_t st);
+hipError_t mw_gemm_tn_add_c(int64_t m, int64_t n, int64_t k, const void* A, int64_t lda,
+                            const void* B, int64_t ldb, void* C, int64_t ldc, hipStream_t st);
 hipError_t mw_launch_sparse_rows(int64_t nrows, int maxnz, const int32_t* idx, const double* val,
                                  const void* X, int64_t ldx, int64_t ncols, void* out,
                                  int64_t ldo, hipStream_t st);
@@ -521,6 +523,16 @@ int mw_zgemm_tn_add(int64_t m, int64_t n, int64_t k, const void* A, int64_t lda,
   if (lda < m || ldb < n || ldc < n) return fail(MW_EINVAL, "%sbad leading dimension");
   return hip_status(mw_gemm_tn_add_z(m, n, k, A, lda, B, ldb, C, ldc, (hipStream_t)stream),
                     "mw_zgemm_tn_add");
+}
+
+int mw_cgemm_tn_add(int64_t m, int64_t n, int64_t k, const void* A, int64_t lda, const void* B,
+                    int64_t ldb, void* C, int64_t ldc, void* stream) {
+  g_err[0] = 0;
+  if (m < 0 || n < 0 || k < 0) return fail(MW_EINVAL, "%snegative size");
+  if ((m && n && k) && (!A || !B || !C)) return fail(MW_EINVAL, "%sNULL argument");
+  if (lda < m || ldb < n || ldc < n) return fail(MW_EINVAL, "%sbad leading dimension");
+  return hip_status(mw_gemm_tn_add_c(m, n, k, A, lda, B, ldb, C, ldc, (hipStream_t)stream),
+                    "mw_cgemm_tn_add");
 }
 
 int mw_zsparse_rows(int64_t nrows, int32_t maxnz, const int32_t* idx, const double* val,

@@ -4032,6 +4032,10 @@ hipError_t mw_gemm_tn_add_z(int64_t m, int64_t n, int64_t k, const void* A, int6
                             const void* B, int64_t ldb, void* C, int64_t ldc, hipStream_t st) {
   return launch_gemm_sub<double, true>(m, n, k, A, lda, B, ldb, C, ldc, st);
 }
+hipError_t mw_gemm_tn_add_c(int64_t m, int64_t n, int64_t k, const void* A, int64_t lda,
+                            const void* B, int64_t ldb, void* C, int64_t ldc, hipStream_t st) {
+  return launch_gemm_sub<float, true>(m, n, k, A, lda, B, ldb, C, ldc, st);
+}
 
 hipError_t mw_launch_sparse_rows(int64_t nrows, int maxnz, const int32_t* idx, const double* val,
                                  const void* X, int64_t ldx, int64_t ncols, void* out,
