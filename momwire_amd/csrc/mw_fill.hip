@@ -351,6 +351,43 @@ MW_DEV void far_phase32(double k, double R, double& gr, double& gi) {
   gi = -(double)(s32 * inv);
 }
 
+// R = sqrt(s) and 1 / R together: the hardware reciprocal square root and two
+// Newton steps y <- y (3/2 - (s/2) y^2) (s > 0: a^2 is part of s).
+MW_DEV void r_and_rinv(double s, double& R, double& rinv) {
+  const double h = 0.5 * s;
+  double y = __builtin_amdgcn_rsq(s);
+  y = y * fma(-h * y, y, 1.5);
+  y = y * fma(-h * y, y, 1.5);
+  R = s * y;
+  rinv = y;
+}
+
+// cos x and sin x for |x| <= 0.6 from their Taylor series through x^14 and
+// x^15 (the first omitted terms are below 1.3e-17 there).
+MW_DEV void cos_sin_small(double x, double& c, double& sn) {
+  const double x2 = x * x;
+  double pc = -1.0 / 87178291200.0;  // -1/14!
+  pc = fma(pc, x2, 1.0 / 479001600.0);
+  pc = fma(pc, x2, -1.0 / 3628800.0);
+  pc = fma(pc, x2, 1.0 / 40320.0);
+  pc = fma(pc, x2, -1.0 / 720.0);
+  pc = fma(pc, x2, 1.0 / 24.0);
+  pc = fma(pc, x2, -0.5);
+  c = fma(pc, x2, 1.0);
+  double ps = -1.0 / 1307674368000.0;  // -1/15!
+  ps = fma(ps, x2, 1.0 / 6227020800.0);
+  ps = fma(ps, x2, -1.0 / 39916800.0);
+  ps = fma(ps, x2, 1.0 / 362880.0);
+  ps = fma(ps, x2, -1.0 / 5040.0);
+  ps = fma(ps, x2, 1.0 / 120.0);
+  ps = fma(ps, x2, -1.0 / 6.0);
+  sn = x * fma(ps, x2, 1.0);
+}
+
+// Largest |k (R - Rc)| that cos_sin_small takes, Rc the distance between the
+// segments' midpoints: |R - Rc| <= (L_i + L_j) / 2 for points on the segments.
+constexpr double KDR_MAX = 0.6;
+
 // FAR32: segment pairs whose midpoints lie more than sqrt(near2) x the longer
 // segment apart take the kernel's phase in FP32 (far_phase32); the others,
 // and every same-edge, extended-kernel or image term, stay FP64.
@@ -400,11 +437,21 @@ MW_DEV void fill_body(bool sym, int ti, int tj, const mw_mesh& M, double k, doub
     } else if (live) {
       const double a2 = I.a * I.a;
       const bool ext = I.ekl >= 0 && I.ekl == J.ekl;  // extended kernel on this pair
+      const double mx = I.mid[0] - J.mid[0], my = I.mid[1] - J.mid[1], mz = I.mid[2] - J.mid[2];
+      const double md2 = mx * mx + my * my + mz * mz;
       bool far = false;
       if (FAR32 && !ext) {
-        const double mx = I.mid[0] - J.mid[0], my = I.mid[1] - J.mid[1], mz = I.mid[2] - J.mid[2];
         const double lm = fmax(I.len, J.len);
-        far = mx * mx + my * my + mz * mz > near2 * lm * lm;
+        far = md2 > near2 * lm * lm;
+      }
+      // exp(-jkR) = exp(-jk Rc) exp(-jk (R - Rc)): one full sincos per pair
+      // at the midpoints' distance Rc, a short series per point pair
+      const bool series = !(FAR32 && far) && k * 0.5 * (I.len + J.len) <= KDR_MAX;
+      double Rc = 0.0, sc = 0.0, cc = 1.0;
+      if (series) {
+        double rci;
+        r_and_rinv(md2 + a2, Rc, rci);
+        sincos(k * Rc, &sc, &cc);
       }
 #pragma unroll
       for (int qq = 0; qq < QH; ++qq) {
@@ -415,14 +462,21 @@ MW_DEV void fill_body(bool sym, int ti, int tj, const mw_mesh& M, double k, doub
 #pragma unroll
         for (int r = 0; r < NQ; ++r) {
           const double dx = x - J.pos[r][0], dy = y - J.pos[r][1], dz = z - J.pos[r][2];
-          const double R = sqrt(dx * dx + dy * dy + dz * dz + a2);
+          double R, rinv;
+          r_and_rinv(dx * dx + dy * dy + dz * dz + a2, R, rinv);
           double gr, gi;  // exp(-jkR)/(4 pi R)
           if (FAR32 && far) {
             far_phase32(k, R, gr, gi);
+          } else if (series) {
+            double cd, sd;
+            cos_sin_small(k * (R - Rc), cd, sd);
+            const double inv = INV4PI * rinv;
+            gr = fma(cc, cd, -sc * sd) * inv;
+            gi = -fma(sc, cd, cc * sd) * inv;
           } else {
             double sn, cn;
             sincos(k * R, &sn, &cn);
-            const double inv = INV4PI / R;
+            const double inv = INV4PI * rinv;
             gr = cn * inv;
             gi = -sn * inv;
           }
