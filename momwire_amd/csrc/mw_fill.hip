@@ -459,6 +459,7 @@ MW_DEV void fill_body(bool sym, int ti, int tj, const mw_mesh& M, double k, doub
         if (q >= NQ) break;
         const double x = I.pos[q][0], y = I.pos[q][1], z = I.pos[q][2];
         const double wq0 = I.wu[0][q], wq1 = I.wu[1][q], wq2 = I.wu[2][q];
+        double tr[NM] = {0.0, 0.0, 0.0}, ti[NM] = {0.0, 0.0, 0.0};  // sum_r w_P(r) g(q, r)
 #pragma unroll
         for (int r = 0; r < NQ; ++r) {
           const double dx = x - J.pos[r][0], dy = y - J.pos[r][1], dz = z - J.pos[r][2];
@@ -486,15 +487,18 @@ MW_DEV void fill_body(bool sym, int ti, int tj, const mw_mesh& M, double k, doub
             gr = g0 * f.x - gi * f.y;
             gi = g0 * f.y + gi * f.x;
           }
-          const double w0 = J.wu[0][r], w1 = J.wu[1][r], w2 = J.wu[2][r];
-          const double ar[3] = {w0 * gr, w1 * gr, w2 * gr};
-          const double ai[3] = {w0 * gi, w1 * gi, w2 * gi};
 #pragma unroll
           for (int P = 0; P < NM; ++P) {
-            jr[0][P] += wq0 * ar[P]; jim[0][P] += wq0 * ai[P];
-            jr[1][P] += wq1 * ar[P]; jim[1][P] += wq1 * ai[P];
-            jr[2][P] += wq2 * ar[P]; jim[2][P] += wq2 * ai[P];
+            tr[P] = fma(J.wu[P][r], gr, tr[P]);
+            ti[P] = fma(J.wu[P][r], gi, ti[P]);
           }
+        }
+        // the observer point's weights times its source sums
+#pragma unroll
+        for (int P = 0; P < NM; ++P) {
+          jr[0][P] = fma(wq0, tr[P], jr[0][P]); jim[0][P] = fma(wq0, ti[P], jim[0][P]);
+          jr[1][P] = fma(wq1, tr[P], jr[1][P]); jim[1][P] = fma(wq1, ti[P], jim[1][P]);
+          jr[2][P] = fma(wq2, tr[P], jr[2][P]); jim[2][P] = fma(wq2, ti[P], jim[2][P]);
         }
       }
     }
