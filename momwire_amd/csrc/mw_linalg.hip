@@ -540,14 +540,22 @@ __global__ __launch_bounds__(256, MINB) void k_gemm_sub_pf(int64_t m, int64_t n,
       As[buf][a_row][a_h * AQ + q] = -ra[q].x;         // real k 2 kc: position kc
       As[buf][a_row][HALF + a_h * AQ + q] = -ra[q].y;  // real k 2 kc + 1: HALF + kc
     }
+    // B's BQ consecutive k of a column, 16 bytes per store (scalar stores two
+    // rows apart fell on a quarter of the banks)
 #pragma unroll
-    for (int q = 0; q < BQ; ++q) {
-      const float2 v = rb[q];
-      const int kc = b_k0 + q;
-      Bt[buf][2 * b_col][kc] = v.x;
-      Bt[buf][2 * b_col + 1][kc] = v.y;
-      Bt[buf][2 * b_col][HALF + kc] = -v.y;
-      Bt[buf][2 * b_col + 1][HALF + kc] = v.x;
+    for (int q0 = 0; q0 < BQ; q0 += 4) {
+      v4f re, im, nim;
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        re[q] = rb[q0 + q].x;
+        im[q] = rb[q0 + q].y;
+        nim[q] = -rb[q0 + q].y;
+      }
+      const int kc = b_k0 + q0;
+      *reinterpret_cast<v4f*>(&Bt[buf][2 * b_col][kc]) = re;
+      *reinterpret_cast<v4f*>(&Bt[buf][2 * b_col + 1][kc]) = im;
+      *reinterpret_cast<v4f*>(&Bt[buf][2 * b_col][HALF + kc]) = nim;
+      *reinterpret_cast<v4f*>(&Bt[buf][2 * b_col + 1][HALF + kc]) = re;
     }
   };
   const int g = lane >> 5;
