@@ -511,6 +511,23 @@ def test_gemm_tn_add():
         assert rel_err(dC.cpu().numpy(), C + A.T @ B) < 1e-13
 
 
+def test_cgemm_tn_add():
+    """The FP32 form (the mixed readout's C^T R): against the product of the
+    same complex64 inputs in FP64, to FP32's accumulation error over k."""
+    _say("cgemm tn add")
+    rng = np.random.default_rng(6)
+    for m, n, k in [(64, 64, 2624), (200, 130, 517), (1024, 1024, 700)]:
+        A = (rng.standard_normal((k, m)) + 1j * rng.standard_normal((k, m))).astype(np.complex64)
+        B = (rng.standard_normal((k, n)) + 1j * rng.standard_normal((k, n))).astype(np.complex64)
+        C = (rng.standard_normal((m, n)) + 1j * rng.standard_normal((m, n))).astype(np.complex64)
+        dA, dB, dC = (torch.from_numpy(x).to("cuda") for x in (A, B, C))
+        _lib.check(_lib.load().mw_cgemm_tn_add(m, n, k, dA.data_ptr(), m, dB.data_ptr(), n,
+                                               dC.data_ptr(), n, _stream()), "tn32")
+        torch.cuda.synchronize()
+        ref = C.astype(np.complex128) + A.astype(np.complex128).T @ B.astype(np.complex128)
+        assert rel_err(dC.cpu().numpy().astype(np.complex128), ref) < 2e-6
+
+
 def test_compute_port_solution_replays_graph_across_k():
     """compute_port_solution: eager on the first solve at a k, the captured
     graph from the second on.  Moving to another k and back replays the
