@@ -513,7 +513,9 @@ def test_gemm_tn_add():
 
 def test_cgemm_tn_add():
     """The FP32 form (the mixed readout's C^T R): against the product of the
-    same complex64 inputs in FP64, to FP32's accumulation error over k."""
+    same complex64 inputs in FP64.  FP32 sums over k = 2,624 terms leave
+    ~2.4e-6 of the largest entry (measured); a transposed operand or a wrong
+    index would leave O(1), so the bar is 1e-5."""
     _say("cgemm tn add")
     rng = np.random.default_rng(6)
     for m, n, k in [(64, 64, 2624), (200, 130, 517), (1024, 1024, 700)]:
@@ -525,7 +527,7 @@ def test_cgemm_tn_add():
                                                dC.data_ptr(), n, _stream()), "tn32")
         torch.cuda.synchronize()
         ref = C.astype(np.complex128) + A.astype(np.complex128).T @ B.astype(np.complex128)
-        assert rel_err(dC.cpu().numpy().astype(np.complex128), ref) < 2e-6
+        assert rel_err(dC.cpu().numpy().astype(np.complex128), ref) < 1e-5
 
 
 def test_compute_port_solution_replays_graph_across_k():
