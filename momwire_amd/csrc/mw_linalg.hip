@@ -174,12 +174,16 @@ __global__ __launch_bounds__(256) void k_gemm_sub(int64_t m, int64_t n, int64_t 
 
 // Small-tile variant (64 rows x 32 complex columns, each wave 2 x 2 MFMA
 // tiles) for products too small to occupy the chip with 128 x 64 tiles.
-constexpr int SBM = 64, SBN = 32;
+// SKC complex k per stage; the next stage's operands load into registers
+// while a stage's MFMAs run (8 complex k per stage with no prefetch left the
+// long-K readout products waiting on their loads: 34 TF/s at 1024 x 1024 x
+// 15360).
+constexpr int SBM = 64, SBN = 32, SKC = 32;
 
 template <typename T>
 struct GemmSmallLds {
-  T As[SBM][2 * GKC + 1];
-  typename cplx_t<T>::type Bs[GKC][SBN + 1];
+  T As[SBM][2 * SKC + 1];
+  typename cplx_t<T>::type Bs[SKC][SBN + 1];
 };
 
 template <typename T, bool TA>
@@ -189,7 +193,7 @@ MW_DEV void gemm_small_tile(GemmSmallLds<T>& L, int bx, int by, int64_t m, int64
                             int64_t ldb, typename cplx_t<T>::type* C, int64_t ldc) {
   typedef typename cplx_t<T>::type CT;
   typedef typename Mfma<T>::acc_t acc_t;
-  T (*As)[2 * GKC + 1] = L.As;
+  T (*As)[2 * SKC + 1] = L.As;
   CT (*Bs)[SBN + 1] = L.Bs;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wr = wave >> 1, wc = wave & 1;
@@ -207,26 +211,46 @@ MW_DEV void gemm_small_tile(GemmSmallLds<T>& L, int bx, int by, int64_t m, int64
         const int64_t gcr = 2 * col0 + wc * 32 + j * 16 + (lane & 15);
         acc[i][j][r] = (gr < m && gcr < 2 * n) ? Cr[gr * 2 * ldc + gcr] : (T)0;
       }
-  for (int64_t k0 = 0; k0 < kdim; k0 += GKC) {
-    for (int e = tid; e < SBM * GKC; e += 256) {
-      const int r = TA ? (e % SBM) : (e / GKC), c = TA ? (e / SBM) : (e % GKC);
+  constexpr int AU = SBM * SKC / 256, BU = SKC * SBN / 256;
+  CT ra[AU], rb[BU];
+  auto gload = [&](int64_t k0) {
+#pragma unroll
+    for (int u = 0; u < AU; ++u) {
+      const int e = tid + 256 * u;
+      const int r = TA ? (e % SBM) : (e / SKC), c = TA ? (e / SBM) : (e % SKC);
       CT v;
       v.x = 0; v.y = 0;
       if (row0 + r < m && k0 + c < kdim)
         v = TA ? A[(k0 + c) * lda + row0 + r] : A[(row0 + r) * lda + k0 + c];
-      As[r][2 * c] = TA ? v.x : -v.x;
-      As[r][2 * c + 1] = TA ? v.y : -v.y;
+      ra[u] = v;
     }
-    {
-      const int r = tid / SBN, c = tid % SBN;
+#pragma unroll
+    for (int u = 0; u < BU; ++u) {
+      const int e = tid + 256 * u, r = e / SBN, c = e % SBN;
       CT v;
       v.x = 0; v.y = 0;
       if (k0 + r < kdim && col0 + c < n) v = B[(k0 + r) * ldb + col0 + c];
-      Bs[r][c] = v;
+      rb[u] = v;
+    }
+  };
+  gload(0);
+  for (int64_t k0 = 0; k0 < kdim; k0 += SKC) {
+#pragma unroll
+    for (int u = 0; u < AU; ++u) {
+      const int e = tid + 256 * u;
+      const int r = TA ? (e % SBM) : (e / SKC), c = TA ? (e / SBM) : (e % SKC);
+      As[r][2 * c] = TA ? ra[u].x : -ra[u].x;
+      As[r][2 * c + 1] = TA ? ra[u].y : -ra[u].y;
+    }
+#pragma unroll
+    for (int u = 0; u < BU; ++u) {
+      const int e = tid + 256 * u;
+      Bs[e / SBN][e % SBN] = rb[u];
     }
     __syncthreads();
+    if (k0 + SKC < kdim) gload(k0 + SKC);
 #pragma unroll
-    for (int ks = 0; ks < 2 * GKC; ks += 4) {
+    for (int ks = 0; ks < 2 * SKC; ks += 4) {
       const int kr = ks + (lane >> 4);
       const int kc = kr >> 1, kpar = kr & 1;
       T a[2], b[2];
