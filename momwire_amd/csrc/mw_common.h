@@ -88,3 +88,40 @@ MW_DEV double2 ek_factor_minus1(double R, double a, double k) {
 }
 
 // Status codes (MW_OK ...) come from the public header.
+
+// R = sqrt(s) and 1 / R together: the hardware reciprocal square root and two
+// Newton steps y <- y (3/2 - (s/2) y^2) (s > 0: a^2 is part of s).
+MW_DEV void r_and_rinv(double s, double& R, double& rinv) {
+  const double h = 0.5 * s;
+  double y = __builtin_amdgcn_rsq(s);
+  y = y * fma(-h * y, y, 1.5);
+  y = y * fma(-h * y, y, 1.5);
+  R = s * y;
+  rinv = y;
+}
+
+// cos x and sin x for |x| <= 0.6 from their Taylor series through x^14 and
+// x^15 (the first omitted terms are below 1.3e-17 there).
+MW_DEV void cos_sin_small(double x, double& c, double& sn) {
+  const double x2 = x * x;
+  double pc = -1.0 / 87178291200.0;  // -1/14!
+  pc = fma(pc, x2, 1.0 / 479001600.0);
+  pc = fma(pc, x2, -1.0 / 3628800.0);
+  pc = fma(pc, x2, 1.0 / 40320.0);
+  pc = fma(pc, x2, -1.0 / 720.0);
+  pc = fma(pc, x2, 1.0 / 24.0);
+  pc = fma(pc, x2, -0.5);
+  c = fma(pc, x2, 1.0);
+  double ps = -1.0 / 1307674368000.0;  // -1/15!
+  ps = fma(ps, x2, 1.0 / 6227020800.0);
+  ps = fma(ps, x2, -1.0 / 39916800.0);
+  ps = fma(ps, x2, 1.0 / 362880.0);
+  ps = fma(ps, x2, -1.0 / 5040.0);
+  ps = fma(ps, x2, 1.0 / 120.0);
+  ps = fma(ps, x2, -1.0 / 6.0);
+  sn = x * fma(ps, x2, 1.0);
+}
+
+// Largest |k (R - Rc)| that cos_sin_small takes, Rc the distance between the
+// segments' midpoints: |R - Rc| <= (L_i + L_j) / 2 for points on the segments.
+constexpr double KDR_MAX = 0.6;

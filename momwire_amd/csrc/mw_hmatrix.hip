@@ -77,6 +77,19 @@ MW_DEV double2 band_entry(const mw_mesh& M, int si, int sa, int sj, int sb, doub
     const double* w01 = M.gl + nq;
     const bool ext = I.ekl >= 0 && I.ekl == J.ekl;
     const double a2 = I.a * I.a;
+    // exp(-jkR) = exp(-jk Rc) exp(-jk (R - Rc)), Rc between the midpoints: one
+    // full sincos per pair, a short series per point pair (as the dense fill)
+    const bool series = k * 0.5 * (I.L + J.L) <= KDR_MAX;
+    double Rc = 0.0, sc = 0.0, cc = 1.0;
+    if (series) {
+      double m2 = a2, rci;
+      for (int c = 0; c < 3; ++c) {
+        const double d = 0.5 * (I.l[c] + I.r[c] - J.l[c] - J.r[c]);
+        m2 += d * d;
+      }
+      r_and_rinv(m2, Rc, rci);
+      sincos(k * Rc, &sc, &cc);
+    }
     for (int q = 0; q < nq; ++q) {
       const double tq = t01[q], uq = tq * I.L, wq = w01[q] * I.L;
       const double Pq = wq * (ci[0] + uq * (ci[1] + uq * ci[2]));
@@ -92,11 +105,21 @@ MW_DEV double2 band_entry(const mw_mesh& M, int si, int sa, int sj, int sb, doub
         const double dx = x - (J.l[0] + tr * (J.r[0] - J.l[0]));
         const double dy = y - (J.l[1] + tr * (J.r[1] - J.l[1]));
         const double dz = z - (J.l[2] + tr * (J.r[2] - J.l[2]));
-        const double R = sqrt(dx * dx + dy * dy + dz * dz + a2);
-        double sn, cn;
-        sincos(k * R, &sn, &cn);
-        const double inv = INV4PI / R;
-        double gr = cn * inv, gi = -sn * inv;
+        double R, rinv;
+        r_and_rinv(dx * dx + dy * dy + dz * dz + a2, R, rinv);
+        const double inv = INV4PI * rinv;
+        double gr, gi;
+        if (series) {
+          double cd, sd;
+          cos_sin_small(k * (R - Rc), cd, sd);
+          gr = fma(cc, cd, -sc * sd) * inv;
+          gi = -fma(sc, cd, cc * sd) * inv;
+        } else {
+          double sn, cn;
+          sincos(k * R, &sn, &cn);
+          gr = cn * inv;
+          gi = -sn * inv;
+        }
         if (ext) {
           const double2 f = ek_factor(R, I.a, k);
           const double g0 = gr;
